@@ -1,0 +1,284 @@
+/*
+ * Driver for the reference build (oracle/_ref). TEST INFRASTRUCTURE ONLY.
+ *
+ * Links the reference's own assembly, right-hand side, linear-solve and flux
+ * sources (compiled in place from the reference tree by oracle/Makefile) and
+ * its header-only AMG/PCG toolbox, and drives them on one problem read from a
+ * binary file. It replaces only main.c's role (main.c:123-514): it defines
+ * the globals that those sources read (variables.h) and sets them from the
+ * input instead of from hard-coded constants.
+ *
+ * Outputs (binary, little-endian, into the given prefix):
+ *   <prefix>.A      n, nnz, cnt[n], col[nnz], ele[nnz]  (row order d,l,c,r,u,t,b)
+ *   <prefix>.b      b[n]
+ *   <prefix>.p8     pressure from solve_linsys() (reference stop 1e-8)
+ *   <prefix>.p12    pressure from the same AMG-PCG driven to `eps_tight`
+ *   <prefix>.v      fluxos() on p12, 7 doubles per cell (u,d,l,r,c,t,b)
+ *   <prefix>.json   iteration counts, level sizes, timings, divergence
+ *
+ * Input file layout (all little-endian):
+ *   int32  nx, ny, nz
+ *   f64    Ix, Iy, Iz, viscosidade, eps_tight
+ *   int32  bc_type[6]   order l, r, u, d, t, b   (0 Dirichlet, 1 Neumann)
+ *   f64    bc_val[6]    same order
+ *   int32  has_sources  (1: KI, WI, pc, pw arrays follow the K arrays)
+ *   f64    Kx[n], Ky[n], Kz[n] [, KI[n], WI[n], pc[n], pw[n]]
+ */
+#define NOMPI
+#include "toolbox/toolbox.h"
+
+#include "structs.h"
+#include "variables.h"
+#include "functions.h"
+
+#include <sys/time.h>
+#include <cstdio>
+#include <cstdlib>
+
+/* Globals that the linked reference sources read or write (variables.h). */
+Dim N;
+grid I;
+grid h;
+Perm *P;
+ELE *perm;
+IND *indcol;
+ELE *v;
+Bound *B;
+FACE boundary;
+FACEK *BoundKarst;
+IND *indcol1;
+ELEK *uc;
+Perm *KI;
+DimK *M;
+double *wi;
+int *IDwi;
+int Nwi;
+int ix0, ixf, jy0, jyf;
+double karst;
+double viscosidade;
+double **beta;
+double rho;
+int *n_brechas;
+int **brechas;
+double **pc_old;
+double **ucL_old;
+double **ucR_old;
+double *hs;
+double *Kc;
+double **cont_karst;
+int number_karst;
+int Mmax;
+double **pc;
+double **C;
+double **C_contorno;
+double nxg, nyg, nzg;
+char fnam[1024];
+const char *fn;
+const char *fnwi;
+const char *dirname;
+double time_total;
+double time_amg;
+int iter_amg;
+int layer;
+int lay;
+
+static double now()
+{
+    struct timeval t;
+    gettimeofday(&t, 0);
+    return (double)t.tv_sec + 1e-6 * (double)t.tv_usec;
+}
+
+static void put(const char *prefix, const char *ext, const void *data, size_t bytes)
+{
+    char name[2048];
+    snprintf(name, sizeof(name), "%s%s", prefix, ext);
+    FILE *f = fopen(name, "wb");
+    if (!f) { perror(name); exit(2); }
+    if (bytes && fwrite(data, 1, bytes, f) != bytes) { perror(name); exit(2); }
+    fclose(f);
+}
+
+static void get(FILE *f, void *dst, size_t bytes)
+{
+    if (fread(dst, 1, bytes, f) != bytes) { fprintf(stderr, "short input\n"); exit(2); }
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) { fprintf(stderr, "usage: %s input.bin out_prefix\n", argv[0]); return 2; }
+    FILE *fi = fopen(argv[1], "rb");
+    if (!fi) { perror(argv[1]); return 2; }
+    int dims[3];
+    double geo[5];
+    int bct[6];
+    double bcv[6];
+    int has_src;
+    get(fi, dims, sizeof(dims));
+    get(fi, geo, sizeof(geo));
+    get(fi, bct, sizeof(bct));
+    get(fi, bcv, sizeof(bcv));
+    get(fi, &has_src, sizeof(has_src));
+
+    N.x = dims[0]; N.y = dims[1]; N.z = dims[2];
+    N.xy = N.x * N.y; N.t = N.xy * N.z;
+    N.px = N.x / 2; N.py = N.y / 2; N.pz = 0;
+    I.x = geo[0]; I.y = geo[1]; I.z = geo[2];
+    h.x = I.x / N.x; h.y = I.y / N.y; h.z = I.z / N.z;
+    viscosidade = geo[3];
+    double eps_tight = geo[4];
+    boundary.l = bct[0]; boundary.r = bct[1]; boundary.u = bct[2];
+    boundary.d = bct[3]; boundary.t = bct[4]; boundary.b = bct[5];
+    number_karst = 0;
+    Mmax = 1;
+    int n = N.t;
+
+    Perm Kp;
+    Kp.X = create_vector(n); Kp.Y = create_vector(n); Kp.Z = create_vector(n);
+    Kp.S = 0; Kp.permeability = 1;
+    get(fi, Kp.X, sizeof(double) * n);
+    get(fi, Kp.Y, sizeof(double) * n);
+    get(fi, Kp.Z, sizeof(double) * n);
+    double *KIv = create_vector(n), *WIv = create_vector(n), *pcv = create_vector(n), *pwv = create_vector(n);
+    if (has_src) {
+        get(fi, KIv, sizeof(double) * n);
+        get(fi, WIv, sizeof(double) * n);
+        get(fi, pcv, sizeof(double) * n);
+        get(fi, pwv, sizeof(double) * n);
+    }
+    fclose(fi);
+
+    /* Boundary value arrays, as main.c:376-387 + create_boundary(). */
+    double val_bound[7];
+    /* create_boundary order: [1]=l, [2]=r, [3]=u, [4]=d, [5]=t, [6]=b */
+    val_bound[1] = bcv[0]; val_bound[2] = bcv[1]; val_bound[3] = bcv[2];
+    val_bound[4] = bcv[3]; val_bound[5] = bcv[4]; val_bound[6] = bcv[5];
+    Bound Bb;
+    Bb.l = create_vector(n); Bb.r = create_vector(n); Bb.u = create_vector(n);
+    Bb.d = create_vector(n); Bb.t = create_vector(n); Bb.b = create_vector(n);
+    create_boundary(val_bound, &Bb, N, boundary);
+
+    ELE *pm = (ELE *)calloc(n + 1, sizeof(ELE));
+    IND *ic = (IND *)calloc(n + 1, sizeof(IND));
+    ELE *vv = (ELE *)calloc(n + 1, sizeof(ELE));
+    double *b = create_vector(n);
+    double *p8 = create_vector(n);
+    double *p12 = create_vector(n);
+
+    double t0 = now();
+    permeabilidade_arestas(&Kp, pm, ic, h, N, boundary, WIv, KIv, viscosidade);
+    source_term(&Bb, pm, h, N, boundary, pcv, pwv, KIv, WIv, viscosidade, b);
+    double t_assemble = now() - t0;
+
+    /* Reference seam at its own stopping rule (compute.cpp:41, 1e-8). */
+    t0 = now();
+    solve_linsys(pm, ic, b, p8, N);
+    double t_solve8 = now() - t0;
+    int iter8 = iter_amg;
+
+    /* Count-CSR exactly as solve_linsys() builds it (solve_linearsystem_AMG.c:57-143). */
+    std::vector<int> cnt(n), col;
+    std::vector<double> ele;
+    col.reserve((size_t)7 * n); ele.reserve((size_t)7 * n);
+    for (int s = 0; s < n; s++) {
+        int c = 0;
+        if (ic[s].d != -1) { col.push_back(ic[s].d); ele.push_back(-pm[s].d); c++; }
+        if (ic[s].l != -1) { col.push_back(ic[s].l); ele.push_back(-pm[s].l); c++; }
+        if (ic[s].c != -1) { col.push_back(ic[s].c); ele.push_back(pm[s].c); c++; }
+        if (ic[s].r != -1) { col.push_back(ic[s].r); ele.push_back(-pm[s].r); c++; }
+        if (ic[s].u != -1) { col.push_back(ic[s].u); ele.push_back(-pm[s].u); c++; }
+        if (ic[s].t != -1) { col.push_back(ic[s].t); ele.push_back(-pm[s].t); c++; }
+        if (ic[s].b != -1) { col.push_back(ic[s].b); ele.push_back(-pm[s].b); c++; }
+        cnt[s] = c;
+    }
+    int nnz = (int)col.size();
+
+    /* Same solver objects as compute.cpp:31-45, stop at eps_tight. */
+    std::vector<int> nod(n);
+    for (int i = 0; i < n; i++) nod[i] = i;
+    std::vector<double> bv(b, b + n), xv(n, 0.0);
+    t0 = now();
+    communicator<int, double> com(nod);
+    linear_operator<int, double> lin(cnt, col, ele);
+    vector_product<int, double> sca(com);
+    amg_solver<int, double> amg(nod, cnt, col, ele, 16, 1, 0.0, 1.0, true);
+    double t_setup = now() - t0;
+    t0 = now();
+    conjugate_gradient<int, double> cg(lin, amg, sca, eps_tight, 10000, false);
+    cg(bv, xv);
+    double t_pcg = now() - t0;
+    int iter12 = cg.iterations();
+    for (int i = 0; i < n; i++) p12[i] = xv[i];
+
+    /* Level sizes: rerun the hierarchy loop of amg.h:1188-1215 with the
+       reference's own amg_restriction, recording rows and nonzeros. */
+    std::vector<int> lev_rows, lev_nnz;
+    {
+        std::vector<int> anod(nod), acnt(cnt), acol(col);
+        std::vector<double> aele(ele);
+        amg_restriction<int, double> restr;
+        int l = 0;
+        double rel = 0.0;
+        int g = n;
+        lev_rows.push_back(n); lev_nnz.push_back(nnz);
+        while (l < 15 && g > 1 && rel < 1.0) {
+            communicator<int, double> c(anod);
+            c.assign(anod);
+            std::vector<double> adia;
+            std::vector<char> abou, rnod;
+            std::vector<int> rcnt, rcol, cnod, ccnt, ccol;
+            std::vector<double> rele, cele;
+            restr(0.0, anod, acnt, acol, aele, adia, abou, c, rnod, rcnt, rcol, rele, cnod, ccnt, ccol, cele);
+            anod.swap(cnod); acnt.swap(ccnt); acol.swap(ccol); aele.swap(cele);
+            int gold = g;
+            g = (int)acnt.size();
+            rel = (double)g / (double)gold;
+            lev_rows.push_back(g); lev_nnz.push_back((int)aele.size());
+            l++;
+        }
+    }
+
+    /* Fluxes of the tight solution (flux_reconstruction.c:9-279); fluxos()
+       reads the global `boundary` and prints divvel_pontual()'s min/max. */
+    t0 = now();
+    fluxos(p12, pm, vv, &Bb, h, N);
+    double t_flux = now() - t0;
+    double divmax = 0.0, divmin = 100.0;
+    for (int s = 0; s < n; s++) {
+        double dv = vv[s].r / h.x + vv[s].l / h.x + vv[s].d / h.y + vv[s].u / h.y + vv[s].b / h.z + vv[s].t / h.z;
+        if (fabs(dv) > divmax) divmax = fabs(dv);
+        if (fabs(dv) < divmin) divmin = fabs(dv);
+    }
+
+    const char *pre = argv[2];
+    {
+        char name[2048];
+        snprintf(name, sizeof(name), "%s.A", pre);
+        FILE *f = fopen(name, "wb");
+        if (!f) { perror(name); return 2; }
+        fwrite(&n, sizeof(int), 1, f);
+        fwrite(&nnz, sizeof(int), 1, f);
+        fwrite(cnt.data(), sizeof(int), n, f);
+        fwrite(col.data(), sizeof(int), nnz, f);
+        fwrite(ele.data(), sizeof(double), nnz, f);
+        fclose(f);
+    }
+    put(pre, ".b", b, sizeof(double) * n);
+    put(pre, ".p8", p8, sizeof(double) * n);
+    put(pre, ".p12", p12, sizeof(double) * n);
+    put(pre, ".v", vv, sizeof(ELE) * n);
+    {
+        char name[2048];
+        snprintf(name, sizeof(name), "%s.json", pre);
+        FILE *f = fopen(name, "w");
+        fprintf(f, "{\"n\": %d, \"nnz\": %d, \"iter_amg_1e-8\": %d, \"iter_tight\": %d, \"eps_tight\": %.3e,\n", n, nnz, iter8, iter12, eps_tight);
+        fprintf(f, " \"levels_rows\": [");
+        for (size_t i = 0; i < lev_rows.size(); i++) fprintf(f, "%s%d", i ? ", " : "", lev_rows[i]);
+        fprintf(f, "], \"levels_nnz\": [");
+        for (size_t i = 0; i < lev_nnz.size(); i++) fprintf(f, "%s%d", i ? ", " : "", lev_nnz[i]);
+        fprintf(f, "],\n \"t_assemble\": %.6f, \"t_solve_1e-8\": %.6f, \"t_setup\": %.6f, \"t_pcg_tight\": %.6f, \"t_flux\": %.6f,\n", t_assemble, t_solve8, t_setup, t_pcg, t_flux);
+        fprintf(f, " \"div_max\": %.17g, \"div_min\": %.17g}\n", divmax, divmin);
+        fclose(f);
+    }
+    return 0;
+}

@@ -1,0 +1,122 @@
+"""Write the golden fixtures in this folder from the reference build.
+
+Runs oracle/_ref/hmfem_ref (the reference's own assembly, count-CSR, AMG-PCG
+and flux sources compiled in place by ``make -C oracle ref``) on small seeded
+problems and stores inputs and outputs as compressed .npz files. Only this
+script runs the reference build; tests read the .npz files.
+
+Usage:  make -C oracle ref && python tests/golden/make_golden.py
+"""
+import json
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from hmfem_amd import fields  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "hmfem_ref")
+
+# face order l, r, u, d, t, b; 0 Dirichlet, 1 Neumann
+X_DIRICHLET = dict(bc_type=(0, 0, 1, 1, 1, 1), bc_val=(1.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+
+
+def case_list():
+    cases = []
+    # C1 of the benchmark set: 16^3 homogeneous, closed form p_i = 1 - (i+1/2)/16
+    cases.append(dict(name="c1_16_homogeneous", dims=(16, 16, 16), K=("homogeneous", 1e-14), **X_DIRICHLET))
+    cases.append(dict(name="lognormal_8", dims=(8, 8, 8), K=("lognormal", 2.0, 2.0, 11), **X_DIRICHLET))
+    cases.append(dict(name="lognormal_24", dims=(24, 24, 24), K=("lognormal", 2.0, 4.0, 12), **X_DIRICHLET, full=False))
+    cases.append(dict(name="karst_12x10x8_mixed", dims=(12, 10, 8), K=("karst", 3, 13),
+                      bc_type=(0, 0, 1, 0, 1, 1), bc_val=(2.0, 0.5, 1e-14, 0.25, -2e-14, 3e-14)))
+    cases.append(dict(name="karst_24", dims=(24, 24, 24), K=("karst", 4, 14), **X_DIRICHLET, full=False))
+    cases.append(dict(name="allneumann_6x5x4", dims=(6, 5, 4), K=("lognormal", 1.0, 2.0, 15),
+                      bc_type=(1, 1, 1, 1, 1, 1), bc_val=(1e-14, 1e-14, 0.0, 0.0, 0.0, 0.0)))
+    cases.append(dict(name="sources_10x8x6", dims=(10, 8, 6), K=("lognormal", 1.5, 2.0, 16),
+                      bc_type=(0, 1, 1, 0, 1, 0), bc_val=(1.0, 0.0, 0.0, 0.3, 0.0, 0.7), sources=17,
+                      visc=1e-3))
+    return cases
+
+
+def make_K(dims, spec):
+    nx, ny, nz = dims
+    if spec[0] == "homogeneous":
+        return fields.homogeneous(nx, ny, nz, spec[1])
+    if spec[0] == "lognormal":
+        return fields.lognormal(nx, ny, nz, np.log(1e-14), spec[1], spec[2], spec[3])
+    if spec[0] == "karst":
+        K = fields.homogeneous(nx, ny, nz, 1e-14)
+        fields.conduits(K, nx, ny, nz, spec[1], spec[2])
+        return K
+    raise ValueError(spec)
+
+
+def run_case(c, tmp):
+    nx, ny, nz = c["dims"]
+    n = nx * ny * nz
+    K = make_K(c["dims"], c["K"])
+    visc = c.get("visc", 1.0)
+    srcs = None
+    if c.get("sources"):
+        rng = np.random.default_rng(c["sources"])
+        KI = np.where(rng.random(n) < 0.1, rng.random(n) * 1e-15, 0.0)
+        WI = np.where(rng.random(n) < 0.05, rng.random(n) * 1e-15, 0.0)
+        pc = rng.random(n)
+        pw = rng.random(n) * 2.0
+        srcs = (KI, WI, pc, pw)
+    inp = os.path.join(tmp, c["name"] + ".in")
+    with open(inp, "wb") as f:
+        f.write(struct.pack("<3i", nx, ny, nz))
+        f.write(struct.pack("<5d", 1.0, 1.0, 1.0, visc, 1e-12))
+        f.write(struct.pack("<6i", *c["bc_type"]))
+        f.write(struct.pack("<6d", *c["bc_val"]))
+        f.write(struct.pack("<i", 1 if srcs else 0))
+        for _ in range(3):
+            f.write(np.ascontiguousarray(K, "<f8").tobytes())
+        if srcs:
+            for a in srcs:
+                f.write(np.ascontiguousarray(a, "<f8").tobytes())
+    pre = os.path.join(tmp, c["name"])
+    subprocess.run([REF, inp, pre], check=True, stdout=subprocess.DEVNULL)
+    meta = json.load(open(pre + ".json"))
+    raw = np.fromfile(pre + ".A", dtype="<i4", count=2)
+    nn, nnz = int(raw[0]), int(raw[1])
+    buf = open(pre + ".A", "rb").read()
+    cnt = np.frombuffer(buf, "<i4", nn, 8)
+    col = np.frombuffer(buf, "<i4", nnz, 8 + 4 * nn)
+    ele = np.frombuffer(buf, "<f8", nnz, 8 + 4 * nn + 4 * nnz)
+    out = dict(
+        dims=np.array(c["dims"], np.int32), bc_type=np.array(c["bc_type"], np.int32),
+        bc_val=np.array(c["bc_val"], np.float64), visc=np.float64(visc), K=K,
+        b=np.fromfile(pre + ".b", "<f8"), p8=np.fromfile(pre + ".p8", "<f8"),
+        p12=np.fromfile(pre + ".p12", "<f8"),
+        iter8=np.int32(meta["iter_amg_1e-8"]), iter12=np.int32(meta["iter_tight"]),
+        levels_rows=np.array(meta["levels_rows"], np.int32), levels_nnz=np.array(meta["levels_nnz"], np.int64),
+        div_max=np.float64(meta["div_max"]), div_min=np.float64(meta["div_min"]),
+    )
+    if srcs:
+        out.update(KI=srcs[0], WI=srcs[1], pc=srcs[2], pw=srcs[3])
+    if c.get("full", True):
+        out.update(cnt=cnt.copy(), col=col.copy(), ele=ele.copy(),
+                   v=np.fromfile(pre + ".v", "<f8").reshape(n, 7))
+    np.savez_compressed(os.path.join(HERE, c["name"] + ".npz"), **out)
+    print(c["name"], "n=%d nnz=%d iter8=%d iter12=%d levels=%s" % (n, nnz, out["iter8"], out["iter12"], meta["levels_rows"]))
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("missing %s: run `make -C oracle ref` first" % REF)
+    with tempfile.TemporaryDirectory() as tmp:
+        for c in case_list():
+            run_case(c, tmp)
+
+
+if __name__ == "__main__":
+    main()
