@@ -1,0 +1,230 @@
+"""Benchmark: solves/sec and HBM GB/s of the MI355X pressure solve.
+
+Workload (one "step" = one reservoir() pass: transmissibility assembly, AMG
+setup, FP64 flexible CG with the FP32 AMG V-cycle to a relative residual of
+1e-8, then flux reconstruction and divergence):
+  128x128x128 cells on the unit cube, lognormal K (ln K ~ N(ln 1e-14, 2^2),
+  squared-exponential correlation of 8 cells, seed 20261202), Dirichlet
+  p=1 / p=0 on the x faces, no-flow elsewhere, lambda = 1.
+Inputs (K and the face values) are resident in HBM before timing starts.
+
+Prints one JSON line (rank 0). Run `python bench.py --help` for options.
+"""
+import argparse
+import json
+import os
+import platform
+import struct
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/sec + HBM GB/s at 1e-8 rel. residual, FP32-AMG vs FP64, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+SEED = 20261202
+WORKLOADS = {
+    "c2": dict(dims=(128, 128, 128), kind="lognormal", sigma=2.0, corr=8.0,
+               label="128^3 lognormal K, sigma=2, corr 8 cells, x-Dirichlet, rtol 1e-8"),
+    "c1": dict(dims=(16, 16, 16), kind="homogeneous",
+               label="16^3 homogeneous K=1e-14, x-Dirichlet, rtol 1e-10"),
+    "c3": dict(dims=(256, 256, 256), kind="karst", conduits=20,
+               label="256^3 K=1e-14 + 20 karst conduits x10^U(4,6), x-Dirichlet, rtol 1e-8"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_K(w):
+    from hmfem_amd import fields
+    nx, ny, nz = w["dims"]
+    if w["kind"] == "lognormal":
+        return fields.lognormal(nx, ny, nz, np.log(1e-14), w["sigma"], w["corr"], SEED)
+    K = fields.homogeneous(nx, ny, nz, 1e-14)
+    if w["kind"] == "karst":
+        fields.conduits(K, nx, ny, nz, w["conduits"], SEED)
+    return K
+
+
+def cpu_baseline(K, dims, budget_s):
+    """Time the reference build (oracle/_ref) on the same problem: one
+    reservoir() step single-threaded. Falls back to the oracle restatement."""
+    nx, ny, nz = dims
+    ref = os.path.join(ROOT, "oracle", "_ref", "hmfem_ref")
+    cores = 1
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as tmp:
+            inp = os.path.join(tmp, "in.bin")
+            with open(inp, "wb") as f:
+                f.write(struct.pack("<3i", nx, ny, nz))
+                f.write(struct.pack("<5d", 1.0, 1.0, 1.0, 1.0, 1e-12))
+                f.write(struct.pack("<6i", 0, 0, 1, 1, 1, 1))
+                f.write(struct.pack("<6d", 1.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+                f.write(struct.pack("<i", 0))
+                kb = np.ascontiguousarray(K, "<f8").tobytes()
+                f.write(kb); f.write(kb); f.write(kb)
+            t0 = time.time()
+            subprocess.run([ref, inp, os.path.join(tmp, "out"), "time"], check=True, stdout=subprocess.DEVNULL,
+                           timeout=max(60, 4 * budget_s), env=dict(os.environ, OMP_NUM_THREADS="1"))
+            wall = time.time() - t0
+            meta = json.load(open(os.path.join(tmp, "out.json")))
+        t = meta["t_assemble"] + meta["t_solve_1e-8"] + meta["t_flux"]
+        return {"value": 1.0 / t, "unit": "solves/s", "cores": cores, "kind": "reference",
+                "sample": "one full %dx%dx%d step of the reference build (assembly, AMG setup + PCG to its 1e-8 "
+                          "stop, fluxes), single thread; %d PCG iterations; %.2f s (process wall %.2f s)"
+                          % (nx, ny, nz, meta["iter_amg_1e-8"] - 1, t, wall),
+                "cpu": platform.processor() or platform.machine()}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    P = O.Problem(dims, K, (0, 0, 1, 1, 1, 1), (1, 0, 0, 0, 0, 0))
+    t0 = time.time()
+    perm, indcol, b = O.assemble(P)
+    cnt, col, ele = O.build_csr(perm, indcol)
+    x, it = O.compute(cnt, col, ele, b, eps=1e-8)
+    O.fluxos(P, x, perm)
+    t = time.time() - t0
+    return {"value": 1.0 / t, "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": "one full %dx%dx%d step of the oracle restatement, single thread; %.2f s" % (nx, ny, nz, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--profile-only", action="store_true", help="timed steps only (for rocprofv3 runs)")
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (every box here has one); --gpus %d is not supported"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hmfem_amd import solver as S
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    w = WORKLOADS[args.workload]
+    nx, ny, nz = w["dims"]
+    n = nx * ny * nz
+    rtol = 1e-10 if args.workload == "c1" else 1e-8
+    log("workload", args.workload, w["dims"], "generating K")
+    K = make_K(w)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    Kd = dev(K)
+    faces = [np.full(ny * nz, 1.0), np.zeros(ny * nz), np.zeros(nx * nz), np.zeros(nx * nz), np.zeros(nx * ny),
+             np.zeros(nx * ny)]
+    bval = [dev(a) for a in faces]
+    p = torch.zeros(n, dtype=torch.float64, device="cuda")
+    v = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
+    g = S.Context.grid(nx, ny, nz, None, 1.0, (0, 0, 1, 1, 1, 1))
+    ctx = S.Context()
+    prec = S.PREC_FP64 if args.precision == "fp64" else S.PREC_FP32
+    opt = S.default_options(rtol=rtol, precision=prec)
+
+    def step(o=opt):
+        p.zero_()   # each step solves from the reference's zero initial guess
+        return ctx.reservoir(g, Kd, Kd, Kd, bval, p, v, o)
+
+    for _ in range(args.warmup):
+        st = step()
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(step())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = args.steps / elapsed
+    st = stats[-1]
+    log("timed", args.steps, "steps in %.3f s" % elapsed, "iterations", st.iterations, "levels", st.levels)
+    if args.profile_only:
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "solves/s", "steps": args.steps}))
+        return
+
+    # Per-kernel durations with HIP events on the solver's own stream.
+    ctx.profile_enable(40000)
+    prof_steps = max(2, min(5, args.steps))
+    for _ in range(prof_steps):
+        step()
+    prof = ctx.profile_read()
+    ctx.profile_enable(0)
+    fine = {k: d for k, d in prof.items() if d["count"] > 0 and k != "coarse"}
+    dom = max(fine, key=lambda k: fine[k]["ms"])
+    d = fine[dom]
+    avg_ms = d["ms"] / d["count"]
+    achieved = d["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {"avg_us": 1e3 * x["ms"] / x["count"], "launches_per_step": x["count"] / prof_steps,
+                   "gbps": x["bytes_per_launch"] / (x["ms"] / x["count"] * 1e-3) / 1e9 if x["bytes_per_launch"] else None}
+               for k, x in prof.items() if x["count"] > 0}
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get(dom)
+
+    # Plain FP64 hierarchy on the same workload (the north star's comparison).
+    fp64 = None
+    if not args.no_fp64 and prec == S.PREC_FP32:
+        o64 = S.default_options(rtol=rtol, precision=S.PREC_FP64)
+        step(o64)
+        torch.cuda.synchronize()
+        k64 = max(3, args.steps // 2)
+        ta = time.perf_counter()
+        for _ in range(k64):
+            s64 = step(o64)
+        torch.cuda.synchronize()
+        fp64 = {"value": k64 / (time.perf_counter() - ta), "iterations": s64.iterations}
+        fp64["speedup_fp32_over_fp64"] = value / fp64["value"]
+        # accuracy of the FP32-AMG solution against an FP64 solve driven to 1e-12
+        pf = p.clone()
+        step(opt)
+        p32 = p.clone()
+        step(S.default_options(rtol=1e-12, precision=S.PREC_FP64))
+        pf = p.clone()
+        fp64["rel_l2_fp32amg_vs_fp64_1e-12"] = float(torch.linalg.norm(p32 - pf) / torch.linalg.norm(pf))
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(K, w["dims"], args.cpu_budget)
+        cpu["cores"] = 1
+
+    solve_gbs = st.solve_bytes / (st.t_solve_ms * 1e-3) / 1e9 if st.t_solve_ms > 0 else None
+    out = {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64" if prec == S.PREC_FP32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": w["label"], "cells": n, "precision": "FP32 AMG in FP64 flexible CG"
+                   if prec == S.PREC_FP32 else "FP64 AMG in FP64 flexible CG", "rtol": rtol},
+        "iterations": st.iterations, "levels": st.levels,
+        "phase_ms": {"assemble": st.t_assemble_ms, "setup": st.t_setup_ms, "solve": st.t_solve_ms,
+                     "flux": st.t_flux_ms},
+        "hbm_gbps_solve_phase": solve_gbs,
+        "rel_residual": st.rel_residual, "div_max": st.div_max,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": 1e3 * avg_ms},
+        "kernels": kernels,
+        "fp64_hierarchy": fp64,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,66 @@
+"""Build the package's native libraries in-tree.
+
+libhmfem_fields.so  host C (gcc): seeded permeability fields
+libhmfem_gpu.so     HIP for gfx950 (hipcc): the solver and its C ABI
+"""
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+
+HIP_SOURCES = ["hmfem_assemble.hip", "hmfem_solver.hip", "hmfem_capi.hip"]
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # no packed FP32 instructions (see the build notes in DESIGN.md)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    if verbose and r.stderr.strip():
+        print(r.stderr[-4000:])
+
+
+def build_fields(verbose=False):
+    out = os.path.join(HERE, "libhmfem_fields.so")
+    _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-o", out,
+          os.path.join(CSRC, "hmfem_fields.c"), "-lm"], verbose)
+    return out
+
+
+def build_gpu(verbose=False):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    out = os.path.join(HERE, "libhmfem_gpu.so")
+    tmp = out + ".tmp"
+    _run([hipcc] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include"), "-o", tmp]
+         + [os.path.join(CSRC, s) for s in HIP_SOURCES], verbose)
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle(verbose=False):
+    """The checker (oracle/liboracle.so) and, where the reference tree is
+    present, the reference build oracle/_ref/hmfem_ref."""
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", odir, "liboracle.so"], verbose)
+    if os.path.isdir("/root/reference"):
+        _run(["make", "-s", "-C", odir, "ref"], verbose)
+
+
+def build_all(verbose=False):
+    build_fields(verbose)
+    build_gpu(verbose)
+    build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)

@@ -1,0 +1,607 @@
+// C ABI of libhmfem_gpu: the reference-compatible entry points (host buffers)
+// and the device-resident API (see include/hmfem_gpu.h).
+#include "hmfem_internal.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace hmfem;
+
+extern "C" {
+int iter_amg = 0;
+FACE boundary;
+}
+
+struct hmfem_context {
+    hipStream_t st = nullptr;
+    hipEvent_t ev[8];
+    std::map<std::string, std::pair<void *, size_t>> bufs;
+    CgScalars *h_sc = nullptr;
+    ProfileSink prof{};
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<int> prof_type;
+    bool prof_on = false;
+
+    // device buffer `name` with at least `bytes`, grown on demand
+    void *get(const char *name, size_t bytes)
+    {
+        auto &slot = bufs[name];
+        if (slot.second >= bytes && slot.first) return slot.first;
+        if (slot.first) {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(slot.first);
+            slot.first = nullptr;
+            slot.second = 0;
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes < 256 ? 256 : bytes) != hipSuccess) return nullptr;
+        slot.first = p;
+        slot.second = bytes;
+        return p;
+    }
+};
+
+static hmfem_context *g_default = nullptr;
+
+static hmfem_context *default_ctx()
+{
+    if (!g_default) {
+        if (hmfem_create(&g_default) != HMFEM_OK) {
+            fprintf(stderr, "hmfem_gpu: cannot create a HIP context (no device?)\n");
+            exit(1);
+        }
+    }
+    return g_default;
+}
+
+extern "C" void hmfem_default_options(hmfem_options *o)
+{
+    o->rtol = 1e-8;
+    o->max_iterations = 10000;
+    o->precision = HMFEM_PREC_FP32;
+    o->stop_rule = HMFEM_STOP_RESIDUAL;
+    o->nu = 2;
+    o->omega = 0.8;
+    o->ocf = 1.7;
+    o->coarse_max = kCoarseMax;
+    o->check_every = 4;
+}
+
+extern "C" int hmfem_create(hmfem_context **out)
+{
+    if (!out) return HMFEM_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return HMFEM_ERR_HIP;
+    hmfem_context *c = new hmfem_context();
+    HMFEM_CHECK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    for (int q = 0; q < 8; q++) HMFEM_CHECK(hipEventCreate(&c->ev[q]));
+    HMFEM_CHECK(hipHostMalloc((void **)&c->h_sc, sizeof(CgScalars), hipHostMallocDefault));
+    *out = c;
+    return HMFEM_OK;
+}
+
+extern "C" void hmfem_destroy(hmfem_context *c)
+{
+    if (!c) return;
+    (void)hipStreamSynchronize(c->st);
+    for (auto &kv : c->bufs)
+        if (kv.second.first) (void)hipFree(kv.second.first);
+    for (auto e : c->prof_ev) (void)hipEventDestroy(e);
+    for (int q = 0; q < 8; q++) (void)hipEventDestroy(c->ev[q]);
+    if (c->h_sc) (void)hipHostFree(c->h_sc);
+    (void)hipStreamDestroy(c->st);
+    if (c == g_default) g_default = nullptr;
+    delete c;
+}
+
+extern "C" void *hmfem_stream(hmfem_context *c) { return c ? (void *)c->st : nullptr; }
+
+// Per-kernel profiling (event pairs around fine-level and CG launches).
+extern "C" int hmfem_profile_enable(hmfem_context *c, int max_launches)
+{
+    if (!c || max_launches < 0) return HMFEM_ERR_ARG;
+    for (auto e : c->prof_ev) (void)hipEventDestroy(e);
+    c->prof_ev.assign(2 * (size_t)max_launches, nullptr);
+    for (auto &e : c->prof_ev) HMFEM_CHECK(hipEventCreate(&e));
+    c->prof_type.assign(max_launches, -1);
+    c->prof_on = max_launches > 0;
+    memset(&c->prof, 0, sizeof(c->prof));
+    return HMFEM_OK;
+}
+
+extern "C" int hmfem_profile_read(hmfem_context *c, double *ms, long *count, double *bytes, int ntypes)
+{
+    if (!c) return HMFEM_ERR_ARG;
+    for (int t = 0; t < ntypes && t < HMFEM_PROF_TYPES; t++) {
+        if (ms) ms[t] = c->prof.ms[t];
+        if (count) count[t] = c->prof.count[t];
+        if (bytes) bytes[t] = c->prof.bytes[t];
+    }
+    return HMFEM_PROF_TYPES;
+}
+
+extern "C" void hmfem_profile_reset(hmfem_context *c)
+{
+    if (c) {
+        for (int t = 0; t < HMFEM_PROF_TYPES; t++) { c->prof.ms[t] = 0; c->prof.count[t] = 0; }
+    }
+}
+
+static ProfileSink *sink_of(hmfem_context *c)
+{
+    if (!c->prof_on) return nullptr;
+    c->prof.ev = c->prof_ev.data();
+    c->prof.type = c->prof_type.data();
+    c->prof.cap = (int)c->prof_ev.size();
+    return &c->prof;
+}
+
+static bool grid_ok(const hmfem_grid *g)
+{
+    if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1) return false;
+    if ((double)g->nx * g->ny * g->nz >= 2147483647.0) return false;
+    if (!(g->hx > 0) || !(g->hy > 0) || !(g->hz > 0) || !(g->viscosidade > 0)) return false;
+    return true;
+}
+
+static int solve_on_stencil(hmfem_context *c, Dims d, const Stencil<double> &A, const double *b, double *x,
+                            const hmfem_options *opt_in, hmfem_status *st)
+{
+    hmfem_options opt;
+    if (opt_in) opt = *opt_in;
+    else hmfem_default_options(&opt);
+    if (opt.coarse_max > kCoarseMax || opt.coarse_max < 1) opt.coarse_max = kCoarseMax;
+    long n = d.n();
+    int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
+    size_t hb = hierarchy_bytes(d, opt.coarse_max, prec);
+    char *arena = (char *)c->get("arena", hb);
+    CgBuffers B;
+    B.r = (double *)c->get("r", n * 8);
+    B.s0 = (double *)c->get("s0", n * 8);
+    B.s1 = (double *)c->get("s1", n * 8);
+    B.v = (double *)c->get("v", n * 8);
+    B.sc = (CgScalars *)c->get("sc", sizeof(CgScalars));
+    B.partial = (double *)c->get("partial", cg_partial_count(n) * 8);
+    int *info = (int *)c->get("info", 64);
+    if (!arena || !B.r || !B.s0 || !B.s1 || !B.v || !B.sc || !B.partial || !info) return HMFEM_ERR_NOMEM;
+    // the reduction counters must start at zero once; kernels reset them after use
+    static_assert(sizeof(CgScalars) % 4 == 0, "scalars");
+    if (!c->bufs["sc_init"].first) {
+        HMFEM_CHECK(hipMemsetAsync(B.sc, 0, sizeof(CgScalars), c->st));
+        c->get("sc_init", 8);
+    }
+    hmfem_status tmp{};
+    hmfem_status *out = st ? st : &tmp;
+    return solve_structured(c->st, d, A, b, x, opt, arena, hb, B, c->h_sc, info, out, c->ev, sink_of(c));
+}
+
+// ------------------------------------------------------------ device API
+
+extern "C" int hmfem_assemble_device(hmfem_context *c, const hmfem_grid *g, const double *Kx, const double *Ky,
+                                     const double *Kz, const double *KI, const double *WI, const double *pc,
+                                     const double *pw, const double *const bval[6], ELE *perm, IND *indcol,
+                                     double *b)
+{
+    if (!c || !grid_ok(g) || !Kx || !Ky || !Kz || !bval || !b) return HMFEM_ERR_ARG;
+    int rc = launch_assemble(c->st, g, Kx, Ky, Kz, KI, WI, pc, pw, bval, nullptr, nullptr, nullptr, nullptr, b, perm,
+                             indcol);
+    if (rc) return rc;
+    HMFEM_CHECK(hipStreamSynchronize(c->st));
+    return HMFEM_OK;
+}
+
+extern "C" int hmfem_flux_device(hmfem_context *c, const hmfem_grid *g, const double *Kx, const double *Ky,
+                                 const double *Kz, const double *const bval[6], const double *p, ELE *v,
+                                 double *div_min, double *div_max)
+{
+    if (!c || !grid_ok(g) || !Kx || !Ky || !Kz || !bval || !p) return HMFEM_ERR_ARG;
+    long n = (long)g->nx * g->ny * g->nz;
+    double *part = (double *)c->get("mm_partial", (size_t)2 * grid_for(n, kBlock) * 8 + 64);
+    unsigned *cnt = (unsigned *)c->get("mm_counter", 256);
+    double *out = (double *)c->get("mm_out", 64);
+    if (!part || !cnt || !out) return HMFEM_ERR_NOMEM;
+    if (!c->bufs["mm_init"].first) {
+        HMFEM_CHECK(hipMemsetAsync(cnt, 0, 256, c->st));
+        c->get("mm_init", 8);
+    }
+    int rc = launch_flux(c->st, g, Kx, Ky, Kz, nullptr, bval, p, v, part, cnt, out);
+    if (rc) return rc;
+    double h[2];
+    HMFEM_CHECK(hipMemcpyAsync(h, out, 16, hipMemcpyDeviceToHost, c->st));
+    HMFEM_CHECK(hipStreamSynchronize(c->st));
+    if (div_min) *div_min = h[0];
+    if (div_max) *div_max = h[1];
+    return HMFEM_OK;
+}
+
+extern "C" int hmfem_reservoir_device(hmfem_context *c, const hmfem_grid *g, const double *Kx, const double *Ky,
+                                      const double *Kz, const double *KI, const double *WI, const double *pc,
+                                      const double *pw, const double *const bval[6], double *p, ELE *v,
+                                      const hmfem_options *opt, hmfem_status *st)
+{
+    if (!c || !grid_ok(g) || !Kx || !Ky || !Kz || !bval || !p) return HMFEM_ERR_ARG;
+    Dims d{g->nx, g->ny, g->nz};
+    long n = d.n();
+    Stencil<double> A;
+    A.e = (double *)c->get("e", n * 8);
+    A.wx = (double *)c->get("wx", n * 8);
+    A.wy = (double *)c->get("wy", n * 8);
+    A.wz = (double *)c->get("wz", n * 8);
+    double *b = (double *)c->get("b", n * 8);
+    if (!A.e || !A.wx || !A.wy || !A.wz || !b) return HMFEM_ERR_NOMEM;
+    hmfem_status tmp{};
+    hmfem_status *out = st ? st : &tmp;
+    HMFEM_CHECK(hipEventRecord(c->ev[4], c->st));
+    int rc = launch_assemble(c->st, g, Kx, Ky, Kz, KI, WI, pc, pw, bval, A.e, A.wx, A.wy, A.wz, b, nullptr, nullptr);
+    if (rc) return rc;
+    HMFEM_CHECK(hipEventRecord(c->ev[5], c->st));
+    rc = solve_on_stencil(c, d, A, b, p, opt, out);
+    if (rc) return rc;
+    float ta = 0.f;
+    (void)hipEventElapsedTime(&ta, c->ev[4], c->ev[5]);
+    out->t_assemble_ms = ta;
+    if (v) {
+        HMFEM_CHECK(hipEventRecord(c->ev[6], c->st));
+        rc = hmfem_flux_device(c, g, Kx, Ky, Kz, bval, p, v, &out->div_min, &out->div_max);
+        if (rc) return rc;
+        HMFEM_CHECK(hipEventRecord(c->ev[7], c->st));
+        HMFEM_CHECK(hipEventSynchronize(c->ev[7]));
+        float tf = 0.f;
+        (void)hipEventElapsedTime(&tf, c->ev[6], c->ev[7]);
+        out->t_flux_ms = tf;
+    }
+    return out->status == HMFEM_OK ? HMFEM_OK : out->status;
+}
+
+extern "C" int hmfem_solve_structured_device(hmfem_context *c, int nx, int ny, int nz, const ELE *perm,
+                                             const IND *indcol, const double *b, double *x,
+                                             const hmfem_options *opt, hmfem_status *st)
+{
+    if (!c || nx < 1 || ny < 1 || nz < 1 || !perm || !indcol || !b || !x) return HMFEM_ERR_ARG;
+    Dims d{nx, ny, nz};
+    long n = d.n();
+    Stencil<double> A;
+    A.e = (double *)c->get("e", n * 8);
+    A.wx = (double *)c->get("wx", n * 8);
+    A.wy = (double *)c->get("wy", n * 8);
+    A.wz = (double *)c->get("wz", n * 8);
+    int *ok = (int *)c->get("ok", 64);
+    if (!A.e || !A.wx || !A.wy || !A.wz || !ok) return HMFEM_ERR_NOMEM;
+    int one = 1;
+    HMFEM_CHECK(hipMemcpyAsync(ok, &one, sizeof(int), hipMemcpyHostToDevice, c->st));
+    int rc = launch_perm_to_stencil(c->st, nx, ny, nz, perm, indcol, A.e, A.wx, A.wy, A.wz, ok);
+    if (rc) return rc;
+    int hok = 0;
+    HMFEM_CHECK(hipMemcpyAsync(&hok, ok, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HMFEM_CHECK(hipStreamSynchronize(c->st));
+    if (!hok) return HMFEM_ERR_NOT_STRUCTURED;
+    hmfem_status tmp{};
+    hmfem_status *out = st ? st : &tmp;
+    rc = solve_on_stencil(c, d, A, b, x, opt, out);
+    if (rc) return rc;
+    return out->status == HMFEM_OK ? HMFEM_OK : out->status;
+}
+
+extern "C" double hmfem_kernel_bytes_per_cell(int which, int precision)
+{
+    double vs = precision == HMFEM_PREC_FP64 ? 8.0 : 4.0;
+    switch (which) {
+    case 0: return 32 + 8 + vs + 8 + 8;
+    case 1: return 4 * vs + 8 + 2 * vs;
+    case 2: return 4 * vs + 8 + vs;
+    default: return 0.0;
+    }
+}
+
+// ------------------------------------------------------------ host-buffer drop-ins
+
+static double env_rtol()
+{
+    const char *s = getenv("HMFEM_RTOL");
+    double r = s ? atof(s) : 0.0;
+    return r > 0 ? r : 1e-8;
+}
+
+static void face_types(FACE f, int bc[6])
+{
+    bc[0] = f.l; bc[1] = f.r; bc[2] = f.u; bc[3] = f.d; bc[4] = f.t; bc[5] = f.b;
+}
+
+template <typename T>
+static T *upload(hmfem_context *c, const char *name, const T *h, size_t count)
+{
+    T *d = (T *)c->get(name, count * sizeof(T) + 16);
+    if (d && h) (void)hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, c->st);
+    return d;
+}
+
+static void fatal(const char *what, int rc)
+{
+    fprintf(stderr, "hmfem_gpu: %s failed (code %d)\n", what, rc);
+    exit(1);
+}
+
+// Device copies of the six Bound arrays, sized by face.
+static void upload_bound(hmfem_context *c, const Bound *B, Dim N, const double *dv[6])
+{
+    size_t nyz = (size_t)N.y * N.z, nxz = (size_t)N.x * N.z, nxy = (size_t)N.x * N.y;
+    dv[0] = upload(c, "B_l", B->l, nyz);
+    dv[1] = upload(c, "B_r", B->r, nyz);
+    dv[2] = upload(c, "B_u", B->u, nxz);
+    dv[3] = upload(c, "B_d", B->d, nxz);
+    dv[4] = upload(c, "B_t", B->t, nxy);
+    dv[5] = upload(c, "B_b", B->b, nxy);
+}
+
+static hmfem_grid grid_of(Dim N, grid h, FACE f, double mu)
+{
+    hmfem_grid g;
+    g.nx = N.x; g.ny = N.y; g.nz = N.z;
+    g.hx = h.x; g.hy = h.y; g.hz = h.z;
+    g.viscosidade = mu;
+    face_types(f, g.bc_type);
+    return g;
+}
+
+extern "C" int permeabilidade_arestas(Perm *K, ELE *perm, IND *indcol, grid h, Dim N, FACE bt, double *WI,
+                                      double *KI, double viscosidade)
+{
+    hmfem_context *c = default_ctx();
+    hmfem_grid g = grid_of(N, h, bt, viscosidade);
+    size_t n = (size_t)N.t;
+    const double *dKx = upload(c, "Kx", K->X, n), *dKy = upload(c, "Ky", K->Y, n), *dKz = upload(c, "Kz", K->Z, n);
+    const double *dWI = WI ? upload(c, "WI", WI, n) : nullptr, *dKI = KI ? upload(c, "KI", KI, n) : nullptr;
+    ELE *dperm = (ELE *)c->get("perm", n * sizeof(ELE));
+    IND *dind = (IND *)c->get("indcol", n * sizeof(IND));
+    double *db = (double *)c->get("b_tmp", n * 8);
+    const double *zero_faces[6];
+    double *z = (double *)c->get("zero_face", 8 * ((size_t)N.x * N.y + (size_t)N.y * N.z + (size_t)N.x * N.z + 8));
+    (void)hipMemsetAsync(z, 0, 8 * ((size_t)N.x * N.y + (size_t)N.y * N.z + (size_t)N.x * N.z + 8), c->st);
+    for (int f = 0; f < 6; f++) zero_faces[f] = z;
+    int rc = launch_assemble(c->st, &g, dKx, dKy, dKz, dKI, dWI, nullptr, nullptr, zero_faces, nullptr, nullptr,
+                             nullptr, nullptr, db, dperm, dind);
+    if (rc) fatal("permeabilidade_arestas", rc);
+    (void)hipMemcpyAsync(perm, dperm, n * sizeof(ELE), hipMemcpyDeviceToHost, c->st);
+    (void)hipMemcpyAsync(indcol, dind, n * sizeof(IND), hipMemcpyDeviceToHost, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("permeabilidade_arestas", HMFEM_ERR_HIP);
+    return 0;
+}
+
+extern "C" void source_term(Bound *B, ELE *perm, grid h, Dim N, FACE bt, double *pc_aux, double *pw, double *KI,
+                            double *WI, double viscosidade, double *b)
+{
+    hmfem_context *c = default_ctx();
+    hmfem_grid g = grid_of(N, h, bt, viscosidade);
+    size_t n = (size_t)N.t;
+    const double *dv[6];
+    upload_bound(c, B, N, dv);
+    ELE *dperm = upload(c, "perm", perm, n);
+    const double *dKI = KI ? upload(c, "KI", KI, n) : nullptr, *dWI = WI ? upload(c, "WI", WI, n) : nullptr;
+    const double *dpc = pc_aux ? upload(c, "pc", pc_aux, n) : nullptr, *dpw = pw ? upload(c, "pw", pw, n) : nullptr;
+    double *db = upload(c, "b_tmp", b, n);
+    int rc = launch_source_from_perm(c->st, &g, dperm, nullptr, dKI, dWI, dpc, dpw, dv, db);
+    if (rc) fatal("source_term", rc);
+    (void)hipMemcpyAsync(b, db, n * 8, hipMemcpyDeviceToHost, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("source_term", HMFEM_ERR_HIP);
+}
+
+static int structured_from_csr(int n, const int *cnt, const int *col, const double *ele, Dims &d,
+                               std::vector<double> &e, std::vector<double> &wx, std::vector<double> &wy,
+                               std::vector<double> &wz);
+
+extern "C" int compute(int *hdr, int *cnt, int *col, double *ele, double *b, double *x)
+{
+    hmfem_context *c = default_ctx();
+    int n = hdr[0];
+    Dims d;
+    std::vector<double> e, wx, wy, wz;
+    if (structured_from_csr(n, cnt, col, ele, d, e, wx, wy, wz) != 0) {
+        fprintf(stderr, "hmfem_gpu: compute(): the matrix is not a symmetric 7-point Cartesian stencil; "
+                        "general sparse input is not supported by this library\n");
+        return HMFEM_ERR_NOT_STRUCTURED;
+    }
+    Stencil<double> A;
+    A.e = upload(c, "e", e.data(), n);
+    A.wx = upload(c, "wx", wx.data(), n);
+    A.wy = upload(c, "wy", wy.data(), n);
+    A.wz = upload(c, "wz", wz.data(), n);
+    double *db = upload(c, "b", b, n);
+    double *dx = upload(c, "x", x, n);
+    hmfem_options opt;
+    hmfem_default_options(&opt);
+    opt.rtol = env_rtol();
+    hmfem_status st{};
+    int rc = solve_on_stencil(c, d, A, db, dx, &opt, &st);
+    if (rc) fatal("compute", rc);
+    (void)hipMemcpyAsync(x, dx, (size_t)n * 8, hipMemcpyDeviceToHost, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("compute", HMFEM_ERR_HIP);
+    iter_amg = st.iterations + 1;
+    return 0;
+}
+
+extern "C" void solve_linsys(ELE *perm, IND *indcol, double *b, double *x, Dim N)
+{
+    hmfem_context *c = default_ctx();
+    size_t n = (size_t)N.t;
+    bool any = false;
+    for (size_t s = 0; s < n && !any; s++) {
+        const IND &q = indcol[s];
+        any = q.d != -1 || q.l != -1 || q.c != -1 || q.r != -1 || q.u != -1 || q.t != -1 || q.b != -1;
+    }
+    if (!any) {   // solve_linearsystem_AMG.c:92-96
+        printf("ERROR: No nonzero elements. \n");
+        exit(0);
+    }
+    ELE *dperm = upload(c, "perm", perm, n);
+    IND *dind = upload(c, "indcol", indcol, n);
+    double *db = upload(c, "b", b, n);
+    double *dx = upload(c, "x", x, n);
+    hmfem_options opt;
+    hmfem_default_options(&opt);
+    opt.rtol = env_rtol();
+    hmfem_status st{};
+    int rc = hmfem_solve_structured_device(c, N.x, N.y, N.z, dperm, dind, db, dx, &opt, &st);
+    if (rc == HMFEM_ERR_NOT_STRUCTURED) {
+        // not the Cartesian pattern: build the count-CSR as the reference does and use compute()
+        std::vector<int> cnt(n), col;
+        std::vector<double> ele;
+        for (size_t s = 0; s < n; s++) {
+            int k = 0;
+            const IND &q = indcol[s];
+            const ELE &p = perm[s];
+            if (q.d != -1) { col.push_back(q.d); ele.push_back(-p.d); k++; }
+            if (q.l != -1) { col.push_back(q.l); ele.push_back(-p.l); k++; }
+            if (q.c != -1) { col.push_back(q.c); ele.push_back(p.c); k++; }
+            if (q.r != -1) { col.push_back(q.r); ele.push_back(-p.r); k++; }
+            if (q.u != -1) { col.push_back(q.u); ele.push_back(-p.u); k++; }
+            if (q.t != -1) { col.push_back(q.t); ele.push_back(-p.t); k++; }
+            if (q.b != -1) { col.push_back(q.b); ele.push_back(-p.b); k++; }
+            cnt[s] = k;
+        }
+        int hdr[2] = {(int)n, (int)col.size()};
+        if (compute(hdr, cnt.data(), col.data(), ele.data(), b, x) != 0) fatal("solve_linsys", HMFEM_ERR_NOT_STRUCTURED);
+        return;
+    }
+    if (rc < 0) fatal("solve_linsys", rc);
+    (void)hipMemcpyAsync(x, dx, n * 8, hipMemcpyDeviceToHost, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("solve_linsys", HMFEM_ERR_HIP);
+    iter_amg = st.iterations + 1;
+}
+
+static void divergence_print(double mn, double mx) { printf("Divv_min = %e, Divv_max = %e\n", mn, mx); }
+
+extern "C" int fluxos(double *p, ELE *perm, ELE *v, Bound *B, grid h, Dim N)
+{
+    hmfem_context *c = default_ctx();
+    hmfem_grid g = grid_of(N, h, boundary, 1.0);
+    size_t n = (size_t)N.t;
+    const double *dv[6];
+    upload_bound(c, B, N, dv);
+    ELE *dperm = upload(c, "perm", perm, n);
+    double *dp = upload(c, "x", p, n);
+    ELE *dvout = upload(c, "vout", v, n);   // keeps v[s].c, which fluxos() does not write
+    double *part = (double *)c->get("mm_partial", (size_t)2 * grid_for(n, kBlock) * 8 + 64);
+    unsigned *cnt = (unsigned *)c->get("mm_counter", 256);
+    double *out = (double *)c->get("mm_out", 64);
+    if (!c->bufs["mm_init"].first) {
+        (void)hipMemsetAsync(cnt, 0, 256, c->st);
+        c->get("mm_init", 8);
+    }
+    int rc = launch_flux(c->st, &g, nullptr, nullptr, nullptr, dperm, dv, dp, dvout, part, cnt, out);
+    if (rc) fatal("fluxos", rc);
+    double hmm[2];
+    (void)hipMemcpyAsync(v, dvout, n * sizeof(ELE), hipMemcpyDeviceToHost, c->st);
+    (void)hipMemcpyAsync(hmm, out, 16, hipMemcpyDeviceToHost, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("fluxos", HMFEM_ERR_HIP);
+    divergence_print(hmm[0], hmm[1]);
+    return 0;
+}
+
+extern "C" void divvel_pontual(ELE *v, grid h, Dim N)
+{
+    hmfem_context *c = default_ctx();
+    hmfem_grid g = grid_of(N, h, boundary, 1.0);
+    size_t n = (size_t)N.t;
+    ELE *dvv = upload(c, "vout", v, n);
+    double *part = (double *)c->get("mm_partial", (size_t)2 * grid_for(n, kBlock) * 8 + 64);
+    unsigned *cnt = (unsigned *)c->get("mm_counter", 256);
+    double *out = (double *)c->get("mm_out", 64);
+    if (!c->bufs["mm_init"].first) {
+        (void)hipMemsetAsync(cnt, 0, 256, c->st);
+        c->get("mm_init", 8);
+    }
+    int rc = launch_divergence(c->st, &g, dvv, part, cnt, out);
+    if (rc) fatal("divvel_pontual", rc);
+    double hmm[2];
+    (void)hipMemcpyAsync(hmm, out, 16, hipMemcpyDeviceToHost, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("divvel_pontual", HMFEM_ERR_HIP);
+    divergence_print(hmm[0], hmm[1]);
+}
+
+extern "C" void prod_mat_vec(ELE *perm, IND *indcol, double *x, double *res, Dim N)
+{
+    hmfem_context *c = default_ctx();
+    size_t n = (size_t)N.t;
+    ELE *dperm = upload(c, "perm", perm, n);
+    IND *dind = upload(c, "indcol", indcol, n);
+    double *dx = upload(c, "x", x, n);
+    double *dres = upload(c, "res", res, n);
+    int rc = launch_prod_mat_vec(c->st, (long)n, dperm, dind, dx, dres);
+    if (rc) fatal("prod_mat_vec", rc);
+    (void)hipMemcpyAsync(res, dres, n * 8, hipMemcpyDeviceToHost, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("prod_mat_vec", HMFEM_ERR_HIP);
+}
+
+// Recognise a count-CSR matrix that is a symmetric 7-point operator on an
+// nx*ny*nz Cartesian grid (x fastest) and convert it to flux form. This is
+// input validation and layout conversion; the solve itself runs on the GPU.
+static int structured_from_csr(int n, const int *cnt, const int *col, const double *ele, Dims &d,
+                               std::vector<double> &e, std::vector<double> &wx, std::vector<double> &wy,
+                               std::vector<double> &wz)
+{
+    if (n < 1) return -1;
+    long offs[3] = {0, 0, 0};
+    int noff = 0;
+    long m = 0;
+    for (int i = 0; i < n; i++)
+        for (int q = 0; q < cnt[i]; q++, m++) {
+            long o = (long)col[m] - i;
+            if (col[m] < 0 || col[m] >= n) return -1;
+            if (o <= 0) continue;
+            bool seen = false;
+            for (int t = 0; t < noff; t++) seen |= offs[t] == o;
+            if (!seen) {
+                if (noff == 3) return -1;
+                offs[noff++] = o;
+            }
+        }
+    std::sort(offs, offs + noff);
+    long sx = 0, sy = 0, sz = 0;
+    if (noff == 0) { d = Dims{n, 1, 1}; }
+    else if (offs[0] != 1) return -1;
+    else if (noff == 1) { d = Dims{n, 1, 1}; }
+    else if (noff == 2) {
+        if (n % offs[1]) return -1;
+        d = Dims{(int)offs[1], (int)(n / offs[1]), 1};
+    } else {
+        if (offs[2] % offs[1] || n % offs[2]) return -1;
+        d = Dims{(int)offs[1], (int)(offs[2] / offs[1]), (int)(n / offs[2])};
+    }
+    sx = 1; sy = d.nx; sz = (long)d.nx * d.ny;
+    e.assign(n, 0.0); wx.assign(n, 0.0); wy.assign(n, 0.0); wz.assign(n, 0.0);
+    std::vector<double> wxm(n, 0.0), wym(n, 0.0), wzm(n, 0.0), dia(n, 0.0);
+    m = 0;
+    for (int s = 0; s < n; s++) {
+        int i = (int)(s % d.nx), j = (int)((s / d.nx) % d.ny), k = (int)(s / ((long)d.nx * d.ny));
+        for (int q = 0; q < cnt[s]; q++, m++) {
+            long o = (long)col[m] - s;
+            double a = ele[m];
+            if (o == 0) dia[s] += a;
+            else if (o == sx && i < d.nx - 1) wx[s] -= a;
+            else if (o == -sx && i > 0) wxm[s] -= a;
+            else if (o == sy && j < d.ny - 1) wy[s] -= a;
+            else if (o == -sy && j > 0) wym[s] -= a;
+            else if (o == sz && k < d.nz - 1) wz[s] -= a;
+            else if (o == -sz && k > 0) wzm[s] -= a;
+            else return -1;
+        }
+    }
+    for (int s = 0; s < n; s++) {
+        int i = (int)(s % d.nx), j = (int)((s / d.nx) % d.ny), k = (int)(s / ((long)d.nx * d.ny));
+        double tol = 1e-12 * fabs(dia[s]);
+        if (i < d.nx - 1 && fabs(wx[s] - wxm[s + 1]) > tol) return -1;
+        if (j < d.ny - 1 && fabs(wy[s] - wym[s + sy]) > tol) return -1;
+        if (k < d.nz - 1 && fabs(wz[s] - wzm[s + sz]) > tol) return -1;
+        double off = wx[s] + wxm[s] + wy[s] + wym[s] + wz[s] + wzm[s];
+        double ex = dia[s] - off;
+        if (ex < 0 && -ex <= 64 * 2.220446049250313e-16 * fabs(dia[s])) ex = 0.0;
+        e[s] = ex;
+    }
+    return 0;
+}

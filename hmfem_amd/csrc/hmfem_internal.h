@@ -1,0 +1,138 @@
+// Internal declarations shared by the HIP translation units of libhmfem_gpu.
+//
+// Data layout in HBM (one grid of nx*ny*nz cells, s = i + nx*(j + ny*k)):
+//   a level of the hierarchy is a structured 7-point operator in flux form,
+//     (A u)_s = e_s u_s + sum_faces w_f (u_s - u_nb),
+//   stored as four SoA arrays: e (boundary/source "excess" of the diagonal),
+//   wx, wy, wz (coupling to the +x, +y, +z neighbour; 0 on the last plane).
+//   The -x coupling of cell s is wx[s-1], and so on, so each face is stored
+//   once and the operator is symmetric by construction.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/hmfem_gpu.h"
+
+#define HMFEM_CHECK(call)                                                                      \
+    do {                                                                                       \
+        hipError_t _e = (call);                                                                \
+        if (_e != hipSuccess) {                                                                \
+            fprintf(stderr, "hmfem_gpu: %s failed at %s:%d: %s\n", #call, __FILE__, __LINE__, \
+                    hipGetErrorString(_e));                                                    \
+            return HMFEM_ERR_HIP;                                                              \
+        }                                                                                      \
+    } while (0)
+
+namespace hmfem {
+
+constexpr int kBlock = 256;        // threads per block for stencil / vector kernels
+constexpr int kCellsPerThread = 4; // cells per thread in reduction kernels
+constexpr int kMaxLevels = 24;
+constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, blocks dispatched round-robin
+constexpr int kCoarseMax = 80;     // largest coarsest level inverted densely in LDS (FP64, <= 64 KB)
+
+struct Dims {
+    int nx, ny, nz;
+    long n() const { return (long)nx * ny * nz; }
+};
+
+// Structured stencil level. T is the storage type of the coefficients.
+template <typename T>
+struct Stencil {
+    T *e = nullptr, *wx = nullptr, *wy = nullptr, *wz = nullptr;
+};
+
+struct Level {
+    Dims d{0, 0, 0};
+    int fx = 1, fy = 1, fz = 1;      // aggregation factors towards the next level
+    Stencil<double> s64;             // FP64 coefficients (setup, FP64 V-cycle)
+    Stencil<float> s32;              // FP32 coefficients (FP32 V-cycle)
+    Stencil<__half> s16;             // FP16 scaled fine-level values (ablation)
+    float *sc = nullptr;             // FP16 row scaling (sqrt of diagonal), fine level only
+    void *f = nullptr, *u = nullptr, *t = nullptr; // V-cycle vectors (float or double)
+};
+
+// Device scalars of one CG solve, kept in one small allocation.
+struct CgScalars {
+    double bb, rr, sigma, sigma_first, sigma_last, tau, alpha, beta, gamma;
+    double rtol2;
+    int done, iter, maxit, stop_rule, status, pad;
+    unsigned int counter[8];         // last-block-done counters of the reductions
+};
+
+// Grid launch helpers with an XCD-aware block order: logical block b of a
+// launch with G blocks (G a multiple of 8) is placed so that each XCD gets a
+// contiguous range of the domain and keeps its neighbours in its own L2.
+__device__ __forceinline__ int xcd_block()
+{
+    int G = gridDim.x;
+    int b = blockIdx.x;
+    return (b % kNumXcd) * (G / kNumXcd) + b / kNumXcd;
+}
+
+inline int grid_for(long items, int per_block)
+{
+    long g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
+    return (int)g;
+}
+
+} // namespace hmfem
+
+namespace hmfem {
+
+// launchers defined in hmfem_assemble.hip
+int launch_assemble(hipStream_t st, const hmfem_grid *h, const double *Kx, const double *Ky, const double *Kz,
+                    const double *KI, const double *WI, const double *pc, const double *pw,
+                    const double *const bval[6], double *e, double *wx, double *wy, double *wz, double *b,
+                    ELE *perm, IND *indcol);
+int launch_source_from_perm(hipStream_t st, const hmfem_grid *h, const ELE *perm, const IND *indcol,
+                            const double *KI, const double *WI, const double *pc, const double *pw,
+                            const double *const bval[6], double *b);
+int launch_perm_to_stencil(hipStream_t st, int nx, int ny, int nz, const ELE *perm, const IND *indcol, double *e,
+                           double *wx, double *wy, double *wz, int *ok);
+int launch_flux(hipStream_t st, const hmfem_grid *h, const double *Kx, const double *Ky, const double *Kz,
+                const ELE *perm, const double *const bval[6], const double *p, ELE *v, double *partial,
+                unsigned int *counter, double *out);
+int launch_divergence(hipStream_t st, const hmfem_grid *h, const ELE *v, double *partial, unsigned int *counter,
+                      double *out);
+int launch_prod_mat_vec(hipStream_t st, long n, const ELE *perm, const IND *indcol, const double *x, double *res);
+
+// per-kernel timing collected by the solver (profile classes below)
+enum {
+    HMFEM_PROF_CG_SPMV = 0,
+    HMFEM_PROF_CG_UPDATE,
+    HMFEM_PROF_CG_DOTS,
+    HMFEM_PROF_SMOOTH0,
+    HMFEM_PROF_RESTRICT,
+    HMFEM_PROF_PROLONG,
+    HMFEM_PROF_JACOBI,
+    HMFEM_PROF_COARSE,
+    HMFEM_PROF_TYPES
+};
+struct ProfileSink {
+    hipEvent_t *ev;
+    int *type;
+    int cap;
+    double ms[HMFEM_PROF_TYPES];
+    long count[HMFEM_PROF_TYPES];
+    double bytes[HMFEM_PROF_TYPES];
+};
+
+// defined in hmfem_solver.hip
+struct CgBuffers {
+    double *r, *s0, *s1, *v;
+    CgScalars *sc;
+    double *partial;
+};
+size_t hierarchy_bytes(Dims fine, int coarse_max, int prec);
+size_t cg_partial_count(long n);
+int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
+                     const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
+                     int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink);
+
+} // namespace hmfem

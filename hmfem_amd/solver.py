@@ -1,0 +1,302 @@
+"""Python host layer over libhmfem_gpu's C ABI (include/hmfem_gpu.h).
+
+Two groups of functions, mirroring the library:
+
+* the reference's own functions on the pressure path, same names and
+  argument meaning, on host numpy arrays (permeabilidade_arestas,
+  source_term, solve_linsys, compute, fluxos, divvel_pontual, prod_mat_vec);
+* a device-resident `Context` whose methods take device pointers (for
+  example torch CUDA tensors) and run the whole reservoir() step on the GPU.
+
+There is no CPU fallback: every call goes through the HIP library and
+raises if it is not built or the GPU call fails.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import load_gpu
+
+ELE = np.dtype([(f, "<f8") for f in ("u", "d", "l", "r", "c", "t", "b")])
+IND = np.dtype([(f, "<i4") for f in ("c", "u", "d", "l", "r", "t", "b")])
+
+c_int, c_dbl, c_vp, c_long = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_long
+
+PREC_FP64, PREC_FP32, PREC_FP16 = 0, 1, 2
+STOP_RESIDUAL, STOP_PRECONDITIONED = 0, 1
+OK, NOT_CONVERGED = 0, 1
+PROFILE_CLASSES = ("cg_spmv", "cg_update", "cg_dots", "smooth_zero", "resid_restrict",
+                   "prolong_jacobi", "jacobi", "coarse")
+
+
+class Dim(ctypes.Structure):      # structs.h:93-104
+    _fields_ = [(f, c_int) for f in ("x", "y", "z", "xy", "t", "px", "pz", "py")]
+
+
+class GridH(ctypes.Structure):    # structs.h:83-88 (`grid`)
+    _fields_ = [("x", c_dbl), ("y", c_dbl), ("z", c_dbl)]
+
+
+class FACE(ctypes.Structure):     # structs.h:50-59
+    _fields_ = [(f, c_int) for f in ("u", "d", "l", "r", "t", "b")]
+
+
+class Bound(ctypes.Structure):    # structs.h:27-36
+    _fields_ = [(f, c_vp) for f in ("u", "d", "l", "r", "t", "b")]
+
+
+class Perm(ctypes.Structure):     # structs.h:71-79
+    _fields_ = [("X", c_vp), ("Y", c_vp), ("Z", c_vp), ("S", c_vp), ("permeability", c_int)]
+
+
+class HmfemGrid(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("hx", c_dbl), ("hy", c_dbl), ("hz", c_dbl),
+                ("viscosidade", c_dbl), ("bc_type", c_int * 6)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("rtol", c_dbl), ("max_iterations", c_int), ("precision", c_int), ("stop_rule", c_int),
+                ("nu", c_int), ("omega", c_dbl), ("ocf", c_dbl), ("coarse_max", c_int), ("check_every", c_int)]
+
+
+class Status(ctypes.Structure):
+    _fields_ = [("status", c_int), ("iterations", c_int), ("levels", c_int), ("rel_residual", c_dbl),
+                ("precond_ratio", c_dbl), ("t_assemble_ms", c_dbl), ("t_setup_ms", c_dbl),
+                ("t_solve_ms", c_dbl), ("t_flux_ms", c_dbl), ("div_max", c_dbl), ("div_min", c_dbl),
+                ("solve_bytes", c_dbl)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_typed = False
+
+
+def lib():
+    global _typed
+    L = load_gpu()
+    if not _typed:
+        P = ctypes.POINTER
+        L.solve_linsys.argtypes = [c_vp, c_vp, c_vp, c_vp, Dim]
+        L.solve_linsys.restype = None
+        L.compute.argtypes = [c_vp] * 6
+        L.permeabilidade_arestas.argtypes = [P(Perm), c_vp, c_vp, GridH, Dim, FACE, c_vp, c_vp, c_dbl]
+        L.source_term.argtypes = [P(Bound), c_vp, GridH, Dim, FACE, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp]
+        L.source_term.restype = None
+        L.fluxos.argtypes = [c_vp, c_vp, c_vp, P(Bound), GridH, Dim]
+        L.divvel_pontual.argtypes = [c_vp, GridH, Dim]
+        L.divvel_pontual.restype = None
+        L.prod_mat_vec.argtypes = [c_vp, c_vp, c_vp, c_vp, Dim]
+        L.prod_mat_vec.restype = None
+        L.hmfem_default_options.argtypes = [P(Options)]
+        L.hmfem_default_options.restype = None
+        L.hmfem_create.argtypes = [P(c_vp)]
+        L.hmfem_destroy.argtypes = [c_vp]
+        L.hmfem_destroy.restype = None
+        L.hmfem_stream.argtypes = [c_vp]
+        L.hmfem_stream.restype = c_vp
+        L.hmfem_assemble_device.argtypes = [c_vp, P(HmfemGrid)] + [c_vp] * 7 + [c_vp, c_vp, c_vp, c_vp]
+        L.hmfem_reservoir_device.argtypes = [c_vp, P(HmfemGrid)] + [c_vp] * 7 + [c_vp, c_vp, c_vp, P(Options),
+                                                                              P(Status)]
+        L.hmfem_flux_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(c_dbl), P(c_dbl)]
+        L.hmfem_solve_structured_device.argtypes = [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, P(Options),
+                                                    P(Status)]
+        L.hmfem_profile_enable.argtypes = [c_vp, c_int]
+        L.hmfem_profile_read.argtypes = [c_vp, c_vp, c_vp, c_vp, c_int]
+        L.hmfem_profile_reset.argtypes = [c_vp]
+        L.hmfem_profile_reset.restype = None
+        L.hmfem_kernel_bytes_per_cell.argtypes = [c_int, c_int]
+        L.hmfem_kernel_bytes_per_cell.restype = c_dbl
+        _typed = True
+    return L
+
+
+EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", "fluxos", "divvel_pontual",
+            "prod_mat_vec", "hmfem_default_options", "hmfem_create", "hmfem_destroy", "hmfem_stream",
+            "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device",
+            "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
+            "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "iter_amg", "boundary")
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def make_dim(nx, ny, nz):
+    return Dim(nx, ny, nz, nx * ny, nx * ny * nz, nx // 2, 0, ny // 2)
+
+
+def face_types(bc_type):
+    """(l, r, u, d, t, b) -> FACE"""
+    l, r, u, d, t, b = (int(v) for v in bc_type)
+    return FACE(u=u, d=d, l=l, r=r, t=t, b=b)
+
+
+def default_options(**kw):
+    o = Options()
+    lib().hmfem_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+# ----------------------------------------------------------------- reference names
+
+
+def iter_amg():
+    return c_int.in_dll(lib(), "iter_amg").value
+
+
+def set_boundary(bc_type):
+    """Set the library's global `boundary` (read by fluxos, as in the reference)."""
+    FACE.in_dll(lib(), "boundary").__init__(*[0] * 6)
+    f = FACE.in_dll(lib(), "boundary")
+    src = face_types(bc_type)
+    for name, _ in FACE._fields_:
+        setattr(f, name, getattr(src, name))
+
+
+def permeabilidade_arestas(Kx, Ky, Kz, perm, indcol, h, N, bc_type, WI, KI, viscosidade):
+    K = Perm(_p(Kx), _p(Ky), _p(Kz), None, 1)
+    rc = lib().permeabilidade_arestas(ctypes.byref(K), _p(perm), _p(indcol), GridH(*h), N, face_types(bc_type),
+                                      _p(WI), _p(KI), viscosidade)
+    if rc != 0:
+        raise RuntimeError("permeabilidade_arestas failed: %d" % rc)
+
+
+def bound_struct(faces):
+    """faces: six arrays in face order l, r, u, d, t, b."""
+    l, r, u, d, t, b = (_p(a) for a in faces)
+    return Bound(u=u, d=d, l=l, r=r, t=t, b=b)
+
+
+def source_term(faces, perm, h, N, bc_type, pc, pw, KI, WI, viscosidade, b):
+    B = bound_struct(faces)
+    lib().source_term(ctypes.byref(B), _p(perm), GridH(*h), N, face_types(bc_type), _p(pc), _p(pw), _p(KI), _p(WI),
+                      viscosidade, _p(b))
+
+
+def solve_linsys(perm, indcol, b, x, N):
+    lib().solve_linsys(_p(perm), _p(indcol), _p(b), _p(x), N)
+
+
+def compute(hdr, cnt, col, ele, b, x):
+    rc = lib().compute(_p(hdr), _p(cnt), _p(col), _p(ele), _p(b), _p(x))
+    if rc != 0:
+        raise RuntimeError("compute failed: %d" % rc)
+    return rc
+
+
+def fluxos(p, perm, v, faces, h, N):
+    B = bound_struct(faces)
+    return lib().fluxos(_p(p), _p(perm), _p(v), ctypes.byref(B), GridH(*h), N)
+
+
+def divvel_pontual(v, h, N):
+    lib().divvel_pontual(_p(v), GridH(*h), N)
+
+
+def prod_mat_vec(perm, indcol, x, res, N):
+    lib().prod_mat_vec(_p(perm), _p(indcol), _p(x), _p(res), N)
+
+
+# ----------------------------------------------------------------- device API
+
+
+def _dptr(t):
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+class Context:
+    """A HIP stream plus cached device buffers of the library."""
+
+    def __init__(self):
+        h = c_vp()
+        rc = lib().hmfem_create(ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError("hmfem_create failed: %d (no HIP device?)" % rc)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hmfem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return lib().hmfem_stream(self.h)
+
+    @staticmethod
+    def grid(nx, ny, nz, h=None, viscosidade=1.0, bc_type=(0, 0, 1, 1, 1, 1)):
+        if h is None:
+            h = (1.0 / nx, 1.0 / ny, 1.0 / nz)
+        g = HmfemGrid(nx, ny, nz, h[0], h[1], h[2], viscosidade)
+        for f in range(6):
+            g.bc_type[f] = int(bc_type[f])
+        return g
+
+    @staticmethod
+    def _faces(bval):
+        arr = (c_vp * 6)(*[_dptr(t) for t in bval])
+        return arr
+
+    def assemble(self, g, Kx, Ky, Kz, bval, b, perm=None, indcol=None, KI=None, WI=None, pc=None, pw=None):
+        rc = lib().hmfem_assemble_device(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz), _dptr(KI),
+                                         _dptr(WI), _dptr(pc), _dptr(pw), self._faces(bval), _dptr(perm),
+                                         _dptr(indcol), _dptr(b))
+        if rc != 0:
+            raise RuntimeError("hmfem_assemble_device failed: %d" % rc)
+
+    def reservoir(self, g, Kx, Ky, Kz, bval, p, v=None, options=None, KI=None, WI=None, pc=None, pw=None):
+        st = Status()
+        o = options if options is not None else default_options()
+        rc = lib().hmfem_reservoir_device(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz), _dptr(KI),
+                                          _dptr(WI), _dptr(pc), _dptr(pw), self._faces(bval), _dptr(p), _dptr(v),
+                                          ctypes.byref(o), ctypes.byref(st))
+        if rc < 0:
+            raise RuntimeError("hmfem_reservoir_device failed: %d" % rc)
+        return st
+
+    def flux(self, g, Kx, Ky, Kz, bval, p, v=None):
+        mn, mx = c_dbl(), c_dbl()
+        rc = lib().hmfem_flux_device(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz), self._faces(bval),
+                                     _dptr(p), _dptr(v), ctypes.byref(mn), ctypes.byref(mx))
+        if rc != 0:
+            raise RuntimeError("hmfem_flux_device failed: %d" % rc)
+        return mn.value, mx.value
+
+    def solve_structured(self, nx, ny, nz, perm, indcol, b, x, options=None):
+        st = Status()
+        o = options if options is not None else default_options()
+        rc = lib().hmfem_solve_structured_device(self.h, nx, ny, nz, _dptr(perm), _dptr(indcol), _dptr(b), _dptr(x),
+                                                 ctypes.byref(o), ctypes.byref(st))
+        if rc < 0:
+            raise RuntimeError("hmfem_solve_structured_device failed: %d" % rc)
+        return st
+
+    def profile_enable(self, max_launches=20000):
+        rc = lib().hmfem_profile_enable(self.h, max_launches)
+        if rc != 0:
+            raise RuntimeError("hmfem_profile_enable failed: %d" % rc)
+
+    def profile_reset(self):
+        lib().hmfem_profile_reset(self.h)
+
+    def profile_read(self):
+        k = len(PROFILE_CLASSES)
+        ms = np.zeros(k)
+        cnt = np.zeros(k, np.int64)
+        by = np.zeros(k)
+        lib().hmfem_profile_read(self.h, _p(ms), _p(cnt), _p(by), k)
+        return {name: dict(ms=float(ms[i]), count=int(cnt[i]), bytes_per_launch=float(by[i]))
+                for i, name in enumerate(PROFILE_CLASSES)}

@@ -1,0 +1,185 @@
+/*
+ * libhmfem_gpu: MI355X pressure solver for hmfem's cell-centred Darcy model.
+ *
+ * C ABI only (plain pointers and sizes). Two layers:
+ *
+ *  1. Drop-in replacements for the reference's functions on the pressure
+ *     path, same names, signatures and argument meaning. A host that keeps
+ *     reservoir() (reservoir.c:8-57) links this library instead of the
+ *     reference's permeabilities.c / set_source_term.c /
+ *     solve_linearsystem_AMG.c / compute.cpp / flux_reconstruction.c.
+ *     Host buffers in, host buffers out; the work runs on the GPU.
+ *
+ *  2. A device-resident API (hmfem_*) with all arrays in HBM, options for
+ *     precision and stopping rule, and a status record.
+ *
+ * Types: if the reference's structs.h was included first (guard STRUCTS_H),
+ * its definitions are used; otherwise this header defines layout-compatible
+ * copies of the types the path needs.
+ */
+#ifndef HMFEM_GPU_H
+#define HMFEM_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef STRUCTS_H
+/* structs.h:7-17 */
+typedef struct ELEstruct { double u, d, l, r, c, t, b; } ELE;
+/* structs.h:27-36 */
+typedef struct Boundary { double *u, *d, *l, *r, *t, *b; } Bound;
+/* structs.h:38-48 */
+typedef struct Index { int c, u, d, l, r, t, b; } IND;
+/* structs.h:50-59 */
+typedef struct Faces { int u, d, l, r, t, b; } FACE;
+/* structs.h:71-79 */
+typedef struct Permeab { double *X, *Y, *Z; double **S; int permeability; } Perm;
+/* structs.h:83-88 */
+typedef struct Grid { double x, y, z; } grid;
+/* structs.h:93-104 */
+typedef struct Dimension { int x, y, z, xy, t, px, pz, py; } Dim;
+#endif
+
+/* ---------------------------------------------------------------- layer 1 */
+
+/* Globals the reference's path reads or writes (variables.h). Defined here
+   with default visibility: a host executable that defines them too wins by
+   ELF interposition, so the library and the host share one copy. */
+extern int iter_amg;    /* written by compute()/solve_linsys(): PCG loop count (compute.cpp:46) */
+extern FACE boundary;   /* read by fluxos() (flux_reconstruction.c:104-270) */
+
+/* solve_linearsystem_AMG.c:37 solve_linsys(); prototype functions.h:54.
+   perm/indcol as filled by permeabilidade_arestas(), b the RHS, x in/out
+   (initial guess in, pressure out). Stops at relative residual 1e-8
+   (HMFEM_RTOL overrides). Writes iter_amg. Exits like the reference when the
+   matrix has no entries. */
+void solve_linsys(ELE *perm, IND *indcol, double *b, double *x, Dim N);
+
+/* compute.cpp:7 compute(); prototype functions.h:53. Count-CSR: hdr[0] = n,
+   hdr[1] = nnz, row i holds the next cnt[i] entries, 0-based columns, FP64
+   values; the matrix must be symmetric. x in/out. Returns 0. */
+int compute(int *hdr, int *cnt, int *col, double *ele, double *b, double *x);
+
+/* permeabilities.c:196 permeabilidade_arestas(); functions.h:31. */
+int permeabilidade_arestas(Perm *K, ELE *perm, IND *indcol, grid h, Dim N, FACE boundary_types,
+                           double *WI, double *KI, double viscosidade);
+
+/* set_source_term.c:9 source_term(); functions.h:45. Accumulates into b. */
+void source_term(Bound *B, ELE *perm, grid h, Dim N, FACE boundary_types, double *pc_aux, double *pw,
+                 double *KI, double *WI, double viscosidade, double *b);
+
+/* flux_reconstruction.c:9 fluxos(); functions.h:50. Reads the global
+   `boundary`; writes u,d,l,r,t,b of v; prints the divergence line. */
+int fluxos(double *p, ELE *perm, ELE *v, Bound *B, grid h, Dim N);
+
+/* flux_reconstruction.c:281 divvel_pontual(); functions.h:51. */
+void divvel_pontual(ELE *v, grid h, Dim N);
+
+/* solve_linearsystem_AMG.c:160 prod_mat_vec(); functions.h:56. res += A x. */
+void prod_mat_vec(ELE *perm, IND *indcol, double *x, double *res, Dim N);
+
+/* ---------------------------------------------------------------- layer 2 */
+
+enum {
+    HMFEM_OK = 0,
+    HMFEM_NOT_CONVERGED = 1,
+    HMFEM_ERR_ARG = -1,
+    HMFEM_ERR_HIP = -2,
+    HMFEM_ERR_NOMEM = -3,
+    HMFEM_ERR_NOT_STRUCTURED = -4,
+    HMFEM_ERR_BREAKDOWN = -5
+};
+
+enum { HMFEM_PREC_FP64 = 0, HMFEM_PREC_FP32 = 1, HMFEM_PREC_FP16 = 2 };
+
+/* HMFEM_STOP_RESIDUAL: ||b - A x||_2 <= rtol ||b||_2.
+   HMFEM_STOP_PRECONDITIONED: <M r, r> <= rtol^2 <M r0, r0> (pcg.h:57). */
+enum { HMFEM_STOP_RESIDUAL = 0, HMFEM_STOP_PRECONDITIONED = 1 };
+
+typedef struct {
+    int nx, ny, nz;
+    double hx, hy, hz;
+    double viscosidade;
+    int bc_type[6];          /* faces l, r, u, d, t, b: 0 Dirichlet, 1 Neumann */
+} hmfem_grid;
+
+typedef struct {
+    double rtol;             /* stopping tolerance (default 1e-8) */
+    int max_iterations;      /* default 10000, as pcg.h's _max */
+    int precision;           /* storage/arithmetic of the AMG hierarchy (default FP32) */
+    int stop_rule;           /* default HMFEM_STOP_RESIDUAL */
+    int nu;                  /* Jacobi sweeps before and after the coarse correction (default 2) */
+    double omega;            /* Jacobi damping (default 0.8) */
+    double ocf;              /* coarse-correction scaling (default 1.7) */
+    int coarse_max;          /* largest coarsest level, solved densely (default 128, at most 128) */
+    int check_every;         /* CG iterations queued between host convergence checks (default 4) */
+} hmfem_options;
+
+typedef struct {
+    int status;              /* HMFEM_OK, HMFEM_NOT_CONVERGED or an error */
+    int iterations;          /* CG iterations performed */
+    int levels;              /* hierarchy levels including the dense coarsest */
+    double rel_residual;     /* ||b - A x|| / ||b|| at exit (recursively updated r) */
+    double precond_ratio;    /* sqrt(<M r, r> / <M r0, r0>) at exit */
+    double t_assemble_ms, t_setup_ms, t_solve_ms, t_flux_ms;
+    double div_max, div_min; /* divvel_pontual() of the fluxes, when computed */
+    double solve_bytes;      /* algorithmic HBM bytes of the solve phase */
+} hmfem_status;
+
+typedef struct hmfem_context hmfem_context;
+
+void hmfem_default_options(hmfem_options *opt);
+int hmfem_create(hmfem_context **ctx);
+void hmfem_destroy(hmfem_context *ctx);
+/* The HIP stream (hipStream_t) every call of this context is ordered on. */
+void *hmfem_stream(hmfem_context *ctx);
+
+/* Assembly only (permeabilidade_arestas + source_term), device pointers.
+   perm/indcol (AoS, may be NULL) and b (must be non-NULL, overwritten).
+   KI, WI, pc, pw may be NULL (zero). bval[f] is face f's value array in the
+   Bound layout (faces l, r, u, d, t, b). */
+int hmfem_assemble_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
+                          const double *Kz, const double *KI, const double *WI, const double *pc,
+                          const double *pw, const double *const bval[6], ELE *perm, IND *indcol, double *b);
+
+/* The whole reservoir() step on the device: assembly, AMG setup, FCG solve
+   (x in/out in p), then fluxes v (AoS, may be NULL) and their divergence. */
+int hmfem_reservoir_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
+                           const double *Kz, const double *KI, const double *WI, const double *pc,
+                           const double *pw, const double *const bval[6], double *p, ELE *v,
+                           const hmfem_options *opt, hmfem_status *st);
+
+/* Fluxes and divergence from a pressure (fluxos + divvel_pontual), device pointers. */
+int hmfem_flux_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
+                      const double *Kz, const double *const bval[6], const double *p, ELE *v,
+                      double *div_min, double *div_max);
+
+/* Solve on an already assembled structured operator given in AoS form (device
+   perm/indcol as from hmfem_assemble_device), b and x device FP64. */
+int hmfem_solve_structured_device(hmfem_context *ctx, int nx, int ny, int nz, const ELE *perm,
+                                  const IND *indcol, const double *b, double *x,
+                                  const hmfem_options *opt, hmfem_status *st);
+
+/* Per-kernel timing: with profiling enabled, every fine-level and CG launch
+   of the following calls is bracketed by HIP events on the context's stream
+   (capacity: max_launches launches per call). hmfem_profile_read returns, per
+   kernel class, the summed milliseconds, the launch count and the algorithmic
+   HBM bytes of one launch. Classes: 0 CG SpMV+dot, 1 CG update+dot, 2 CG
+   dots, 3 pre-smoothing, 4 residual+restriction, 5 prolongation+Jacobi,
+   6 Jacobi, 7 coarse levels. Returns the number of classes. */
+int hmfem_profile_enable(hmfem_context *ctx, int max_launches);
+int hmfem_profile_read(hmfem_context *ctx, double *ms, long *count, double *bytes, int nclasses);
+void hmfem_profile_reset(hmfem_context *ctx);
+
+/* Algorithmic HBM bytes per cell of the main kernels at a precision, for the
+   roofline record: which = 0 CG SpMV+dot, 1 fused Jacobi sweep, 2 residual+restriction. */
+double hmfem_kernel_bytes_per_cell(int which, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

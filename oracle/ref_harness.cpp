@@ -16,6 +16,9 @@
  *   <prefix>.v      fluxos() on p12, 7 doubles per cell (u,d,l,r,c,t,b)
  *   <prefix>.json   iteration counts, level sizes, timings, divergence
  *
+ * With a third argument "time" it only times one reservoir() step (assembly,
+ * solve_linsys at the reference's 1e-8, fluxos) and writes <prefix>.json.
+ *
  * Input file layout (all little-endian):
  *   int32  nx, ny, nz
  *   f64    Ix, Iy, Iz, viscosidade, eps_tight
@@ -34,6 +37,7 @@
 #include <sys/time.h>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 /* Globals that the linked reference sources read or write (variables.h). */
 Dim N;
@@ -175,6 +179,21 @@ int main(int argc, char **argv)
     solve_linsys(pm, ic, b, p8, N);
     double t_solve8 = now() - t0;
     int iter8 = iter_amg;
+
+    if (argc > 3 && strcmp(argv[3], "time") == 0) {
+        /* timing mode: one reservoir() step as the reference runs it
+           (assembly + solve_linsys at 1e-8) plus fluxos(); no outputs but JSON */
+        t0 = now();
+        fluxos(p8, pm, vv, &Bb, h, N);
+        double t_flux8 = now() - t0;
+        char name[2048];
+        snprintf(name, sizeof(name), "%s.json", argv[2]);
+        FILE *f = fopen(name, "w");
+        fprintf(f, "{\"n\": %d, \"iter_amg_1e-8\": %d, \"t_assemble\": %.6f, \"t_solve_1e-8\": %.6f, \"t_flux\": %.6f}\n",
+                n, iter8, t_assemble, t_solve8, t_flux8);
+        fclose(f);
+        return 0;
+    }
 
     /* Count-CSR exactly as solve_linsys() builds it (solve_linearsystem_AMG.c:57-143). */
     std::vector<int> cnt(n), col;

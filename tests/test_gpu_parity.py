@@ -1,0 +1,240 @@
+"""GPU parity: the HIP path through the C ABI against the oracle.
+
+Bars: assembly, right-hand side, fluxes and divergence bit for bit (FP64,
+no contraction); pressures within 1e-8 relative L2 of the oracle's AMG-PCG
+solution at 1e-12 when the GPU solve is driven to a true relative residual
+of 1e-12 (the reference's own iterate at its 1e-8 stop is itself ~1e-9 from
+that solution).
+"""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from gpu_helpers import (GOLD, Dev, face_arrays, make_problem, oracle_solution, rel_l2, torch_cuda,
+                         true_residual)
+from hmfem_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = sorted(glob.glob(os.path.join(GOLD, "*.npz")))
+PRESSURE_TOL = 1e-8
+
+
+def _names(paths):
+    return [os.path.splitext(os.path.basename(p))[0] for p in paths]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    return torch_cuda()
+
+
+@pytest.fixture(scope="module")
+def ctx(torch):
+    c = S.Context()
+    yield c
+    c.close()
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=_names(FIXTURES))
+def test_assembly_device_bitexact(path, ctx, torch):
+    z = np.load(path)
+    P = O.problem_from_fixture(z)
+    _say("assembly", os.path.basename(path), P.nx, P.ny, P.nz)
+    D = Dev(P, torch)
+    n = P.n
+    perm = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
+    indcol = torch.zeros(n * 7, dtype=torch.int32, device="cuda")
+    b = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ctx.assemble(D.g, D.K, D.K, D.K, D.bval, b, perm, indcol, D.KI, D.WI, D.pc, D.pw)
+    operm, oind, ob = O.assemble(P)
+    gperm = perm.cpu().numpy().view(O.ELE).reshape(n)
+    gind = indcol.cpu().numpy().view(O.IND).reshape(n)
+    assert np.array_equal(b.cpu().numpy().view(np.uint64), ob.view(np.uint64))
+    assert np.array_equal(b.cpu().numpy().view(np.uint64), z["b"].view(np.uint64))
+    for f in ("u", "d", "l", "r", "c", "t", "b"):
+        assert np.array_equal(gperm[f].view(np.uint64), operm[f].view(np.uint64)), f
+        assert np.array_equal(gind[f], oind[f]), f
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=_names(FIXTURES))
+def test_dropin_assembly_source_flux_bitexact(path):
+    z = np.load(path)
+    P = O.problem_from_fixture(z)
+    _say("drop-in assembly/flux", os.path.basename(path))
+    n = P.n
+    N = S.make_dim(P.nx, P.ny, P.nz)
+    perm = np.zeros(n, S.ELE)
+    indcol = np.zeros(n, S.IND)
+    S.permeabilidade_arestas(P.Kx, P.Ky, P.Kz, perm, indcol, P.h, N, P.bc_type, P.WI, P.KI, P.visc)
+    operm, oind, ob = O.assemble(P)
+    for f in ("u", "d", "l", "r", "c", "t", "b"):
+        assert np.array_equal(perm[f].view(np.uint64), operm[f].view(np.uint64)), f
+        assert np.array_equal(indcol[f], oind[f]), f
+    b = np.zeros(n)
+    faces = face_arrays(P)
+    S.source_term(faces, perm, P.h, N, P.bc_type, P.pc, P.pw, P.KI, P.WI, P.visc, b)
+    assert np.array_equal(b.view(np.uint64), z["b"].view(np.uint64))
+    # fluxes of the reference's own tight pressure
+    S.set_boundary(P.bc_type)
+    v = np.zeros(n, S.ELE)
+    S.fluxos(z["p12"], perm, v, faces, P.h, N)
+    ov = O.fluxos(P, z["p12"], operm)
+    for f in ("u", "d", "l", "r", "t", "b"):
+        assert np.array_equal(v[f].view(np.uint64), ov[f].view(np.uint64)), f
+    if "v" in z.files:
+        ref = z["v"]
+        for k, f in enumerate(("u", "d", "l", "r", "c", "t", "b")):
+            if f != "c":
+                assert np.array_equal(v[f].view(np.uint64), ref[:, k].view(np.uint64)), f
+    # prod_mat_vec accumulates, as the reference's
+    res = np.ones(n)
+    S.prod_mat_vec(perm, indcol, z["p12"], res, N)
+    ores = O.prod_mat_vec(operm, oind, z["p12"]) + 1.0
+    assert np.allclose(res, ores, rtol=1e-13, atol=1e-13 * np.abs(ores).max())
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=_names(FIXTURES))
+def test_dropin_solve_linsys_and_compute(path, monkeypatch):
+    z = np.load(path)
+    P = O.problem_from_fixture(z)
+    _say("drop-in solve", os.path.basename(path))
+    n = P.n
+    N = S.make_dim(P.nx, P.ny, P.nz)
+    operm, oind, ob = O.assemble(P)
+    monkeypatch.setenv("HMFEM_RTOL", "1e-12")
+    x = np.zeros(n)
+    S.solve_linsys(operm, oind, ob.copy(), x, N)
+    assert rel_l2(x, z["p12"]) < PRESSURE_TOL
+    assert S.iter_amg() >= 2
+    cnt, col, ele = O.build_csr(operm, oind)
+    hdr = np.array([n, col.size], np.int32)
+    x2 = np.zeros(n)
+    S.compute(hdr, cnt, col, ele, ob.copy(), x2)
+    assert rel_l2(x2, z["p12"]) < PRESSURE_TOL
+
+
+def test_c1_closed_form_device(ctx, torch):
+    """16^3 homogeneous: p_i = 1 - (i+1/2)/16, flux K*lambda, zero divergence,
+    for both hierarchy precisions and both stopping rules."""
+    P = make_problem((16, 16, 16), "homogeneous")
+    D = Dev(P, torch)
+    exact = 1.0 - (np.arange(P.n) % 16 + 0.5) / 16.0
+    for prec in (S.PREC_FP32, S.PREC_FP64):
+        for rule in (S.STOP_RESIDUAL, S.STOP_PRECONDITIONED):
+            _say("C1 precision", prec, "rule", rule)
+            p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+            v = torch.zeros(P.n * 7, dtype=torch.float64, device="cuda")
+            st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, v, S.default_options(rtol=1e-10, precision=prec,
+                                                                                    stop_rule=rule))
+            assert st.status == S.OK
+            assert rel_l2(p.cpu().numpy(), exact) < 1e-9
+            vv = v.cpu().numpy().view(S.ELE)
+            assert np.allclose(vv["r"], 1e-14, rtol=1e-8, atol=0)
+            assert st.div_max < 1e-8 * 1e-14 * 16
+
+
+@pytest.mark.parametrize("kind", ["lognormal", "karst"])
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_FP64])
+def test_pipeline_vs_oracle_32(kind, prec, ctx, torch):
+    P = make_problem((32, 32, 32), kind, seed=5, conduits=6, corr=4.0)
+    _say("pipeline 32^3", kind, prec)
+    xo, operm, oind, ob = oracle_solution(P, eps=1e-12)
+    D = Dev(P, torch)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    v = torch.zeros(P.n * 7, dtype=torch.float64, device="cuda")
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, v, S.default_options(rtol=1e-12, precision=prec))
+    assert st.status == S.OK, st.as_dict()
+    x = p.cpu().numpy()
+    assert rel_l2(x, xo) < PRESSURE_TOL
+    assert true_residual(operm, oind, ob, x) < 1e-11
+    # the device fluxes of this pressure equal the oracle's fluxes of it, bit for bit
+    ov = O.fluxos(P, x, operm)
+    vv = v.cpu().numpy().view(S.ELE)
+    for f in ("u", "d", "l", "r", "t", "b"):
+        assert np.array_equal(vv[f].view(np.uint64), ov[f].view(np.uint64)), f
+    dmin, dmax = O.divvel_pontual(P, ov)
+    assert st.div_max == dmax and st.div_min == dmin
+
+
+@pytest.mark.parametrize("dims,kind,bc", [
+    ((17, 9, 5), "lognormal", (0, 0, 1, 1, 1, 1)),          # odd sizes, ragged aggregates
+    ((300, 1, 1), "lognormal", (0, 0, 1, 1, 1, 1)),         # 1-D (a conduit-like line)
+    ((1, 1, 200), "homogeneous", (1, 1, 1, 1, 0, 0)),       # 1-D along z
+    ((24, 20, 1), "karst", (0, 1, 1, 0, 1, 1)),             # 2-D, mixed faces
+    ((4, 3, 5), "lognormal", (0, 0, 1, 1, 1, 1)),           # coarsest level only (dense)
+    ((6, 5, 4), "lognormal", (1, 1, 1, 1, 1, 1)),           # all Neumann: pinned cell
+])
+def test_edge_shapes_vs_oracle(dims, kind, bc, ctx, torch):
+    bval = (1.0, 0.25, 1e-14, 0.5, -1e-14, 2e-14)
+    P = make_problem(dims, kind, seed=9, corr=2.0, conduits=2, bc_type=bc, bc_val=bval)
+    _say("edge", dims, kind, bc)
+    xo, operm, oind, ob = oracle_solution(P, eps=1e-13)
+    D = Dev(P, torch)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-12))
+    assert st.status == S.OK, st.as_dict()
+    assert rel_l2(p.cpu().numpy(), xo) < PRESSURE_TOL
+
+
+def test_warm_start_and_repeat(ctx, torch):
+    """x is in/out (reservoir.c:52 passes the previous p): a converged start
+    stops at once; repeated calls reuse the context's buffers."""
+    P = make_problem((24, 24, 24), "lognormal", seed=3)
+    D = Dev(P, torch)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    o = S.default_options(rtol=1e-10)
+    st1 = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, o)
+    p1 = p.clone()
+    st2 = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, o)
+    assert st1.iterations > 2 and st2.iterations <= 1
+    assert rel_l2(p.cpu().numpy(), p1.cpu().numpy()) < 1e-9
+
+
+def test_compute_rejects_unstructured():
+    """A symmetric matrix that is not a Cartesian 7-point stencil is refused
+    with an error code, not solved wrongly."""
+    n = 5
+    cnt = np.array([2, 2, 2, 2, 2], np.int32)
+    col = np.array([0, 3, 1, 4, 2, 0, 3, 0, 4, 1], np.int32)   # a 0-3 coupling
+    ele = np.array([2.0, -1.0, 2.0, -1.0, 2.0, -1.0, 2.0, -1.0, 2.0, -1.0])
+    hdr = np.array([n, col.size], np.int32)
+    with pytest.raises(RuntimeError):
+        S.compute(hdr, cnt, col, ele, np.ones(n), np.zeros(n))
+
+
+def test_full_size_c2_properties(ctx, torch):
+    """128^3 lognormal sigma=2 (the benchmark workload) at rtol 1e-8:
+    true residual, maximum principle, global mass balance."""
+    P = make_problem((128, 128, 128), "lognormal", seed=20261202, sigma=2.0, corr=8.0)
+    _say("C2 128^3")
+    D = Dev(P, torch)
+    n = P.n
+    p = torch.zeros(n, dtype=torch.float64, device="cuda")
+    v = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, v, S.default_options(rtol=1e-8))
+    _say("C2 status", st.as_dict())
+    assert st.status == S.OK
+    # independent FP64 residual from the device-assembled AoS stencil
+    perm = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
+    indcol = torch.zeros(n * 7, dtype=torch.int32, device="cuda")
+    b = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ctx.assemble(D.g, D.K, D.K, D.K, D.bval, b, perm, indcol)
+    x = p.cpu().numpy()
+    r = true_residual(perm.cpu().numpy().view(O.ELE).reshape(n), indcol.cpu().numpy().view(O.IND).reshape(n),
+                      b.cpu().numpy(), x)
+    assert r < 2e-8
+    assert x.min() > -1e-6 and x.max() < 1 + 1e-6
+    vv = v.cpu().numpy().view(S.ELE).reshape(128, 128, 128)
+    inflow = -vv["l"][:, :, 0].sum()
+    outflow = vv["r"][:, :, -1].sum()
+    assert abs(inflow - outflow) <= 1e-6 * abs(inflow)
