@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--profile-only", action="store_true", help="timed steps only (for rocprofv3 runs)")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every box here has one); --gpus %d is not supported"
@@ -117,6 +119,15 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nb, "launches": 5}))
+        return
     w = WORKLOADS[args.workload]
     nx, ny, nz = w["dims"]
     n = nx * ny * nz
@@ -135,8 +146,11 @@ def main():
     prec = S.PREC_FP64 if args.precision == "fp64" else S.PREC_FP32
     opt = S.default_options(rtol=rtol, precision=prec)
 
+    solver_stream = torch.cuda.ExternalStream(ctx.stream)
+
     def step(o=opt):
-        p.zero_()   # each step solves from the reference's zero initial guess
+        with torch.cuda.stream(solver_stream):
+            p.zero_()   # each step solves from the reference's zero initial guess
         return ctx.reservoir(g, Kd, Kd, Kd, bval, p, v, o)
 
     for _ in range(args.warmup):

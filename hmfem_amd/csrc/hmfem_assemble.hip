@@ -259,12 +259,11 @@ __device__ __forceinline__ double flux_cell(const GridArgs &g, const Faces6 &f, 
     return vr / g.hx + vl / g.hx + vd / g.hy + vu / g.hy + vb / g.hz + vt / g.hz;
 }
 
-// Max/min |div| over the grid: block partials, then the last block to finish
-// combines them in a fixed order and resets the counter.
-__device__ void minmax_finish(double mx, double mn, double *partial, unsigned int *counter, double *out)
+// Max/min |div| of a block, written as one partial pair per block; the
+// k_minmax_final kernel combines the partials (no inter-block fences).
+__device__ void minmax_block(double mx, double mn, double *partial)
 {
     __shared__ double smx[kBlock / 64], smn[kBlock / 64];
-    __shared__ int last;
     for (int o = 32; o > 0; o >>= 1) {
         mx = fmax(mx, __shfl_xor(mx, o));
         mn = fmin(mn, __shfl_xor(mn, o));
@@ -276,35 +275,36 @@ __device__ void minmax_finish(double mx, double mn, double *partial, unsigned in
         for (int q = 1; q < kBlock / 64; q++) { mx = fmax(mx, smx[q]); mn = fmin(mn, smn[q]); }
         partial[2 * blockIdx.x] = mx;
         partial[2 * blockIdx.x + 1] = mn;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        unsigned int prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = prev == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    mx = 0.0; mn = 100.0;
-    for (int q = threadIdx.x; q < (int)gridDim.x; q += kBlock) {
+}
+
+// out[0] = min |div|, out[1] = max |div| (divvel_pontual's minn / maxx).
+__global__ void __launch_bounds__(kBlock)
+k_minmax_final(const double *partial, int G, double *out)
+{
+    double mx = 0.0, mn = 100.0;
+    for (int q = threadIdx.x; q < G; q += kBlock) {
         mx = fmax(mx, partial[2 * q]);
         mn = fmin(mn, partial[2 * q + 1]);
     }
+    __shared__ double smx[kBlock / 64], smn[kBlock / 64];
     for (int o = 32; o > 0; o >>= 1) {
         mx = fmax(mx, __shfl_xor(mx, o));
         mn = fmin(mn, __shfl_xor(mn, o));
     }
+    int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     if (lane == 0) { smx[w] = mx; smn[w] = mn; }
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int q = 1; q < kBlock / 64; q++) { mx = fmax(mx, smx[q]); mn = fmin(mn, smn[q]); }
         out[0] = fabs(mn);
         out[1] = fabs(mx);
-        *counter = 0;
     }
 }
 
 __global__ void __launch_bounds__(kBlock)
 k_flux_from_K(GridArgs g, const double *Kx, const double *Ky, const double *Kz, FaceVals B, const double *p,
-              ELE *v, double *partial, unsigned int *counter, double *out)
+              ELE *v, double *partial)
 {
     long n = (long)g.nx * g.ny * g.nz;
     long s = (long)xcd_block() * kBlock + threadIdx.x;
@@ -316,12 +316,11 @@ k_flux_from_K(GridArgs g, const double *Kx, const double *Ky, const double *Kz, 
         double dv = fabs(flux_cell(g, f, B, p, s, i, j, k, v));
         mx = dv; mn = dv;
     }
-    minmax_finish(mx, mn, partial, counter, out);
+    minmax_block(mx, mn, partial);
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_flux_from_perm(GridArgs g, const ELE *perm, FaceVals B, const double *p, ELE *v, double *partial,
-                 unsigned int *counter, double *out)
+k_flux_from_perm(GridArgs g, const ELE *perm, FaceVals B, const double *p, ELE *v, double *partial)
 {
     long n = (long)g.nx * g.ny * g.nz;
     long s = (long)xcd_block() * kBlock + threadIdx.x;
@@ -333,12 +332,12 @@ k_flux_from_perm(GridArgs g, const ELE *perm, FaceVals B, const double *p, ELE *
         double dv = fabs(flux_cell(g, f, B, p, s, i, j, k, v));
         mx = dv; mn = dv;
     }
-    minmax_finish(mx, mn, partial, counter, out);
+    minmax_block(mx, mn, partial);
 }
 
 // divvel_pontual() of an existing AoS v.
 __global__ void __launch_bounds__(kBlock)
-k_divergence(GridArgs g, const ELE *v, double *partial, unsigned int *counter, double *out)
+k_divergence(GridArgs g, const ELE *v, double *partial)
 {
 #pragma clang fp contract(off)
     long n = (long)g.nx * g.ny * g.nz;
@@ -349,7 +348,7 @@ k_divergence(GridArgs g, const ELE *v, double *partial, unsigned int *counter, d
         double dv = fabs(q.r / g.hx + q.l / g.hx + q.d / g.hy + q.u / g.hy + q.b / g.hz + q.t / g.hz);
         mx = dv; mn = dv;
     }
-    minmax_finish(mx, mn, partial, counter, out);
+    minmax_block(mx, mn, partial);
 }
 
 // prod_mat_vec(): res += A x with A from AoS perm/indcol, in the reference's
@@ -437,11 +436,12 @@ int launch_flux(hipStream_t st, const hmfem_grid *h, const double *Kx, const dou
     long n = (long)g.nx * g.ny * g.nz;
     int G = grid_for(n, kBlock);
     if (perm)
-        hipLaunchKernelGGL(k_flux_from_perm, dim3(G), dim3(kBlock), 0, st, g, perm, make_faces(bval), p, v, partial,
-                           counter, out);
+        hipLaunchKernelGGL(k_flux_from_perm, dim3(G), dim3(kBlock), 0, st, g, perm, make_faces(bval), p, v, partial);
     else
         hipLaunchKernelGGL(k_flux_from_K, dim3(G), dim3(kBlock), 0, st, g, Kx, Ky, Kz, make_faces(bval), p, v,
-                           partial, counter, out);
+                           partial);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(kBlock), 0, st, partial, G, out);
+    (void)counter;
     HMFEM_CHECK(hipGetLastError());
     return HMFEM_OK;
 }
@@ -451,7 +451,10 @@ int launch_divergence(hipStream_t st, const hmfem_grid *h, const ELE *v, double 
 {
     GridArgs g = make_grid(h);
     long n = (long)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_divergence, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, g, v, partial, counter, out);
+    int G = grid_for(n, kBlock);
+    hipLaunchKernelGGL(k_divergence, dim3(G), dim3(kBlock), 0, st, g, v, partial);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(kBlock), 0, st, partial, G, out);
+    (void)counter;
     HMFEM_CHECK(hipGetLastError());
     return HMFEM_OK;
 }

@@ -27,6 +27,8 @@ struct hmfem_context {
     std::vector<hipEvent_t> prof_ev;
     std::vector<int> prof_type;
     bool prof_on = false;
+    std::vector<char> stage;   // host staging of small uploads (tail op list)
+    GraphCache graph;          // captured CG iteration of the last solve
 
     // device buffer `name` with at least `bytes`, grown on demand
     void *get(const char *name, size_t bytes)
@@ -68,9 +70,12 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->stop_rule = HMFEM_STOP_RESIDUAL;
     o->nu = 2;
     o->omega = 0.8;
-    o->ocf = 1.7;
+    o->ocf = 1.9;
     o->coarse_max = kCoarseMax;
     o->check_every = 4;
+    o->tail_max = 4096;
+    o->tail_gamma = 2;
+    o->use_graph = 1;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)
@@ -93,6 +98,7 @@ extern "C" void hmfem_destroy(hmfem_context *c)
     for (auto &kv : c->bufs)
         if (kv.second.first) (void)hipFree(kv.second.first);
     for (auto e : c->prof_ev) (void)hipEventDestroy(e);
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
     for (int q = 0; q < 8; q++) (void)hipEventDestroy(c->ev[q]);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     (void)hipStreamDestroy(c->st);
@@ -158,8 +164,7 @@ static int solve_on_stencil(hmfem_context *c, Dims d, const Stencil<double> &A, 
     else hmfem_default_options(&opt);
     if (opt.coarse_max > kCoarseMax || opt.coarse_max < 1) opt.coarse_max = kCoarseMax;
     long n = d.n();
-    int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
-    size_t hb = hierarchy_bytes(d, opt.coarse_max, prec);
+    size_t hb = hierarchy_bytes(d, opt);
     char *arena = (char *)c->get("arena", hb);
     CgBuffers B;
     B.r = (double *)c->get("r", n * 8);
@@ -178,7 +183,8 @@ static int solve_on_stencil(hmfem_context *c, Dims d, const Stencil<double> &A, 
     }
     hmfem_status tmp{};
     hmfem_status *out = st ? st : &tmp;
-    return solve_structured(c->st, d, A, b, x, opt, arena, hb, B, c->h_sc, info, out, c->ev, sink_of(c));
+    return solve_structured(c->st, d, A, b, x, opt, arena, hb, B, c->h_sc, info, out, c->ev, sink_of(c), c->stage,
+                            &c->graph);
 }
 
 // ------------------------------------------------------------ device API

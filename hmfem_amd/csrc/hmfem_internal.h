@@ -14,6 +14,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <vector>
+
 #include "../../include/hmfem_gpu.h"
 
 #define HMFEM_CHECK(call)                                                                      \
@@ -33,6 +35,7 @@ constexpr int kCellsPerThread = 4; // cells per thread in reduction kernels
 constexpr int kMaxLevels = 24;
 constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, blocks dispatched round-robin
 constexpr int kCoarseMax = 80;     // largest coarsest level inverted densely in LDS (FP64, <= 64 KB)
+constexpr int kMaxReduceBlocks = 1024; // blocks of a chunked reduction launch (one partial each)
 
 struct Dims {
     int nx, ny, nz;
@@ -43,6 +46,7 @@ struct Dims {
 template <typename T>
 struct Stencil {
     T *e = nullptr, *wx = nullptr, *wy = nullptr, *wz = nullptr;
+    T *di = nullptr;   // inverse diagonal (smoothers); computed at setup
 };
 
 struct Level {
@@ -53,14 +57,26 @@ struct Level {
     Stencil<__half> s16;             // FP16 scaled fine-level values (ablation)
     float *sc = nullptr;             // FP16 row scaling (sqrt of diagonal), fine level only
     void *f = nullptr, *u = nullptr, *t = nullptr; // V-cycle vectors (float or double)
+    void *r = nullptr, *w = nullptr;               // second right-hand side and saved u (W-cycle)
 };
 
 // Device scalars of one CG solve, kept in one small allocation.
+// sigma and alpha keep two slots (by parity of the CG iteration) so that a
+// kernel never overwrites a value its own blocks still read.
+// The iteration index lives on the device too (kspmv: next SpMV; kupd: the
+// SpMV's index handed to the update), so one captured graph serves every
+// iteration.
 struct CgScalars {
-    double bb, rr, sigma, sigma_first, sigma_last, tau, alpha, beta, gamma;
+    double bb, rr, sigma_first, tau, beta;
+    double sigma[2], alpha[2];
     double rtol2;
-    int done, iter, maxit, stop_rule, status, pad;
-    unsigned int counter[8];         // last-block-done counters of the reductions
+    int done, iter, maxit, stop_rule, status, kspmv, kupd, pad;
+};
+
+// A captured CG iteration, reused while its launch parameters are unchanged.
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    std::vector<char> key;
 };
 
 // Grid launch helpers with an XCD-aware block order: logical block b of a
@@ -129,10 +145,11 @@ struct CgBuffers {
     CgScalars *sc;
     double *partial;
 };
-size_t hierarchy_bytes(Dims fine, int coarse_max, int prec);
+size_t hierarchy_bytes(Dims fine, const hmfem_options &opt);
 size_t cg_partial_count(long n);
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
-                     int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink);
+                     int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink,
+                     std::vector<char> &stage, GraphCache *graph);
 
 } // namespace hmfem

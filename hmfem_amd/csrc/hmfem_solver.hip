@@ -15,10 +15,24 @@
 // implicit-index stencil layout. The coarsest level (<= 80 cells) is
 // inverted densely in LDS in FP64.
 //
+// Cycle. Large levels run one multi-block kernel per step (pre-smoothing,
+// residual + restriction, prolongation + smoothing). Levels of at most
+// `tail_max` cells run inside a single one-workgroup "tail" kernel that
+// executes a host-built list of steps (V- or W-cycle) with a barrier between
+// steps, so a whole coarse sub-cycle costs one launch.
+//
+// Reductions. CG dot products are deterministic and need no inter-block
+// fences: each reduction kernel writes one partial per block, and the next
+// kernel that needs the value sums the partials in a fixed order itself.
+//
 // Precision: coefficients are accumulated in FP64; the V-cycle runs on an
 // FP32 copy (or FP64 for the plain-FP64 solve); CG vectors, dots, alpha and
 // beta are FP64 and stay on the device.
 #include "hmfem_internal.h"
+
+#include <string.h>
+
+#include <vector>
 
 namespace hmfem {
 
@@ -26,18 +40,9 @@ namespace hmfem {
 
 template <typename T>
 struct OpArgs {
-    const T *e, *wx, *wy, *wz;
+    const T *e, *wx, *wy, *wz, *di;
     int nx, ny, nz;
 };
-
-template <typename T>
-__device__ __forceinline__ OpArgs<T> op_of(const Stencil<T> &s, const Dims &d)
-{
-    OpArgs<T> o;
-    o.e = s.e; o.wx = s.wx; o.wy = s.wy; o.wz = s.wz;
-    o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
-    return o;
-}
 
 struct Cell {
     int s;
@@ -64,6 +69,9 @@ __device__ __forceinline__ Cell shifted(const Cell &c, int di, int dj, int dk, i
     q.s = c.s + di + dj * nx + dk * nxy;
     return q;
 }
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 
 // Diagonal of A at a cell: e + all six couplings.
 template <typename T, typename C>
@@ -98,53 +106,97 @@ __device__ __forceinline__ C apply_at(const OpArgs<T> &o, const Cell &c, U u, C 
     return a;
 }
 
-// ---------------------------------------------------------------- V-cycle
+// ---------------------------------------------------------------- reductions
+
+// Sum of NV values over the block in a fixed order; every thread gets the sums.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV])
+{
+    __shared__ double sh[kBlock / 64][NV];
+    for (int q = 0; q < NV; q++)
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0)
+        for (int q = 0; q < NV; q++) sh[w][q] = v[q];
+    __syncthreads();
+    for (int q = 0; q < NV; q++) {
+        double t = sh[0][q];
+        for (int x = 1; x < kBlock / 64; x++) t += sh[x][q];
+        v[q] = t;
+    }
+    __syncthreads();
+}
+
+// Sum of the per-block partials of a previous kernel (NV interleaved values
+// per block, G blocks), in a fixed order; every thread gets the sums.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *P, int G, double (&out)[NV])
+{
+    for (int q = 0; q < NV; q++) out[q] = 0.0;
+    for (int b = threadIdx.x; b < G; b += kBlock)
+        for (int q = 0; q < NV; q++) out[q] += P[(long)NV * b + q];
+    block_sum<NV>(out);
+}
+
+// Contiguous range of cells of this (logical) block in a chunked launch.
+__device__ __forceinline__ void chunk_range(int n, int chunk, int &lb, int &beg, int &end)
+{
+    lb = xcd_block();
+    beg = lb * chunk;
+    end = min(n, beg + chunk);
+}
+
+// ---------------------------------------------------------------- V-cycle, large levels
 
 // Pre-smoothing from a zero guess: one sweep gives u1 = w f/d; a second sweep
 // u2 = u1 + w (f - A u1)/d evaluates u1 at the neighbours on the fly.
 template <typename T, typename FT>
 __global__ void __launch_bounds__(kBlock)
-k_smooth_zero(OpArgs<T> o, const FT *__restrict__ f, T *__restrict__ u, T omega, int sweeps)
+k_smooth_zero(OpArgs<T> o, const FT *__restrict__ f, T *__restrict__ u, T omega, int sweeps, const int *done)
 {
     using C = T;
+    if (*done) return;
     int n = o.nx * o.ny * o.nz;
     int s = xcd_block() * kBlock + threadIdx.x;
     if (s >= n) return;
     Cell c = cell_at(o, s);
+    const C ds = (C)o.di[s];
     if (sweeps == 1) {
-        C d = diag_at<T, C>(o, c);
-        u[s] = omega * (C)f[s] / d;
+        u[s] = omega * (C)f[s] * ds;
         return;
     }
-    auto u1 = [&](const Cell &q) -> C { return omega * (C)f[q.s] / diag_at<T, C>(o, q); };
+    auto u1 = [&](const Cell &q) -> C { return omega * (C)f[q.s] * (C)o.di[q.s]; };
     C d;
     C au = apply_at<T, C>(o, c, u1, d);
-    C u1s = omega * (C)f[s] / d;
-    u[s] = u1s + omega * ((C)f[s] - au) / d;
+    C u1s = omega * (C)f[s] * ds;
+    u[s] = u1s + omega * ((C)f[s] - au) * ds;
 }
 
 // One damped Jacobi sweep u_out = u + w (f - A u)/d.
 template <typename T, typename FT>
 __global__ void __launch_bounds__(kBlock)
-k_jacobi(OpArgs<T> o, const FT *__restrict__ f, const T *__restrict__ u, T *__restrict__ uo, T omega)
+k_jacobi(OpArgs<T> o, const FT *__restrict__ f, const T *__restrict__ u, T *__restrict__ uo, T omega,
+         const int *done)
 {
     using C = T;
+    if (*done) return;
     int n = o.nx * o.ny * o.nz;
     int s = xcd_block() * kBlock + threadIdx.x;
     if (s >= n) return;
     Cell c = cell_at(o, s);
     C d;
     C au = apply_at<T, C>(o, c, [&](const Cell &q) -> C { return (C)u[q.s]; }, d);
-    uo[s] = (C)u[s] + omega * ((C)f[s] - au) / d;
+    uo[s] = (C)u[s] + omega * ((C)f[s] - au) * rcp(d);
 }
 
 // Residual r = f - A u summed over each aggregate: one thread per coarse cell.
 template <typename T, typename FT>
 __global__ void __launch_bounds__(kBlock)
 k_resid_restrict(OpArgs<T> o, int fx, int fy, int fz, int cx, int cy, int cz, const FT *__restrict__ f,
-                 const T *__restrict__ u, T *__restrict__ fc)
+                 const T *__restrict__ u, T *__restrict__ fc, const int *done)
 {
     using C = T;
+    if (*done) return;
     int nc = cx * cy * cz;
     int I = xcd_block() * kBlock + threadIdx.x;
     if (I >= nc) return;
@@ -178,9 +230,10 @@ k_resid_restrict(OpArgs<T> o, int fx, int fy, int fz, int cx, int cy, int cz, co
 template <typename T, typename FT>
 __global__ void __launch_bounds__(kBlock)
 k_prolong_jacobi(OpArgs<T> o, int fx, int fy, int fz, int cx, int cy, const T *__restrict__ uc, T ocf,
-                 const FT *__restrict__ f, const T *__restrict__ u, T *__restrict__ uo, T omega)
+                 const FT *__restrict__ f, const T *__restrict__ u, T *__restrict__ uo, T omega, const int *done)
 {
     using C = T;
+    if (*done) return;
     int n = o.nx * o.ny * o.nz;
     int s = xcd_block() * kBlock + threadIdx.x;
     if (s >= n) return;
@@ -191,15 +244,52 @@ k_prolong_jacobi(OpArgs<T> o, int fx, int fy, int fz, int cx, int cy, const T *_
     };
     C d;
     C au = apply_at<T, C>(o, c, up, d);
-    uo[s] = up(c) + omega * ((C)f[s] - au) / d;
+    uo[s] = up(c) + omega * ((C)f[s] - au) * rcp(d);
+}
+
+// Last smoothing sweep of the fine level fused with the CG dot products:
+// z = u_out, partials of <z, r> and <z, v> (r is the fine right-hand side f).
+// With uc != nullptr the sweep also applies the coarse correction first.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_final_sweep_dots(OpArgs<T> o, int fx, int fy, int cx, int cy, int fz, const T *__restrict__ uc, T ocf,
+                   const double *__restrict__ f, const T *__restrict__ u, T *__restrict__ uo, T omega,
+                   const double *__restrict__ v, int chunk, double *Pdot, const int *done)
+{
+    using C = T;
+    if (*done) return;
+    int n = o.nx * o.ny * o.nz;
+    int lb, beg, end;
+    chunk_range(n, chunk, lb, beg, end);
+    double acc[2] = {0.0, 0.0};
+    for (int base = beg; base < end; base += kBlock) {
+        int s = base + threadIdx.x;
+        if (s >= end) break;
+        Cell c = cell_at(o, s);
+        auto up = [&](const Cell &q) -> C {
+            C x = (C)u[q.s];
+            if (uc) x += ocf * (C)uc[q.i / fx + cx * (q.j / fy + cy * (q.k / fz))];
+            return x;
+        };
+        C d;
+        C au = apply_at<T, C>(o, c, up, d);
+        double fs = f[s];
+        C z = up(c) + omega * ((C)fs - au) * rcp(d);
+        uo[s] = z;
+        acc[0] += (double)z * fs;
+        if (v) acc[1] += (double)z * v[s];
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) { Pdot[2 * lb] = acc[0]; Pdot[2 * lb + 1] = acc[1]; }
 }
 
 // Dense coarsest solve u = A^-1 f, one block.
 template <typename T, typename FT>
 __global__ void __launch_bounds__(kBlock)
-k_dense_apply(int n, const double *__restrict__ inv, const FT *__restrict__ f, T *__restrict__ u)
+k_dense_apply(int n, const double *__restrict__ inv, const FT *__restrict__ f, T *__restrict__ u, const int *done)
 {
     __shared__ double fs[kCoarseMax];
+    if (*done) return;
     for (int q = threadIdx.x; q < n; q += kBlock) fs[q] = (double)f[q];
     __syncthreads();
     for (int r = threadIdx.x; r < n; r += kBlock) {
@@ -207,6 +297,171 @@ k_dense_apply(int n, const double *__restrict__ inv, const FT *__restrict__ f, T
         const double *row = inv + (long)r * n;
         for (int q = 0; q < n; q++) acc += row[q] * fs[q];
         u[r] = (T)acc;
+    }
+}
+
+// Dense solve with the CG dots, for a single-level hierarchy: block 0 does the
+// work, the other blocks of the chunked launch write zero partials.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_dense_dots(int n, const double *__restrict__ inv, const double *__restrict__ f, const double *__restrict__ v,
+             T *__restrict__ z, double *Pdot, const int *done)
+{
+    __shared__ double fs[kCoarseMax];
+    if (*done) return;
+    double acc[2] = {0.0, 0.0};
+    const int lb = xcd_block();
+    if (lb == 0) {
+        for (int q = threadIdx.x; q < n; q += kBlock) fs[q] = f[q];
+    }
+    __syncthreads();
+    if (lb == 0) {
+        for (int r = threadIdx.x; r < n; r += kBlock) {
+            double a = 0.0;
+            const double *row = inv + (long)r * n;
+            for (int q = 0; q < n; q++) a += row[q] * fs[q];
+            T zr = (T)a;
+            z[r] = zr;
+            acc[0] += (double)zr * fs[r];
+            if (v) acc[1] += (double)zr * v[r];
+        }
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) { Pdot[2 * lb] = acc[0]; Pdot[2 * lb + 1] = acc[1]; }
+}
+
+// ---------------------------------------------------------------- tail kernel
+
+enum TailOp { OP_SMOOTH0 = 0, OP_JACOBI, OP_RESTRICT, OP_PROLONG_JACOBI, OP_DENSE, OP_RESID, OP_SAVE, OP_ADDSAVED,
+              OP_FINAL };
+
+template <typename T>
+struct TailLevel {
+    OpArgs<T> o;
+    T *f, *u, *t, *r, *w;
+    int n, fx, fy, fz;
+};
+
+constexpr int kTailThreads = 1024;
+
+// Executes ops[0..nops) in order, each over all cells of its level with the
+// 1024 threads of one workgroup, and a barrier after each op. Every thread
+// runs the same op list, so the per-level "which buffer holds u" parity is
+// kept in a register and agrees across threads.
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads)
+k_tail(const TailLevel<T> *__restrict__ Lv, const int4 *__restrict__ ops, int nops, const double *__restrict__ inv,
+       int nc, T omega, T ocf, const int *done)
+{
+    using C = T;
+    __shared__ double fs[kCoarseMax];
+    if (*done) return;
+    unsigned parity = 0;
+    const int tid = threadIdx.x;
+    for (int q = 0; q < nops; q++) {
+        const int4 op = ops[q];
+        const int l = op.y;
+        const TailLevel<T> L = Lv[l];
+        const bool pl = (parity >> l) & 1u;
+        T *cur = pl ? L.t : L.u;
+        T *oth = pl ? L.u : L.t;
+        const T *f = op.z ? L.r : L.f;
+        const OpArgs<T> &o = L.o;
+        switch (op.x) {
+        case OP_SMOOTH0:
+            for (int s = tid; s < L.n; s += kTailThreads) cur[s] = omega * (C)f[s] * (C)o.di[s];
+            break;
+        case OP_JACOBI:
+            for (int s = tid; s < L.n; s += kTailThreads) {
+                Cell c = cell_at(o, s);
+                C d;
+                C au = apply_at<T, C>(o, c, [&](const Cell &x) -> C { return (C)cur[x.s]; }, d);
+                oth[s] = (C)cur[s] + omega * ((C)f[s] - au) * rcp(d);
+            }
+            parity ^= 1u << l;
+            break;
+        case OP_RESID:
+            for (int s = tid; s < L.n; s += kTailThreads) {
+                Cell c = cell_at(o, s);
+                C d;
+                C au = apply_at<T, C>(o, c, [&](const Cell &x) -> C { return (C)cur[x.s]; }, d);
+                L.r[s] = (C)L.f[s] - au;
+            }
+            break;
+        case OP_SAVE:
+            for (int s = tid; s < L.n; s += kTailThreads) L.w[s] = cur[s];
+            break;
+        case OP_ADDSAVED:
+            for (int s = tid; s < L.n; s += kTailThreads) cur[s] += L.w[s];
+            break;
+        case OP_FINAL:
+            if (pl) {
+                for (int s = tid; s < L.n; s += kTailThreads) L.u[s] = L.t[s];
+                parity &= ~(1u << l);
+            }
+            break;
+        case OP_RESTRICT: {
+            const TailLevel<T> Cn = Lv[l + 1];
+            T *dst = op.w ? Cn.r : Cn.f;
+            const int cx = Cn.o.nx, cy = Cn.o.ny;
+            for (int I = tid; I < Cn.n; I += kTailThreads) {
+                int ck = I / (cx * cy);
+                int rem = I - ck * cx * cy;
+                int cj = rem / cx, ci = rem - cj * cx;
+                C acc = 0;
+                for (int dk = 0; dk < L.fz; dk++) {
+                    int k = ck * L.fz + dk;
+                    if (k >= o.nz) break;
+                    for (int dj = 0; dj < L.fy; dj++) {
+                        int j = cj * L.fy + dj;
+                        if (j >= o.ny) break;
+                        for (int di = 0; di < L.fx; di++) {
+                            int i = ci * L.fx + di;
+                            if (i >= o.nx) break;
+                            Cell c;
+                            c.i = i; c.j = j; c.k = k;
+                            c.s = i + j * o.nx + k * o.nx * o.ny;
+                            C d;
+                            C au = apply_at<T, C>(o, c, [&](const Cell &x) -> C { return (C)cur[x.s]; }, d);
+                            acc += (C)f[c.s] - au;
+                        }
+                    }
+                }
+                dst[I] = acc;
+            }
+            break;
+        }
+        case OP_PROLONG_JACOBI: {
+            const TailLevel<T> Cn = Lv[l + 1];
+            const T *uc = ((parity >> (l + 1)) & 1u) ? Cn.t : Cn.u;
+            const int cx = Cn.o.nx, cy = Cn.o.ny;
+            const int fx = L.fx, fy = L.fy, fz = L.fz;
+            for (int s = tid; s < L.n; s += kTailThreads) {
+                Cell c = cell_at(o, s);
+                auto up = [&](const Cell &x) -> C {
+                    return (C)cur[x.s] + ocf * (C)uc[x.i / fx + cx * (x.j / fy + cy * (x.k / fz))];
+                };
+                C d;
+                C au = apply_at<T, C>(o, c, up, d);
+                oth[s] = up(c) + omega * ((C)f[s] - au) * rcp(d);
+            }
+            parity ^= 1u << l;
+            break;
+        }
+        case OP_DENSE:
+            for (int s = tid; s < nc; s += kTailThreads) fs[s] = (double)f[s];
+            __syncthreads();
+            for (int r = tid; r < nc; r += kTailThreads) {
+                double acc = 0.0;
+                const double *row = inv + (long)r * nc;
+                for (int j = 0; j < nc; j++) acc += row[j] * fs[j];
+                cur[r] = (T)acc;
+            }
+            break;
+        default:
+            break;
+        }
+        __syncthreads();
     }
 }
 
@@ -247,6 +502,18 @@ k_to_f32(long n, const double *e, const double *wx, const double *wy, const doub
     long s = (long)xcd_block() * kBlock + threadIdx.x;
     if (s >= n) return;
     e32[s] = (float)e[s]; wx32[s] = (float)wx[s]; wy32[s] = (float)wy[s]; wz32[s] = (float)wz[s];
+}
+
+// Inverse diagonal of a level, in its storage type.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_dinv(OpArgs<T> o, T *di)
+{
+    int n = o.nx * o.ny * o.nz;
+    int s = xcd_block() * kBlock + threadIdx.x;
+    if (s >= n) return;
+    Cell c = cell_at(o, s);
+    di[s] = (T)(1.0 / (double)diag_at<T, double>(o, c));
 }
 
 // Dense inverse of the coarsest operator by Gauss-Jordan elimination in LDS
@@ -293,65 +560,27 @@ k_dense_invert(OpArgs<double> o, double *inv, int *info)
 }
 
 // ---------------------------------------------------------------- CG kernels
+//
+// Iteration k:  SPMV_k (s_k = z_k + beta s_{k-1}, v = A s_k, partial tau),
+//               UPDATE_k (alpha_k = sigma_k / tau, x += alpha s, r -= alpha v,
+//               partial ||r||^2), V-cycle on r whose last sweep writes z_{k+1}
+//               and partials of <z, r>, <z, v>.
+// SPMV_k sums the partials of UPDATE_{k-1} and of the dots: it decides the
+// stop and computes beta. alpha and sigma live in two slots by parity of k so
+// that no kernel overwrites a value that its own blocks still read.
 
-// Block sum of up to three FP64 values, then the last block to finish sums
-// the per-block partials in a fixed order (deterministic) and calls fin().
-template <int NV, typename F>
-__device__ __forceinline__ void reduce_finish(double (&v)[NV], double *partial, unsigned int *counter, F fin)
-{
-    __shared__ double sh[kBlock / 64][NV];
-    __shared__ int last;
-    for (int q = 0; q < NV; q++)
-        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
-    int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-    if (lane == 0)
-        for (int q = 0; q < NV; q++) sh[w][q] = v[q];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < NV; q++) {
-            double t = sh[0][q];
-            for (int x = 1; x < kBlock / 64; x++) t += sh[x][q];
-            partial[(long)NV * blockIdx.x + q] = t;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        unsigned int prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = prev == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    double t[NV];
-    for (int q = 0; q < NV; q++) t[q] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
-        for (int q = 0; q < NV; q++) t[q] += partial[(long)NV * b + q];
-    for (int q = 0; q < NV; q++)
-        for (int o = 32; o > 0; o >>= 1) t[q] += __shfl_xor(t[q], o);
-    __syncthreads();
-    if (lane == 0)
-        for (int q = 0; q < NV; q++) sh[w][q] = t[q];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < NV; q++) {
-            double a = sh[0][q];
-            for (int x = 1; x < kBlock / 64; x++) a += sh[x][q];
-            t[q] = a;
-        }
-        *counter = 0;
-        fin(t);
-    }
-}
-
-// r = b - A x, ||b||^2, ||r||^2.
+// r = b - A x; partials of ||b||^2 and ||r||^2.
 __global__ void __launch_bounds__(kBlock)
 k_cg_init(OpArgs<double> o, const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r,
-          CgScalars *sc, double *partial)
+          int chunk, double *Pinit)
 {
     int n = o.nx * o.ny * o.nz;
-    int base = xcd_block() * kBlock * kCellsPerThread + threadIdx.x;
+    int lb, beg, end;
+    chunk_range(n, chunk, lb, beg, end);
     double v[2] = {0.0, 0.0};
-    for (int q = 0; q < kCellsPerThread; q++) {
-        int s = base + q * kBlock;
-        if (s >= n) break;
+    for (int base = beg; base < end; base += kBlock) {
+        int s = base + threadIdx.x;
+        if (s >= end) break;
         Cell c = cell_at(o, s);
         double d;
         double ax = apply_at<double, double>(o, c, [&](const Cell &t) { return x[t.s]; }, d);
@@ -361,54 +590,64 @@ k_cg_init(OpArgs<double> o, const double *__restrict__ b, const double *__restri
         v[0] += bs * bs;
         v[1] += rs * rs;
     }
-    reduce_finish<2>(v, partial, &sc->counter[0], [&](double *t) {
-        sc->bb = t[0];
-        sc->rr = t[1];
-        sc->iter = 0;
-        sc->status = 0;
-        sc->done = (t[0] == 0.0 || t[1] <= sc->rtol2 * t[0]) ? 1 : 0;
-        if (t[0] == 0.0) sc->rr = 0.0;
-    });
+    block_sum<2>(v);
+    if (threadIdx.x == 0) { Pinit[2 * lb] = v[0]; Pinit[2 * lb + 1] = v[1]; }
 }
 
-// sigma = <z, r> after the first preconditioning.
 template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
-k_cg_first_dot(long n, const ZT *__restrict__ z, const double *__restrict__ r, CgScalars *sc, double *partial)
+k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
+          double *__restrict__ v, int chunk, int G, CgScalars *sc, const double *Pinit, const double *Prr,
+          const double *Pdot, double *Ptau)
 {
     if (sc->done) return;
-    int base = xcd_block() * kBlock * kCellsPerThread + threadIdx.x;
-    double v[1] = {0.0};
-    for (int q = 0; q < kCellsPerThread; q++) {
-        int s = base + q * kBlock;
-        if (s >= n) break;
-        v[0] += (double)z[s] * r[s];
+    const int n = o.nx * o.ny * o.nz;
+    const int k = sc->kspmv;
+    const double *__restrict__ sin = (k & 1) ? s1 : s0;
+    double *__restrict__ sout = (k & 1) ? s0 : s1;
+    double dots[2];
+    sum_partials<2>(Pdot, G, dots);
+    double bb, rr;
+    if (k == 0) {
+        double ir[2];
+        sum_partials<2>(Pinit, G, ir);
+        bb = ir[0];
+        rr = ir[1];
+    } else {
+        double t[1];
+        sum_partials<1>(Prr, G, t);
+        rr = t[0];
+        bb = sc->bb;
     }
-    reduce_finish<1>(v, partial, &sc->counter[1], [&](double *t) {
-        double sg = fabs(t[0]);
-        sc->sigma = sg;
-        sc->sigma_first = sg == 0.0 ? 1.0 : sg;
-        sc->sigma_last = sg;
-        if (sg == 0.0) sc->done = 1;
-    });
-}
-
-// s_out = z + beta s_in (s_out = z on the first iteration), v = A s_out,
-// tau = <s_out, v>, alpha = sigma / tau.
-template <typename ZT>
-__global__ void __launch_bounds__(kBlock)
-k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, const double *__restrict__ sin, double *__restrict__ sout,
-          double *__restrict__ v, int first, CgScalars *sc, double *partial)
-{
-    if (sc->done) return;
-    int n = o.nx * o.ny * o.nz;
-    const double beta = first ? 0.0 : sc->beta;
-    int base = xcd_block() * kBlock * kCellsPerThread + threadIdx.x;
+    const double sigma = fabs(dots[0]);
+    int stop = 0, status = HMFEM_OK;
+    double beta = 0.0, sigma_first = k == 0 ? (sigma == 0.0 ? 1.0 : sigma) : sc->sigma_first;
+    if (k == 0) {
+        if (bb == 0.0 || rr <= sc->rtol2 * bb || sigma == 0.0) stop = 1;
+    } else {
+        beta = -sc->alpha[(k - 1) & 1] * dots[1] / sc->sigma[(k - 1) & 1];
+        if (sc->stop_rule == HMFEM_STOP_RESIDUAL && rr <= sc->rtol2 * bb) stop = 1;
+        if (sc->stop_rule == HMFEM_STOP_PRECONDITIONED && sigma <= sc->rtol2 * sigma_first) stop = 1;
+        if (!(sigma == sigma) || !(beta == beta)) { stop = 1; status = HMFEM_ERR_BREAKDOWN; }
+        else if (!stop && k >= sc->maxit) { stop = 1; status = HMFEM_NOT_CONVERGED; }
+    }
+    if (xcd_block() == 0 && threadIdx.x == 0) {
+        if (k == 0) { sc->bb = bb; sc->sigma_first = sigma_first; }
+        sc->rr = rr;
+        sc->sigma[k & 1] = sigma;
+        sc->beta = beta;
+        sc->iter = k;
+        sc->kupd = k;
+        if (stop) { sc->done = 1; sc->status = status; }
+    }
+    if (stop) return;
+    int lb, beg, end;
+    chunk_range(n, chunk, lb, beg, end);
     double acc[1] = {0.0};
-    auto sv = [&](const Cell &t) -> double { return first ? (double)z[t.s] : (double)z[t.s] + beta * sin[t.s]; };
-    for (int q = 0; q < kCellsPerThread; q++) {
-        int s = base + q * kBlock;
-        if (s >= n) break;
+    auto sv = [&](const Cell &t) -> double { return k == 0 ? (double)z[t.s] : (double)z[t.s] + beta * sin[t.s]; };
+    for (int base = beg; base < end; base += kBlock) {
+        int s = base + threadIdx.x;
+        if (s >= end) break;
         Cell c = cell_at(o, s);
         double d;
         double av = apply_at<double, double>(o, c, sv, d);
@@ -417,64 +656,40 @@ k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, const double *__restrict__
         v[s] = av;
         acc[0] += ss * av;
     }
-    reduce_finish<1>(acc, partial, &sc->counter[2], [&](double *t) {
-        double tau = fabs(t[0]);
-        sc->tau = tau;
-        if (tau == 0.0 || !(tau == tau)) { sc->done = 1; sc->status = HMFEM_ERR_BREAKDOWN; return; }
-        sc->alpha = sc->sigma / tau;
-    });
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) Ptau[lb] = acc[0];
 }
 
-// x += alpha s, r -= alpha v, rr = ||r||^2; stopping test on the true-residual rule.
 __global__ void __launch_bounds__(kBlock)
-k_cg_update(long n, const double *__restrict__ s, const double *__restrict__ v, double *__restrict__ x,
-            double *__restrict__ r, CgScalars *sc, double *partial)
+k_cg_update(int n, const double *__restrict__ s0, const double *__restrict__ s1, const double *__restrict__ v,
+            double *__restrict__ x, double *__restrict__ r, int chunk, int G, CgScalars *sc, const double *Ptau,
+            double *Prr)
 {
     if (sc->done) return;
-    const double alpha = sc->alpha;
-    int base = xcd_block() * kBlock * kCellsPerThread + threadIdx.x;
+    const int k = sc->kupd;
+    const double *__restrict__ s = (k & 1) ? s0 : s1;
+    double t[1];
+    sum_partials<1>(Ptau, G, t);
+    const double tau = fabs(t[0]);
+    if (tau == 0.0 || !(tau == tau)) {
+        if (xcd_block() == 0 && threadIdx.x == 0) { sc->done = 1; sc->status = HMFEM_ERR_BREAKDOWN; sc->tau = tau; }
+        return;
+    }
+    const double alpha = sc->sigma[k & 1] / tau;
+    if (xcd_block() == 0 && threadIdx.x == 0) { sc->alpha[k & 1] = alpha; sc->tau = tau; sc->kspmv = k + 1; }
+    int lb, beg, end;
+    chunk_range(n, chunk, lb, beg, end);
     double acc[1] = {0.0};
-    for (int q = 0; q < kCellsPerThread; q++) {
-        int i = base + q * kBlock;
-        if (i >= n) break;
+    for (int base = beg; base < end; base += kBlock) {
+        int i = base + threadIdx.x;
+        if (i >= end) break;
         x[i] += alpha * s[i];
         double ri = r[i] - alpha * v[i];
         r[i] = ri;
         acc[0] += ri * ri;
     }
-    reduce_finish<1>(acc, partial, &sc->counter[3], [&](double *t) {
-        sc->rr = t[0];
-        sc->iter += 1;
-        if (sc->stop_rule == HMFEM_STOP_RESIDUAL && t[0] <= sc->rtol2 * sc->bb) sc->done = 1;
-        else if (sc->iter >= sc->maxit) { sc->done = 1; sc->status = HMFEM_NOT_CONVERGED; }
-    });
-}
-
-// sigma_new = <z, r>, gamma = <z, v>; Polak-Ribiere beta = -alpha gamma / sigma.
-template <typename ZT>
-__global__ void __launch_bounds__(kBlock)
-k_cg_dots(long n, const ZT *__restrict__ z, const double *__restrict__ r, const double *__restrict__ v,
-          CgScalars *sc, double *partial)
-{
-    if (sc->done) return;
-    int base = xcd_block() * kBlock * kCellsPerThread + threadIdx.x;
-    double acc[2] = {0.0, 0.0};
-    for (int q = 0; q < kCellsPerThread; q++) {
-        int i = base + q * kBlock;
-        if (i >= n) break;
-        double zi = (double)z[i];
-        acc[0] += zi * r[i];
-        acc[1] += zi * v[i];
-    }
-    reduce_finish<2>(acc, partial, &sc->counter[4], [&](double *t) {
-        double sg = fabs(t[0]);
-        sc->gamma = t[1];
-        sc->beta = -sc->alpha * t[1] / sc->sigma;
-        sc->sigma_last = sc->sigma;
-        sc->sigma = sg;
-        if (sc->stop_rule == HMFEM_STOP_PRECONDITIONED && sg <= sc->rtol2 * sc->sigma_first) sc->done = 1;
-        if (!(sg == sg)) { sc->done = 1; sc->status = HMFEM_ERR_BREAKDOWN; }
-    });
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) Prr[lb] = acc[0];
 }
 
 // ---------------------------------------------------------------- host side
@@ -485,12 +700,10 @@ struct Hierarchy {
     double *inv = nullptr;   // dense inverse of the coarsest level
     int nc = 0;
     int prec = HMFEM_PREC_FP32;
-};
-
-struct Workspace {
-    // allocation sizes to reuse buffers across calls
-    size_t bytes = 0;
-    char *base = nullptr;
+    int tail0 = -1;          // first level run by the tail kernel (-1: none)
+    void *tail_levels = nullptr;
+    int4 *tail_ops = nullptr;
+    int tail_nops = 0;
 };
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -510,6 +723,9 @@ struct Arena {
     }
 };
 
+constexpr int kMaxTailOps = 4096;
+constexpr int kMaxTailLevels = 16;
+
 static void plan_levels(Dims fine, int coarse_max, Level *lev, int &nlev)
 {
     nlev = 0;
@@ -528,31 +744,108 @@ static void plan_levels(Dims fine, int coarse_max, Level *lev, int &nlev)
     }
 }
 
-size_t hierarchy_bytes(Dims fine, int coarse_max, int prec)
+static int tail_start(const Level *lev, int nlev, long tail_max)
+{
+    if (nlev < 3 || tail_max <= 0) return -1;
+    for (int l = 1; l < nlev - 1; l++)
+        if (lev[l].d.n() <= tail_max) return (nlev - l <= kMaxTailLevels) ? l : -1;
+    return -1;
+}
+
+static int clamp_coarse(int c) { return c > kCoarseMax ? kCoarseMax : (c < 1 ? 1 : c); }
+
+size_t hierarchy_bytes(Dims fine, const hmfem_options &opt)
 {
     Level lev[kMaxLevels];
     int nlev;
-    plan_levels(fine, coarse_max, lev, nlev);
+    plan_levels(fine, clamp_coarse(opt.coarse_max), lev, nlev);
     size_t b = 0;
-    size_t vs = prec == HMFEM_PREC_FP64 ? 8 : 4;
+    size_t vs = opt.precision == HMFEM_PREC_FP64 ? 8 : 4;
     for (int l = 0; l < nlev; l++) {
         size_t n = lev[l].d.n();
         if (l > 0) b += 4 * align256(n * 8);            // FP64 coefficients (fine level is the caller's)
-        b += 4 * align256(n * 4);                       // FP32 coefficients
-        b += 3 * align256(n * vs);                      // f, u, t
+        if (vs == 8) b += align256(n * 8);              // FP64 inverse diagonal
+        b += 5 * align256(n * 4);                       // FP32 coefficients and inverse diagonal
+        b += 5 * align256(n * vs);                      // f, u, t, r, w
     }
     b += align256((size_t)kCoarseMax * kCoarseMax * 8);
-    return b + 4096;
+    b += align256(sizeof(TailLevel<double>) * kMaxTailLevels) + align256(sizeof(int4) * kMaxTailOps);
+    return b + 8192;
+}
+
+// Op list of the tail: cycle(l) leaves u_l solving A_l u = rhs(fsel).
+static void gen_tail(std::vector<int4> &ops, int l, int fsel, int last, int nu, int gamma)
+{
+    if (l == last) { ops.push_back(make_int4(OP_DENSE, l, fsel, 0)); return; }
+    ops.push_back(make_int4(OP_SMOOTH0, l, fsel, 0));
+    for (int q = 1; q < nu; q++) ops.push_back(make_int4(OP_JACOBI, l, fsel, 0));
+    ops.push_back(make_int4(OP_RESTRICT, l, fsel, 0));
+    gen_tail(ops, l + 1, 0, last, nu, gamma);
+    if (gamma >= 2 && l + 1 != last) {
+        ops.push_back(make_int4(OP_RESID, l + 1, 0, 0));
+        ops.push_back(make_int4(OP_SAVE, l + 1, 0, 0));
+        gen_tail(ops, l + 1, 1, last, nu, gamma);
+        ops.push_back(make_int4(OP_ADDSAVED, l + 1, 0, 0));
+    }
+    ops.push_back(make_int4(OP_PROLONG_JACOBI, l, fsel, 0));
+    for (int q = 1; q < nu; q++) ops.push_back(make_int4(OP_JACOBI, l, fsel, 0));
+}
+
+template <typename T>
+static OpArgs<T> opargs(const Level &L);
+template <>
+OpArgs<float> opargs<float>(const Level &L)
+{
+    OpArgs<float> o;
+    o.e = L.s32.e; o.wx = L.s32.wx; o.wy = L.s32.wy; o.wz = L.s32.wz; o.di = L.s32.di;
+    o.nx = L.d.nx; o.ny = L.d.ny; o.nz = L.d.nz;
+    return o;
+}
+template <>
+OpArgs<double> opargs<double>(const Level &L)
+{
+    OpArgs<double> o;
+    o.e = L.s64.e; o.wx = L.s64.wx; o.wy = L.s64.wy; o.wz = L.s64.wz; o.di = L.s64.di;
+    o.nx = L.d.nx; o.ny = L.d.ny; o.nz = L.d.nz;
+    return o;
+}
+
+template <typename T>
+static int upload_tail(hipStream_t st, Hierarchy &H, const hmfem_options &opt, std::vector<char> &stage)
+{
+    int last = H.nlev - 1;
+    std::vector<TailLevel<T>> tl(H.nlev - H.tail0);
+    for (int l = H.tail0; l < H.nlev; l++) {
+        const Level &L = H.lev[l];
+        TailLevel<T> &t = tl[l - H.tail0];
+        t.o = opargs<T>(L);
+        t.f = (T *)L.f; t.u = (T *)L.u; t.t = (T *)L.t; t.r = (T *)L.r; t.w = (T *)L.w;
+        t.n = (int)L.d.n();
+        t.fx = L.fx; t.fy = L.fy; t.fz = L.fz;
+    }
+    std::vector<int4> ops;
+    int gamma = opt.tail_gamma < 1 ? 1 : (opt.tail_gamma > 2 ? 2 : opt.tail_gamma);
+    gen_tail(ops, 0, 0, last - H.tail0, opt.nu < 1 ? 1 : opt.nu, gamma);
+    ops.push_back(make_int4(OP_FINAL, 0, 0, 0));
+    if ((int)ops.size() > kMaxTailOps) return HMFEM_ERR_ARG;
+    H.tail_nops = (int)ops.size();
+    size_t lb = sizeof(TailLevel<T>) * tl.size(), ob = sizeof(int4) * ops.size();
+    stage.resize(lb + ob);
+    memcpy(stage.data(), tl.data(), lb);
+    memcpy(stage.data() + lb, ops.data(), ob);
+    HMFEM_CHECK(hipMemcpyAsync(H.tail_levels, stage.data(), lb, hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipMemcpyAsync(H.tail_ops, stage.data() + lb, ob, hipMemcpyHostToDevice, st));
+    return HMFEM_OK;
 }
 
 // Build the hierarchy from a fine FP64 stencil (device), inside `arena`.
-int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, int coarse_max, int prec, Arena &ar,
-                    Hierarchy &H, int *d_info)
+int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const hmfem_options &opt, Arena &ar,
+                    Hierarchy &H, int *d_info, std::vector<char> &stage)
 {
-    if (coarse_max > kCoarseMax) coarse_max = kCoarseMax;
-    if (coarse_max < 1) coarse_max = 1;
-    plan_levels(fine, coarse_max, H.lev, H.nlev);
+    plan_levels(fine, clamp_coarse(opt.coarse_max), H.lev, H.nlev);
+    int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
     H.prec = prec;
+    H.tail0 = tail_start(H.lev, H.nlev, opt.tail_max);
     size_t vs = prec == HMFEM_PREC_FP64 ? 8 : 4;
     for (int l = 0; l < H.nlev; l++) {
         Level &L = H.lev[l];
@@ -562,15 +855,19 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, int co
             L.s64.e = ar.take<double>(n); L.s64.wx = ar.take<double>(n);
             L.s64.wy = ar.take<double>(n); L.s64.wz = ar.take<double>(n);
         }
+        if (prec == HMFEM_PREC_FP64) L.s64.di = ar.take<double>(n);
         L.s32.e = ar.take<float>(n); L.s32.wx = ar.take<float>(n);
-        L.s32.wy = ar.take<float>(n); L.s32.wz = ar.take<float>(n);
+        L.s32.wy = ar.take<float>(n); L.s32.wz = ar.take<float>(n); L.s32.di = ar.take<float>(n);
         L.f = ar.take<char>(n * vs); L.u = ar.take<char>(n * vs); L.t = ar.take<char>(n * vs);
-        if (!L.s32.wz || !L.t || (l > 0 && !L.s64.wz)) return HMFEM_ERR_NOMEM;
+        L.r = ar.take<char>(n * vs); L.w = ar.take<char>(n * vs);
+        if (!L.s32.di || !L.w || (l > 0 && !L.s64.wz) || (prec == HMFEM_PREC_FP64 && !L.s64.di)) return HMFEM_ERR_NOMEM;
     }
     H.nc = (int)H.lev[H.nlev - 1].d.n();
     if (H.nc > kCoarseMax) return HMFEM_ERR_ARG;   // too many levels would be needed
     H.inv = ar.take<double>((size_t)H.nc * H.nc);
-    if (!H.inv) return HMFEM_ERR_NOMEM;
+    H.tail_levels = ar.take<char>(sizeof(TailLevel<double>) * kMaxTailLevels);
+    H.tail_ops = ar.take<int4>(kMaxTailOps);
+    if (!H.inv || !H.tail_levels || !H.tail_ops) return HMFEM_ERR_NOMEM;
 
     {
         Level &L = H.lev[0];
@@ -580,43 +877,32 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, int co
     }
     for (int l = 0; l + 1 < H.nlev; l++) {
         Level &L = H.lev[l], &C = H.lev[l + 1];
-        OpArgs<double> o;
-        o.e = L.s64.e; o.wx = L.s64.wx; o.wy = L.s64.wy; o.wz = L.s64.wz;
-        o.nx = L.d.nx; o.ny = L.d.ny; o.nz = L.d.nz;
+        OpArgs<double> o = opargs<double>(L);
         long nc = C.d.n();
         hipLaunchKernelGGL(k_rap_box, dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, st, o, L.fx, L.fy, L.fz, C.d.nx,
                            C.d.ny, C.d.nz, C.s64.e, C.s64.wx, C.s64.wy, C.s64.wz, C.s32.e, C.s32.wx, C.s32.wy,
                            C.s32.wz);
     }
+    for (int l = 0; l < H.nlev; l++) {
+        Level &L = H.lev[l];
+        int G = grid_for(L.d.n(), kBlock);
+        if (prec == HMFEM_PREC_FP64)
+            hipLaunchKernelGGL(k_dinv<double>, dim3(G), dim3(kBlock), 0, st, opargs<double>(L), L.s64.di);
+        else
+            hipLaunchKernelGGL(k_dinv<float>, dim3(G), dim3(kBlock), 0, st, opargs<float>(L), L.s32.di);
+    }
     {
         Level &L = H.lev[H.nlev - 1];
-        OpArgs<double> o;
-        o.e = L.s64.e; o.wx = L.s64.wx; o.wy = L.s64.wy; o.wz = L.s64.wz;
-        o.nx = L.d.nx; o.ny = L.d.ny; o.nz = L.d.nz;
+        OpArgs<double> o = opargs<double>(L);
         size_t lds = (size_t)H.nc * H.nc * sizeof(double);
         hipLaunchKernelGGL(k_dense_invert, dim3(1), dim3(1024), lds, st, o, H.inv, d_info);
     }
     HMFEM_CHECK(hipGetLastError());
+    if (H.tail0 > 0) {
+        int rc = prec == HMFEM_PREC_FP64 ? upload_tail<double>(st, H, opt, stage) : upload_tail<float>(st, H, opt, stage);
+        if (rc) return rc;
+    }
     return HMFEM_OK;
-}
-
-template <typename T>
-static OpArgs<T> opargs(const Level &L);
-template <>
-OpArgs<float> opargs<float>(const Level &L)
-{
-    OpArgs<float> o;
-    o.e = L.s32.e; o.wx = L.s32.wx; o.wy = L.s32.wy; o.wz = L.s32.wz;
-    o.nx = L.d.nx; o.ny = L.d.ny; o.nz = L.d.nz;
-    return o;
-}
-template <>
-OpArgs<double> opargs<double>(const Level &L)
-{
-    OpArgs<double> o;
-    o.e = L.s64.e; o.wx = L.s64.wx; o.wy = L.s64.wy; o.wz = L.s64.wz;
-    o.nx = L.d.nx; o.ny = L.d.ny; o.nz = L.d.nz;
-    return o;
 }
 
 struct CycleParams {
@@ -651,49 +937,116 @@ struct Prof {
         if (P) (P)->end();                \
     } while (0)
 
-// Recursive V-cycle on level l. f0 is the level-0 right-hand side (FP64 CG
-// residual) when l == 0; returns the buffer holding u_l.
-template <typename T, typename FT>
-static T *vcycle_level(hipStream_t st, Hierarchy &H, int l, const FT *f, const CycleParams &cp, Prof *P)
+struct Chunking {
+    int G, chunk;
+};
+
+static Chunking chunking(long n)
+{
+    Chunking c;
+    long g = (n + kBlock * kCellsPerThread - 1) / (kBlock * kCellsPerThread);
+    if (g > kMaxReduceBlocks) g = kMaxReduceBlocks;
+    if (g < kNumXcd) g = kNumXcd;
+    g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
+    long chunk = (n + g - 1) / g;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    c.G = (int)g;
+    c.chunk = (int)chunk;
+    return c;
+}
+
+// Device buffers and state of one CG solve.
+struct CgState {
+    CgBuffers B;
+    Chunking ch;
+    double *Pinit, *Ptau, *Prr, *Pdot;
+    const int *done;
+};
+
+// Recursive V-cycle on level l >= 1 (multi-block levels) down to the tail.
+// The result is left in L.u.
+template <typename T>
+static T *vcycle_coarse(hipStream_t st, Hierarchy &H, int l, const CycleParams &cp, Prof *P, const int *done)
 {
     Level &L = H.lev[l];
-    long n = L.d.n();
     T *u = (T *)L.u, *t = (T *)L.t;
-    const int pc = l == 0 ? 0 : 1;   // profile class: fine level vs coarse levels
+    const T *f = (const T *)L.f;
     if (l == H.nlev - 1) {
-        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_dense_apply<T, FT>), dim3(1), dim3(kBlock), 0, st, H.nc, H.inv, f, u);
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_dense_apply<T, T>), dim3(1), dim3(kBlock), 0, st, H.nc, H.inv, f, u,
+                    done);
+        return u;
+    }
+    if (H.tail0 > 0 && l == H.tail0) {
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, k_tail<T>, dim3(1), dim3(kTailThreads), 0, st,
+                    (const TailLevel<T> *)H.tail_levels, H.tail_ops, H.tail_nops, H.inv, H.nc, (T)cp.omega,
+                    (T)cp.ocf, done);
         return u;
     }
     OpArgs<T> o = opargs<T>(L);
     const T om = (T)cp.omega;
-    int G = grid_for(n, kBlock);
-    // pre-smoothing from zero
-    PROF_LAUNCH(P, pc ? HMFEM_PROF_COARSE : HMFEM_PROF_SMOOTH0, (k_smooth_zero<T, FT>), dim3(G), dim3(kBlock), 0, st,
-                o, f, u, om, cp.nu >= 2 ? 2 : 1);
+    int G = grid_for(L.d.n(), kBlock);
+    PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_smooth_zero<T, T>), dim3(G), dim3(kBlock), 0, st, o, f, u, om,
+                cp.nu >= 2 ? 2 : 1, done);
     for (int q = 2; q < cp.nu; q++) {
-        PROF_LAUNCH(P, pc ? HMFEM_PROF_COARSE : HMFEM_PROF_JACOBI, (k_jacobi<T, FT>), dim3(G), dim3(kBlock), 0, st, o,
-                    f, u, t, om);
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_jacobi<T, T>), dim3(G), dim3(kBlock), 0, st, o, f, u, t, om, done);
         T *tmp = u; u = t; t = tmp;
     }
-    // residual and restriction
     Level &C = H.lev[l + 1];
-    T *fc = (T *)C.f;
-    long nc = C.d.n();
-    PROF_LAUNCH(P, pc ? HMFEM_PROF_COARSE : HMFEM_PROF_RESTRICT, (k_resid_restrict<T, FT>), dim3(grid_for(nc, kBlock)),
-                dim3(kBlock), 0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, f, u, fc);
-    T *uc = vcycle_level<T, T>(st, H, l + 1, fc, cp, P);
-    // correction + post-smoothing
-    PROF_LAUNCH(P, pc ? HMFEM_PROF_COARSE : HMFEM_PROF_PROLONG, (k_prolong_jacobi<T, FT>), dim3(G), dim3(kBlock), 0,
-                st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, uc, (T)cp.ocf, f, u, t, om);
+    PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_resid_restrict<T, T>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock), 0, st,
+                o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, f, u, (T *)C.f, done);
+    T *uc = vcycle_coarse<T>(st, H, l + 1, cp, P, done);
+    PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_prolong_jacobi<T, T>), dim3(G), dim3(kBlock), 0, st, o, L.fx, L.fy, L.fz,
+                C.d.nx, C.d.ny, uc, (T)cp.ocf, f, u, t, om, done);
     { T *tmp = u; u = t; t = tmp; }
     for (int q = 1; q < cp.nu; q++) {
-        PROF_LAUNCH(P, pc ? HMFEM_PROF_COARSE : HMFEM_PROF_JACOBI, (k_jacobi<T, FT>), dim3(G), dim3(kBlock), 0, st, o,
-                    f, u, t, om);
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_jacobi<T, T>), dim3(G), dim3(kBlock), 0, st, o, f, u, t, om, done);
         T *tmp = u; u = t; t = tmp;
     }
-    // keep the level's buffers consistent with where the result lives
-    L.u = u; L.t = t;
+    if (u != (T *)L.u) { L.t = L.u; L.u = u; }
     return u;
+}
+
+// Fine-level V-cycle: z = M r, with the CG dots fused into its last sweep.
+// v is nullptr before the first iteration.
+template <typename T>
+static T *vcycle_fine(hipStream_t st, Hierarchy &H, const double *r, const double *v, const CycleParams &cp, Prof *P,
+                      CgState &S)
+{
+    Level &L = H.lev[0];
+    T *u = (T *)L.u, *t = (T *)L.t;
+    const int *done = S.done;
+    if (H.nlev == 1) {
+        hipLaunchKernelGGL(k_dense_dots<T>, dim3(S.ch.G), dim3(kBlock), 0, st, H.nc, H.inv, r, v, u, S.Pdot, done);
+        return u;
+    }
+    OpArgs<T> o = opargs<T>(L);
+    const T om = (T)cp.omega;
+    int G = grid_for(L.d.n(), kBlock);
+    PROF_LAUNCH(P, HMFEM_PROF_SMOOTH0, (k_smooth_zero<T, double>), dim3(G), dim3(kBlock), 0, st, o, r, u, om,
+                cp.nu >= 2 ? 2 : 1, done);
+    for (int q = 2; q < cp.nu; q++) {
+        PROF_LAUNCH(P, HMFEM_PROF_JACOBI, (k_jacobi<T, double>), dim3(G), dim3(kBlock), 0, st, o, r, u, t, om, done);
+        T *tmp = u; u = t; t = tmp;
+    }
+    Level &C = H.lev[1];
+    PROF_LAUNCH(P, HMFEM_PROF_RESTRICT, (k_resid_restrict<T, double>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock),
+                0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, r, u, (T *)C.f, done);
+    T *uc = vcycle_coarse<T>(st, H, 1, cp, P, done);
+    if (cp.nu == 1) {
+        PROF_LAUNCH(P, HMFEM_PROF_PROLONG, k_final_sweep_dots<T>, dim3(S.ch.G), dim3(kBlock), 0, st, o, L.fx, L.fy,
+                    C.d.nx, C.d.ny, L.fz, (const T *)uc, (T)cp.ocf, r, u, t, om, v, S.ch.chunk, S.Pdot, done);
+        return t;
+    }
+    PROF_LAUNCH(P, HMFEM_PROF_PROLONG, (k_prolong_jacobi<T, double>), dim3(G), dim3(kBlock), 0, st, o, L.fx, L.fy,
+                L.fz, C.d.nx, C.d.ny, uc, (T)cp.ocf, r, u, t, om, done);
+    { T *tmp = u; u = t; t = tmp; }
+    for (int q = 1; q + 1 < cp.nu; q++) {
+        PROF_LAUNCH(P, HMFEM_PROF_JACOBI, (k_jacobi<T, double>), dim3(G), dim3(kBlock), 0, st, o, r, u, t, om, done);
+        T *tmp = u; u = t; t = tmp;
+    }
+    PROF_LAUNCH(P, HMFEM_PROF_JACOBI, k_final_sweep_dots<T>, dim3(S.ch.G), dim3(kBlock), 0, st, o, 1, 1, 1, 1, 1,
+                (const T *)nullptr, (T)0, r, u, t, om, v, S.ch.chunk, S.Pdot, done);
+    return t;
 }
 
 struct SolveResult {
@@ -703,60 +1056,88 @@ struct SolveResult {
 
 template <typename T>
 static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d, const double *b, double *x,
-                  const hmfem_options &opt, CgBuffers &B, CgScalars *h_sc, SolveResult &res, Prof *P)
+                  const hmfem_options &opt, CgBuffers &B, CgScalars *h_sc, SolveResult &res, Prof *P,
+                  GraphCache *graph)
 {
-    long n = d.n();
+    const int n = (int)d.n();
     OpArgs<double> o;
-    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz;
+    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
     o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
     CycleParams cp{opt.nu < 1 ? 1 : opt.nu, opt.omega, opt.ocf};
-    int Gr = grid_for(n, kBlock * kCellsPerThread);
+    CgState S;
+    S.B = B;
+    S.ch = chunking(n);
+    const int G = S.ch.G;
+    S.Pinit = B.partial;
+    S.Ptau = B.partial + 2 * kMaxReduceBlocks;
+    S.Prr = B.partial + 3 * kMaxReduceBlocks;
+    S.Pdot = B.partial + 4 * kMaxReduceBlocks;
+    S.done = &B.sc->done;
 
-    // scalars: set the constant fields from the host
     CgScalars init{};
     init.rtol2 = opt.rtol * opt.rtol;
     init.maxit = opt.max_iterations;
     init.stop_rule = opt.stop_rule;
     HMFEM_CHECK(hipMemcpyAsync(B.sc, &init, sizeof(CgScalars), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_cg_init, dim3(Gr), dim3(kBlock), 0, st, o, b, x, B.r, B.sc, B.partial);
-    T *z = vcycle_level<T, double>(st, H, 0, B.r, cp, P);
-    hipLaunchKernelGGL(k_cg_first_dot<T>, dim3(Gr), dim3(kBlock), 0, st, n, (const T *)z, B.r, B.sc, B.partial);
+    hipLaunchKernelGGL(k_cg_init, dim3(G), dim3(kBlock), 0, st, o, b, x, B.r, S.ch.chunk, S.Pinit);
+    T *z = vcycle_fine<T>(st, H, B.r, nullptr, cp, P, S);
     HMFEM_CHECK(hipGetLastError());
 
-    double *s_in = B.s0, *s_out = B.s1;
-    int first = 1;
-    int queued = 0;
-    const int chunk = opt.check_every < 1 ? 1 : opt.check_every;
+    // One CG iteration; identical launches every iteration (the index and the
+    // buffer parity are read from the device).
+    auto iteration = [&](Prof *PP) -> T * {
+        PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv<T>, dim3(G), dim3(kBlock), 0, st, o, (const T *)z, B.s0, B.s1,
+                    B.v, S.ch.chunk, G, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+        PROF_LAUNCH(PP, HMFEM_PROF_CG_UPDATE, k_cg_update, dim3(G), dim3(kBlock), 0, st, n, B.s0, B.s1, B.v, x, B.r,
+                    S.ch.chunk, G, B.sc, S.Ptau, S.Prr);
+        return vcycle_fine<T>(st, H, B.r, B.v, cp, PP, S);
+    };
+    hipGraphExec_t exec = nullptr;
+    if (graph && !P) {
+        // key: everything the captured launches depend on
+        std::vector<char> key;
+        auto put = [&](const void *p, size_t nb) { key.insert(key.end(), (const char *)p, (const char *)p + nb); };
+        put(&d, sizeof(d)); put(&opt, sizeof(opt)); put(&H.lev[0].s32.e, sizeof(void *));
+        put(&A, sizeof(A)); put(&x, sizeof(x)); put(&B, sizeof(B)); put(&z, sizeof(z));
+        put(&H.tail_ops, sizeof(void *)); put(&H.inv, sizeof(void *));
+        if (graph->exec && graph->key == key) exec = graph->exec;
+        else {
+            if (graph->exec) { (void)hipGraphExecDestroy(graph->exec); graph->exec = nullptr; }
+            hipGraph_t g = nullptr;
+            HMFEM_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            T *zc = iteration(nullptr);
+            HMFEM_CHECK(hipStreamEndCapture(st, &g));
+            HMFEM_CHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            if (zc != z) { (void)hipGraphExecDestroy(exec); exec = nullptr; }   // z must be a fixed buffer
+            else { graph->exec = exec; graph->key = key; }
+        }
+    }
+    int k = 0;
+    const int every = opt.check_every < 1 ? 1 : opt.check_every;
     while (true) {
-        for (int q = 0; q < chunk; q++) {
-            PROF_LAUNCH(P, HMFEM_PROF_CG_SPMV, k_cg_spmv<T>, dim3(Gr), dim3(kBlock), 0, st, o, (const T *)z, s_in,
-                        s_out, B.v, first, B.sc, B.partial);
-            PROF_LAUNCH(P, HMFEM_PROF_CG_UPDATE, k_cg_update, dim3(Gr), dim3(kBlock), 0, st, n, s_out, B.v, x, B.r,
-                        B.sc, B.partial);
-            z = vcycle_level<T, double>(st, H, 0, B.r, cp, P);
-            PROF_LAUNCH(P, HMFEM_PROF_CG_DOTS, k_cg_dots<T>, dim3(Gr), dim3(kBlock), 0, st, n, (const T *)z, B.r, B.v,
-                        B.sc, B.partial);
-            double *tmp = s_in; s_in = s_out; s_out = tmp;
-            first = 0;
-            queued++;
+        for (int q = 0; q < every; q++, k++) {
+            if (exec) HMFEM_CHECK(hipGraphLaunch(exec, st));
+            else z = iteration(P);
         }
         HMFEM_CHECK(hipGetLastError());
         HMFEM_CHECK(hipMemcpyAsync(h_sc, B.sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
         HMFEM_CHECK(hipStreamSynchronize(st));
-        if (h_sc->done || queued >= opt.max_iterations + chunk) break;
+        if (h_sc->done || k > opt.max_iterations + every) break;
     }
     res.iterations = h_sc->iter;
-    res.status = h_sc->status;
+    res.status = h_sc->done ? h_sc->status : HMFEM_NOT_CONVERGED;
     res.rr = h_sc->rr;
     res.bb = h_sc->bb;
-    res.sigma = h_sc->sigma;
+    res.sigma = h_sc->sigma[h_sc->iter & 1];
     res.sigma_first = h_sc->sigma_first;
     return HMFEM_OK;
 }
 
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
-                     int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink)
+                     int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink, std::vector<char> &stage,
+                     GraphCache *graph)
 {
     Prof prof, *P = nullptr;
     if (sink && sink->ev && sink->cap >= 2) {
@@ -767,12 +1148,13 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
     Hierarchy H;
     int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
     HMFEM_CHECK(hipEventRecord(ev[0], st));
-    int rc = build_hierarchy(st, d, A, opt.coarse_max, prec, ar, H, d_info);
+    int rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage);
     if (rc) return rc;
     HMFEM_CHECK(hipEventRecord(ev[1], st));
     SolveResult res{};
-    if (prec == HMFEM_PREC_FP64) rc = run_cg<double>(st, H, A, d, b, x, opt, B, h_sc, res, P);
-    else rc = run_cg<float>(st, H, A, d, b, x, opt, B, h_sc, res, P);
+    GraphCache *gc = opt.use_graph ? graph : nullptr;
+    if (prec == HMFEM_PREC_FP64) rc = run_cg<double>(st, H, A, d, b, x, opt, B, h_sc, res, P, gc);
+    else rc = run_cg<float>(st, H, A, d, b, x, opt, B, h_sc, res, P, gc);
     if (rc) return rc;
     HMFEM_CHECK(hipEventRecord(ev[2], st));
     HMFEM_CHECK(hipEventSynchronize(ev[2]));
@@ -785,19 +1167,18 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
             int t = P->type[q / 2];
             if (t >= 0 && t < HMFEM_PROF_TYPES) { sink->ms[t] += ms; sink->count[t] += 1; }
         }
-        // bytes per launch of the fine-level kernels (algorithmic, actual cells)
+        // algorithmic bytes per launch of the fine-level kernels (actual cells)
         double n0 = (double)d.n();
         double vs = prec == HMFEM_PREC_FP64 ? 8.0 : 4.0;
         double c4 = 4.0 * vs;
-        const Level &L1 = H.lev[H.nlev > 1 ? 1 : 0];
-        double n1 = (double)L1.d.n();
+        double n1 = (double)H.lev[H.nlev > 1 ? 1 : 0].d.n();
         sink->bytes[HMFEM_PROF_CG_SPMV] = n0 * (32 + 8 + vs + 8 + 8);        // A64, s_in, z, s_out, v
         sink->bytes[HMFEM_PROF_CG_UPDATE] = n0 * (6 * 8);                    // x, s, r, v in; x, r out
-        sink->bytes[HMFEM_PROF_CG_DOTS] = n0 * (vs + 16);                    // z, r, v
+        sink->bytes[HMFEM_PROF_CG_DOTS] = 0.0;                               // fused into the last sweep
         sink->bytes[HMFEM_PROF_SMOOTH0] = n0 * (c4 + 8 + vs);                // A, f, u out
         sink->bytes[HMFEM_PROF_RESTRICT] = n0 * (c4 + 8 + vs) + n1 * vs;     // A, f, u; f_c out
         sink->bytes[HMFEM_PROF_PROLONG] = n0 * (c4 + 8 + 2 * vs) + n1 * vs;  // A, f, u, u_c; u out
-        sink->bytes[HMFEM_PROF_JACOBI] = n0 * (c4 + 8 + 2 * vs);             // A, f, u; u out
+        sink->bytes[HMFEM_PROF_JACOBI] = n0 * (c4 + 8 + 2 * vs + 8);         // A, f, u, v; z out (+dots)
     }
     if (out) {
         float t0 = 0, t1 = 0;
@@ -810,24 +1191,29 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         out->status = info ? HMFEM_ERR_BREAKDOWN : res.status;
         out->rel_residual = res.bb > 0 ? sqrt(res.rr / res.bb) : 0.0;
         out->precond_ratio = res.sigma_first > 0 ? sqrt(res.sigma / res.sigma_first) : 0.0;
-        // algorithmic bytes of the solve phase (per cell, per iteration)
+        // algorithmic HBM bytes of the solve phase: per CG iteration the
+        // fine-level kernels (CG SpMV, update, V-cycle sweeps) plus the
+        // multi-block coarse levels
+        double vs = prec == HMFEM_PREC_FP64 ? 8.0 : 4.0, cs = vs;
         double per = 0.0;
-        double vs = prec == HMFEM_PREC_FP64 ? 8.0 : 4.0;
-        double cs = vs;  // coefficient bytes
-        for (int l = 0; l + 1 < H.nlev; l++) {
+        int lim = H.tail0 > 0 ? H.tail0 : H.nlev - 1;
+        for (int l = 0; l < lim; l++) {
             double nl = (double)H.lev[l].d.n();
             double fb = l == 0 ? 8.0 : vs;
-            double sweeps = 2.0 * opt.nu;  // pre + post
-            // each sweep: 4 coefficients + f + u (read) + u (write)
-            per += nl * (sweeps * (4 * cs + fb + 2 * vs) + (4 * cs + fb + vs) + vs);
+            double sweeps = 2.0 * opt.nu;
+            per += nl * (sweeps * (4 * cs + fb + 2 * vs) + (4 * cs + fb + vs));
         }
         double n0 = (double)d.n();
-        per += n0 * (32 + 8 + vs + 8 + 8) + n0 * 48 + n0 * (vs + 16);
+        per += n0 * (32 + 8 + vs + 8 + 8) + n0 * 48 + n0 * 8;
         out->solve_bytes = per * (res.iterations + 1);
     }
     return HMFEM_OK;
 }
 
-size_t cg_partial_count(long n) { return (size_t)2 * grid_for(n, kBlock * kCellsPerThread) + 64; }
+size_t cg_partial_count(long n)
+{
+    (void)n;
+    return (size_t)6 * kMaxReduceBlocks + 64;
+}
 
 } // namespace hmfem

@@ -56,7 +56,8 @@ class HmfemGrid(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("rtol", c_dbl), ("max_iterations", c_int), ("precision", c_int), ("stop_rule", c_int),
-                ("nu", c_int), ("omega", c_dbl), ("ocf", c_dbl), ("coarse_max", c_int), ("check_every", c_int)]
+                ("nu", c_int), ("omega", c_dbl), ("ocf", c_dbl), ("coarse_max", c_int), ("check_every", c_int),
+                ("tail_max", c_int), ("tail_gamma", c_int), ("use_graph", c_int)]
 
 
 class Status(ctypes.Structure):
@@ -211,6 +212,16 @@ def _dptr(t):
     return t.data_ptr()
 
 
+def _order_after_torch(*tensors):
+    """The library runs on its own stream. Work that torch queued on its
+    current stream for these tensors (fills, copies) must finish first."""
+    for t in tensors:
+        if hasattr(t, "data_ptr") and getattr(t, "is_cuda", False):
+            import torch
+            torch.cuda.current_stream(t.device).synchronize()
+            return
+
+
 class Context:
     """A HIP stream plus cached device buffers of the library."""
 
@@ -251,6 +262,7 @@ class Context:
         return arr
 
     def assemble(self, g, Kx, Ky, Kz, bval, b, perm=None, indcol=None, KI=None, WI=None, pc=None, pw=None):
+        _order_after_torch(Kx, Ky, Kz, b, perm, indcol, KI, WI, pc, pw, *bval)
         rc = lib().hmfem_assemble_device(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz), _dptr(KI),
                                          _dptr(WI), _dptr(pc), _dptr(pw), self._faces(bval), _dptr(perm),
                                          _dptr(indcol), _dptr(b))
@@ -260,6 +272,7 @@ class Context:
     def reservoir(self, g, Kx, Ky, Kz, bval, p, v=None, options=None, KI=None, WI=None, pc=None, pw=None):
         st = Status()
         o = options if options is not None else default_options()
+        _order_after_torch(Kx, Ky, Kz, p, v, KI, WI, pc, pw, *bval)
         rc = lib().hmfem_reservoir_device(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz), _dptr(KI),
                                           _dptr(WI), _dptr(pc), _dptr(pw), self._faces(bval), _dptr(p), _dptr(v),
                                           ctypes.byref(o), ctypes.byref(st))
@@ -269,6 +282,7 @@ class Context:
 
     def flux(self, g, Kx, Ky, Kz, bval, p, v=None):
         mn, mx = c_dbl(), c_dbl()
+        _order_after_torch(Kx, Ky, Kz, p, v, *bval)
         rc = lib().hmfem_flux_device(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz), self._faces(bval),
                                      _dptr(p), _dptr(v), ctypes.byref(mn), ctypes.byref(mx))
         if rc != 0:
@@ -278,6 +292,7 @@ class Context:
     def solve_structured(self, nx, ny, nz, perm, indcol, b, x, options=None):
         st = Status()
         o = options if options is not None else default_options()
+        _order_after_torch(perm, indcol, b, x)
         rc = lib().hmfem_solve_structured_device(self.h, nx, ny, nz, _dptr(perm), _dptr(indcol), _dptr(b), _dptr(x),
                                                  ctypes.byref(o), ctypes.byref(st))
         if rc < 0:

@@ -113,9 +113,13 @@ typedef struct {
     int stop_rule;           /* default HMFEM_STOP_RESIDUAL */
     int nu;                  /* Jacobi sweeps before and after the coarse correction (default 2) */
     double omega;            /* Jacobi damping (default 0.8) */
-    double ocf;              /* coarse-correction scaling (default 1.7) */
-    int coarse_max;          /* largest coarsest level, solved densely (default 128, at most 128) */
+    double ocf;              /* coarse-correction scaling (default 1.9) */
+    int coarse_max;          /* largest coarsest level, solved densely (default and maximum 80) */
     int check_every;         /* CG iterations queued between host convergence checks (default 4) */
+    int tail_max;            /* levels of at most this many cells run in one single-workgroup
+                                kernel (default 4096; 0 disables) */
+    int tail_gamma;          /* cycle inside that kernel: 1 V-cycle, 2 W-cycle (default 2) */
+    int use_graph;           /* replay each CG iteration from a captured HIP graph (default 1) */
 } hmfem_options;
 
 typedef struct {
