@@ -29,6 +29,7 @@ struct hmfem_context {
     bool prof_on = false;
     std::vector<char> stage;   // host staging of small uploads (tail op list)
     GraphCache graph;          // captured CG iteration of the last solve
+    long lds_limit = 65536;    // dynamic LDS a workgroup may use on this device
 
     // device buffer `name` with at least `bytes`, grown on demand
     void *get(const char *name, size_t bytes)
@@ -68,14 +69,15 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->max_iterations = 10000;
     o->precision = HMFEM_PREC_FP32;
     o->stop_rule = HMFEM_STOP_RESIDUAL;
-    o->nu = 2;
-    o->omega = 0.8;
-    o->ocf = 1.9;
+    o->nu = 3;
+    o->omega = 0.95;
+    o->ocf = 2.0;
     o->coarse_max = kCoarseMax;
     o->check_every = 4;
-    o->tail_max = 4096;
+    o->tail_max = 512;
     o->tail_gamma = 2;
     o->use_graph = 1;
+    o->tail_lds = 1;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)
@@ -87,6 +89,10 @@ extern "C" int hmfem_create(hmfem_context **out)
     HMFEM_CHECK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     for (int q = 0; q < 8; q++) HMFEM_CHECK(hipEventCreate(&c->ev[q]));
     HMFEM_CHECK(hipHostMalloc((void **)&c->h_sc, sizeof(CgScalars), hipHostMallocDefault));
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+        c->lds_limit = (lds < 65536 ? lds : 65536) - 8192;   // room for the kernel's static LDS
     *out = c;
     return HMFEM_OK;
 }
@@ -184,7 +190,7 @@ static int solve_on_stencil(hmfem_context *c, Dims d, const Stencil<double> &A, 
     hmfem_status tmp{};
     hmfem_status *out = st ? st : &tmp;
     return solve_structured(c->st, d, A, b, x, opt, arena, hb, B, c->h_sc, info, out, c->ev, sink_of(c), c->stage,
-                            &c->graph);
+                            &c->graph, c->lds_limit);
 }
 
 // ------------------------------------------------------------ device API

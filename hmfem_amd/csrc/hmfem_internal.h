@@ -150,6 +150,6 @@ size_t cg_partial_count(long n);
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink,
-                     std::vector<char> &stage, GraphCache *graph);
+                     std::vector<char> &stage, GraphCache *graph, long lds_limit);
 
 } // namespace hmfem

@@ -106,6 +106,8 @@ __device__ __forceinline__ C apply_at(const OpArgs<T> &o, const Cell &c, U u, C 
     return a;
 }
 
+constexpr int kUnroll = 4;   // cells per thread in flight in chunked kernels
+
 // ---------------------------------------------------------------- reductions
 
 // Sum of NV values over the block in a fixed order; every thread gets the sums.
@@ -190,13 +192,15 @@ k_jacobi(OpArgs<T> o, const FT *__restrict__ f, const T *__restrict__ u, T *__re
 }
 
 // Residual r = f - A u summed over each aggregate: one thread per coarse cell.
-template <typename T, typename FT>
+// F = 2: all aggregation factors are 2 (loops unrolled); F = 0: run-time factors.
+template <typename T, typename FT, int F>
 __global__ void __launch_bounds__(kBlock)
 k_resid_restrict(OpArgs<T> o, int fx, int fy, int fz, int cx, int cy, int cz, const FT *__restrict__ f,
                  const T *__restrict__ u, T *__restrict__ fc, const int *done)
 {
     using C = T;
     if (*done) return;
+    if (F) { fx = F; fy = F; fz = F; }
     int nc = cx * cy * cz;
     int I = xcd_block() * kBlock + threadIdx.x;
     if (I >= nc) return;
@@ -205,13 +209,19 @@ k_resid_restrict(OpArgs<T> o, int fx, int fy, int fz, int cx, int cy, int cz, co
     int cj = rem / cx, ci = rem - cj * cx;
     C acc = 0;
     auto uf = [&](const Cell &q) -> C { return (C)u[q.s]; };
-    for (int dk = 0; dk < fz; dk++) {
+#pragma unroll
+    for (int dk = 0; dk < (F ? F : 8); dk++) {
+        if (dk >= fz) break;
         int k = ck * fz + dk;
         if (k >= o.nz) break;
-        for (int dj = 0; dj < fy; dj++) {
+#pragma unroll
+        for (int dj = 0; dj < (F ? F : 8); dj++) {
+            if (dj >= fy) break;
             int j = cj * fy + dj;
             if (j >= o.ny) break;
-            for (int di = 0; di < fx; di++) {
+#pragma unroll
+            for (int di = 0; di < (F ? F : 8); di++) {
+                if (di >= fx) break;
                 int i = ci * fx + di;
                 if (i >= o.nx) break;
                 Cell c;
@@ -262,25 +272,56 @@ k_final_sweep_dots(OpArgs<T> o, int fx, int fy, int cx, int cy, int fz, const T 
     int lb, beg, end;
     chunk_range(n, chunk, lb, beg, end);
     double acc[2] = {0.0, 0.0};
-    for (int base = beg; base < end; base += kBlock) {
-        int s = base + threadIdx.x;
-        if (s >= end) break;
-        Cell c = cell_at(o, s);
-        auto up = [&](const Cell &q) -> C {
-            C x = (C)u[q.s];
-            if (uc) x += ocf * (C)uc[q.i / fx + cx * (q.j / fy + cy * (q.k / fz))];
-            return x;
-        };
-        C d;
-        C au = apply_at<T, C>(o, c, up, d);
-        double fs = f[s];
-        C z = up(c) + omega * ((C)fs - au) * rcp(d);
-        uo[s] = z;
-        acc[0] += (double)z * fs;
-        if (v) acc[1] += (double)z * v[s];
+    auto up = [&](const Cell &q) -> C {
+        C x = (C)u[q.s];
+        if (uc) x += ocf * (C)uc[q.i / fx + cx * (q.j / fy + cy * (q.k / fz))];
+        return x;
+    };
+    for (int base = beg; base < end; base += kUnroll * kBlock) {
+        C zz[kUnroll];
+        double ff[kUnroll], vv[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) {
+            int s = base + q * kBlock + threadIdx.x;
+            zz[q] = 0; ff[q] = 0.0; vv[q] = 0.0;
+            if (s < end) {
+                Cell c = cell_at(o, s);
+                C d;
+                C au = apply_at<T, C>(o, c, up, d);
+                ff[q] = f[s];
+                if (v) vv[q] = v[s];
+                zz[q] = up(c) + omega * ((C)ff[q] - au) * rcp(d);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) {
+            int s = base + q * kBlock + threadIdx.x;
+            if (s < end) {
+                uo[s] = zz[q];
+                acc[0] += (double)zz[q] * ff[q];
+                acc[1] += (double)zz[q] * vv[q];
+            }
+        }
     }
     block_sum<2>(acc);
     if (threadIdx.x == 0) { Pdot[2 * lb] = acc[0]; Pdot[2 * lb + 1] = acc[1]; }
+}
+
+// u = A^-1 f for the dense coarsest level: groups of L lanes (a power of two
+// dividing 64) share one row and combine their partial sums by shuffles.
+template <typename T>
+__device__ __forceinline__ void dense_gemv(const double *inv, const double *fs, T *u, int nc, int tid, int nthreads)
+{
+    int L = 16;
+    while (L > 1 && L * nc > nthreads) L >>= 1;
+    const int row = tid / L, lane = tid % L;
+    double acc = 0.0;
+    if (row < nc) {
+        const double *r = inv + (long)row * nc;
+        for (int j = lane; j < nc; j += L) acc += r[j] * fs[j];
+    }
+    for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (row < nc && lane == 0) u[row] = (T)acc;
 }
 
 // Dense coarsest solve u = A^-1 f, one block.
@@ -292,12 +333,7 @@ k_dense_apply(int n, const double *__restrict__ inv, const FT *__restrict__ f, T
     if (*done) return;
     for (int q = threadIdx.x; q < n; q += kBlock) fs[q] = (double)f[q];
     __syncthreads();
-    for (int r = threadIdx.x; r < n; r += kBlock) {
-        double acc = 0.0;
-        const double *row = inv + (long)r * n;
-        for (int q = 0; q < n; q++) acc += row[q] * fs[q];
-        u[r] = (T)acc;
-    }
+    dense_gemv<T>(inv, fs, u, n, threadIdx.x, kBlock);
 }
 
 // Dense solve with the CG dots, for a single-level hierarchy: block 0 does the
@@ -340,9 +376,11 @@ struct TailLevel {
     OpArgs<T> o;
     T *f, *u, *t, *r, *w;
     int n, fx, fy, fz;
+    int lds_off;   // byte offset of this level's copy in LDS (LDS mode), -1 otherwise
 };
 
 constexpr int kTailThreads = 1024;
+constexpr int kMaxTailLevels = 16;
 
 // Executes ops[0..nops) in order, each over all cells of its level with the
 // 1024 threads of one workgroup, and a barrier after each op. Every thread
@@ -350,14 +388,47 @@ constexpr int kTailThreads = 1024;
 // kept in a register and agrees across threads.
 template <typename T>
 __global__ void __launch_bounds__(kTailThreads)
-k_tail(const TailLevel<T> *__restrict__ Lv, const int4 *__restrict__ ops, int nops, const double *__restrict__ inv,
-       int nc, T omega, T ocf, const int *done)
+k_tail(const TailLevel<T> *__restrict__ Lg, const int4 *__restrict__ ops, int nops, const double *__restrict__ invg,
+       int nc, T omega, T ocf, int nlev, int lds_inv_off, const int *done)
 {
     using C = T;
+    extern __shared__ char lds[];
+    __shared__ TailLevel<T> Ls[kMaxTailLevels];
     __shared__ double fs[kCoarseMax];
     if (*done) return;
-    unsigned parity = 0;
     const int tid = threadIdx.x;
+    // LDS mode: copy coefficients, the top level's rhs and the dense inverse
+    // into LDS and run every op on the copies.
+    const bool in_lds = lds_inv_off >= 0;
+    const TailLevel<T> *Lv = Lg;
+    const double *inv = invg;
+    if (in_lds) {
+        if (tid < nlev) {
+            TailLevel<T> L = Lg[tid];
+            T *b = (T *)(lds + L.lds_off);
+            const int n = L.n;
+            TailLevel<T> M = L;
+            M.o.e = b; M.o.wx = b + n; M.o.wy = b + 2 * n; M.o.wz = b + 3 * n; M.o.di = b + 4 * n;
+            M.f = b + 5 * n; M.u = b + 6 * n; M.t = b + 7 * n; M.r = b + 8 * n; M.w = b + 9 * n;
+            Ls[tid] = M;
+        }
+        for (int l = 0; l < nlev; l++) {
+            const TailLevel<T> L = Lg[l];
+            T *b = (T *)(lds + L.lds_off);
+            const int n = L.n;
+            for (int s = tid; s < n; s += kTailThreads) {
+                b[s] = L.o.e[s]; b[n + s] = L.o.wx[s]; b[2 * n + s] = L.o.wy[s];
+                b[3 * n + s] = L.o.wz[s]; b[4 * n + s] = L.o.di[s];
+                if (l == 0) b[5 * n + s] = L.f[s];
+            }
+        }
+        double *ilds = (double *)(lds + lds_inv_off);
+        for (int q = tid; q < nc * nc; q += kTailThreads) ilds[q] = invg[q];
+        __syncthreads();
+        Lv = Ls;
+        inv = ilds;
+    }
+    unsigned parity = 0;
     for (int q = 0; q < nops; q++) {
         const int4 op = ops[q];
         const int l = op.y;
@@ -451,17 +522,17 @@ k_tail(const TailLevel<T> *__restrict__ Lv, const int4 *__restrict__ ops, int no
         case OP_DENSE:
             for (int s = tid; s < nc; s += kTailThreads) fs[s] = (double)f[s];
             __syncthreads();
-            for (int r = tid; r < nc; r += kTailThreads) {
-                double acc = 0.0;
-                const double *row = inv + (long)r * nc;
-                for (int j = 0; j < nc; j++) acc += row[j] * fs[j];
-                cur[r] = (T)acc;
-            }
+            dense_gemv<T>(inv, fs, cur, nc, tid, kTailThreads);
             break;
         default:
             break;
         }
         __syncthreads();
+    }
+    if (in_lds) {
+        const TailLevel<T> G0 = Lg[0];
+        const T *src = Lv[0].u;   // OP_FINAL left the result in u
+        for (int s = tid; s < G0.n; s += kTailThreads) G0.u[s] = src[s];
     }
 }
 
@@ -578,17 +649,27 @@ k_cg_init(OpArgs<double> o, const double *__restrict__ b, const double *__restri
     int lb, beg, end;
     chunk_range(n, chunk, lb, beg, end);
     double v[2] = {0.0, 0.0};
-    for (int base = beg; base < end; base += kBlock) {
-        int s = base + threadIdx.x;
-        if (s >= end) break;
-        Cell c = cell_at(o, s);
-        double d;
-        double ax = apply_at<double, double>(o, c, [&](const Cell &t) { return x[t.s]; }, d);
-        double bs = b[s];
-        double rs = bs - ax;
-        r[s] = rs;
-        v[0] += bs * bs;
-        v[1] += rs * rs;
+    for (int base = beg; base < end; base += kUnroll * kBlock) {
+        double bs[kUnroll], rs[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) {
+            int s = base + q * kBlock + threadIdx.x;
+            bs[q] = 0.0; rs[q] = 0.0;
+            if (s < end) {
+                Cell c = cell_at(o, s);
+                double d;
+                double ax = apply_at<double, double>(o, c, [&](const Cell &t) { return x[t.s]; }, d);
+                bs[q] = b[s];
+                rs[q] = bs[q] - ax;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) {
+            int s = base + q * kBlock + threadIdx.x;
+            if (s < end) r[s] = rs[q];
+            v[0] += bs[q] * bs[q];
+            v[1] += rs[q] * rs[q];
+        }
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) { Pinit[2 * lb] = v[0]; Pinit[2 * lb + 1] = v[1]; }
@@ -645,16 +726,25 @@ k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, d
     chunk_range(n, chunk, lb, beg, end);
     double acc[1] = {0.0};
     auto sv = [&](const Cell &t) -> double { return k == 0 ? (double)z[t.s] : (double)z[t.s] + beta * sin[t.s]; };
-    for (int base = beg; base < end; base += kBlock) {
-        int s = base + threadIdx.x;
-        if (s >= end) break;
-        Cell c = cell_at(o, s);
-        double d;
-        double av = apply_at<double, double>(o, c, sv, d);
-        double ss = sv(c);
-        sout[s] = ss;
-        v[s] = av;
-        acc[0] += ss * av;
+    for (int base = beg; base < end; base += kUnroll * kBlock) {
+        double ss[kUnroll], av[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) {
+            int s = base + q * kBlock + threadIdx.x;
+            ss[q] = 0.0; av[q] = 0.0;
+            if (s < end) {
+                Cell c = cell_at(o, s);
+                double d;
+                av[q] = apply_at<double, double>(o, c, sv, d);
+                ss[q] = sv(c);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) {
+            int s = base + q * kBlock + threadIdx.x;
+            if (s < end) { sout[s] = ss[q]; v[s] = av[q]; }
+            acc[0] += ss[q] * av[q];
+        }
     }
     block_sum<1>(acc);
     if (threadIdx.x == 0) Ptau[lb] = acc[0];
@@ -680,13 +770,21 @@ k_cg_update(int n, const double *__restrict__ s0, const double *__restrict__ s1,
     int lb, beg, end;
     chunk_range(n, chunk, lb, beg, end);
     double acc[1] = {0.0};
-    for (int base = beg; base < end; base += kBlock) {
-        int i = base + threadIdx.x;
-        if (i >= end) break;
-        x[i] += alpha * s[i];
-        double ri = r[i] - alpha * v[i];
-        r[i] = ri;
-        acc[0] += ri * ri;
+    for (int base = beg; base < end; base += kUnroll * kBlock) {
+        double xs[kUnroll], ss[kUnroll], rs[kUnroll], vs[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) {
+            int i = base + q * kBlock + threadIdx.x;
+            xs[q] = 0.0; ss[q] = 0.0; rs[q] = 0.0; vs[q] = 0.0;
+            if (i < end) { xs[q] = x[i]; ss[q] = s[i]; rs[q] = r[i]; vs[q] = v[i]; }
+        }
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) {
+            int i = base + q * kBlock + threadIdx.x;
+            double ri = rs[q] - alpha * vs[q];
+            if (i < end) { x[i] = xs[q] + alpha * ss[q]; r[i] = ri; }
+            acc[0] += ri * ri;
+        }
     }
     block_sum<1>(acc);
     if (threadIdx.x == 0) Prr[lb] = acc[0];
@@ -704,6 +802,8 @@ struct Hierarchy {
     void *tail_levels = nullptr;
     int4 *tail_ops = nullptr;
     int tail_nops = 0;
+    int tail_lds_bytes = 0;    // dynamic LDS of the tail kernel (0: global mode)
+    int tail_lds_inv = -1;     // LDS offset of the dense inverse (-1: global mode)
 };
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -724,7 +824,6 @@ struct Arena {
 };
 
 constexpr int kMaxTailOps = 4096;
-constexpr int kMaxTailLevels = 16;
 
 static void plan_levels(Dims fine, int coarse_max, Level *lev, int &nlev)
 {
@@ -811,10 +910,12 @@ OpArgs<double> opargs<double>(const Level &L)
 }
 
 template <typename T>
-static int upload_tail(hipStream_t st, Hierarchy &H, const hmfem_options &opt, std::vector<char> &stage)
+static int upload_tail(hipStream_t st, Hierarchy &H, const hmfem_options &opt, std::vector<char> &stage,
+                       long lds_limit)
 {
     int last = H.nlev - 1;
     std::vector<TailLevel<T>> tl(H.nlev - H.tail0);
+    size_t off = 0;
     for (int l = H.tail0; l < H.nlev; l++) {
         const Level &L = H.lev[l];
         TailLevel<T> &t = tl[l - H.tail0];
@@ -822,6 +923,18 @@ static int upload_tail(hipStream_t st, Hierarchy &H, const hmfem_options &opt, s
         t.f = (T *)L.f; t.u = (T *)L.u; t.t = (T *)L.t; t.r = (T *)L.r; t.w = (T *)L.w;
         t.n = (int)L.d.n();
         t.fx = L.fx; t.fy = L.fy; t.fz = L.fz;
+        t.lds_off = (int)off;
+        off += (10 * (size_t)t.n * sizeof(T) + 15) / 16 * 16;
+    }
+    size_t inv_off = off;
+    off += (size_t)H.nc * H.nc * sizeof(double);
+    if ((long)off <= lds_limit) {
+        H.tail_lds_bytes = (int)off;
+        H.tail_lds_inv = (int)inv_off;
+    } else {
+        H.tail_lds_bytes = 0;
+        H.tail_lds_inv = -1;
+        for (auto &t : tl) t.lds_off = -1;
     }
     std::vector<int4> ops;
     int gamma = opt.tail_gamma < 1 ? 1 : (opt.tail_gamma > 2 ? 2 : opt.tail_gamma);
@@ -840,7 +953,7 @@ static int upload_tail(hipStream_t st, Hierarchy &H, const hmfem_options &opt, s
 
 // Build the hierarchy from a fine FP64 stencil (device), inside `arena`.
 int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const hmfem_options &opt, Arena &ar,
-                    Hierarchy &H, int *d_info, std::vector<char> &stage)
+                    Hierarchy &H, int *d_info, std::vector<char> &stage, long lds_limit)
 {
     plan_levels(fine, clamp_coarse(opt.coarse_max), H.lev, H.nlev);
     int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
@@ -899,7 +1012,9 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const 
     }
     HMFEM_CHECK(hipGetLastError());
     if (H.tail0 > 0) {
-        int rc = prec == HMFEM_PREC_FP64 ? upload_tail<double>(st, H, opt, stage) : upload_tail<float>(st, H, opt, stage);
+        long lim = opt.tail_lds ? lds_limit : 0;
+        int rc = prec == HMFEM_PREC_FP64 ? upload_tail<double>(st, H, opt, stage, lim)
+                                         : upload_tail<float>(st, H, opt, stage, lim);
         if (rc) return rc;
     }
     return HMFEM_OK;
@@ -949,7 +1064,7 @@ static Chunking chunking(long n)
     if (g < kNumXcd) g = kNumXcd;
     g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
     long chunk = (n + g - 1) / g;
-    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    chunk = (chunk + kUnroll * kBlock - 1) / (kUnroll * kBlock) * (kUnroll * kBlock);
     c.G = (int)g;
     c.chunk = (int)chunk;
     return c;
@@ -977,9 +1092,9 @@ static T *vcycle_coarse(hipStream_t st, Hierarchy &H, int l, const CycleParams &
         return u;
     }
     if (H.tail0 > 0 && l == H.tail0) {
-        PROF_LAUNCH(P, HMFEM_PROF_COARSE, k_tail<T>, dim3(1), dim3(kTailThreads), 0, st,
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, k_tail<T>, dim3(1), dim3(kTailThreads), H.tail_lds_bytes, st,
                     (const TailLevel<T> *)H.tail_levels, H.tail_ops, H.tail_nops, H.inv, H.nc, (T)cp.omega,
-                    (T)cp.ocf, done);
+                    (T)cp.ocf, H.nlev - H.tail0, H.tail_lds_inv, done);
         return u;
     }
     OpArgs<T> o = opargs<T>(L);
@@ -992,8 +1107,12 @@ static T *vcycle_coarse(hipStream_t st, Hierarchy &H, int l, const CycleParams &
         T *tmp = u; u = t; t = tmp;
     }
     Level &C = H.lev[l + 1];
-    PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_resid_restrict<T, T>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock), 0, st,
-                o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, f, u, (T *)C.f, done);
+    if (L.fx == 2 && L.fy == 2 && L.fz == 2)
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_resid_restrict<T, T, 2>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock),
+                    0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, f, u, (T *)C.f, done);
+    else
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_resid_restrict<T, T, 0>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock),
+                    0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, f, u, (T *)C.f, done);
     T *uc = vcycle_coarse<T>(st, H, l + 1, cp, P, done);
     PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_prolong_jacobi<T, T>), dim3(G), dim3(kBlock), 0, st, o, L.fx, L.fy, L.fz,
                 C.d.nx, C.d.ny, uc, (T)cp.ocf, f, u, t, om, done);
@@ -1029,8 +1148,12 @@ static T *vcycle_fine(hipStream_t st, Hierarchy &H, const double *r, const doubl
         T *tmp = u; u = t; t = tmp;
     }
     Level &C = H.lev[1];
-    PROF_LAUNCH(P, HMFEM_PROF_RESTRICT, (k_resid_restrict<T, double>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock),
-                0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, r, u, (T *)C.f, done);
+    if (L.fx == 2 && L.fy == 2 && L.fz == 2)
+        PROF_LAUNCH(P, HMFEM_PROF_RESTRICT, (k_resid_restrict<T, double, 2>), dim3(grid_for(C.d.n(), kBlock)),
+                    dim3(kBlock), 0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, r, u, (T *)C.f, done);
+    else
+        PROF_LAUNCH(P, HMFEM_PROF_RESTRICT, (k_resid_restrict<T, double, 0>), dim3(grid_for(C.d.n(), kBlock)),
+                    dim3(kBlock), 0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, r, u, (T *)C.f, done);
     T *uc = vcycle_coarse<T>(st, H, 1, cp, P, done);
     if (cp.nu == 1) {
         PROF_LAUNCH(P, HMFEM_PROF_PROLONG, k_final_sweep_dots<T>, dim3(S.ch.G), dim3(kBlock), 0, st, o, L.fx, L.fy,
@@ -1137,7 +1260,7 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink, std::vector<char> &stage,
-                     GraphCache *graph)
+                     GraphCache *graph, long lds_limit)
 {
     Prof prof, *P = nullptr;
     if (sink && sink->ev && sink->cap >= 2) {
@@ -1148,7 +1271,7 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
     Hierarchy H;
     int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
     HMFEM_CHECK(hipEventRecord(ev[0], st));
-    int rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage);
+    int rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit);
     if (rc) return rc;
     HMFEM_CHECK(hipEventRecord(ev[1], st));
     SolveResult res{};

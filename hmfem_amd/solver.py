@@ -57,7 +57,7 @@ class HmfemGrid(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("rtol", c_dbl), ("max_iterations", c_int), ("precision", c_int), ("stop_rule", c_int),
                 ("nu", c_int), ("omega", c_dbl), ("ocf", c_dbl), ("coarse_max", c_int), ("check_every", c_int),
-                ("tail_max", c_int), ("tail_gamma", c_int), ("use_graph", c_int)]
+                ("tail_max", c_int), ("tail_gamma", c_int), ("use_graph", c_int), ("tail_lds", c_int)]
 
 
 class Status(ctypes.Structure):

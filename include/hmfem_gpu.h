@@ -111,15 +111,16 @@ typedef struct {
     int max_iterations;      /* default 10000, as pcg.h's _max */
     int precision;           /* storage/arithmetic of the AMG hierarchy (default FP32) */
     int stop_rule;           /* default HMFEM_STOP_RESIDUAL */
-    int nu;                  /* Jacobi sweeps before and after the coarse correction (default 2) */
-    double omega;            /* Jacobi damping (default 0.8) */
-    double ocf;              /* coarse-correction scaling (default 1.9) */
+    int nu;                  /* Jacobi sweeps before and after the coarse correction (default 3) */
+    double omega;            /* Jacobi damping (default 0.95) */
+    double ocf;              /* coarse-correction scaling (default 2.0) */
     int coarse_max;          /* largest coarsest level, solved densely (default and maximum 80) */
     int check_every;         /* CG iterations queued between host convergence checks (default 4) */
     int tail_max;            /* levels of at most this many cells run in one single-workgroup
-                                kernel (default 4096; 0 disables) */
+                                kernel (default 512; 0 disables) */
     int tail_gamma;          /* cycle inside that kernel: 1 V-cycle, 2 W-cycle (default 2) */
     int use_graph;           /* replay each CG iteration from a captured HIP graph (default 1) */
+    int tail_lds;            /* run that kernel on LDS copies of its levels when they fit (default 1) */
 } hmfem_options;
 
 typedef struct {
