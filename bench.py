@@ -35,7 +35,30 @@ WORKLOADS = {
                label="16^3 homogeneous K=1e-14, x-Dirichlet, rtol 1e-10"),
     "c3": dict(dims=(256, 256, 256), kind="karst", conduits=20,
                label="256^3 K=1e-14 + 20 karst conduits x10^U(4,6), x-Dirichlet, rtol 1e-8"),
+    "c4": dict(batch=96, edges=(32, 48, 64, 80, 96),
+               label="96 independent instances in one batch, edges from {32,48,64,80,96} per axis; even: lognormal "
+                     "sigma U[0.5,2.5] corr 8; odd: K=1e-14 + 1-5 conduits; x-Dirichlet, rtol 1e-8"),
 }
+
+
+def c4_instances(w):
+    """Per-instance dims and K (host), drawn from the instance's own stream."""
+    from hmfem_amd import fields
+    out = []
+    for i in range(w["batch"]):
+        rs = fields.Stream(SEED, i)
+        d = tuple(rs.choice(w["edges"]) for _ in range(3))
+        seed = fields.instance_seed(SEED, i)
+        if i % 2 == 0:
+            sig = rs.uniform(0.5, 2.5)
+            K = fields.lognormal(d[0], d[1], d[2], np.log(1e-14), sig, 8.0, seed)
+            desc = "lognormal sigma=%.3f" % sig
+        else:
+            nc = 1 + min(4, int(rs.uniform() * 5))
+            K = fields.conduits(fields.homogeneous(d[0], d[1], d[2], 1e-14), d[0], d[1], d[2], nc, seed)
+            desc = "homogeneous + %d conduits" % nc
+        out.append((d, K, desc))
+    return out
 
 
 def log(*a):
@@ -94,6 +117,81 @@ def cpu_baseline(K, dims, budget_s):
             "sample": "one full %dx%dx%d step of the oracle restatement, single thread; %.2f s" % (nx, ny, nz, t)}
 
 
+def run_batch(args, w, torch, S):
+    inst = c4_instances(w)
+    dims = [d for d, _, _ in inst]
+    K = np.concatenate([k for _, k, _ in inst])
+    ntot = K.size
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    Kd = dev(K)
+    big = max(max(a * b, b * c, a * c) for a, b, c in dims)
+    bval = [dev(np.full(big, 1.0))] + [dev(np.zeros(big)) for _ in range(5)]
+    p = torch.zeros(ntot, dtype=torch.float64, device="cuda")
+    v = torch.zeros(ntot * 7, dtype=torch.float64, device="cuda")
+    ctx = S.Context()
+    solver_stream = torch.cuda.ExternalStream(ctx.stream)
+    prec = S.PREC_FP64 if args.precision == "fp64" else S.PREC_FP32
+    opt = S.default_options(rtol=1e-8, precision=prec, check_every=2)
+
+    def step(o=opt):
+        with torch.cuda.stream(solver_stream):
+            p.zero_()
+        return ctx.reservoir_batch(dims, Kd, Kd, Kd, bval, p, v, o)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st, per = res[-1]
+    nb = w["batch"]
+    value = nb * args.steps / elapsed
+    done = np.sort(per["done_ms"])
+    its = per["iterations"]
+    log("batch of %d: %.3f s for %d steps, iterations %d..%d" % (nb, elapsed, args.steps, its.min(), its.max()))
+    fp64 = None
+    if not args.no_fp64 and prec == S.PREC_FP32:
+        o64 = S.default_options(rtol=1e-8, precision=S.PREC_FP64, check_every=2)
+        step(o64)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        k64 = max(2, args.steps // 2)
+        for _ in range(k64):
+            step(o64)
+        torch.cuda.synchronize()
+        fp64 = {"value": nb * k64 / (time.perf_counter() - ta)}
+        fp64["speedup_fp32_over_fp64"] = value / fp64["value"]
+    cpu = None
+    if not args.no_cpu:
+        # the reference build on the batch's first instances, up to the budget
+        tot, cnt, t_start = 0.0, 0, time.time()
+        for d, k, _ in inst:
+            r = cpu_baseline(k, d, args.cpu_budget)
+            tot += 1.0 / r["value"]
+            cnt += 1
+            if time.time() - t_start > args.cpu_budget:
+                break
+        cpu = {"value": cnt / tot, "unit": "solves/s", "cores": 1, "kind": r["kind"],
+               "sample": "the first %d instances of the batch, one after the other, single thread, %.1f s" % (cnt, tot)}
+    out = {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64" if prec == S.PREC_FP32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": w["label"], "instances": nb, "cells": int(ntot), "rtol": 1e-8},
+        "per_instance_done_ms": {"p50": float(np.percentile(done, 50)), "p99": float(np.percentile(done, 99)),
+                                 "max": float(done[-1])},
+        "iterations": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
+        "max_rel_residual": float(per["rel_residual"].max()),
+        "phase_ms": {"assemble": st.t_assemble_ms, "setup": st.t_setup_ms, "solve": st.t_solve_ms,
+                     "flux": st.t_flux_ms},
+        "fp64_hierarchy": fp64, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +227,8 @@ def main():
         print(json.dumps({"copy_bytes": nb, "launches": 5}))
         return
     w = WORKLOADS[args.workload]
+    if "batch" in w:
+        return run_batch(args, w, torch, S)
     nx, ny, nz = w["dims"]
     n = nx * ny * nz
     rtol = 1e-10 if args.workload == "c1" else 1e-8

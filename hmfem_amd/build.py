@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
-HIP_SOURCES = ["hmfem_assemble.hip", "hmfem_solver.hip", "hmfem_capi.hip"]
+HIP_SOURCES = ["hmfem_assemble.hip", "hmfem_solver.hip", "hmfem_batch.hip", "hmfem_capi.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # no packed FP32 instructions (see the build notes in DESIGN.md)

@@ -9,6 +9,8 @@
 // stencil (4 x 8 B) and b (8 B): HBM-bound, ~64 algorithmic bytes per cell.
 #include "hmfem_internal.h"
 
+#include <string.h>
+
 namespace hmfem {
 
 struct GridArgs {
@@ -462,6 +464,145 @@ int launch_divergence(hipStream_t st, const hmfem_grid *h, const ELE *v, double 
 int launch_prod_mat_vec(hipStream_t st, long n, const ELE *perm, const IND *indcol, const double *x, double *res)
 {
     hipLaunchKernelGGL(k_prod_mat_vec, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, n, perm, indcol, x, res);
+    HMFEM_CHECK(hipGetLastError());
+    return HMFEM_OK;
+}
+
+// ------------------------------------------------------------ batched variants
+//
+// A batch concatenates independent grids; instance q occupies cells
+// [off, off + n) of every array and blocks [blk0, blk0 + ceil(n / kBlock))
+// of a launch, and bmap[block] = q. Blocks never straddle two instances.
+
+struct BGrid {
+    GridArgs g;
+    int off, blk0, nblk;
+};
+
+__global__ void __launch_bounds__(kBlock)
+kb_assemble(const BGrid *__restrict__ G, const int *__restrict__ bmap, const double *__restrict__ Kx,
+            const double *__restrict__ Ky, const double *__restrict__ Kz, FaceVals B, double *e, double *wx,
+            double *wy, double *wz, double *b)
+{
+#pragma clang fp contract(off)
+    const BGrid D = G[bmap[blockIdx.x]];
+    const GridArgs &g = D.g;
+    long n = (long)g.nx * g.ny * g.nz;
+    long s = (long)(blockIdx.x - D.blk0) * kBlock + threadIdx.x;
+    if (s >= n) return;
+    int i, j, k;
+    cell_coords(g, s, i, j, k);
+    const double *kx = Kx + D.off, *ky = Ky + D.off, *kz = Kz + D.off;
+    Faces6 f = faces_from_K(g, kx, ky, kz, s, i, j, k);
+    double x = (f.il ? 0.0 : f.l) + (f.ir ? 0.0 : f.r) + (f.iu ? 0.0 : f.u) + (f.id ? 0.0 : f.d)
+             + (f.it ? 0.0 : f.t) + (f.ib ? 0.0 : f.b);
+    long o = D.off + s;
+    e[o] = x;
+    wx[o] = f.ir ? f.r : 0.0;
+    wy[o] = f.iu ? f.u : 0.0;
+    wz[o] = f.it ? f.t : 0.0;
+    b[o] = rhs_cell(g, f, B, 0.0, s, i, j, k, nullptr, nullptr, nullptr, nullptr);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_flux(const BGrid *__restrict__ G, const int *__restrict__ bmap, const double *__restrict__ Kx,
+        const double *__restrict__ Ky, const double *__restrict__ Kz, FaceVals B, const double *p, ELE *v,
+        double *partial)
+{
+    const BGrid D = G[bmap[blockIdx.x]];
+    const GridArgs &g = D.g;
+    long n = (long)g.nx * g.ny * g.nz;
+    long s = (long)(blockIdx.x - D.blk0) * kBlock + threadIdx.x;
+    double mx = 0.0, mn = 100.0;
+    if (s < n) {
+        int i, j, k;
+        cell_coords(g, s, i, j, k);
+        Faces6 f = faces_from_K(g, Kx + D.off, Ky + D.off, Kz + D.off, s, i, j, k);
+        double dv = fabs(flux_cell(g, f, B, p + D.off, s, i, j, k, v ? v + D.off : nullptr));
+        mx = dv; mn = dv;
+    }
+    minmax_block(mx, mn, partial);
+}
+
+// Per-instance max/min of the flux partials: one block per instance.
+__global__ void __launch_bounds__(kBlock)
+kb_minmax_final(const BGrid *__restrict__ G, const double *partial, double *out)
+{
+    const BGrid D = G[blockIdx.x];
+    double mx = 0.0, mn = 100.0;
+    for (int q = threadIdx.x; q < D.nblk; q += kBlock) {
+        mx = fmax(mx, partial[2 * (D.blk0 + q)]);
+        mn = fmin(mn, partial[2 * (D.blk0 + q) + 1]);
+    }
+    __shared__ double smx[kBlock / 64], smn[kBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        mn = fmin(mn, __shfl_xor(mn, o));
+    }
+    int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) { smx[w] = mx; smn[w] = mn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kBlock / 64; q++) { mx = fmax(mx, smx[q]); mn = fmin(mn, smn[q]); }
+        out[2 * blockIdx.x] = fabs(mn);
+        out[2 * blockIdx.x + 1] = fabs(mx);
+    }
+}
+
+// Batch layout of the fine grids: per-instance descriptors and block map on
+// the device (in `arena`), uniform boundary types and values.
+int batch_grids(hipStream_t st, int ninst, const int *dims, const double *lengths, double mu, const int bc[6],
+                std::vector<char> &stage, void *d_grids, int *d_bmap, int *nblk_out)
+{
+    std::vector<BGrid> G(ninst);
+    std::vector<int> bmap;
+    long off = 0;
+    for (int q = 0; q < ninst; q++) {
+        hmfem_grid h;
+        h.nx = dims[3 * q]; h.ny = dims[3 * q + 1]; h.nz = dims[3 * q + 2];
+        double Lx = lengths ? lengths[3 * q] : 1.0, Ly = lengths ? lengths[3 * q + 1] : 1.0;
+        double Lz = lengths ? lengths[3 * q + 2] : 1.0;
+        h.hx = Lx / h.nx; h.hy = Ly / h.ny; h.hz = Lz / h.nz;
+        h.viscosidade = mu;
+        for (int f = 0; f < 6; f++) h.bc_type[f] = bc[f];
+        G[q].g = make_grid(&h);
+        long n = (long)h.nx * h.ny * h.nz;
+        G[q].off = (int)off;
+        G[q].blk0 = (int)bmap.size();
+        G[q].nblk = (int)((n + kBlock - 1) / kBlock);
+        for (int b = 0; b < G[q].nblk; b++) bmap.push_back(q);
+        off += n;
+    }
+    *nblk_out = (int)bmap.size();
+    size_t gb = sizeof(BGrid) * ninst, mb = sizeof(int) * bmap.size();
+    stage.resize(gb + mb);
+    memcpy(stage.data(), G.data(), gb);
+    memcpy(stage.data() + gb, bmap.data(), mb);
+    HMFEM_CHECK(hipMemcpyAsync(d_grids, stage.data(), gb, hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipMemcpyAsync(d_bmap, stage.data() + gb, mb, hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipStreamSynchronize(st));
+    return HMFEM_OK;
+}
+
+size_t batch_grid_bytes(int ninst) { return sizeof(BGrid) * (size_t)ninst; }
+
+int launch_batch_assemble(hipStream_t st, const void *d_grids, const int *d_bmap, int nblk, const double *Kx,
+                          const double *Ky, const double *Kz, const double *const bval[6], double *e, double *wx,
+                          double *wy, double *wz, double *b)
+{
+    hipLaunchKernelGGL(kb_assemble, dim3(nblk), dim3(kBlock), 0, st, (const BGrid *)d_grids, d_bmap, Kx, Ky, Kz,
+                       make_faces(bval), e, wx, wy, wz, b);
+    HMFEM_CHECK(hipGetLastError());
+    return HMFEM_OK;
+}
+
+int launch_batch_flux(hipStream_t st, int ninst, const void *d_grids, const int *d_bmap, int nblk, const double *Kx,
+                      const double *Ky, const double *Kz, const double *const bval[6], const double *p, ELE *v,
+                      double *partial, double *out)
+{
+    hipLaunchKernelGGL(kb_flux, dim3(nblk), dim3(kBlock), 0, st, (const BGrid *)d_grids, d_bmap, Kx, Ky, Kz,
+                       make_faces(bval), p, v, partial);
+    hipLaunchKernelGGL(kb_minmax_final, dim3(ninst), dim3(kBlock), 0, st, (const BGrid *)d_grids, partial, out);
     HMFEM_CHECK(hipGetLastError());
     return HMFEM_OK;
 }

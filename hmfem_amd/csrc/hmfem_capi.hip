@@ -300,6 +300,84 @@ extern "C" int hmfem_solve_structured_device(hmfem_context *c, int nx, int ny, i
     return out->status == HMFEM_OK ? HMFEM_OK : out->status;
 }
 
+static char *ctx_alloc(void *c, size_t bytes) { return (char *)((hmfem_context *)c)->get("batch_arena", bytes); }
+
+extern "C" int hmfem_reservoir_batch_device(hmfem_context *c, int ninst, const int *dims, const double *lengths,
+                                            double viscosidade, const int bc_type[6], const double *const bval[6],
+                                            const double *Kx, const double *Ky, const double *Kz, double *p, ELE *v,
+                                            const hmfem_options *opt_in, hmfem_batch_status *st, int *iterations,
+                                            double *rel_residual, double *done_ms, double *div_max)
+{
+    if (!c || ninst < 1 || !dims || !bc_type || !bval || !Kx || !Ky || !Kz || !p) return HMFEM_ERR_ARG;
+    long total = 0;
+    for (int q = 0; q < ninst; q++) {
+        if (dims[3 * q] < 1 || dims[3 * q + 1] < 1 || dims[3 * q + 2] < 1) return HMFEM_ERR_ARG;
+        total += (long)dims[3 * q] * dims[3 * q + 1] * dims[3 * q + 2];
+    }
+    if (total >= 2147483647L || !(viscosidade > 0)) return HMFEM_ERR_ARG;
+    hmfem_options opt;
+    if (opt_in) opt = *opt_in;
+    else hmfem_default_options(&opt);
+    long nblk_max = total / kBlock + ninst + 1;
+    void *grids = c->get("batch_grids", batch_grid_bytes(ninst));
+    int *bmap = (int *)c->get("batch_bmap", sizeof(int) * nblk_max);
+    Stencil<double> A;
+    A.e = (double *)c->get("e", total * 8);
+    A.wx = (double *)c->get("wx", total * 8);
+    A.wy = (double *)c->get("wy", total * 8);
+    A.wz = (double *)c->get("wz", total * 8);
+    double *b = (double *)c->get("b", total * 8);
+    std::vector<CgScalars> hsc(ninst);
+    if (!grids || !bmap || !A.e || !A.wx || !A.wy || !A.wz || !b) return HMFEM_ERR_NOMEM;
+    int nblk = 0;
+    int rc = batch_grids(c->st, ninst, dims, lengths, viscosidade, bc_type, c->stage, grids, bmap, &nblk);
+    if (rc) return rc;
+    HMFEM_CHECK(hipEventRecord(c->ev[4], c->st));
+    rc = launch_batch_assemble(c->st, grids, bmap, nblk, Kx, Ky, Kz, bval, A.e, A.wx, A.wy, A.wz, b);
+    if (rc) return rc;
+    HMFEM_CHECK(hipEventRecord(c->ev[5], c->st));
+    std::vector<int> its(ninst), sts(ninst);
+    std::vector<double> rr(ninst), dm(ninst);
+    double setup = 0, solve = 0;
+    std::vector<char> stage2;
+    rc = batch_solve(c->st, ninst, dims, A, b, p, opt, ctx_alloc, c, hsc.data(), opt.use_graph ? &c->graph : nullptr,
+                     stage2, c->ev, its.data(), rr.data(), dm.data(), sts.data(), &setup, &solve);
+    if (rc) return rc;
+    float ta = 0.f;
+    (void)hipEventElapsedTime(&ta, c->ev[4], c->ev[5]);
+    hmfem_batch_status tmp{};
+    hmfem_batch_status *out = st ? st : &tmp;
+    out->t_assemble_ms = ta;
+    out->t_setup_ms = setup;
+    out->t_solve_ms = solve;
+    out->status = HMFEM_OK;
+    out->max_iterations = 0;
+    for (int q = 0; q < ninst; q++) {
+        if (sts[q] != HMFEM_OK && out->status == HMFEM_OK) out->status = sts[q];
+        out->max_iterations = its[q] > out->max_iterations ? its[q] : out->max_iterations;
+        if (iterations) iterations[q] = its[q];
+        if (rel_residual) rel_residual[q] = rr[q];
+        if (done_ms) done_ms[q] = dm[q];
+    }
+    if (v || div_max) {
+        double *part = (double *)c->get("mm_partial", (size_t)2 * nblk * 8 + 64);
+        double *mm = (double *)c->get("batch_mm", (size_t)2 * ninst * 8);
+        HMFEM_CHECK(hipEventRecord(c->ev[6], c->st));
+        rc = launch_batch_flux(c->st, ninst, grids, bmap, nblk, Kx, Ky, Kz, bval, p, v, part, mm);
+        if (rc) return rc;
+        HMFEM_CHECK(hipEventRecord(c->ev[7], c->st));
+        std::vector<double> h(2 * (size_t)ninst);
+        HMFEM_CHECK(hipMemcpyAsync(h.data(), mm, 16 * (size_t)ninst, hipMemcpyDeviceToHost, c->st));
+        HMFEM_CHECK(hipStreamSynchronize(c->st));
+        float tf = 0.f;
+        (void)hipEventElapsedTime(&tf, c->ev[6], c->ev[7]);
+        out->t_flux_ms = tf;
+        if (div_max)
+            for (int q = 0; q < ninst; q++) div_max[q] = h[2 * q + 1];
+    }
+    return out->status == HMFEM_OK ? HMFEM_OK : out->status;
+}
+
 extern "C" double hmfem_kernel_bytes_per_cell(int which, int precision)
 {
     double vs = precision == HMFEM_PREC_FP64 ? 8.0 : 4.0;

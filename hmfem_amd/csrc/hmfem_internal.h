@@ -152,4 +152,19 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink,
                      std::vector<char> &stage, GraphCache *graph, long lds_limit);
 
+// batched solve (hmfem_batch.hip) and batch layout / kernels (hmfem_assemble.hip)
+int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A0, const double *b, double *x,
+                const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *alloc_ctx, CgScalars *h_sc,
+                GraphCache *graph, std::vector<char> &stage, hipEvent_t ev[4], int *iters_out,
+                double *relres_out, double *done_ms_out, int *status_out, double *setup_ms, double *solve_ms);
+int batch_grids(hipStream_t st, int ninst, const int *dims, const double *lengths, double mu, const int bc[6],
+                std::vector<char> &stage, void *d_grids, int *d_bmap, int *nblk_out);
+size_t batch_grid_bytes(int ninst);
+int launch_batch_assemble(hipStream_t st, const void *d_grids, const int *d_bmap, int nblk, const double *Kx,
+                          const double *Ky, const double *Kz, const double *const bval[6], double *e, double *wx,
+                          double *wy, double *wz, double *b);
+int launch_batch_flux(hipStream_t st, int ninst, const void *d_grids, const int *d_bmap, int nblk, const double *Kx,
+                      const double *Ky, const double *Kz, const double *const bval[6], const double *p, ELE *v,
+                      double *partial, double *out);
+
 } // namespace hmfem

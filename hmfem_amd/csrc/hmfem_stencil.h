@@ -1,0 +1,121 @@
+// Device helpers shared by the single-instance and batched solvers: the
+// flux-form 7-point operator, deterministic block reductions, chunking.
+#pragma once
+
+#include "hmfem_internal.h"
+
+namespace hmfem {
+
+// ---------------------------------------------------------------- operator
+
+template <typename T>
+struct OpArgs {
+    const T *e, *wx, *wy, *wz, *di;
+    int nx, ny, nz;
+};
+
+struct Cell {
+    int s;
+    int i, j, k;
+};
+
+template <typename T>
+__device__ __forceinline__ Cell cell_at(const OpArgs<T> &o, int s)
+{
+    Cell c;
+    unsigned nxy = (unsigned)o.nx * (unsigned)o.ny;
+    c.s = s;
+    c.k = (int)((unsigned)s / nxy);
+    unsigned rem = (unsigned)s - (unsigned)c.k * nxy;
+    c.j = (int)(rem / (unsigned)o.nx);
+    c.i = (int)(rem - (unsigned)c.j * (unsigned)o.nx);
+    return c;
+}
+
+__device__ __forceinline__ Cell shifted(const Cell &c, int di, int dj, int dk, int nx, int nxy)
+{
+    Cell q;
+    q.i = c.i + di; q.j = c.j + dj; q.k = c.k + dk;
+    q.s = c.s + di + dj * nx + dk * nxy;
+    return q;
+}
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+
+// Diagonal of A at a cell: e + all six couplings.
+template <typename T, typename C>
+__device__ __forceinline__ C diag_at(const OpArgs<T> &o, const Cell &c)
+{
+    const int nx = o.nx, nxy = o.nx * o.ny;
+    C d = (C)o.e[c.s] + (C)o.wx[c.s] + (C)o.wy[c.s] + (C)o.wz[c.s];
+    if (c.i > 0) d += (C)o.wx[c.s - 1];
+    if (c.j > 0) d += (C)o.wy[c.s - nx];
+    if (c.k > 0) d += (C)o.wz[c.s - nxy];
+    return d;
+}
+
+// (A u)_s in flux form and the diagonal, for a functor u(Cell) giving the
+// value of u at a cell (so callers can build u on the fly).
+template <typename T, typename C, typename U>
+__device__ __forceinline__ C apply_at(const OpArgs<T> &o, const Cell &c, U u, C &diag)
+{
+    const int nx = o.nx, nxy = o.nx * o.ny;
+    const int s = c.s;
+    C us = u(c);
+    C w, d, a;
+    d = (C)o.e[s];
+    a = d * us;
+    if (c.i < o.nx - 1) { w = (C)o.wx[s]; d += w; a += w * (us - u(shifted(c, 1, 0, 0, nx, nxy))); }
+    if (c.i > 0) { w = (C)o.wx[s - 1]; d += w; a += w * (us - u(shifted(c, -1, 0, 0, nx, nxy))); }
+    if (c.j < o.ny - 1) { w = (C)o.wy[s]; d += w; a += w * (us - u(shifted(c, 0, 1, 0, nx, nxy))); }
+    if (c.j > 0) { w = (C)o.wy[s - nx]; d += w; a += w * (us - u(shifted(c, 0, -1, 0, nx, nxy))); }
+    if (c.k < o.nz - 1) { w = (C)o.wz[s]; d += w; a += w * (us - u(shifted(c, 0, 0, 1, nx, nxy))); }
+    if (c.k > 0) { w = (C)o.wz[s - nxy]; d += w; a += w * (us - u(shifted(c, 0, 0, -1, nx, nxy))); }
+    diag = d;
+    return a;
+}
+
+constexpr int kUnroll = 4;   // cells per thread in flight in chunked kernels
+
+// ---------------------------------------------------------------- reductions
+
+// Sum of NV values over the block in a fixed order; every thread gets the sums.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV])
+{
+    __shared__ double sh[kBlock / 64][NV];
+    for (int q = 0; q < NV; q++)
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0)
+        for (int q = 0; q < NV; q++) sh[w][q] = v[q];
+    __syncthreads();
+    for (int q = 0; q < NV; q++) {
+        double t = sh[0][q];
+        for (int x = 1; x < kBlock / 64; x++) t += sh[x][q];
+        v[q] = t;
+    }
+    __syncthreads();
+}
+
+// Sum of the per-block partials of a previous kernel (NV interleaved values
+// per block, G blocks), in a fixed order; every thread gets the sums.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *P, int G, double (&out)[NV])
+{
+    for (int q = 0; q < NV; q++) out[q] = 0.0;
+    for (int b = threadIdx.x; b < G; b += kBlock)
+        for (int q = 0; q < NV; q++) out[q] += P[(long)NV * b + q];
+    block_sum<NV>(out);
+}
+
+// Contiguous range of cells of this (logical) block in a chunked launch.
+__device__ __forceinline__ void chunk_range(int n, int chunk, int &lb, int &beg, int &end)
+{
+    lb = xcd_block();
+    beg = lb * chunk;
+    end = min(n, beg + chunk);
+}
+
+} // namespace hmfem

@@ -51,6 +51,10 @@ def conduits(K, nx, ny, nz, count, seed):
     return K
 
 
+def instance_seed(seed, instance):
+    return int(_lib().hmfem_field_instance_seed(seed, instance))
+
+
 class Stream:
     """Per-instance parameter draws (uniform in [0,1)) from the same PRNG."""
 

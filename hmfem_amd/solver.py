@@ -70,6 +70,14 @@ class Status(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BatchStatus(ctypes.Structure):
+    _fields_ = [("status", c_int), ("max_iterations", c_int), ("t_assemble_ms", c_dbl), ("t_setup_ms", c_dbl),
+                ("t_solve_ms", c_dbl), ("t_flux_ms", c_dbl)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _typed = False
 
 
@@ -106,6 +114,8 @@ def lib():
         L.hmfem_profile_read.argtypes = [c_vp, c_vp, c_vp, c_vp, c_int]
         L.hmfem_profile_reset.argtypes = [c_vp]
         L.hmfem_profile_reset.restype = None
+        L.hmfem_reservoir_batch_device.argtypes = [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                                   c_vp, c_vp, P(Options), P(BatchStatus), c_vp, c_vp, c_vp, c_vp]
         L.hmfem_kernel_bytes_per_cell.argtypes = [c_int, c_int]
         L.hmfem_kernel_bytes_per_cell.restype = c_dbl
         _typed = True
@@ -116,7 +126,8 @@ EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", 
             "prod_mat_vec", "hmfem_default_options", "hmfem_create", "hmfem_destroy", "hmfem_stream",
             "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device",
             "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
-            "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "iter_amg", "boundary")
+            "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device", "iter_amg",
+            "boundary")
 
 
 def _p(a):
@@ -298,6 +309,30 @@ class Context:
         if rc < 0:
             raise RuntimeError("hmfem_solve_structured_device failed: %d" % rc)
         return st
+
+    def reservoir_batch(self, dims, Kx, Ky, Kz, bval, p, v=None, options=None, viscosidade=1.0,
+                        bc_type=(0, 0, 1, 1, 1, 1), lengths=None):
+        """Solve len(dims) independent grids together. K*, p, v: concatenated
+        device arrays; bval: six device arrays at least as long as the largest
+        face. Returns (BatchStatus, per-instance dict of numpy arrays)."""
+        dims_a = np.ascontiguousarray(np.asarray(dims, np.int32).reshape(-1, 3))
+        ninst = dims_a.shape[0]
+        len_a = None if lengths is None else np.ascontiguousarray(np.asarray(lengths, np.float64).reshape(-1, 3))
+        bct = np.ascontiguousarray(np.asarray(bc_type, np.int32))
+        its = np.zeros(ninst, np.int32)
+        rr = np.zeros(ninst)
+        dm = np.zeros(ninst)
+        dv = np.zeros(ninst)
+        st = BatchStatus()
+        o = options if options is not None else default_options()
+        _order_after_torch(Kx, Ky, Kz, p, v, *bval)
+        rc = lib().hmfem_reservoir_batch_device(self.h, ninst, _p(dims_a), None if len_a is None else _p(len_a),
+                                                viscosidade, _p(bct), self._faces(bval), _dptr(Kx), _dptr(Ky),
+                                                _dptr(Kz), _dptr(p), _dptr(v), ctypes.byref(o), ctypes.byref(st),
+                                                _p(its), _p(rr), _p(dm), _p(dv))
+        if rc < 0:
+            raise RuntimeError("hmfem_reservoir_batch_device failed: %d" % rc)
+        return st, dict(iterations=its, rel_residual=rr, done_ms=dm, div_max=dv)
 
     def profile_enable(self, max_launches=20000):
         rc = lib().hmfem_profile_enable(self.h, max_launches)

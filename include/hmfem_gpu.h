@@ -168,6 +168,28 @@ int hmfem_solve_structured_device(hmfem_context *ctx, int nx, int ny, int nz, co
                                   const IND *indcol, const double *b, double *x,
                                   const hmfem_options *opt, hmfem_status *st);
 
+/* Batch of independent problems solved together (every launch covers all
+   of them): instance q is an nx*ny*nz grid given by dims[3q..3q+2] (host),
+   with extents lengths[3q..3q+2] (host; NULL = unit cube). All instances
+   share viscosidade, the face types and the face value arrays bval[f]
+   (device, each at least as long as the largest face of the batch).
+   Kx, Ky, Kz, p and v are the instances' arrays concatenated in order
+   (device). Per-instance results go to the optional host arrays
+   iterations, rel_residual, done_ms (time from the start of the solve
+   phase at which the instance's stop test first held, resolution
+   check_every iterations) and div_max. */
+typedef struct {
+    int status;              /* HMFEM_OK if every instance converged */
+    int max_iterations;      /* largest iteration count over the instances */
+    double t_assemble_ms, t_setup_ms, t_solve_ms, t_flux_ms;
+} hmfem_batch_status;
+
+int hmfem_reservoir_batch_device(hmfem_context *ctx, int ninst, const int *dims, const double *lengths,
+                                 double viscosidade, const int bc_type[6], const double *const bval[6],
+                                 const double *Kx, const double *Ky, const double *Kz, double *p, ELE *v,
+                                 const hmfem_options *opt, hmfem_batch_status *st, int *iterations,
+                                 double *rel_residual, double *done_ms, double *div_max);
+
 /* Per-kernel timing: with profiling enabled, every fine-level and CG launch
    of the following calls is bracketed by HIP events on the context's stream
    (capacity: max_launches launches per call). hmfem_profile_read returns, per

@@ -245,3 +245,35 @@ def test_full_size_c2_properties(ctx, torch):
     hx = 1.0 / 128
     assert abs((outflow - inflow) - hx * res_sum) <= 1e-9 * abs(inflow)
     assert abs(outflow - inflow) <= hx * np.sqrt(n) * r * np.linalg.norm(bh) * 1.01
+
+
+def test_batch_uneven_matches_single(ctx, torch):
+    """Uneven instances solved together give each instance's own solution:
+    against the single-instance solver and, for the small ones, the oracle."""
+    specs = [((20, 16, 12), "lognormal", 1), ((32, 24, 16), "karst", 2), ((8, 8, 8), "lognormal", 3),
+             ((40, 12, 20), "homogeneous", 4), ((5, 7, 3), "lognormal", 5), ((17, 9, 33), "karst", 6)]
+    probs = [make_problem(d, k, seed=s, corr=3.0, conduits=3) for d, k, s in specs]
+    _say("batch", [P.n for P in probs])
+    K = torch.from_numpy(np.concatenate([P.Kx for P in probs])).to("cuda")
+    big = max(max(P.nx * P.ny, P.ny * P.nz, P.nx * P.nz) for P in probs)
+    bval = [torch.full((big,), 1.0, dtype=torch.float64, device="cuda")] + \
+           [torch.zeros(big, dtype=torch.float64, device="cuda") for _ in range(5)]
+    ntot = sum(P.n for P in probs)
+    p = torch.zeros(ntot, dtype=torch.float64, device="cuda")
+    v = torch.zeros(ntot * 7, dtype=torch.float64, device="cuda")
+    o = S.default_options(rtol=1e-11)
+    st, per = ctx.reservoir_batch([(P.nx, P.ny, P.nz) for P in probs], K, K, K, bval, p, v, o)
+    assert st.status == S.OK, (st.as_dict(), per)
+    assert np.all(per["iterations"] > 0) and np.all(per["done_ms"] >= 0)
+    xb = p.cpu().numpy()
+    off = 0
+    for P, (d, k, s) in zip(probs, specs):
+        D = Dev(P, torch)
+        ps = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+        ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, ps, None, o)
+        x = xb[off:off + P.n]
+        assert rel_l2(x, ps.cpu().numpy()) < 1e-9, d
+        if P.n <= 6000:
+            xo, _, _, _ = oracle_solution(P, eps=1e-13)
+            assert rel_l2(x, xo) < PRESSURE_TOL, d
+        off += P.n
