@@ -199,6 +199,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--smoother", default=None, choices=["jacobi", "blockgs"],
+                    help="smoother of the multi-block levels (default: the library's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--profile-only", action="store_true", help="timed steps only (for rocprofv3 runs)")
@@ -245,6 +247,8 @@ def main():
     ctx = S.Context()
     prec = S.PREC_FP64 if args.precision == "fp64" else S.PREC_FP32
     opt = S.default_options(rtol=rtol, precision=prec)
+    if args.smoother:
+        opt.smoother = S.SMOOTH_JACOBI if args.smoother == "jacobi" else S.SMOOTH_BLOCK_GS
 
     solver_stream = torch.cuda.ExternalStream(ctx.stream)
 
@@ -294,6 +298,7 @@ def main():
     fp64 = None
     if not args.no_fp64 and prec == S.PREC_FP32:
         o64 = S.default_options(rtol=rtol, precision=S.PREC_FP64)
+        o64.smoother = opt.smoother
         step(o64)
         torch.cuda.synchronize()
         k64 = max(3, args.steps // 2)
@@ -324,7 +329,9 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64" if prec == S.PREC_FP32 else "f64",
         "data": "synthetic",
         "config": {"workload": w["label"], "cells": n, "precision": "FP32 AMG in FP64 flexible CG"
-                   if prec == S.PREC_FP32 else "FP64 AMG in FP64 flexible CG", "rtol": rtol},
+                   if prec == S.PREC_FP32 else "FP64 AMG in FP64 flexible CG", "rtol": rtol,
+                   "smoother": "block red-black Gauss-Seidel" if opt.smoother == S.SMOOTH_BLOCK_GS
+                   else "damped Jacobi"},
         "iterations": st.iterations, "levels": st.levels,
         "phase_ms": {"assemble": st.t_assemble_ms, "setup": st.t_setup_ms, "solve": st.t_solve_ms,
                      "flux": st.t_flux_ms},

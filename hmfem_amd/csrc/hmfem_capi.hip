@@ -71,13 +71,15 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->stop_rule = HMFEM_STOP_RESIDUAL;
     o->nu = 3;
     o->omega = 0.95;
-    o->ocf = 2.0;
+    o->ocf = 2.5;
     o->coarse_max = kCoarseMax;
     o->check_every = 4;
     o->tail_max = 512;
     o->tail_gamma = 2;
     o->use_graph = 1;
     o->tail_lds = 1;
+    o->smoother = HMFEM_SMOOTH_BLOCK_GS;
+    o->gs_sweeps = 3;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)

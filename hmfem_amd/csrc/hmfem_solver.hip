@@ -124,6 +124,44 @@ k_resid_restrict(OpArgs<T> o, int fx, int fy, int fz, int cx, int cy, int cz, co
     fc[I] = acc;
 }
 
+// Residual r = f - A u summed over 2x2x2 aggregates, one thread per fine
+// cell: a block covers a brick of 32 x 4 x 2 fine cells (16 x 2 x 1
+// aggregates). Lanes i and i^1 hold an x pair, lanes l and l^32 a y pair
+// (rows j and j+1 are the two halves of a wavefront), and the two z planes
+// are the two halves of the block, combined through LDS.
+template <typename T, typename FT>
+__global__ void __launch_bounds__(kBlock)
+k_restrict_brick(OpArgs<T> o, int cx, int cy, int cz, const FT *__restrict__ f, const T *__restrict__ u,
+                 T *__restrict__ fc, const int *done)
+{
+    using C = T;
+    if (*done) return;
+    __shared__ C upper[2][16];
+    const int nbx = (o.nx + 31) / 32, nby = (o.ny + 3) / 4;
+    const int b = xcd_block();
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int t = threadIdx.x;
+    const int i = bx * 32 + (t & 31), j = by * 4 + ((t >> 5) & 3), k = bz * 2 + (t >> 7);
+    C r = 0;
+    if (i < o.nx && j < o.ny && k < o.nz) {
+        Cell c;
+        c.i = i; c.j = j; c.k = k;
+        c.s = i + j * o.nx + k * o.nx * o.ny;
+        C d;
+        C au = apply_at<T, C>(o, c, [&](const Cell &q) -> C { return (C)u[q.s]; }, d);
+        r = (C)f[c.s] - au;
+    }
+    r += __shfl_xor(r, 1);
+    r += __shfl_xor(r, 32);
+    const int lane = t & 63, cjl = (t >> 6) & 1, cil = (t & 31) >> 1;
+    if ((t >> 7) == 1 && lane < 32 && !(lane & 1)) upper[cjl][cil] = r;
+    __syncthreads();
+    if ((t >> 7) == 0 && lane < 32 && !(lane & 1)) {
+        const int ci = bx * 16 + cil, cj = by * 2 + cjl, ck = bz;
+        if (ci < cx && cj < cy && ck < cz) fc[ci + cx * (cj + cy * ck)] = r + upper[cjl][cil];
+    }
+}
+
 // Coarse correction u' = u + ocf * u_c[agg] followed by one Jacobi sweep on u'.
 template <typename T, typename FT>
 __global__ void __launch_bounds__(kBlock)
@@ -193,6 +231,171 @@ k_final_sweep_dots(OpArgs<T> o, int fx, int fy, int cx, int cy, int fz, const T 
     }
     block_sum<2>(acc);
     if (threadIdx.x == 0) { Pdot[2 * lb] = acc[0]; Pdot[2 * lb + 1] = acc[1]; }
+}
+
+// ---------------------------------------------------------------- block Gauss-Seidel smoother
+//
+// The grid is cut into tiles of kGsTx x kGsTy x kGsTz cells, one workgroup
+// per tile and one z-column of kGsTz cells per thread. Inside a tile the
+// smoother runs red-black Gauss-Seidel sweeps (colour (i+j+k) mod 2) while
+// the values just outside the tile stay at what they were when the kernel
+// started: a block-Jacobi smoother whose block solves are approximated by
+// red-black sweeps. All sweeps of one smoothing step cost one pass over
+// the level (coefficients in registers, the tile and its x/y ring in LDS, the
+// column's z neighbours in registers), where damped Jacobi needs one pass
+// per sweep. The post-smoother runs the colours in the opposite order.
+//
+// GS update of a cell: u_s = (f_s + sum_nb w u_nb) / d_s with d_s = e_s + sum w
+// (the inverse diagonal di is precomputed), i.e. (A u)_s = f_s afterwards.
+constexpr int kGsTx = 32, kGsTy = 8, kGsTz = 8;
+static_assert(kGsTx * kGsTy == kBlock, "one thread per tile column");
+
+// The x/y ring of one tile plane: rows hy = 0 and hy = kGsTy + 1 (kGsTx + 2
+// cells each) and columns hx = 0 and hx = kGsTx + 1 of rows 1..kGsTy.
+constexpr int kGsRing = 2 * (kGsTx + 2) + 2 * kGsTy;
+constexpr int kGsRingPer = (kGsTz * kGsRing + kBlock - 1) / kBlock;   // ring cells per thread
+
+// LDS coordinates of ring cell q (0 <= q < kGsTz * kGsRing).
+__device__ __forceinline__ void gs_ring_cell(int q, int &hx, int &hy, int &hz)
+{
+    hz = q / kGsRing;
+    const int r = q - hz * kGsRing;
+    if (r < kGsTx + 2) { hx = r; hy = 0; }
+    else if (r < 2 * (kGsTx + 2)) { hx = r - (kGsTx + 2); hy = kGsTy + 1; }
+    else { const int r2 = r - 2 * (kGsTx + 2); hy = 1 + (r2 >> 1); hx = (r2 & 1) ? kGsTx + 1 : 0; }
+}
+
+inline int gs_tiles(const Dims &d)
+{
+    return ((d.nx + kGsTx - 1) / kGsTx) * ((d.ny + kGsTy - 1) / kGsTy) * ((d.nz + kGsTz - 1) / kGsTz);
+}
+
+// MODE 0: pre-smoothing from u = 0 (u and uc unused), result in uo.
+// MODE 1: post-smoothing from u + ocf * uc[aggregate] (computed on the fly,
+//         also for the frozen ring), result in uo (uo must not alias u).
+// MODE 2: MODE 1 on the fine level plus the CG dot partials <z, f>, <z, v>
+//         of this block's tiles (Pdot[2 lb], Pdot[2 lb + 1]).
+// sx, sy, sz: log2 of the aggregation factors (0 or 1) towards uc's level.
+// Each (logical) block smooths tiles [lb * tpb, (lb + 1) * tpb).
+template <typename T, typename FT, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict__ u, T *__restrict__ uo, int sx,
+          int sy, int sz, int cx, int cy, const T *__restrict__ uc, T ocf, const double *__restrict__ v, int tpb,
+          double *Pdot, const int *done)
+{
+    using C = T;
+    if (*done) return;
+    __shared__ C sh[kGsTz][kGsTy + 2][kGsTx + 2];
+    const int nx = o.nx, ny = o.ny, nz = o.nz;
+    const long nxy = (long)nx * ny;
+    const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy, ntz = (nz + kGsTz - 1) / kGsTz;
+    const int ntiles = ntx * nty * ntz;
+    const int lb = xcd_block();
+    const int tbeg = lb * tpb, tend = min(ntiles, tbeg + tpb);
+    const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+    double acc[2] = {0.0, 0.0};
+
+    // starting value of a cell (zero outside the grid)
+    auto start = [&](int ii, int jj, int kk) -> C {
+        if (MODE == 0 || ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) return (C)0;
+        long s = ii + (long)jj * nx + (long)kk * nxy;
+        int I = (ii >> sx) + cx * ((jj >> sy) + cy * (kk >> sz));
+        return (C)u[s] + ocf * (C)uc[I];
+    };
+
+    for (int t = tbeg; t < tend; t++) {
+        const int ti = t % ntx, tj = (t / ntx) % nty, tk = t / (ntx * nty);
+        const int i0 = ti * kGsTx, j0 = tj * kGsTy, k0 = tk * kGsTz;
+        const int i = i0 + tx, j = j0 + ty;
+        const bool col = i < nx && j < ny;
+        const long s0 = i + (long)j * nx + (long)k0 * nxy;
+
+        // this column's coefficients (zero for cells outside the grid)
+        C wxp[kGsTz], wxm[kGsTz], wyp[kGsTz], wym[kGsTz], wzp[kGsTz], dinv[kGsTz], ff[kGsTz], uu[kGsTz];
+        double fd[MODE == 2 ? kGsTz : 1];
+#pragma unroll
+        for (int kk = 0; kk < kGsTz; kk++) {
+            const bool act = col && k0 + kk < nz;
+            const long s = s0 + kk * nxy;
+            wxp[kk] = act ? (C)o.wx[s] : (C)0;
+            wxm[kk] = act && i > 0 ? (C)o.wx[s - 1] : (C)0;
+            wyp[kk] = act ? (C)o.wy[s] : (C)0;
+            wym[kk] = act && j > 0 ? (C)o.wy[s - nx] : (C)0;
+            wzp[kk] = act ? (C)o.wz[s] : (C)0;
+            dinv[kk] = act ? (C)o.di[s] : (C)0;
+            double fv = act ? (double)f[s] : 0.0;
+            ff[kk] = (C)fv;
+            if constexpr (MODE == 2) fd[kk] = fv;
+        }
+        const C wzlo = col && k0 > 0 ? (C)o.wz[s0 - nxy] : (C)0;
+
+        // starting values of the column, its two z neighbours and kGsRingPer
+        // cells of the x/y ring, all loads issued before the first use
+        C ring[kGsRingPer];
+#pragma unroll
+        for (int kk = 0; kk < kGsTz; kk++) uu[kk] = start(i, j, k0 + kk);
+        const C zlo = start(i, j, k0 - 1);
+        const C zhi = start(i, j, k0 + kGsTz);
+#pragma unroll
+        for (int m = 0; m < kGsRingPer; m++) {
+            const int q = threadIdx.x + m * kBlock;
+            int hx, hy, hz;
+            gs_ring_cell(q, hx, hy, hz);
+            ring[m] = q < kGsTz * kGsRing ? start(i0 + hx - 1, j0 + hy - 1, k0 + hz) : (C)0;
+        }
+#pragma unroll
+        for (int kk = 0; kk < kGsTz; kk++) sh[kk][ty + 1][tx + 1] = uu[kk];
+#pragma unroll
+        for (int m = 0; m < kGsRingPer; m++) {
+            const int q = threadIdx.x + m * kBlock;
+            int hx, hy, hz;
+            gs_ring_cell(q, hx, hy, hz);
+            if (q < kGsTz * kGsRing) sh[hz][hy][hx] = ring[m];
+        }
+        __syncthreads();
+
+        for (int sw = 0; sw < sweeps; sw++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int c = MODE == 0 ? h : 1 - h;       // colour of this half sweep
+                const int sel = (c + i + j + k0) & 1;      // cell 2m + sel of each pair has colour c
+#pragma unroll
+                for (int m = 0; m < kGsTz / 2; m++) {
+                    const int ka = 2 * m, kb = 2 * m + 1;
+                    const int kk = ka + sel;
+                    const C uzm = sel ? uu[ka] : (m > 0 ? uu[ka - 1] : zlo);
+                    const C uzp = sel ? (kb + 1 < kGsTz ? uu[kb + 1] : zhi) : uu[kb];
+                    const C wzm_ = sel ? wzp[ka] : (m > 0 ? wzp[ka - 1] : wzlo);
+                    const C wzp_ = sel ? wzp[kb] : wzp[ka];
+                    const C a = (sel ? ff[kb] : ff[ka]) + (sel ? wxp[kb] : wxp[ka]) * sh[kk][ty + 1][tx + 2] +
+                                (sel ? wxm[kb] : wxm[ka]) * sh[kk][ty + 1][tx] +
+                                (sel ? wyp[kb] : wyp[ka]) * sh[kk][ty + 2][tx + 1] +
+                                (sel ? wym[kb] : wym[ka]) * sh[kk][ty][tx + 1] + wzp_ * uzp + wzm_ * uzm;
+                    const C un = a * (sel ? dinv[kb] : dinv[ka]);
+                    uu[ka] = sel ? uu[ka] : un;
+                    uu[kb] = sel ? un : uu[kb];
+                    sh[kk][ty + 1][tx + 1] = un;
+                }
+                __syncthreads();
+            }
+        }
+
+#pragma unroll
+        for (int kk = 0; kk < kGsTz; kk++) {
+            if (col && k0 + kk < nz) {
+                const long s = s0 + kk * nxy;
+                uo[s] = uu[kk];
+                if constexpr (MODE == 2) {
+                    acc[0] += (double)uu[kk] * fd[kk];
+                    if (v) acc[1] += (double)uu[kk] * v[s];
+                }
+            }
+        }
+    }
+    if (MODE == 2) {
+        block_sum<2>(acc);
+        if (threadIdx.x == 0) { Pdot[2 * lb] = acc[0]; Pdot[2 * lb + 1] = acc[1]; }
+    }
 }
 
 // u = A^-1 f for the dense coarsest level: groups of L lanes (a power of two
@@ -421,6 +624,173 @@ k_tail(const TailLevel<T> *__restrict__ Lg, const int4 *__restrict__ ops, int no
         const TailLevel<T> G0 = Lg[0];
         const T *src = Lv[0].u;   // OP_FINAL left the result in u
         for (int s = tid; s < G0.n; s += kTailThreads) G0.u[s] = src[s];
+    }
+}
+
+// The same op list with every level resident in LDS: levels are addressed by
+// element offsets from the LDS arena (so every access is a direct LDS access),
+// and the op list is copied into LDS first. Layout of level l at
+// A + base_l: e, wx, wy, wz, di, f, u, t, r, w (n_l values each).
+constexpr int kTailLdsOps = 256;
+
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads)
+k_tail_lds(const TailLevel<T> *__restrict__ Lg, const int4 *__restrict__ opsg, int nops,
+           const double *__restrict__ invg, int nc, T omega, T ocf, int nlev, int inv_off, const int *done)
+{
+    using C = T;
+    extern __shared__ char lds[];
+    __shared__ int dsc[kMaxTailLevels][8];   // base, n, nx, ny, nz, fx, fy, fz
+    __shared__ int4 ops[kTailLdsOps];
+    __shared__ double fs[kCoarseMax];
+    if (*done) return;
+    T *A = (T *)lds;
+    double *inv = (double *)(lds + inv_off);
+    const int tid = threadIdx.x;
+    if (tid < nlev) {
+        const TailLevel<T> &L = Lg[tid];
+        dsc[tid][0] = L.lds_off / (int)sizeof(T); dsc[tid][1] = L.n;
+        dsc[tid][2] = L.o.nx; dsc[tid][3] = L.o.ny; dsc[tid][4] = L.o.nz;
+        dsc[tid][5] = L.fx; dsc[tid][6] = L.fy; dsc[tid][7] = L.fz;
+    }
+    for (int q = tid; q < nops; q += kTailThreads) ops[q] = opsg[q];
+    for (int q = tid; q < nc * nc; q += kTailThreads) inv[q] = invg[q];
+    __syncthreads();
+    for (int l = 0; l < nlev; l++) {
+        const TailLevel<T> &L = Lg[l];
+        T *b = A + dsc[l][0];
+        const int n = dsc[l][1];
+        for (int s = tid; s < n; s += kTailThreads) {
+            b[s] = L.o.e[s]; b[n + s] = L.o.wx[s]; b[2 * n + s] = L.o.wy[s];
+            b[3 * n + s] = L.o.wz[s]; b[4 * n + s] = L.o.di[s];
+            if (l == 0) b[5 * n + s] = L.f[s];
+        }
+    }
+    __syncthreads();
+
+    auto level = [&](int l, OpArgs<T> &o, T *&f, T *&u, T *&t, T *&r, T *&w, int &n) {
+        const int base = dsc[l][0];
+        n = dsc[l][1];
+        o.e = A + base; o.wx = A + base + n; o.wy = A + base + 2 * n; o.wz = A + base + 3 * n;
+        o.di = A + base + 4 * n;
+        o.nx = dsc[l][2]; o.ny = dsc[l][3]; o.nz = dsc[l][4];
+        f = A + base + 5 * n; u = A + base + 6 * n; t = A + base + 7 * n; r = A + base + 8 * n;
+        w = A + base + 9 * n;
+    };
+    unsigned parity = 0;
+    for (int q = 0; q < nops; q++) {
+        const int4 op = ops[q];
+        const int l = op.y;
+        OpArgs<T> o;
+        T *Lf, *Lu, *Lt, *Lr, *Lw;
+        int n;
+        level(l, o, Lf, Lu, Lt, Lr, Lw, n);
+        const bool pl = (parity >> l) & 1u;
+        T *cur = pl ? Lt : Lu;
+        T *oth = pl ? Lu : Lt;
+        const T *f = op.z ? Lr : Lf;
+        switch (op.x) {
+        case OP_SMOOTH0:
+            for (int s = tid; s < n; s += kTailThreads) cur[s] = omega * (C)f[s] * (C)o.di[s];
+            break;
+        case OP_JACOBI:
+            for (int s = tid; s < n; s += kTailThreads) {
+                Cell c = cell_at(o, s);
+                C d;
+                C au = apply_at<T, C>(o, c, [&](const Cell &x) -> C { return (C)cur[x.s]; }, d);
+                oth[s] = (C)cur[s] + omega * ((C)f[s] - au) * rcp(d);
+            }
+            parity ^= 1u << l;
+            break;
+        case OP_RESID:
+            for (int s = tid; s < n; s += kTailThreads) {
+                Cell c = cell_at(o, s);
+                C d;
+                C au = apply_at<T, C>(o, c, [&](const Cell &x) -> C { return (C)cur[x.s]; }, d);
+                Lr[s] = (C)Lf[s] - au;
+            }
+            break;
+        case OP_SAVE:
+            for (int s = tid; s < n; s += kTailThreads) Lw[s] = cur[s];
+            break;
+        case OP_ADDSAVED:
+            for (int s = tid; s < n; s += kTailThreads) cur[s] += Lw[s];
+            break;
+        case OP_FINAL:
+            if (pl) {
+                for (int s = tid; s < n; s += kTailThreads) Lu[s] = Lt[s];
+                parity &= ~(1u << l);
+            }
+            break;
+        case OP_RESTRICT: {
+            OpArgs<T> oc;
+            T *Cf, *Cu, *Ct, *Cr, *Cw;
+            int ncl;
+            level(l + 1, oc, Cf, Cu, Ct, Cr, Cw, ncl);
+            T *dst = op.w ? Cr : Cf;
+            const int cx = oc.nx, cy = oc.ny;
+            const int fx = dsc[l][5], fy = dsc[l][6], fz = dsc[l][7];
+            for (int I = tid; I < ncl; I += kTailThreads) {
+                int ck = I / (cx * cy);
+                int rem = I - ck * cx * cy;
+                int cj = rem / cx, ci = rem - cj * cx;
+                C acc = 0;
+                for (int dk = 0; dk < fz; dk++) {
+                    int k = ck * fz + dk;
+                    if (k >= o.nz) break;
+                    for (int dj = 0; dj < fy; dj++) {
+                        int j = cj * fy + dj;
+                        if (j >= o.ny) break;
+                        for (int di = 0; di < fx; di++) {
+                            int i = ci * fx + di;
+                            if (i >= o.nx) break;
+                            Cell c;
+                            c.i = i; c.j = j; c.k = k;
+                            c.s = i + j * o.nx + k * o.nx * o.ny;
+                            C d;
+                            C au = apply_at<T, C>(o, c, [&](const Cell &x) -> C { return (C)cur[x.s]; }, d);
+                            acc += (C)f[c.s] - au;
+                        }
+                    }
+                }
+                dst[I] = acc;
+            }
+            break;
+        }
+        case OP_PROLONG_JACOBI: {
+            OpArgs<T> oc;
+            T *Cf, *Cu, *Ct, *Cr, *Cw;
+            int ncl;
+            level(l + 1, oc, Cf, Cu, Ct, Cr, Cw, ncl);
+            const T *uc = ((parity >> (l + 1)) & 1u) ? Ct : Cu;
+            const int cx = oc.nx, cy = oc.ny;
+            const int fx = dsc[l][5], fy = dsc[l][6], fz = dsc[l][7];
+            for (int s = tid; s < n; s += kTailThreads) {
+                Cell c = cell_at(o, s);
+                auto up = [&](const Cell &x) -> C {
+                    return (C)cur[x.s] + ocf * (C)uc[x.i / fx + cx * (x.j / fy + cy * (x.k / fz))];
+                };
+                C d;
+                C au = apply_at<T, C>(o, c, up, d);
+                oth[s] = up(c) + omega * ((C)f[s] - au) * rcp(d);
+            }
+            parity ^= 1u << l;
+            break;
+        }
+        case OP_DENSE:
+            for (int s = tid; s < nc; s += kTailThreads) fs[s] = (double)f[s];
+            __syncthreads();
+            dense_gemv<T>(inv, fs, cur, nc, tid, kTailThreads);
+            break;
+        default:
+            break;
+        }
+        __syncthreads();
+    }
+    {
+        const TailLevel<T> &G0 = Lg[0];
+        const T *src = A + dsc[0][0] + 6 * dsc[0][1];   // OP_FINAL left the result in u
+        for (int s = tid; s < dsc[0][1]; s += kTailThreads) G0.u[s] = src[s];
     }
 }
 
@@ -911,7 +1281,10 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const 
 struct CycleParams {
     int nu;
     double omega, ocf;
+    int smoother, gs;    // smoother of the multi-block levels and its sweeps
 };
+
+static int lg2f(int f) { return f == 2 ? 1 : 0; }
 
 // Optional per-kernel timing: an event pair around each profiled launch.
 struct Prof {
@@ -939,6 +1312,23 @@ struct Prof {
         hipLaunchKernelGGL(__VA_ARGS__);  \
         if (P) (P)->end();                \
     } while (0)
+
+// Residual + restriction from level L to C: the brick kernel for 2x2x2
+// aggregation, the general one otherwise.
+template <typename T, typename FT>
+static void launch_restrict(hipStream_t st, Prof *P, int cls, const Level &L, const Level &C, const FT *f,
+                            const T *u, const int *done)
+{
+    OpArgs<T> o = opargs<T>(L);
+    if (L.fx == 2 && L.fy == 2 && L.fz == 2) {
+        const int nb = ((L.d.nx + 31) / 32) * ((L.d.ny + 3) / 4) * ((L.d.nz + 1) / 2);
+        PROF_LAUNCH(P, cls, (k_restrict_brick<T, FT>), dim3(grid_for(nb, 1)), dim3(kBlock), 0, st, o, C.d.nx,
+                    C.d.ny, C.d.nz, f, u, (T *)C.f, done);
+    } else {
+        PROF_LAUNCH(P, cls, (k_resid_restrict<T, FT, 0>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock), 0, st, o,
+                    L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, f, u, (T *)C.f, done);
+    }
+}
 
 struct Chunking {
     int G, chunk;
@@ -979,6 +1369,12 @@ static T *vcycle_coarse(hipStream_t st, Hierarchy &H, int l, const CycleParams &
                     done);
         return u;
     }
+    if (H.tail0 > 0 && l == H.tail0 && H.tail_lds_inv >= 0 && H.tail_nops <= kTailLdsOps) {
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, k_tail_lds<T>, dim3(1), dim3(kTailThreads), H.tail_lds_bytes, st,
+                    (const TailLevel<T> *)H.tail_levels, H.tail_ops, H.tail_nops, H.inv, H.nc, (T)cp.omega,
+                    (T)cp.ocf, H.nlev - H.tail0, H.tail_lds_inv, done);
+        return u;
+    }
     if (H.tail0 > 0 && l == H.tail0) {
         PROF_LAUNCH(P, HMFEM_PROF_COARSE, k_tail<T>, dim3(1), dim3(kTailThreads), H.tail_lds_bytes, st,
                     (const TailLevel<T> *)H.tail_levels, H.tail_ops, H.tail_nops, H.inv, H.nc, (T)cp.omega,
@@ -988,19 +1384,27 @@ static T *vcycle_coarse(hipStream_t st, Hierarchy &H, int l, const CycleParams &
     OpArgs<T> o = opargs<T>(L);
     const T om = (T)cp.omega;
     int G = grid_for(L.d.n(), kBlock);
+    Level &C = H.lev[l + 1];
+    if (cp.smoother == HMFEM_SMOOTH_BLOCK_GS) {
+        const int GT = grid_for(gs_tiles(L.d), 1);
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 0>), dim3(GT), dim3(kBlock), 0, st, o, cp.gs, f,
+                    (const T *)nullptr, u, 0, 0, 0, 0, 0, (const T *)nullptr, (T)0, (const double *)nullptr, 1,
+                    (double *)nullptr, done);
+        launch_restrict<T, T>(st, P, HMFEM_PROF_COARSE, L, C, f, (const T *)u, done);
+        T *uc = vcycle_coarse<T>(st, H, l + 1, cp, P, done);
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 1>), dim3(GT), dim3(kBlock), 0, st, o, cp.gs, f,
+                    (const T *)u, t, lg2f(L.fx), lg2f(L.fy), lg2f(L.fz), C.d.nx, C.d.ny, (const T *)uc, (T)cp.ocf,
+                    (const double *)nullptr, 1, (double *)nullptr, done);
+        L.u = t; L.t = u;
+        return t;
+    }
     PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_smooth_zero<T, T>), dim3(G), dim3(kBlock), 0, st, o, f, u, om,
                 cp.nu >= 2 ? 2 : 1, done);
     for (int q = 2; q < cp.nu; q++) {
         PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_jacobi<T, T>), dim3(G), dim3(kBlock), 0, st, o, f, u, t, om, done);
         T *tmp = u; u = t; t = tmp;
     }
-    Level &C = H.lev[l + 1];
-    if (L.fx == 2 && L.fy == 2 && L.fz == 2)
-        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_resid_restrict<T, T, 2>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock),
-                    0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, f, u, (T *)C.f, done);
-    else
-        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_resid_restrict<T, T, 0>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock),
-                    0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, f, u, (T *)C.f, done);
+    launch_restrict<T, T>(st, P, HMFEM_PROF_COARSE, L, C, f, (const T *)u, done);
     T *uc = vcycle_coarse<T>(st, H, l + 1, cp, P, done);
     PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_prolong_jacobi<T, T>), dim3(G), dim3(kBlock), 0, st, o, L.fx, L.fy, L.fz,
                 C.d.nx, C.d.ny, uc, (T)cp.ocf, f, u, t, om, done);
@@ -1029,6 +1433,21 @@ static T *vcycle_fine(hipStream_t st, Hierarchy &H, const double *r, const doubl
     OpArgs<T> o = opargs<T>(L);
     const T om = (T)cp.omega;
     int G = grid_for(L.d.n(), kBlock);
+    if (cp.smoother == HMFEM_SMOOTH_BLOCK_GS) {
+        Level &C = H.lev[1];
+        const int nt = gs_tiles(L.d);
+        PROF_LAUNCH(P, HMFEM_PROF_SMOOTH0, (k_gs_tile<T, double, 0>), dim3(grid_for(nt, 1)), dim3(kBlock), 0, st, o,
+                    cp.gs, r, (const T *)nullptr, u, 0, 0, 0, 0, 0, (const T *)nullptr, (T)0,
+                    (const double *)nullptr, 1, (double *)nullptr, done);
+        launch_restrict<T, double>(st, P, HMFEM_PROF_RESTRICT, L, C, r, (const T *)u, done);
+        T *uc = vcycle_coarse<T>(st, H, 1, cp, P, done);
+        // the dot partials must come from exactly S.ch.G blocks
+        const int tpb = (nt + S.ch.G - 1) / S.ch.G;
+        PROF_LAUNCH(P, HMFEM_PROF_PROLONG, (k_gs_tile<T, double, 2>), dim3(S.ch.G), dim3(kBlock), 0, st, o, cp.gs,
+                    r, (const T *)u, t, lg2f(L.fx), lg2f(L.fy), lg2f(L.fz), C.d.nx, C.d.ny, (const T *)uc,
+                    (T)cp.ocf, v, tpb, S.Pdot, done);
+        return t;
+    }
     PROF_LAUNCH(P, HMFEM_PROF_SMOOTH0, (k_smooth_zero<T, double>), dim3(G), dim3(kBlock), 0, st, o, r, u, om,
                 cp.nu >= 2 ? 2 : 1, done);
     for (int q = 2; q < cp.nu; q++) {
@@ -1036,12 +1455,7 @@ static T *vcycle_fine(hipStream_t st, Hierarchy &H, const double *r, const doubl
         T *tmp = u; u = t; t = tmp;
     }
     Level &C = H.lev[1];
-    if (L.fx == 2 && L.fy == 2 && L.fz == 2)
-        PROF_LAUNCH(P, HMFEM_PROF_RESTRICT, (k_resid_restrict<T, double, 2>), dim3(grid_for(C.d.n(), kBlock)),
-                    dim3(kBlock), 0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, r, u, (T *)C.f, done);
-    else
-        PROF_LAUNCH(P, HMFEM_PROF_RESTRICT, (k_resid_restrict<T, double, 0>), dim3(grid_for(C.d.n(), kBlock)),
-                    dim3(kBlock), 0, st, o, L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, r, u, (T *)C.f, done);
+    launch_restrict<T, double>(st, P, HMFEM_PROF_RESTRICT, L, C, r, (const T *)u, done);
     T *uc = vcycle_coarse<T>(st, H, 1, cp, P, done);
     if (cp.nu == 1) {
         PROF_LAUNCH(P, HMFEM_PROF_PROLONG, k_final_sweep_dots<T>, dim3(S.ch.G), dim3(kBlock), 0, st, o, L.fx, L.fy,
@@ -1074,7 +1488,7 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     OpArgs<double> o;
     o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
     o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
-    CycleParams cp{opt.nu < 1 ? 1 : opt.nu, opt.omega, opt.ocf};
+    CycleParams cp{opt.nu < 1 ? 1 : opt.nu, opt.omega, opt.ocf, opt.smoother, opt.gs_sweeps < 1 ? 1 : opt.gs_sweeps};
     CgState S;
     S.B = B;
     S.ch = chunking(n);
@@ -1172,11 +1586,21 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
     int info = 0;
     HMFEM_CHECK(hipMemcpy(&info, d_info, sizeof(int), hipMemcpyDeviceToHost));
     if (P) {
+        // Launches queued after the solve had converged (the host checks every
+        // check_every iterations) return at once; they are left out: a launch
+        // counts if it ran for at least a quarter of its class's longest one.
+        std::vector<float> dur(P->used / 2, 0.f);
+        float longest[HMFEM_PROF_TYPES] = {};
         for (int q = 0; q + 1 < P->used; q += 2) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, P->ev[q], P->ev[q + 1]);
+            (void)hipEventElapsedTime(&dur[q / 2], P->ev[q], P->ev[q + 1]);
             int t = P->type[q / 2];
-            if (t >= 0 && t < HMFEM_PROF_TYPES) { sink->ms[t] += ms; sink->count[t] += 1; }
+            if (t >= 0 && t < HMFEM_PROF_TYPES && dur[q / 2] > longest[t]) longest[t] = dur[q / 2];
+        }
+        for (int q = 0; q + 1 < P->used; q += 2) {
+            int t = P->type[q / 2];
+            if (t < 0 || t >= HMFEM_PROF_TYPES || dur[q / 2] < 0.25f * longest[t]) continue;
+            sink->ms[t] += dur[q / 2];
+            sink->count[t] += 1;
         }
         // algorithmic bytes per launch of the fine-level kernels (actual cells)
         double n0 = (double)d.n();
@@ -1190,6 +1614,11 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         sink->bytes[HMFEM_PROF_RESTRICT] = n0 * (c4 + 8 + vs) + n1 * vs;     // A, f, u; f_c out
         sink->bytes[HMFEM_PROF_PROLONG] = n0 * (c4 + 8 + 2 * vs) + n1 * vs;  // A, f, u, u_c; u out
         sink->bytes[HMFEM_PROF_JACOBI] = n0 * (c4 + 8 + 2 * vs + 8);         // A, f, u, v; z out (+dots)
+        if (opt.smoother == HMFEM_SMOOTH_BLOCK_GS) {
+            // pre: wx, wy, wz, di, f; u out. post: the same plus u, u_c, v; z out (+dots)
+            sink->bytes[HMFEM_PROF_PROLONG] = n0 * (c4 + 8 + 2 * vs + 8) + n1 * vs;
+            sink->bytes[HMFEM_PROF_JACOBI] = 0.0;
+        }
     }
     if (out) {
         float t0 = 0, t1 = 0;
@@ -1211,6 +1640,11 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         for (int l = 0; l < lim; l++) {
             double nl = (double)H.lev[l].d.n();
             double fb = l == 0 ? 8.0 : vs;
+            if (opt.smoother == HMFEM_SMOOTH_BLOCK_GS) {
+                // one pass each: pre-smoothing, residual + restriction, post-smoothing
+                per += nl * (3 * (4 * cs + fb) + 4 * vs);
+                continue;
+            }
             double sweeps = 2.0 * opt.nu;
             per += nl * (sweeps * (4 * cs + fb + 2 * vs) + (4 * cs + fb + vs));
         }

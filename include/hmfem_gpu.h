@@ -98,6 +98,8 @@ enum { HMFEM_PREC_FP64 = 0, HMFEM_PREC_FP32 = 1, HMFEM_PREC_FP16 = 2 };
 /* HMFEM_STOP_RESIDUAL: ||b - A x||_2 <= rtol ||b||_2.
    HMFEM_STOP_PRECONDITIONED: <M r, r> <= rtol^2 <M r0, r0> (pcg.h:57). */
 enum { HMFEM_STOP_RESIDUAL = 0, HMFEM_STOP_PRECONDITIONED = 1 };
+/* Smoother of the multi-block V-cycle levels (hmfem_options.smoother). */
+enum { HMFEM_SMOOTH_JACOBI = 0, HMFEM_SMOOTH_BLOCK_GS = 1 };
 
 typedef struct {
     int nx, ny, nz;
@@ -113,7 +115,7 @@ typedef struct {
     int stop_rule;           /* default HMFEM_STOP_RESIDUAL */
     int nu;                  /* Jacobi sweeps before and after the coarse correction (default 3) */
     double omega;            /* Jacobi damping (default 0.95) */
-    double ocf;              /* coarse-correction scaling (default 2.0) */
+    double ocf;              /* coarse-correction scaling (default 2.5) */
     int coarse_max;          /* largest coarsest level, solved densely (default and maximum 80) */
     int check_every;         /* CG iterations queued between host convergence checks (default 4) */
     int tail_max;            /* levels of at most this many cells run in one single-workgroup
@@ -121,6 +123,11 @@ typedef struct {
     int tail_gamma;          /* cycle inside that kernel: 1 V-cycle, 2 W-cycle (default 2) */
     int use_graph;           /* replay each CG iteration from a captured HIP graph (default 1) */
     int tail_lds;            /* run that kernel on LDS copies of its levels when they fit (default 1) */
+    int smoother;            /* smoother of the multi-block levels: HMFEM_SMOOTH_JACOBI (nu damped Jacobi
+                                sweeps) or HMFEM_SMOOTH_BLOCK_GS (gs_sweeps red-black Gauss-Seidel sweeps
+                                inside 32x8x8 tiles, one pass per smoothing step; the default); the
+                                levels of the single-workgroup kernel always use Jacobi */
+    int gs_sweeps;           /* sweeps per smoothing step of HMFEM_SMOOTH_BLOCK_GS (default 3) */
 } hmfem_options;
 
 typedef struct {
@@ -177,7 +184,8 @@ int hmfem_solve_structured_device(hmfem_context *ctx, int nx, int ny, int nz, co
    (device). Per-instance results go to the optional host arrays
    iterations, rel_residual, done_ms (time from the start of the solve
    phase at which the instance's stop test first held, resolution
-   check_every iterations) and div_max. */
+   check_every iterations) and div_max. The batch smooths with damped
+   Jacobi (opt->nu, omega, ocf) whatever opt->smoother says. */
 typedef struct {
     int status;              /* HMFEM_OK if every instance converged */
     int max_iterations;      /* largest iteration count over the instances */
@@ -195,8 +203,11 @@ int hmfem_reservoir_batch_device(hmfem_context *ctx, int ninst, const int *dims,
    (capacity: max_launches launches per call). hmfem_profile_read returns, per
    kernel class, the summed milliseconds, the launch count and the algorithmic
    HBM bytes of one launch. Classes: 0 CG SpMV+dot, 1 CG update+dot, 2 CG
-   dots, 3 pre-smoothing, 4 residual+restriction, 5 prolongation+Jacobi,
-   6 Jacobi, 7 coarse levels. Returns the number of classes. */
+   dots, 3 pre-smoothing, 4 residual+restriction, 5 post-smoothing (block GS:
+   prolongation + all sweeps + CG dots; Jacobi: prolongation + one sweep),
+   6 further Jacobi sweeps, 7 coarse levels. Launches that returned at once
+   because the solve had already converged (shorter than a quarter of their
+   class's longest launch) are not counted. Returns the number of classes. */
 int hmfem_profile_enable(hmfem_context *ctx, int max_launches);
 int hmfem_profile_read(hmfem_context *ctx, double *ms, long *count, double *bytes, int nclasses);
 void hmfem_profile_reset(hmfem_context *ctx);

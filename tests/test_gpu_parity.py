@@ -129,13 +129,18 @@ def test_c1_closed_form_device(ctx, torch):
     P = make_problem((16, 16, 16), "homogeneous")
     D = Dev(P, torch)
     exact = 1.0 - (np.arange(P.n) % 16 + 0.5) / 16.0
-    for prec in (S.PREC_FP32, S.PREC_FP64):
-        for rule in (S.STOP_RESIDUAL, S.STOP_PRECONDITIONED):
-            _say("C1 precision", prec, "rule", rule)
+    for prec, rule, sm in [(S.PREC_FP32, S.STOP_RESIDUAL, S.SMOOTH_JACOBI),
+                           (S.PREC_FP32, S.STOP_PRECONDITIONED, S.SMOOTH_JACOBI),
+                           (S.PREC_FP64, S.STOP_RESIDUAL, S.SMOOTH_JACOBI),
+                           (S.PREC_FP64, S.STOP_PRECONDITIONED, S.SMOOTH_JACOBI),
+                           (S.PREC_FP32, S.STOP_RESIDUAL, S.SMOOTH_BLOCK_GS),
+                           (S.PREC_FP64, S.STOP_PRECONDITIONED, S.SMOOTH_BLOCK_GS)]:
+        for _ in (0,):
+            _say("C1 precision", prec, "rule", rule, "smoother", sm)
             p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
             v = torch.zeros(P.n * 7, dtype=torch.float64, device="cuda")
             st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, v, S.default_options(rtol=1e-10, precision=prec,
-                                                                                    stop_rule=rule))
+                                                                                    stop_rule=rule, smoother=sm))
             assert st.status == S.OK
             assert rel_l2(p.cpu().numpy(), exact) < 1e-9
             vv = v.cpu().numpy().view(S.ELE)
@@ -145,14 +150,16 @@ def test_c1_closed_form_device(ctx, torch):
 
 @pytest.mark.parametrize("kind", ["lognormal", "karst"])
 @pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_FP64])
-def test_pipeline_vs_oracle_32(kind, prec, ctx, torch):
+@pytest.mark.parametrize("smoother", [S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS], ids=["jacobi", "blockgs"])
+def test_pipeline_vs_oracle_32(kind, prec, smoother, ctx, torch):
     P = make_problem((32, 32, 32), kind, seed=5, conduits=6, corr=4.0)
-    _say("pipeline 32^3", kind, prec)
+    _say("pipeline 32^3", kind, prec, smoother)
     xo, operm, oind, ob = oracle_solution(P, eps=1e-12)
     D = Dev(P, torch)
     p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
     v = torch.zeros(P.n * 7, dtype=torch.float64, device="cuda")
-    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, v, S.default_options(rtol=1e-12, precision=prec))
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, v, S.default_options(rtol=1e-12, precision=prec,
+                                                                            smoother=smoother))
     assert st.status == S.OK, st.as_dict()
     x = p.cpu().numpy()
     assert rel_l2(x, xo) < PRESSURE_TOL
@@ -174,14 +181,15 @@ def test_pipeline_vs_oracle_32(kind, prec, ctx, torch):
     ((4, 3, 5), "lognormal", (0, 0, 1, 1, 1, 1)),           # coarsest level only (dense)
     ((6, 5, 4), "lognormal", (1, 1, 1, 1, 1, 1)),           # all Neumann: pinned cell
 ])
-def test_edge_shapes_vs_oracle(dims, kind, bc, ctx, torch):
+@pytest.mark.parametrize("smoother", [S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS], ids=["jacobi", "blockgs"])
+def test_edge_shapes_vs_oracle(dims, kind, bc, smoother, ctx, torch):
     bval = (1.0, 0.25, 1e-14, 0.5, -1e-14, 2e-14)
     P = make_problem(dims, kind, seed=9, corr=2.0, conduits=2, bc_type=bc, bc_val=bval)
-    _say("edge", dims, kind, bc)
+    _say("edge", dims, kind, bc, smoother)
     xo, operm, oind, ob = oracle_solution(P, eps=1e-13)
     D = Dev(P, torch)
     p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
-    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-12))
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-12, smoother=smoother))
     assert st.status == S.OK, st.as_dict()
     assert rel_l2(p.cpu().numpy(), xo) < PRESSURE_TOL
 
@@ -212,16 +220,17 @@ def test_compute_rejects_unstructured():
         S.compute(hdr, cnt, col, ele, np.ones(n), np.zeros(n))
 
 
-def test_full_size_c2_properties(ctx, torch):
+@pytest.mark.parametrize("smoother", [S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS], ids=["jacobi", "blockgs"])
+def test_full_size_c2_properties(smoother, ctx, torch):
     """128^3 lognormal sigma=2 (the benchmark workload) at rtol 1e-8:
     true residual, maximum principle, global mass balance."""
     P = make_problem((128, 128, 128), "lognormal", seed=20261202, sigma=2.0, corr=8.0)
-    _say("C2 128^3")
+    _say("C2 128^3", smoother)
     D = Dev(P, torch)
     n = P.n
     p = torch.zeros(n, dtype=torch.float64, device="cuda")
     v = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
-    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, v, S.default_options(rtol=1e-8))
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, v, S.default_options(rtol=1e-8, smoother=smoother))
     _say("C2 status", st.as_dict())
     assert st.status == S.OK
     # independent FP64 residual from the device-assembled AoS stencil
@@ -247,6 +256,30 @@ def test_full_size_c2_properties(ctx, torch):
     assert abs(outflow - inflow) <= hx * np.sqrt(n) * r * np.linalg.norm(bh) * 1.01
 
 
+@pytest.mark.parametrize("smoother", [S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS], ids=["jacobi", "blockgs"])
+def test_graph_replay_matches_eager(smoother, ctx, torch):
+    """The captured CG iteration replayed from a HIP graph gives the same
+    pressure, bit for bit, as eager launches, on every replay, with unrelated
+    device work and synchronisations between the solves."""
+    P = make_problem((48, 40, 36), "lognormal", seed=11, corr=4.0)
+    D = Dev(P, torch)
+    _say("graph replay", smoother)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    eager = S.default_options(rtol=1e-10, use_graph=0, smoother=smoother)
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, eager)
+    assert st.status == S.OK and st.iterations > 4
+    ref = p.cpu().numpy().copy()
+    junk = torch.randn(1 << 20, device="cuda", dtype=torch.float64)
+    for rep in range(3):
+        p.zero_()
+        junk = (junk * 1.0001).sin_()
+        torch.cuda.synchronize()
+        st2 = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None,
+                            S.default_options(rtol=1e-10, use_graph=1, smoother=smoother))
+        assert st2.status == S.OK and st2.iterations == st.iterations
+        assert np.array_equal(p.cpu().numpy().view(np.uint64), ref.view(np.uint64)), rep
+
+
 def test_batch_uneven_matches_single(ctx, torch):
     """Uneven instances solved together give each instance's own solution:
     against the single-instance solver and, for the small ones, the oracle."""
@@ -261,7 +294,7 @@ def test_batch_uneven_matches_single(ctx, torch):
     ntot = sum(P.n for P in probs)
     p = torch.zeros(ntot, dtype=torch.float64, device="cuda")
     v = torch.zeros(ntot * 7, dtype=torch.float64, device="cuda")
-    o = S.default_options(rtol=1e-11)
+    o = S.default_options(rtol=1e-12)
     st, per = ctx.reservoir_batch([(P.nx, P.ny, P.nz) for P in probs], K, K, K, bval, p, v, o)
     assert st.status == S.OK, (st.as_dict(), per)
     assert np.all(per["iterations"] > 0) and np.all(per["done_ms"] >= 0)
@@ -270,7 +303,9 @@ def test_batch_uneven_matches_single(ctx, torch):
     for P, (d, k, s) in zip(probs, specs):
         D = Dev(P, torch)
         ps = torch.zeros(P.n, dtype=torch.float64, device="cuda")
-        ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, ps, None, o)
+        # the batch smooths with damped Jacobi; compare with the same smoother
+        ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, ps, None,
+                      S.default_options(rtol=1e-12, smoother=S.SMOOTH_JACOBI))
         x = xb[off:off + P.n]
         assert rel_l2(x, ps.cpu().numpy()) < 1e-9, d
         if P.n <= 6000:
