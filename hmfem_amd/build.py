@@ -3,18 +3,21 @@
 libhmfem_fields.so  host C (gcc): seeded permeability fields
 libhmfem_gpu.so     HIP for gfx950 (hipcc): the solver and its C ABI
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
-HIP_SOURCES = ["hmfem_assemble.hip", "hmfem_solver.hip", "hmfem_batch.hip", "hmfem_capi.hip"]
+HIP_SOURCES = ["hmfem_assemble.hip", "hmfem_solver.hip", "hmfem_batch.hip", "hmfem_block.hip", "hmfem_capi.hip"]
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    # no packed FP32 instructions (see the build notes in DESIGN.md)
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # no packed FP32 arithmetic instructions: some of their forms have given
+    # wrong results on this hardware without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
@@ -38,12 +41,21 @@ def build_fields(verbose=False):
 
 
 def build_gpu(verbose=False):
+    """Compile every HIP source to an object in parallel, then link."""
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     out = os.path.join(HERE, "libhmfem_gpu.so")
-    tmp = out + ".tmp"
-    _run([hipcc] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include"), "-o", tmp]
-         + [os.path.join(CSRC, s) for s in HIP_SOURCES], verbose)
-    os.replace(tmp, out)
+    tmpdir = tempfile.mkdtemp(prefix="hmfem_build_")
+    try:
+        objs = [os.path.join(tmpdir, os.path.splitext(src)[0] + ".o") for src in HIP_SOURCES]
+        cmds = [[hipcc] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include"), "-c", "-o", obj,
+                                         os.path.join(CSRC, src)] for src, obj in zip(HIP_SOURCES, objs)]
+        with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as ex:
+            list(ex.map(lambda c: _run(c, verbose), cmds))
+        tmp = out + ".tmp"
+        _run([hipcc, "--offload-arch=gfx950", "-fPIC", "-shared", "-o", tmp] + objs, verbose)
+        os.replace(tmp, out)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
     return out
 
 

@@ -1,0 +1,705 @@
+// Block solve: several right-hand sides of one operator share one hierarchy
+// and one pass over the coefficients per kernel.
+//
+// Reference counterpart: the toolbox solves k interleaved right-hand sides
+// with one operator (linear_operator8, generic_operator.h:335-391; BLAS-1 over
+// packed vectors with stride k, algorithm.h:7-324). Here every vector of the
+// block is stored interleaved [cell][NR] (FP64 CG vectors, FP32 V-cycle
+// vectors), so one thread loads a cell's coefficients once and updates all NR
+// right-hand sides; each right-hand side keeps its own CG scalars and stops on
+// its own test (a finished one is frozen: alpha = 0, its search direction is
+// zeroed).
+//
+// The V-cycle is the single-RHS cycle with the block red-black Gauss-Seidel
+// smoother (tiles of 32 x 8 x TZ cells, TZ = 4 so the LDS tile holds NR values
+// per cell), the brick residual-restriction, and the single-workgroup tail run
+// once per right-hand side on strided views of the block vectors.
+#include "hmfem_hier.h"
+
+#include <string.h>
+
+#include <vector>
+
+namespace hmfem {
+
+constexpr int kMaxRhs = 8;
+constexpr int kBTz = 4;   // z extent of a block-smoother tile
+
+struct BlockScalars {
+    double bb[kMaxRhs], rr[kMaxRhs], sigma_first[kMaxRhs], beta[kMaxRhs], tau[kMaxRhs];
+    double sigma[2][kMaxRhs], alpha[2][kMaxRhs];
+    double rtol2;
+    int done[kMaxRhs], status[kMaxRhs], iter[kMaxRhs];
+    int all_done, kspmv, kupd, maxit, stop_rule, nr;
+};
+
+// Couplings of a cell to its six neighbours (0 where there is none) and the
+// neighbour indices (the cell itself where there is none).
+template <typename T>
+struct Nb {
+    T e, w[6];
+    int s, q[6];
+};
+
+template <typename T>
+__device__ __forceinline__ Nb<T> nb_at(const OpArgs<T> &o, int s)
+{
+    Cell c = cell_at(o, s);
+    const int nx = o.nx, nxy = o.nx * o.ny;
+    Nb<T> b;
+    b.s = s;
+    b.e = o.e[s];
+    const bool xp = c.i < o.nx - 1, xm = c.i > 0, yp = c.j < o.ny - 1, ym = c.j > 0, zp = c.k < o.nz - 1,
+               zm = c.k > 0;
+    b.w[0] = xp ? o.wx[s] : (T)0;       b.q[0] = xp ? s + 1 : s;
+    b.w[1] = xm ? o.wx[s - 1] : (T)0;   b.q[1] = xm ? s - 1 : s;
+    b.w[2] = yp ? o.wy[s] : (T)0;       b.q[2] = yp ? s + nx : s;
+    b.w[3] = ym ? o.wy[s - nx] : (T)0;  b.q[3] = ym ? s - nx : s;
+    b.w[4] = zp ? o.wz[s] : (T)0;       b.q[4] = zp ? s + nxy : s;
+    b.w[5] = zm ? o.wz[s - nxy] : (T)0; b.q[5] = zm ? s - nxy : s;
+    return b;
+}
+
+// (A u)_s for NR interleaved vectors given by a functor val(cell, r).
+template <int NR, typename C, typename T, typename V>
+__device__ __forceinline__ void apply_block(const Nb<T> &b, V val, C (&out)[NR])
+{
+    C us[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) { us[r] = val(b.s, r); out[r] = (C)b.e * us[r]; }
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) out[r] += (C)b.w[f] * (us[r] - val(b.q[f], r));
+    }
+}
+
+// ---------------------------------------------------------------- CG kernels
+
+// r = b - A x, partials (b.b, r.r) per right-hand side: P[lb][2 NR].
+template <int NR>
+__global__ void __launch_bounds__(kBlock)
+k_b_init(OpArgs<double> o, const double *__restrict__ B, const double *__restrict__ X, double *__restrict__ R,
+         int chunk, double *P)
+{
+    const int n = o.nx * o.ny * o.nz;
+    int lb, beg, end;
+    chunk_range(n, chunk, lb, beg, end);
+    double acc[2 * NR];
+#pragma unroll
+    for (int q = 0; q < 2 * NR; q++) acc[q] = 0.0;
+    for (int s = beg + threadIdx.x; s < end; s += kBlock) {
+        Nb<double> b = nb_at(o, s);
+        double ax[NR];
+        apply_block<NR, double>(b, [&](int q, int r) { return X[(long)q * NR + r]; }, ax);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const double bv = B[(long)s * NR + r];
+            const double rv = bv - ax[r];
+            R[(long)s * NR + r] = rv;
+            acc[r] += bv * bv;
+            acc[NR + r] += rv * rv;
+        }
+    }
+    block_sum<2 * NR>(acc);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 2 * NR; q++) P[(long)2 * NR * lb + q] = acc[q];
+}
+
+// Stopping tests and beta per right-hand side, then s = z + beta s_old,
+// v = A s, partials <s, v> (P_tau[lb][NR]).
+template <int NR>
+__global__ void __launch_bounds__(kBlock)
+k_b_spmv(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S0, double *__restrict__ S1,
+         double *__restrict__ V, int chunk, int G, BlockScalars *sc, const double *Pinit, const double *Prr,
+         const double *Pdot, double *Ptau)
+{
+    if (sc->all_done) return;
+    const int n = o.nx * o.ny * o.nz;
+    const int k = sc->kspmv;
+    const double *__restrict__ sin = (k & 1) ? S1 : S0;
+    double *__restrict__ sout = (k & 1) ? S0 : S1;
+    double dots[2 * NR], rrs[NR], bbs[NR];
+    sum_partials<2 * NR>(Pdot, G, dots);
+    if (k == 0) {
+        double ir[2 * NR];
+        sum_partials<2 * NR>(Pinit, G, ir);
+#pragma unroll
+        for (int r = 0; r < NR; r++) { bbs[r] = ir[r]; rrs[r] = ir[NR + r]; }
+    } else {
+        double t[NR];
+        sum_partials<NR>(Prr, G, t);
+#pragma unroll
+        for (int r = 0; r < NR; r++) { rrs[r] = t[r]; bbs[r] = sc->bb[r]; }
+    }
+    double beta[NR];
+    int live[NR];
+    int all = 1;
+    const bool writer = xcd_block() == 0 && threadIdx.x == 0;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        beta[r] = 0.0;
+        live[r] = 0;
+        if (sc->done[r]) continue;
+        const double sigma = fabs(dots[r]);
+        int stop = 0, status = HMFEM_OK;
+        const double sf = k == 0 ? (sigma == 0.0 ? 1.0 : sigma) : sc->sigma_first[r];
+        if (k == 0) {
+            if (bbs[r] == 0.0 || rrs[r] <= sc->rtol2 * bbs[r] || sigma == 0.0) stop = 1;
+        } else {
+            beta[r] = -sc->alpha[(k - 1) & 1][r] * dots[NR + r] / sc->sigma[(k - 1) & 1][r];
+            if (sc->stop_rule == HMFEM_STOP_RESIDUAL && rrs[r] <= sc->rtol2 * bbs[r]) stop = 1;
+            if (sc->stop_rule == HMFEM_STOP_PRECONDITIONED && sigma <= sc->rtol2 * sf) stop = 1;
+            if (!(sigma == sigma) || !(beta[r] == beta[r])) { stop = 1; status = HMFEM_ERR_BREAKDOWN; }
+            else if (!stop && k >= sc->maxit) { stop = 1; status = HMFEM_NOT_CONVERGED; }
+        }
+        if (writer) {
+            if (k == 0) { sc->bb[r] = bbs[r]; sc->sigma_first[r] = sf; }
+            sc->rr[r] = rrs[r];
+            sc->sigma[k & 1][r] = sigma;
+            sc->beta[r] = beta[r];
+            if (stop) { sc->done[r] = 1; sc->status[r] = status; sc->iter[r] = k; }
+        }
+        if (stop) beta[r] = 0.0;
+        else { live[r] = 1; all = 0; }
+    }
+    if (writer) {
+        sc->kupd = k;
+        if (all) sc->all_done = 1;
+    }
+    if (all) return;
+    int lb, beg, end;
+    chunk_range(n, chunk, lb, beg, end);
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc[r] = 0.0;
+    auto sv = [&](int q, int r) -> double {
+        if (!live[r]) return 0.0;
+        const double z = (double)Z[(long)q * NR + r];
+        return k == 0 ? z : z + beta[r] * sin[(long)q * NR + r];
+    };
+    for (int s = beg + threadIdx.x; s < end; s += kBlock) {
+        Nb<double> b = nb_at(o, s);
+        double av[NR];
+        apply_block<NR, double>(b, sv, av);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const double ss = sv(s, r);
+            sout[(long)s * NR + r] = ss;
+            V[(long)s * NR + r] = av[r];
+            acc[r] += ss * av[r];
+        }
+    }
+    block_sum<NR>(acc);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < NR; r++) Ptau[(long)NR * lb + r] = acc[r];
+}
+
+// alpha = sigma / tau per right-hand side; x += alpha s, r -= alpha v; partials r.r.
+template <int NR>
+__global__ void __launch_bounds__(kBlock)
+k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, const double *__restrict__ V,
+           double *__restrict__ X, double *__restrict__ R, int chunk, int G, BlockScalars *sc, const double *Ptau,
+           double *Prr)
+{
+    if (sc->all_done) return;
+    const int k = sc->kupd;
+    const double *__restrict__ S = (k & 1) ? S0 : S1;
+    double t[NR];
+    sum_partials<NR>(Ptau, G, t);
+    double alpha[NR];
+    const bool writer = xcd_block() == 0 && threadIdx.x == 0;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        alpha[r] = 0.0;
+        if (sc->done[r]) continue;
+        const double tau = fabs(t[r]);
+        if (tau == 0.0 || !(tau == tau)) {
+            if (writer) { sc->done[r] = 1; sc->status[r] = HMFEM_ERR_BREAKDOWN; sc->iter[r] = k; }
+            continue;
+        }
+        alpha[r] = sc->sigma[k & 1][r] / tau;
+    }
+    if (writer) {
+        for (int r = 0; r < NR; r++) { sc->alpha[k & 1][r] = alpha[r]; sc->tau[r] = t[r]; }
+        sc->kspmv = k + 1;
+    }
+    int lb, beg, end;
+    chunk_range(n, chunk, lb, beg, end);
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc[r] = 0.0;
+    for (int s = beg + threadIdx.x; s < end; s += kBlock) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const long q = (long)s * NR + r;
+            const double ri = R[q] - alpha[r] * V[q];
+            X[q] += alpha[r] * S[q];
+            R[q] = ri;
+            acc[r] += ri * ri;
+        }
+    }
+    block_sum<NR>(acc);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < NR; r++) Prr[(long)NR * lb + r] = acc[r];
+}
+
+// ---------------------------------------------------------------- V-cycle kernels
+
+// Block red-black Gauss-Seidel on tiles of kGsTx x kGsTy x kBTz cells for NR
+// interleaved right-hand sides. MODE 0: from zero; MODE 1: from u + ocf *
+// uc[aggregate] (uo must not alias u); MODE 2: MODE 1 plus the CG dot
+// partials <z, f>_r, <z, v>_r of this block's tiles (Pdot[lb][2 NR]).
+template <int NR, typename FT, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *__restrict__ u,
+            float *__restrict__ uo, int sx, int sy, int sz, int cx, int cy, const float *__restrict__ uc, float ocf,
+            const double *__restrict__ v, int tpb, double *Pdot, const int *done)
+{
+    constexpr int TZ = kBTz;
+    constexpr int RING = 2 * (kGsTx + 2) + 2 * kGsTy;
+    constexpr int RPER = (TZ * RING + kBlock - 1) / kBlock;
+    if (*done) return;
+    __shared__ float sh[TZ][kGsTy + 2][kGsTx + 2][NR];
+    const int nx = o.nx, ny = o.ny, nz = o.nz;
+    const long nxy = (long)nx * ny;
+    const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy, ntz = (nz + TZ - 1) / TZ;
+    const int ntiles = ntx * nty * ntz;
+    const int lb = xcd_block();
+    const int tbeg = lb * tpb, tend = min(ntiles, tbeg + tpb);
+    const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+    double acc[MODE == 2 ? 2 * NR : 1];
+    if constexpr (MODE == 2)
+        for (int q = 0; q < 2 * NR; q++) acc[q] = 0.0;
+
+    auto start = [&](int ii, int jj, int kk, float (&val)[NR]) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) val[r] = 0.f;
+        if (MODE == 0 || ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) return;
+        const long s = ii + (long)jj * nx + (long)kk * nxy;
+        const long I = (ii >> sx) + (long)cx * ((jj >> sy) + (long)cy * (kk >> sz));
+#pragma unroll
+        for (int r = 0; r < NR; r++) val[r] = u[s * NR + r] + ocf * uc[I * NR + r];
+    };
+
+    for (int t = tbeg; t < tend; t++) {
+        const int ti = t % ntx, tj = (t / ntx) % nty, tk = t / (ntx * nty);
+        const int i0 = ti * kGsTx, j0 = tj * kGsTy, k0 = tk * TZ;
+        const int i = i0 + tx, j = j0 + ty;
+        const bool col = i < nx && j < ny;
+        const long s0 = i + (long)j * nx + (long)k0 * nxy;
+
+        float wxp[TZ], wxm[TZ], wyp[TZ], wym[TZ], wzp[TZ], dinv[TZ], ff[TZ][NR], uu[TZ][NR];
+#pragma unroll
+        for (int kk = 0; kk < TZ; kk++) {
+            const bool act = col && k0 + kk < nz;
+            const long s = s0 + kk * nxy;
+            wxp[kk] = act ? o.wx[s] : 0.f;
+            wxm[kk] = act && i > 0 ? o.wx[s - 1] : 0.f;
+            wyp[kk] = act ? o.wy[s] : 0.f;
+            wym[kk] = act && j > 0 ? o.wy[s - nx] : 0.f;
+            wzp[kk] = act ? o.wz[s] : 0.f;
+            dinv[kk] = act ? o.di[s] : 0.f;
+#pragma unroll
+            for (int r = 0; r < NR; r++) ff[kk][r] = act ? (float)f[s * NR + r] : 0.f;
+        }
+        const float wzlo = col && k0 > 0 ? o.wz[s0 - nxy] : 0.f;
+        float zlo[NR], zhi[NR], ring[RPER][NR];
+#pragma unroll
+        for (int kk = 0; kk < TZ; kk++) start(i, j, k0 + kk, uu[kk]);
+        start(i, j, k0 - 1, zlo);
+        start(i, j, k0 + TZ, zhi);
+#pragma unroll
+        for (int m = 0; m < RPER; m++) {
+            const int q = threadIdx.x + m * kBlock;
+            const int hz = q / RING, rr = q - hz * RING;
+            int hx, hy;
+            if (rr < kGsTx + 2) { hx = rr; hy = 0; }
+            else if (rr < 2 * (kGsTx + 2)) { hx = rr - (kGsTx + 2); hy = kGsTy + 1; }
+            else { const int r2 = rr - 2 * (kGsTx + 2); hy = 1 + (r2 >> 1); hx = (r2 & 1) ? kGsTx + 1 : 0; }
+            if (q < TZ * RING) start(i0 + hx - 1, j0 + hy - 1, k0 + hz, ring[m]);
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+                if (q < TZ * RING) sh[hz][hy][hx][r] = ring[m][r];
+        }
+#pragma unroll
+        for (int kk = 0; kk < TZ; kk++)
+#pragma unroll
+            for (int r = 0; r < NR; r++) sh[kk][ty + 1][tx + 1][r] = uu[kk][r];
+        __syncthreads();
+
+        for (int sw = 0; sw < sweeps; sw++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int c = MODE == 0 ? h : 1 - h;
+                const int sel = (c + i + j + k0) & 1;
+#pragma unroll
+                for (int m = 0; m < TZ / 2; m++) {
+                    const int ka = 2 * m, kb = 2 * m + 1;
+                    const int kk = ka + sel;
+                    const float cxp = sel ? wxp[kb] : wxp[ka], cxm = sel ? wxm[kb] : wxm[ka];
+                    const float cyp = sel ? wyp[kb] : wyp[ka], cym = sel ? wym[kb] : wym[ka];
+                    const float czp = sel ? wzp[kb] : wzp[ka];
+                    const float czm = sel ? wzp[ka] : (m > 0 ? wzp[ka - 1] : wzlo);
+                    const float di_ = sel ? dinv[kb] : dinv[ka];
+#pragma unroll
+                    for (int r = 0; r < NR; r++) {
+                        const float uzm = sel ? uu[ka][r] : (m > 0 ? uu[ka - 1][r] : zlo[r]);
+                        const float uzp = sel ? (kb + 1 < TZ ? uu[kb + 1][r] : zhi[r]) : uu[kb][r];
+                        const float a = (sel ? ff[kb][r] : ff[ka][r]) + cxp * sh[kk][ty + 1][tx + 2][r] +
+                                        cxm * sh[kk][ty + 1][tx][r] + cyp * sh[kk][ty + 2][tx + 1][r] +
+                                        cym * sh[kk][ty][tx + 1][r] + czp * uzp + czm * uzm;
+                        const float un = a * di_;
+                        uu[ka][r] = sel ? uu[ka][r] : un;
+                        uu[kb][r] = sel ? un : uu[kb][r];
+                        sh[kk][ty + 1][tx + 1][r] = un;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+#pragma unroll
+        for (int kk = 0; kk < TZ; kk++) {
+            if (col && k0 + kk < nz) {
+                const long s = s0 + kk * nxy;
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    uo[s * NR + r] = uu[kk][r];
+                    if constexpr (MODE == 2) {
+                        acc[r] += (double)uu[kk][r] * (double)f[s * NR + r];
+                        if (v) acc[NR + r] += (double)uu[kk][r] * v[s * NR + r];
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (MODE == 2) {
+        block_sum<2 * NR>(acc);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 2 * NR; q++) Pdot[(long)2 * NR * lb + q] = acc[q];
+    }
+}
+
+// Residual f - A u summed over 2x2x2 aggregates for NR right-hand sides, one
+// thread per fine cell (bricks of 32 x 4 x 2 fine cells as in the single-RHS
+// kernel).
+template <int NR, typename FT>
+__global__ void __launch_bounds__(kBlock)
+k_b_restrict(OpArgs<float> o, int cx, int cy, int cz, const FT *__restrict__ f, const float *__restrict__ u,
+             float *__restrict__ fc, const int *done)
+{
+    if (*done) return;
+    __shared__ float upper[2][16][NR];
+    const int nbx = (o.nx + 31) / 32, nby = (o.ny + 3) / 4;
+    const int b = xcd_block();
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int t = threadIdx.x;
+    const int i = bx * 32 + (t & 31), j = by * 4 + ((t >> 5) & 3), k = bz * 2 + (t >> 7);
+    float res[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) res[r] = 0.f;
+    if (i < o.nx && j < o.ny && k < o.nz) {
+        const int s = i + j * o.nx + k * o.nx * o.ny;
+        Nb<float> nb = nb_at(o, s);
+        float au[NR];
+        apply_block<NR, float>(nb, [&](int q, int r) { return u[(long)q * NR + r]; }, au);
+#pragma unroll
+        for (int r = 0; r < NR; r++) res[r] = (float)f[(long)s * NR + r] - au[r];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        res[r] += __shfl_xor(res[r], 1);
+        res[r] += __shfl_xor(res[r], 32);
+    }
+    const int lane = t & 63, cjl = (t >> 6) & 1, cil = (t & 31) >> 1;
+    if ((t >> 7) == 1 && lane < 32 && !(lane & 1))
+        for (int r = 0; r < NR; r++) upper[cjl][cil][r] = res[r];
+    __syncthreads();
+    if ((t >> 7) == 0 && lane < 32 && !(lane & 1)) {
+        const int ci = bx * 16 + cil, cj = by * 2 + cjl, ck = bz;
+        if (ci < cx && cj < cy && ck < cz) {
+            const long I = ci + (long)cx * (cj + (long)cy * ck);
+            for (int r = 0; r < NR; r++) fc[I * NR + r] = res[r] + upper[cjl][cil][r];
+        }
+    }
+}
+
+// Column r of a block <- a vector (plus an optional second vector).
+__global__ void __launch_bounds__(kBlock)
+k_b_put(long n, int nr, int r, const double *__restrict__ a, const double *__restrict__ add, double *__restrict__ Bk)
+{
+    long s = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    double x = a ? a[s] : 0.0;
+    if (add) x += add[s];
+    Bk[s * nr + r] = x;
+}
+
+// A vector <- column r of a block.
+__global__ void __launch_bounds__(kBlock)
+k_b_get(long n, int nr, int r, const double *__restrict__ Bk, double *__restrict__ a)
+{
+    long s = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    a[s] = Bk[s * nr + r];
+}
+
+// ---------------------------------------------------------------- host side
+
+struct BlockLevelVecs {
+    float *f = nullptr, *u = nullptr, *t = nullptr;
+};
+
+struct BlockState {
+    int nr;
+    Chunking ch;
+    double *R, *S0, *S1, *V, *X;
+    float *Z;                         // fine-level V-cycle output (interleaved)
+    BlockScalars *sc;
+    double *Pinit, *Ptau, *Prr, *Pdot;
+    BlockLevelVecs lv[kMaxLevels];
+    int *done;                        // &sc->all_done
+};
+
+static int lg2(int f) { return f == 2 ? 1 : 0; }
+
+template <int NR, typename FT>
+static void b_restrict(hipStream_t st, const Level &L, const Level &C, const FT *f, const float *u, float *fc,
+                       const int *done)
+{
+    const int nb = ((L.d.nx + 31) / 32) * ((L.d.ny + 3) / 4) * ((L.d.nz + 1) / 2);
+    hipLaunchKernelGGL((k_b_restrict<NR, FT>), dim3(grid_for(nb, 1)), dim3(kBlock), 0, st, opargs<float>(L),
+                       C.d.nx, C.d.ny, C.d.nz, f, u, fc, done);
+}
+
+static int b_tiles(const Dims &d)
+{
+    return ((d.nx + kGsTx - 1) / kGsTx) * ((d.ny + kGsTy - 1) / kGsTy) * ((d.nz + kBTz - 1) / kBTz);
+}
+
+// V-cycle on level l >= 1 for all right-hand sides; result in S.lv[l].u.
+template <int NR>
+static int b_vcycle_coarse(hipStream_t st, Hierarchy &H, BlockState &S, int l, const hmfem_options &opt)
+{
+    Level &L = H.lev[l];
+    BlockLevelVecs &X = S.lv[l];
+    if (l == H.tail0) {
+        for (int r = 0; r < NR; r++) {
+            int rc = launch_tail_strided(st, H, opt.omega, opt.ocf, X.f + r, X.u + r, NR, S.done);
+            if (rc) return rc;
+        }
+        return HMFEM_OK;
+    }
+    Level &C = H.lev[l + 1];
+    const int GT = grid_for(b_tiles(L.d), 1);
+    const int sw = opt.gs_sweeps < 1 ? 1 : opt.gs_sweeps;
+    hipLaunchKernelGGL((k_b_gs_tile<NR, float, 0>), dim3(GT), dim3(kBlock), 0, st, opargs<float>(L), sw,
+                       (const float *)X.f, (const float *)nullptr, X.u, 0, 0, 0, 0, 0, (const float *)nullptr, 0.f,
+                       (const double *)nullptr, 1, (double *)nullptr, S.done);
+    b_restrict<NR, float>(st, L, C, X.f, X.u, S.lv[l + 1].f, S.done);
+    int rc = b_vcycle_coarse<NR>(st, H, S, l + 1, opt);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_b_gs_tile<NR, float, 1>), dim3(GT), dim3(kBlock), 0, st, opargs<float>(L), sw,
+                       (const float *)X.f, (const float *)X.u, X.t, lg2(L.fx), lg2(L.fy), lg2(L.fz), C.d.nx, C.d.ny,
+                       (const float *)S.lv[l + 1].u, (float)opt.ocf, (const double *)nullptr, 1, (double *)nullptr,
+                       S.done);
+    float *tmp = X.u; X.u = X.t; X.t = tmp;
+    return HMFEM_OK;
+}
+
+// Fine-level V-cycle: S.Z = M R with the dot partials of <z, r>, <z, v>.
+template <int NR>
+static int b_vcycle_fine(hipStream_t st, Hierarchy &H, BlockState &S, const double *Vp, const hmfem_options &opt)
+{
+    Level &L = H.lev[0], &C = H.lev[1];
+    BlockLevelVecs &X = S.lv[0];
+    const int nt = b_tiles(L.d);
+    const int sw = opt.gs_sweeps < 1 ? 1 : opt.gs_sweeps;
+    hipLaunchKernelGGL((k_b_gs_tile<NR, double, 0>), dim3(grid_for(nt, 1)), dim3(kBlock), 0, st, opargs<float>(L),
+                       sw, (const double *)S.R, (const float *)nullptr, X.u, 0, 0, 0, 0, 0, (const float *)nullptr,
+                       0.f, (const double *)nullptr, 1, (double *)nullptr, S.done);
+    b_restrict<NR, double>(st, L, C, S.R, X.u, S.lv[1].f, S.done);
+    int rc = b_vcycle_coarse<NR>(st, H, S, 1, opt);
+    if (rc) return rc;
+    const int tpb = (nt + S.ch.G - 1) / S.ch.G;
+    hipLaunchKernelGGL((k_b_gs_tile<NR, double, 2>), dim3(S.ch.G), dim3(kBlock), 0, st, opargs<float>(L), sw,
+                       (const double *)S.R, (const float *)X.u, S.Z, lg2(L.fx), lg2(L.fy), lg2(L.fz), C.d.nx,
+                       C.d.ny, (const float *)S.lv[1].u, (float)opt.ocf, Vp, tpb, S.Pdot, S.done);
+    return HMFEM_OK;
+}
+
+static Chunking b_chunking(long n)
+{
+    Chunking c;
+    long g = (n + kBlock * 4 - 1) / (kBlock * 4);
+    if (g > kMaxReduceBlocks) g = kMaxReduceBlocks;
+    if (g < kNumXcd) g = kNumXcd;
+    g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
+    long chunk = (n + g - 1) / g;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    c.G = (int)g;
+    c.chunk = (int)chunk;
+    return c;
+}
+
+size_t block_bytes(Dims fine, int nr, const hmfem_options &opt)
+{
+    hmfem_options o = opt;
+    o.precision = HMFEM_PREC_FP32;
+    const size_t n = fine.n();
+    size_t b = hierarchy_bytes(fine, o);
+    b += 4 * align256(n * nr * 8);                                   // R, S0, S1, V
+    b += align256(n * nr * 4);                                       // Z
+    Dims d = fine;                                                   // f, u, t per level
+    while (true) {
+        b += 3 * align256((size_t)d.n() * nr * 4);
+        if (d.n() <= 1 || (d.nx == 1 && d.ny == 1 && d.nz == 1)) break;
+        d = Dims{d.nx > 1 ? (d.nx + 1) / 2 : 1, d.ny > 1 ? (d.ny + 1) / 2 : 1, d.nz > 1 ? (d.nz + 1) / 2 : 1};
+    }
+    b += align256(sizeof(BlockScalars)) + 4 * align256((size_t)kMaxReduceBlocks * 2 * kMaxRhs * 8);
+    b += align256(64 * sizeof(int)) + 64 * 256;
+    return b;
+}
+
+template <int NR>
+static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d, BlockState &S,
+                    const hmfem_options &opt, BlockScalars *h_sc)
+{
+    const int n = (int)d.n();
+    OpArgs<double> o;
+    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
+    o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
+    const int G = S.ch.G;
+    int rc = b_vcycle_fine<NR>(st, H, S, nullptr, opt);
+    if (rc) return rc;
+    HMFEM_CHECK(hipGetLastError());
+    int k = 0;
+    const int every = opt.check_every < 1 ? 1 : opt.check_every;
+    while (true) {
+        for (int q = 0; q < every; q++, k++) {
+            hipLaunchKernelGGL(k_b_spmv<NR>, dim3(G), dim3(kBlock), 0, st, o, (const float *)S.Z, S.S0, S.S1, S.V,
+                               S.ch.chunk, G, S.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+            hipLaunchKernelGGL(k_b_update<NR>, dim3(G), dim3(kBlock), 0, st, n, S.S0, S.S1, S.V, S.X, S.R,
+                               S.ch.chunk, G, S.sc, S.Ptau, S.Prr);
+            rc = b_vcycle_fine<NR>(st, H, S, S.V, opt);
+            if (rc) return rc;
+        }
+        HMFEM_CHECK(hipGetLastError());
+        HMFEM_CHECK(hipMemcpyAsync(h_sc, S.sc, sizeof(BlockScalars), hipMemcpyDeviceToHost, st));
+        HMFEM_CHECK(hipStreamSynchronize(st));
+        if (h_sc->all_done || k > opt.max_iterations + every) break;
+    }
+    return HMFEM_OK;
+}
+
+// Solve A X = B for nr interleaved right-hand sides (B, X: [n][nr] FP64,
+// device; X holds the initial guess). Memory comes from `alloc`.
+int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const double *Bk, double *Xk,
+                const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *actx, long lds_limit,
+                std::vector<char> &stage, int *iterations, double *rel_residual, int *status, double *setup_ms,
+                double *solve_ms)
+{
+    if (nr != 2 && nr != 4 && nr != 8) return HMFEM_ERR_ARG;
+    hmfem_options opt = opt_in;
+    opt.precision = HMFEM_PREC_FP32;
+    if (opt.coarse_max > kCoarseMax || opt.coarse_max < 1) opt.coarse_max = kCoarseMax;
+    if (opt.tail_max < kCoarseMax) opt.tail_max = kCoarseMax;
+    if (opt.tail_max > 512) opt.tail_max = 512;   // the tail must run in LDS
+    opt.tail_lds = 1;
+    const size_t bytes = block_bytes(d, nr, opt);
+    char *mem = alloc(actx, bytes);
+    if (!mem) return HMFEM_ERR_NOMEM;
+    Arena ar{mem, 0, bytes};
+    const long n = d.n();
+    BlockState S;
+    S.nr = nr;
+    S.ch = b_chunking(n);
+    S.R = ar.take<double>(n * nr); S.S0 = ar.take<double>(n * nr); S.S1 = ar.take<double>(n * nr);
+    S.V = ar.take<double>(n * nr); S.X = Xk;
+    S.Z = ar.take<float>(n * nr);
+    S.sc = ar.take<BlockScalars>(1);
+    S.Pinit = ar.take<double>((size_t)kMaxReduceBlocks * 2 * kMaxRhs);
+    S.Ptau = ar.take<double>((size_t)kMaxReduceBlocks * 2 * kMaxRhs);
+    S.Prr = ar.take<double>((size_t)kMaxReduceBlocks * 2 * kMaxRhs);
+    S.Pdot = ar.take<double>((size_t)kMaxReduceBlocks * 2 * kMaxRhs);
+    int *d_info = ar.take<int>(64);
+    if (!S.R || !S.S0 || !S.S1 || !S.V || !S.Z || !S.sc || !S.Pdot || !d_info) return HMFEM_ERR_NOMEM;
+    S.done = &S.sc->all_done;
+
+    hipEvent_t e0, e1, e2;
+    HMFEM_CHECK(hipEventCreate(&e0));
+    HMFEM_CHECK(hipEventCreate(&e1));
+    HMFEM_CHECK(hipEventCreate(&e2));
+    HMFEM_CHECK(hipEventRecord(e0, st));
+    Hierarchy H;
+    HMFEM_CHECK(hipMemsetAsync(d_info, 0, sizeof(int), st));
+    int rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit);
+    if (rc) return rc;
+    if (H.nlev < 3 || H.tail0 <= 0 || H.tail_lds_inv < 0 || H.tail_nops > kTailLdsOps) return HMFEM_ERR_ARG;
+    for (int l = 0; l <= H.tail0; l++) {
+        const size_t nl = H.lev[l].d.n();
+        if (l > 0) S.lv[l].f = ar.take<float>(nl * nr);
+        S.lv[l].u = ar.take<float>(nl * nr);
+        if (l < H.tail0) S.lv[l].t = ar.take<float>(nl * nr);
+        if ((l > 0 && !S.lv[l].f) || !S.lv[l].u || (l < H.tail0 && !S.lv[l].t)) return HMFEM_ERR_NOMEM;
+    }
+    BlockScalars init{};
+    init.rtol2 = opt.rtol * opt.rtol;
+    init.maxit = opt.max_iterations;
+    init.stop_rule = opt.stop_rule;
+    init.nr = nr;
+    for (int r = nr; r < kMaxRhs; r++) init.done[r] = 1;
+    HMFEM_CHECK(hipMemcpyAsync(S.sc, &init, sizeof(BlockScalars), hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipEventRecord(e1, st));
+    OpArgs<double> o;
+    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
+    o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
+    BlockScalars hs{};
+    switch (nr) {
+    case 2:
+        hipLaunchKernelGGL(k_b_init<2>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
+        rc = block_cg<2>(st, H, A, d, S, opt, &hs);
+        break;
+    case 4:
+        hipLaunchKernelGGL(k_b_init<4>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
+        rc = block_cg<4>(st, H, A, d, S, opt, &hs);
+        break;
+    default:
+        hipLaunchKernelGGL(k_b_init<8>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
+        rc = block_cg<8>(st, H, A, d, S, opt, &hs);
+        break;
+    }
+    if (rc) return rc;
+    HMFEM_CHECK(hipEventRecord(e2, st));
+    HMFEM_CHECK(hipEventSynchronize(e2));
+    int info = 0;
+    HMFEM_CHECK(hipMemcpy(&info, d_info, sizeof(int), hipMemcpyDeviceToHost));
+    float t0 = 0.f, t1 = 0.f;
+    (void)hipEventElapsedTime(&t0, e0, e1);
+    (void)hipEventElapsedTime(&t1, e1, e2);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (setup_ms) *setup_ms = t0;
+    if (solve_ms) *solve_ms = t1;
+    for (int r = 0; r < nr; r++) {
+        if (iterations) iterations[r] = hs.iter[r];
+        if (rel_residual) rel_residual[r] = hs.bb[r] > 0 ? sqrt(hs.rr[r] / hs.bb[r]) : 0.0;
+        if (status) status[r] = info ? HMFEM_ERR_BREAKDOWN : (hs.done[r] ? hs.status[r] : HMFEM_NOT_CONVERGED);
+    }
+    return HMFEM_OK;
+}
+
+void block_put(hipStream_t st, long n, int nr, int r, const double *a, const double *add, double *Bk)
+{
+    hipLaunchKernelGGL(k_b_put, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n, nr, r, a, add,
+                       Bk);
+}
+
+void block_get(hipStream_t st, long n, int nr, int r, const double *Bk, double *a)
+{
+    hipLaunchKernelGGL(k_b_get, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n, nr, r, Bk, a);
+}
+
+} // namespace hmfem

@@ -302,6 +302,95 @@ extern "C" int hmfem_solve_structured_device(hmfem_context *c, int nx, int ny, i
     return out->status == HMFEM_OK ? HMFEM_OK : out->status;
 }
 
+static char *ctx_alloc_block(void *c, size_t bytes) { return (char *)((hmfem_context *)c)->get("block_arena", bytes); }
+
+extern "C" int hmfem_reservoir_block_device(hmfem_context *c, const hmfem_grid *g, const double *Kx,
+                                            const double *Ky, const double *Kz, int nrhs,
+                                            const double *const *bval, const double *const *q, double *const *p,
+                                            ELE *const *v, const hmfem_options *opt_in, hmfem_block_status *st,
+                                            int *iterations, double *rel_residual, double *div_max)
+{
+    if (!c || !grid_ok(g) || !Kx || !Ky || !Kz || nrhs < 1 || nrhs > 8 || !bval || !p) return HMFEM_ERR_ARG;
+    for (int r = 0; r < nrhs; r++) {
+        if (!p[r]) return HMFEM_ERR_ARG;
+        for (int f = 0; f < 6; f++)
+            if (!bval[6 * r + f]) return HMFEM_ERR_ARG;
+    }
+    hmfem_options opt;
+    if (opt_in) opt = *opt_in;
+    else hmfem_default_options(&opt);
+    Dims d{g->nx, g->ny, g->nz};
+    const long n = d.n();
+    const int nr = nrhs <= 2 ? 2 : (nrhs <= 4 ? 4 : 8);
+    Stencil<double> A;
+    A.e = (double *)c->get("e", n * 8);
+    A.wx = (double *)c->get("wx", n * 8);
+    A.wy = (double *)c->get("wy", n * 8);
+    A.wz = (double *)c->get("wz", n * 8);
+    double *b = (double *)c->get("b", n * 8);
+    double *Bk = (double *)c->get("block_b", (size_t)n * nr * 8);
+    double *Xk = (double *)c->get("block_x", (size_t)n * nr * 8);
+    if (!A.e || !A.wx || !A.wy || !A.wz || !b || !Bk || !Xk) return HMFEM_ERR_NOMEM;
+    hmfem_block_status tmp{};
+    hmfem_block_status *out = st ? st : &tmp;
+    *out = hmfem_block_status{};
+    HMFEM_CHECK(hipEventRecord(c->ev[4], c->st));
+    for (int r = 0; r < nr; r++) {
+        if (r < nrhs) {
+            int rc = launch_assemble(c->st, g, Kx, Ky, Kz, nullptr, nullptr, nullptr, nullptr, bval + 6 * r,
+                                     r == 0 ? A.e : nullptr, r == 0 ? A.wx : nullptr, r == 0 ? A.wy : nullptr,
+                                     r == 0 ? A.wz : nullptr, b, nullptr, nullptr);
+            if (rc) return rc;
+            block_put(c->st, n, nr, r, b, q ? q[r] : nullptr, Bk);
+            block_put(c->st, n, nr, r, p[r], nullptr, Xk);
+        } else {
+            block_put(c->st, n, nr, r, nullptr, nullptr, Bk);
+            block_put(c->st, n, nr, r, nullptr, nullptr, Xk);
+        }
+    }
+    HMFEM_CHECK(hipGetLastError());
+    HMFEM_CHECK(hipEventRecord(c->ev[5], c->st));
+    int it[8] = {0}, stat[8] = {0};
+    double rel[8] = {0};
+    double setup_ms = 0, solve_ms = 0;
+    int rc = block_solve(c->st, d, A, nr, Bk, Xk, opt, ctx_alloc_block, c, c->lds_limit, c->stage, it, rel, stat,
+                         &setup_ms, &solve_ms);
+    if (rc) return rc;
+    for (int r = 0; r < nrhs; r++) block_get(c->st, n, nr, r, Xk, p[r]);
+    HMFEM_CHECK(hipGetLastError());
+    float ta = 0.f;
+    HMFEM_CHECK(hipEventSynchronize(c->ev[5]));
+    (void)hipEventElapsedTime(&ta, c->ev[4], c->ev[5]);
+    out->t_assemble_ms = ta;
+    out->t_setup_ms = setup_ms;
+    out->t_solve_ms = solve_ms;
+    out->status = HMFEM_OK;
+    for (int r = 0; r < nrhs; r++) {
+        if (iterations) iterations[r] = it[r];
+        if (rel_residual) rel_residual[r] = rel[r];
+        if (it[r] > out->max_iterations) out->max_iterations = it[r];
+        if (stat[r] != HMFEM_OK && out->status == HMFEM_OK) out->status = stat[r];
+    }
+    if (v) {
+        HMFEM_CHECK(hipEventRecord(c->ev[6], c->st));
+        for (int r = 0; r < nrhs; r++) {
+            if (!v[r]) continue;
+            double dmin = 0, dmax = 0;
+            rc = hmfem_flux_device(c, g, Kx, Ky, Kz, bval + 6 * r, p[r], v[r], &dmin, &dmax);
+            if (rc) return rc;
+            if (div_max) div_max[r] = dmax;
+        }
+        HMFEM_CHECK(hipEventRecord(c->ev[7], c->st));
+        HMFEM_CHECK(hipEventSynchronize(c->ev[7]));
+        float tf = 0.f;
+        (void)hipEventElapsedTime(&tf, c->ev[6], c->ev[7]);
+        out->t_flux_ms = tf;
+    }
+    HMFEM_CHECK(hipStreamSynchronize(c->st));
+    out->footprint_bytes = (double)block_bytes(d, nr, opt) + 5.0 * n * 8 + 2.0 * n * nr * 8;
+    return out->status;
+}
+
 static char *ctx_alloc(void *c, size_t bytes) { return (char *)((hmfem_context *)c)->get("batch_arena", bytes); }
 
 extern "C" int hmfem_reservoir_batch_device(hmfem_context *c, int ninst, const int *dims, const double *lengths,

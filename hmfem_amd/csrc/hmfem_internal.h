@@ -160,6 +160,15 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
 int batch_grids(hipStream_t st, int ninst, const int *dims, const double *lengths, double mu, const int bc[6],
                 std::vector<char> &stage, void *d_grids, int *d_bmap, int *nblk_out);
 size_t batch_grid_bytes(int ninst);
+
+// block solve of several right-hand sides (hmfem_block.hip): B, X interleaved [n][nr]
+int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const double *Bk, double *Xk,
+                const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *actx, long lds_limit,
+                std::vector<char> &stage, int *iterations, double *rel_residual, int *status, double *setup_ms,
+                double *solve_ms);
+size_t block_bytes(Dims fine, int nr, const hmfem_options &opt);
+void block_put(hipStream_t st, long n, int nr, int r, const double *a, const double *add, double *Bk);
+void block_get(hipStream_t st, long n, int nr, int r, const double *Bk, double *a);
 int launch_batch_assemble(hipStream_t st, const void *d_grids, const int *d_bmap, int nblk, const double *Kx,
                           const double *Ky, const double *Kz, const double *const bval[6], double *e, double *wx,
                           double *wy, double *wz, double *b);

@@ -28,7 +28,7 @@
 // Precision: coefficients are accumulated in FP64; the V-cycle runs on an
 // FP32 copy (or FP64 for the plain-FP64 solve); CG vectors, dots, alpha and
 // beta are FP64 and stay on the device.
-#include "hmfem_stencil.h"
+#include "hmfem_hier.h"
 
 #include <string.h>
 
@@ -247,7 +247,7 @@ k_final_sweep_dots(OpArgs<T> o, int fx, int fy, int cx, int cy, int fz, const T 
 //
 // GS update of a cell: u_s = (f_s + sum_nb w u_nb) / d_s with d_s = e_s + sum w
 // (the inverse diagonal di is precomputed), i.e. (A u)_s = f_s afterwards.
-constexpr int kGsTx = 32, kGsTy = 8, kGsTz = 8;
+constexpr int kGsTz = 8;
 static_assert(kGsTx * kGsTy == kBlock, "one thread per tile column");
 
 // The x/y ring of one tile plane: rows hy = 0 and hy = kGsTy + 1 (kGsTx + 2
@@ -462,16 +462,6 @@ k_dense_dots(int n, const double *__restrict__ inv, const double *__restrict__ f
 enum TailOp { OP_SMOOTH0 = 0, OP_JACOBI, OP_RESTRICT, OP_PROLONG_JACOBI, OP_DENSE, OP_RESID, OP_SAVE, OP_ADDSAVED,
               OP_FINAL };
 
-template <typename T>
-struct TailLevel {
-    OpArgs<T> o;
-    T *f, *u, *t, *r, *w;
-    int n, fx, fy, fz;
-    int lds_off;   // byte offset of this level's copy in LDS (LDS mode), -1 otherwise
-};
-
-constexpr int kTailThreads = 1024;
-constexpr int kMaxTailLevels = 16;
 
 // Executes ops[0..nops) in order, each over all cells of its level with the
 // 1024 threads of one workgroup, and a barrier after each op. Every thread
@@ -631,12 +621,12 @@ k_tail(const TailLevel<T> *__restrict__ Lg, const int4 *__restrict__ ops, int no
 // element offsets from the LDS arena (so every access is a direct LDS access),
 // and the op list is copied into LDS first. Layout of level l at
 // A + base_l: e, wx, wy, wz, di, f, u, t, r, w (n_l values each).
-constexpr int kTailLdsOps = 256;
 
 template <typename T>
 __global__ void __launch_bounds__(kTailThreads)
 k_tail_lds(const TailLevel<T> *__restrict__ Lg, const int4 *__restrict__ opsg, int nops,
-           const double *__restrict__ invg, int nc, T omega, T ocf, int nlev, int inv_off, const int *done)
+           const double *__restrict__ invg, int nc, T omega, T ocf, int nlev, int inv_off, const T *ftop, T *utop,
+           int stride, const int *done)
 {
     using C = T;
     extern __shared__ char lds[];
@@ -663,7 +653,7 @@ k_tail_lds(const TailLevel<T> *__restrict__ Lg, const int4 *__restrict__ opsg, i
         for (int s = tid; s < n; s += kTailThreads) {
             b[s] = L.o.e[s]; b[n + s] = L.o.wx[s]; b[2 * n + s] = L.o.wy[s];
             b[3 * n + s] = L.o.wz[s]; b[4 * n + s] = L.o.di[s];
-            if (l == 0) b[5 * n + s] = L.f[s];
+            if (l == 0) b[5 * n + s] = ftop ? ftop[(long)s * stride] : L.f[s];
         }
     }
     __syncthreads();
@@ -790,7 +780,10 @@ k_tail_lds(const TailLevel<T> *__restrict__ Lg, const int4 *__restrict__ opsg, i
     {
         const TailLevel<T> &G0 = Lg[0];
         const T *src = A + dsc[0][0] + 6 * dsc[0][1];   // OP_FINAL left the result in u
-        for (int s = tid; s < dsc[0][1]; s += kTailThreads) G0.u[s] = src[s];
+        for (int s = tid; s < dsc[0][1]; s += kTailThreads) {
+            if (utop) utop[(long)s * stride] = src[s];
+            else G0.u[s] = src[s];
+        }
     }
 }
 
@@ -1050,38 +1043,6 @@ k_cg_update(int n, const double *__restrict__ s0, const double *__restrict__ s1,
 
 // ---------------------------------------------------------------- host side
 
-struct Hierarchy {
-    int nlev = 0;
-    Level lev[kMaxLevels];
-    double *inv = nullptr;   // dense inverse of the coarsest level
-    int nc = 0;
-    int prec = HMFEM_PREC_FP32;
-    int tail0 = -1;          // first level run by the tail kernel (-1: none)
-    void *tail_levels = nullptr;
-    int4 *tail_ops = nullptr;
-    int tail_nops = 0;
-    int tail_lds_bytes = 0;    // dynamic LDS of the tail kernel (0: global mode)
-    int tail_lds_inv = -1;     // LDS offset of the dense inverse (-1: global mode)
-};
-
-static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
-
-// Carve the device memory of one hierarchy out of a single arena.
-struct Arena {
-    char *p;
-    size_t used, cap;
-    template <typename T>
-    T *take(size_t count)
-    {
-        size_t b = align256(count * sizeof(T));
-        if (used + b > cap) return nullptr;
-        T *r = (T *)(p + used);
-        used += b;
-        return r;
-    }
-};
-
-constexpr int kMaxTailOps = 4096;
 
 static void plan_levels(Dims fine, int coarse_max, Level *lev, int &nlev)
 {
@@ -1148,24 +1109,6 @@ static void gen_tail(std::vector<int4> &ops, int l, int fsel, int last, int nu, 
     for (int q = 1; q < nu; q++) ops.push_back(make_int4(OP_JACOBI, l, fsel, 0));
 }
 
-template <typename T>
-static OpArgs<T> opargs(const Level &L);
-template <>
-OpArgs<float> opargs<float>(const Level &L)
-{
-    OpArgs<float> o;
-    o.e = L.s32.e; o.wx = L.s32.wx; o.wy = L.s32.wy; o.wz = L.s32.wz; o.di = L.s32.di;
-    o.nx = L.d.nx; o.ny = L.d.ny; o.nz = L.d.nz;
-    return o;
-}
-template <>
-OpArgs<double> opargs<double>(const Level &L)
-{
-    OpArgs<double> o;
-    o.e = L.s64.e; o.wx = L.s64.wx; o.wy = L.s64.wy; o.wz = L.s64.wz; o.di = L.s64.di;
-    o.nx = L.d.nx; o.ny = L.d.ny; o.nz = L.d.nz;
-    return o;
-}
 
 template <typename T>
 static int upload_tail(hipStream_t st, Hierarchy &H, const hmfem_options &opt, std::vector<char> &stage,
@@ -1330,9 +1273,6 @@ static void launch_restrict(hipStream_t st, Prof *P, int cls, const Level &L, co
     }
 }
 
-struct Chunking {
-    int G, chunk;
-};
 
 static Chunking chunking(long n)
 {
@@ -1372,7 +1312,7 @@ static T *vcycle_coarse(hipStream_t st, Hierarchy &H, int l, const CycleParams &
     if (H.tail0 > 0 && l == H.tail0 && H.tail_lds_inv >= 0 && H.tail_nops <= kTailLdsOps) {
         PROF_LAUNCH(P, HMFEM_PROF_COARSE, k_tail_lds<T>, dim3(1), dim3(kTailThreads), H.tail_lds_bytes, st,
                     (const TailLevel<T> *)H.tail_levels, H.tail_ops, H.tail_nops, H.inv, H.nc, (T)cp.omega,
-                    (T)cp.ocf, H.nlev - H.tail0, H.tail_lds_inv, done);
+                    (T)cp.ocf, H.nlev - H.tail0, H.tail_lds_inv, (const T *)nullptr, (T *)nullptr, 1, done);
         return u;
     }
     if (H.tail0 > 0 && l == H.tail0) {
@@ -1652,6 +1592,17 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         per += n0 * (32 + 8 + vs + 8 + 8) + n0 * 48 + n0 * 8;
         out->solve_bytes = per * (res.iterations + 1);
     }
+    return HMFEM_OK;
+}
+
+int launch_tail_strided(hipStream_t st, const Hierarchy &H, double omega, double ocf, const float *ftop,
+                        float *utop, int stride, const int *done)
+{
+    if (H.prec != HMFEM_PREC_FP32 || H.tail0 <= 0 || H.tail_lds_inv < 0 || H.tail_nops > kTailLdsOps)
+        return HMFEM_ERR_ARG;
+    hipLaunchKernelGGL(k_tail_lds<float>, dim3(1), dim3(kTailThreads), H.tail_lds_bytes, st,
+                       (const TailLevel<float> *)H.tail_levels, H.tail_ops, H.tail_nops, H.inv, H.nc, (float)omega,
+                       (float)ocf, H.nlev - H.tail0, H.tail_lds_inv, ftop, utop, stride, done);
     return HMFEM_OK;
 }
 

@@ -66,3 +66,63 @@ class Stream:
 
     def choice(self, items):
         return items[min(int(self.uniform() * len(items)), len(items) - 1)]
+
+
+# ------------------------------------------------------------------ block right-hand sides
+
+def face_index_sizes(nx, ny, nz):
+    """Lengths of the six face value arrays (faces l, r, u, d, t, b): face l/r
+    is indexed k*ny + j, u/d k*nx + i, t/b j*nx + i (set_source_term.c)."""
+    return (ny * nz, ny * nz, nx * nz, nx * nz, nx * ny, nx * ny)
+
+
+def dirichlet_pair_faces(nx, ny, nz, axis, forward):
+    """Face values of an all-Dirichlet box with p = 1 on one face of `axis`
+    (0 x, 1 y, 2 z; the low face if forward, else the high face), p = 0 on the
+    opposite face, and the linear interpolant of the two on the four lateral
+    faces, evaluated at the face centres (cell-centre coordinates along the
+    axis)."""
+    n_ax = (nx, ny, nz)[axis]
+    frac = (np.arange(n_ax) + 0.5) / n_ax
+    prof = 1.0 - frac if forward else frac            # value at cell index along the axis
+    ii, jj, kk = np.arange(nx), np.arange(ny), np.arange(nz)
+    # lateral faces: l/r over (j, k), u/d over (i, k), t/b over (i, j)
+    grids = {
+        "lr": np.meshgrid(jj, kk, indexing="xy"),     # arrays [k][j] -> flattened k*ny + j
+        "ud": np.meshgrid(ii, kk, indexing="xy"),     # [k][i]
+        "tb": np.meshgrid(ii, jj, indexing="xy"),     # [j][i]
+    }
+    coord = {  # which face-array coordinate is the axis' index, per face family
+        "lr": {1: 0, 2: 1}, "ud": {0: 0, 2: 1}, "tb": {0: 0, 1: 1},
+    }
+    faces = [None] * 6
+    fam_axis = {"lr": 0, "ud": 1, "tb": 2}
+    fam_faces = {"lr": (0, 1), "ud": (3, 2), "tb": (5, 4)}     # (low face, high face)
+    for fam, ax in fam_axis.items():
+        lo, hi = fam_faces[fam]
+        size = face_index_sizes(nx, ny, nz)[lo]
+        if ax == axis:
+            faces[lo] = np.full(size, 1.0 if forward else 0.0)
+            faces[hi] = np.full(size, 0.0 if forward else 1.0)
+        else:
+            a, b = grids[fam]
+            idx = (a, b)[coord[fam][axis]]
+            v = prof[idx].ravel()
+            faces[lo] = v.copy()
+            faces[hi] = v.copy()
+    return [np.ascontiguousarray(f, np.float64) for f in faces]
+
+
+def source_sink_pair(nx, ny, nz, seed, volume):
+    """b-only source term of a unit source and a unit sink in two distinct
+    seeded cells: +1/volume and -1/volume (q/V)."""
+    st = Stream(seed, 0)
+    n = nx * ny * nz
+    a = int(st.uniform() * n) % n
+    b = a
+    while b == a:
+        b = int(st.uniform() * n) % n
+    q = np.zeros(n)
+    q[a] = 1.0 / volume
+    q[b] = -1.0 / volume
+    return q

@@ -72,6 +72,14 @@ class Status(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BlockStatus(ctypes.Structure):
+    _fields_ = [("status", c_int), ("max_iterations", c_int), ("t_assemble_ms", c_dbl), ("t_setup_ms", c_dbl),
+                ("t_solve_ms", c_dbl), ("t_flux_ms", c_dbl), ("footprint_bytes", c_dbl)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class BatchStatus(ctypes.Structure):
     _fields_ = [("status", c_int), ("max_iterations", c_int), ("t_assemble_ms", c_dbl), ("t_setup_ms", c_dbl),
                 ("t_solve_ms", c_dbl), ("t_flux_ms", c_dbl)]
@@ -118,6 +126,8 @@ def lib():
         L.hmfem_profile_reset.restype = None
         L.hmfem_reservoir_batch_device.argtypes = [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp,
                                                    c_vp, c_vp, P(Options), P(BatchStatus), c_vp, c_vp, c_vp, c_vp]
+        L.hmfem_reservoir_block_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
+                                                   c_vp, P(Options), P(BlockStatus), c_vp, c_vp, c_vp]
         L.hmfem_kernel_bytes_per_cell.argtypes = [c_int, c_int]
         L.hmfem_kernel_bytes_per_cell.restype = c_dbl
         _typed = True
@@ -128,8 +138,8 @@ EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", 
             "prod_mat_vec", "hmfem_default_options", "hmfem_create", "hmfem_destroy", "hmfem_stream",
             "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device",
             "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
-            "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device", "iter_amg",
-            "boundary")
+            "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device",
+            "hmfem_reservoir_block_device", "iter_amg", "boundary")
 
 
 def _p(a):
@@ -335,6 +345,30 @@ class Context:
         if rc < 0:
             raise RuntimeError("hmfem_reservoir_batch_device failed: %d" % rc)
         return st, dict(iterations=its, rel_residual=rr, done_ms=dm, div_max=dv)
+
+    def reservoir_block(self, g, Kx, Ky, Kz, bvals, p, v=None, q=None, options=None):
+        """Solve len(p) right-hand sides of one grid together. bvals[r]: six
+        device face arrays of right-hand side r; p[r]: device pressure (initial
+        guess in, solution out); v[r]: device ELE fluxes or None; q[r]: device
+        array added to b, or None. Returns (BlockStatus, per-RHS dict)."""
+        nr = len(p)
+        fa = (c_vp * (6 * nr))(*[_dptr(t) for bv in bvals for t in bv])
+        pa = (c_vp * nr)(*[_dptr(t) for t in p])
+        va = None if v is None else (c_vp * nr)(*[_dptr(t) for t in v])
+        qa = None if q is None else (c_vp * nr)(*[_dptr(t) for t in q])
+        its = np.zeros(nr, np.int32)
+        rr = np.zeros(nr)
+        dv = np.zeros(nr)
+        st = BlockStatus()
+        o = options if options is not None else default_options()
+        _order_after_torch(Kx, Ky, Kz, *p, *(v or []), *(t for t in (q or []) if t is not None),
+                           *[t for bv in bvals for t in bv])
+        rc = lib().hmfem_reservoir_block_device(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz), nr, fa,
+                                                qa, pa, va, ctypes.byref(o), ctypes.byref(st), _p(its), _p(rr),
+                                                _p(dv))
+        if rc < 0:
+            raise RuntimeError("hmfem_reservoir_block_device failed: %d" % rc)
+        return st, dict(iterations=its, rel_residual=rr, div_max=dv)
 
     def profile_enable(self, max_launches=20000):
         rc = lib().hmfem_profile_enable(self.h, max_launches)

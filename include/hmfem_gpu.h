@@ -198,6 +198,32 @@ int hmfem_reservoir_batch_device(hmfem_context *ctx, int ninst, const int *dims,
                                  const hmfem_options *opt, hmfem_batch_status *st, int *iterations,
                                  double *rel_residual, double *done_ms, double *div_max);
 
+/* Block solve: nrhs (1..8) right-hand sides of one operator, solved
+   together (one hierarchy; every kernel reads the coefficients once for all
+   right-hand sides; vectors interleaved [cell][right-hand side] inside).
+   Right-hand side r has the face values bval[6 r + f] (device, Bound layout,
+   faces l, r, u, d, t, b; the face TYPES, hence the matrix, are g->bc_type
+   for all) plus, if q[r] is not NULL, the device array q[r] (n values) added
+   to b (sources entered in b only, the matrix unchanged). p[r] (device, n) is
+   the initial guess in and the pressure out; v[r] (device ELE[n], or NULL)
+   receives the fluxes. Per right-hand side: iterations, rel_residual and
+   div_max (host arrays of nrhs, or NULL). Uses the FP32 hierarchy and the
+   block Gauss-Seidel smoother whatever opt says; counts of right-hand sides
+   other than 2, 4 or 8 are padded with zero right-hand sides. No counterpart
+   in the reference's API: it is the interleaved multi-RHS solve of
+   generic_operator.h:335-391 behind the reservoir() step. */
+typedef struct {
+    int status;              /* HMFEM_OK if every right-hand side converged */
+    int max_iterations;
+    double t_assemble_ms, t_setup_ms, t_solve_ms, t_flux_ms;
+    double footprint_bytes;  /* device memory the call used */
+} hmfem_block_status;
+
+int hmfem_reservoir_block_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
+                                 const double *Kz, int nrhs, const double *const *bval, const double *const *q,
+                                 double *const *p, ELE *const *v, const hmfem_options *opt,
+                                 hmfem_block_status *st, int *iterations, double *rel_residual, double *div_max);
+
 /* Per-kernel timing: with profiling enabled, every fine-level and CG launch
    of the following calls is bracketed by HIP events on the context's stream
    (capacity: max_launches launches per call). hmfem_profile_read returns, per

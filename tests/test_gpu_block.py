@@ -1,0 +1,148 @@
+"""GPU parity of the block solve (several right-hand sides of one operator
+through hmfem_reservoir_block_device).
+
+Bars: the closed-form linear pressure of a homogeneous all-Dirichlet box for
+each face-pair right-hand side; against the oracle's AMG-PCG solution at
+1e-12 within 1e-8 relative L2 (and fluxes bit for bit); against the
+single-right-hand-side GPU solve within 1e-9 at 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from gpu_helpers import make_problem, oracle_solution, rel_l2, torch_cuda
+from hmfem_amd import fields as F
+from hmfem_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+PRESSURE_TOL = 1e-8
+ALL_DIRICHLET = (0, 0, 0, 0, 0, 0)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    return torch_cuda()
+
+
+@pytest.fixture(scope="module")
+def ctx(torch):
+    c = S.Context()
+    yield c
+    c.close()
+
+
+def _say(*a):
+    print("[block]", *a, flush=True)
+
+
+def _rhs_set(nx, ny, nz, nrhs, seed=7):
+    """Right-hand sides of the block workload: face pairs along x, y, z in both
+    orientations, then zero faces with a seeded source/sink pair."""
+    h3 = 1.0 / (nx * ny * nz)
+    out = []
+    for r in range(nrhs):
+        if r < 6:
+            out.append((F.dirichlet_pair_faces(nx, ny, nz, r // 2, r % 2 == 0), None))
+        else:
+            faces = [np.zeros(s) for s in F.face_index_sizes(nx, ny, nz)]
+            out.append((faces, F.source_sink_pair(nx, ny, nz, seed + r, h3)))
+    return out
+
+
+def _oracle_problem(dims, K, faces):
+    P = O.Problem(dims, K, ALL_DIRICHLET, (0, 0, 0, 0, 0, 0))
+    for f in range(6):
+        P.B[f][:faces[f].size] = faces[f]
+    return P
+
+
+def _run_block(ctx, torch, dims, K, rhs, rtol, with_flux=False):
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    Kd = t(K)
+    bvals = [[t(a) for a in faces] for faces, _ in rhs]
+    qs = [None if q is None else t(q) for _, q in rhs]
+    ps = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in rhs]
+    vs = [torch.zeros(n * 7, dtype=torch.float64, device="cuda") for _ in rhs] if with_flux else None
+    g = S.Context.grid(nx, ny, nz, None, 1.0, ALL_DIRICHLET)
+    st, per = ctx.reservoir_block(g, Kd, Kd, Kd, bvals, ps, vs, qs if any(q is not None for q in qs) else None,
+                                  S.default_options(rtol=rtol))
+    return st, per, [p.cpu().numpy() for p in ps], vs
+
+
+def test_block_closed_form(ctx, torch):
+    """Homogeneous K, all faces Dirichlet: each face-pair right-hand side has
+    the exact linear pressure p = 1 - (x+1/2)h along its axis (or its mirror)."""
+    dims = (16, 12, 10)
+    nx, ny, nz = dims
+    K = F.homogeneous(nx, ny, nz, 1e-14)
+    rhs = _rhs_set(nx, ny, nz, 6)
+    _say("closed form", dims)
+    st, per, ps, _ = _run_block(ctx, torch, dims, K, rhs, 1e-11)
+    assert st.status == S.OK, (st.as_dict(), per)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    coords = [(i.ravel() + 0.5) / nx, (j.ravel() + 0.5) / ny, (k.ravel() + 0.5) / nz]
+    for r in range(6):
+        c = coords[r // 2]
+        exact = 1.0 - c if r % 2 == 0 else c
+        assert rel_l2(ps[r], exact) < 1e-9, r
+        assert per["iterations"][r] > 0
+
+
+@pytest.mark.parametrize("nrhs", [3, 8])
+def test_block_vs_oracle(nrhs, ctx, torch):
+    """Lognormal field with conduits; every right-hand side against the oracle's
+    solution of the same system; fluxes of the block pressure bit for bit."""
+    dims = (24, 20, 18)
+    nx, ny, nz = dims
+    K = F.conduits(F.lognormal(nx, ny, nz, np.log(1e-14), 1.5, 3.0, 31), nx, ny, nz, 3, 32)
+    rhs = _rhs_set(nx, ny, nz, nrhs) if nrhs == 8 else [_rhs_set(nx, ny, nz, 8)[r] for r in (0, 3, 6)]
+    _say("vs oracle", dims, "nrhs", nrhs)
+    st, per, ps, vs = _run_block(ctx, torch, dims, K, rhs, 1e-12, with_flux=True)
+    assert st.status == S.OK, (st.as_dict(), per)
+    for r, (faces, q) in enumerate(rhs):
+        P = _oracle_problem(dims, K, faces)
+        perm, indcol, b = O.assemble(P)
+        if q is not None:
+            b = b + q
+        cnt, col, ele = O.build_csr(perm, indcol)
+        xo, _ = O.AMG(cnt, col, ele).pcg(b, eps=1e-12)
+        assert rel_l2(ps[r], xo) < PRESSURE_TOL, r
+        ov = O.fluxos(P, ps[r], perm)
+        vv = vs[r].cpu().numpy().view(S.ELE)
+        for f in ("u", "d", "l", "r", "t", "b"):
+            assert np.array_equal(vv[f].view(np.uint64), ov[f].view(np.uint64)), (r, f)
+
+
+def test_block_vs_single(ctx, torch):
+    """64 x 48 x 40, eight right-hand sides: each equals the single-right-hand-side
+    GPU solve of its own system."""
+    dims = (64, 48, 40)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    K = F.conduits(F.lognormal(nx, ny, nz, np.log(1e-14), 2.0, 6.0, 41), nx, ny, nz, 6, 42)
+    rhs = _rhs_set(nx, ny, nz, 8)
+    _say("vs single", dims)
+    st, per, ps, _ = _run_block(ctx, torch, dims, K, rhs, 1e-12)
+    assert st.status == S.OK, (st.as_dict(), per)
+    assert st.footprint_bytes > 0
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    Kd = t(K)
+    g = S.Context.grid(nx, ny, nz, None, 1.0, ALL_DIRICHLET)
+    o = S.default_options(rtol=1e-12)
+    for r, (faces, q) in enumerate(rhs):
+        bval = [t(a) for a in faces]
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        if q is None:
+            s1 = ctx.reservoir(g, Kd, Kd, Kd, bval, x, None, o)
+        else:
+            perm = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
+            indcol = torch.zeros(n * 7, dtype=torch.int32, device="cuda")
+            b = torch.zeros(n, dtype=torch.float64, device="cuda")
+            ctx.assemble(g, Kd, Kd, Kd, bval, b, perm, indcol)
+            b += t(q)
+            torch.cuda.synchronize()
+            s1 = ctx.solve_structured(nx, ny, nz, perm, indcol, b, x, o)
+        assert s1.status == S.OK
+        assert rel_l2(ps[r], x.cpu().numpy()) < 1e-9, r
