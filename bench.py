@@ -35,6 +35,10 @@ WORKLOADS = {
                label="16^3 homogeneous K=1e-14, x-Dirichlet, rtol 1e-10"),
     "c3": dict(dims=(256, 256, 256), kind="karst", conduits=20,
                label="256^3 K=1e-14 + 20 karst conduits x10^U(4,6), x-Dirichlet, rtol 1e-8"),
+    "c5": dict(dims=(512, 512, 512), block=8, sigma=3.0, corr=16.0, conduits=50,
+               label="512^3 lognormal K sigma=3 corr 16 cells + 50 karst conduits, all six faces Dirichlet; 8 "
+                     "right-hand sides solved as one block: p=1/p=0 face pairs along x, y, z in both orientations "
+                     "(linear lateral faces) and two zero-face source/sink pairs; rtol 1e-8"),
     "c4": dict(batch=96, edges=(32, 48, 64, 80, 96),
                label="96 independent instances in one batch, edges from {32,48,64,80,96} per axis; even: lognormal "
                      "sigma U[0.5,2.5] corr 8; odd: K=1e-14 + 1-5 conduits; x-Dirichlet, rtol 1e-8"),
@@ -76,7 +80,7 @@ def make_K(w):
     return K
 
 
-def cpu_baseline(K, dims, budget_s):
+def cpu_baseline(K, dims, budget_s, bc_type=(0, 0, 1, 1, 1, 1), bc_val=(1.0, 0.0, 0.0, 0.0, 0.0, 0.0)):
     """Time the reference build (oracle/_ref) on the same problem: one
     reservoir() step single-threaded. Falls back to the oracle restatement."""
     nx, ny, nz = dims
@@ -88,8 +92,8 @@ def cpu_baseline(K, dims, budget_s):
             with open(inp, "wb") as f:
                 f.write(struct.pack("<3i", nx, ny, nz))
                 f.write(struct.pack("<5d", 1.0, 1.0, 1.0, 1.0, 1e-12))
-                f.write(struct.pack("<6i", 0, 0, 1, 1, 1, 1))
-                f.write(struct.pack("<6d", 1.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+                f.write(struct.pack("<6i", *bc_type))
+                f.write(struct.pack("<6d", *bc_val))
                 f.write(struct.pack("<i", 0))
                 kb = np.ascontiguousarray(K, "<f8").tobytes()
                 f.write(kb); f.write(kb); f.write(kb)
@@ -106,7 +110,7 @@ def cpu_baseline(K, dims, budget_s):
                 "cpu": platform.processor() or platform.machine()}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as O
-    P = O.Problem(dims, K, (0, 0, 1, 1, 1, 1), (1, 0, 0, 0, 0, 0))
+    P = O.Problem(dims, K, bc_type, bc_val)
     t0 = time.time()
     perm, indcol, b = O.assemble(P)
     cnt, col, ele = O.build_csr(perm, indcol)
@@ -192,11 +196,139 @@ def run_batch(args, w, torch, S):
     print(json.dumps(out))
 
 
+def block_rhs(nx, ny, nz, nrhs):
+    """The block workload's right-hand sides: (six face arrays, b-only source or None)."""
+    from hmfem_amd import fields
+    out = []
+    for r in range(nrhs):
+        if r < 6:
+            out.append((fields.dirichlet_pair_faces(nx, ny, nz, r // 2, r % 2 == 0), None))
+        else:
+            faces = [np.zeros(sz) for sz in fields.face_index_sizes(nx, ny, nz)]
+            out.append((faces, fields.source_sink_pair(nx, ny, nz, SEED + r, 1.0 / (nx * ny * nz))))
+    return out
+
+
+def block_bytes_per_iteration(dims, nr):
+    """Algorithmic HBM bytes of one block CG iteration on the fine level
+    (coefficients once, nr interleaved right-hand sides): SpMV (A64, s, z in;
+    s, v out), update (x, s, r, v in; x, r out), pre-smoothing (wx, wy, wz, di,
+    r in; u out), residual-restriction (A32, r, u in), post-smoothing (wx, wy,
+    wz, di, r, u, v in; z out). Coarse levels add about 1/7 and are left out."""
+    n = dims[0] * dims[1] * dims[2]
+    per = (32 + nr * (8 + 4 + 8 + 8)) + nr * 48 + (16 + nr * (8 + 4)) + (16 + nr * (8 + 4)) + \
+        (16 + nr * (8 + 4 + 8 + 4))
+    return n * per
+
+
+def run_block(args, w, torch, S):
+    from hmfem_amd import fields
+    nx, ny, nz = w["dims"]
+    n = nx * ny * nz
+    nr = w["block"]
+    log("workload c5", w["dims"], "generating K")
+    K = fields.lognormal(nx, ny, nz, np.log(1e-14), w["sigma"], w["corr"], SEED)
+    fields.conduits(K, nx, ny, nz, w["conduits"], SEED + 1)
+    log("K ready; right-hand sides")
+    rhs = block_rhs(nx, ny, nz, nr)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    Kd = dev(K)
+    bvals = [[dev(a) for a in faces] for faces, _ in rhs]
+    qs = [None if q is None else dev(q) for _, q in rhs]
+    ps = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(nr)]
+    vs = [torch.zeros(n * 7, dtype=torch.float64, device="cuda") for _ in range(nr)]
+    g = S.Context.grid(nx, ny, nz, None, 1.0, (0, 0, 0, 0, 0, 0))
+    ctx = S.Context()
+    solver_stream = torch.cuda.ExternalStream(ctx.stream)
+    opt = S.default_options(rtol=1e-8)
+
+    def step():
+        with torch.cuda.stream(solver_stream):
+            for p in ps:
+                p.zero_()
+        return ctx.reservoir_block(g, Kd, Kd, Kd, bvals, ps, vs, qs, opt)
+
+    for q in range(args.warmup):
+        st, per = step()
+        log("warmup block step %d: iterations %s, %.0f ms" % (q, per["iterations"].tolist(), st.t_solve_ms))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = []
+    for q in range(args.steps):
+        res.append(step())
+        log("timed block step %d done" % q)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st, per = res[-1]
+    value = nr * args.steps / elapsed
+    its = per["iterations"]
+    gbs = block_bytes_per_iteration(w["dims"], nr) * (int(its.max()) + 1) / (st.t_solve_ms * 1e-3) / 1e9
+    log("block: %.3f s per step, iterations %s" % (elapsed / args.steps, its.tolist()))
+
+    # the same eight systems through the single-right-hand-side path, one after the other
+    separate = None
+    if not args.no_fp64:
+        o1 = S.default_options(rtol=1e-8)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        perm = indcol = b = None
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        its1 = []
+        for r, (bv, q) in enumerate(zip(bvals, qs)):
+            with torch.cuda.stream(solver_stream):
+                x.zero_()
+            if q is None:
+                s1 = ctx.reservoir(g, Kd, Kd, Kd, bv, x, vs[r], o1)
+            else:
+                if perm is None:
+                    perm = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
+                    indcol = torch.zeros(n * 7, dtype=torch.int32, device="cuda")
+                    b = torch.zeros(n, dtype=torch.float64, device="cuda")
+                ctx.assemble(g, Kd, Kd, Kd, bv, b, perm, indcol)
+                b += q
+                s1 = ctx.solve_structured(nx, ny, nz, perm, indcol, b, x, o1)
+                ctx.flux(g, Kd, Kd, Kd, bv, x, vs[r])
+            its1.append(s1.iterations)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter() - ta
+        separate = {"value": nr / t1, "unit": "solves/s", "iterations": its1,
+                    "speedup_block_over_separate": value / (nr / t1)}
+        log("separate: %.3f s" % t1)
+
+    cpu = None
+    if not args.no_cpu:
+        # the reference on a 128^3 member of the same family (sigma 3, correlation
+        # scaled to 4 cells, 50/64 conduits rounded up), extrapolated per unknown
+        m = 128
+        Ks = fields.lognormal(m, m, m, np.log(1e-14), w["sigma"], w["corr"] * m / nx, SEED)
+        fields.conduits(Ks, m, m, m, max(1, (w["conduits"] * m * m) // (nx * nx)), SEED + 1)
+        r = cpu_baseline(Ks, (m, m, m), args.cpu_budget, (0, 0, 0, 0, 0, 0), (1.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+        scale = (n / m ** 3) * nr
+        cpu = {"value": nr * r["value"] / scale, "unit": "solves/s", "cores": 1, "kind": r["kind"],
+               "sample": "extrapolated: " + r["sample"] + "; scaled by %d (cells) per right-hand side, iteration "
+                         "growth with size not included" % (n // m ** 3)}
+    out = {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
+        "config": {"workload": w["label"], "cells": n, "right_hand_sides": nr, "rtol": 1e-8,
+                   "precision": "FP32 AMG in FP64 flexible CG, block of %d" % nr},
+        "iterations": its.tolist(), "max_rel_residual": float(per["rel_residual"].max()),
+        "div_max": per["div_max"].tolist(),
+        "phase_ms": {"assemble": st.t_assemble_ms, "setup": st.t_setup_ms, "solve": st.t_solve_ms,
+                     "flux": st.t_flux_ms},
+        "hbm_gbps_solve_phase_fine_level": gbs,
+        "footprint_gb": st.footprint_bytes / 1e9, "hbm_capacity_gb": 288.0,
+        "separate_solves": separate, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; c3, c4: 3; c5: 1)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 3; c3, c4, c5: 1)")
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--smoother", default=None, choices=["jacobi", "blockgs"],
@@ -208,6 +340,11 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
+    big = {"c3": (3, 1), "c4": (3, 1), "c5": (1, 1)}.get(args.workload, (20, 3))
+    if args.steps is None:
+        args.steps = big[0]
+    if args.warmup is None:
+        args.warmup = big[1]
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every box here has one); --gpus %d is not supported"
               % args.gpus, file=sys.stderr)
@@ -231,6 +368,8 @@ def main():
     w = WORKLOADS[args.workload]
     if "batch" in w:
         return run_batch(args, w, torch, S)
+    if "block" in w:
+        return run_block(args, w, torch, S)
     nx, ny, nz = w["dims"]
     n = nx * ny * nz
     rtol = 1e-10 if args.workload == "c1" else 1e-8

@@ -926,17 +926,13 @@ k_cg_init(OpArgs<double> o, const double *__restrict__ b, const double *__restri
     if (threadIdx.x == 0) { Pinit[2 * lb] = v[0]; Pinit[2 * lb + 1] = v[1]; }
 }
 
-template <typename ZT>
-__global__ void __launch_bounds__(kBlock)
-k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
-          double *__restrict__ v, int chunk, int G, CgScalars *sc, const double *Pinit, const double *Prr,
-          const double *Pdot, double *Ptau)
+// Start of a CG iteration k (read from the device): the stopping tests on
+// the partial sums of the previous kernels, beta (Polak-Ribiere form), and the
+// scalars written by one thread. Returns true if the solve stops here.
+__device__ __forceinline__ bool cg_iteration_scalars(int G, CgScalars *sc, const double *Pinit, const double *Prr,
+                                                     const double *Pdot, int &k, double &beta)
 {
-    if (sc->done) return;
-    const int n = o.nx * o.ny * o.nz;
-    const int k = sc->kspmv;
-    const double *__restrict__ sin = (k & 1) ? s1 : s0;
-    double *__restrict__ sout = (k & 1) ? s0 : s1;
+    k = sc->kspmv;
     double dots[2];
     sum_partials<2>(Pdot, G, dots);
     double bb, rr;
@@ -953,7 +949,8 @@ k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, d
     }
     const double sigma = fabs(dots[0]);
     int stop = 0, status = HMFEM_OK;
-    double beta = 0.0, sigma_first = k == 0 ? (sigma == 0.0 ? 1.0 : sigma) : sc->sigma_first;
+    beta = 0.0;
+    double sigma_first = k == 0 ? (sigma == 0.0 ? 1.0 : sigma) : sc->sigma_first;
     if (k == 0) {
         if (bb == 0.0 || rr <= sc->rtol2 * bb || sigma == 0.0) stop = 1;
     } else {
@@ -972,6 +969,22 @@ k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, d
         sc->kupd = k;
         if (stop) { sc->done = 1; sc->status = status; }
     }
+    return stop != 0;
+}
+
+template <typename ZT>
+__global__ void __launch_bounds__(kBlock)
+k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
+          double *__restrict__ v, int chunk, int G, CgScalars *sc, const double *Pinit, const double *Prr,
+          const double *Pdot, double *Ptau)
+{
+    if (sc->done) return;
+    const int n = o.nx * o.ny * o.nz;
+    int k;
+    double beta;
+    const bool stop = cg_iteration_scalars(G, sc, Pinit, Prr, Pdot, k, beta);
+    const double *__restrict__ sin = (k & 1) ? s1 : s0;
+    double *__restrict__ sout = (k & 1) ? s0 : s1;
     if (stop) return;
     int lb, beg, end;
     chunk_range(n, chunk, lb, beg, end);
@@ -1001,16 +1014,153 @@ k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, d
     if (threadIdx.x == 0) Ptau[lb] = acc[0];
 }
 
+// The CG SpMV marching along z: s = z + beta s_old, v = A s and <s, v>.
+// A block owns a tile of kGsTx x kGsTy columns and the planes [k0, k0 + zlen);
+// s of the tile and its x/y ring is kept in LDS for three planes (k-1, k,
+// k+1), so each value of s_old and z is read once per block (the ring is
+// also read by the neighbouring tiles, from L2), however large a plane is.
+// One partial <s, v> per logical block (Ptau[lb]; blocks past the last tile
+// write 0).
+template <typename ZT>
+__global__ void __launch_bounds__(kBlock)
+k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
+            double *__restrict__ v, int zlen, int G, CgScalars *sc, const double *Pinit, const double *Prr,
+            const double *Pdot, double *Ptau)
+{
+    if (sc->done) return;
+    int k;
+    double beta;
+    const bool stop = cg_iteration_scalars(G, sc, Pinit, Prr, Pdot, k, beta);
+    if (stop) return;
+    const double *__restrict__ sin = (k & 1) ? s1 : s0;
+    double *__restrict__ sout = (k & 1) ? s0 : s1;
+    __shared__ double sh[3][kGsTy + 2][kGsTx + 2];
+    const int nx = o.nx, ny = o.ny, nz = o.nz;
+    const long nxy = (long)nx * ny;
+    const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy;
+    const int ntxy = ntx * nty, nzc = (nz + zlen - 1) / zlen;
+    const int lb = xcd_block();
+    double acc[1] = {0.0};
+    if (lb < ntxy * nzc) {
+        const int tile = lb % ntxy, zc = lb / ntxy;
+        const int i0 = (tile % ntx) * kGsTx, j0 = (tile / ntx) * kGsTy;
+        const int kb = zc * zlen, ke = min(nz, kb + zlen);
+        const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+        const int i = i0 + tx, j = j0 + ty;
+        const bool col = i < nx && j < ny;
+        // ring cell of this thread (threads 0 .. kGsRing-1), as in the smoother
+        int rx = 0, ry = 0;
+        const bool has_ring = threadIdx.x < kGsRing;
+        if (has_ring) {
+            int hz;
+            gs_ring_cell(threadIdx.x, rx, ry, hz);
+        }
+        const int ri = i0 + rx - 1, rj = j0 + ry - 1;
+        const bool ring_in = has_ring && ri >= 0 && rj >= 0 && ri < nx && rj < ny;
+        // Loads run one plane ahead of the LDS writes and two planes ahead of
+        // the arithmetic: raw z and s_old of plane kk + 2 and the coefficients
+        // of plane kk + 1 are in flight while plane kk is computed.
+        struct Raw {
+            double zo, so, zr, sr;
+        };
+        auto load_raw = [&](int kk) -> Raw {
+            Raw r{0.0, 0.0, 0.0, 0.0};
+            if (kk >= 0 && kk < nz) {
+                if (col) {
+                    const long q = i + (long)j * nx + (long)kk * nxy;
+                    r.zo = (double)z[q];
+                    if (k) r.so = sin[q];
+                }
+                if (ring_in) {
+                    const long q = ri + (long)rj * nx + (long)kk * nxy;
+                    r.zr = (double)z[q];
+                    if (k) r.sr = sin[q];
+                }
+            }
+            return r;
+        };
+        auto store_raw = [&](int kk, const Raw &r) {
+            const int slot = (kk + 3) % 3;
+            sh[slot][ty + 1][tx + 1] = r.zo + beta * r.so;   // beta = 0 and s_old unread at k = 0
+            if (has_ring) sh[slot][ry][rx] = r.zr + beta * r.sr;
+        };
+        struct Coef {
+            double e, xp, xm, yp, ym, zp;
+        };
+        auto load_coef = [&](int kk) -> Coef {
+            Coef c{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (col && kk < nz) {
+                const long q = i + (long)j * nx + (long)kk * nxy;
+                c.e = o.e[q];
+                c.xp = i < nx - 1 ? o.wx[q] : 0.0;
+                c.xm = i > 0 ? o.wx[q - 1] : 0.0;
+                c.yp = j < ny - 1 ? o.wy[q] : 0.0;
+                c.ym = j > 0 ? o.wy[q - nx] : 0.0;
+                c.zp = kk < nz - 1 ? o.wz[q] : 0.0;
+            }
+            return c;
+        };
+        store_raw(kb - 1, load_raw(kb - 1));
+        store_raw(kb, load_raw(kb));
+        Raw nxt = load_raw(kb + 1);
+        Coef cf = load_coef(kb);
+        double wz_lo = col && kb > 0 ? o.wz[i + (long)j * nx + (long)(kb - 1) * nxy] : 0.0;
+        for (int kk = kb; kk < ke; kk++) {
+            store_raw(kk + 1, nxt);
+            nxt = load_raw(kk + 2);
+            const Coef cn = load_coef(kk + 1);
+            __syncthreads();
+            const int c = (kk + 3) % 3, lo = (kk + 2) % 3, hi = (kk + 1) % 3;
+            if (col) {
+                const long q = i + (long)j * nx + (long)kk * nxy;
+                const double us = sh[c][ty + 1][tx + 1];
+                double a = cf.e * us;
+                a += cf.xp * (us - sh[c][ty + 1][tx + 2]);
+                a += cf.xm * (us - sh[c][ty + 1][tx]);
+                a += cf.yp * (us - sh[c][ty + 2][tx + 1]);
+                a += cf.ym * (us - sh[c][ty][tx + 1]);
+                a += cf.zp * (us - sh[hi][ty + 1][tx + 1]);
+                a += wz_lo * (us - sh[lo][ty + 1][tx + 1]);
+                wz_lo = cf.zp;
+                sout[q] = us;
+                v[q] = a;
+                acc[0] += us * a;
+            }
+            cf = cn;
+            __syncthreads();
+        }
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) Ptau[lb] = acc[0];
+}
+
+// Planes per block of k_cg_spmv_z so that it needs at most kMaxReduceBlocks
+// blocks (at least 4 planes each); 0 if a single plane has more tiles than that.
+inline int spmv_zlen(const Dims &d)
+{
+    const long ntxy = (long)((d.nx + kGsTx - 1) / kGsTx) * ((d.ny + kGsTy - 1) / kGsTy);
+    if (ntxy > kMaxReduceBlocks) return 0;
+    long zl = 4;
+    while (ntxy * ((d.nz + zl - 1) / zl) > kMaxReduceBlocks) zl++;
+    return (int)zl;
+}
+
+inline int spmv_z_blocks(const Dims &d, int zl)
+{
+    const int ntxy = ((d.nx + kGsTx - 1) / kGsTx) * ((d.ny + kGsTy - 1) / kGsTy);
+    return grid_for((long)ntxy * ((d.nz + zl - 1) / zl), 1);
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_cg_update(int n, const double *__restrict__ s0, const double *__restrict__ s1, const double *__restrict__ v,
-            double *__restrict__ x, double *__restrict__ r, int chunk, int G, CgScalars *sc, const double *Ptau,
+            double *__restrict__ x, double *__restrict__ r, int chunk, int Gtau, CgScalars *sc, const double *Ptau,
             double *Prr)
 {
     if (sc->done) return;
     const int k = sc->kupd;
     const double *__restrict__ s = (k & 1) ? s0 : s1;
     double t[1];
-    sum_partials<1>(Ptau, G, t);
+    sum_partials<1>(Ptau, Gtau, t);
     const double tau = fabs(t[0]);
     if (tau == 0.0 || !(tau == tau)) {
         if (xcd_block() == 0 && threadIdx.x == 0) { sc->done = 1; sc->status = HMFEM_ERR_BREAKDOWN; sc->tau = tau; }
@@ -1448,13 +1598,19 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     T *z = vcycle_fine<T>(st, H, B.r, nullptr, cp, P, S);
     HMFEM_CHECK(hipGetLastError());
 
+    const int zlen = spmv_zlen(d);
+    const int Gz = zlen > 0 ? spmv_z_blocks(d, zlen) : G;
     // One CG iteration; identical launches every iteration (the index and the
     // buffer parity are read from the device).
     auto iteration = [&](Prof *PP) -> T * {
-        PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv<T>, dim3(G), dim3(kBlock), 0, st, o, (const T *)z, B.s0, B.s1,
-                    B.v, S.ch.chunk, G, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+        if (zlen > 0)
+            PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv_z<T>, dim3(Gz), dim3(kBlock), 0, st, o, (const T *)z, B.s0,
+                        B.s1, B.v, zlen, G, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+        else
+            PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv<T>, dim3(G), dim3(kBlock), 0, st, o, (const T *)z, B.s0,
+                        B.s1, B.v, S.ch.chunk, G, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
         PROF_LAUNCH(PP, HMFEM_PROF_CG_UPDATE, k_cg_update, dim3(G), dim3(kBlock), 0, st, n, B.s0, B.s1, B.v, x, B.r,
-                    S.ch.chunk, G, B.sc, S.Ptau, S.Prr);
+                    S.ch.chunk, zlen > 0 ? Gz : G, B.sc, S.Ptau, S.Prr);
         return vcycle_fine<T>(st, H, B.r, B.v, cp, PP, S);
     };
     hipGraphExec_t exec = nullptr;
