@@ -23,7 +23,7 @@
 namespace hmfem {
 
 constexpr int kMaxRhs = 8;
-constexpr int kBTz = 4;   // z extent of a block-smoother tile
+constexpr int kBTz = 8;   // z extent of a block-smoother tile
 
 struct BlockScalars {
     double bb[kMaxRhs], rr[kMaxRhs], sigma_first[kMaxRhs], beta[kMaxRhs], tau[kMaxRhs];
@@ -106,19 +106,14 @@ k_b_init(OpArgs<double> o, const double *__restrict__ B, const double *__restric
         for (int q = 0; q < 2 * NR; q++) P[(long)2 * NR * lb + q] = acc[q];
 }
 
-// Stopping tests and beta per right-hand side, then s = z + beta s_old,
-// v = A s, partials <s, v> (P_tau[lb][NR]).
+// Start of block CG iteration k: per right-hand side the stopping tests on
+// the partial sums of the previous kernels and beta; live[r] = 0 for a
+// finished one. Returns true once every right-hand side has finished.
 template <int NR>
-__global__ void __launch_bounds__(kBlock)
-k_b_spmv(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S0, double *__restrict__ S1,
-         double *__restrict__ V, int chunk, int G, BlockScalars *sc, const double *Pinit, const double *Prr,
-         const double *Pdot, double *Ptau)
+__device__ __forceinline__ bool b_iteration_scalars(int G, BlockScalars *sc, const double *Pinit, const double *Prr,
+                                                    const double *Pdot, int &k, double (&beta)[NR], int (&live)[NR])
 {
-    if (sc->all_done) return;
-    const int n = o.nx * o.ny * o.nz;
-    const int k = sc->kspmv;
-    const double *__restrict__ sin = (k & 1) ? S1 : S0;
-    double *__restrict__ sout = (k & 1) ? S0 : S1;
+    k = sc->kspmv;
     double dots[2 * NR], rrs[NR], bbs[NR];
     sum_partials<2 * NR>(Pdot, G, dots);
     if (k == 0) {
@@ -132,8 +127,6 @@ k_b_spmv(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S0,
 #pragma unroll
         for (int r = 0; r < NR; r++) { rrs[r] = t[r]; bbs[r] = sc->bb[r]; }
     }
-    double beta[NR];
-    int live[NR];
     int all = 1;
     const bool writer = xcd_block() == 0 && threadIdx.x == 0;
 #pragma unroll
@@ -167,27 +160,97 @@ k_b_spmv(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S0,
         sc->kupd = k;
         if (all) sc->all_done = 1;
     }
-    if (all) return;
-    int lb, beg, end;
-    chunk_range(n, chunk, lb, beg, end);
+    return all != 0;
+}
+
+// Block CG SpMV marching along z (as the single-RHS kernel): s = z + beta s_old
+// for every live right-hand side (0 for finished ones), v = A s, partials
+// <s, v>_r per logical block (Ptau[lb][NR]; blocks past the last tile write 0).
+template <int NR>
+__global__ void __launch_bounds__(kBlock)
+k_b_spmv_z(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S0, double *__restrict__ S1,
+           double *__restrict__ V, int zlen, int G, BlockScalars *sc, const double *Pinit, const double *Prr,
+           const double *Pdot, double *Ptau)
+{
+    if (sc->all_done) return;
+    int k;
+    double beta[NR];
+    int live[NR];
+    if (b_iteration_scalars<NR>(G, sc, Pinit, Prr, Pdot, k, beta, live)) return;
+    const double *__restrict__ sin = (k & 1) ? S1 : S0;
+    double *__restrict__ sout = (k & 1) ? S0 : S1;
+    __shared__ double sh[3][kGsTy + 2][kGsTx + 2][NR];
+    const int nx = o.nx, ny = o.ny, nz = o.nz;
+    const long nxy = (long)nx * ny;
+    const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy;
+    const int ntxy = ntx * nty, nzc = (nz + zlen - 1) / zlen;
+    const int lb = xcd_block();
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) acc[r] = 0.0;
-    auto sv = [&](int q, int r) -> double {
-        if (!live[r]) return 0.0;
-        const double z = (double)Z[(long)q * NR + r];
-        return k == 0 ? z : z + beta[r] * sin[(long)q * NR + r];
-    };
-    for (int s = beg + threadIdx.x; s < end; s += kBlock) {
-        Nb<double> b = nb_at(o, s);
-        double av[NR];
-        apply_block<NR, double>(b, sv, av);
+    if (lb < ntxy * nzc) {
+        const int tile = lb % ntxy, zc = lb / ntxy;
+        const int i0 = (tile % ntx) * kGsTx, j0 = (tile / ntx) * kGsTy;
+        const int kb = zc * zlen, ke = min(nz, kb + zlen);
+        const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+        const int i = i0 + tx, j = j0 + ty;
+        const bool col = i < nx && j < ny;
+        int rx = 0, ry = 0;
+        const bool has_ring = threadIdx.x < kGsRing;
+        if (has_ring) {
+            int hz;
+            gs_ring_cell(threadIdx.x, rx, ry, hz);
+        }
+        const int ri = i0 + rx - 1, rj = j0 + ry - 1;
+        const bool ring_in = has_ring && ri >= 0 && rj >= 0 && ri < nx && rj < ny;
+        auto fill = [&](int kk) {
+            const int slot = (kk + 3) % 3;
+            const bool in = kk >= 0 && kk < nz;
 #pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const double ss = sv(s, r);
-            sout[(long)s * NR + r] = ss;
-            V[(long)s * NR + r] = av[r];
-            acc[r] += ss * av[r];
+            for (int r = 0; r < NR; r++) {
+                double a = 0.0, b = 0.0;
+                if (in && col && live[r]) {
+                    const long q = (i + (long)j * nx + (long)kk * nxy) * NR + r;
+                    a = (double)Z[q] + (k ? beta[r] * sin[q] : 0.0);
+                }
+                if (in && ring_in && live[r]) {
+                    const long q = (ri + (long)rj * nx + (long)kk * nxy) * NR + r;
+                    b = (double)Z[q] + (k ? beta[r] * sin[q] : 0.0);
+                }
+                sh[slot][ty + 1][tx + 1][r] = a;
+                if (has_ring) sh[slot][ry][rx][r] = b;
+            }
+        };
+        fill(kb - 1);
+        fill(kb);
+        double wz_lo = col && kb > 0 ? o.wz[i + (long)j * nx + (long)(kb - 1) * nxy] : 0.0;
+        for (int kk = kb; kk < ke; kk++) {
+            fill(kk + 1);
+            __syncthreads();
+            const int c = (kk + 3) % 3, lo = (kk + 2) % 3, hi = (kk + 1) % 3;
+            if (col) {
+                const long q = i + (long)j * nx + (long)kk * nxy;
+                const double e = o.e[q];
+                const double xp = i < nx - 1 ? o.wx[q] : 0.0, xm = i > 0 ? o.wx[q - 1] : 0.0;
+                const double yp = j < ny - 1 ? o.wy[q] : 0.0, ym = j > 0 ? o.wy[q - nx] : 0.0;
+                const double zp = kk < nz - 1 ? o.wz[q] : 0.0;
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    const double us = sh[c][ty + 1][tx + 1][r];
+                    double a = e * us;
+                    a += xp * (us - sh[c][ty + 1][tx + 2][r]);
+                    a += xm * (us - sh[c][ty + 1][tx][r]);
+                    a += yp * (us - sh[c][ty + 2][tx + 1][r]);
+                    a += ym * (us - sh[c][ty][tx + 1][r]);
+                    a += zp * (us - sh[hi][ty + 1][tx + 1][r]);
+                    a += wz_lo * (us - sh[lo][ty + 1][tx + 1][r]);
+                    sout[q * NR + r] = us;
+                    V[q * NR + r] = a;
+                    acc[r] += us * a;
+                }
+                wz_lo = zp;
+            }
+            __syncthreads();
         }
     }
     block_sum<NR>(acc);
@@ -199,14 +262,14 @@ k_b_spmv(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S0,
 template <int NR>
 __global__ void __launch_bounds__(kBlock)
 k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, const double *__restrict__ V,
-           double *__restrict__ X, double *__restrict__ R, int chunk, int G, BlockScalars *sc, const double *Ptau,
+           double *__restrict__ X, double *__restrict__ R, int chunk, int Gtau, BlockScalars *sc, const double *Ptau,
            double *Prr)
 {
     if (sc->all_done) return;
     const int k = sc->kupd;
     const double *__restrict__ S = (k & 1) ? S0 : S1;
     double t[NR];
-    sum_partials<NR>(Ptau, G, t);
+    sum_partials<NR>(Ptau, Gtau, t);
     double alpha[NR];
     const bool writer = xcd_block() == 0 && threadIdx.x == 0;
 #pragma unroll
@@ -246,19 +309,46 @@ k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, 
 
 // ---------------------------------------------------------------- V-cycle kernels
 
+// Sum of NV values over a block of NT threads in a fixed order; every thread
+// gets the sums.
+template <int NV, int NT>
+__device__ __forceinline__ void block_sum_n(double (&v)[NV])
+{
+    __shared__ double sh[NT / 64][NV];
+    for (int q = 0; q < NV; q++)
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0)
+        for (int q = 0; q < NV; q++) sh[w][q] = v[q];
+    __syncthreads();
+    for (int q = 0; q < NV; q++) {
+        double t = sh[0][q];
+        for (int x = 1; x < NT / 64; x++) t += sh[x][q];
+        v[q] = t;
+    }
+    __syncthreads();
+}
+
+constexpr int kBCz = 4;                          // cells of a tile column per thread
+constexpr int kBThreads = kGsTx * kGsTy * (kBTz / kBCz);
+
 // Block red-black Gauss-Seidel on tiles of kGsTx x kGsTy x kBTz cells for NR
-// interleaved right-hand sides. MODE 0: from zero; MODE 1: from u + ocf *
-// uc[aggregate] (uo must not alias u); MODE 2: MODE 1 plus the CG dot
-// partials <z, f>_r, <z, v>_r of this block's tiles (Pdot[lb][2 NR]).
+// interleaved right-hand sides, as the single-RHS smoother (the ring of the
+// tile frozen); a tile column is shared by kBTz / kBCz threads, so the z
+// neighbour across their boundary is read from LDS. MODE 0: from zero;
+// MODE 1: from u + ocf * uc[aggregate] (uo must not alias u); MODE 2: MODE 1
+// plus the CG dot partials <z, f>_r, <z, v>_r of this block's tiles
+// (Pdot[lb][2 NR]).
 template <int NR, typename FT, int MODE>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBThreads)
 k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *__restrict__ u,
             float *__restrict__ uo, int sx, int sy, int sz, int cx, int cy, const float *__restrict__ uc, float ocf,
             const double *__restrict__ v, int tpb, double *Pdot, const int *done)
 {
-    constexpr int TZ = kBTz;
-    constexpr int RING = 2 * (kGsTx + 2) + 2 * kGsTy;
-    constexpr int RPER = (TZ * RING + kBlock - 1) / kBlock;
+    constexpr int TZ = kBTz, CZ = kBCz, NTZ = kBTz / kBCz;
+    constexpr int RING = kGsRing;
+    constexpr int RPER = (TZ * RING + kBThreads - 1) / kBThreads;
+    static_assert(CZ % 2 == 0, "colour pairs");
     if (*done) return;
     __shared__ float sh[TZ][kGsTy + 2][kGsTx + 2][NR];
     const int nx = o.nx, ny = o.ny, nz = o.nz;
@@ -267,7 +357,8 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
     const int ntiles = ntx * nty * ntz;
     const int lb = xcd_block();
     const int tbeg = lb * tpb, tend = min(ntiles, tbeg + tpb);
-    const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+    const int tx = threadIdx.x % kGsTx, ty = (threadIdx.x / kGsTx) % kGsTy, tz = threadIdx.x / (kGsTx * kGsTy);
+    const int kbase = tz * CZ;   // first tile plane of this thread
     double acc[MODE == 2 ? 2 * NR : 1];
     if constexpr (MODE == 2)
         for (int q = 0; q < 2 * NR; q++) acc[q] = 0.0;
@@ -287,12 +378,13 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
         const int i0 = ti * kGsTx, j0 = tj * kGsTy, k0 = tk * TZ;
         const int i = i0 + tx, j = j0 + ty;
         const bool col = i < nx && j < ny;
-        const long s0 = i + (long)j * nx + (long)k0 * nxy;
+        const int kc = k0 + kbase;   // grid plane of this thread's first cell
+        const long s0 = i + (long)j * nx + (long)kc * nxy;
 
-        float wxp[TZ], wxm[TZ], wyp[TZ], wym[TZ], wzp[TZ], dinv[TZ], ff[TZ][NR], uu[TZ][NR];
+        float wxp[CZ], wxm[CZ], wyp[CZ], wym[CZ], wzp[CZ], dinv[CZ], ff[CZ][NR], uu[CZ][NR];
 #pragma unroll
-        for (int kk = 0; kk < TZ; kk++) {
-            const bool act = col && k0 + kk < nz;
+        for (int kk = 0; kk < CZ; kk++) {
+            const bool act = col && kc + kk < nz;
             const long s = s0 + kk * nxy;
             wxp[kk] = act ? o.wx[s] : 0.f;
             wxm[kk] = act && i > 0 ? o.wx[s - 1] : 0.f;
@@ -303,40 +395,37 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
 #pragma unroll
             for (int r = 0; r < NR; r++) ff[kk][r] = act ? (float)f[s * NR + r] : 0.f;
         }
-        const float wzlo = col && k0 > 0 ? o.wz[s0 - nxy] : 0.f;
+        const float wzlo = col && kc > 0 && kc < nz ? o.wz[s0 - nxy] : 0.f;
         float zlo[NR], zhi[NR], ring[RPER][NR];
 #pragma unroll
-        for (int kk = 0; kk < TZ; kk++) start(i, j, k0 + kk, uu[kk]);
-        start(i, j, k0 - 1, zlo);
-        start(i, j, k0 + TZ, zhi);
+        for (int kk = 0; kk < CZ; kk++) start(i, j, kc + kk, uu[kk]);
+        if (tz == 0) start(i, j, k0 - 1, zlo);
+        if (tz == NTZ - 1) start(i, j, k0 + TZ, zhi);
 #pragma unroll
         for (int m = 0; m < RPER; m++) {
-            const int q = threadIdx.x + m * kBlock;
-            const int hz = q / RING, rr = q - hz * RING;
-            int hx, hy;
-            if (rr < kGsTx + 2) { hx = rr; hy = 0; }
-            else if (rr < 2 * (kGsTx + 2)) { hx = rr - (kGsTx + 2); hy = kGsTy + 1; }
-            else { const int r2 = rr - 2 * (kGsTx + 2); hy = 1 + (r2 >> 1); hx = (r2 & 1) ? kGsTx + 1 : 0; }
+            const int q = threadIdx.x + m * kBThreads;
+            int hx = 0, hy = 0, hz = 0;
+            if (q < TZ * RING) gs_ring_cell(q, hx, hy, hz);
             if (q < TZ * RING) start(i0 + hx - 1, j0 + hy - 1, k0 + hz, ring[m]);
 #pragma unroll
             for (int r = 0; r < NR; r++)
                 if (q < TZ * RING) sh[hz][hy][hx][r] = ring[m][r];
         }
 #pragma unroll
-        for (int kk = 0; kk < TZ; kk++)
+        for (int kk = 0; kk < CZ; kk++)
 #pragma unroll
-            for (int r = 0; r < NR; r++) sh[kk][ty + 1][tx + 1][r] = uu[kk][r];
+            for (int r = 0; r < NR; r++) sh[kbase + kk][ty + 1][tx + 1][r] = uu[kk][r];
         __syncthreads();
 
         for (int sw = 0; sw < sweeps; sw++) {
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int c = MODE == 0 ? h : 1 - h;
-                const int sel = (c + i + j + k0) & 1;
+                const int sel = (c + i + j + kc) & 1;
 #pragma unroll
-                for (int m = 0; m < TZ / 2; m++) {
+                for (int m = 0; m < CZ / 2; m++) {
                     const int ka = 2 * m, kb = 2 * m + 1;
-                    const int kk = ka + sel;
+                    const int kk = kbase + ka + sel;   // tile plane of the cell updated
                     const float cxp = sel ? wxp[kb] : wxp[ka], cxm = sel ? wxm[kb] : wxm[ka];
                     const float cyp = sel ? wyp[kb] : wyp[ka], cym = sel ? wym[kb] : wym[ka];
                     const float czp = sel ? wzp[kb] : wzp[ka];
@@ -344,8 +433,13 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
                     const float di_ = sel ? dinv[kb] : dinv[ka];
 #pragma unroll
                     for (int r = 0; r < NR; r++) {
-                        const float uzm = sel ? uu[ka][r] : (m > 0 ? uu[ka - 1][r] : zlo[r]);
-                        const float uzp = sel ? (kb + 1 < TZ ? uu[kb + 1][r] : zhi[r]) : uu[kb][r];
+                        float below, above;
+                        if (m > 0) below = uu[ka - 1][r];
+                        else below = tz == 0 ? zlo[r] : sh[kbase - 1][ty + 1][tx + 1][r];
+                        if (kb + 1 < CZ) above = uu[kb + 1][r];
+                        else above = tz == NTZ - 1 ? zhi[r] : sh[kbase + CZ][ty + 1][tx + 1][r];
+                        const float uzm = sel ? uu[ka][r] : below;
+                        const float uzp = sel ? above : uu[kb][r];
                         const float a = (sel ? ff[kb][r] : ff[ka][r]) + cxp * sh[kk][ty + 1][tx + 2][r] +
                                         cxm * sh[kk][ty + 1][tx][r] + cyp * sh[kk][ty + 2][tx + 1][r] +
                                         cym * sh[kk][ty][tx + 1][r] + czp * uzp + czm * uzm;
@@ -360,8 +454,8 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
         }
 
 #pragma unroll
-        for (int kk = 0; kk < TZ; kk++) {
-            if (col && k0 + kk < nz) {
+        for (int kk = 0; kk < CZ; kk++) {
+            if (col && kc + kk < nz) {
                 const long s = s0 + kk * nxy;
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
@@ -375,7 +469,7 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
         }
     }
     if constexpr (MODE == 2) {
-        block_sum<2 * NR>(acc);
+        block_sum_n<2 * NR, kBThreads>(acc);
         if (threadIdx.x == 0)
             for (int q = 0; q < 2 * NR; q++) Pdot[(long)2 * NR * lb + q] = acc[q];
     }
@@ -494,13 +588,13 @@ static int b_vcycle_coarse(hipStream_t st, Hierarchy &H, BlockState &S, int l, c
     Level &C = H.lev[l + 1];
     const int GT = grid_for(b_tiles(L.d), 1);
     const int sw = opt.gs_sweeps < 1 ? 1 : opt.gs_sweeps;
-    hipLaunchKernelGGL((k_b_gs_tile<NR, float, 0>), dim3(GT), dim3(kBlock), 0, st, opargs<float>(L), sw,
+    hipLaunchKernelGGL((k_b_gs_tile<NR, float, 0>), dim3(GT), dim3(kBThreads), 0, st, opargs<float>(L), sw,
                        (const float *)X.f, (const float *)nullptr, X.u, 0, 0, 0, 0, 0, (const float *)nullptr, 0.f,
                        (const double *)nullptr, 1, (double *)nullptr, S.done);
     b_restrict<NR, float>(st, L, C, X.f, X.u, S.lv[l + 1].f, S.done);
     int rc = b_vcycle_coarse<NR>(st, H, S, l + 1, opt);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_b_gs_tile<NR, float, 1>), dim3(GT), dim3(kBlock), 0, st, opargs<float>(L), sw,
+    hipLaunchKernelGGL((k_b_gs_tile<NR, float, 1>), dim3(GT), dim3(kBThreads), 0, st, opargs<float>(L), sw,
                        (const float *)X.f, (const float *)X.u, X.t, lg2(L.fx), lg2(L.fy), lg2(L.fz), C.d.nx, C.d.ny,
                        (const float *)S.lv[l + 1].u, (float)opt.ocf, (const double *)nullptr, 1, (double *)nullptr,
                        S.done);
@@ -516,14 +610,14 @@ static int b_vcycle_fine(hipStream_t st, Hierarchy &H, BlockState &S, const doub
     BlockLevelVecs &X = S.lv[0];
     const int nt = b_tiles(L.d);
     const int sw = opt.gs_sweeps < 1 ? 1 : opt.gs_sweeps;
-    hipLaunchKernelGGL((k_b_gs_tile<NR, double, 0>), dim3(grid_for(nt, 1)), dim3(kBlock), 0, st, opargs<float>(L),
+    hipLaunchKernelGGL((k_b_gs_tile<NR, double, 0>), dim3(grid_for(nt, 1)), dim3(kBThreads), 0, st, opargs<float>(L),
                        sw, (const double *)S.R, (const float *)nullptr, X.u, 0, 0, 0, 0, 0, (const float *)nullptr,
                        0.f, (const double *)nullptr, 1, (double *)nullptr, S.done);
     b_restrict<NR, double>(st, L, C, S.R, X.u, S.lv[1].f, S.done);
     int rc = b_vcycle_coarse<NR>(st, H, S, 1, opt);
     if (rc) return rc;
     const int tpb = (nt + S.ch.G - 1) / S.ch.G;
-    hipLaunchKernelGGL((k_b_gs_tile<NR, double, 2>), dim3(S.ch.G), dim3(kBlock), 0, st, opargs<float>(L), sw,
+    hipLaunchKernelGGL((k_b_gs_tile<NR, double, 2>), dim3(S.ch.G), dim3(kBThreads), 0, st, opargs<float>(L), sw,
                        (const double *)S.R, (const float *)X.u, S.Z, lg2(L.fx), lg2(L.fy), lg2(L.fz), C.d.nx,
                        C.d.ny, (const float *)S.lv[1].u, (float)opt.ocf, Vp, tpb, S.Pdot, S.done);
     return HMFEM_OK;
@@ -571,6 +665,12 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
     o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
     o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
     const int G = S.ch.G;
+    // planes per block of the z-marching SpMV: at most kMaxReduceBlocks blocks
+    const int ntxy = ((d.nx + kGsTx - 1) / kGsTx) * ((d.ny + kGsTy - 1) / kGsTy);
+    if (ntxy > kMaxReduceBlocks) return HMFEM_ERR_ARG;
+    int zlen = 4;
+    while ((long)ntxy * ((d.nz + zlen - 1) / zlen) > kMaxReduceBlocks) zlen++;
+    const int Gz = grid_for((long)ntxy * ((d.nz + zlen - 1) / zlen), 1);
     int rc = b_vcycle_fine<NR>(st, H, S, nullptr, opt);
     if (rc) return rc;
     HMFEM_CHECK(hipGetLastError());
@@ -578,10 +678,10 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
     const int every = opt.check_every < 1 ? 1 : opt.check_every;
     while (true) {
         for (int q = 0; q < every; q++, k++) {
-            hipLaunchKernelGGL(k_b_spmv<NR>, dim3(G), dim3(kBlock), 0, st, o, (const float *)S.Z, S.S0, S.S1, S.V,
-                               S.ch.chunk, G, S.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+            hipLaunchKernelGGL(k_b_spmv_z<NR>, dim3(Gz), dim3(kBlock), 0, st, o, (const float *)S.Z, S.S0, S.S1,
+                               S.V, zlen, G, S.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
             hipLaunchKernelGGL(k_b_update<NR>, dim3(G), dim3(kBlock), 0, st, n, S.S0, S.S1, S.V, S.X, S.R,
-                               S.ch.chunk, G, S.sc, S.Ptau, S.Prr);
+                               S.ch.chunk, Gz, S.sc, S.Ptau, S.Prr);
             rc = b_vcycle_fine<NR>(st, H, S, S.V, opt);
             if (rc) return rc;
         }
@@ -600,7 +700,7 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
                 std::vector<char> &stage, int *iterations, double *rel_residual, int *status, double *setup_ms,
                 double *solve_ms)
 {
-    if (nr != 2 && nr != 4 && nr != 8) return HMFEM_ERR_ARG;
+    if (nr != 2 && nr != 4) return HMFEM_ERR_ARG;
     hmfem_options opt = opt_in;
     opt.precision = HMFEM_PREC_FP32;
     if (opt.coarse_max > kCoarseMax || opt.coarse_max < 1) opt.coarse_max = kCoarseMax;
@@ -661,13 +761,9 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
         hipLaunchKernelGGL(k_b_init<2>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
         rc = block_cg<2>(st, H, A, d, S, opt, &hs);
         break;
-    case 4:
+    default:
         hipLaunchKernelGGL(k_b_init<4>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
         rc = block_cg<4>(st, H, A, d, S, opt, &hs);
-        break;
-    default:
-        hipLaunchKernelGGL(k_b_init<8>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
-        rc = block_cg<8>(st, H, A, d, S, opt, &hs);
         break;
     }
     if (rc) return rc;

@@ -321,7 +321,8 @@ extern "C" int hmfem_reservoir_block_device(hmfem_context *c, const hmfem_grid *
     else hmfem_default_options(&opt);
     Dims d{g->nx, g->ny, g->nz};
     const long n = d.n();
-    const int nr = nrhs <= 2 ? 2 : (nrhs <= 4 ? 4 : 8);
+    // groups of at most four right-hand sides (two-wide for one or two)
+    const int nr = nrhs <= 2 ? 2 : 4;
     Stencil<double> A;
     A.e = (double *)c->get("e", n * 8);
     A.wx = (double *)c->get("wx", n * 8);
@@ -334,34 +335,42 @@ extern "C" int hmfem_reservoir_block_device(hmfem_context *c, const hmfem_grid *
     hmfem_block_status tmp{};
     hmfem_block_status *out = st ? st : &tmp;
     *out = hmfem_block_status{};
-    HMFEM_CHECK(hipEventRecord(c->ev[4], c->st));
-    for (int r = 0; r < nr; r++) {
-        if (r < nrhs) {
-            int rc = launch_assemble(c->st, g, Kx, Ky, Kz, nullptr, nullptr, nullptr, nullptr, bval + 6 * r,
-                                     r == 0 ? A.e : nullptr, r == 0 ? A.wx : nullptr, r == 0 ? A.wy : nullptr,
-                                     r == 0 ? A.wz : nullptr, b, nullptr, nullptr);
-            if (rc) return rc;
-            block_put(c->st, n, nr, r, b, q ? q[r] : nullptr, Bk);
-            block_put(c->st, n, nr, r, p[r], nullptr, Xk);
-        } else {
-            block_put(c->st, n, nr, r, nullptr, nullptr, Bk);
-            block_put(c->st, n, nr, r, nullptr, nullptr, Xk);
-        }
-    }
-    HMFEM_CHECK(hipGetLastError());
-    HMFEM_CHECK(hipEventRecord(c->ev[5], c->st));
     int it[8] = {0}, stat[8] = {0};
     double rel[8] = {0};
-    double setup_ms = 0, solve_ms = 0;
-    int rc = block_solve(c->st, d, A, nr, Bk, Xk, opt, ctx_alloc_block, c, c->lds_limit, c->stage, it, rel, stat,
-                         &setup_ms, &solve_ms);
-    if (rc) return rc;
-    for (int r = 0; r < nrhs; r++) block_get(c->st, n, nr, r, Xk, p[r]);
+    double t_asm = 0, setup_ms = 0, solve_ms = 0;
+    for (int g0 = 0; g0 < nrhs; g0 += nr) {
+        HMFEM_CHECK(hipEventRecord(c->ev[4], c->st));
+        for (int r = 0; r < nr; r++) {
+            const int R = g0 + r;
+            if (R < nrhs) {
+                const bool first = g0 == 0 && r == 0;
+                int rc = launch_assemble(c->st, g, Kx, Ky, Kz, nullptr, nullptr, nullptr, nullptr, bval + 6 * R,
+                                         first ? A.e : nullptr, first ? A.wx : nullptr, first ? A.wy : nullptr,
+                                         first ? A.wz : nullptr, b, nullptr, nullptr);
+                if (rc) return rc;
+                block_put(c->st, n, nr, r, b, q ? q[R] : nullptr, Bk);
+                block_put(c->st, n, nr, r, p[R], nullptr, Xk);
+            } else {
+                block_put(c->st, n, nr, r, nullptr, nullptr, Bk);
+                block_put(c->st, n, nr, r, nullptr, nullptr, Xk);
+            }
+        }
+        HMFEM_CHECK(hipGetLastError());
+        HMFEM_CHECK(hipEventRecord(c->ev[5], c->st));
+        double su = 0, so = 0;
+        int rc = block_solve(c->st, d, A, nr, Bk, Xk, opt, ctx_alloc_block, c, c->lds_limit, c->stage, it + g0,
+                             rel + g0, stat + g0, &su, &so);
+        if (rc) return rc;
+        for (int r = 0; r < nr && g0 + r < nrhs; r++) block_get(c->st, n, nr, r, Xk, p[g0 + r]);
+        float ta = 0.f;
+        HMFEM_CHECK(hipEventSynchronize(c->ev[5]));
+        (void)hipEventElapsedTime(&ta, c->ev[4], c->ev[5]);
+        t_asm += ta;
+        setup_ms += su;
+        solve_ms += so;
+    }
     HMFEM_CHECK(hipGetLastError());
-    float ta = 0.f;
-    HMFEM_CHECK(hipEventSynchronize(c->ev[5]));
-    (void)hipEventElapsedTime(&ta, c->ev[4], c->ev[5]);
-    out->t_assemble_ms = ta;
+    out->t_assemble_ms = t_asm;
     out->t_setup_ms = setup_ms;
     out->t_solve_ms = solve_ms;
     out->status = HMFEM_OK;
@@ -376,7 +385,7 @@ extern "C" int hmfem_reservoir_block_device(hmfem_context *c, const hmfem_grid *
         for (int r = 0; r < nrhs; r++) {
             if (!v[r]) continue;
             double dmin = 0, dmax = 0;
-            rc = hmfem_flux_device(c, g, Kx, Ky, Kz, bval + 6 * r, p[r], v[r], &dmin, &dmax);
+            int rc = hmfem_flux_device(c, g, Kx, Ky, Kz, bval + 6 * r, p[r], v[r], &dmin, &dmax);
             if (rc) return rc;
             if (div_max) div_max[r] = dmax;
         }

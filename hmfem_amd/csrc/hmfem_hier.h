@@ -19,7 +19,22 @@ struct TailLevel {
 };
 
 // Block Gauss-Seidel tiles: kGsTx x kGsTy columns, one per thread.
-constexpr int kGsTx = 32, kGsTy = 8;
+constexpr int kGsTx = 32, kGsTy = 8, kGsTz = 8;
+
+// The x/y ring of one tile plane: rows hy = 0 and hy = kGsTy + 1 (kGsTx + 2
+// cells each) and columns hx = 0 and hx = kGsTx + 1 of rows 1..kGsTy.
+constexpr int kGsRing = 2 * (kGsTx + 2) + 2 * kGsTy;
+constexpr int kGsRingPer = (kGsTz * kGsRing + kBlock - 1) / kBlock;   // ring cells per thread
+
+// LDS coordinates of ring cell q (0 <= q < kGsTz * kGsRing).
+__device__ __forceinline__ void gs_ring_cell(int q, int &hx, int &hy, int &hz)
+{
+    hz = q / kGsRing;
+    const int r = q - hz * kGsRing;
+    if (r < kGsTx + 2) { hx = r; hy = 0; }
+    else if (r < 2 * (kGsTx + 2)) { hx = r - (kGsTx + 2); hy = kGsTy + 1; }
+    else { const int r2 = r - 2 * (kGsTx + 2); hy = 1 + (r2 >> 1); hx = (r2 & 1) ? kGsTx + 1 : 0; }
+}
 
 // A reduction launch: G blocks, each over a contiguous chunk of cells.
 struct Chunking {

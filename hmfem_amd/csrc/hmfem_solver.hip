@@ -247,23 +247,7 @@ k_final_sweep_dots(OpArgs<T> o, int fx, int fy, int cx, int cy, int fz, const T 
 //
 // GS update of a cell: u_s = (f_s + sum_nb w u_nb) / d_s with d_s = e_s + sum w
 // (the inverse diagonal di is precomputed), i.e. (A u)_s = f_s afterwards.
-constexpr int kGsTz = 8;
 static_assert(kGsTx * kGsTy == kBlock, "one thread per tile column");
-
-// The x/y ring of one tile plane: rows hy = 0 and hy = kGsTy + 1 (kGsTx + 2
-// cells each) and columns hx = 0 and hx = kGsTx + 1 of rows 1..kGsTy.
-constexpr int kGsRing = 2 * (kGsTx + 2) + 2 * kGsTy;
-constexpr int kGsRingPer = (kGsTz * kGsRing + kBlock - 1) / kBlock;   // ring cells per thread
-
-// LDS coordinates of ring cell q (0 <= q < kGsTz * kGsRing).
-__device__ __forceinline__ void gs_ring_cell(int q, int &hx, int &hy, int &hz)
-{
-    hz = q / kGsRing;
-    const int r = q - hz * kGsRing;
-    if (r < kGsTx + 2) { hx = r; hy = 0; }
-    else if (r < 2 * (kGsTx + 2)) { hx = r - (kGsTx + 2); hy = kGsTy + 1; }
-    else { const int r2 = r - 2 * (kGsTx + 2); hy = 1 + (r2 >> 1); hx = (r2 & 1) ? kGsTx + 1 : 0; }
-}
 
 inline int gs_tiles(const Dims &d)
 {

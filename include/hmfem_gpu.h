@@ -208,8 +208,9 @@ int hmfem_reservoir_batch_device(hmfem_context *ctx, int ninst, const int *dims,
    the initial guess in and the pressure out; v[r] (device ELE[n], or NULL)
    receives the fluxes. Per right-hand side: iterations, rel_residual and
    div_max (host arrays of nrhs, or NULL). Uses the FP32 hierarchy and the
-   block Gauss-Seidel smoother whatever opt says; counts of right-hand sides
-   other than 2, 4 or 8 are padded with zero right-hand sides. No counterpart
+   block Gauss-Seidel smoother whatever opt says; the right-hand sides are
+   solved in groups of four (two if there are only one or two), the last
+   group padded with zero right-hand sides. No counterpart
    in the reference's API: it is the interleaved multi-RHS solve of
    generic_operator.h:335-391 behind the reservoir() step. */
 typedef struct {
