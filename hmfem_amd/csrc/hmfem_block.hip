@@ -309,26 +309,6 @@ k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, 
 
 // ---------------------------------------------------------------- V-cycle kernels
 
-// Sum of NV values over a block of NT threads in a fixed order; every thread
-// gets the sums.
-template <int NV, int NT>
-__device__ __forceinline__ void block_sum_n(double (&v)[NV])
-{
-    __shared__ double sh[NT / 64][NV];
-    for (int q = 0; q < NV; q++)
-        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
-    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-    if (lane == 0)
-        for (int q = 0; q < NV; q++) sh[w][q] = v[q];
-    __syncthreads();
-    for (int q = 0; q < NV; q++) {
-        double t = sh[0][q];
-        for (int x = 1; x < NT / 64; x++) t += sh[x][q];
-        v[q] = t;
-    }
-    __syncthreads();
-}
-
 constexpr int kBCz = 4;                          // cells of a tile column per thread
 constexpr int kBThreads = kGsTx * kGsTy * (kBTz / kBCz);
 

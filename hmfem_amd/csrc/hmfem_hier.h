@@ -20,11 +20,12 @@ struct TailLevel {
 
 // Block Gauss-Seidel tiles: kGsTx x kGsTy columns, one per thread.
 constexpr int kGsTx = 32, kGsTy = 8, kGsTz = 8;
+constexpr int kGsCz = 4;                                   // cells of a tile column per thread
+constexpr int kGsThreads = kGsTx * kGsTy * (kGsTz / kGsCz); // threads of a smoother block
 
 // The x/y ring of one tile plane: rows hy = 0 and hy = kGsTy + 1 (kGsTx + 2
 // cells each) and columns hx = 0 and hx = kGsTx + 1 of rows 1..kGsTy.
 constexpr int kGsRing = 2 * (kGsTx + 2) + 2 * kGsTy;
-constexpr int kGsRingPer = (kGsTz * kGsRing + kBlock - 1) / kBlock;   // ring cells per thread
 
 // LDS coordinates of ring cell q (0 <= q < kGsTz * kGsRing).
 __device__ __forceinline__ void gs_ring_cell(int q, int &hx, int &hy, int &hz)

@@ -262,12 +262,14 @@ inline int gs_tiles(const Dims &d)
 // sx, sy, sz: log2 of the aggregation factors (0 or 1) towards uc's level.
 // Each (logical) block smooths tiles [lb * tpb, (lb + 1) * tpb).
 template <typename T, typename FT, int MODE>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kGsThreads)
 k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict__ u, T *__restrict__ uo, int sx,
           int sy, int sz, int cx, int cy, const T *__restrict__ uc, T ocf, const double *__restrict__ v, int tpb,
           double *Pdot, const int *done)
 {
     using C = T;
+    constexpr int CZ = kGsCz, NTZ = kGsTz / kGsCz;
+    constexpr int RPER = (kGsTz * kGsRing + kGsThreads - 1) / kGsThreads;
     if (*done) return;
     __shared__ C sh[kGsTz][kGsTy + 2][kGsTx + 2];
     const int nx = o.nx, ny = o.ny, nz = o.nz;
@@ -276,7 +278,8 @@ k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict
     const int ntiles = ntx * nty * ntz;
     const int lb = xcd_block();
     const int tbeg = lb * tpb, tend = min(ntiles, tbeg + tpb);
-    const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+    const int tx = threadIdx.x % kGsTx, ty = (threadIdx.x / kGsTx) % kGsTy, tz = threadIdx.x / (kGsTx * kGsTy);
+    const int kbase = tz * CZ;   // first tile plane of this thread's cells
     double acc[2] = {0.0, 0.0};
 
     // starting value of a cell (zero outside the grid)
@@ -292,14 +295,15 @@ k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict
         const int i0 = ti * kGsTx, j0 = tj * kGsTy, k0 = tk * kGsTz;
         const int i = i0 + tx, j = j0 + ty;
         const bool col = i < nx && j < ny;
-        const long s0 = i + (long)j * nx + (long)k0 * nxy;
+        const int kc = k0 + kbase;   // grid plane of this thread's first cell
+        const long s0 = i + (long)j * nx + (long)kc * nxy;
 
-        // this column's coefficients (zero for cells outside the grid)
-        C wxp[kGsTz], wxm[kGsTz], wyp[kGsTz], wym[kGsTz], wzp[kGsTz], dinv[kGsTz], ff[kGsTz], uu[kGsTz];
-        double fd[MODE == 2 ? kGsTz : 1];
+        // this thread's coefficients (zero for cells outside the grid)
+        C wxp[CZ], wxm[CZ], wyp[CZ], wym[CZ], wzp[CZ], dinv[CZ], ff[CZ], uu[CZ];
+        double fd[MODE == 2 ? CZ : 1];
 #pragma unroll
-        for (int kk = 0; kk < kGsTz; kk++) {
-            const bool act = col && k0 + kk < nz;
+        for (int kk = 0; kk < CZ; kk++) {
+            const bool act = col && kc + kk < nz;
             const long s = s0 + kk * nxy;
             wxp[kk] = act ? (C)o.wx[s] : (C)0;
             wxm[kk] = act && i > 0 ? (C)o.wx[s - 1] : (C)0;
@@ -311,30 +315,33 @@ k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict
             ff[kk] = (C)fv;
             if constexpr (MODE == 2) fd[kk] = fv;
         }
-        const C wzlo = col && k0 > 0 ? (C)o.wz[s0 - nxy] : (C)0;
+        const C wzlo = col && kc > 0 && kc < nz ? (C)o.wz[s0 - nxy] : (C)0;
 
-        // starting values of the column, its two z neighbours and kGsRingPer
-        // cells of the x/y ring, all loads issued before the first use
-        C ring[kGsRingPer];
+        // starting values of this thread's cells, the tile's two z neighbours
+        // of the column and RPER cells of the x/y ring, all loads issued
+        // before the first use
+        C ring[RPER];
 #pragma unroll
-        for (int kk = 0; kk < kGsTz; kk++) uu[kk] = start(i, j, k0 + kk);
-        const C zlo = start(i, j, k0 - 1);
-        const C zhi = start(i, j, k0 + kGsTz);
+        for (int kk = 0; kk < CZ; kk++) uu[kk] = start(i, j, kc + kk);
+        const C zlo = tz == 0 ? start(i, j, k0 - 1) : (C)0;
+        const C zhi = tz == NTZ - 1 ? start(i, j, k0 + kGsTz) : (C)0;
 #pragma unroll
-        for (int m = 0; m < kGsRingPer; m++) {
-            const int q = threadIdx.x + m * kBlock;
-            int hx, hy, hz;
-            gs_ring_cell(q, hx, hy, hz);
+        for (int m = 0; m < RPER; m++) {
+            const int q = threadIdx.x + m * kGsThreads;
+            int hx = 0, hy = 0, hz = 0;
+            if (q < kGsTz * kGsRing) gs_ring_cell(q, hx, hy, hz);
             ring[m] = q < kGsTz * kGsRing ? start(i0 + hx - 1, j0 + hy - 1, k0 + hz) : (C)0;
         }
 #pragma unroll
-        for (int kk = 0; kk < kGsTz; kk++) sh[kk][ty + 1][tx + 1] = uu[kk];
+        for (int kk = 0; kk < CZ; kk++) sh[kbase + kk][ty + 1][tx + 1] = uu[kk];
 #pragma unroll
-        for (int m = 0; m < kGsRingPer; m++) {
-            const int q = threadIdx.x + m * kBlock;
-            int hx, hy, hz;
-            gs_ring_cell(q, hx, hy, hz);
-            if (q < kGsTz * kGsRing) sh[hz][hy][hx] = ring[m];
+        for (int m = 0; m < RPER; m++) {
+            const int q = threadIdx.x + m * kGsThreads;
+            if (q < kGsTz * kGsRing) {
+                int hx, hy, hz;
+                gs_ring_cell(q, hx, hy, hz);
+                sh[hz][hy][hx] = ring[m];
+            }
         }
         __syncthreads();
 
@@ -342,13 +349,17 @@ k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int c = MODE == 0 ? h : 1 - h;       // colour of this half sweep
-                const int sel = (c + i + j + k0) & 1;      // cell 2m + sel of each pair has colour c
+                const int sel = (c + i + j + kc) & 1;      // cell 2m + sel of each pair has colour c
 #pragma unroll
-                for (int m = 0; m < kGsTz / 2; m++) {
+                for (int m = 0; m < CZ / 2; m++) {
                     const int ka = 2 * m, kb = 2 * m + 1;
-                    const int kk = ka + sel;
-                    const C uzm = sel ? uu[ka] : (m > 0 ? uu[ka - 1] : zlo);
-                    const C uzp = sel ? (kb + 1 < kGsTz ? uu[kb + 1] : zhi) : uu[kb];
+                    const int kk = kbase + ka + sel;       // tile plane of the cell updated
+                    // z neighbours across the boundary between the threads of
+                    // a column come from LDS (updated in the previous half sweep)
+                    const C below = m > 0 ? uu[ka - 1] : (tz == 0 ? zlo : sh[kbase - 1][ty + 1][tx + 1]);
+                    const C above = kb + 1 < CZ ? uu[kb + 1] : (tz == NTZ - 1 ? zhi : sh[kbase + CZ][ty + 1][tx + 1]);
+                    const C uzm = sel ? uu[ka] : below;
+                    const C uzp = sel ? above : uu[kb];
                     const C wzm_ = sel ? wzp[ka] : (m > 0 ? wzp[ka - 1] : wzlo);
                     const C wzp_ = sel ? wzp[kb] : wzp[ka];
                     const C a = (sel ? ff[kb] : ff[ka]) + (sel ? wxp[kb] : wxp[ka]) * sh[kk][ty + 1][tx + 2] +
@@ -365,8 +376,8 @@ k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict
         }
 
 #pragma unroll
-        for (int kk = 0; kk < kGsTz; kk++) {
-            if (col && k0 + kk < nz) {
+        for (int kk = 0; kk < CZ; kk++) {
+            if (col && kc + kk < nz) {
                 const long s = s0 + kk * nxy;
                 uo[s] = uu[kk];
                 if constexpr (MODE == 2) {
@@ -377,7 +388,7 @@ k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict
         }
     }
     if (MODE == 2) {
-        block_sum<2>(acc);
+        block_sum_n<2, kGsThreads>(acc);
         if (threadIdx.x == 0) { Pdot[2 * lb] = acc[0]; Pdot[2 * lb + 1] = acc[1]; }
     }
 }
@@ -1461,12 +1472,12 @@ static T *vcycle_coarse(hipStream_t st, Hierarchy &H, int l, const CycleParams &
     Level &C = H.lev[l + 1];
     if (cp.smoother == HMFEM_SMOOTH_BLOCK_GS) {
         const int GT = grid_for(gs_tiles(L.d), 1);
-        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 0>), dim3(GT), dim3(kBlock), 0, st, o, cp.gs, f,
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 0>), dim3(GT), dim3(kGsThreads), 0, st, o, cp.gs, f,
                     (const T *)nullptr, u, 0, 0, 0, 0, 0, (const T *)nullptr, (T)0, (const double *)nullptr, 1,
                     (double *)nullptr, done);
         launch_restrict<T, T>(st, P, HMFEM_PROF_COARSE, L, C, f, (const T *)u, done);
         T *uc = vcycle_coarse<T>(st, H, l + 1, cp, P, done);
-        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 1>), dim3(GT), dim3(kBlock), 0, st, o, cp.gs, f,
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 1>), dim3(GT), dim3(kGsThreads), 0, st, o, cp.gs, f,
                     (const T *)u, t, lg2f(L.fx), lg2f(L.fy), lg2f(L.fz), C.d.nx, C.d.ny, (const T *)uc, (T)cp.ocf,
                     (const double *)nullptr, 1, (double *)nullptr, done);
         L.u = t; L.t = u;
@@ -1510,14 +1521,14 @@ static T *vcycle_fine(hipStream_t st, Hierarchy &H, const double *r, const doubl
     if (cp.smoother == HMFEM_SMOOTH_BLOCK_GS) {
         Level &C = H.lev[1];
         const int nt = gs_tiles(L.d);
-        PROF_LAUNCH(P, HMFEM_PROF_SMOOTH0, (k_gs_tile<T, double, 0>), dim3(grid_for(nt, 1)), dim3(kBlock), 0, st, o,
+        PROF_LAUNCH(P, HMFEM_PROF_SMOOTH0, (k_gs_tile<T, double, 0>), dim3(grid_for(nt, 1)), dim3(kGsThreads), 0, st, o,
                     cp.gs, r, (const T *)nullptr, u, 0, 0, 0, 0, 0, (const T *)nullptr, (T)0,
                     (const double *)nullptr, 1, (double *)nullptr, done);
         launch_restrict<T, double>(st, P, HMFEM_PROF_RESTRICT, L, C, r, (const T *)u, done);
         T *uc = vcycle_coarse<T>(st, H, 1, cp, P, done);
         // the dot partials must come from exactly S.ch.G blocks
         const int tpb = (nt + S.ch.G - 1) / S.ch.G;
-        PROF_LAUNCH(P, HMFEM_PROF_PROLONG, (k_gs_tile<T, double, 2>), dim3(S.ch.G), dim3(kBlock), 0, st, o, cp.gs,
+        PROF_LAUNCH(P, HMFEM_PROF_PROLONG, (k_gs_tile<T, double, 2>), dim3(S.ch.G), dim3(kGsThreads), 0, st, o, cp.gs,
                     r, (const T *)u, t, lg2f(L.fx), lg2f(L.fy), lg2f(L.fz), C.d.nx, C.d.ny, (const T *)uc,
                     (T)cp.ocf, v, tpb, S.Pdot, done);
         return t;
