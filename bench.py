@@ -136,6 +136,8 @@ def run_batch(args, w, torch, S):
     solver_stream = torch.cuda.ExternalStream(ctx.stream)
     prec = S.PREC_FP64 if args.precision == "fp64" else S.PREC_FP32
     opt = S.default_options(rtol=1e-8, precision=prec, check_every=2)
+    if args.smoother:
+        opt.smoother = S.SMOOTH_JACOBI if args.smoother == "jacobi" else S.SMOOTH_BLOCK_GS
 
     def step(o=opt):
         with torch.cuda.stream(solver_stream):
@@ -158,6 +160,7 @@ def run_batch(args, w, torch, S):
     fp64 = None
     if not args.no_fp64 and prec == S.PREC_FP32:
         o64 = S.default_options(rtol=1e-8, precision=S.PREC_FP64, check_every=2)
+        o64.smoother = opt.smoother
         step(o64)
         torch.cuda.synchronize()
         ta = time.perf_counter()
@@ -184,7 +187,9 @@ def run_batch(args, w, torch, S):
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64" if prec == S.PREC_FP32 else "f64",
         "data": "synthetic",
-        "config": {"workload": w["label"], "instances": nb, "cells": int(ntot), "rtol": 1e-8},
+        "config": {"workload": w["label"], "instances": nb, "cells": int(ntot), "rtol": 1e-8,
+                   "smoother": "block red-black Gauss-Seidel" if opt.smoother == S.SMOOTH_BLOCK_GS
+                   else "damped Jacobi"},
         "per_instance_done_ms": {"p50": float(np.percentile(done, 50)), "p99": float(np.percentile(done, 99)),
                                  "max": float(done[-1])},
         "iterations": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},

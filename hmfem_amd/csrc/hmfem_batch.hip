@@ -13,7 +13,7 @@
 // instance sum only their own partials). Kernels skip converged instances, so
 // a finished instance costs nothing while the others iterate. One captured
 // HIP graph replays a CG iteration of the whole batch.
-#include "hmfem_stencil.h"
+#include "hmfem_gs.h"
 
 #include <string.h>
 
@@ -25,10 +25,12 @@ namespace hmfem {
 struct BInst {
     int nx, ny, nz, n, off, blk0, nblk;   // this level
     int fx, fy, fz;                       // aggregation factors to the next level
+    int tile0, ntile;                     // Gauss-Seidel tiles of this instance
 };
 
 struct CInst {                            // fine level, chunked launches (CG, final sweep)
     int nx, ny, nz, n, off, cblk0, cG, chunk;
+    int tpb;                              // smoother tiles per chunk block
 };
 
 template <typename T>
@@ -193,6 +195,53 @@ kb_final_sweep_dots(const CInst *__restrict__ CI, const int *__restrict__ cbmap,
         }
     }
     block_sum<2>(acc);
+    if (threadIdx.x == 0) { Pdot[2 * blockIdx.x] = acc[0]; Pdot[2 * blockIdx.x + 1] = acc[1]; }
+}
+
+// Block Gauss-Seidel smoothing (MODE 0 from zero, MODE 1 after the coarse
+// correction) of one tile per block; tmap gives the block's instance.
+template <typename T, typename FT, int MODE>
+__global__ void __launch_bounds__(kGsThreads)
+kb_gs_tile(const BInst *__restrict__ D, const BInst *__restrict__ Dc, const int *__restrict__ tmap, LPtr<T> A,
+           int sweeps, const FT *__restrict__ f, const T *__restrict__ u, T *__restrict__ uo,
+           const T *__restrict__ uc, T ocf, const CgScalars *sc)
+{
+    const int q = tmap[blockIdx.x];
+    if (sc[q].done) return;
+    const BInst I = D[q];
+    const OpArgs<T> o = bop(A, I.off, I.nx, I.ny, I.nz);
+    double acc[2] = {0.0, 0.0};
+    if (MODE == 0) {
+        gs_tile_body<T, FT, 0>(o, sweeps, f + I.off, (const T *)nullptr, uo + I.off, 0, 0, 0, 0, 0,
+                               (const T *)nullptr, (T)0, (const double *)nullptr, blockIdx.x - I.tile0, acc);
+    } else {
+        const BInst Ic = Dc[q];
+        gs_tile_body<T, FT, 1>(o, sweeps, f + I.off, u + I.off, uo + I.off, I.fx == 2, I.fy == 2, I.fz == 2, Ic.nx,
+                               Ic.ny, uc + Ic.off, ocf, (const double *)nullptr, blockIdx.x - I.tile0, acc);
+    }
+}
+
+// Post-smoothing of the fine level with the CG dots of each instance, over
+// the instance's chunk blocks (tiles [lb tpb, (lb + 1) tpb) of it each).
+template <typename T>
+__global__ void __launch_bounds__(kGsThreads)
+kb_gs_tile_dots(const CInst *__restrict__ CI, const int *__restrict__ cbmap, const BInst *__restrict__ D,
+                const BInst *__restrict__ Dc, LPtr<T> A, int sweeps, const double *__restrict__ f,
+                const T *__restrict__ u, T *__restrict__ uo, const T *__restrict__ uc, T ocf,
+                const double *__restrict__ v, double *Pdot, const CgScalars *sc)
+{
+    const int q = cbmap[blockIdx.x];
+    if (sc[q].done) return;
+    const CInst C = CI[q];
+    const BInst I = D[q], Ic = Dc[q];
+    const OpArgs<T> o = bop(A, I.off, I.nx, I.ny, I.nz);
+    const int lb = blockIdx.x - C.cblk0;
+    const int tbeg = lb * C.tpb, tend = min(I.ntile, tbeg + C.tpb);
+    double acc[2] = {0.0, 0.0};
+    for (int t = tbeg; t < tend; t++)
+        gs_tile_body<T, double, 2>(o, sweeps, f + I.off, u + I.off, uo + I.off, I.fx == 2, I.fy == 2, I.fz == 2,
+                                   Ic.nx, Ic.ny, uc + Ic.off, ocf, v ? v + I.off : nullptr, t, acc);
+    block_sum_n<2, kGsThreads>(acc);
     if (threadIdx.x == 0) { Pdot[2 * blockIdx.x] = acc[0]; Pdot[2 * blockIdx.x + 1] = acc[1]; }
 }
 
@@ -468,6 +517,8 @@ struct BLevel {
     int *bmap = nullptr;
     int nblk = 0;
     long ncell = 0;
+    int *tmap = nullptr;      // smoother tile -> instance
+    int ntiles = 0;
     LPtr<double> a64{};
     LPtr<float> a32{};
     double *e64 = nullptr, *wx64 = nullptr, *wy64 = nullptr, *wz64 = nullptr, *di64 = nullptr;
@@ -541,6 +592,9 @@ void plan_batch(int ninst, const int *dims, int coarse_max, Batch &B)
             I.fx = coarsen && d.nx > 1 ? 2 : 1;
             I.fy = coarsen && d.ny > 1 ? 2 : 1;
             I.fz = coarsen && d.nz > 1 ? 2 : 1;
+            I.tile0 = L.ntiles;
+            I.ntile = gs_tiles(d);
+            L.ntiles += I.ntile;
             off += I.n;
             blk += I.nblk;
             cur[q] = Dims{(d.nx + I.fx - 1) / I.fx, (d.ny + I.fy - 1) / I.fy, (d.nz + I.fz - 1) / I.fz};
@@ -561,6 +615,7 @@ void plan_batch(int ninst, const int *dims, int coarse_max, Batch &B)
         chunk = (chunk + kBlock - 1) / kBlock * kBlock;
         G = (I.n + chunk - 1) / chunk;
         c.cblk0 = cb; c.cG = G; c.chunk = chunk;
+        c.tpb = (I.ntile + G - 1) / G;
         cb += G;
     }
     B.ncblk = cb;
@@ -574,7 +629,7 @@ size_t batch_bytes(const Batch &B, int prec)
         if (l > 0) b += 4 * al(n * 8);
         if (vs == 8) b += al(n * 8);
         b += 5 * al(n * 4) + 3 * al(n * vs);
-        b += al(sizeof(BInst) * B.ninst) + al(sizeof(int) * B.lev[l].nblk);
+        b += al(sizeof(BInst) * B.ninst) + al(sizeof(int) * B.lev[l].nblk) + al(sizeof(int) * B.lev[l].ntiles);
     }
     size_t n0 = B.lev[0].ncell;
     b += 4 * al(n0 * 8);                                      // r, s0, s1, v
@@ -594,6 +649,7 @@ LPtr<double> lp<double>(const BLevel &L) { return LPtr<double>{L.e64, L.wx64, L.
 struct BCycle {
     int nu;
     double omega, ocf;
+    int smoother, gs;
 };
 
 template <typename T, typename FT>
@@ -607,6 +663,25 @@ T *vb_level(hipStream_t st, Batch &B, int l, const FT *f, const double *vdot, co
         return u;
     }
     LPtr<T> A = lp<T>(L);
+    if (cp.smoother == HMFEM_SMOOTH_BLOCK_GS) {
+        BLevel &C = B.lev[l + 1];
+        hipLaunchKernelGGL((kb_gs_tile<T, FT, 0>), dim3(L.ntiles), dim3(kGsThreads), 0, st, L.d, C.d, L.tmap, A,
+                           cp.gs, f, (const T *)nullptr, u, (const T *)nullptr, (T)0, B.sc);
+        hipLaunchKernelGGL((kb_resid_restrict<T, FT>), dim3(C.nblk), dim3(kBlock), 0, st, L.d, C.d, C.bmap, A, f, u,
+                           (T *)C.f, B.sc);
+        T *uc = vb_level<T, T>(st, B, l + 1, (const T *)C.f, nullptr, cp, nullptr);
+        if (l == 0) {
+            hipLaunchKernelGGL(kb_gs_tile_dots<T>, dim3(B.ncblk), dim3(kGsThreads), 0, st, B.dch, B.cbmap, L.d, C.d,
+                               A, cp.gs, (const double *)f, (const T *)u, t, (const T *)uc, (T)cp.ocf, vdot, Pdot,
+                               B.sc);
+            return t;
+        }
+        hipLaunchKernelGGL((kb_gs_tile<T, FT, 1>), dim3(L.ntiles), dim3(kGsThreads), 0, st, L.d, C.d, L.tmap, A,
+                           cp.gs, f, (const T *)u, t, (const T *)uc, (T)cp.ocf, B.sc);
+        L.t = u;
+        L.u = t;
+        return t;
+    }
     hipLaunchKernelGGL((kb_smooth_zero<T, FT>), dim3(L.nblk), dim3(kBlock), 0, st, L.d, L.bmap, A, f, u, om,
                        cp.nu >= 2 ? 2 : 1, B.sc);
     for (int q = 2; q < cp.nu; q++) {
@@ -666,7 +741,8 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
         L.f = cv.take<char>(n * vs); L.u = cv.take<char>(n * vs); L.t = cv.take<char>(n * vs);
         L.d = cv.take<BInst>(B.ninst);
         L.bmap = cv.take<int>(L.nblk);
-        if (!L.bmap || !L.t || !L.di32) return HMFEM_ERR_NOMEM;
+        L.tmap = cv.take<int>(L.ntiles);
+        if (!L.bmap || !L.tmap || !L.t || !L.di32) return HMFEM_ERR_NOMEM;
     }
     size_t n0 = B.lev[0].ncell;
     B.r = cv.take<double>(n0); B.s0 = cv.take<double>(n0); B.s1 = cv.take<double>(n0); B.v = cv.take<double>(n0);
@@ -684,7 +760,8 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     // descriptors
     {
         size_t total = 0;
-        for (int l = 0; l < B.nlev; l++) total += sizeof(BInst) * B.ninst + sizeof(int) * B.lev[l].nblk;
+        for (int l = 0; l < B.nlev; l++)
+            total += sizeof(BInst) * B.ninst + sizeof(int) * (B.lev[l].nblk + B.lev[l].ntiles);
         total += sizeof(CInst) * B.ninst + sizeof(int) * B.ncblk + sizeof(CgScalars) * B.ninst;
         stage.resize(total);
         char *p = stage.data();
@@ -699,6 +776,12 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
             memcpy(p, bm.data(), sizeof(int) * L.nblk);
             HMFEM_CHECK(hipMemcpyAsync(L.bmap, p, sizeof(int) * L.nblk, hipMemcpyHostToDevice, st));
             p += sizeof(int) * L.nblk;
+            std::vector<int> tm(L.ntiles);
+            for (int q = 0; q < B.ninst; q++)
+                for (int k = 0; k < L.h[q].ntile; k++) tm[L.h[q].tile0 + k] = q;
+            memcpy(p, tm.data(), sizeof(int) * L.ntiles);
+            HMFEM_CHECK(hipMemcpyAsync(L.tmap, p, sizeof(int) * L.ntiles, hipMemcpyHostToDevice, st));
+            p += sizeof(int) * L.ntiles;
         }
         std::vector<int> cbm(B.ncblk);
         for (int q = 0; q < B.ninst; q++)
@@ -744,7 +827,7 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     HMFEM_CHECK(hipEventRecord(ev[1], st));
 
     // CG
-    BCycle cp{opt.nu, opt.omega, opt.ocf};
+    BCycle cp{opt.nu, opt.omega, opt.ocf, opt.smoother, opt.gs_sweeps < 1 ? 1 : opt.gs_sweeps};
     LPtr<double> A64 = lp<double>(B.lev[0]);
     hipLaunchKernelGGL(kb_cg_init, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, A64, b, x, B.r, B.Pinit);
     auto vcycle = [&](const double *vd) -> void * {

@@ -184,8 +184,7 @@ int hmfem_solve_structured_device(hmfem_context *ctx, int nx, int ny, int nz, co
    (device). Per-instance results go to the optional host arrays
    iterations, rel_residual, done_ms (time from the start of the solve
    phase at which the instance's stop test first held, resolution
-   check_every iterations) and div_max. The batch smooths with damped
-   Jacobi (opt->nu, omega, ocf) whatever opt->smoother says. */
+   check_every iterations) and div_max. */
 typedef struct {
     int status;              /* HMFEM_OK if every instance converged */
     int max_iterations;      /* largest iteration count over the instances */

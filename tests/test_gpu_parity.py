@@ -280,7 +280,8 @@ def test_graph_replay_matches_eager(smoother, ctx, torch):
         assert np.array_equal(p.cpu().numpy().view(np.uint64), ref.view(np.uint64)), rep
 
 
-def test_batch_uneven_matches_single(ctx, torch):
+@pytest.mark.parametrize("smoother", [S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS], ids=["jacobi", "blockgs"])
+def test_batch_uneven_matches_single(smoother, ctx, torch):
     """Uneven instances solved together give each instance's own solution:
     against the single-instance solver and, for the small ones, the oracle."""
     specs = [((20, 16, 12), "lognormal", 1), ((32, 24, 16), "karst", 2), ((8, 8, 8), "lognormal", 3),
@@ -294,7 +295,8 @@ def test_batch_uneven_matches_single(ctx, torch):
     ntot = sum(P.n for P in probs)
     p = torch.zeros(ntot, dtype=torch.float64, device="cuda")
     v = torch.zeros(ntot * 7, dtype=torch.float64, device="cuda")
-    o = S.default_options(rtol=1e-12)
+    o = S.default_options(rtol=1e-12, smoother=smoother)
+    _say("batch smoother", smoother)
     st, per = ctx.reservoir_batch([(P.nx, P.ny, P.nz) for P in probs], K, K, K, bval, p, v, o)
     assert st.status == S.OK, (st.as_dict(), per)
     assert np.all(per["iterations"] > 0) and np.all(per["done_ms"] >= 0)
@@ -303,9 +305,7 @@ def test_batch_uneven_matches_single(ctx, torch):
     for P, (d, k, s) in zip(probs, specs):
         D = Dev(P, torch)
         ps = torch.zeros(P.n, dtype=torch.float64, device="cuda")
-        # the batch smooths with damped Jacobi; compare with the same smoother
-        ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, ps, None,
-                      S.default_options(rtol=1e-12, smoother=S.SMOOTH_JACOBI))
+        ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, ps, None, o)
         x = xb[off:off + P.n]
         assert rel_l2(x, ps.cpu().numpy()) < 1e-9, d
         if P.n <= 6000:
