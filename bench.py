@@ -338,7 +338,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; c3, c4: 3; c5: 1)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 3; c3, c4, c5: 1)")
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16"],
+                    help="hierarchy precision; fp16 = FP32 hierarchy with the fine level stored in FP16")
     ap.add_argument("--smoother", default=None, choices=["jacobi", "blockgs"],
                     help="smoother of the multi-block levels (default: the library's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -392,7 +393,7 @@ def main():
     v = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
     g = S.Context.grid(nx, ny, nz, None, 1.0, (0, 0, 1, 1, 1, 1))
     ctx = S.Context()
-    prec = S.PREC_FP64 if args.precision == "fp64" else S.PREC_FP32
+    prec = {"fp64": S.PREC_FP64, "fp16": S.PREC_FP16}.get(args.precision, S.PREC_FP32)
     opt = S.default_options(rtol=rtol, precision=prec)
     if args.smoother:
         opt.smoother = S.SMOOTH_JACOBI if args.smoother == "jacobi" else S.SMOOTH_BLOCK_GS
@@ -463,6 +464,21 @@ def main():
         pf = p.clone()
         fp64["rel_l2_fp32amg_vs_fp64_1e-12"] = float(torch.linalg.norm(p32 - pf) / torch.linalg.norm(pf))
 
+    # FP16-stored fine level (ablation) on the same workload
+    fp16 = None
+    if not args.no_fp64 and prec == S.PREC_FP32 and opt.smoother == S.SMOOTH_BLOCK_GS:
+        o16 = S.default_options(rtol=rtol, precision=S.PREC_FP16)
+        step(o16)
+        torch.cuda.synchronize()
+        k16 = max(3, args.steps // 2)
+        ta = time.perf_counter()
+        for _ in range(k16):
+            s16 = step(o16)
+        torch.cuda.synchronize()
+        fp16 = {"value": k16 / (time.perf_counter() - ta), "iterations": s16.iterations,
+                "rel_residual": s16.rel_residual}
+        fp16["speedup_over_fp32"] = fp16["value"] / value
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -473,10 +489,12 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64" if prec == S.PREC_FP32 else "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": {S.PREC_FP32: "f32/f64", S.PREC_FP16: "f16/f32/f64"}.get(prec, "f64"),
         "data": "synthetic",
-        "config": {"workload": w["label"], "cells": n, "precision": "FP32 AMG in FP64 flexible CG"
-                   if prec == S.PREC_FP32 else "FP64 AMG in FP64 flexible CG", "rtol": rtol,
+        "config": {"workload": w["label"], "cells": n, "precision": {
+                       S.PREC_FP32: "FP32 AMG in FP64 flexible CG",
+                       S.PREC_FP16: "FP32 AMG with an FP16-stored fine level in FP64 flexible CG"}.get(
+                       prec, "FP64 AMG in FP64 flexible CG"), "rtol": rtol,
                    "smoother": "block red-black Gauss-Seidel" if opt.smoother == S.SMOOTH_BLOCK_GS
                    else "damped Jacobi"},
         "iterations": st.iterations, "levels": st.levels,
@@ -489,6 +507,7 @@ def main():
                      "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": 1e3 * avg_ms},
         "kernels": kernels,
         "fp64_hierarchy": fp64,
+        "fp16_fine_level": fp16,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -31,11 +31,22 @@ __host__ __device__ inline int gs_tiles(const Dims &d)
 // sx, sy, sz: log2 of the aggregation factors (0 or 1) towards uc's level.
 // gs_tile_body smooths tile t of the grid of o; MODE 2 adds its dot
 // products to acc. Every thread of the workgroup calls it.
-template <typename T, typename FT, int MODE>
+// The fine level stored in FP16: couplings of the symmetrically scaled
+// operator D^-1/2 A D^-1/2 (unit diagonal, couplings w / sqrt(d_s d_nb) in
+// (0, 1)) and the FP32 scale sq = sqrt(d). With it (H16) the smoother runs on
+// scaled values u^ = sq u, f^ = f / sq, the same red-black update with a unit
+// diagonal, and returns u = u^ / sq.
+struct Fine16 {
+    const __half *wx, *wy, *wz;
+    const float *sq;
+};
+
+template <typename T, typename FT, int MODE, bool H16 = false>
 __device__ __forceinline__ void gs_tile_body(const OpArgs<T> &o, int sweeps, const FT *__restrict__ f,
                                              const T *__restrict__ u, T *__restrict__ uo, int sx, int sy, int sz,
                                              int cx, int cy, const T *__restrict__ uc, T ocf,
-                                             const double *__restrict__ v, int t, double (&acc)[2])
+                                             const double *__restrict__ v, int t, double (&acc)[2],
+                                             Fine16 h = Fine16{})
 {
     using C = T;
     constexpr int CZ = kGsCz, NTZ = kGsTz / kGsCz;
@@ -52,6 +63,7 @@ __device__ __forceinline__ void gs_tile_body(const OpArgs<T> &o, int sweeps, con
         if (MODE == 0 || ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) return (C)0;
         long s = ii + (long)jj * nx + (long)kk * nxy;
         int I = (ii >> sx) + cx * ((jj >> sy) + cy * (kk >> sz));
+        if constexpr (H16) return ((C)u[s] + ocf * (C)uc[I]) * (C)h.sq[s];
         return (C)u[s] + ocf * (C)uc[I];
     };
 
@@ -69,17 +81,38 @@ __device__ __forceinline__ void gs_tile_body(const OpArgs<T> &o, int sweeps, con
     for (int kk = 0; kk < CZ; kk++) {
         const bool act = col && kc + kk < nz;
         const long s = s0 + kk * nxy;
-        wxp[kk] = act ? (C)o.wx[s] : (C)0;
-        wxm[kk] = act && i > 0 ? (C)o.wx[s - 1] : (C)0;
-        wyp[kk] = act ? (C)o.wy[s] : (C)0;
-        wym[kk] = act && j > 0 ? (C)o.wy[s - nx] : (C)0;
-        wzp[kk] = act ? (C)o.wz[s] : (C)0;
-        dinv[kk] = act ? (C)o.di[s] : (C)0;
         double fv = act ? (double)f[s] : 0.0;
-        ff[kk] = (C)fv;
+        if constexpr (H16) {
+            wxp[kk] = act ? (C)__half2float(h.wx[s]) : (C)0;
+            wxm[kk] = act && i > 0 ? (C)__half2float(h.wx[s - 1]) : (C)0;
+            wyp[kk] = act ? (C)__half2float(h.wy[s]) : (C)0;
+            wym[kk] = act && j > 0 ? (C)__half2float(h.wy[s - nx]) : (C)0;
+            wzp[kk] = act ? (C)__half2float(h.wz[s]) : (C)0;
+            dinv[kk] = act ? (C)h.sq[s] : (C)0;   // holds sq until the sweeps (unit diagonal there)
+            ff[kk] = act ? (C)fv / dinv[kk] : (C)0;
+        } else {
+            wxp[kk] = act ? (C)o.wx[s] : (C)0;
+            wxm[kk] = act && i > 0 ? (C)o.wx[s - 1] : (C)0;
+            wyp[kk] = act ? (C)o.wy[s] : (C)0;
+            wym[kk] = act && j > 0 ? (C)o.wy[s - nx] : (C)0;
+            wzp[kk] = act ? (C)o.wz[s] : (C)0;
+            dinv[kk] = act ? (C)o.di[s] : (C)0;
+            ff[kk] = (C)fv;
+        }
         if constexpr (MODE == 2) fd[kk] = fv;
     }
-    const C wzlo = col && kc > 0 && kc < nz ? (C)o.wz[s0 - nxy] : (C)0;
+    C wzlo;
+    if constexpr (H16) wzlo = col && kc > 0 && kc < nz ? (C)__half2float(h.wz[s0 - nxy]) : (C)0;
+    else wzlo = col && kc > 0 && kc < nz ? (C)o.wz[s0 - nxy] : (C)0;
+    // scaled mode: the cell's scale, and the unit diagonal of active cells
+    C scale[H16 ? CZ : 1];
+    if constexpr (H16) {
+#pragma unroll
+        for (int kk = 0; kk < CZ; kk++) {
+            scale[kk] = dinv[kk];
+            dinv[kk] = dinv[kk] > (C)0 ? (C)1 : (C)0;
+        }
+    }
 
     // starting values of this thread's cells, the tile's two z neighbours
     // of the column and RPER cells of the x/y ring, all loads issued
@@ -143,12 +176,55 @@ __device__ __forceinline__ void gs_tile_body(const OpArgs<T> &o, int sweeps, con
     for (int kk = 0; kk < CZ; kk++) {
         if (col && kc + kk < nz) {
             const long s = s0 + kk * nxy;
-            uo[s] = uu[kk];
+            C z = uu[kk];
+            if constexpr (H16) z = z / scale[kk];
+            uo[s] = z;
             if constexpr (MODE == 2) {
-                acc[0] += (double)uu[kk] * fd[kk];
-                if (v) acc[1] += (double)uu[kk] * v[s];
+                acc[0] += (double)z * fd[kk];
+                if (v) acc[1] += (double)z * v[s];
             }
         }
+    }
+}
+
+// Residual f - A u summed over 2x2x2 aggregates on the FP16-stored fine level:
+// (A u)_s = sq_s (sq_s u_s - sum_nb w^_nb sq_nb u_nb). Bricks of 32 x 4 x 2
+// fine cells as in the FP32 kernel.
+template <typename T, typename FT>
+__global__ void __launch_bounds__(kBlock)
+k_restrict_brick16(int nx, int ny, int nz, Fine16 h, int cx, int cy, int cz, const FT *__restrict__ f,
+                   const T *__restrict__ u, T *__restrict__ fc, const int *done)
+{
+    using C = T;
+    if (*done) return;
+    __shared__ C upper[2][16];
+    const int nbx = (nx + 31) / 32, nby = (ny + 3) / 4;
+    const int b = xcd_block();
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int t = threadIdx.x;
+    const int i = bx * 32 + (t & 31), j = by * 4 + ((t >> 5) & 3), k = bz * 2 + (t >> 7);
+    C r = 0;
+    if (i < nx && j < ny && k < nz) {
+        const int nxy = nx * ny;
+        const int s = i + j * nx + k * nxy;
+        auto su = [&](int q) -> C { return (C)h.sq[q] * (C)u[q]; };
+        C a = su(s);
+        if (i < nx - 1) a -= (C)__half2float(h.wx[s]) * su(s + 1);
+        if (i > 0) a -= (C)__half2float(h.wx[s - 1]) * su(s - 1);
+        if (j < ny - 1) a -= (C)__half2float(h.wy[s]) * su(s + nx);
+        if (j > 0) a -= (C)__half2float(h.wy[s - nx]) * su(s - nx);
+        if (k < nz - 1) a -= (C)__half2float(h.wz[s]) * su(s + nxy);
+        if (k > 0) a -= (C)__half2float(h.wz[s - nxy]) * su(s - nxy);
+        r = (C)f[s] - (C)h.sq[s] * a;
+    }
+    r += __shfl_xor(r, 1);
+    r += __shfl_xor(r, 32);
+    const int lane = t & 63, cjl = (t >> 6) & 1, cil = (t & 31) >> 1;
+    if ((t >> 7) == 1 && lane < 32 && !(lane & 1)) upper[cjl][cil] = r;
+    __syncthreads();
+    if ((t >> 7) == 0 && lane < 32 && !(lane & 1)) {
+        const int ci = bx * 16 + cil, cj = by * 2 + cjl, ck = bz;
+        if (ci < cx && cj < cy && ck < cz) fc[ci + cx * (cj + cy * ck)] = r + upper[cjl][cil];
     }
 }
 

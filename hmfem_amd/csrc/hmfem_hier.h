@@ -59,6 +59,9 @@ struct Hierarchy {
     int tail_nops = 0;
     int tail_lds_bytes = 0;    // dynamic LDS of the tail kernel (0: global mode)
     int tail_lds_inv = -1;     // LDS offset of the dense inverse (-1: global mode)
+    // fine level stored in FP16 (HMFEM_PREC_FP16): scaled couplings and scale
+    __half *f16x = nullptr, *f16y = nullptr, *f16z = nullptr;
+    float *f16sq = nullptr;
 };
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }

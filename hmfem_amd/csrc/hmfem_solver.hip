@@ -236,11 +236,11 @@ k_final_sweep_dots(OpArgs<T> o, int fx, int fy, int cx, int cy, int fz, const T 
 // ---------------------------------------------------------------- block Gauss-Seidel smoother
 //
 // Each (logical) block smooths tiles [lb * tpb, (lb + 1) * tpb) of the level.
-template <typename T, typename FT, int MODE>
+template <typename T, typename FT, int MODE, bool H16 = false>
 __global__ void __launch_bounds__(kGsThreads)
 k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict__ u, T *__restrict__ uo, int sx,
           int sy, int sz, int cx, int cy, const T *__restrict__ uc, T ocf, const double *__restrict__ v, int tpb,
-          double *Pdot, const int *done)
+          double *Pdot, const int *done, Fine16 h = Fine16{})
 {
     if (*done) return;
     const int ntiles = gs_tiles(Dims{o.nx, o.ny, o.nz});
@@ -248,7 +248,7 @@ k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict
     const int tbeg = lb * tpb, tend = min(ntiles, tbeg + tpb);
     double acc[2] = {0.0, 0.0};
     for (int t = tbeg; t < tend; t++)
-        gs_tile_body<T, FT, MODE>(o, sweeps, f, u, uo, sx, sy, sz, cx, cy, uc, ocf, v, t, acc);
+        gs_tile_body<T, FT, MODE, H16>(o, sweeps, f, u, uo, sx, sy, sz, cx, cy, uc, ocf, v, t, acc, h);
     if (MODE == 2) {
         block_sum_n<2, kGsThreads>(acc);
         if (threadIdx.x == 0) { Pdot[2 * lb] = acc[0]; Pdot[2 * lb + 1] = acc[1]; }
@@ -683,6 +683,32 @@ k_to_f32(long n, const double *e, const double *wx, const double *wy, const doub
     e32[s] = (float)e[s]; wx32[s] = (float)wx[s]; wy32[s] = (float)wy[s]; wz32[s] = (float)wz[s];
 }
 
+// sq = sqrt(d) of the fine level from the FP64 stencil.
+__global__ void __launch_bounds__(kBlock)
+k_fine16_scale(OpArgs<double> o, float *sq)
+{
+    const int n = o.nx * o.ny * o.nz;
+    const int s = xcd_block() * kBlock + threadIdx.x;
+    if (s >= n) return;
+    Cell c = cell_at(o, s);
+    sq[s] = (float)sqrt(diag_at<double, double>(o, c));
+}
+
+// Scaled FP16 couplings w / sqrt(d_s d_nb) towards +x, +y, +z.
+__global__ void __launch_bounds__(kBlock)
+k_fine16_couplings(OpArgs<double> o, const float *__restrict__ sq, __half *hx, __half *hy, __half *hz)
+{
+    const int n = o.nx * o.ny * o.nz;
+    const int s = xcd_block() * kBlock + threadIdx.x;
+    if (s >= n) return;
+    Cell c = cell_at(o, s);
+    const int nx = o.nx, nxy = o.nx * o.ny;
+    const double ss = sq[s];
+    hx[s] = __float2half((float)(c.i < o.nx - 1 ? o.wx[s] / (ss * (double)sq[s + 1]) : 0.0));
+    hy[s] = __float2half((float)(c.j < o.ny - 1 ? o.wy[s] / (ss * (double)sq[s + nx]) : 0.0));
+    hz[s] = __float2half((float)(c.k < o.nz - 1 ? o.wz[s] / (ss * (double)sq[s + nxy]) : 0.0));
+}
+
 // Inverse diagonal of a level, in its storage type.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -1095,6 +1121,7 @@ size_t hierarchy_bytes(Dims fine, const hmfem_options &opt)
     }
     b += align256((size_t)kCoarseMax * kCoarseMax * 8);
     b += align256(sizeof(TailLevel<double>) * kMaxTailLevels) + align256(sizeof(int4) * kMaxTailOps);
+    if (opt.precision == HMFEM_PREC_FP16) b += 3 * align256(fine.n() * 2) + align256(fine.n() * 4);
     return b + 8192;
 }
 
@@ -1183,6 +1210,12 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const 
         L.r = ar.take<char>(n * vs); L.w = ar.take<char>(n * vs);
         if (!L.s32.di || !L.w || (l > 0 && !L.s64.wz) || (prec == HMFEM_PREC_FP64 && !L.s64.di)) return HMFEM_ERR_NOMEM;
     }
+    if (opt.precision == HMFEM_PREC_FP16 && H.nlev > 1) {
+        const size_t n = H.lev[0].d.n();
+        H.f16x = ar.take<__half>(n); H.f16y = ar.take<__half>(n); H.f16z = ar.take<__half>(n);
+        H.f16sq = ar.take<float>(n);
+        if (!H.f16sq) return HMFEM_ERR_NOMEM;
+    }
     H.nc = (int)H.lev[H.nlev - 1].d.n();
     if (H.nc > kCoarseMax) return HMFEM_ERR_ARG;   // too many levels would be needed
     H.inv = ar.take<double>((size_t)H.nc * H.nc);
@@ -1195,6 +1228,12 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const 
         long n = L.d.n();
         hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, n, L.s64.e, L.s64.wx, L.s64.wy,
                            L.s64.wz, L.s32.e, L.s32.wx, L.s32.wy, L.s32.wz);
+        if (H.f16sq) {
+            hipLaunchKernelGGL(k_fine16_scale, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, opargs<double>(L),
+                               H.f16sq);
+            hipLaunchKernelGGL(k_fine16_couplings, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+                               opargs<double>(L), (const float *)H.f16sq, H.f16x, H.f16y, H.f16z);
+        }
     }
     for (int l = 0; l + 1 < H.nlev; l++) {
         Level &L = H.lev[l], &C = H.lev[l + 1];
@@ -1380,6 +1419,28 @@ static T *vcycle_fine(hipStream_t st, Hierarchy &H, const double *r, const doubl
     OpArgs<T> o = opargs<T>(L);
     const T om = (T)cp.omega;
     int G = grid_for(L.d.n(), kBlock);
+    if (cp.smoother == HMFEM_SMOOTH_BLOCK_GS && H.f16sq) {
+        // fine level stored in FP16 (scaled), FP32 arithmetic
+        Level &C = H.lev[1];
+        const int nt = gs_tiles(L.d);
+        const Fine16 h{H.f16x, H.f16y, H.f16z, H.f16sq};
+        PROF_LAUNCH(P, HMFEM_PROF_SMOOTH0, (k_gs_tile<T, double, 0, true>), dim3(grid_for(nt, 1)), dim3(kGsThreads), 0,
+                    st, o, cp.gs, r, (const T *)nullptr, u, 0, 0, 0, 0, 0, (const T *)nullptr, (T)0,
+                    (const double *)nullptr, 1, (double *)nullptr, done, h);
+        if (L.fx == 2 && L.fy == 2 && L.fz == 2) {
+            const int nb = ((L.d.nx + 31) / 32) * ((L.d.ny + 3) / 4) * ((L.d.nz + 1) / 2);
+            PROF_LAUNCH(P, HMFEM_PROF_RESTRICT, (k_restrict_brick16<T, double>), dim3(grid_for(nb, 1)), dim3(kBlock),
+                        0, st, L.d.nx, L.d.ny, L.d.nz, h, C.d.nx, C.d.ny, C.d.nz, r, (const T *)u, (T *)C.f, done);
+        } else {
+            launch_restrict<T, double>(st, P, HMFEM_PROF_RESTRICT, L, C, r, (const T *)u, done);
+        }
+        T *uc = vcycle_coarse<T>(st, H, 1, cp, P, done);
+        const int tpb = (nt + S.ch.G - 1) / S.ch.G;
+        PROF_LAUNCH(P, HMFEM_PROF_PROLONG, (k_gs_tile<T, double, 2, true>), dim3(S.ch.G), dim3(kGsThreads), 0, st, o,
+                    cp.gs, r, (const T *)u, t, lg2f(L.fx), lg2f(L.fy), lg2f(L.fz), C.d.nx, C.d.ny, (const T *)uc,
+                    (T)cp.ocf, v, tpb, S.Pdot, done, h);
+        return t;
+    }
     if (cp.smoother == HMFEM_SMOOTH_BLOCK_GS) {
         Level &C = H.lev[1];
         const int nt = gs_tiles(L.d);
@@ -1522,6 +1583,7 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         prof.ev = sink->ev; prof.type = sink->type; prof.cap = sink->cap; prof.st = st;
         P = &prof;
     }
+    if (opt.precision == HMFEM_PREC_FP16 && opt.smoother != HMFEM_SMOOTH_BLOCK_GS) return HMFEM_ERR_ARG;
     Arena ar{arena, 0, arena_bytes};
     Hierarchy H;
     int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
@@ -1571,6 +1633,11 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
             // pre: wx, wy, wz, di, f; u out. post: the same plus u, u_c, v; z out (+dots)
             sink->bytes[HMFEM_PROF_PROLONG] = n0 * (c4 + 8 + 2 * vs + 8) + n1 * vs;
             sink->bytes[HMFEM_PROF_JACOBI] = 0.0;
+            if (H.f16sq) {   // FP16 couplings (6 B) and the FP32 scale (4 B) instead of 16 B
+                sink->bytes[HMFEM_PROF_SMOOTH0] = n0 * (10 + 8 + 4);
+                sink->bytes[HMFEM_PROF_RESTRICT] = n0 * (10 + 8 + 4) + n1 * 4;
+                sink->bytes[HMFEM_PROF_PROLONG] = n0 * (10 + 8 + 8 + 8) + n1 * 4;
+            }
         }
     }
     if (out) {

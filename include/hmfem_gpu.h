@@ -111,7 +111,11 @@ typedef struct {
 typedef struct {
     double rtol;             /* stopping tolerance (default 1e-8) */
     int max_iterations;      /* default 10000, as pcg.h's _max */
-    int precision;           /* storage/arithmetic of the AMG hierarchy (default FP32) */
+    int precision;           /* storage/arithmetic of the AMG hierarchy (default FP32). HMFEM_PREC_FP16:
+                                FP32 hierarchy whose fine level the smoother and the restriction
+                                read as FP16 couplings of the diagonally scaled operator plus an
+                                FP32 scale (single solves with HMFEM_SMOOTH_BLOCK_GS only; the
+                                batched and block solves use FP32) */
     int stop_rule;           /* default HMFEM_STOP_RESIDUAL */
     int nu;                  /* Jacobi sweeps before and after the coarse correction (default 3) */
     double omega;            /* Jacobi damping (default 0.95) */

@@ -134,7 +134,8 @@ def test_c1_closed_form_device(ctx, torch):
                            (S.PREC_FP64, S.STOP_RESIDUAL, S.SMOOTH_JACOBI),
                            (S.PREC_FP64, S.STOP_PRECONDITIONED, S.SMOOTH_JACOBI),
                            (S.PREC_FP32, S.STOP_RESIDUAL, S.SMOOTH_BLOCK_GS),
-                           (S.PREC_FP64, S.STOP_PRECONDITIONED, S.SMOOTH_BLOCK_GS)]:
+                           (S.PREC_FP64, S.STOP_PRECONDITIONED, S.SMOOTH_BLOCK_GS),
+                           (S.PREC_FP16, S.STOP_RESIDUAL, S.SMOOTH_BLOCK_GS)]:
         for _ in (0,):
             _say("C1 precision", prec, "rule", rule, "smoother", sm)
             p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
@@ -149,11 +150,18 @@ def test_c1_closed_form_device(ctx, torch):
 
 
 @pytest.mark.parametrize("kind", ["lognormal", "karst"])
-@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_FP64])
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_FP64, S.PREC_FP16], ids=["fp32", "fp64", "fp16"])
 @pytest.mark.parametrize("smoother", [S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS], ids=["jacobi", "blockgs"])
 def test_pipeline_vs_oracle_32(kind, prec, smoother, ctx, torch):
     P = make_problem((32, 32, 32), kind, seed=5, conduits=6, corr=4.0)
     _say("pipeline 32^3", kind, prec, smoother)
+    if prec == S.PREC_FP16 and smoother == S.SMOOTH_JACOBI:
+        # the FP16-stored fine level exists for the block Gauss-Seidel smoother only
+        D = Dev(P, torch)
+        p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+        with pytest.raises(RuntimeError):
+            ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(precision=prec, smoother=smoother))
+        return
     xo, operm, oind, ob = oracle_solution(P, eps=1e-12)
     D = Dev(P, torch)
     p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
