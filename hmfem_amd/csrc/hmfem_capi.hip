@@ -30,6 +30,8 @@ struct hmfem_context {
     std::vector<char> stage;   // host staging of small uploads (tail op list)
     GraphCache graph;          // captured CG iteration of the last solve
     long lds_limit = 65536;    // dynamic LDS a workgroup may use on this device
+    // options chosen by the autotuner, per grid shape and precision
+    std::map<std::vector<int>, hmfem_options> tuned;
 
     // device buffer `name` with at least `bytes`, grown on demand
     void *get(const char *name, size_t bytes)
@@ -80,6 +82,7 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->tail_lds = 1;
     o->smoother = HMFEM_SMOOTH_BLOCK_GS;
     o->gs_sweeps = 3;
+    o->autotune = 0;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)
@@ -234,12 +237,86 @@ extern "C" int hmfem_flux_device(hmfem_context *c, const hmfem_grid *g, const do
     return HMFEM_OK;
 }
 
+// Candidate cycles tried by the autotuner, in order (the library defaults
+// first): smoother sweeps, coarse-correction scaling, tail cycle.
+struct TuneCand {
+    int smoother, gs_sweeps, nu;
+    double ocf;
+    int tail_gamma;
+};
+static const TuneCand kTuneCands[] = {
+    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.5, 2}, {HMFEM_SMOOTH_BLOCK_GS, 2, 3, 2.5, 2},
+    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.0, 2}, {HMFEM_SMOOTH_BLOCK_GS, 2, 3, 2.0, 2},
+    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.8, 2}, {HMFEM_SMOOTH_BLOCK_GS, 4, 3, 2.5, 2},
+    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.5, 1}, {HMFEM_SMOOTH_BLOCK_GS, 3, 2, 2.5, 2},
+    {HMFEM_SMOOTH_JACOBI, 3, 3, 2.0, 2},   {HMFEM_SMOOTH_JACOBI, 3, 3, 2.5, 2},
+};
+
+extern "C" int hmfem_autotune_device(hmfem_context *c, const hmfem_grid *g, const double *Kx, const double *Ky,
+                                     const double *Kz, const double *const bval[6], hmfem_options *opt,
+                                     double *best_ms)
+{
+    if (!c || !grid_ok(g) || !Kx || !Ky || !Kz || !bval || !opt) return HMFEM_ERR_ARG;
+    const long n = (long)g->nx * g->ny * g->nz;
+    double *x = (double *)c->get("tune_x", n * 8);
+    if (!x) return HMFEM_ERR_NOMEM;
+    hmfem_options base = *opt;
+    base.autotune = 0;
+    hmfem_options best = base;
+    double tbest = 1e300;
+    for (const TuneCand &k : kTuneCands) {
+        if (base.precision == HMFEM_PREC_FP16 && k.smoother != HMFEM_SMOOTH_BLOCK_GS) continue;
+        hmfem_options o = base;
+        o.smoother = k.smoother;
+        o.gs_sweeps = k.gs_sweeps;
+        o.nu = k.nu;
+        o.ocf = k.ocf;
+        o.tail_gamma = k.tail_gamma;
+        double t = 0;
+        for (int rep = 0; rep < 2; rep++) {   // the first run also builds the graph
+            HMFEM_CHECK(hipMemsetAsync(x, 0, n * 8, c->st));
+            hmfem_status s{};
+            int rc = hmfem_reservoir_device(c, g, Kx, Ky, Kz, nullptr, nullptr, nullptr, nullptr, bval, x, nullptr,
+                                            &o, &s);
+            if (rc < 0) return rc;
+            if (s.status != HMFEM_OK) { t = 1e300; break; }
+            t = s.t_setup_ms + s.t_solve_ms;
+        }
+        if (t < tbest) { tbest = t; best = o; }
+    }
+    if (tbest >= 1e300) return HMFEM_NOT_CONVERGED;
+    best.autotune = opt->autotune;
+    *opt = best;
+    if (best_ms) *best_ms = tbest;
+    return HMFEM_OK;
+}
+
+static std::vector<int> tune_key(const hmfem_grid *g, const hmfem_options &o)
+{
+    return {g->nx, g->ny, g->nz, o.precision, o.stop_rule, (int)(-log10(o.rtol > 0 ? o.rtol : 1e-8) * 100)};
+}
+
 extern "C" int hmfem_reservoir_device(hmfem_context *c, const hmfem_grid *g, const double *Kx, const double *Ky,
                                       const double *Kz, const double *KI, const double *WI, const double *pc,
                                       const double *pw, const double *const bval[6], double *p, ELE *v,
-                                      const hmfem_options *opt, hmfem_status *st)
+                                      const hmfem_options *opt_in, hmfem_status *st)
 {
     if (!c || !grid_ok(g) || !Kx || !Ky || !Kz || !bval || !p) return HMFEM_ERR_ARG;
+    hmfem_options tuned_opt;
+    const hmfem_options *opt = opt_in;
+    if (opt_in && opt_in->autotune) {
+        auto key = tune_key(g, *opt_in);
+        auto it = c->tuned.find(key);
+        if (it == c->tuned.end()) {
+            tuned_opt = *opt_in;
+            int rc = hmfem_autotune_device(c, g, Kx, Ky, Kz, bval, &tuned_opt, nullptr);
+            if (rc) return rc;
+            it = c->tuned.emplace(key, tuned_opt).first;
+        }
+        tuned_opt = it->second;
+        tuned_opt.max_iterations = opt_in->max_iterations;
+        opt = &tuned_opt;
+    }
     Dims d{g->nx, g->ny, g->nz};
     long n = d.n();
     Stencil<double> A;

@@ -59,7 +59,7 @@ class Options(ctypes.Structure):
     _fields_ = [("rtol", c_dbl), ("max_iterations", c_int), ("precision", c_int), ("stop_rule", c_int),
                 ("nu", c_int), ("omega", c_dbl), ("ocf", c_dbl), ("coarse_max", c_int), ("check_every", c_int),
                 ("tail_max", c_int), ("tail_gamma", c_int), ("use_graph", c_int), ("tail_lds", c_int),
-                ("smoother", c_int), ("gs_sweeps", c_int)]
+                ("smoother", c_int), ("gs_sweeps", c_int), ("autotune", c_int)]
 
 
 class Status(ctypes.Structure):
@@ -128,6 +128,7 @@ def lib():
                                                    c_vp, c_vp, P(Options), P(BatchStatus), c_vp, c_vp, c_vp, c_vp]
         L.hmfem_reservoir_block_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
                                                    c_vp, P(Options), P(BlockStatus), c_vp, c_vp, c_vp]
+        L.hmfem_autotune_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, P(Options), P(c_dbl)]
         L.hmfem_kernel_bytes_per_cell.argtypes = [c_int, c_int]
         L.hmfem_kernel_bytes_per_cell.restype = c_dbl
         _typed = True
@@ -139,7 +140,7 @@ EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", 
             "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device",
             "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
             "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device",
-            "hmfem_reservoir_block_device", "iter_amg", "boundary")
+            "hmfem_reservoir_block_device", "hmfem_autotune_device", "iter_amg", "boundary")
 
 
 def _p(a):
@@ -345,6 +346,19 @@ class Context:
         if rc < 0:
             raise RuntimeError("hmfem_reservoir_batch_device failed: %d" % rc)
         return st, dict(iterations=its, rel_residual=rr, done_ms=dm, div_max=dv)
+
+    def autotune(self, g, Kx, Ky, Kz, bval, options=None):
+        """Choose cycle options for this problem; returns (Options, best ms)."""
+        o = Options()
+        ctypes.memmove(ctypes.byref(o), ctypes.byref(options if options is not None else default_options()),
+                       ctypes.sizeof(Options))
+        ms = c_dbl()
+        _order_after_torch(Kx, Ky, Kz, *bval)
+        rc = lib().hmfem_autotune_device(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz), self._faces(bval),
+                                         ctypes.byref(o), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError("hmfem_autotune_device failed: %d" % rc)
+        return o, ms.value
 
     def reservoir_block(self, g, Kx, Ky, Kz, bvals, p, v=None, q=None, options=None):
         """Solve len(p) right-hand sides of one grid together. bvals[r]: six

@@ -132,6 +132,9 @@ typedef struct {
                                 inside 32x8x8 tiles, one pass per smoothing step; the default); the
                                 levels of the single-workgroup kernel always use Jacobi */
     int gs_sweeps;           /* sweeps per smoothing step of HMFEM_SMOOTH_BLOCK_GS (default 3) */
+    int autotune;            /* 1: hmfem_reservoir_device first runs hmfem_autotune_device for each new
+                                grid shape / precision / tolerance of the context and then uses the
+                                options it chose (default 0) */
 } hmfem_options;
 
 typedef struct {
@@ -178,6 +181,15 @@ int hmfem_flux_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx,
 int hmfem_solve_structured_device(hmfem_context *ctx, int nx, int ny, int nz, const ELE *perm,
                                   const IND *indcol, const double *b, double *x,
                                   const hmfem_options *opt, hmfem_status *st);
+
+/* Autotuner: solves the given problem (no sources, zero initial guess) with
+   a fixed list of cycle variants (smoother and sweeps, coarse-correction
+   scaling, tail cycle), twice each, and writes the fastest converged one
+   (setup + solve time) into *opt; best_ms (may be NULL) receives its time.
+   The tolerance, precision and stopping rule of *opt are kept. No
+   counterpart in the reference (its cycle is fixed, amg.h:1135-1239). */
+int hmfem_autotune_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
+                          const double *Kz, const double *const bval[6], hmfem_options *opt, double *best_ms);
 
 /* Batch of independent problems solved together (every launch covers all
    of them): instance q is an nx*ny*nz grid given by dims[3q..3q+2] (host),

@@ -288,6 +288,27 @@ def test_graph_replay_matches_eager(smoother, ctx, torch):
         assert np.array_equal(p.cpu().numpy().view(np.uint64), ref.view(np.uint64)), rep
 
 
+def test_autotune(ctx, torch):
+    """The autotuner returns converging options for the problem; solving with
+    them (explicitly, and through opt.autotune on the context's cache) gives
+    the oracle's solution."""
+    P = make_problem((40, 36, 28), "karst", seed=13, conduits=5)
+    _say("autotune")
+    D = Dev(P, torch)
+    o, ms = ctx.autotune(D.g, D.K, D.K, D.K, D.bval, S.default_options(rtol=1e-8))
+    assert ms > 0 and o.autotune == 0 and o.rtol == 1e-8
+    assert o.smoother in (S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS) and o.ocf > 1.0
+    xo, _, _, _ = oracle_solution(P, eps=1e-12)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    o.rtol = 1e-12
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, o)
+    assert st.status == S.OK and rel_l2(p.cpu().numpy(), xo) < PRESSURE_TOL
+    for rep in range(2):
+        p.zero_()
+        st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-12, autotune=1))
+        assert st.status == S.OK and rel_l2(p.cpu().numpy(), xo) < PRESSURE_TOL, rep
+
+
 @pytest.mark.parametrize("smoother", [S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS], ids=["jacobi", "blockgs"])
 def test_batch_uneven_matches_single(smoother, ctx, torch):
     """Uneven instances solved together give each instance's own solution:
