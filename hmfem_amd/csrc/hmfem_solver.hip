@@ -32,6 +32,8 @@
 
 #include <string.h>
 
+#include <algorithm>
+
 #include <vector>
 
 namespace hmfem {
@@ -1603,17 +1605,23 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
     if (P) {
         // Launches queued after the solve had converged (the host checks every
         // check_every iterations) return at once; they are left out: a launch
-        // counts if it ran for at least a quarter of its class's longest one.
+        // counts if it ran for at least a quarter of its class's median.
         std::vector<float> dur(P->used / 2, 0.f);
-        float longest[HMFEM_PROF_TYPES] = {};
+        std::vector<float> per[HMFEM_PROF_TYPES];
+        float median[HMFEM_PROF_TYPES] = {};
         for (int q = 0; q + 1 < P->used; q += 2) {
             (void)hipEventElapsedTime(&dur[q / 2], P->ev[q], P->ev[q + 1]);
             int t = P->type[q / 2];
-            if (t >= 0 && t < HMFEM_PROF_TYPES && dur[q / 2] > longest[t]) longest[t] = dur[q / 2];
+            if (t >= 0 && t < HMFEM_PROF_TYPES) per[t].push_back(dur[q / 2]);
+        }
+        for (int t = 0; t < HMFEM_PROF_TYPES; t++) {
+            if (per[t].empty()) continue;
+            std::sort(per[t].begin(), per[t].end());
+            median[t] = per[t][per[t].size() / 2];
         }
         for (int q = 0; q + 1 < P->used; q += 2) {
             int t = P->type[q / 2];
-            if (t < 0 || t >= HMFEM_PROF_TYPES || dur[q / 2] < 0.25f * longest[t]) continue;
+            if (t < 0 || t >= HMFEM_PROF_TYPES || dur[q / 2] < 0.25f * median[t]) continue;
             sink->ms[t] += dur[q / 2];
             sink->count[t] += 1;
         }

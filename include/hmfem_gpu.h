@@ -249,7 +249,7 @@ int hmfem_reservoir_block_device(hmfem_context *ctx, const hmfem_grid *g, const 
    prolongation + all sweeps + CG dots; Jacobi: prolongation + one sweep),
    6 further Jacobi sweeps, 7 coarse levels. Launches that returned at once
    because the solve had already converged (shorter than a quarter of their
-   class's longest launch) are not counted. Returns the number of classes. */
+   class's median launch) are not counted. Returns the number of classes. */
 int hmfem_profile_enable(hmfem_context *ctx, int max_launches);
 int hmfem_profile_read(hmfem_context *ctx, double *ms, long *count, double *bytes, int nclasses);
 void hmfem_profile_reset(hmfem_context *ctx);
