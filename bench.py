@@ -526,6 +526,7 @@ def main():
                    "smoother": "block red-black Gauss-Seidel" if opt.smoother == S.SMOOTH_BLOCK_GS
                    else "damped Jacobi"},
         "iterations": st.iterations, "levels": st.levels,
+        "deflation": {"clusters": st.deflation_clusters, "cells": st.deflation_cells},
         "phase_ms": {"assemble": st.t_assemble_ms, "setup": st.t_setup_ms, "solve": st.t_solve_ms,
                      "flux": st.t_flux_ms},
         "hbm_gbps_solve_phase": solve_gbs,

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
-HIP_SOURCES = ["hmfem_assemble.hip", "hmfem_solver.hip", "hmfem_batch.hip", "hmfem_block.hip", "hmfem_capi.hip"]
+HIP_SOURCES = ["hmfem_assemble.hip", "hmfem_solver.hip", "hmfem_batch.hip", "hmfem_block.hip", "hmfem_defl.hip",
+               "hmfem_capi.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed FP32 arithmetic instructions: some of their forms have given

@@ -29,6 +29,7 @@ struct hmfem_context {
     bool prof_on = false;
     std::vector<char> stage;   // host staging of small uploads (tail op list)
     GraphCache graph;          // captured CG iteration of the last solve
+    Deflation defl;            // cluster correction of the last solve
     long lds_limit = 65536;    // dynamic LDS a workgroup may use on this device
     // options chosen by the autotuner, per grid shape and precision
     std::map<std::vector<int>, hmfem_options> tuned;
@@ -83,6 +84,8 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->smoother = HMFEM_SMOOTH_BLOCK_GS;
     o->gs_sweeps = 3;
     o->autotune = 0;
+    o->deflate = 1;
+    o->deflate_ratio = 1000.0;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)
@@ -94,6 +97,8 @@ extern "C" int hmfem_create(hmfem_context **out)
     HMFEM_CHECK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     for (int q = 0; q < 8; q++) HMFEM_CHECK(hipEventCreate(&c->ev[q]));
     HMFEM_CHECK(hipHostMalloc((void **)&c->h_sc, sizeof(CgScalars), hipHostMallocDefault));
+    HMFEM_CHECK(hipHostMalloc((void **)&c->defl.h_hist, Deflation::kHistBins * sizeof(unsigned), hipHostMallocDefault));
+    HMFEM_CHECK(hipEventCreateWithFlags(&c->defl.hist_ready, hipEventDisableTiming));
     int dev = 0, lds = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
@@ -112,6 +117,8 @@ extern "C" void hmfem_destroy(hmfem_context *c)
     if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
     for (int q = 0; q < 8; q++) (void)hipEventDestroy(c->ev[q]);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
+    if (c->defl.h_hist) (void)hipHostFree(c->defl.h_hist);
+    if (c->defl.hist_ready) (void)hipEventDestroy(c->defl.hist_ready);
     (void)hipStreamDestroy(c->st);
     if (c == g_default) g_default = nullptr;
     delete c;
@@ -167,6 +174,13 @@ static bool grid_ok(const hmfem_grid *g)
     return true;
 }
 
+static char *ctx_alloc_defl(void *c, size_t bytes, int slot)
+{
+    static const char *names[8] = {"defl_hist", "defl_flags", "defl_cells", "defl_comp",
+                                   "defl_ptr",  "defl_einv",  "defl_sums",  "defl_coef"};
+    return (char *)((hmfem_context *)c)->get(names[slot & 7], bytes);
+}
+
 static int solve_on_stencil(hmfem_context *c, Dims d, const Stencil<double> &A, const double *b, double *x,
                             const hmfem_options *opt_in, hmfem_status *st)
 {
@@ -195,7 +209,7 @@ static int solve_on_stencil(hmfem_context *c, Dims d, const Stencil<double> &A, 
     hmfem_status tmp{};
     hmfem_status *out = st ? st : &tmp;
     return solve_structured(c->st, d, A, b, x, opt, arena, hb, B, c->h_sc, info, out, c->ev, sink_of(c), c->stage,
-                            &c->graph, c->lds_limit);
+                            &c->graph, c->lds_limit, &c->defl, ctx_alloc_defl, c);
 }
 
 // ------------------------------------------------------------ device API

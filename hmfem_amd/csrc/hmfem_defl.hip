@@ -1,0 +1,366 @@
+// Conduit deflation: an additive coarse correction on clusters of strongly
+// coupled cells.
+//
+// Karst conduits are thin connected sets of cells whose permeability exceeds
+// the matrix rock's by 1e4..1e6. Along a conduit the pressure is nearly
+// constant, and the box-aggregation V-cycle resolves such long, thin, stiff
+// paths slowly (tiles freeze their halos, boxes mix conduit and rock). The
+// correction adds, per cluster c of "strong" cells (diagonal d_s at least
+// `ratio` times the median diagonal, 7-point connected, >= kMinCluster cells),
+// one piecewise-constant basis vector 1_c:
+//     z = V(r) + Z E^-1 Z^T r,   Z = [1_c], E = Z^T A Z  (dense, FP64),
+// which keeps the preconditioner symmetric positive definite. No counterpart
+// in the reference (its AMG has no such stage, amg.h:1135-1239).
+//
+// Setup: a histogram of log2(d) gives the median; the flags of strong cells
+// go to the host, which finds the clusters (union-find over the strong cells
+// in index order, deterministic), gathers the coefficients of their cells on
+// the device, builds E in a fixed order and inverts it (Cholesky). Per CG
+// iteration: per-cluster sums of r and v (one block per cluster, fixed order),
+// c = E^-1 (Z^T r) with the dot products c.(Z^T r), c.(Z^T v) written as one
+// more partial of the fine-level dots, and z += Z c after the V-cycle.
+#include "hmfem_stencil.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace hmfem {
+
+constexpr int kHistBins = Deflation::kHistBins;
+constexpr int kMinCluster = 8;
+constexpr int kMaxClusters = 256;
+
+__global__ void __launch_bounds__(kBlock)
+k_diag_hist(OpArgs<double> o, unsigned *hist)
+{
+    __shared__ unsigned h[kHistBins];
+    for (int q = threadIdx.x; q < kHistBins; q += kBlock) h[q] = 0;
+    __syncthreads();
+    const int n = o.nx * o.ny * o.nz;
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
+        Cell c = cell_at(o, s);
+        const double d = diag_at<double, double>(o, c);
+        int b = d > 0 ? ilogb(d) + 1100 : 0;
+        b = b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
+        atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kHistBins; q += kBlock)
+        if (h[q]) atomicAdd(&hist[q], h[q]);
+}
+
+// Indices of the cells whose diagonal exceeds thr, appended in any order to
+// list[] behind a counter (list[-1]); one global atomic per wave. cap bounds
+// the writes (the histogram gives an exact upper bound on the count).
+__global__ void __launch_bounds__(kBlock)
+k_strong_list(OpArgs<double> o, double thr, int *count, int *list, int cap)
+{
+    const int n = o.nx * o.ny * o.nz;
+    const int s = xcd_block() * kBlock + threadIdx.x;
+    bool f = false;
+    if (s < n) {
+        Cell c = cell_at(o, s);
+        f = diag_at<double, double>(o, c) > thr;
+    }
+    const unsigned long long m = __ballot(f);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, 0);
+    const int at = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (f && at < cap) list[at] = s;
+}
+
+// Coefficients of listed cells: e and the couplings to -x,+x,-y,+y,-z,+z (0 where none).
+__global__ void __launch_bounds__(kBlock)
+k_defl_gather(OpArgs<double> o, const int *__restrict__ cells, int m, double *out)
+{
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= m) return;
+    const int s = cells[q];
+    Cell c = cell_at(o, s);
+    const int nx = o.nx, nxy = o.nx * o.ny;
+    double *w = out + 7L * q;
+    w[0] = o.e[s];
+    w[1] = c.i > 0 ? o.wx[s - 1] : 0.0;
+    w[2] = c.i < o.nx - 1 ? o.wx[s] : 0.0;
+    w[3] = c.j > 0 ? o.wy[s - nx] : 0.0;
+    w[4] = c.j < o.ny - 1 ? o.wy[s] : 0.0;
+    w[5] = c.k > 0 ? o.wz[s - nxy] : 0.0;
+    w[6] = c.k < o.nz - 1 ? o.wz[s] : 0.0;
+}
+
+// Per cluster (one block each): sums of r and v over its cells, fixed order.
+__global__ void __launch_bounds__(kBlock)
+k_defl_sums(const int *__restrict__ ptr, const int *__restrict__ cells, const double *__restrict__ r,
+            const double *__restrict__ v, double *sr, double *sv, const int *done)
+{
+    if (*done) return;
+    const int cl = blockIdx.x;
+    double acc[2] = {0.0, 0.0};
+    for (int q = ptr[cl] + threadIdx.x; q < ptr[cl + 1]; q += kBlock) {
+        const int s = cells[q];
+        acc[0] += r[s];
+        if (v) acc[1] += v[s];
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) { sr[cl] = acc[0]; sv[cl] = acc[1]; }
+}
+
+// c = E^-1 sr; the extra dot partials <Zc, r> = c.sr and <Zc, v> = c.sv.
+__global__ void __launch_bounds__(kBlock)
+k_defl_apply(int nc, const double *__restrict__ einv, const double *__restrict__ sr, const double *__restrict__ sv,
+             double *c, double *Pextra, const int *done)
+{
+    if (*done) return;
+    __shared__ double cs[kMaxClusters];
+    for (int a = threadIdx.x; a < nc; a += kBlock) {
+        double t = 0.0;
+        for (int b = 0; b < nc; b++) t += einv[(long)a * nc + b] * sr[b];
+        cs[a] = t;
+        c[a] = t;
+    }
+    __syncthreads();
+    double acc[2] = {0.0, 0.0};
+    for (int a = threadIdx.x; a < nc; a += kBlock) {
+        acc[0] += cs[a] * sr[a];
+        acc[1] += cs[a] * sv[a];
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) { Pextra[0] = acc[0]; Pextra[1] = acc[1]; }
+}
+
+// z += c[cluster] on the cells of every cluster.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_defl_scatter(const int *__restrict__ cells, const int *__restrict__ comp, int m, const double *__restrict__ c,
+               T *z, const int *done)
+{
+    if (*done) return;
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= m) return;
+    const int s = cells[q];
+    z[s] = (T)((double)z[s] + c[comp[q]]);
+}
+
+// ---------------------------------------------------------------- host side
+
+static int find_root(std::vector<int> &p, int a)
+{
+    while (p[a] != a) { p[a] = p[p[a]]; a = p[a]; }
+    return a;
+}
+
+int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation &D,
+                    char *(*alloc)(void *, size_t, int), void *actx)
+{
+    D.nc = 0;
+    D.m = 0;
+    D.pending = false;
+    const long n = d.n();
+    if (n < 64 || !D.h_hist || !D.hist_ready) return HMFEM_OK;
+    OpArgs<double> o;
+    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
+    o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
+    unsigned *hist = (unsigned *)alloc(actx, kHistBins * sizeof(unsigned), 0);
+    if (!hist) return HMFEM_ERR_NOMEM;
+    HMFEM_CHECK(hipMemsetAsync(hist, 0, kHistBins * sizeof(unsigned), st));
+    // a grid-stride loop over at most 512 blocks keeps the global bin updates few
+    const int G = std::min(grid_for(n, kBlock), 512);
+    hipLaunchKernelGGL(k_diag_hist, dim3(G), dim3(kBlock), 0, st, o, hist);
+    HMFEM_CHECK(hipMemcpyAsync(D.h_hist, hist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HMFEM_CHECK(hipEventRecord(D.hist_ready, st));
+    D.pending = true;
+    return HMFEM_OK;
+}
+
+int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ratio, Deflation &D,
+                     char *(*alloc)(void *, size_t, int), void *actx)
+{
+    if (!D.pending) return HMFEM_OK;
+    D.pending = false;
+    const long n = d.n();
+    if (!(ratio > 1.0)) return HMFEM_OK;
+    OpArgs<double> o;
+    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
+    o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
+    const int G = grid_for(n, kBlock);
+    HMFEM_CHECK(hipEventSynchronize(D.hist_ready));
+    const unsigned *h = D.h_hist;
+    // median bin, threshold, and the count of cells certainly above it
+    long acc = 0;
+    int mb = 0;
+    for (; mb < kHistBins; mb++) {
+        acc += h[mb];
+        if (2 * acc >= n) break;
+    }
+    const double thr = ratio * ldexp(1.0, mb - 1100) * 1.4142135623730951;   // ~ ratio x median
+    const int tb = ilogb(thr) + 1100;
+    long above = 0;
+    for (int b = tb + 1; b < kHistBins; b++) above += h[b];
+    if (above < kMinCluster) return HMFEM_OK;
+
+    // the strong cells, compacted on the device (at most ub of them)
+    long ub = 0;
+    for (int b = tb; b < kHistBins; b++) ub += h[b];
+    if (ub > (1L << 22)) return HMFEM_OK;   // not conduits: a large part of the domain
+    int *slist = (int *)alloc(actx, sizeof(int) * (ub + 1), 1);
+    if (!slist) return HMFEM_ERR_NOMEM;
+    HMFEM_CHECK(hipMemsetAsync(slist, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_strong_list, dim3(G), dim3(kBlock), 0, st, o, thr, slist, slist + 1, (int)ub);
+    std::vector<int> cell(ub + 1);
+    HMFEM_CHECK(hipMemcpyAsync(cell.data(), slist, sizeof(int) * (ub + 1), hipMemcpyDeviceToHost, st));
+    HMFEM_CHECK(hipStreamSynchronize(st));
+    const int ms = std::min(cell[0], (int)ub);
+    cell.erase(cell.begin());
+    cell.resize(ms);
+    if (ms < kMinCluster) return HMFEM_OK;
+    std::sort(cell.begin(), cell.end());   // index order: a deterministic labelling
+
+    // union-find over +x, +y, +z neighbours
+    std::vector<int> par(ms);
+    std::iota(par.begin(), par.end(), 0);
+    auto ord = [&](long s) -> int {   // ordinal of strong cell s, or -1
+        if (s < 0 || s >= n) return -1;
+        auto it = std::lower_bound(cell.begin(), cell.end(), (int)s);
+        return it != cell.end() && *it == s ? (int)(it - cell.begin()) : -1;
+    };
+    const long nx = d.nx, nxy = (long)d.nx * d.ny;
+    for (int a = 0; a < ms; a++) {
+        const long s = cell[a];
+        const long i = s % nx, j = (s / nx) % d.ny, k = s / nxy;
+        const long nb[3] = {i + 1 < d.nx ? s + 1 : -1, j + 1 < d.ny ? s + nx : -1, k + 1 < d.nz ? s + nxy : -1};
+        for (long t : nb) {
+            const int b = t < 0 ? -1 : ord(t);
+            if (b < 0) continue;
+            int ra = find_root(par, a), rb = find_root(par, b);
+            if (ra != rb) par[std::max(ra, rb)] = std::min(ra, rb);
+        }
+    }
+    std::vector<int> root(ms), size(ms, 0);
+    for (int a = 0; a < ms; a++) { root[a] = find_root(par, a); size[root[a]]++; }
+    std::vector<int> roots;
+    for (int a = 0; a < ms; a++)
+        if (root[a] == a && size[a] >= kMinCluster) roots.push_back(a);
+    std::stable_sort(roots.begin(), roots.end(), [&](int x, int y) { return size[x] > size[y]; });
+    if ((int)roots.size() > kMaxClusters) roots.resize(kMaxClusters);
+    std::sort(roots.begin(), roots.end());   // cluster order: by smallest member
+    const int nc = (int)roots.size();
+    if (nc == 0) return HMFEM_OK;
+    std::vector<int> cid(ms, -1);
+    for (int q = 0; q < nc; q++) cid[roots[q]] = q;
+    for (int a = 0; a < ms; a++) cid[a] = cid[root[a]];
+    std::vector<int> ptr(nc + 1, 0), lcell, lcomp;
+    for (int a = 0; a < ms; a++)
+        if (cid[a] >= 0) ptr[cid[a] + 1]++;
+    for (int q = 0; q < nc; q++) ptr[q + 1] += ptr[q];
+    const int m = ptr[nc];
+    lcell.resize(m);
+    lcomp.resize(m);
+    {
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (int a = 0; a < ms; a++)
+            if (cid[a] >= 0) { lcell[fill[cid[a]]] = cell[a]; lcomp[fill[cid[a]]++] = cid[a]; }
+    }
+
+    // device copies, coefficients of the listed cells
+    D.cells = (int *)alloc(actx, sizeof(int) * m, 2);
+    D.comp = (int *)alloc(actx, sizeof(int) * m, 3);
+    D.ptr = (int *)alloc(actx, sizeof(int) * (nc + 1), 4);
+    D.einv = (double *)alloc(actx, sizeof(double) * nc * nc, 5);
+    D.sr = (double *)alloc(actx, sizeof(double) * 3 * kMaxClusters, 6);
+    double *coef = (double *)alloc(actx, sizeof(double) * 7 * m, 7);
+    if (!D.cells || !D.comp || !D.ptr || !D.einv || !D.sr || !coef) return HMFEM_ERR_NOMEM;
+    D.sv = D.sr + kMaxClusters;
+    D.c = D.sr + 2 * kMaxClusters;
+    HMFEM_CHECK(hipMemcpyAsync(D.cells, lcell.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipMemcpyAsync(D.comp, lcomp.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipMemcpyAsync(D.ptr, ptr.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_defl_gather, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, o, D.cells, m, coef);
+    std::vector<double> w(7L * m);
+    HMFEM_CHECK(hipMemcpyAsync(w.data(), coef, sizeof(double) * 7 * m, hipMemcpyDeviceToHost, st));
+    HMFEM_CHECK(hipStreamSynchronize(st));
+
+    // E = Z^T A Z in a fixed order: E_aa = sum (e + couplings leaving the
+    // cluster), E_ab = - couplings between clusters a and b
+    std::vector<double> E((size_t)nc * nc, 0.0);
+    auto comp_of = [&](long t) -> int {
+        const int b = ord(t);
+        return b < 0 ? -1 : cid[b];
+    };
+    for (int q = 0; q < m; q++) {
+        const long s = lcell[q];
+        const int a = lcomp[q];
+        const double *ws = &w[7L * q];
+        const long nbs[6] = {s - 1, s + 1, s - nx, s + nx, s - nxy, s + nxy};
+        double daa = ws[0];
+        for (int f = 0; f < 6; f++) {
+            const double wf = ws[1 + f];
+            if (wf == 0.0) continue;
+            const int b = comp_of(nbs[f]);
+            if (b == a) continue;
+            daa += wf;
+            if (b >= 0) E[(size_t)a * nc + b] -= wf;
+        }
+        E[(size_t)a * nc + a] += daa;
+    }
+    // inverse via Cholesky E = L L^T (E is symmetric positive definite)
+    std::vector<double> L = E;
+    for (int j = 0; j < nc; j++) {
+        double djj = L[(size_t)j * nc + j];
+        for (int k = 0; k < j; k++) djj -= L[(size_t)j * nc + k] * L[(size_t)j * nc + k];
+        if (!(djj > 0)) return HMFEM_OK;   // not positive definite: leave the correction off
+        djj = sqrt(djj);
+        L[(size_t)j * nc + j] = djj;
+        for (int i = j + 1; i < nc; i++) {
+            double t = L[(size_t)i * nc + j];
+            for (int k = 0; k < j; k++) t -= L[(size_t)i * nc + k] * L[(size_t)j * nc + k];
+            L[(size_t)i * nc + j] = t / djj;
+        }
+    }
+    std::vector<double> inv((size_t)nc * nc, 0.0);
+    for (int col = 0; col < nc; col++) {   // solve L L^T x = e_col
+        std::vector<double> y(nc, 0.0);
+        for (int i = 0; i < nc; i++) {
+            double t = i == col ? 1.0 : 0.0;
+            for (int k = 0; k < i; k++) t -= L[(size_t)i * nc + k] * y[k];
+            y[i] = t / L[(size_t)i * nc + i];
+        }
+        for (int i = nc - 1; i >= 0; i--) {
+            double t = y[i];
+            for (int k = i + 1; k < nc; k++) t -= L[(size_t)k * nc + i] * inv[(size_t)k * nc + col];
+            inv[(size_t)i * nc + col] = t / L[(size_t)i * nc + i];
+        }
+    }
+    HMFEM_CHECK(hipMemcpyAsync(D.einv, inv.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipStreamSynchronize(st));
+    D.nc = nc;
+    D.m = m;
+    return HMFEM_OK;
+}
+
+void deflation_sums(hipStream_t st, const Deflation &D, const double *r, const double *v, double *Pextra,
+                    const int *done)
+{
+    hipLaunchKernelGGL(k_defl_sums, dim3(D.nc), dim3(kBlock), 0, st, D.ptr, D.cells, r, v, D.sr, D.sv, done);
+    hipLaunchKernelGGL(k_defl_apply, dim3(1), dim3(kBlock), 0, st, D.nc, D.einv, D.sr, D.sv, D.c, Pextra, done);
+}
+
+void deflation_scatter(hipStream_t st, const Deflation &D, float *z, const int *done)
+{
+    hipLaunchKernelGGL(k_defl_scatter<float>, dim3((D.m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.cells,
+                       D.comp, D.m, D.c, z, done);
+}
+
+void deflation_scatter(hipStream_t st, const Deflation &D, double *z, const int *done)
+{
+    hipLaunchKernelGGL(k_defl_scatter<double>, dim3((D.m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.cells,
+                       D.comp, D.m, D.c, z, done);
+}
+
+} // namespace hmfem

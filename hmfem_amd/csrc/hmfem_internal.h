@@ -145,12 +145,38 @@ struct CgBuffers {
     CgScalars *sc;
     double *partial;
 };
+// Additive correction on clusters of strongly coupled cells (hmfem_defl.hip).
+// nc == 0: no correction for this operator.
+struct Deflation {
+    int nc = 0, m = 0;                       // clusters, cells listed (grouped by cluster)
+    int *cells = nullptr, *comp = nullptr;   // cell index and cluster of each listed cell
+    int *ptr = nullptr;                      // cluster q owns listed cells [ptr[q], ptr[q+1])
+    double *einv = nullptr;                  // (Z^T A Z)^-1, nc x nc
+    double *sr = nullptr, *sv = nullptr, *c = nullptr;   // per-cluster sums and correction
+    static constexpr int kHistBins = 2048;   // bins of floor(log2 diagonal) + 1100
+    unsigned *h_hist = nullptr;              // pinned host copy of the diagonal histogram (context-owned)
+    hipEvent_t hist_ready = nullptr;         // recorded after that copy (context-owned)
+    bool pending = false;                    // deflation_begin queued a histogram
+};
+// deflation_begin queues the diagonal histogram and its copy to the host;
+// deflation_finish (after other work has been queued) decides and builds the
+// correction. D.nc == 0 afterwards if there is none.
+int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation &D,
+                    char *(*alloc)(void *, size_t, int), void *actx);
+int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ratio, Deflation &D,
+                     char *(*alloc)(void *, size_t, int), void *actx);
+void deflation_sums(hipStream_t st, const Deflation &D, const double *r, const double *v, double *Pextra,
+                    const int *done);
+void deflation_scatter(hipStream_t st, const Deflation &D, float *z, const int *done);
+void deflation_scatter(hipStream_t st, const Deflation &D, double *z, const int *done);
+
 size_t hierarchy_bytes(Dims fine, const hmfem_options &opt);
 size_t cg_partial_count(long n);
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink,
-                     std::vector<char> &stage, GraphCache *graph, long lds_limit);
+                     std::vector<char> &stage, GraphCache *graph, long lds_limit, Deflation *defl = nullptr,
+                     char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr);
 
 // batched solve (hmfem_batch.hip) and batch layout / kernels (hmfem_assemble.hip)
 int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A0, const double *b, double *x,

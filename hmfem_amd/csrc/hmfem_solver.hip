@@ -814,12 +814,12 @@ k_cg_init(OpArgs<double> o, const double *__restrict__ b, const double *__restri
 // Start of a CG iteration k (read from the device): the stopping tests on
 // the partial sums of the previous kernels, beta (Polak-Ribiere form), and the
 // scalars written by one thread. Returns true if the solve stops here.
-__device__ __forceinline__ bool cg_iteration_scalars(int G, CgScalars *sc, const double *Pinit, const double *Prr,
-                                                     const double *Pdot, int &k, double &beta)
+__device__ __forceinline__ bool cg_iteration_scalars(int G, int Gdot, CgScalars *sc, const double *Pinit,
+                                                     const double *Prr, const double *Pdot, int &k, double &beta)
 {
     k = sc->kspmv;
     double dots[2];
-    sum_partials<2>(Pdot, G, dots);
+    sum_partials<2>(Pdot, Gdot, dots);
     double bb, rr;
     if (k == 0) {
         double ir[2];
@@ -860,14 +860,14 @@ __device__ __forceinline__ bool cg_iteration_scalars(int G, CgScalars *sc, const
 template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
 k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
-          double *__restrict__ v, int chunk, int G, CgScalars *sc, const double *Pinit, const double *Prr,
+          double *__restrict__ v, int chunk, int G, int Gdot, CgScalars *sc, const double *Pinit, const double *Prr,
           const double *Pdot, double *Ptau)
 {
     if (sc->done) return;
     const int n = o.nx * o.ny * o.nz;
     int k;
     double beta;
-    const bool stop = cg_iteration_scalars(G, sc, Pinit, Prr, Pdot, k, beta);
+    const bool stop = cg_iteration_scalars(G, Gdot, sc, Pinit, Prr, Pdot, k, beta);
     const double *__restrict__ sin = (k & 1) ? s1 : s0;
     double *__restrict__ sout = (k & 1) ? s0 : s1;
     if (stop) return;
@@ -909,13 +909,13 @@ k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, d
 template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
 k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
-            double *__restrict__ v, int zlen, int G, CgScalars *sc, const double *Pinit, const double *Prr,
+            double *__restrict__ v, int zlen, int G, int Gdot, CgScalars *sc, const double *Pinit, const double *Prr,
             const double *Pdot, double *Ptau)
 {
     if (sc->done) return;
     int k;
     double beta;
-    const bool stop = cg_iteration_scalars(G, sc, Pinit, Prr, Pdot, k, beta);
+    const bool stop = cg_iteration_scalars(G, Gdot, sc, Pinit, Prr, Pdot, k, beta);
     if (stop) return;
     const double *__restrict__ sin = (k & 1) ? s1 : s0;
     double *__restrict__ sout = (k & 1) ? s0 : s1;
@@ -1492,7 +1492,7 @@ struct SolveResult {
 template <typename T>
 static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d, const double *b, double *x,
                   const hmfem_options &opt, CgBuffers &B, CgScalars *h_sc, SolveResult &res, Prof *P,
-                  GraphCache *graph)
+                  GraphCache *graph, const Deflation *D)
 {
     const int n = (int)d.n();
     OpArgs<double> o;
@@ -1515,7 +1515,17 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     init.stop_rule = opt.stop_rule;
     HMFEM_CHECK(hipMemcpyAsync(B.sc, &init, sizeof(CgScalars), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_cg_init, dim3(G), dim3(kBlock), 0, st, o, b, x, B.r, S.ch.chunk, S.Pinit);
-    T *z = vcycle_fine<T>(st, H, B.r, nullptr, cp, P, S);
+    // z = M r: the V-cycle plus, where clusters of strong cells were found, the
+    // additive cluster correction (its dot products are partial number G)
+    const bool defl = D && D->nc > 0;
+    const int Gdot = defl ? G + 1 : G;
+    auto precond = [&](const double *vv, Prof *PP) -> T * {
+        if (defl) deflation_sums(st, *D, B.r, vv, S.Pdot + 2 * G, S.done);
+        T *zz = vcycle_fine<T>(st, H, B.r, vv, cp, PP, S);
+        if (defl) deflation_scatter(st, *D, zz, S.done);
+        return zz;
+    };
+    T *z = precond(nullptr, P);
     HMFEM_CHECK(hipGetLastError());
 
     const int zlen = spmv_zlen(d);
@@ -1525,13 +1535,13 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     auto iteration = [&](Prof *PP) -> T * {
         if (zlen > 0)
             PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv_z<T>, dim3(Gz), dim3(kBlock), 0, st, o, (const T *)z, B.s0,
-                        B.s1, B.v, zlen, G, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+                        B.s1, B.v, zlen, G, Gdot, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
         else
             PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv<T>, dim3(G), dim3(kBlock), 0, st, o, (const T *)z, B.s0,
-                        B.s1, B.v, S.ch.chunk, G, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+                        B.s1, B.v, S.ch.chunk, G, Gdot, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
         PROF_LAUNCH(PP, HMFEM_PROF_CG_UPDATE, k_cg_update, dim3(G), dim3(kBlock), 0, st, n, B.s0, B.s1, B.v, x, B.r,
                     S.ch.chunk, zlen > 0 ? Gz : G, B.sc, S.Ptau, S.Prr);
-        return vcycle_fine<T>(st, H, B.r, B.v, cp, PP, S);
+        return precond(B.v, PP);
     };
     hipGraphExec_t exec = nullptr;
     if (graph && !P) {
@@ -1541,6 +1551,11 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
         put(&d, sizeof(d)); put(&opt, sizeof(opt)); put(&H.lev[0].s32.e, sizeof(void *));
         put(&A, sizeof(A)); put(&x, sizeof(x)); put(&B, sizeof(B)); put(&z, sizeof(z));
         put(&H.tail_ops, sizeof(void *)); put(&H.inv, sizeof(void *));
+        if (defl) {   // the cluster correction's sizes and buffers
+            put(&D->nc, sizeof(int)); put(&D->m, sizeof(int)); put(&D->cells, sizeof(void *));
+            put(&D->comp, sizeof(void *)); put(&D->ptr, sizeof(void *)); put(&D->einv, sizeof(void *));
+            put(&D->sr, sizeof(void *));
+        }
         if (graph->exec && graph->key == key) exec = graph->exec;
         else {
             if (graph->exec) { (void)hipGraphExecDestroy(graph->exec); graph->exec = nullptr; }
@@ -1578,7 +1593,8 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink, std::vector<char> &stage,
-                     GraphCache *graph, long lds_limit)
+                     GraphCache *graph, long lds_limit, Deflation *defl, char *(*dalloc)(void *, size_t, int),
+                     void *dctx)
 {
     Prof prof, *P = nullptr;
     if (sink && sink->ev && sink->cap >= 2) {
@@ -1590,13 +1606,26 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
     Hierarchy H;
     int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
     HMFEM_CHECK(hipEventRecord(ev[0], st));
-    int rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit);
+    int rc = HMFEM_OK;
+    if (defl) {   // the histogram first: its copy to the host overlaps the hierarchy setup
+        defl->nc = 0;
+        defl->pending = false;
+        if (opt.deflate && dalloc) {
+            rc = deflation_begin(st, d, A, *defl, dalloc, dctx);
+            if (rc) return rc;
+        }
+    }
+    rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit);
     if (rc) return rc;
+    if (defl && defl->pending) {
+        rc = deflation_finish(st, d, A, opt.deflate_ratio, *defl, dalloc, dctx);
+        if (rc) return rc;
+    }
     HMFEM_CHECK(hipEventRecord(ev[1], st));
     SolveResult res{};
     GraphCache *gc = opt.use_graph ? graph : nullptr;
-    if (prec == HMFEM_PREC_FP64) rc = run_cg<double>(st, H, A, d, b, x, opt, B, h_sc, res, P, gc);
-    else rc = run_cg<float>(st, H, A, d, b, x, opt, B, h_sc, res, P, gc);
+    if (prec == HMFEM_PREC_FP64) rc = run_cg<double>(st, H, A, d, b, x, opt, B, h_sc, res, P, gc, defl);
+    else rc = run_cg<float>(st, H, A, d, b, x, opt, B, h_sc, res, P, gc, defl);
     if (rc) return rc;
     HMFEM_CHECK(hipEventRecord(ev[2], st));
     HMFEM_CHECK(hipEventSynchronize(ev[2]));
@@ -1656,6 +1685,8 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         out->t_solve_ms = t1;
         out->iterations = res.iterations;
         out->levels = H.nlev;
+        out->deflation_clusters = defl ? defl->nc : 0;
+        out->deflation_cells = defl && defl->nc ? defl->m : 0;
         out->status = info ? HMFEM_ERR_BREAKDOWN : res.status;
         out->rel_residual = res.bb > 0 ? sqrt(res.rr / res.bb) : 0.0;
         out->precond_ratio = res.sigma_first > 0 ? sqrt(res.sigma / res.sigma_first) : 0.0;

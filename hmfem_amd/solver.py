@@ -59,14 +59,15 @@ class Options(ctypes.Structure):
     _fields_ = [("rtol", c_dbl), ("max_iterations", c_int), ("precision", c_int), ("stop_rule", c_int),
                 ("nu", c_int), ("omega", c_dbl), ("ocf", c_dbl), ("coarse_max", c_int), ("check_every", c_int),
                 ("tail_max", c_int), ("tail_gamma", c_int), ("use_graph", c_int), ("tail_lds", c_int),
-                ("smoother", c_int), ("gs_sweeps", c_int), ("autotune", c_int)]
+                ("smoother", c_int), ("gs_sweeps", c_int), ("autotune", c_int), ("deflate", c_int),
+                ("deflate_ratio", c_dbl)]
 
 
 class Status(ctypes.Structure):
     _fields_ = [("status", c_int), ("iterations", c_int), ("levels", c_int), ("rel_residual", c_dbl),
                 ("precond_ratio", c_dbl), ("t_assemble_ms", c_dbl), ("t_setup_ms", c_dbl),
                 ("t_solve_ms", c_dbl), ("t_flux_ms", c_dbl), ("div_max", c_dbl), ("div_min", c_dbl),
-                ("solve_bytes", c_dbl)]
+                ("solve_bytes", c_dbl), ("deflation_clusters", c_int), ("deflation_cells", c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -135,6 +135,10 @@ typedef struct {
     int autotune;            /* 1: hmfem_reservoir_device first runs hmfem_autotune_device for each new
                                 grid shape / precision / tolerance of the context and then uses the
                                 options it chose (default 0) */
+    int deflate;             /* 1: add a coarse correction on each 7-point-connected cluster of at least
+                                8 cells whose diagonal exceeds deflate_ratio times the median diagonal
+                                (karst conduits); off when there are none (default 1; single solves) */
+    double deflate_ratio;    /* default 1000 */
 } hmfem_options;
 
 typedef struct {
@@ -146,6 +150,8 @@ typedef struct {
     double t_assemble_ms, t_setup_ms, t_solve_ms, t_flux_ms;
     double div_max, div_min; /* divvel_pontual() of the fluxes, when computed */
     double solve_bytes;      /* algorithmic HBM bytes of the solve phase */
+    int deflation_clusters;  /* clusters of the conduit correction (0: none) */
+    int deflation_cells;     /* cells in those clusters */
 } hmfem_status;
 
 typedef struct hmfem_context hmfem_context;

@@ -288,6 +288,79 @@ def test_graph_replay_matches_eager(smoother, ctx, torch):
         assert np.array_equal(p.cpu().numpy().view(np.uint64), ref.view(np.uint64)), rep
 
 
+def test_deflation_karst(ctx, torch):
+    """The cluster correction on a conduit field: same solution as without it
+    (and as the oracle) at 1e-12, fewer iterations at 1e-8, and a captured-graph
+    replay bit-identical to eager launches."""
+    P = make_problem((64, 64, 48), "karst", seed=21, conduits=8)
+    _say("deflation karst")
+    D = Dev(P, torch)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    its = {}
+    for dfl in (0, 1):
+        p.zero_()
+        st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-8, deflate=dfl))
+        assert st.status == S.OK
+        its[dfl] = st.iterations
+    assert its[1] < its[0], its
+    xo, _, _, _ = oracle_solution(P, eps=1e-12)
+    ref = None
+    for use_graph in (0, 1, 1):
+        p.zero_()
+        st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None,
+                           S.default_options(rtol=1e-12, deflate=1, use_graph=use_graph))
+        assert st.status == S.OK
+        x = p.cpu().numpy()
+        assert rel_l2(x, xo) < PRESSURE_TOL
+        if ref is None:
+            ref = x.copy()
+        else:
+            assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
+
+
+def _expected_clusters(P, ratio=1000.0):
+    """Clusters the conduit correction should find, from the oracle's
+    assembled diagonal: median of floor(log2 d) by histogram, threshold
+    ratio * 2^median_bin * sqrt(2), 6-connected components of the cells
+    above it (scipy.ndimage.label), those of at least 8 cells kept."""
+    from scipy import ndimage
+    perm, _, _ = O.assemble(P)
+    d = np.abs(perm["c"]).reshape(P.nz, P.ny, P.nx)
+    b = np.clip(np.frexp(d)[1] - 1 + 1100, 0, 2047)
+    hist = np.bincount(b.ravel(), minlength=2048)
+    mb = int(np.argmax(2 * np.cumsum(hist) >= P.n))
+    thr = ratio * 2.0 ** (mb - 1100) * np.sqrt(2.0)
+    tb = int(np.frexp(thr)[1] - 1 + 1100)
+    if hist[tb + 1:].sum() < 8:
+        return 0, 0
+    lab, nl = ndimage.label(d > thr)
+    sizes = np.bincount(lab.ravel())[1:]
+    keep = np.sort(sizes[sizes >= 8])[::-1][:256]
+    return int(keep.size), int(keep.sum())
+
+
+@pytest.mark.parametrize("kind,dims,seed", [("karst", (64, 64, 48), 21), ("karst", (48, 40, 36), 3),
+                                            ("lognormal", (48, 48, 40), 5)])
+def test_deflation_clusters(kind, dims, seed, ctx, torch):
+    """The clusters the solver builds its correction on match an independent
+    labelling of the oracle's diagonal (count and cells); with deflate=0 there
+    are none."""
+    kw = {"conduits": 8} if kind == "karst" else {"sigma": 3.0}
+    P = make_problem(dims, kind, seed=seed, **kw)
+    _say("deflation clusters", kind, dims, seed)
+    D = Dev(P, torch)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    nc, m = _expected_clusters(P)
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-8, deflate=1))
+    assert st.status == S.OK
+    assert (st.deflation_clusters, st.deflation_cells) == (nc, m)
+    if kind == "karst":
+        assert nc >= 1
+    p.zero_()
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-8, deflate=0))
+    assert st.status == S.OK and st.deflation_clusters == 0 and st.deflation_cells == 0
+
+
 def test_autotune(ctx, torch):
     """The autotuner returns converging options for the problem; solving with
     them (explicitly, and through opt.autotune on the context's cache) gives
