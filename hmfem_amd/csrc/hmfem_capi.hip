@@ -119,6 +119,7 @@ extern "C" void hmfem_destroy(hmfem_context *c)
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->defl.h_hist) (void)hipHostFree(c->defl.h_hist);
     if (c->defl.hist_ready) (void)hipEventDestroy(c->defl.hist_ready);
+    if (c->defl.h_stage) (void)hipHostFree(c->defl.h_stage);
     (void)hipStreamDestroy(c->st);
     if (c == g_default) g_default = nullptr;
     delete c;

@@ -53,27 +53,71 @@ k_diag_hist(OpArgs<double> o, unsigned *hist)
         if (h[q]) atomicAdd(&hist[q], h[q]);
 }
 
-// Indices of the cells whose diagonal exceeds thr, appended in any order to
-// list[] behind a counter (list[-1]); one global atomic per wave. cap bounds
-// the writes (the histogram gives an exact upper bound on the count).
+// Strong cells (diagonal above thr) in index order, in three steps: a bit
+// mask per wave and a count per block (logical block lb covers cells
+// [lb*kBlock, lb*kBlock + kBlock)), an exclusive scan of the counts, and the
+// ordered list.
+constexpr int kWavesPerBlock = kBlock / 64;
+
 __global__ void __launch_bounds__(kBlock)
-k_strong_list(OpArgs<double> o, double thr, int *count, int *list, int cap)
+k_strong_mask(OpArgs<double> o, double thr, unsigned long long *mask, int *bcnt)
 {
+    __shared__ int wc[kWavesPerBlock];
     const int n = o.nx * o.ny * o.nz;
-    const int s = xcd_block() * kBlock + threadIdx.x;
+    const int lb = xcd_block();
+    const int s = lb * kBlock + threadIdx.x;
     bool f = false;
     if (s < n) {
         Cell c = cell_at(o, s);
         f = diag_at<double, double>(o, c) > thr;
     }
     const unsigned long long m = __ballot(f);
-    if (!m) return;
-    const int lane = threadIdx.x & 63;
-    int base = 0;
-    if (lane == 0) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, 0);
-    const int at = base + __popcll(m & ((1ull << lane) - 1ull));
-    if (f && at < cap) list[at] = s;
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        mask[(long)lb * kWavesPerBlock + w] = m;
+        wc[w] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int q = 0; q < kWavesPerBlock; q++) t += wc[q];
+        bcnt[lb] = t;
+    }
+}
+
+// In place exclusive scan of cnt[0..G), total in cnt[G]; one block.
+constexpr int kScanThreads = 1024;
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_counts(int *cnt, int G)
+{
+    __shared__ int part[kScanThreads];
+    const int per = (G + kScanThreads - 1) / kScanThreads;
+    const int beg = threadIdx.x * per, end = min(G, beg + per);
+    int t = 0;
+    for (int q = beg; q < end; q++) t += cnt[q];
+    part[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {   // 1024 partial sums: sequential is cheap enough for a setup step
+        int a = 0;
+        for (int q = 0; q < kScanThreads; q++) { const int v = part[q]; part[q] = a; a += v; }
+        cnt[G] = a;
+    }
+    __syncthreads();
+    int a = part[threadIdx.x];
+    for (int q = beg; q < end; q++) { const int v = cnt[q]; cnt[q] = a; a += v; }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_strong_emit(const unsigned long long *__restrict__ mask, const int *__restrict__ boff, int *list, int cap)
+{
+    const int lb = xcd_block();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned long long m = mask[(long)lb * kWavesPerBlock + w];
+    if (!((m >> lane) & 1ull)) return;
+    int at = boff[lb];
+    for (int q = 0; q < w; q++) at += __popcll(mask[(long)lb * kWavesPerBlock + q]);
+    at += __popcll(m & ((1ull << lane) - 1ull));
+    if (at < cap) list[at] = lb * kBlock + threadIdx.x;
 }
 
 // Coefficients of listed cells: e and the couplings to -x,+x,-y,+y,-z,+z (0 where none).
@@ -150,6 +194,18 @@ k_defl_scatter(const int *__restrict__ cells, const int *__restrict__ comp, int 
 
 // ---------------------------------------------------------------- host side
 
+// pinned host staging of at least `bytes` (grown on demand, freed with the context)
+static int stage_reserve(Deflation &D, size_t bytes)
+{
+    if (D.h_stage_bytes >= bytes && D.h_stage) return 0;
+    if (D.h_stage) (void)hipHostFree(D.h_stage);
+    D.h_stage = nullptr;
+    D.h_stage_bytes = 0;
+    if (hipHostMalloc(&D.h_stage, bytes, hipHostMallocDefault) != hipSuccess) return 1;
+    D.h_stage_bytes = bytes;
+    return 0;
+}
+
 static int find_root(std::vector<int> &p, int a)
 {
     while (p[a] != a) { p[a] = p[p[a]]; a = p[a]; }
@@ -209,39 +265,59 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     long ub = 0;
     for (int b = tb; b < kHistBins; b++) ub += h[b];
     if (ub > (1L << 22)) return HMFEM_OK;   // not conduits: a large part of the domain
-    int *slist = (int *)alloc(actx, sizeof(int) * (ub + 1), 1);
-    if (!slist) return HMFEM_ERR_NOMEM;
-    HMFEM_CHECK(hipMemsetAsync(slist, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_strong_list, dim3(G), dim3(kBlock), 0, st, o, thr, slist, slist + 1, (int)ub);
-    std::vector<int> cell(ub + 1);
-    HMFEM_CHECK(hipMemcpyAsync(cell.data(), slist, sizeof(int) * (ub + 1), hipMemcpyDeviceToHost, st));
+    // slot 1: masks (G * kWavesPerBlock words), block offsets (G + 1), list (ub)
+    const size_t mb_bytes = sizeof(unsigned long long) * (size_t)G * kWavesPerBlock;
+    char *sbuf = alloc(actx, mb_bytes + sizeof(int) * ((size_t)G + 1 + ub), 1);
+    if (!sbuf) return HMFEM_ERR_NOMEM;
+    unsigned long long *mask = (unsigned long long *)sbuf;
+    int *boff = (int *)(sbuf + mb_bytes);
+    int *slist = boff + G + 1;
+    hipLaunchKernelGGL(k_strong_mask, dim3(G), dim3(kBlock), 0, st, o, thr, mask, boff);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, st, boff, G);
+    hipLaunchKernelGGL(k_strong_emit, dim3(G), dim3(kBlock), 0, st, mask, boff, slist, (int)ub);
+    // the total (boff[G]) and the list in one copy to pinned staging
+    const size_t cb = sizeof(int) * (1 + ub);
+    if (stage_reserve(D, cb + sizeof(double) * 7 * (size_t)ub)) return HMFEM_ERR_NOMEM;
+    int *hl = (int *)D.h_stage;
+    HMFEM_CHECK(hipMemcpyAsync(hl, boff + G, cb, hipMemcpyDeviceToHost, st));
     HMFEM_CHECK(hipStreamSynchronize(st));
-    const int ms = std::min(cell[0], (int)ub);
-    cell.erase(cell.begin());
-    cell.resize(ms);
+    const int ms = std::min(hl[0], (int)ub);
     if (ms < kMinCluster) return HMFEM_OK;
-    std::sort(cell.begin(), cell.end());   // index order: a deterministic labelling
+    std::vector<int> cell(hl + 1, hl + 1 + ms);   // in index order: a deterministic labelling
 
-    // union-find over +x, +y, +z neighbours
+    // strong neighbours of each strong cell: ordinals of the +x, +y, +z
+    // neighbours by one monotone sweep per direction (cell[] is sorted), the
+    // -x, -y, -z ones by symmetry
+    const long nx = d.nx, nxy = (long)d.nx * d.ny;
+    std::vector<int> nbr(6L * ms, -1);   // [a][-x,+x,-y,+y,-z,+z]
+    {
+        const long off[3] = {1, nx, nxy};
+        for (int f = 0; f < 3; f++) {
+            int p = 0;
+            for (int a = 0; a < ms; a++) {
+                const long s = cell[a];
+                const bool inside = f == 0 ? s % nx + 1 < d.nx : f == 1 ? (s / nx) % d.ny + 1 < d.ny
+                                                                        : s / nxy + 1 < d.nz;
+                if (!inside) continue;
+                const long t = s + off[f];
+                while (p < ms && cell[p] < t) p++;
+                if (p < ms && cell[p] == t) {
+                    nbr[6L * a + 2 * f + 1] = p;
+                    nbr[6L * p + 2 * f] = a;
+                }
+            }
+        }
+    }
+    // union-find over the +x, +y, +z links
     std::vector<int> par(ms);
     std::iota(par.begin(), par.end(), 0);
-    auto ord = [&](long s) -> int {   // ordinal of strong cell s, or -1
-        if (s < 0 || s >= n) return -1;
-        auto it = std::lower_bound(cell.begin(), cell.end(), (int)s);
-        return it != cell.end() && *it == s ? (int)(it - cell.begin()) : -1;
-    };
-    const long nx = d.nx, nxy = (long)d.nx * d.ny;
-    for (int a = 0; a < ms; a++) {
-        const long s = cell[a];
-        const long i = s % nx, j = (s / nx) % d.ny, k = s / nxy;
-        const long nb[3] = {i + 1 < d.nx ? s + 1 : -1, j + 1 < d.ny ? s + nx : -1, k + 1 < d.nz ? s + nxy : -1};
-        for (long t : nb) {
-            const int b = t < 0 ? -1 : ord(t);
+    for (int a = 0; a < ms; a++)
+        for (int f = 1; f < 6; f += 2) {
+            const int b = nbr[6L * a + f];
             if (b < 0) continue;
             int ra = find_root(par, a), rb = find_root(par, b);
             if (ra != rb) par[std::max(ra, rb)] = std::min(ra, rb);
         }
-    }
     std::vector<int> root(ms), size(ms, 0);
     for (int a = 0; a < ms; a++) { root[a] = find_root(par, a); size[root[a]]++; }
     std::vector<int> roots;
@@ -255,17 +331,23 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     std::vector<int> cid(ms, -1);
     for (int q = 0; q < nc; q++) cid[roots[q]] = q;
     for (int a = 0; a < ms; a++) cid[a] = cid[root[a]];
-    std::vector<int> ptr(nc + 1, 0), lcell, lcomp;
+    std::vector<int> ptr(nc + 1, 0), lcell, lcomp, lord;
     for (int a = 0; a < ms; a++)
         if (cid[a] >= 0) ptr[cid[a] + 1]++;
     for (int q = 0; q < nc; q++) ptr[q + 1] += ptr[q];
     const int m = ptr[nc];
     lcell.resize(m);
     lcomp.resize(m);
+    lord.resize(m);
     {
         std::vector<int> fill(ptr.begin(), ptr.end() - 1);
         for (int a = 0; a < ms; a++)
-            if (cid[a] >= 0) { lcell[fill[cid[a]]] = cell[a]; lcomp[fill[cid[a]]++] = cid[a]; }
+            if (cid[a] >= 0) {
+                const int q = fill[cid[a]]++;
+                lcell[q] = cell[a];
+                lcomp[q] = cid[a];
+                lord[q] = a;
+            }
     }
 
     // device copies, coefficients of the listed cells
@@ -282,27 +364,22 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     HMFEM_CHECK(hipMemcpyAsync(D.comp, lcomp.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipMemcpyAsync(D.ptr, ptr.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_defl_gather, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, o, D.cells, m, coef);
-    std::vector<double> w(7L * m);
-    HMFEM_CHECK(hipMemcpyAsync(w.data(), coef, sizeof(double) * 7 * m, hipMemcpyDeviceToHost, st));
+    const double *w = (const double *)D.h_stage;   // 7 m <= 7 ub doubles: reserved above
+    HMFEM_CHECK(hipMemcpyAsync(D.h_stage, coef, sizeof(double) * 7 * m, hipMemcpyDeviceToHost, st));
     HMFEM_CHECK(hipStreamSynchronize(st));
 
     // E = Z^T A Z in a fixed order: E_aa = sum (e + couplings leaving the
     // cluster), E_ab = - couplings between clusters a and b
     std::vector<double> E((size_t)nc * nc, 0.0);
-    auto comp_of = [&](long t) -> int {
-        const int b = ord(t);
-        return b < 0 ? -1 : cid[b];
-    };
     for (int q = 0; q < m; q++) {
-        const long s = lcell[q];
         const int a = lcomp[q];
         const double *ws = &w[7L * q];
-        const long nbs[6] = {s - 1, s + 1, s - nx, s + nx, s - nxy, s + nxy};
+        const int *nb = &nbr[6L * lord[q]];
         double daa = ws[0];
         for (int f = 0; f < 6; f++) {
             const double wf = ws[1 + f];
             if (wf == 0.0) continue;
-            const int b = comp_of(nbs[f]);
+            const int b = nb[f] < 0 ? -1 : cid[nb[f]];
             if (b == a) continue;
             daa += wf;
             if (b >= 0) E[(size_t)a * nc + b] -= wf;

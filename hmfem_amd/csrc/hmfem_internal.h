@@ -156,6 +156,8 @@ struct Deflation {
     static constexpr int kHistBins = 2048;   // bins of floor(log2 diagonal) + 1100
     unsigned *h_hist = nullptr;              // pinned host copy of the diagonal histogram (context-owned)
     hipEvent_t hist_ready = nullptr;         // recorded after that copy (context-owned)
+    void *h_stage = nullptr;                 // pinned staging of the cell list and coefficients
+    size_t h_stage_bytes = 0;
     bool pending = false;                    // deflation_begin queued a histogram
 };
 // deflation_begin queues the diagonal histogram and its copy to the host;
