@@ -108,14 +108,16 @@ k_b_init(OpArgs<double> o, const double *__restrict__ B, const double *__restric
 
 // Start of block CG iteration k: per right-hand side the stopping tests on
 // the partial sums of the previous kernels and beta; live[r] = 0 for a
-// finished one. Returns true once every right-hand side has finished.
+// finished one. Returns true once every right-hand side has finished. The dot
+// partials have Gdot rows (G, plus one for the conduit correction).
 template <int NR>
-__device__ __forceinline__ bool b_iteration_scalars(int G, BlockScalars *sc, const double *Pinit, const double *Prr,
-                                                    const double *Pdot, int &k, double (&beta)[NR], int (&live)[NR])
+__device__ __forceinline__ bool b_iteration_scalars(int G, int Gdot, BlockScalars *sc, const double *Pinit,
+                                                    const double *Prr, const double *Pdot, int &k, double (&beta)[NR],
+                                                    int (&live)[NR])
 {
     k = sc->kspmv;
     double dots[2 * NR], rrs[NR], bbs[NR];
-    sum_partials<2 * NR>(Pdot, G, dots);
+    sum_partials<2 * NR>(Pdot, Gdot, dots);
     if (k == 0) {
         double ir[2 * NR];
         sum_partials<2 * NR>(Pinit, G, ir);
@@ -169,14 +171,14 @@ __device__ __forceinline__ bool b_iteration_scalars(int G, BlockScalars *sc, con
 template <int NR>
 __global__ void __launch_bounds__(kBlock)
 k_b_spmv_z(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S0, double *__restrict__ S1,
-           double *__restrict__ V, int zlen, int G, BlockScalars *sc, const double *Pinit, const double *Prr,
-           const double *Pdot, double *Ptau)
+           double *__restrict__ V, int zlen, int G, int Gdot, BlockScalars *sc, const double *Pinit,
+           const double *Prr, const double *Pdot, double *Ptau)
 {
     if (sc->all_done) return;
     int k;
     double beta[NR];
     int live[NR];
-    if (b_iteration_scalars<NR>(G, sc, Pinit, Prr, Pdot, k, beta, live)) return;
+    if (b_iteration_scalars<NR>(G, Gdot, sc, Pinit, Prr, Pdot, k, beta, live)) return;
     const double *__restrict__ sin = (k & 1) ? S1 : S0;
     double *__restrict__ sout = (k & 1) ? S0 : S1;
     __shared__ double sh[3][kGsTy + 2][kGsTx + 2][NR];
@@ -534,6 +536,7 @@ struct BlockState {
     double *Pinit, *Ptau, *Prr, *Pdot;
     BlockLevelVecs lv[kMaxLevels];
     int *done;                        // &sc->all_done
+    const Deflation *D = nullptr;     // conduit correction (used when D->nc > 0)
 };
 
 static int lg2(int f) { return f == 2 ? 1 : 0; }
@@ -590,6 +593,11 @@ static int b_vcycle_fine(hipStream_t st, Hierarchy &H, BlockState &S, const doub
     BlockLevelVecs &X = S.lv[0];
     const int nt = b_tiles(L.d);
     const int sw = opt.gs_sweeps < 1 ? 1 : opt.gs_sweeps;
+    const bool defl = S.D && S.D->nc > 0;
+    if (defl) {   // per-cluster sums of r and v, the correction and its dot partials (row G of Pdot)
+        int rc = deflation_sums_block(st, *S.D, NR, S.R, Vp, S.Pdot + (long)2 * NR * S.ch.G, S.done);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL((k_b_gs_tile<NR, double, 0>), dim3(grid_for(nt, 1)), dim3(kBThreads), 0, st, opargs<float>(L),
                        sw, (const double *)S.R, (const float *)nullptr, X.u, 0, 0, 0, 0, 0, (const float *)nullptr,
                        0.f, (const double *)nullptr, 1, (double *)nullptr, S.done);
@@ -600,6 +608,7 @@ static int b_vcycle_fine(hipStream_t st, Hierarchy &H, BlockState &S, const doub
     hipLaunchKernelGGL((k_b_gs_tile<NR, double, 2>), dim3(S.ch.G), dim3(kBThreads), 0, st, opargs<float>(L), sw,
                        (const double *)S.R, (const float *)X.u, S.Z, lg2(L.fx), lg2(L.fy), lg2(L.fz), C.d.nx,
                        C.d.ny, (const float *)S.lv[1].u, (float)opt.ocf, Vp, tpb, S.Pdot, S.done);
+    if (defl) return deflation_scatter_block(st, *S.D, NR, S.Z, S.done);
     return HMFEM_OK;
 }
 
@@ -631,7 +640,7 @@ size_t block_bytes(Dims fine, int nr, const hmfem_options &opt)
         if (d.n() <= 1 || (d.nx == 1 && d.ny == 1 && d.nz == 1)) break;
         d = Dims{d.nx > 1 ? (d.nx + 1) / 2 : 1, d.ny > 1 ? (d.ny + 1) / 2 : 1, d.nz > 1 ? (d.nz + 1) / 2 : 1};
     }
-    b += align256(sizeof(BlockScalars)) + 4 * align256((size_t)kMaxReduceBlocks * 2 * kMaxRhs * 8);
+    b += align256(sizeof(BlockScalars)) + 4 * align256((size_t)(kMaxReduceBlocks + 1) * 2 * kMaxRhs * 8);
     b += align256(64 * sizeof(int)) + 64 * 256;
     return b;
 }
@@ -645,6 +654,7 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
     o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
     o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
     const int G = S.ch.G;
+    const int Gdot = S.D && S.D->nc > 0 ? G + 1 : G;
     // planes per block of the z-marching SpMV: at most kMaxReduceBlocks blocks
     const int ntxy = ((d.nx + kGsTx - 1) / kGsTx) * ((d.ny + kGsTy - 1) / kGsTy);
     if (ntxy > kMaxReduceBlocks) return HMFEM_ERR_ARG;
@@ -659,7 +669,7 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
     while (true) {
         for (int q = 0; q < every; q++, k++) {
             hipLaunchKernelGGL(k_b_spmv_z<NR>, dim3(Gz), dim3(kBlock), 0, st, o, (const float *)S.Z, S.S0, S.S1,
-                               S.V, zlen, G, S.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+                               S.V, zlen, G, Gdot, S.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
             hipLaunchKernelGGL(k_b_update<NR>, dim3(G), dim3(kBlock), 0, st, n, S.S0, S.S1, S.V, S.X, S.R,
                                S.ch.chunk, Gz, S.sc, S.Ptau, S.Prr);
             rc = b_vcycle_fine<NR>(st, H, S, S.V, opt);
@@ -678,7 +688,7 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
 int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const double *Bk, double *Xk,
                 const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *actx, long lds_limit,
                 std::vector<char> &stage, int *iterations, double *rel_residual, int *status, double *setup_ms,
-                double *solve_ms)
+                double *solve_ms, Deflation *defl, bool defl_setup, char *(*dalloc)(void *, size_t, int), void *dctx)
 {
     if (nr != 2 && nr != 4) return HMFEM_ERR_ARG;
     hmfem_options opt = opt_in;
@@ -699,10 +709,10 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
     S.V = ar.take<double>(n * nr); S.X = Xk;
     S.Z = ar.take<float>(n * nr);
     S.sc = ar.take<BlockScalars>(1);
-    S.Pinit = ar.take<double>((size_t)kMaxReduceBlocks * 2 * kMaxRhs);
-    S.Ptau = ar.take<double>((size_t)kMaxReduceBlocks * 2 * kMaxRhs);
-    S.Prr = ar.take<double>((size_t)kMaxReduceBlocks * 2 * kMaxRhs);
-    S.Pdot = ar.take<double>((size_t)kMaxReduceBlocks * 2 * kMaxRhs);
+    S.Pinit = ar.take<double>((size_t)(kMaxReduceBlocks + 1) * 2 * kMaxRhs);
+    S.Ptau = ar.take<double>((size_t)(kMaxReduceBlocks + 1) * 2 * kMaxRhs);
+    S.Prr = ar.take<double>((size_t)(kMaxReduceBlocks + 1) * 2 * kMaxRhs);
+    S.Pdot = ar.take<double>((size_t)(kMaxReduceBlocks + 1) * 2 * kMaxRhs);
     int *d_info = ar.take<int>(64);
     if (!S.R || !S.S0 || !S.S1 || !S.V || !S.Z || !S.sc || !S.Pdot || !d_info) return HMFEM_ERR_NOMEM;
     S.done = &S.sc->all_done;
@@ -714,8 +724,22 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
     HMFEM_CHECK(hipEventRecord(e0, st));
     Hierarchy H;
     HMFEM_CHECK(hipMemsetAsync(d_info, 0, sizeof(int), st));
-    int rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit);
+    int rc = HMFEM_OK;
+    if (defl && defl_setup) {   // the conduit correction of this operator (kept for later groups)
+        defl->nc = 0;
+        defl->pending = false;
+        if (opt.deflate && dalloc) {
+            rc = deflation_begin(st, d, A, *defl, dalloc, dctx);
+            if (rc) return rc;
+        }
+    }
+    rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit);
     if (rc) return rc;
+    if (defl && defl->pending) {
+        rc = deflation_finish(st, d, A, opt.deflate_ratio, *defl, dalloc, dctx);
+        if (rc) return rc;
+    }
+    S.D = defl && opt.deflate ? defl : nullptr;
     if (H.nlev < 3 || H.tail0 <= 0 || H.tail_lds_inv < 0 || H.tail_nops > kTailLdsOps) return HMFEM_ERR_ARG;
     for (int l = 0; l <= H.tail0; l++) {
         const size_t nl = H.lev[l].d.n();

@@ -451,7 +451,7 @@ extern "C" int hmfem_reservoir_block_device(hmfem_context *c, const hmfem_grid *
         HMFEM_CHECK(hipEventRecord(c->ev[5], c->st));
         double su = 0, so = 0;
         int rc = block_solve(c->st, d, A, nr, Bk, Xk, opt, ctx_alloc_block, c, c->lds_limit, c->stage, it + g0,
-                             rel + g0, stat + g0, &su, &so);
+                             rel + g0, stat + g0, &su, &so, &c->defl, g0 == 0, ctx_alloc_defl, c);
         if (rc) return rc;
         for (int r = 0; r < nr && g0 + r < nrhs; r++) block_get(c->st, n, nr, r, Xk, p[g0 + r]);
         float ta = 0.f;
@@ -465,6 +465,8 @@ extern "C" int hmfem_reservoir_block_device(hmfem_context *c, const hmfem_grid *
     out->t_assemble_ms = t_asm;
     out->t_setup_ms = setup_ms;
     out->t_solve_ms = solve_ms;
+    out->deflation_clusters = c->defl.nc;
+    out->deflation_cells = c->defl.nc ? c->defl.m : 0;
     out->status = HMFEM_OK;
     for (int r = 0; r < nrhs; r++) {
         if (iterations) iterations[r] = it[r];

@@ -139,48 +139,72 @@ k_defl_gather(OpArgs<double> o, const int *__restrict__ cells, int m, double *ou
     w[6] = c.k < o.nz - 1 ? o.wz[s] : 0.0;
 }
 
+// The per-iteration kernels take NR interleaved right-hand sides (vectors
+// [cell][NR]; NR = 1 for a single solve); per-cluster values are [cluster][NR].
+
 // Per cluster (one block each): sums of r and v over its cells, fixed order.
+template <int NR>
 __global__ void __launch_bounds__(kBlock)
 k_defl_sums(const int *__restrict__ ptr, const int *__restrict__ cells, const double *__restrict__ r,
             const double *__restrict__ v, double *sr, double *sv, const int *done)
 {
     if (*done) return;
     const int cl = blockIdx.x;
-    double acc[2] = {0.0, 0.0};
+    double acc[2 * NR];
+#pragma unroll
+    for (int j = 0; j < 2 * NR; j++) acc[j] = 0.0;
     for (int q = ptr[cl] + threadIdx.x; q < ptr[cl + 1]; q += kBlock) {
-        const int s = cells[q];
-        acc[0] += r[s];
-        if (v) acc[1] += v[s];
+        const long s = cells[q];
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            acc[j] += r[s * NR + j];
+            if (v) acc[NR + j] += v[s * NR + j];
+        }
     }
-    block_sum<2>(acc);
-    if (threadIdx.x == 0) { sr[cl] = acc[0]; sv[cl] = acc[1]; }
+    block_sum<2 * NR>(acc);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < NR; j++) { sr[cl * NR + j] = acc[j]; sv[cl * NR + j] = acc[NR + j]; }
 }
 
-// c = E^-1 sr; the extra dot partials <Zc, r> = c.sr and <Zc, v> = c.sv.
+// c = E^-1 sr; the extra dot partials <Zc, r> = c.sr and <Zc, v> = c.sv
+// (Pextra[0..NR) and Pextra[NR..2NR)).
+template <int NR>
 __global__ void __launch_bounds__(kBlock)
 k_defl_apply(int nc, const double *__restrict__ einv, const double *__restrict__ sr, const double *__restrict__ sv,
              double *c, double *Pextra, const int *done)
 {
     if (*done) return;
-    __shared__ double cs[kMaxClusters];
+    __shared__ double cs[kMaxClusters * NR];
     for (int a = threadIdx.x; a < nc; a += kBlock) {
-        double t = 0.0;
-        for (int b = 0; b < nc; b++) t += einv[(long)a * nc + b] * sr[b];
-        cs[a] = t;
-        c[a] = t;
+        double t[NR];
+#pragma unroll
+        for (int j = 0; j < NR; j++) t[j] = 0.0;
+        for (int b = 0; b < nc; b++) {
+            const double e = einv[(long)a * nc + b];
+#pragma unroll
+            for (int j = 0; j < NR; j++) t[j] += e * sr[b * NR + j];
+        }
+#pragma unroll
+        for (int j = 0; j < NR; j++) { cs[a * NR + j] = t[j]; c[a * NR + j] = t[j]; }
     }
     __syncthreads();
-    double acc[2] = {0.0, 0.0};
+    double acc[2 * NR];
+#pragma unroll
+    for (int j = 0; j < 2 * NR; j++) acc[j] = 0.0;
     for (int a = threadIdx.x; a < nc; a += kBlock) {
-        acc[0] += cs[a] * sr[a];
-        acc[1] += cs[a] * sv[a];
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            acc[j] += cs[a * NR + j] * sr[a * NR + j];
+            acc[NR + j] += cs[a * NR + j] * sv[a * NR + j];
+        }
     }
-    block_sum<2>(acc);
-    if (threadIdx.x == 0) { Pextra[0] = acc[0]; Pextra[1] = acc[1]; }
+    block_sum<2 * NR>(acc);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 2 * NR; j++) Pextra[j] = acc[j];
 }
 
 // z += c[cluster] on the cells of every cluster.
-template <typename T>
+template <int NR, typename T>
 __global__ void __launch_bounds__(kBlock)
 k_defl_scatter(const int *__restrict__ cells, const int *__restrict__ comp, int m, const double *__restrict__ c,
                T *z, const int *done)
@@ -188,8 +212,10 @@ k_defl_scatter(const int *__restrict__ cells, const int *__restrict__ comp, int 
     if (*done) return;
     const int q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= m) return;
-    const int s = cells[q];
-    z[s] = (T)((double)z[s] + c[comp[q]]);
+    const long s = cells[q];
+    const int cl = comp[q];
+#pragma unroll
+    for (int j = 0; j < NR; j++) z[s * NR + j] = (T)((double)z[s * NR + j] + c[cl * NR + j]);
 }
 
 // ---------------------------------------------------------------- host side
@@ -355,11 +381,11 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     D.comp = (int *)alloc(actx, sizeof(int) * m, 3);
     D.ptr = (int *)alloc(actx, sizeof(int) * (nc + 1), 4);
     D.einv = (double *)alloc(actx, sizeof(double) * nc * nc, 5);
-    D.sr = (double *)alloc(actx, sizeof(double) * 3 * kMaxClusters, 6);
+    D.sr = (double *)alloc(actx, sizeof(double) * 3 * kMaxClusters * Deflation::kMaxRhs, 6);
     double *coef = (double *)alloc(actx, sizeof(double) * 7 * m, 7);
     if (!D.cells || !D.comp || !D.ptr || !D.einv || !D.sr || !coef) return HMFEM_ERR_NOMEM;
-    D.sv = D.sr + kMaxClusters;
-    D.c = D.sr + 2 * kMaxClusters;
+    D.sv = D.sr + kMaxClusters * Deflation::kMaxRhs;
+    D.c = D.sr + 2 * kMaxClusters * Deflation::kMaxRhs;
     HMFEM_CHECK(hipMemcpyAsync(D.cells, lcell.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipMemcpyAsync(D.comp, lcomp.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipMemcpyAsync(D.ptr, ptr.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice, st));
@@ -421,23 +447,55 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     return HMFEM_OK;
 }
 
+template <int NR>
+static void defl_sums(hipStream_t st, const Deflation &D, const double *r, const double *v, double *Pextra,
+                      const int *done)
+{
+    hipLaunchKernelGGL(k_defl_sums<NR>, dim3(D.nc), dim3(kBlock), 0, st, D.ptr, D.cells, r, v, D.sr, D.sv, done);
+    hipLaunchKernelGGL(k_defl_apply<NR>, dim3(1), dim3(kBlock), 0, st, D.nc, D.einv, D.sr, D.sv, D.c, Pextra,
+                       done);
+}
+
+template <int NR, typename T>
+static void defl_scatter(hipStream_t st, const Deflation &D, T *z, const int *done)
+{
+    hipLaunchKernelGGL((k_defl_scatter<NR, T>), dim3((D.m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.cells,
+                       D.comp, D.m, D.c, z, done);
+}
+
 void deflation_sums(hipStream_t st, const Deflation &D, const double *r, const double *v, double *Pextra,
                     const int *done)
 {
-    hipLaunchKernelGGL(k_defl_sums, dim3(D.nc), dim3(kBlock), 0, st, D.ptr, D.cells, r, v, D.sr, D.sv, done);
-    hipLaunchKernelGGL(k_defl_apply, dim3(1), dim3(kBlock), 0, st, D.nc, D.einv, D.sr, D.sv, D.c, Pextra, done);
+    defl_sums<1>(st, D, r, v, Pextra, done);
 }
 
 void deflation_scatter(hipStream_t st, const Deflation &D, float *z, const int *done)
 {
-    hipLaunchKernelGGL(k_defl_scatter<float>, dim3((D.m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.cells,
-                       D.comp, D.m, D.c, z, done);
+    defl_scatter<1, float>(st, D, z, done);
 }
 
 void deflation_scatter(hipStream_t st, const Deflation &D, double *z, const int *done)
 {
-    hipLaunchKernelGGL(k_defl_scatter<double>, dim3((D.m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.cells,
-                       D.comp, D.m, D.c, z, done);
+    defl_scatter<1, double>(st, D, z, done);
+}
+
+int deflation_sums_block(hipStream_t st, const Deflation &D, int nr, const double *r, const double *v,
+                         double *Pextra, const int *done)
+{
+    switch (nr) {
+    case 2: defl_sums<2>(st, D, r, v, Pextra, done); return HMFEM_OK;
+    case 4: defl_sums<4>(st, D, r, v, Pextra, done); return HMFEM_OK;
+    default: return HMFEM_ERR_ARG;
+    }
+}
+
+int deflation_scatter_block(hipStream_t st, const Deflation &D, int nr, float *z, const int *done)
+{
+    switch (nr) {
+    case 2: defl_scatter<2, float>(st, D, z, done); return HMFEM_OK;
+    case 4: defl_scatter<4, float>(st, D, z, done); return HMFEM_OK;
+    default: return HMFEM_ERR_ARG;
+    }
 }
 
 } // namespace hmfem

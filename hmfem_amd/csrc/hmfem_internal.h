@@ -154,6 +154,7 @@ struct Deflation {
     double *einv = nullptr;                  // (Z^T A Z)^-1, nc x nc
     double *sr = nullptr, *sv = nullptr, *c = nullptr;   // per-cluster sums and correction
     static constexpr int kHistBins = 2048;   // bins of floor(log2 diagonal) + 1100
+    static constexpr int kMaxRhs = 8;        // interleaved right-hand sides of the per-cluster buffers
     unsigned *h_hist = nullptr;              // pinned host copy of the diagonal histogram (context-owned)
     hipEvent_t hist_ready = nullptr;         // recorded after that copy (context-owned)
     void *h_stage = nullptr;                 // pinned staging of the cell list and coefficients
@@ -171,6 +172,11 @@ void deflation_sums(hipStream_t st, const Deflation &D, const double *r, const d
                     const int *done);
 void deflation_scatter(hipStream_t st, const Deflation &D, float *z, const int *done);
 void deflation_scatter(hipStream_t st, const Deflation &D, double *z, const int *done);
+// The same for nr (2 or 4) interleaved right-hand sides ([cell][nr] vectors;
+// Pextra gets c.sr for each, then c.sv for each).
+int deflation_sums_block(hipStream_t st, const Deflation &D, int nr, const double *r, const double *v,
+                         double *Pextra, const int *done);
+int deflation_scatter_block(hipStream_t st, const Deflation &D, int nr, float *z, const int *done);
 
 size_t hierarchy_bytes(Dims fine, const hmfem_options &opt);
 size_t cg_partial_count(long n);
@@ -193,7 +199,8 @@ size_t batch_grid_bytes(int ninst);
 int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const double *Bk, double *Xk,
                 const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *actx, long lds_limit,
                 std::vector<char> &stage, int *iterations, double *rel_residual, int *status, double *setup_ms,
-                double *solve_ms);
+                double *solve_ms, Deflation *defl = nullptr, bool defl_setup = false,
+                char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr);
 size_t block_bytes(Dims fine, int nr, const hmfem_options &opt);
 void block_put(hipStream_t st, long n, int nr, int r, const double *a, const double *add, double *Bk);
 void block_get(hipStream_t st, long n, int nr, int r, const double *Bk, double *a);

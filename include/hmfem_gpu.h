@@ -137,7 +137,8 @@ typedef struct {
                                 options it chose (default 0) */
     int deflate;             /* 1: add a coarse correction on each 7-point-connected cluster of at least
                                 8 cells whose diagonal exceeds deflate_ratio times the median diagonal
-                                (karst conduits); off when there are none (default 1; single solves) */
+                                (karst conduits); off when there are none (default 1; single and block
+                                solves) */
     double deflate_ratio;    /* default 1000 */
 } hmfem_options;
 
@@ -231,7 +232,8 @@ int hmfem_reservoir_batch_device(hmfem_context *ctx, int ninst, const int *dims,
    div_max (host arrays of nrhs, or NULL). Uses the FP32 hierarchy and the
    block Gauss-Seidel smoother whatever opt says; the right-hand sides are
    solved in groups of four (two if there are only one or two), the last
-   group padded with zero right-hand sides. No counterpart
+   group padded with zero right-hand sides. The conduit correction
+   (opt.deflate) is built once per call and shared by every group. No counterpart
    in the reference's API: it is the interleaved multi-RHS solve of
    generic_operator.h:335-391 behind the reservoir() step. */
 typedef struct {
@@ -239,6 +241,8 @@ typedef struct {
     int max_iterations;
     double t_assemble_ms, t_setup_ms, t_solve_ms, t_flux_ms;
     double footprint_bytes;  /* device memory the call used */
+    int deflation_clusters;  /* clusters of the conduit correction (0: none) */
+    int deflation_cells;
 } hmfem_block_status;
 
 int hmfem_reservoir_block_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,

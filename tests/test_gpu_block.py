@@ -4,7 +4,9 @@ through hmfem_reservoir_block_device).
 Bars: the closed-form linear pressure of a homogeneous all-Dirichlet box for
 each face-pair right-hand side; against the oracle's AMG-PCG solution at
 1e-12 within 1e-8 relative L2 (and fluxes bit for bit); against the
-single-right-hand-side GPU solve within 1e-9 at 1e-12.
+single-right-hand-side GPU solve within 1e-8 at 1e-12 (two solutions at a
+true relative residual of 1e-12 differ by up to the condition number times
+that; ~1e-9 is seen on conduit fields).
 """
 import numpy as np
 import pytest
@@ -56,7 +58,7 @@ def _oracle_problem(dims, K, faces):
     return P
 
 
-def _run_block(ctx, torch, dims, K, rhs, rtol, with_flux=False):
+def _run_block(ctx, torch, dims, K, rhs, rtol, with_flux=False, **opts):
     nx, ny, nz = dims
     n = nx * ny * nz
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
@@ -67,7 +69,7 @@ def _run_block(ctx, torch, dims, K, rhs, rtol, with_flux=False):
     vs = [torch.zeros(n * 7, dtype=torch.float64, device="cuda") for _ in rhs] if with_flux else None
     g = S.Context.grid(nx, ny, nz, None, 1.0, ALL_DIRICHLET)
     st, per = ctx.reservoir_block(g, Kd, Kd, Kd, bvals, ps, vs, qs if any(q is not None for q in qs) else None,
-                                  S.default_options(rtol=rtol))
+                                  S.default_options(rtol=rtol, **opts))
     return st, per, [p.cpu().numpy() for p in ps], vs
 
 
@@ -117,7 +119,7 @@ def test_block_vs_oracle(nrhs, ctx, torch):
 
 def test_block_vs_single(ctx, torch):
     """64 x 48 x 40, eight right-hand sides: each equals the single-right-hand-side
-    GPU solve of its own system."""
+    GPU solve of its own system (both at 1e-12)."""
     dims = (64, 48, 40)
     nx, ny, nz = dims
     n = nx * ny * nz
@@ -145,4 +147,42 @@ def test_block_vs_single(ctx, torch):
             torch.cuda.synchronize()
             s1 = ctx.solve_structured(nx, ny, nz, perm, indcol, b, x, o)
         assert s1.status == S.OK
-        assert rel_l2(ps[r], x.cpu().numpy()) < 1e-9, r
+        assert rel_l2(ps[r], x.cpu().numpy()) < PRESSURE_TOL, r
+
+
+def test_block_deflation_karst(ctx, torch):
+    """Conduits in a homogeneous matrix, six face-pair right-hand sides: the
+    block solve builds the same clusters as the single solve, needs fewer
+    iterations with the correction than without, and both reach the same
+    solution at 1e-12."""
+    dims = (64, 64, 48)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    K = F.conduits(F.homogeneous(nx, ny, nz, 1e-14), nx, ny, nz, 8, 21)
+    rhs = _rhs_set(nx, ny, nz, 6)
+    _say("deflation karst", dims)
+    its = {}
+    sols = {}
+    for dfl in (0, 1):
+        st, per, ps, _ = _run_block(ctx, torch, dims, K, rhs, 1e-8, deflate=dfl)
+        assert st.status == S.OK, (st.as_dict(), per)
+        its[dfl] = max(per["iterations"])
+        if dfl:
+            assert st.deflation_clusters >= 1 and st.deflation_cells >= 8
+            clusters = (st.deflation_clusters, st.deflation_cells)
+        else:
+            assert st.deflation_clusters == 0
+        st, per, ps, _ = _run_block(ctx, torch, dims, K, rhs, 1e-12, deflate=dfl)
+        assert st.status == S.OK, (st.as_dict(), per)
+        sols[dfl] = ps
+    assert its[1] < its[0], its
+    for r in range(len(rhs)):
+        assert rel_l2(sols[1][r], sols[0][r]) < PRESSURE_TOL, r
+    # the single solve of the first system finds the same clusters
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    Kd = t(K)
+    g = S.Context.grid(nx, ny, nz, None, 1.0, ALL_DIRICHLET)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    s1 = ctx.reservoir(g, Kd, Kd, Kd, [t(a) for a in rhs[0][0]], x, None, S.default_options(rtol=1e-8))
+    assert s1.status == S.OK
+    assert (s1.deflation_clusters, s1.deflation_cells) == clusters
