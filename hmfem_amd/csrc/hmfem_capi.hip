@@ -85,7 +85,7 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->gs_sweeps = 3;
     o->autotune = 0;
     o->deflate = 1;
-    o->deflate_ratio = 1000.0;
+    o->deflate_ratio = 3000.0;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)

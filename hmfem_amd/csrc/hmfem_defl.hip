@@ -142,18 +142,21 @@ k_defl_gather(OpArgs<double> o, const int *__restrict__ cells, int m, double *ou
 // The per-iteration kernels take NR interleaved right-hand sides (vectors
 // [cell][NR]; NR = 1 for a single solve); per-cluster values are [cluster][NR].
 
-// Per cluster (one block each): sums of r and v over its cells, fixed order.
+// Per piece of a cluster (one block each, at most kPiece cells): sums of r
+// and v over its cells, fixed order.
+constexpr int kPiece = 1024;
+
 template <int NR>
 __global__ void __launch_bounds__(kBlock)
-k_defl_sums(const int *__restrict__ ptr, const int *__restrict__ cells, const double *__restrict__ r,
-            const double *__restrict__ v, double *sr, double *sv, const int *done)
+k_defl_sums(const int *__restrict__ pb, const int *__restrict__ cells, const double *__restrict__ r,
+            const double *__restrict__ v, double *psr, double *psv, const int *done)
 {
     if (*done) return;
-    const int cl = blockIdx.x;
+    const int p = blockIdx.x;
     double acc[2 * NR];
 #pragma unroll
     for (int j = 0; j < 2 * NR; j++) acc[j] = 0.0;
-    for (int q = ptr[cl] + threadIdx.x; q < ptr[cl + 1]; q += kBlock) {
+    for (int q = pb[p] + threadIdx.x; q < pb[p + 1]; q += kBlock) {
         const long s = cells[q];
 #pragma unroll
         for (int j = 0; j < NR; j++) {
@@ -163,39 +166,47 @@ k_defl_sums(const int *__restrict__ ptr, const int *__restrict__ cells, const do
     }
     block_sum<2 * NR>(acc);
     if (threadIdx.x == 0)
-        for (int j = 0; j < NR; j++) { sr[cl * NR + j] = acc[j]; sv[cl * NR + j] = acc[NR + j]; }
+        for (int j = 0; j < NR; j++) { psr[p * NR + j] = acc[j]; psv[p * NR + j] = acc[NR + j]; }
 }
 
-// c = E^-1 sr; the extra dot partials <Zc, r> = c.sr and <Zc, v> = c.sv
-// (Pextra[0..NR) and Pextra[NR..2NR)).
+// Per cluster sr, sv = sums over its pieces (fixed order); c = E^-1 sr; the
+// extra dot partials <Zc, r> = c.sr and <Zc, v> = c.sv (Pextra[0..NR) and
+// Pextra[NR..2NR)). E^-1 is symmetric: thread a reads column a, coalesced.
 template <int NR>
 __global__ void __launch_bounds__(kBlock)
-k_defl_apply(int nc, const double *__restrict__ einv, const double *__restrict__ sr, const double *__restrict__ sv,
-             double *c, double *Pextra, const int *done)
+k_defl_apply(int nc, const int *__restrict__ cp, const double *__restrict__ einv, const double *__restrict__ psr,
+             const double *__restrict__ psv, double *c, double *Pextra, const int *done)
 {
     if (*done) return;
-    __shared__ double cs[kMaxClusters * NR];
+    __shared__ double srs[kMaxClusters * NR], svs[kMaxClusters * NR];
     for (int a = threadIdx.x; a < nc; a += kBlock) {
-        double t[NR];
+        double tr[NR], tv[NR];
 #pragma unroll
-        for (int j = 0; j < NR; j++) t[j] = 0.0;
-        for (int b = 0; b < nc; b++) {
-            const double e = einv[(long)a * nc + b];
+        for (int j = 0; j < NR; j++) { tr[j] = 0.0; tv[j] = 0.0; }
+        for (int p = cp[a]; p < cp[a + 1]; p++)
 #pragma unroll
-            for (int j = 0; j < NR; j++) t[j] += e * sr[b * NR + j];
-        }
+            for (int j = 0; j < NR; j++) { tr[j] += psr[p * NR + j]; tv[j] += psv[p * NR + j]; }
 #pragma unroll
-        for (int j = 0; j < NR; j++) { cs[a * NR + j] = t[j]; c[a * NR + j] = t[j]; }
+        for (int j = 0; j < NR; j++) { srs[a * NR + j] = tr[j]; svs[a * NR + j] = tv[j]; }
     }
     __syncthreads();
     double acc[2 * NR];
 #pragma unroll
     for (int j = 0; j < 2 * NR; j++) acc[j] = 0.0;
     for (int a = threadIdx.x; a < nc; a += kBlock) {
+        double t[NR];
+#pragma unroll
+        for (int j = 0; j < NR; j++) t[j] = 0.0;
+        for (int b = 0; b < nc; b++) {
+            const double e = einv[(long)b * nc + a];
+#pragma unroll
+            for (int j = 0; j < NR; j++) t[j] += e * srs[b * NR + j];
+        }
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            acc[j] += cs[a * NR + j] * sr[a * NR + j];
-            acc[NR + j] += cs[a * NR + j] * sv[a * NR + j];
+            c[a * NR + j] = t[j];
+            acc[j] += t[j] * srs[a * NR + j];
+            acc[NR + j] += t[j] * svs[a * NR + j];
         }
     }
     block_sum<2 * NR>(acc);
@@ -376,19 +387,31 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
             }
     }
 
+    // pieces of at most kPiece cells per cluster, contiguous in the list
+    std::vector<int> pbh(1, 0), cph(1, 0);
+    for (int q = 0; q < nc; q++) {
+        for (int b = ptr[q]; b < ptr[q + 1]; b += kPiece) pbh.push_back(std::min(b + kPiece, ptr[q + 1]));
+        cph.push_back((int)pbh.size() - 1);
+    }
+    const int np = (int)pbh.size() - 1;
+    std::vector<int> pc(pbh);
+    pc.insert(pc.end(), cph.begin(), cph.end());
+
     // device copies, coefficients of the listed cells
     D.cells = (int *)alloc(actx, sizeof(int) * m, 2);
     D.comp = (int *)alloc(actx, sizeof(int) * m, 3);
-    D.ptr = (int *)alloc(actx, sizeof(int) * (nc + 1), 4);
+    D.pb = (int *)alloc(actx, sizeof(int) * pc.size(), 4);
     D.einv = (double *)alloc(actx, sizeof(double) * nc * nc, 5);
-    D.sr = (double *)alloc(actx, sizeof(double) * 3 * kMaxClusters * Deflation::kMaxRhs, 6);
+    const size_t per = (size_t)Deflation::kMaxRhs;
+    D.c = (double *)alloc(actx, sizeof(double) * per * (kMaxClusters + 2 * (size_t)np), 6);
     double *coef = (double *)alloc(actx, sizeof(double) * 7 * m, 7);
-    if (!D.cells || !D.comp || !D.ptr || !D.einv || !D.sr || !coef) return HMFEM_ERR_NOMEM;
-    D.sv = D.sr + kMaxClusters * Deflation::kMaxRhs;
-    D.c = D.sr + 2 * kMaxClusters * Deflation::kMaxRhs;
+    if (!D.cells || !D.comp || !D.pb || !D.einv || !D.c || !coef) return HMFEM_ERR_NOMEM;
+    D.cp = D.pb + np + 1;
+    D.psr = D.c + per * kMaxClusters;
+    D.psv = D.psr + per * np;
     HMFEM_CHECK(hipMemcpyAsync(D.cells, lcell.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipMemcpyAsync(D.comp, lcomp.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
-    HMFEM_CHECK(hipMemcpyAsync(D.ptr, ptr.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipMemcpyAsync(D.pb, pc.data(), sizeof(int) * pc.size(), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_defl_gather, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, o, D.cells, m, coef);
     const double *w = (const double *)D.h_stage;   // 7 m <= 7 ub doubles: reserved above
     HMFEM_CHECK(hipMemcpyAsync(D.h_stage, coef, sizeof(double) * 7 * m, hipMemcpyDeviceToHost, st));
@@ -444,6 +467,7 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     HMFEM_CHECK(hipStreamSynchronize(st));
     D.nc = nc;
     D.m = m;
+    D.np = np;
     return HMFEM_OK;
 }
 
@@ -451,9 +475,9 @@ template <int NR>
 static void defl_sums(hipStream_t st, const Deflation &D, const double *r, const double *v, double *Pextra,
                       const int *done)
 {
-    hipLaunchKernelGGL(k_defl_sums<NR>, dim3(D.nc), dim3(kBlock), 0, st, D.ptr, D.cells, r, v, D.sr, D.sv, done);
-    hipLaunchKernelGGL(k_defl_apply<NR>, dim3(1), dim3(kBlock), 0, st, D.nc, D.einv, D.sr, D.sv, D.c, Pextra,
-                       done);
+    hipLaunchKernelGGL(k_defl_sums<NR>, dim3(D.np), dim3(kBlock), 0, st, D.pb, D.cells, r, v, D.psr, D.psv, done);
+    hipLaunchKernelGGL(k_defl_apply<NR>, dim3(1), dim3(kBlock), 0, st, D.nc, D.cp, D.einv, D.psr, D.psv, D.c,
+                       Pextra, done);
 }
 
 template <int NR, typename T>

@@ -148,11 +148,13 @@ struct CgBuffers {
 // Additive correction on clusters of strongly coupled cells (hmfem_defl.hip).
 // nc == 0: no correction for this operator.
 struct Deflation {
-    int nc = 0, m = 0;                       // clusters, cells listed (grouped by cluster)
+    int nc = 0, m = 0, np = 0;               // clusters, cells listed (grouped by cluster), pieces
     int *cells = nullptr, *comp = nullptr;   // cell index and cluster of each listed cell
-    int *ptr = nullptr;                      // cluster q owns listed cells [ptr[q], ptr[q+1])
-    double *einv = nullptr;                  // (Z^T A Z)^-1, nc x nc
-    double *sr = nullptr, *sv = nullptr, *c = nullptr;   // per-cluster sums and correction
+    int *pb = nullptr;                       // piece p covers listed cells [pb[p], pb[p+1])
+    int *cp = nullptr;                       // cluster q owns pieces [cp[q], cp[q+1])
+    double *einv = nullptr;                  // (Z^T A Z)^-1, nc x nc (symmetric)
+    double *c = nullptr;                     // per-cluster correction [cluster][rhs]
+    double *psr = nullptr, *psv = nullptr;   // per-piece sums of r and v [piece][rhs]
     static constexpr int kHistBins = 2048;   // bins of floor(log2 diagonal) + 1100
     static constexpr int kMaxRhs = 8;        // interleaved right-hand sides of the per-cluster buffers
     unsigned *h_hist = nullptr;              // pinned host copy of the diagonal histogram (context-owned)

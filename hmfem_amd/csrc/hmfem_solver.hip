@@ -1553,8 +1553,9 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
         put(&H.tail_ops, sizeof(void *)); put(&H.inv, sizeof(void *));
         if (defl) {   // the cluster correction's sizes and buffers
             put(&D->nc, sizeof(int)); put(&D->m, sizeof(int)); put(&D->cells, sizeof(void *));
-            put(&D->comp, sizeof(void *)); put(&D->ptr, sizeof(void *)); put(&D->einv, sizeof(void *));
-            put(&D->sr, sizeof(void *));
+            put(&D->comp, sizeof(void *)); put(&D->pb, sizeof(void *)); put(&D->einv, sizeof(void *));
+            put(&D->np, sizeof(int)); put(&D->cp, sizeof(void *)); put(&D->c, sizeof(void *));
+            put(&D->psr, sizeof(void *));
         }
         if (graph->exec && graph->key == key) exec = graph->exec;
         else {

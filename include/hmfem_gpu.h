@@ -139,7 +139,7 @@ typedef struct {
                                 8 cells whose diagonal exceeds deflate_ratio times the median diagonal
                                 (karst conduits); off when there are none (default 1; single and block
                                 solves) */
-    double deflate_ratio;    /* default 1000 */
+    double deflate_ratio;    /* default 3000 */
 } hmfem_options;
 
 typedef struct {

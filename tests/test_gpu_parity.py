@@ -318,7 +318,7 @@ def test_deflation_karst(ctx, torch):
             assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
 
 
-def _expected_clusters(P, ratio=1000.0):
+def _expected_clusters(P, ratio):
     """Clusters the conduit correction should find, from the oracle's
     assembled diagonal: median of floor(log2 d) by histogram, threshold
     ratio * 2^median_bin * sqrt(2), 6-connected components of the cells
@@ -345,12 +345,12 @@ def test_deflation_clusters(kind, dims, seed, ctx, torch):
     """The clusters the solver builds its correction on match an independent
     labelling of the oracle's diagonal (count and cells); with deflate=0 there
     are none."""
-    kw = {"conduits": 8} if kind == "karst" else {"sigma": 3.0}
+    kw = {"conduits": 8} if kind == "karst" else {"sigma": 3.5}
     P = make_problem(dims, kind, seed=seed, **kw)
     _say("deflation clusters", kind, dims, seed)
     D = Dev(P, torch)
     p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
-    nc, m = _expected_clusters(P)
+    nc, m = _expected_clusters(P, S.default_options().deflate_ratio)
     st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-8, deflate=1))
     assert st.status == S.OK
     assert (st.deflation_clusters, st.deflation_cells) == (nc, m)
