@@ -464,6 +464,23 @@ def main():
         pf = p.clone()
         fp64["rel_l2_fp32amg_vs_fp64_1e-12"] = float(torch.linalg.norm(p32 - pf) / torch.linalg.norm(pf))
 
+    # Plain FP64 PCG (diagonal preconditioner, no hierarchy) on the same workload
+    plain = None
+    if not args.no_fp64 and prec == S.PREC_FP32:
+        oj = S.default_options(rtol=rtol, precision=S.PREC_FP64)
+        oj.precond = S.PRECOND_JACOBI
+        oj.max_iterations = 50000
+        sj = step(oj)
+        torch.cuda.synchronize()
+        kj = 2
+        ta = time.perf_counter()
+        for _ in range(kj):
+            sj = step(oj)
+        torch.cuda.synchronize()
+        plain = {"value": kj / (time.perf_counter() - ta), "iterations": sj.iterations,
+                 "converged": sj.status == S.OK, "rel_residual": sj.rel_residual}
+        plain["speedup_fp32amg_over_plain"] = value / plain["value"]
+
     # the autotuner's choice for this problem (tuning time is not in any figure)
     tuned = None
     if not args.no_fp64 and prec == S.PREC_FP32:
@@ -536,6 +553,7 @@ def main():
                      "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": 1e3 * avg_ms},
         "kernels": kernels,
         "fp64_hierarchy": fp64,
+        "plain_fp64_jacobi_pcg": plain,
         "fp16_fine_level": fp16,
         "autotuned": tuned,
         "cpu_baseline": cpu,

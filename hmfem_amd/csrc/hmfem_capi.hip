@@ -86,6 +86,7 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->autotune = 0;
     o->deflate = 1;
     o->deflate_ratio = 3000.0;
+    o->precond = HMFEM_PRECOND_AMG;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)

@@ -235,6 +235,29 @@ k_final_sweep_dots(OpArgs<T> o, int fx, int fy, int cx, int cy, int fz, const T 
     if (threadIdx.x == 0) { Pdot[2 * lb] = acc[0]; Pdot[2 * lb + 1] = acc[1]; }
 }
 
+// Diagonal (Jacobi) preconditioner in FP64, z = r / diag(A), with the dot
+// partials <z, r>, <z, v>: the plain FP64 PCG (no hierarchy).
+__global__ void __launch_bounds__(kBlock)
+k_jacobi_pc(OpArgs<double> o, const double *__restrict__ r, const double *__restrict__ v, double *__restrict__ z,
+            int chunk, double *Pdot, const int *done)
+{
+    if (*done) return;
+    const int n = o.nx * o.ny * o.nz;
+    int lb, beg, end;
+    chunk_range(n, chunk, lb, beg, end);
+    double acc[2] = {0.0, 0.0};
+    for (int s = beg + threadIdx.x; s < end; s += kBlock) {
+        Cell c = cell_at(o, s);
+        const double rs = r[s];
+        const double zs = rs / diag_at<double, double>(o, c);
+        z[s] = zs;
+        acc[0] += zs * rs;
+        if (v) acc[1] += zs * v[s];
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) { Pdot[2 * lb] = acc[0]; Pdot[2 * lb + 1] = acc[1]; }
+}
+
 // ---------------------------------------------------------------- block Gauss-Seidel smoother
 //
 // Each (logical) block smooths tiles [lb * tpb, (lb + 1) * tpb) of the level.
@@ -1492,7 +1515,7 @@ struct SolveResult {
 template <typename T>
 static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d, const double *b, double *x,
                   const hmfem_options &opt, CgBuffers &B, CgScalars *h_sc, SolveResult &res, Prof *P,
-                  GraphCache *graph, const Deflation *D)
+                  GraphCache *graph, const Deflation *D, double *jz = nullptr)
 {
     const int n = (int)d.n();
     OpArgs<double> o;
@@ -1520,6 +1543,11 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     const bool defl = D && D->nc > 0;
     const int Gdot = defl ? G + 1 : G;
     auto precond = [&](const double *vv, Prof *PP) -> T * {
+        if (jz) {   // plain FP64 PCG: T is double
+            PROF_LAUNCH(PP, HMFEM_PROF_JACOBI, k_jacobi_pc, dim3(G), dim3(kBlock), 0, st, o, B.r, vv, jz, S.ch.chunk,
+                        S.Pdot, S.done);
+            return (T *)jz;
+        }
         if (defl) deflation_sums(st, *D, B.r, vv, S.Pdot + 2 * G, S.done);
         T *zz = vcycle_fine<T>(st, H, B.r, vv, cp, PP, S);
         if (defl) deflation_scatter(st, *D, zz, S.done);
@@ -1608,6 +1636,35 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
     int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
     HMFEM_CHECK(hipEventRecord(ev[0], st));
     int rc = HMFEM_OK;
+    if (opt.precond == HMFEM_PRECOND_JACOBI) {   // plain FP64 PCG: no hierarchy, no correction
+        if (defl) { defl->nc = 0; defl->pending = false; }
+        double *jz = ar.take<double>(d.n());
+        if (!jz) return HMFEM_ERR_NOMEM;
+        HMFEM_CHECK(hipEventRecord(ev[1], st));
+        SolveResult res{};
+        rc = run_cg<double>(st, H, A, d, b, x, opt, B, h_sc, res, P, opt.use_graph ? graph : nullptr, nullptr, jz);
+        if (rc) return rc;
+        HMFEM_CHECK(hipEventRecord(ev[2], st));
+        HMFEM_CHECK(hipEventSynchronize(ev[2]));
+        if (out) {
+            float t0 = 0, t1 = 0;
+            (void)hipEventElapsedTime(&t0, ev[0], ev[1]);
+            (void)hipEventElapsedTime(&t1, ev[1], ev[2]);
+            out->t_setup_ms = t0;
+            out->t_solve_ms = t1;
+            out->iterations = res.iterations;
+            out->levels = 0;
+            out->status = res.status;
+            out->rel_residual = res.bb > 0 ? sqrt(res.rr / res.bb) : 0.0;
+            out->precond_ratio = res.sigma_first > 0 ? sqrt(res.sigma / res.sigma_first) : 0.0;
+            out->deflation_clusters = 0;
+            out->deflation_cells = 0;
+            // per iteration: SpMV (A64 32 B, s in, z in, s out, v out), update
+            // (x, s, r, v in; x, r out), Jacobi (A64, r, v in; z out)
+            out->solve_bytes = (double)d.n() * ((32 + 4 * 8) + 6 * 8 + (32 + 3 * 8)) * (res.iterations + 1);
+        }
+        return HMFEM_OK;
+    }
     if (defl) {   // the histogram first: its copy to the host overlaps the hierarchy setup
         defl->nc = 0;
         defl->pending = false;

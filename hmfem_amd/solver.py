@@ -25,6 +25,7 @@ c_int, c_dbl, c_vp, c_long = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, cty
 PREC_FP64, PREC_FP32, PREC_FP16 = 0, 1, 2
 STOP_RESIDUAL, STOP_PRECONDITIONED = 0, 1
 SMOOTH_JACOBI, SMOOTH_BLOCK_GS = 0, 1
+PRECOND_AMG, PRECOND_JACOBI = 0, 1
 OK, NOT_CONVERGED = 0, 1
 PROFILE_CLASSES = ("cg_spmv", "cg_update", "cg_dots", "presmooth", "restrict",
                    "postsmooth", "jacobi", "coarse")
@@ -60,7 +61,7 @@ class Options(ctypes.Structure):
                 ("nu", c_int), ("omega", c_dbl), ("ocf", c_dbl), ("coarse_max", c_int), ("check_every", c_int),
                 ("tail_max", c_int), ("tail_gamma", c_int), ("use_graph", c_int), ("tail_lds", c_int),
                 ("smoother", c_int), ("gs_sweeps", c_int), ("autotune", c_int), ("deflate", c_int),
-                ("deflate_ratio", c_dbl)]
+                ("deflate_ratio", c_dbl), ("precond", c_int)]
 
 
 class Status(ctypes.Structure):

@@ -100,6 +100,8 @@ enum { HMFEM_PREC_FP64 = 0, HMFEM_PREC_FP32 = 1, HMFEM_PREC_FP16 = 2 };
 enum { HMFEM_STOP_RESIDUAL = 0, HMFEM_STOP_PRECONDITIONED = 1 };
 /* Smoother of the multi-block V-cycle levels (hmfem_options.smoother). */
 enum { HMFEM_SMOOTH_JACOBI = 0, HMFEM_SMOOTH_BLOCK_GS = 1 };
+/* Preconditioner of the CG (hmfem_options.precond). */
+enum { HMFEM_PRECOND_AMG = 0, HMFEM_PRECOND_JACOBI = 1 };
 
 typedef struct {
     int nx, ny, nz;
@@ -140,6 +142,9 @@ typedef struct {
                                 (karst conduits); off when there are none (default 1; single and block
                                 solves) */
     double deflate_ratio;    /* default 3000 */
+    int precond;             /* HMFEM_PRECOND_AMG (default), or HMFEM_PRECOND_JACOBI: z = r / diag(A) in
+                                FP64 with no hierarchy, the plain FP64 PCG that the AMG is measured
+                                against (single solves; precision, smoother and deflate are ignored) */
 } hmfem_options;
 
 typedef struct {

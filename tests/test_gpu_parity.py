@@ -361,6 +361,27 @@ def test_deflation_clusters(kind, dims, seed, ctx, torch):
     assert st.status == S.OK and st.deflation_clusters == 0 and st.deflation_cells == 0
 
 
+def test_plain_jacobi_pcg(ctx, torch):
+    """The plain FP64 PCG (diagonal preconditioner, no hierarchy): the oracle's
+    solution at 1e-12, with many more iterations than the AMG preconditioner."""
+    P = make_problem((32, 32, 24), "lognormal", seed=9, sigma=1.5, corr=4.0)
+    _say("plain jacobi pcg")
+    D = Dev(P, torch)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    oj = S.default_options(rtol=1e-12, precision=S.PREC_FP64)
+    oj.precond = S.PRECOND_JACOBI
+    oj.max_iterations = 20000
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, oj)
+    assert st.status == S.OK and st.levels == 0 and st.deflation_clusters == 0
+    xo, operm, oind, ob = oracle_solution(P, eps=1e-12)
+    x = p.cpu().numpy()
+    assert rel_l2(x, xo) < PRESSURE_TOL
+    assert true_residual(operm, oind, ob, x) < 1e-11
+    p.zero_()
+    sa = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-12))
+    assert sa.status == S.OK and st.iterations > 3 * sa.iterations, (st.iterations, sa.iterations)
+
+
 def test_autotune(ctx, torch):
     """The autotuner returns converging options for the problem; solving with
     them (explicitly, and through opt.autotune on the context's cache) gives
