@@ -11,8 +11,8 @@
 // zeroed).
 //
 // The V-cycle is the single-RHS cycle with the block red-black Gauss-Seidel
-// smoother (tiles of 32 x 8 x TZ cells, TZ = 4 so the LDS tile holds NR values
-// per cell), the brick residual-restriction, and the single-workgroup tail run
+// smoother (tiles of 32 x 8 x 8 cells, NR values per cell in LDS), the
+// residual-restriction marching along z, and the single-workgroup tail run
 // once per right-hand side on strided views of the block vectors.
 #include "hmfem_hier.h"
 
@@ -23,7 +23,13 @@
 namespace hmfem {
 
 constexpr int kMaxRhs = 8;
-constexpr int kBTz = 8;   // z extent of a block-smoother tile
+// z extent of a block-smoother tile: 32 x 8 x 8 tiles, two threads per
+// column. At 256^3 with four right-hand sides a CG iteration takes 3.19 ms
+// against 3.05 for 32 x 8 x 4 tiles (one thread per column), 3.82 (32 x 8 x 4,
+// two per column) and 3.95 (32 x 8 x 8, four per column), but on the 512^3
+// sigma=3 conduit field the thinner tiles (more frozen ring) need 10-15% more
+// iterations, which costs more than they save.
+constexpr int kBTz = 8;
 
 struct BlockScalars {
     double bb[kMaxRhs], rr[kMaxRhs], sigma_first[kMaxRhs], beta[kMaxRhs], tau[kMaxRhs];
@@ -291,19 +297,26 @@ k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, 
     }
     int lb, beg, end;
     chunk_range(n, chunk, lb, beg, end);
+    // flat over the interleaved values of the chunk's cells, one 8-byte value
+    // per lane (coalesced); kBlock is a multiple of NR, so a thread always
+    // serves right-hand side threadIdx.x % NR
+    static_assert(kBlock % NR == 0, "a thread serves one right-hand side");
+    const int mine = threadIdx.x % NR;
+    double am = 0.0;
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+        if (r == mine) am = alpha[r];
+    double my = 0.0;
+    const long qe = (long)end * NR;
+    for (long q = (long)beg * NR + threadIdx.x; q < qe; q += kBlock) {
+        const double ri = R[q] - am * V[q];
+        X[q] += am * S[q];
+        R[q] = ri;
+        my += ri * ri;
+    }
     double acc[NR];
 #pragma unroll
-    for (int r = 0; r < NR; r++) acc[r] = 0.0;
-    for (int s = beg + threadIdx.x; s < end; s += kBlock) {
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const long q = (long)s * NR + r;
-            const double ri = R[q] - alpha[r] * V[q];
-            X[q] += alpha[r] * S[q];
-            R[q] = ri;
-            acc[r] += ri * ri;
-        }
-    }
+    for (int r = 0; r < NR; r++) acc[r] = r == mine ? my : 0.0;
     block_sum<NR>(acc);
     if (threadIdx.x == 0)
         for (int r = 0; r < NR; r++) Prr[(long)NR * lb + r] = acc[r];
@@ -458,8 +471,7 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
 }
 
 // Residual f - A u summed over 2x2x2 aggregates for NR right-hand sides, one
-// thread per fine cell (bricks of 32 x 4 x 2 fine cells as in the single-RHS
-// kernel).
+// thread per fine cell (bricks of 32 x 4 x 2 fine cells).
 template <int NR, typename FT>
 __global__ void __launch_bounds__(kBlock)
 k_b_restrict(OpArgs<float> o, int cx, int cy, int cz, const FT *__restrict__ f, const float *__restrict__ u,
@@ -497,6 +509,111 @@ k_b_restrict(OpArgs<float> o, int cx, int cy, int cz, const FT *__restrict__ f, 
         if (ci < cx && cj < cy && ck < cz) {
             const long I = ci + (long)cx * (cj + (long)cy * ck);
             for (int r = 0; r < NR; r++) fc[I * NR + r] = res[r] + upper[cjl][cil][r];
+        }
+    }
+}
+
+// The same marching along z (as the single-RHS k_restrict_z): a block owns a
+// kGsTx x kGsTy column tile and planes [kb, kb + zlen) (both even); the
+// column's NR values at k-1, k, k+1 stay in registers, the plane and its ring
+// go through a double-buffered LDS plane; box sums over two planes in
+// registers, then the x and y pairs by lane shuffles.
+template <int NR, typename FT>
+__global__ void __launch_bounds__(kBlock)
+k_b_restrict_z(OpArgs<float> o, int zlen, int cx, int cy, int cz, const FT *__restrict__ f,
+               const float *__restrict__ u, float *__restrict__ fc, const int *done)
+{
+    if (*done) return;
+    __shared__ float sh[2][kGsTy + 2][kGsTx + 2][NR];
+    const int nx = o.nx, ny = o.ny, nz = o.nz;
+    const long nxy = (long)nx * ny;
+    const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy;
+    const int ntxy = ntx * nty, nzc = (nz + zlen - 1) / zlen;
+    const int lb = xcd_block();
+    if (lb >= ntxy * nzc) return;   // the whole block
+    const int tile = lb % ntxy, zc = lb / ntxy;
+    const int i0 = (tile % ntx) * kGsTx, j0 = (tile / ntx) * kGsTy;
+    const int kb = zc * zlen, ke = min(nz, kb + zlen);
+    const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool col = i < nx && j < ny;
+    int rx = 0, ry = 0;
+    const bool has_ring = threadIdx.x < kGsRing;
+    if (has_ring) {
+        int hz;
+        gs_ring_cell(threadIdx.x, rx, ry, hz);
+    }
+    const int ri = i0 + rx - 1, rj = j0 + ry - 1;
+    const bool ring_in = has_ring && ri >= 0 && rj >= 0 && ri < nx && rj < ny;
+    const long cs = i + (long)j * nx, rs = ri + (long)rj * nx;
+    float um[NR], uc[NR], up[NR], acc[NR];
+    float wzm = 0.f;
+#pragma unroll
+    for (int r = 0; r < NR; r++) { um[r] = 0.f; uc[r] = 0.f; up[r] = 0.f; acc[r] = 0.f; }
+    if (col) {
+        if (kb > 0) {
+            wzm = o.wz[cs + (kb - 1) * nxy];
+#pragma unroll
+            for (int r = 0; r < NR; r++) um[r] = u[(cs + (kb - 1) * nxy) * NR + r];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) uc[r] = u[(cs + kb * nxy) * NR + r];
+        if (kb + 1 < nz)
+#pragma unroll
+            for (int r = 0; r < NR; r++) up[r] = u[(cs + (kb + 1) * nxy) * NR + r];
+    }
+    for (int kk = kb; kk < ke; kk++) {
+        const int bf = kk & 1;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            sh[bf][ty + 1][tx + 1][r] = uc[r];
+            if (has_ring) sh[bf][ry][rx][r] = ring_in ? u[(rs + kk * nxy) * NR + r] : 0.f;
+        }
+        float e = 0.f, xp = 0.f, xm = 0.f, yp = 0.f, ym = 0.f, zp = 0.f, fq[NR], un[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) { fq[r] = 0.f; un[r] = 0.f; }
+        if (col) {
+            const long q = cs + kk * nxy;
+            e = o.e[q];
+            if (i < nx - 1) xp = o.wx[q];
+            if (i > 0) xm = o.wx[q - 1];
+            if (j < ny - 1) yp = o.wy[q];
+            if (j > 0) ym = o.wy[q - nx];
+            if (kk < nz - 1) zp = o.wz[q];
+#pragma unroll
+            for (int r = 0; r < NR; r++) fq[r] = (float)f[q * NR + r];
+            if (kk + 2 < nz)
+#pragma unroll
+                for (int r = 0; r < NR; r++) un[r] = u[(q + 2 * nxy) * NR + r];
+        }
+        __syncthreads();
+        if (col) {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const float us = uc[r];
+                float a = e * us;
+                a += xp * (us - sh[bf][ty + 1][tx + 2][r]);
+                a += xm * (us - sh[bf][ty + 1][tx][r]);
+                a += yp * (us - sh[bf][ty + 2][tx + 1][r]);
+                a += ym * (us - sh[bf][ty][tx + 1][r]);
+                a += zp * (us - up[r]);
+                a += wzm * (us - um[r]);
+                acc[r] += fq[r] - a;
+                um[r] = uc[r]; uc[r] = up[r]; up[r] = un[r];
+            }
+            wzm = zp;
+        }
+        if (((kk - kb) & 1) || kk == ke - 1) {   // block-uniform
+            const int ci = i >> 1, cj = j >> 1, ck = kk >> 1;
+            const bool w = !(tx & 1) && !(ty & 1) && ci < cx && cj < cy && ck < cz;
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                float t = acc[r];
+                t += __shfl_xor(t, 1);
+                t += __shfl_xor(t, 32);
+                if (w) fc[(ci + (long)cx * (cj + (long)cy * ck)) * NR + r] = t;
+                acc[r] = 0.f;
+            }
         }
     }
 }
@@ -545,6 +662,15 @@ template <int NR, typename FT>
 static void b_restrict(hipStream_t st, const Level &L, const Level &C, const FT *f, const float *u, float *fc,
                        const int *done)
 {
+    if (L.fx == 2 && L.fy == 2 && L.fz == 2) {
+        const long ntxy = (long)((L.d.nx + kGsTx - 1) / kGsTx) * ((L.d.ny + kGsTy - 1) / kGsTy);
+        int zl = 16;
+        while (zl > 2 && ntxy * ((L.d.nz + zl - 1) / zl) < 2048) zl -= 2;
+        const long nb = ntxy * ((L.d.nz + zl - 1) / zl);
+        hipLaunchKernelGGL((k_b_restrict_z<NR, FT>), dim3(grid_for(nb, 1)), dim3(kBlock), 0, st, opargs<float>(L),
+                           zl, C.d.nx, C.d.ny, C.d.nz, f, u, fc, done);
+        return;
+    }
     const int nb = ((L.d.nx + 31) / 32) * ((L.d.ny + 3) / 4) * ((L.d.nz + 1) / 2);
     hipLaunchKernelGGL((k_b_restrict<NR, FT>), dim3(grid_for(nb, 1)), dim3(kBlock), 0, st, opargs<float>(L),
                        C.d.nx, C.d.ny, C.d.nz, f, u, fc, done);

@@ -126,42 +126,96 @@ k_resid_restrict(OpArgs<T> o, int fx, int fy, int fz, int cx, int cy, int cz, co
     fc[I] = acc;
 }
 
-// Residual r = f - A u summed over 2x2x2 aggregates, one thread per fine
-// cell: a block covers a brick of 32 x 4 x 2 fine cells (16 x 2 x 1
-// aggregates). Lanes i and i^1 hold an x pair, lanes l and l^32 a y pair
-// (rows j and j+1 are the two halves of a wavefront), and the two z planes
-// are the two halves of the block, combined through LDS.
+// Residual f - A u summed over 2x2x2 boxes, marching along z: a block owns a
+// kGsTx x kGsTy column tile (one thread per column) and the planes
+// [kb, kb + zlen) (kb and zlen even). A thread keeps its column's u at k-1, k,
+// k+1 and wz at k-1 in registers; the plane's u and its x/y ring go through a
+// double-buffered LDS plane (one barrier per plane). The box sums are two
+// planes in a register, then the x and y pairs by lane shuffles (lanes t^1,
+// t^32), written by the lane of the box's first cell. Same operation order as
+// apply_at, with a zero coupling where there is no neighbour.
 template <typename T, typename FT>
 __global__ void __launch_bounds__(kBlock)
-k_restrict_brick(OpArgs<T> o, int cx, int cy, int cz, const FT *__restrict__ f, const T *__restrict__ u,
-                 T *__restrict__ fc, const int *done)
+k_restrict_z(OpArgs<T> o, int zlen, int cx, int cy, int cz, const FT *__restrict__ f, const T *__restrict__ u,
+             T *__restrict__ fc, const int *done)
 {
-    using C = T;
     if (*done) return;
-    __shared__ C upper[2][16];
-    const int nbx = (o.nx + 31) / 32, nby = (o.ny + 3) / 4;
-    const int b = xcd_block();
-    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    const int t = threadIdx.x;
-    const int i = bx * 32 + (t & 31), j = by * 4 + ((t >> 5) & 3), k = bz * 2 + (t >> 7);
-    C r = 0;
-    if (i < o.nx && j < o.ny && k < o.nz) {
-        Cell c;
-        c.i = i; c.j = j; c.k = k;
-        c.s = i + j * o.nx + k * o.nx * o.ny;
-        C d;
-        C au = apply_at<T, C>(o, c, [&](const Cell &q) -> C { return (C)u[q.s]; }, d);
-        r = (C)f[c.s] - au;
+    __shared__ T sh[2][kGsTy + 2][kGsTx + 2];
+    const int nx = o.nx, ny = o.ny, nz = o.nz;
+    const long nxy = (long)nx * ny;
+    const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy;
+    const int ntxy = ntx * nty, nzc = (nz + zlen - 1) / zlen;
+    const int lb = xcd_block();
+    if (lb >= ntxy * nzc) return;   // the whole block
+    const int tile = lb % ntxy, zc = lb / ntxy;
+    const int i0 = (tile % ntx) * kGsTx, j0 = (tile / ntx) * kGsTy;
+    const int kb = zc * zlen, ke = min(nz, kb + zlen);
+    const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool col = i < nx && j < ny;
+    int rx = 0, ry = 0;
+    const bool has_ring = threadIdx.x < kGsRing;
+    if (has_ring) {
+        int hz;
+        gs_ring_cell(threadIdx.x, rx, ry, hz);
     }
-    r += __shfl_xor(r, 1);
-    r += __shfl_xor(r, 32);
-    const int lane = t & 63, cjl = (t >> 6) & 1, cil = (t & 31) >> 1;
-    if ((t >> 7) == 1 && lane < 32 && !(lane & 1)) upper[cjl][cil] = r;
-    __syncthreads();
-    if ((t >> 7) == 0 && lane < 32 && !(lane & 1)) {
-        const int ci = bx * 16 + cil, cj = by * 2 + cjl, ck = bz;
-        if (ci < cx && cj < cy && ck < cz) fc[ci + cx * (cj + cy * ck)] = r + upper[cjl][cil];
+    const int ri = i0 + rx - 1, rj = j0 + ry - 1;
+    const bool ring_in = has_ring && ri >= 0 && rj >= 0 && ri < nx && rj < ny;
+    const long cs = i + (long)j * nx, rs = ri + (long)rj * nx;
+    T um = (T)0, uc = (T)0, up = (T)0, wzm = (T)0;
+    if (col) {
+        if (kb > 0) { um = u[cs + (kb - 1) * nxy]; wzm = o.wz[cs + (kb - 1) * nxy]; }
+        uc = u[cs + kb * nxy];
+        if (kb + 1 < nz) up = u[cs + (kb + 1) * nxy];
     }
+    T acc = (T)0;
+    for (int kk = kb; kk < ke; kk++) {
+        const int bf = kk & 1;
+        sh[bf][ty + 1][tx + 1] = uc;
+        if (has_ring) sh[bf][ry][rx] = ring_in ? u[rs + kk * nxy] : (T)0;
+        T r = (T)0, e = (T)0, xp = (T)0, xm = (T)0, yp = (T)0, ym = (T)0, zp = (T)0, fq = (T)0, un = (T)0;
+        if (col) {
+            const long q = cs + kk * nxy;
+            e = o.e[q];
+            if (i < nx - 1) xp = o.wx[q];
+            if (i > 0) xm = o.wx[q - 1];
+            if (j < ny - 1) yp = o.wy[q];
+            if (j > 0) ym = o.wy[q - nx];
+            if (kk < nz - 1) zp = o.wz[q];
+            fq = (T)f[q];
+            if (kk + 2 < nz) un = u[q + 2 * nxy];
+        }
+        __syncthreads();
+        if (col) {
+            T a = e * uc;
+            a += xp * (uc - sh[bf][ty + 1][tx + 2]);
+            a += xm * (uc - sh[bf][ty + 1][tx]);
+            a += yp * (uc - sh[bf][ty + 2][tx + 1]);
+            a += ym * (uc - sh[bf][ty][tx + 1]);
+            a += zp * (uc - up);
+            a += wzm * (uc - um);
+            r = fq - a;
+            um = uc; uc = up; up = un; wzm = zp;
+        }
+        acc += r;
+        if (((kk - kb) & 1) || kk == ke - 1) {   // block-uniform
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 32);
+            const int ci = i >> 1, cj = j >> 1, ck = kk >> 1;
+            if (!(tx & 1) && !(ty & 1) && ci < cx && cj < cy && ck < cz) fc[ci + (long)cx * (cj + (long)cy * ck)] = acc;
+            acc = (T)0;
+        }
+    }
+}
+
+// Planes per block of k_restrict_z: even, at most 16, and enough blocks to
+// fill the device (2048 when the level allows it).
+inline int restrict_zlen(const Dims &d)
+{
+    const long ntxy = (long)((d.nx + kGsTx - 1) / kGsTx) * ((d.ny + kGsTy - 1) / kGsTy);
+    long zl = 16;
+    while (zl > 2 && ntxy * ((d.nz + zl - 1) / zl) < 2048) zl -= 2;
+    return (int)zl;
 }
 
 // Coarse correction u' = u + ocf * u_c[agg] followed by one Jacobi sweep on u'.
@@ -1327,7 +1381,7 @@ struct Prof {
         if (P) (P)->end();                \
     } while (0)
 
-// Residual + restriction from level L to C: the brick kernel for 2x2x2
+// Residual + restriction from level L to C: the z-marching kernel for 2x2x2
 // aggregation, the general one otherwise.
 template <typename T, typename FT>
 static void launch_restrict(hipStream_t st, Prof *P, int cls, const Level &L, const Level &C, const FT *f,
@@ -1335,8 +1389,9 @@ static void launch_restrict(hipStream_t st, Prof *P, int cls, const Level &L, co
 {
     OpArgs<T> o = opargs<T>(L);
     if (L.fx == 2 && L.fy == 2 && L.fz == 2) {
-        const int nb = ((L.d.nx + 31) / 32) * ((L.d.ny + 3) / 4) * ((L.d.nz + 1) / 2);
-        PROF_LAUNCH(P, cls, (k_restrict_brick<T, FT>), dim3(grid_for(nb, 1)), dim3(kBlock), 0, st, o, C.d.nx,
+        const int zl = restrict_zlen(L.d);
+        const int nb = ((L.d.nx + kGsTx - 1) / kGsTx) * ((L.d.ny + kGsTy - 1) / kGsTy) * ((L.d.nz + zl - 1) / zl);
+        PROF_LAUNCH(P, cls, (k_restrict_z<T, FT>), dim3(grid_for(nb, 1)), dim3(kBlock), 0, st, o, zl, C.d.nx,
                     C.d.ny, C.d.nz, f, u, (T *)C.f, done);
     } else {
         PROF_LAUNCH(P, cls, (k_resid_restrict<T, FT, 0>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock), 0, st, o,
