@@ -1112,14 +1112,20 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
     if (threadIdx.x == 0) Ptau[lb] = acc[0];
 }
 
-// Planes per block of k_cg_spmv_z so that it needs at most kMaxReduceBlocks
-// blocks (at least 4 planes each); 0 if a single plane has more tiles than that.
-inline int spmv_zlen(const Dims &d)
+// Blocks of k_cg_spmv_z (each writes one partial of <s, v>). The kernel
+// holds 78 VGPRs (6 waves per SIMD); measured at 128^3 / 256^3: 1024 blocks
+// 29.6 / 219 us, 512: 36.4 / 321, 1536: 30.5 / 214, 2048: 36.3 / 262 (two
+// rounds of blocks), and capping it at 64 VGPRs spills (43 / 360 us).
+constexpr int kMaxSpmvBlocks = 1024;
+
+// Planes per block of k_cg_spmv_z so that it needs at most maxb blocks (at
+// least 4 planes each); 0 if a single plane has more tiles than that.
+inline int spmv_zlen(const Dims &d, int maxb = kMaxSpmvBlocks)
 {
     const long ntxy = (long)((d.nx + kGsTx - 1) / kGsTx) * ((d.ny + kGsTy - 1) / kGsTy);
-    if (ntxy > kMaxReduceBlocks) return 0;
+    if (ntxy > maxb) return 0;
     long zl = 4;
-    while (ntxy * ((d.nz + zl - 1) / zl) > kMaxReduceBlocks) zl++;
+    while (ntxy * ((d.nz + zl - 1) / zl) > maxb) zl++;
     return (int)zl;
 }
 
@@ -1598,9 +1604,9 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     S.ch = chunking(n);
     const int G = S.ch.G;
     S.Pinit = B.partial;
-    S.Ptau = B.partial + 2 * kMaxReduceBlocks;
-    S.Prr = B.partial + 3 * kMaxReduceBlocks;
-    S.Pdot = B.partial + 4 * kMaxReduceBlocks;
+    S.Prr = B.partial + 2 * kMaxReduceBlocks;
+    S.Pdot = B.partial + 3 * kMaxReduceBlocks;                  // 2 (G + 1) <= 2 kMaxReduceBlocks + 2
+    S.Ptau = B.partial + 5 * kMaxReduceBlocks + 64;             // up to kMaxSpmvBlocks
     S.done = &B.sc->done;
 
     CgScalars init{};
@@ -1857,7 +1863,7 @@ int launch_tail_strided(hipStream_t st, const Hierarchy &H, double omega, double
 size_t cg_partial_count(long n)
 {
     (void)n;
-    return (size_t)6 * kMaxReduceBlocks + 64;
+    return (size_t)5 * kMaxReduceBlocks + 64 + kMaxSpmvBlocks;
 }
 
 } // namespace hmfem
