@@ -403,7 +403,7 @@ template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
 kb_cg_spmv(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<double> A, const ZT *__restrict__ z,
            double *__restrict__ s0, double *__restrict__ s1, double *__restrict__ v, CgScalars *SC,
-           const double *Pinit, const double *Prr, const double *Pdot, double *Ptau)
+           const double *Pinit, const double *Prr, const double *Pdot, double *Ptau, const double *Pext)
 {
     const int q = cbmap[blockIdx.x];
     CgScalars *sc = SC + q;
@@ -412,6 +412,10 @@ kb_cg_spmv(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<dou
     const int k = sc->kspmv;
     double dots[2];
     sum_partials<2>(Pdot + 2 * I.cblk0, I.cG, dots);
+    if (Pext) {   // the conduit correction's share of <z, r> and <z, v>
+        dots[0] += Pext[2 * q];
+        dots[1] += Pext[2 * q + 1];
+    }
     double bb, rr;
     if (k == 0) {
         double ir[2];
@@ -538,6 +542,7 @@ struct Batch {
     double *Pinit = nullptr, *Ptau = nullptr, *Prr = nullptr, *Pdot = nullptr;
     double *r = nullptr, *s0 = nullptr, *s1 = nullptr, *v = nullptr;
     CgScalars *sc = nullptr;
+    int *never = nullptr;   // a device int that stays 0 (the correction's kernels never skip)
 };
 
 size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -714,7 +719,8 @@ T *vb_level(hipStream_t st, Batch &B, int l, const FT *f, const double *vdot, co
 int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A0, const double *b, double *x,
                 const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *alloc_ctx, CgScalars *h_sc,
                 GraphCache *graph, std::vector<char> &stage, hipEvent_t ev[4], int *iters_out,
-                double *relres_out, double *done_ms_out, int *status_out, double *setup_ms, double *solve_ms)
+                double *relres_out, double *done_ms_out, int *status_out, double *setup_ms, double *solve_ms,
+                Deflation *defl, char *(*dalloc)(void *, size_t, int), void *dctx)
 {
     hmfem_options opt = opt_in;
     if (opt.nu < 2) opt.nu = 2;
@@ -755,7 +761,8 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     B.Prr = cv.take<double>(B.ncblk);
     B.Pdot = cv.take<double>(2 * (size_t)B.ncblk);
     B.sc = cv.take<CgScalars>(B.ninst);
-    if (!B.sc || !B.Pdot || !B.inv) return HMFEM_ERR_NOMEM;
+    B.never = cv.take<int>(1);
+    if (!B.sc || !B.Pdot || !B.inv || !B.never) return HMFEM_ERR_NOMEM;
 
     // descriptors
     {
@@ -824,6 +831,18 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     hipLaunchKernelGGL(kb_dense_invert, dim3(B.ninst), dim3(1024), 0, st, B.lev[B.nlev - 1].d,
                        lp<double>(B.lev[B.nlev - 1]), B.inv, B.info);
     HMFEM_CHECK(hipGetLastError());
+    // conduit correction: per-instance thresholds and clusters
+    const Deflation *D = nullptr;
+    if (defl) {
+        defl->nc = 0;
+        defl->pending = false;
+        if (opt.deflate && dalloc) {
+            int rc = deflation_setup_batch(st, ninst, dims, A0, opt.deflate_ratio, *defl, dalloc, dctx);
+            if (rc) return rc;
+            if (defl->nc > 0) D = defl;
+        }
+    }
+    HMFEM_CHECK(hipMemsetAsync(B.never, 0, sizeof(int), st));
     HMFEM_CHECK(hipEventRecord(ev[1], st));
 
     // CG
@@ -831,17 +850,25 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     LPtr<double> A64 = lp<double>(B.lev[0]);
     hipLaunchKernelGGL(kb_cg_init, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, A64, b, x, B.r, B.Pinit);
     auto vcycle = [&](const double *vd) -> void * {
-        if (prec == HMFEM_PREC_FP64) return vb_level<double, double>(st, B, 0, B.r, vd, cp, B.Pdot);
-        return vb_level<float, double>(st, B, 0, B.r, vd, cp, B.Pdot);
+        if (D) deflation_sums_batch(st, *D, B.r, vd, B.never);
+        if (prec == HMFEM_PREC_FP64) {
+            double *zz = vb_level<double, double>(st, B, 0, B.r, vd, cp, B.Pdot);
+            if (D) deflation_scatter(st, *D, zz, B.never);
+            return zz;
+        }
+        float *zz = vb_level<float, double>(st, B, 0, B.r, vd, cp, B.Pdot);
+        if (D) deflation_scatter(st, *D, zz, B.never);
+        return zz;
     };
+    const double *Pext = D ? D->pext : nullptr;
     void *z = vcycle(nullptr);
     auto iteration = [&]() -> void * {
         if (prec == HMFEM_PREC_FP64) {
             hipLaunchKernelGGL(kb_cg_spmv<double>, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, A64,
-                               (const double *)z, B.s0, B.s1, B.v, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau);
+                               (const double *)z, B.s0, B.s1, B.v, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau, Pext);
         } else {
             hipLaunchKernelGGL(kb_cg_spmv<float>, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, A64,
-                               (const float *)z, B.s0, B.s1, B.v, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau);
+                               (const float *)z, B.s0, B.s1, B.v, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau, Pext);
         }
         hipLaunchKernelGGL(kb_cg_update, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, B.s0, B.s1, B.v, x, B.r,
                            B.sc, B.Ptau, B.Prr);

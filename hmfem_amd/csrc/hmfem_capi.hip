@@ -536,7 +536,8 @@ extern "C" int hmfem_reservoir_batch_device(hmfem_context *c, int ninst, const i
     double setup = 0, solve = 0;
     std::vector<char> stage2;
     rc = batch_solve(c->st, ninst, dims, A, b, p, opt, ctx_alloc, c, hsc.data(), opt.use_graph ? &c->graph : nullptr,
-                     stage2, c->ev, its.data(), rr.data(), dm.data(), sts.data(), &setup, &solve);
+                     stage2, c->ev, its.data(), rr.data(), dm.data(), sts.data(), &setup, &solve, &c->defl,
+                     ctx_alloc_defl, c);
     if (rc) return rc;
     float ta = 0.f;
     (void)hipEventElapsedTime(&ta, c->ev[4], c->ev[5]);
@@ -545,6 +546,8 @@ extern "C" int hmfem_reservoir_batch_device(hmfem_context *c, int ninst, const i
     out->t_assemble_ms = ta;
     out->t_setup_ms = setup;
     out->t_solve_ms = solve;
+    out->deflation_clusters = c->defl.nc;
+    out->deflation_cells = c->defl.nc ? c->defl.m : 0;
     out->status = HMFEM_OK;
     out->max_iterations = 0;
     for (int q = 0; q < ninst; q++) {

@@ -53,90 +53,184 @@ k_diag_hist(OpArgs<double> o, unsigned *hist)
         if (h[q]) atomicAdd(&hist[q], h[q]);
 }
 
-// Strong cells (diagonal above thr) in index order, in three steps: a bit
-// mask per wave and a count per block (logical block lb covers cells
-// [lb*kBlock, lb*kBlock + kBlock)), an exclusive scan of the counts, and the
-// ordered list.
+// Grids handled by one setup: a single grid, or the instances of a batch
+// concatenated in one index space. Block b of the per-cell setup kernels
+// covers cells [(b - blk0) * kBlock, ...) of instance bmap[b] (bmap null: one
+// instance).
+struct DefInst {
+    int nx, ny, nz, n;
+    int off, blk0;
+    double thr;   // strong-cell threshold of the instance (inf: none)
+    int hblk0;    // first block of the instance in the histogram launch
+};
+
+constexpr int kHistCells = 16;   // cells per thread of the batch histogram
+
+__device__ __forceinline__ bool inst_cell(const DefInst *__restrict__ I, const int *__restrict__ bmap,
+                                          const Stencil<double> &A, DefInst &in, OpArgs<double> &o, Cell &c)
+{
+    const int q = bmap ? bmap[blockIdx.x] : 0;
+    in = I[q];
+    const int l = (blockIdx.x - in.blk0) * kBlock + threadIdx.x;
+    o.e = A.e + in.off; o.wx = A.wx + in.off; o.wy = A.wy + in.off; o.wz = A.wz + in.off; o.di = nullptr;
+    o.nx = in.nx; o.ny = in.ny; o.nz = in.nz;
+    if (l >= in.n) return false;
+    c = cell_at(o, l);
+    return true;
+}
+
+// Per-instance histogram of floor(log2 d); block b covers kBlock * kHistCells
+// cells of instance hmap[b].
+__global__ void __launch_bounds__(kBlock)
+k_diag_hist_inst(const DefInst *__restrict__ I, const int *__restrict__ hmap, Stencil<double> A, unsigned *hist)
+{
+    __shared__ unsigned h[kHistBins];
+    for (int q = threadIdx.x; q < kHistBins; q += kBlock) h[q] = 0;
+    __syncthreads();
+    const int q = hmap[blockIdx.x];
+    const DefInst in = I[q];
+    OpArgs<double> o;
+    o.e = A.e + in.off; o.wx = A.wx + in.off; o.wy = A.wy + in.off; o.wz = A.wz + in.off; o.di = nullptr;
+    o.nx = in.nx; o.ny = in.ny; o.nz = in.nz;
+    const int l0 = (blockIdx.x - in.hblk0) * kBlock * kHistCells + threadIdx.x;
+    for (int t = 0; t < kHistCells; t++) {
+        const int l = l0 + t * kBlock;
+        if (l >= in.n) break;
+        const double d = diag_at<double, double>(o, cell_at(o, l));
+        int b = d > 0 ? ilogb(d) + 1100 : 0;
+        b = b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
+        atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kHistBins; t += kBlock)
+        if (h[t]) atomicAdd(&hist[(long)q * kHistBins + t], h[t]);
+}
+
+// Strong cells (diagonal above the instance's threshold) in index order, in
+// three steps: a bit mask per wave and a count per block, an exclusive scan
+// of the counts, and the ordered list of global indices.
 constexpr int kWavesPerBlock = kBlock / 64;
 
 __global__ void __launch_bounds__(kBlock)
-k_strong_mask(OpArgs<double> o, double thr, unsigned long long *mask, int *bcnt)
+k_strong_mask(const DefInst *__restrict__ I, const int *__restrict__ bmap, Stencil<double> A,
+              unsigned long long *mask, int *bcnt)
 {
     __shared__ int wc[kWavesPerBlock];
-    const int n = o.nx * o.ny * o.nz;
-    const int lb = xcd_block();
-    const int s = lb * kBlock + threadIdx.x;
+    DefInst in;
+    OpArgs<double> o;
+    Cell c;
     bool f = false;
-    if (s < n) {
-        Cell c = cell_at(o, s);
-        f = diag_at<double, double>(o, c) > thr;
-    }
+    if (inst_cell(I, bmap, A, in, o, c)) f = diag_at<double, double>(o, c) > in.thr;
     const unsigned long long m = __ballot(f);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        mask[(long)lb * kWavesPerBlock + w] = m;
+        mask[(long)blockIdx.x * kWavesPerBlock + w] = m;
         wc[w] = __popcll(m);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int t = 0;
         for (int q = 0; q < kWavesPerBlock; q++) t += wc[q];
-        bcnt[lb] = t;
+        bcnt[blockIdx.x] = t;
     }
 }
 
-// In place exclusive scan of cnt[0..G), total in cnt[G]; one block.
+// In place exclusive scan of cnt[0..G), total in cnt[G]: tiles of
+// kScanThreads counts scanned by one block each (tile totals to tot[]), the
+// totals scanned by one block, then added back.
 constexpr int kScanThreads = 1024;
-__global__ void __launch_bounds__(kScanThreads)
-k_scan_counts(int *cnt, int G)
+
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total)
 {
-    __shared__ int part[kScanThreads];
-    const int per = (G + kScanThreads - 1) / kScanThreads;
-    const int beg = threadIdx.x * per, end = min(G, beg + per);
-    int t = 0;
-    for (int q = beg; q < end; q++) t += cnt[q];
-    part[threadIdx.x] = t;
+    // inclusive scan inside each wave by shuffles, then over the wave totals
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[w] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {   // 1024 partial sums: sequential is cheap enough for a setup step
+    if (threadIdx.x == 0) {
         int a = 0;
-        for (int q = 0; q < kScanThreads; q++) { const int v = part[q]; part[q] = a; a += v; }
-        cnt[G] = a;
+        for (int q = 0; q < kScanThreads / 64; q++) { const int t = sh[q]; sh[q] = a; a += t; }
+        sh[kScanThreads / 64] = a;
     }
     __syncthreads();
-    int a = part[threadIdx.x];
-    for (int q = beg; q < end; q++) { const int v = cnt[q]; cnt[q] = a; a += v; }
+    total = sh[kScanThreads / 64];
+    return sh[w] + x - v;
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_strong_emit(const unsigned long long *__restrict__ mask, const int *__restrict__ boff, int *list, int cap)
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_tiles(int *cnt, int G, int *tot)
 {
-    const int lb = xcd_block();
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const unsigned long long m = mask[(long)lb * kWavesPerBlock + w];
-    if (!((m >> lane) & 1ull)) return;
-    int at = boff[lb];
-    for (int q = 0; q < w; q++) at += __popcll(mask[(long)lb * kWavesPerBlock + q]);
-    at += __popcll(m & ((1ull << lane) - 1ull));
-    if (at < cap) list[at] = lb * kBlock + threadIdx.x;
+    __shared__ int sh[kScanThreads / 64 + 1];
+    const int q = blockIdx.x * kScanThreads + threadIdx.x;
+    const int v = q < G ? cnt[q] : 0;
+    int total;
+    const int e = block_excl_scan(v, sh, total);
+    if (q < G) cnt[q] = e;
+    if (threadIdx.x == 0) tot[blockIdx.x] = total;
 }
 
-// Coefficients of listed cells: e and the couplings to -x,+x,-y,+y,-z,+z (0 where none).
+// one block: exclusive scan of the nt <= kScanThreads tile totals; the grand
+// total to cnt[G]
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_totals(int *tot, int nt, int *cnt, int G)
+{
+    __shared__ int sh[kScanThreads / 64 + 1];
+    const int v = threadIdx.x < nt ? tot[threadIdx.x] : 0;
+    int total;
+    const int e = block_excl_scan(v, sh, total);
+    if (threadIdx.x < nt) tot[threadIdx.x] = e;
+    if (threadIdx.x == 0) cnt[G] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_add(int *cnt, int G, const int *tot)
+{
+    const int q = blockIdx.x * kScanThreads + threadIdx.x;
+    if (q < G) cnt[q] += tot[blockIdx.x];
+}
+
 __global__ void __launch_bounds__(kBlock)
-k_defl_gather(OpArgs<double> o, const int *__restrict__ cells, int m, double *out)
+k_strong_emit(const DefInst *__restrict__ I, const int *__restrict__ bmap, const unsigned long long *__restrict__ mask,
+              const int *__restrict__ boff, int *list, int cap)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned long long m = mask[(long)blockIdx.x * kWavesPerBlock + w];
+    if (!((m >> lane) & 1ull)) return;
+    int at = boff[blockIdx.x];
+    for (int q = 0; q < w; q++) at += __popcll(mask[(long)blockIdx.x * kWavesPerBlock + q]);
+    at += __popcll(m & ((1ull << lane) - 1ull));
+    const DefInst in = I[bmap ? bmap[blockIdx.x] : 0];
+    if (at < cap) list[at] = in.off + (blockIdx.x - in.blk0) * kBlock + threadIdx.x;
+}
+
+// Coefficients of listed cells (global index, instance cinst[q]): e and the
+// couplings to -x,+x,-y,+y,-z,+z (0 where none).
+__global__ void __launch_bounds__(kBlock)
+k_defl_gather(const DefInst *__restrict__ I, const int *__restrict__ cinst, Stencil<double> A,
+              const int *__restrict__ cells, int m, double *out)
 {
     const int q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= m) return;
+    const DefInst in = I[cinst ? cinst[q] : 0];
     const int s = cells[q];
-    Cell c = cell_at(o, s);
-    const int nx = o.nx, nxy = o.nx * o.ny;
+    OpArgs<double> o;
+    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
+    o.nx = in.nx; o.ny = in.ny; o.nz = in.nz;
+    const Cell c = cell_at(o, s - in.off);
+    const int nx = in.nx, nxy = in.nx * in.ny;
     double *w = out + 7L * q;
     w[0] = o.e[s];
     w[1] = c.i > 0 ? o.wx[s - 1] : 0.0;
-    w[2] = c.i < o.nx - 1 ? o.wx[s] : 0.0;
+    w[2] = c.i < in.nx - 1 ? o.wx[s] : 0.0;
     w[3] = c.j > 0 ? o.wy[s - nx] : 0.0;
-    w[4] = c.j < o.ny - 1 ? o.wy[s] : 0.0;
+    w[4] = c.j < in.ny - 1 ? o.wy[s] : 0.0;
     w[5] = c.k > 0 ? o.wz[s - nxy] : 0.0;
-    w[6] = c.k < o.nz - 1 ? o.wz[s] : 0.0;
+    w[6] = c.k < in.nz - 1 ? o.wz[s] : 0.0;
 }
 
 // The per-iteration kernels take NR interleaved right-hand sides (vectors
@@ -214,6 +308,39 @@ k_defl_apply(int nc, const int *__restrict__ cp, const double *__restrict__ einv
         for (int j = 0; j < 2 * NR; j++) Pextra[j] = acc[j];
 }
 
+// The batch form of k_defl_apply (one right-hand side per instance): c = E^-1
+// sr over all clusters (E is block diagonal over the instances), and per
+// instance q the extra dot partials over its clusters [irange[q],
+// irange[q+1]): Pext[2q] = c.sr, Pext[2q+1] = c.sv (0 without clusters).
+__global__ void __launch_bounds__(kBlock)
+k_defl_apply_inst(int nc, const int *__restrict__ cp, const double *__restrict__ einv,
+                  const double *__restrict__ psr, const double *__restrict__ psv, const int *__restrict__ irange,
+                  const int *__restrict__ cinst, int ninst, double *c, double *Pext)
+{
+    __shared__ double srs[kMaxClusters], svs[kMaxClusters], cs[kMaxClusters];
+    for (int a = threadIdx.x; a < nc; a += kBlock) {
+        double tr = 0.0, tv = 0.0;
+        for (int p = cp[a]; p < cp[a + 1]; p++) { tr += psr[p]; tv += psv[p]; }
+        srs[a] = tr;
+        svs[a] = tv;
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < nc; a += kBlock) {   // E^-1 is block diagonal over the instances
+        const int q = cinst[a];
+        double t = 0.0;
+        for (int b = irange[q]; b < irange[q + 1]; b++) t += einv[(long)b * nc + a] * srs[b];
+        cs[a] = t;
+        c[a] = t;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < ninst; q += kBlock) {
+        double er = 0.0, ev = 0.0;
+        for (int a = irange[q]; a < irange[q + 1]; a++) { er += cs[a] * srs[a]; ev += cs[a] * svs[a]; }
+        Pext[2 * q] = er;
+        Pext[2 * q + 1] = ev;
+    }
+}
+
 // z += c[cluster] on the cells of every cluster.
 template <int NR, typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -249,70 +376,60 @@ static int find_root(std::vector<int> &p, int a)
     return a;
 }
 
-int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation &D,
-                    char *(*alloc)(void *, size_t, int), void *actx)
+// Setup shared by the single grid and the batch. hist: per instance the
+// host copy of its histogram; inst: the instances (thr filled here); dinst:
+// their device copy (nullptr: upload to slot 0 after the histogram, at
+// hist_bytes); bmap/G: the per-cell kernels' block map and grid.
+static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *dinst, const int *bmap, int G,
+                           const Stencil<double> &A, const unsigned *hist, double ratio, Deflation &D,
+                           char *(*alloc)(void *, size_t, int), void *actx)
 {
-    D.nc = 0;
-    D.m = 0;
-    D.pending = false;
-    const long n = d.n();
-    if (n < 64 || !D.h_hist || !D.hist_ready) return HMFEM_OK;
-    OpArgs<double> o;
-    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
-    o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
-    unsigned *hist = (unsigned *)alloc(actx, kHistBins * sizeof(unsigned), 0);
-    if (!hist) return HMFEM_ERR_NOMEM;
-    HMFEM_CHECK(hipMemsetAsync(hist, 0, kHistBins * sizeof(unsigned), st));
-    // a grid-stride loop over at most 512 blocks keeps the global bin updates
-    // few (2048 blocks: 35 us instead of 25 at 128^3)
-    const int G = std::min(grid_for(n, kBlock), 512);
-    hipLaunchKernelGGL(k_diag_hist, dim3(G), dim3(kBlock), 0, st, o, hist);
-    HMFEM_CHECK(hipMemcpyAsync(D.h_hist, hist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    HMFEM_CHECK(hipEventRecord(D.hist_ready, st));
-    D.pending = true;
-    return HMFEM_OK;
-}
-
-int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ratio, Deflation &D,
-                     char *(*alloc)(void *, size_t, int), void *actx)
-{
-    if (!D.pending) return HMFEM_OK;
-    D.pending = false;
-    const long n = d.n();
-    if (!(ratio > 1.0)) return HMFEM_OK;
-    OpArgs<double> o;
-    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
-    o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
-    const int G = grid_for(n, kBlock);
-    HMFEM_CHECK(hipEventSynchronize(D.hist_ready));
-    const unsigned *h = D.h_hist;
-    // median bin, threshold, and the count of cells certainly above it
-    long acc = 0;
-    int mb = 0;
-    for (; mb < kHistBins; mb++) {
-        acc += h[mb];
-        if (2 * acc >= n) break;
+    const int ninst = (int)inst.size();
+    // per instance: median bin, threshold ~ ratio x median, count certainly
+    // above it and an upper bound on the count above it
+    long above = 0, ub = 0;
+    for (int q = 0; q < ninst; q++) {
+        const unsigned *h = hist + (long)q * kHistBins;
+        DefInst &in = inst[q];
+        in.thr = INFINITY;
+        long acc = 0;
+        int mb = 0;
+        for (; mb < kHistBins; mb++) {
+            acc += h[mb];
+            if (2 * acc >= in.n) break;
+        }
+        const double thr = ratio * ldexp(1.0, mb - 1100) * 1.4142135623730951;
+        const int tb = ilogb(thr) + 1100;
+        long aq = 0, uq = 0;
+        for (int b = tb + 1; b < kHistBins; b++) aq += h[b];
+        for (int b = tb; b < kHistBins; b++) uq += h[b];
+        if (aq < kMinCluster) continue;
+        in.thr = thr;
+        above += aq;
+        ub += uq;
     }
-    const double thr = ratio * ldexp(1.0, mb - 1100) * 1.4142135623730951;   // ~ ratio x median
-    const int tb = ilogb(thr) + 1100;
-    long above = 0;
-    for (int b = tb + 1; b < kHistBins; b++) above += h[b];
     if (above < kMinCluster) return HMFEM_OK;
-
-    // the strong cells, compacted on the device (at most ub of them)
-    long ub = 0;
-    for (int b = tb; b < kHistBins; b++) ub += h[b];
     if (ub > (1L << 22)) return HMFEM_OK;   // not conduits: a large part of the domain
-    // slot 1: masks (G * kWavesPerBlock words), block offsets (G + 1), list (ub)
+
+    // the strong cells, compacted on the device in index order (at most ub)
+    // slot 1: masks (G * kWavesPerBlock words), block offsets (G + 1), list
+    // (ub; right after the total boff[G], so that one copy fetches both), scan
+    // tile totals (nt)
+    const int nt = (G + kScanThreads - 1) / kScanThreads;
+    if (nt > kScanThreads) return HMFEM_OK;   // beyond 2^28 cells
     const size_t mb_bytes = sizeof(unsigned long long) * (size_t)G * kWavesPerBlock;
-    char *sbuf = alloc(actx, mb_bytes + sizeof(int) * ((size_t)G + 1 + ub), 1);
+    char *sbuf = alloc(actx, mb_bytes + sizeof(int) * ((size_t)G + 1 + nt + ub), 1);
     if (!sbuf) return HMFEM_ERR_NOMEM;
     unsigned long long *mask = (unsigned long long *)sbuf;
     int *boff = (int *)(sbuf + mb_bytes);
     int *slist = boff + G + 1;
-    hipLaunchKernelGGL(k_strong_mask, dim3(G), dim3(kBlock), 0, st, o, thr, mask, boff);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, st, boff, G);
-    hipLaunchKernelGGL(k_strong_emit, dim3(G), dim3(kBlock), 0, st, mask, boff, slist, (int)ub);
+    int *tot = slist + ub;
+    HMFEM_CHECK(hipMemcpyAsync(dinst, inst.data(), sizeof(DefInst) * ninst, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_strong_mask, dim3(G), dim3(kBlock), 0, st, dinst, bmap, A, mask, boff);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, st, boff, G, tot);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, st, tot, nt, boff, G);
+    hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(kScanThreads), 0, st, boff, G, (const int *)tot);
+    hipLaunchKernelGGL(k_strong_emit, dim3(G), dim3(kBlock), 0, st, dinst, bmap, mask, boff, slist, (int)ub);
     // the total (boff[G]) and the list in one copy to pinned staging
     const size_t cb = sizeof(int) * (1 + ub);
     if (stage_reserve(D, cb + sizeof(double) * 7 * (size_t)ub)) return HMFEM_ERR_NOMEM;
@@ -322,27 +439,33 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     const int ms = std::min(hl[0], (int)ub);
     if (ms < kMinCluster) return HMFEM_OK;
     std::vector<int> cell(hl + 1, hl + 1 + ms);   // in index order: a deterministic labelling
+    std::vector<int> cin(ms);                     // instance of each strong cell
+    for (int a = 0, q = 0; a < ms; a++) {
+        while (q + 1 < ninst && cell[a] >= inst[q + 1].off) q++;
+        cin[a] = q;
+    }
 
-    // strong neighbours of each strong cell: ordinals of the +x, +y, +z
-    // neighbours by one monotone sweep per direction (cell[] is sorted), the
-    // -x, -y, -z ones by symmetry
-    const long nx = d.nx, nxy = (long)d.nx * d.ny;
+    // strong neighbours of each strong cell, inside its instance: ordinals of
+    // the +x, +y, +z neighbours by one monotone sweep per direction (cell[]
+    // is sorted and a neighbour stays inside the instance), the -x, -y, -z
+    // ones by symmetry
     std::vector<int> nbr(6L * ms, -1);   // [a][-x,+x,-y,+y,-z,+z]
-    {
-        const long off[3] = {1, nx, nxy};
-        for (int f = 0; f < 3; f++) {
-            int p = 0;
-            for (int a = 0; a < ms; a++) {
-                const long s = cell[a];
-                const bool inside = f == 0 ? s % nx + 1 < d.nx : f == 1 ? (s / nx) % d.ny + 1 < d.ny
-                                                                        : s / nxy + 1 < d.nz;
-                if (!inside) continue;
-                const long t = s + off[f];
-                while (p < ms && cell[p] < t) p++;
-                if (p < ms && cell[p] == t) {
-                    nbr[6L * a + 2 * f + 1] = p;
-                    nbr[6L * p + 2 * f] = a;
-                }
+    std::vector<unsigned char> up(ms);   // bit f: the +f neighbour is inside the instance
+    for (int a = 0; a < ms; a++) {
+        const DefInst &in = inst[cin[a]];
+        const int l = cell[a] - in.off, i = l % in.nx, jk = l / in.nx, j = jk % in.ny, k = jk / in.ny;
+        up[a] = (unsigned char)((i + 1 < in.nx) | ((j + 1 < in.ny) << 1) | ((k + 1 < in.nz) << 2));
+    }
+    for (int f = 0; f < 3; f++) {
+        int p = 0;
+        for (int a = 0; a < ms; a++) {
+            if (!((up[a] >> f) & 1)) continue;
+            const DefInst &in = inst[cin[a]];
+            const long t = cell[a] + (f == 0 ? 1 : f == 1 ? (long)in.nx : (long)in.nx * in.ny);
+            while (p < ms && cell[p] < t) p++;
+            if (p < ms && cell[p] == t) {
+                nbr[6L * a + 2 * f + 1] = p;
+                nbr[6L * p + 2 * f] = a;
             }
         }
     }
@@ -363,13 +486,13 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
         if (root[a] == a && size[a] >= kMinCluster) roots.push_back(a);
     std::stable_sort(roots.begin(), roots.end(), [&](int x, int y) { return size[x] > size[y]; });
     if ((int)roots.size() > kMaxClusters) roots.resize(kMaxClusters);
-    std::sort(roots.begin(), roots.end());   // cluster order: by smallest member
+    std::sort(roots.begin(), roots.end());   // cluster order: by smallest member (so by instance)
     const int nc = (int)roots.size();
     if (nc == 0) return HMFEM_OK;
     std::vector<int> cid(ms, -1);
     for (int q = 0; q < nc; q++) cid[roots[q]] = q;
     for (int a = 0; a < ms; a++) cid[a] = cid[root[a]];
-    std::vector<int> ptr(nc + 1, 0), lcell, lcomp, lord;
+    std::vector<int> ptr(nc + 1, 0), lcell, lcomp, lord, linst;
     for (int a = 0; a < ms; a++)
         if (cid[a] >= 0) ptr[cid[a] + 1]++;
     for (int q = 0; q < nc; q++) ptr[q + 1] += ptr[q];
@@ -377,6 +500,7 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     lcell.resize(m);
     lcomp.resize(m);
     lord.resize(m);
+    linst.resize(m);
     {
         std::vector<int> fill(ptr.begin(), ptr.end() - 1);
         for (int a = 0; a < ms; a++)
@@ -385,8 +509,13 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
                 lcell[q] = cell[a];
                 lcomp[q] = cid[a];
                 lord[q] = a;
+                linst[q] = cin[a];
             }
     }
+    // clusters of each instance: [irange[q], irange[q+1])
+    std::vector<int> irange(ninst + 1, 0);
+    for (int q = 0; q < nc; q++) irange[cin[roots[q]] + 1]++;
+    for (int q = 0; q < ninst; q++) irange[q + 1] += irange[q];
 
     // pieces of at most kPiece cells per cluster, contiguous in the list
     std::vector<int> pbh(1, 0), cph(1, 0);
@@ -397,29 +526,38 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     const int np = (int)pbh.size() - 1;
     std::vector<int> pc(pbh);
     pc.insert(pc.end(), cph.begin(), cph.end());
+    pc.insert(pc.end(), irange.begin(), irange.end());
+    for (int q = 0; q < nc; q++) pc.push_back(cin[roots[q]]);   // instance of each cluster
 
     // device copies, coefficients of the listed cells
     D.cells = (int *)alloc(actx, sizeof(int) * m, 2);
-    D.comp = (int *)alloc(actx, sizeof(int) * m, 3);
+    D.comp = (int *)alloc(actx, sizeof(int) * 2 * (size_t)m, 3);   // cluster, then instance, of each cell
     D.pb = (int *)alloc(actx, sizeof(int) * pc.size(), 4);
     D.einv = (double *)alloc(actx, sizeof(double) * nc * nc, 5);
     const size_t per = (size_t)Deflation::kMaxRhs;
-    D.c = (double *)alloc(actx, sizeof(double) * per * (kMaxClusters + 2 * (size_t)np), 6);
+    D.c = (double *)alloc(actx, sizeof(double) * (per * (kMaxClusters + 2 * (size_t)np) + 2 * (size_t)ninst), 6);
     double *coef = (double *)alloc(actx, sizeof(double) * 7 * m, 7);
     if (!D.cells || !D.comp || !D.pb || !D.einv || !D.c || !coef) return HMFEM_ERR_NOMEM;
     D.cp = D.pb + np + 1;
+    D.irange = D.cp + nc + 1;
+    D.cinst = D.irange + ninst + 1;
     D.psr = D.c + per * kMaxClusters;
     D.psv = D.psr + per * np;
+    D.pext = D.psv + per * np;
+    int *dlinst = D.comp + m;
     HMFEM_CHECK(hipMemcpyAsync(D.cells, lcell.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipMemcpyAsync(D.comp, lcomp.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipMemcpyAsync(dlinst, linst.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipMemcpyAsync(D.pb, pc.data(), sizeof(int) * pc.size(), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_defl_gather, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, o, D.cells, m, coef);
+    hipLaunchKernelGGL(k_defl_gather, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, dinst,
+                       (const int *)dlinst, A, D.cells, m, coef);
     const double *w = (const double *)D.h_stage;   // 7 m <= 7 ub doubles: reserved above
     HMFEM_CHECK(hipMemcpyAsync(D.h_stage, coef, sizeof(double) * 7 * m, hipMemcpyDeviceToHost, st));
     HMFEM_CHECK(hipStreamSynchronize(st));
 
     // E = Z^T A Z in a fixed order: E_aa = sum (e + couplings leaving the
-    // cluster), E_ab = - couplings between clusters a and b
+    // cluster), E_ab = - couplings between clusters a and b (block diagonal
+    // over the instances)
     std::vector<double> E((size_t)nc * nc, 0.0);
     for (int q = 0; q < m; q++) {
         const int a = lcomp[q];
@@ -436,40 +574,142 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
         }
         E[(size_t)a * nc + a] += daa;
     }
-    // inverse via Cholesky E = L L^T (E is symmetric positive definite)
-    std::vector<double> L = E;
-    for (int j = 0; j < nc; j++) {
-        double djj = L[(size_t)j * nc + j];
-        for (int k = 0; k < j; k++) djj -= L[(size_t)j * nc + k] * L[(size_t)j * nc + k];
-        if (!(djj > 0)) return HMFEM_OK;   // not positive definite: leave the correction off
-        djj = sqrt(djj);
-        L[(size_t)j * nc + j] = djj;
-        for (int i = j + 1; i < nc; i++) {
-            double t = L[(size_t)i * nc + j];
-            for (int k = 0; k < j; k++) t -= L[(size_t)i * nc + k] * L[(size_t)j * nc + k];
-            L[(size_t)i * nc + j] = t / djj;
-        }
-    }
+    // inverse via Cholesky E = L L^T (E is symmetric positive definite),
+    // block by block: E has no coupling between clusters of different
+    // instances
     std::vector<double> inv((size_t)nc * nc, 0.0);
-    for (int col = 0; col < nc; col++) {   // solve L L^T x = e_col
-        std::vector<double> y(nc, 0.0);
-        for (int i = 0; i < nc; i++) {
-            double t = i == col ? 1.0 : 0.0;
-            for (int k = 0; k < i; k++) t -= L[(size_t)i * nc + k] * y[k];
-            y[i] = t / L[(size_t)i * nc + i];
+    for (int q = 0; q < ninst; q++) {
+        const int lo = irange[q], nb = irange[q + 1] - lo;
+        if (nb == 0) continue;
+        auto Eb = [&](int i, int j) { return E[(size_t)(lo + i) * nc + lo + j]; };
+        std::vector<double> L((size_t)nb * nb);
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++) L[(size_t)i * nb + j] = Eb(i, j);
+        for (int j = 0; j < nb; j++) {
+            double djj = L[(size_t)j * nb + j];
+            for (int k = 0; k < j; k++) djj -= L[(size_t)j * nb + k] * L[(size_t)j * nb + k];
+            if (!(djj > 0)) return HMFEM_OK;   // not positive definite: leave the correction off
+            djj = sqrt(djj);
+            L[(size_t)j * nb + j] = djj;
+            for (int i = j + 1; i < nb; i++) {
+                double t = L[(size_t)i * nb + j];
+                for (int k = 0; k < j; k++) t -= L[(size_t)i * nb + k] * L[(size_t)j * nb + k];
+                L[(size_t)i * nb + j] = t / djj;
+            }
         }
-        for (int i = nc - 1; i >= 0; i--) {
-            double t = y[i];
-            for (int k = i + 1; k < nc; k++) t -= L[(size_t)k * nc + i] * inv[(size_t)k * nc + col];
-            inv[(size_t)i * nc + col] = t / L[(size_t)i * nc + i];
+        std::vector<double> y(nb), xb((size_t)nb * nb);
+        for (int col = 0; col < nb; col++) {   // solve L L^T x = e_col
+            for (int i = 0; i < nb; i++) {
+                double t = i == col ? 1.0 : 0.0;
+                for (int k = 0; k < i; k++) t -= L[(size_t)i * nb + k] * y[k];
+                y[i] = t / L[(size_t)i * nb + i];
+            }
+            for (int i = nb - 1; i >= 0; i--) {
+                double t = y[i];
+                for (int k = i + 1; k < nb; k++) t -= L[(size_t)k * nb + i] * xb[(size_t)k * nb + col];
+                xb[(size_t)i * nb + col] = t / L[(size_t)i * nb + i];
+            }
         }
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++) inv[(size_t)(lo + i) * nc + lo + j] = xb[(size_t)i * nb + j];
     }
     HMFEM_CHECK(hipMemcpyAsync(D.einv, inv.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipStreamSynchronize(st));
     D.nc = nc;
     D.m = m;
     D.np = np;
+    D.ninst = ninst;
     return HMFEM_OK;
+}
+
+int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation &D,
+                    char *(*alloc)(void *, size_t, int), void *actx)
+{
+    D.nc = 0;
+    D.m = 0;
+    D.ninst = 0;
+    D.pending = false;
+    const long n = d.n();
+    if (n < 64 || !D.h_hist || !D.hist_ready) return HMFEM_OK;
+    OpArgs<double> o;
+    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
+    o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
+    // slot 0: the histogram, then the instance descriptor
+    char *h0 = alloc(actx, kHistBins * sizeof(unsigned) + 256 + sizeof(DefInst), 0);
+    if (!h0) return HMFEM_ERR_NOMEM;
+    unsigned *hist = (unsigned *)h0;
+    HMFEM_CHECK(hipMemsetAsync(hist, 0, kHistBins * sizeof(unsigned), st));
+    // a grid-stride loop over at most 512 blocks keeps the global bin updates
+    // few (2048 blocks: 35 us instead of 25 at 128^3)
+    const int G = std::min(grid_for(n, kBlock), 512);
+    hipLaunchKernelGGL(k_diag_hist, dim3(G), dim3(kBlock), 0, st, o, hist);
+    HMFEM_CHECK(hipMemcpyAsync(D.h_hist, hist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HMFEM_CHECK(hipEventRecord(D.hist_ready, st));
+    D.pending = true;
+    return HMFEM_OK;
+}
+
+int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ratio, Deflation &D,
+                     char *(*alloc)(void *, size_t, int), void *actx)
+{
+    if (!D.pending) return HMFEM_OK;
+    D.pending = false;
+    if (!(ratio > 1.0)) return HMFEM_OK;
+    const long n = d.n();
+    DefInst *dinst = (DefInst *)(alloc(actx, kHistBins * sizeof(unsigned) + 256 + sizeof(DefInst), 0) +
+                                 kHistBins * sizeof(unsigned));
+    std::vector<DefInst> inst(1, DefInst{d.nx, d.ny, d.nz, (int)n, 0, 0, INFINITY, 0});
+    HMFEM_CHECK(hipEventSynchronize(D.hist_ready));
+    return finish_clusters(st, inst, dinst, nullptr, (int)((n + kBlock - 1) / kBlock), A, D.h_hist, ratio, D,
+                           alloc, actx);
+}
+
+int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
+                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx)
+{
+    D.nc = 0;
+    D.m = 0;
+    D.ninst = 0;
+    D.pending = false;
+    if (ninst < 1 || !(ratio > 1.0)) return HMFEM_OK;
+    std::vector<DefInst> inst(ninst);
+    std::vector<int> bm, hm;
+    long off = 0;
+    for (int q = 0; q < ninst; q++) {
+        DefInst &in = inst[q];
+        in.nx = dims[3 * q]; in.ny = dims[3 * q + 1]; in.nz = dims[3 * q + 2];
+        in.n = in.nx * in.ny * in.nz;
+        in.off = (int)off;
+        in.blk0 = (int)bm.size();
+        in.hblk0 = (int)hm.size();
+        in.thr = INFINITY;
+        bm.insert(bm.end(), (in.n + kBlock - 1) / kBlock, q);
+        hm.insert(hm.end(), (in.n + kBlock * kHistCells - 1) / (kBlock * kHistCells), q);
+        off += in.n;
+    }
+    const int G = (int)bm.size(), GH = (int)hm.size();
+    // slot 0: per-instance histograms, the block maps, the instance descriptors
+    auto up256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t hb = up256(sizeof(unsigned) * kHistBins * (size_t)ninst), bb = up256(sizeof(int) * (size_t)(G + GH));
+    char *h0 = alloc(actx, hb + bb + sizeof(DefInst) * ninst, 0);
+    if (!h0) return HMFEM_ERR_NOMEM;
+    unsigned *hist = (unsigned *)h0;
+    int *bmap = (int *)(h0 + hb), *hmap = bmap + G;
+    DefInst *dinst = (DefInst *)(h0 + hb + bb);
+    bm.insert(bm.end(), hm.begin(), hm.end());
+    HMFEM_CHECK(hipMemsetAsync(hist, 0, sizeof(unsigned) * kHistBins * (size_t)ninst, st));
+    HMFEM_CHECK(hipMemcpyAsync(bmap, bm.data(), sizeof(int) * (G + GH), hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipMemcpyAsync(dinst, inst.data(), sizeof(DefInst) * ninst, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_diag_hist_inst, dim3(GH), dim3(kBlock), 0, st, (const DefInst *)dinst, (const int *)hmap, A,
+                       hist);
+    // the histograms go through the pinned staging, which finish_clusters
+    // reads before it reuses (and may regrow) it
+    if (stage_reserve(D, sizeof(unsigned) * kHistBins * (size_t)ninst)) return HMFEM_ERR_NOMEM;
+    HMFEM_CHECK(hipMemcpyAsync(D.h_stage, hist, sizeof(unsigned) * kHistBins * (size_t)ninst, hipMemcpyDeviceToHost,
+                               st));
+    HMFEM_CHECK(hipStreamSynchronize(st));
+    std::vector<unsigned> hh((const unsigned *)D.h_stage, (const unsigned *)D.h_stage + kHistBins * (size_t)ninst);
+    return finish_clusters(st, inst, dinst, bmap, G, A, hh.data(), ratio, D, alloc, actx);
 }
 
 template <int NR>
@@ -502,6 +742,13 @@ void deflation_scatter(hipStream_t st, const Deflation &D, float *z, const int *
 void deflation_scatter(hipStream_t st, const Deflation &D, double *z, const int *done)
 {
     defl_scatter<1, double>(st, D, z, done);
+}
+
+void deflation_sums_batch(hipStream_t st, const Deflation &D, const double *r, const double *v, const int *never)
+{
+    hipLaunchKernelGGL(k_defl_sums<1>, dim3(D.np), dim3(kBlock), 0, st, D.pb, D.cells, r, v, D.psr, D.psv, never);
+    hipLaunchKernelGGL(k_defl_apply_inst, dim3(1), dim3(kBlock), 0, st, D.nc, D.cp, D.einv, D.psr, D.psv, D.irange,
+                       D.cinst, D.ninst, D.c, D.pext);
 }
 
 int deflation_sums_block(hipStream_t st, const Deflation &D, int nr, const double *r, const double *v,

@@ -155,6 +155,10 @@ struct Deflation {
     double *einv = nullptr;                  // (Z^T A Z)^-1, nc x nc (symmetric)
     double *c = nullptr;                     // per-cluster correction [cluster][rhs]
     double *psr = nullptr, *psv = nullptr;   // per-piece sums of r and v [piece][rhs]
+    int ninst = 0;                           // grids of the setup (batch instances; 1 for a single grid)
+    int *irange = nullptr;                   // instance q owns clusters [irange[q], irange[q+1])
+    int *cinst = nullptr;                    // instance of each cluster
+    double *pext = nullptr;                  // batch: per-instance extra dot partials [instance][2]
     static constexpr int kHistBins = 2048;   // bins of floor(log2 diagonal) + 1100
     static constexpr int kMaxRhs = 8;        // interleaved right-hand sides of the per-cluster buffers
     unsigned *h_hist = nullptr;              // pinned host copy of the diagonal histogram (context-owned)
@@ -172,6 +176,12 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
                      char *(*alloc)(void *, size_t, int), void *actx);
 void deflation_sums(hipStream_t st, const Deflation &D, const double *r, const double *v, double *Pextra,
                     const int *done);
+// Batch: the instances (dims[3q..3q+2], concatenated in one index space) get
+// their own thresholds and clusters; per CG iteration the per-instance extra
+// dot partials go to D.pext. `never` points at a device int that stays 0.
+int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
+                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx);
+void deflation_sums_batch(hipStream_t st, const Deflation &D, const double *r, const double *v, const int *never);
 void deflation_scatter(hipStream_t st, const Deflation &D, float *z, const int *done);
 void deflation_scatter(hipStream_t st, const Deflation &D, double *z, const int *done);
 // The same for nr (2 or 4) interleaved right-hand sides ([cell][nr] vectors;
@@ -192,7 +202,8 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
 int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A0, const double *b, double *x,
                 const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *alloc_ctx, CgScalars *h_sc,
                 GraphCache *graph, std::vector<char> &stage, hipEvent_t ev[4], int *iters_out,
-                double *relres_out, double *done_ms_out, int *status_out, double *setup_ms, double *solve_ms);
+                double *relres_out, double *done_ms_out, int *status_out, double *setup_ms, double *solve_ms,
+                Deflation *defl = nullptr, char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr);
 int batch_grids(hipStream_t st, int ninst, const int *dims, const double *lengths, double mu, const int bc[6],
                 std::vector<char> &stage, void *d_grids, int *d_bmap, int *nblk_out);
 size_t batch_grid_bytes(int ninst);

@@ -85,7 +85,8 @@ class BlockStatus(ctypes.Structure):
 
 class BatchStatus(ctypes.Structure):
     _fields_ = [("status", c_int), ("max_iterations", c_int), ("t_assemble_ms", c_dbl), ("t_setup_ms", c_dbl),
-                ("t_solve_ms", c_dbl), ("t_flux_ms", c_dbl)]
+                ("t_solve_ms", c_dbl), ("t_flux_ms", c_dbl), ("deflation_clusters", c_int),
+                ("deflation_cells", c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

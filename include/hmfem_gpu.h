@@ -139,8 +139,8 @@ typedef struct {
                                 options it chose (default 0) */
     int deflate;             /* 1: add a coarse correction on each 7-point-connected cluster of at least
                                 8 cells whose diagonal exceeds deflate_ratio times the median diagonal
-                                (karst conduits); off when there are none (default 1; single and block
-                                solves) */
+                                (karst conduits); off when there are none (default 1; single, block and
+                                batched solves; in a batch each instance has its own threshold) */
     double deflate_ratio;    /* default 3000 */
     int precond;             /* HMFEM_PRECOND_AMG (default), or HMFEM_PRECOND_JACOBI: z = r / diag(A) in
                                 FP64 with no hierarchy, the plain FP64 PCG that the AMG is measured
@@ -217,6 +217,8 @@ typedef struct {
     int status;              /* HMFEM_OK if every instance converged */
     int max_iterations;      /* largest iteration count over the instances */
     double t_assemble_ms, t_setup_ms, t_solve_ms, t_flux_ms;
+    int deflation_clusters;  /* clusters of the conduit correction, all instances (0: none) */
+    int deflation_cells;
 } hmfem_batch_status;
 
 int hmfem_reservoir_batch_device(hmfem_context *ctx, int ninst, const int *dims, const double *lengths,

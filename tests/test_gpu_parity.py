@@ -339,6 +339,50 @@ def _expected_clusters(P, ratio):
     return int(keep.size), int(keep.sum())
 
 
+def test_batch_deflation(ctx, torch):
+    """The batch builds per-instance clusters (each instance its own median
+    and threshold): their total equals the independent labelling of every
+    instance's oracle diagonal; the karst instances need fewer iterations with
+    the correction; every instance still reaches the oracle's solution."""
+    specs = [((32, 32, 24), "karst", 21, {"conduits": 6}), ((20, 16, 12), "lognormal", 1, {"sigma": 3.5}),
+             ((24, 20, 16), "karst", 22, {"conduits": 4}), ((12, 10, 8), "homogeneous", 3, {}),
+             ((16, 16, 16), "lognormal", 6, {"sigma": 3.5, "corr": 4.0})]
+    probs = [make_problem(d, k, seed=sd, **kw) for d, k, sd, kw in specs]
+    ratio = S.default_options().deflate_ratio
+    exp = [_expected_clusters(P, ratio) for P in probs]
+    _say("batch deflation", exp)
+    K = torch.from_numpy(np.concatenate([P.Kx for P in probs])).to("cuda")
+    big = max(max(P.nx * P.ny, P.ny * P.nz, P.nx * P.nz) for P in probs)
+    bval = [torch.full((big,), 1.0, dtype=torch.float64, device="cuda")] + \
+           [torch.zeros(big, dtype=torch.float64, device="cuda") for _ in range(5)]
+    ntot = sum(P.n for P in probs)
+    p = torch.zeros(ntot, dtype=torch.float64, device="cuda")
+    dims = [(P.nx, P.ny, P.nz) for P in probs]
+    its = {}
+    for dfl in (0, 1):
+        p.zero_()
+        st, per = ctx.reservoir_batch(dims, K, K, K, bval, p, None, S.default_options(rtol=1e-8, deflate=dfl))
+        assert st.status == S.OK, (st.as_dict(), per)
+        its[dfl] = per["iterations"].copy()
+        if dfl:
+            assert (st.deflation_clusters, st.deflation_cells) == (sum(e[0] for e in exp), sum(e[1] for e in exp))
+            assert st.deflation_clusters >= 2
+        else:
+            assert st.deflation_clusters == 0
+    for q, (d, k, sd, kw) in enumerate(specs):
+        if k == "karst":
+            assert its[1][q] < its[0][q], (q, its)
+    p.zero_()
+    st, per = ctx.reservoir_batch(dims, K, K, K, bval, p, None, S.default_options(rtol=1e-12))
+    assert st.status == S.OK
+    xb = p.cpu().numpy()
+    off = 0
+    for P in probs:
+        xo, _, _, _ = oracle_solution(P, eps=1e-13)
+        assert rel_l2(xb[off:off + P.n], xo) < PRESSURE_TOL, (P.nx, P.ny, P.nz)
+        off += P.n
+
+
 @pytest.mark.parametrize("kind,dims,seed", [("karst", (64, 64, 48), 21), ("karst", (48, 40, 36), 3),
                                             ("lognormal", (48, 48, 40), 5)])
 def test_deflation_clusters(kind, dims, seed, ctx, torch):
