@@ -836,6 +836,7 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     if (defl) {
         defl->nc = 0;
         defl->pending = false;
+        defl->gen++;
         if (opt.deflate && dalloc) {
             int rc = deflation_setup_batch(st, ninst, dims, A0, opt.deflate_ratio, *defl, dalloc, dctx);
             if (rc) return rc;

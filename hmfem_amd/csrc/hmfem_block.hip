@@ -854,6 +854,7 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
     if (defl && defl_setup) {   // the conduit correction of this operator (kept for later groups)
         defl->nc = 0;
         defl->pending = false;
+        defl->gen++;
         if (opt.deflate && dalloc) {
             rc = deflation_begin(st, d, A, *defl, dalloc, dctx);
             if (rc) return rc;

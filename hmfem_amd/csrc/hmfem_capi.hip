@@ -30,6 +30,7 @@ struct hmfem_context {
     std::vector<char> stage;   // host staging of small uploads (tail op list)
     GraphCache graph;          // captured CG iteration of the last solve
     Deflation defl;            // cluster correction of the last solve
+    SetupCache setup;          // hierarchy of the last single solve (opt.reuse_setup)
     long lds_limit = 65536;    // dynamic LDS a workgroup may use on this device
     // options chosen by the autotuner, per grid shape and precision
     std::map<std::vector<int>, hmfem_options> tuned;
@@ -87,6 +88,7 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->deflate = 1;
     o->deflate_ratio = 3000.0;
     o->precond = HMFEM_PRECOND_AMG;
+    o->reuse_setup = 0;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)
@@ -211,7 +213,7 @@ static int solve_on_stencil(hmfem_context *c, Dims d, const Stencil<double> &A, 
     hmfem_status tmp{};
     hmfem_status *out = st ? st : &tmp;
     return solve_structured(c->st, d, A, b, x, opt, arena, hb, B, c->h_sc, info, out, c->ev, sink_of(c), c->stage,
-                            &c->graph, c->lds_limit, &c->defl, ctx_alloc_defl, c);
+                            &c->graph, c->lds_limit, &c->defl, ctx_alloc_defl, c, &c->setup);
 }
 
 // ------------------------------------------------------------ device API

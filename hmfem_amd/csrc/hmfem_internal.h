@@ -79,6 +79,15 @@ struct GraphCache {
     std::vector<char> key;
 };
 
+// The last single-solve setup of a context (hierarchy bytes and the key it
+// was built for), for opt.reuse_setup.
+struct SetupCache {
+    bool valid = false;
+    std::vector<char> key;
+    std::vector<char> hier;   // the Hierarchy struct (plain data; its buffers live in the arena)
+    unsigned dgen = 0;        // Deflation::gen at the build
+};
+
 // Grid launch helpers with an XCD-aware block order: logical block b of a
 // launch with G blocks (G a multiple of 8) is placed so that each XCD gets a
 // contiguous range of the domain and keeps its neighbours in its own L2.
@@ -166,6 +175,7 @@ struct Deflation {
     void *h_stage = nullptr;                 // pinned staging of the cell list and coefficients
     size_t h_stage_bytes = 0;
     bool pending = false;                    // deflation_begin queued a histogram
+    unsigned gen = 0;                        // bumped by every setup (single, block or batch)
 };
 // deflation_begin queues the diagonal histogram and its copy to the host;
 // deflation_finish (after other work has been queued) decides and builds the
@@ -196,7 +206,8 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink,
                      std::vector<char> &stage, GraphCache *graph, long lds_limit, Deflation *defl = nullptr,
-                     char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr);
+                     char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr,
+                     SetupCache *cache = nullptr);
 
 // batched solve (hmfem_batch.hip) and batch layout / kernels (hmfem_assemble.hip)
 int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A0, const double *b, double *x,

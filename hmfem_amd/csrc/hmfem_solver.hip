@@ -34,6 +34,7 @@
 
 #include <algorithm>
 
+#include <type_traits>
 #include <vector>
 
 namespace hmfem {
@@ -1700,7 +1701,7 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink, std::vector<char> &stage,
                      GraphCache *graph, long lds_limit, Deflation *defl, char *(*dalloc)(void *, size_t, int),
-                     void *dctx)
+                     void *dctx, SetupCache *cache)
 {
     Prof prof, *P = nullptr;
     if (sink && sink->ev && sink->cap >= 2) {
@@ -1714,7 +1715,8 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
     HMFEM_CHECK(hipEventRecord(ev[0], st));
     int rc = HMFEM_OK;
     if (opt.precond == HMFEM_PRECOND_JACOBI) {   // plain FP64 PCG: no hierarchy, no correction
-        if (defl) { defl->nc = 0; defl->pending = false; }
+        if (defl) { defl->nc = 0; defl->pending = false; defl->gen++; }
+        if (cache) cache->valid = false;   // z below overwrites the start of the arena
         double *jz = ar.take<double>(d.n());
         if (!jz) return HMFEM_ERR_NOMEM;
         HMFEM_CHECK(hipEventRecord(ev[1], st));
@@ -1742,19 +1744,43 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         }
         return HMFEM_OK;
     }
-    if (defl) {   // the histogram first: its copy to the host overlaps the hierarchy setup
-        defl->nc = 0;
-        defl->pending = false;
-        if (opt.deflate && dalloc) {
-            rc = deflation_begin(st, d, A, *defl, dalloc, dctx);
+    // what the setup depends on, for opt.reuse_setup
+    std::vector<char> key;
+    {
+        auto put = [&](const void *p, size_t nb) { key.insert(key.end(), (const char *)p, (const char *)p + nb); };
+        const int ints[] = {opt.precision, opt.coarse_max, opt.tail_max, opt.tail_gamma, opt.tail_lds, opt.nu,
+                            opt.smoother, opt.deflate};
+        put(&d, sizeof(d)); put(&arena, sizeof(arena)); put(&arena_bytes, sizeof(arena_bytes));
+        put(ints, sizeof(ints)); put(&opt.deflate_ratio, sizeof(double)); put(&lds_limit, sizeof(lds_limit));
+    }
+    static_assert(std::is_trivially_copyable<Hierarchy>::value, "the hierarchy is cached as bytes");
+    const bool reuse = opt.reuse_setup && cache && cache->valid && cache->key == key &&
+                       cache->hier.size() == sizeof(Hierarchy) && (!defl || defl->gen == cache->dgen);
+    if (reuse) {
+        memcpy((void *)&H, cache->hier.data(), sizeof(Hierarchy));
+    } else {
+        if (cache) cache->valid = false;
+        if (defl) {   // the histogram first: its copy to the host overlaps the hierarchy setup
+            defl->nc = 0;
+            defl->pending = false;
+            defl->gen++;
+            if (opt.deflate && dalloc) {
+                rc = deflation_begin(st, d, A, *defl, dalloc, dctx);
+                if (rc) return rc;
+            }
+        }
+        rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit);
+        if (rc) return rc;
+        if (defl && defl->pending) {
+            rc = deflation_finish(st, d, A, opt.deflate_ratio, *defl, dalloc, dctx);
             if (rc) return rc;
         }
-    }
-    rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit);
-    if (rc) return rc;
-    if (defl && defl->pending) {
-        rc = deflation_finish(st, d, A, opt.deflate_ratio, *defl, dalloc, dctx);
-        if (rc) return rc;
+        if (cache) {
+            cache->key = key;
+            cache->hier.assign((const char *)&H, (const char *)&H + sizeof(Hierarchy));
+            cache->dgen = defl ? defl->gen : 0;
+            cache->valid = true;
+        }
     }
     HMFEM_CHECK(hipEventRecord(ev[1], st));
     SolveResult res{};

@@ -145,6 +145,12 @@ typedef struct {
     int precond;             /* HMFEM_PRECOND_AMG (default), or HMFEM_PRECOND_JACOBI: z = r / diag(A) in
                                 FP64 with no hierarchy, the plain FP64 PCG that the AMG is measured
                                 against (single solves; precision, smoother and deflate are ignored) */
+    int reuse_setup;         /* 1: if the previous single solve on this context built its hierarchy (and
+                                conduit correction) for the same grid and setup options, reuse them
+                                instead of rebuilding: the preconditioner then follows the earlier
+                                operator while the CG solves the current one (for repeated solves with
+                                small operator changes, e.g. the fixed-point passes of reservoir());
+                                default 0 */
 } hmfem_options;
 
 typedef struct {

@@ -405,6 +405,51 @@ def test_deflation_clusters(kind, dims, seed, ctx, torch):
     assert st.status == S.OK and st.deflation_clusters == 0 and st.deflation_cells == 0
 
 
+def test_reuse_setup(ctx, torch):
+    """opt.reuse_setup: a second solve of a changed operator on the same grid
+    reuses the first hierarchy and conduit correction and still reaches the
+    oracle's solution of the changed problem; a different grid, or a plain
+    PCG solve in between, makes the next solve rebuild."""
+    P1 = make_problem((40, 36, 28), "karst", seed=31, conduits=5)
+    P2 = make_problem((40, 36, 28), "karst", seed=31, conduits=5)
+    P2.Kx[: P2.n // 3] *= 3.0   # a changed permeability in a third of the cells
+    P2.Ky[:] = P2.Kx
+    P2.Kz[:] = P2.Kx
+    _say("reuse setup")
+    p = torch.zeros(P1.n, dtype=torch.float64, device="cuda")
+    D1 = Dev(P1, torch)
+    s1 = ctx.reservoir(D1.g, D1.K, D1.K, D1.K, D1.bval, p, None, S.default_options(rtol=1e-12))
+    assert s1.status == S.OK and s1.deflation_clusters >= 1
+    D2 = Dev(P2, torch)
+    p.zero_()
+    s2 = ctx.reservoir(D2.g, D2.K, D2.K, D2.K, D2.bval, p, None, S.default_options(rtol=1e-12, reuse_setup=1))
+    assert s2.status == S.OK
+    assert (s2.levels, s2.deflation_clusters, s2.deflation_cells) == (s1.levels, s1.deflation_clusters,
+                                                                       s1.deflation_cells)
+    xo, operm, oind, ob = oracle_solution(P2, eps=1e-12)
+    x = p.cpu().numpy()
+    assert rel_l2(x, xo) < PRESSURE_TOL
+    # the earlier operator's preconditioner needs more iterations, so the
+    # recursively updated residual drifts a little further from the true one
+    assert true_residual(operm, oind, ob, x) < 5e-11
+    # a different grid rebuilds
+    P3 = make_problem((24, 20, 16), "lognormal", seed=2)
+    D3 = Dev(P3, torch)
+    p3 = torch.zeros(P3.n, dtype=torch.float64, device="cuda")
+    s3 = ctx.reservoir(D3.g, D3.K, D3.K, D3.K, D3.bval, p3, None, S.default_options(rtol=1e-12, reuse_setup=1))
+    assert s3.status == S.OK
+    xo3, _, _, _ = oracle_solution(P3, eps=1e-12)
+    assert rel_l2(p3.cpu().numpy(), xo3) < PRESSURE_TOL
+    # a plain PCG solve in between, then reuse on the first grid: rebuilt, correct
+    oj = S.default_options(rtol=1e-10, precision=S.PREC_FP64)
+    oj.precond = S.PRECOND_JACOBI
+    p.zero_()
+    ctx.reservoir(D1.g, D1.K, D1.K, D1.K, D1.bval, p, None, oj)
+    p.zero_()
+    s4 = ctx.reservoir(D2.g, D2.K, D2.K, D2.K, D2.bval, p, None, S.default_options(rtol=1e-12, reuse_setup=1))
+    assert s4.status == S.OK and rel_l2(p.cpu().numpy(), xo) < PRESSURE_TOL
+
+
 def test_plain_jacobi_pcg(ctx, torch):
     """The plain FP64 PCG (diagonal preconditioner, no hierarchy): the oracle's
     solution at 1e-12, with many more iterations than the AMG preconditioner."""
