@@ -1,0 +1,192 @@
+// Output writers of the reservoir step: pressure_reservoir.dat,
+// flux_reservoir.dat (posprocessing_output.c:200-296) and campos.vts
+// (vtk.c:8-76), byte for byte as the reference writes them, into the
+// directory named by the global `dirname` (main.c:343 sets it from argv; an
+// executable that defines `dirname` is read through ELF interposition, as
+// `boundary`), or by hmfem_set_output_dir() in hosts without that global.
+//
+// The reference formats every line with fprintf in loops whose innermost
+// index is k (stride nx*ny in memory). Here the outermost loop's range is cut
+// into slabs that threads format with snprintf (the same conversions, so the
+// same bytes) into their own buffers, written in order with one fwrite each.
+#include "hmfem_gpu.h"
+
+#include <dlfcn.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <thread>
+#include <vector>
+
+extern "C" {
+const char *dirname = nullptr;
+}
+
+namespace {
+
+const char *g_outdir = nullptr;
+
+// The output directory: hmfem_set_output_dir's, else the host's `dirname`.
+// In a process without its own `dirname` variable the name binds to the C
+// library's dirname() function (which precedes this library in the lookup
+// order); that one is recognised by its object and ignored.
+const char *out_dir()
+{
+    if (g_outdir) return g_outdir;
+    Dl_info a{}, b{};
+    if (dladdr((const void *)&dirname, &a) && dladdr((const void *)&printf, &b) && a.dli_fbase == b.dli_fbase)
+        return ".";
+    return dirname ? dirname : ".";
+}
+
+// Line formatting helpers: append one snprintf conversion to a string.
+template <typename... A>
+void add(std::string &out, const char *fmt, A... a)
+{
+    char line[256];
+    const int k = snprintf(line, sizeof(line), fmt, a...);
+    out.append(line, k > 0 ? (size_t)k : 0);
+}
+
+// Run body(lo, hi, out) over [0, count) in contiguous slabs on up to 16
+// threads and return the slabs' text in order.
+template <typename F>
+std::vector<std::string> slabs(int count, F body)
+{
+    int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    nt = std::max(1, std::min(nt, count));
+    std::vector<std::string> out(nt);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++) {
+        const int lo = (int)((long)count * t / nt), hi = (int)((long)count * (t + 1) / nt);
+        th.emplace_back([&, t, lo, hi] { body(lo, hi, out[t]); });
+    }
+    for (auto &x : th) x.join();
+    return out;
+}
+
+FILE *open_out(const char *name, char *path, size_t cap)
+{
+    snprintf(path, cap, "%s/%s", out_dir(), name);
+    return fopen(path, "w");
+}
+
+void write_all(FILE *f, const std::vector<std::string> &parts)
+{
+    for (const auto &p : parts) fwrite(p.data(), 1, p.size(), f);
+}
+
+inline int pidx(int i, int j, int k, const Dim &N) { return i + j * N.x + k * N.x * N.y; }
+
+} // namespace
+
+extern "C" void hmfem_set_output_dir(const char *dir) { g_outdir = dir; }
+
+extern "C" void save_pressure(double *x, Dim N, grid h)
+{
+    char path[2048];
+    FILE *f = open_out("pressure_reservoir.dat", path, sizeof(path));
+    if (!f) {
+        printf("Unable to open file %s. Terminating run... \n", path);
+        exit(0);
+    }
+    auto parts = slabs(N.x, [&](int lo, int hi, std::string &o) {
+        for (int i = lo; i < hi; i++) {
+            for (int j = 0; j < N.y; j++) {
+                for (int k = 0; k < N.z; k++)
+                    add(o, "%.6lf %.6lf %.6lf %1.9e\n", h.x * (i + 0.5), h.y * (j + 0.5), h.z * (k + 0.5),
+                        x[pidx(i, j, k, N)]);
+                o.append(" \n");
+            }
+            o.append(" \n");
+        }
+    });
+    write_all(f, parts);
+    fclose(f);
+}
+
+extern "C" void save_flux(ELE *v, Dim N, grid h)
+{
+    char path[2048];
+    FILE *f = open_out("flux_reservoir.dat", path, sizeof(path));
+    if (!f) {
+        printf("Unable to open file %s. Terminating run... \n", path);
+        exit(0);
+    }
+    auto parts = slabs(N.x, [&](int lo, int hi, std::string &o) {
+        for (int i = lo; i < hi; i++)
+            for (int j = 0; j < N.y; j++)
+                for (int k = 0; k < N.z; k++) {
+                    const ELE &c = v[pidx(i, j, k, N)];
+                    add(o, "%.6lf %.6lf %.6lf %1.9e %1.9e %1.9e \n", h.x * (i + 0.5), h.y * (j + 0.5),
+                        h.z * (k + 0.5), 0.5 * (c.r - c.l), 0.5 * (c.u - c.d), 0.5 * (c.t - c.b));
+                }
+    });
+    write_all(f, parts);
+    fclose(f);
+}
+
+extern "C" int vts_StructuredGrid_file(ELE *v, double *x, Dim N, grid h)
+{
+    char path[2048];
+    FILE *f = open_out("campos.vts", path, sizeof(path));
+    if (!f) {
+        printf("Unable to open file %s.\n", path);
+        return -1;   // the reference goes on writing to the null stream
+    }
+    std::string head;
+    head.append("<?xml version=\"1.0\"?>\n");
+    head.append("<VTKFile type=\"StructuredGrid\" version=\"0.1\">\n");
+    add(head, "<StructuredGrid WholeExtent=\"0 %d 0 %d 0 %d\">\n", N.x - 1, N.y - 1, N.z - 1);
+    add(head, "<Piece Extent=\"0 %d 0 %d 0 %d\">\n", N.x - 1, N.y - 1, N.z - 1);
+    head.append("<PointData  Vectors=\"Velocity\" Scalars=\"pressure\">\n");
+    head.append("<DataArray type=\"Float32\" Name=\"Velocity\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+    fwrite(head.data(), 1, head.size(), f);
+    // k outermost, i innermost: index order
+    write_all(f, slabs(N.z, [&](int lo, int hi, std::string &o) {
+        for (int k = lo; k < hi; k++)
+            for (int j = 0; j < N.y; j++)
+                for (int i = 0; i < N.x; i++) {
+                    const ELE &c = v[pidx(i, j, k, N)];
+                    add(o, "%1.8e %1.8e %1.8e\n", (0.5) * (c.r - c.l), (0.5) * (c.u - c.d), (0.5) * (c.t - c.b));
+                }
+    }));
+    std::string mid;
+    mid.append("</DataArray>\n");
+    mid.append("<DataArray type=\"Float32\" Name=\"pressure\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+    fwrite(mid.data(), 1, mid.size(), f);
+    write_all(f, slabs(N.z, [&](int lo, int hi, std::string &o) {
+        for (int k = lo; k < hi; k++)
+            for (int j = 0; j < N.y; j++)
+                for (int i = 0; i < N.x; i++) add(o, "%1.8e\n", x[pidx(i, j, k, N)]);
+    }));
+    std::string pts;
+    pts.append("</DataArray>\n");
+    pts.append("</PointData>\n");
+    pts.append("<CellData>\n");
+    pts.append("</CellData>\n");
+    pts.append("<Points>\n");
+    pts.append("<DataArray type=\"Float32\" NumberOfComponents=\"3\">\n");
+    fwrite(pts.data(), 1, pts.size(), f);
+    write_all(f, slabs(N.z, [&](int lo, int hi, std::string &o) {
+        for (int k = lo; k < hi; k++) {
+            const double z_pos = h.z * (k + 0.5);
+            for (int j = 0; j < N.y; j++) {
+                const double y_pos = h.y * (j + 0.5);
+                for (int i = 0; i < N.x; i++) add(o, "%.4f %.4f %.4f \n", h.x * (i + 0.5), y_pos, z_pos);
+            }
+        }
+    }));
+    std::string tail;
+    tail.append("</DataArray>\n");
+    tail.append("</Points>\n");
+    tail.append("</Piece>\n");
+    tail.append("</StructuredGrid>\n");
+    tail.append("</VTKFile>\n");
+    fwrite(tail.data(), 1, tail.size(), f);
+    fclose(f);
+    return 0;
+}

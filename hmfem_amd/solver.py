@@ -12,6 +12,7 @@ There is no CPU fallback: every call goes through the HIP library and
 raises if it is not built or the GPU call fails.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -135,6 +136,13 @@ def lib():
         L.hmfem_autotune_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, P(Options), P(c_dbl)]
         L.hmfem_kernel_bytes_per_cell.argtypes = [c_int, c_int]
         L.hmfem_kernel_bytes_per_cell.restype = c_dbl
+        L.save_pressure.argtypes = [c_vp, Dim, GridH]
+        L.save_pressure.restype = None
+        L.save_flux.argtypes = [c_vp, Dim, GridH]
+        L.save_flux.restype = None
+        L.vts_StructuredGrid_file.argtypes = [c_vp, c_vp, Dim, GridH]
+        L.hmfem_set_output_dir.argtypes = [ctypes.c_char_p]
+        L.hmfem_set_output_dir.restype = None
         _typed = True
     return L
 
@@ -144,7 +152,8 @@ EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", 
             "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device",
             "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
             "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device",
-            "hmfem_reservoir_block_device", "hmfem_autotune_device", "iter_amg", "boundary")
+            "hmfem_reservoir_block_device", "hmfem_autotune_device", "save_pressure", "save_flux",
+            "vts_StructuredGrid_file", "hmfem_set_output_dir", "iter_amg", "boundary", "dirname")
 
 
 def _p(a):
@@ -227,6 +236,31 @@ def divvel_pontual(v, h, N):
 
 def prod_mat_vec(perm, indcol, x, res, N):
     lib().prod_mat_vec(_p(perm), _p(indcol), _p(x), _p(res), N)
+
+
+_dirname_keep = None
+
+
+def set_dirname(path):
+    """The output directory of the writers (hmfem_set_output_dir)."""
+    global _dirname_keep
+    _dirname_keep = ctypes.create_string_buffer(os.fsencode(path))
+    lib().hmfem_set_output_dir(_dirname_keep)
+
+
+def save_pressure(x, N, h):
+    """pressure_reservoir.dat in the `dirname` directory (posprocessing_output.c:200)."""
+    lib().save_pressure(_p(np.ascontiguousarray(x, np.float64)), N, GridH(*h))
+
+
+def save_flux(v, N, h):
+    """flux_reservoir.dat (posprocessing_output.c:236); v: ELE array (host)."""
+    lib().save_flux(_p(v), N, GridH(*h))
+
+
+def vts_StructuredGrid_file(v, x, N, h):
+    """campos.vts (vtk.c:8); returns 0, or -1 if the file cannot be opened."""
+    return lib().vts_StructuredGrid_file(_p(v), _p(np.ascontiguousarray(x, np.float64)), N, GridH(*h))
 
 
 # ----------------------------------------------------------------- device API

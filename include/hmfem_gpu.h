@@ -81,6 +81,24 @@ void divvel_pontual(ELE *v, grid h, Dim N);
 /* solve_linearsystem_AMG.c:160 prod_mat_vec(); functions.h:56. res += A x. */
 void prod_mat_vec(ELE *perm, IND *indcol, double *x, double *res, Dim N);
 
+/* Output writers, byte for byte the reference's files, into the directory
+   `dirname` (variables.h; main.c:343 sets it from argv; default "."). Host
+   buffers; the lines are formatted on several threads. */
+extern const char *dirname;
+/* No counterpart in the reference: the output directory for hosts that do
+   not define `dirname` (it takes precedence; the string must stay alive). */
+void hmfem_set_output_dir(const char *dir);
+/* posprocessing_output.c:200 save_pressure(); functions.h:67. Writes
+   pressure_reservoir.dat; exits like the reference if it cannot open it. */
+void save_pressure(double *x, Dim N, grid h);
+/* posprocessing_output.c:236 save_flux(); functions.h:68. Writes
+   flux_reservoir.dat (cell-centre velocities 0.5(r-l), 0.5(u-d), 0.5(t-b)). */
+void save_flux(ELE *v, Dim N, grid h);
+/* vtk.c:8 vts_StructuredGrid_file(). Writes campos.vts (velocity, pressure,
+   cell centres). Returns -1 if it cannot open the file (the reference
+   carries on writing to a null stream). */
+int vts_StructuredGrid_file(ELE *v, double *x, Dim N, grid h);
+
 /* ---------------------------------------------------------------- layer 2 */
 
 enum {

@@ -18,6 +18,10 @@
  *
  * With a third argument "time" it only times one reservoir() step (assembly,
  * solve_linsys at the reference's 1e-8, fluxos) and writes <prefix>.json.
+ * With "write <dir>" it also runs the reference's output writers on the tight
+ * pressure and its fluxes: save_pressure(), save_flux()
+ * (posprocessing_output.c:200-296) and vts_StructuredGrid_file() (vtk.c:8-76)
+ * write pressure_reservoir.dat, flux_reservoir.dat and campos.vts into <dir>.
  *
  * Input file layout (all little-endian):
  *   int32  nx, ny, nz
@@ -269,6 +273,12 @@ int main(int argc, char **argv)
         if (fabs(dv) < divmin) divmin = fabs(dv);
     }
 
+    if (argc > 4 && strcmp(argv[3], "write") == 0) {
+        dirname = argv[4];
+        save_pressure(p12, N, h);
+        save_flux(vv, N, h);
+        vts_StructuredGrid_file(vv, p12, N, h);
+    }
     const char *pre = argv[2];
     {
         char name[2048];

@@ -5,7 +5,12 @@ and flux sources compiled in place by ``make -C oracle ref``) on small seeded
 problems and stores inputs and outputs as compressed .npz files. Only this
 script runs the reference build; tests read the .npz files.
 
-Usage:  make -C oracle ref && python tests/golden/make_golden.py
+Usage:  make -C oracle ref && python tests/golden/make_golden.py [writers]
+
+With `writers` it only runs the cases marked `writers`, with the reference's
+output writers (save_pressure, save_flux, vts_StructuredGrid_file) on the
+tight pressure and its fluxes, and stores the three files' bytes (plus the
+p12 and v they were written from) in writers/<name>.npz.
 """
 import json
 import os
@@ -35,13 +40,13 @@ def case_list():
     cases.append(dict(name="lognormal_8", dims=(8, 8, 8), K=("lognormal", 2.0, 2.0, 11), **X_DIRICHLET))
     cases.append(dict(name="lognormal_24", dims=(24, 24, 24), K=("lognormal", 2.0, 4.0, 12), **X_DIRICHLET, full=False))
     cases.append(dict(name="karst_12x10x8_mixed", dims=(12, 10, 8), K=("karst", 3, 13),
-                      bc_type=(0, 0, 1, 0, 1, 1), bc_val=(2.0, 0.5, 1e-14, 0.25, -2e-14, 3e-14)))
+                      bc_type=(0, 0, 1, 0, 1, 1), bc_val=(2.0, 0.5, 1e-14, 0.25, -2e-14, 3e-14), writers=True))
     cases.append(dict(name="karst_24", dims=(24, 24, 24), K=("karst", 4, 14), **X_DIRICHLET, full=False))
     cases.append(dict(name="allneumann_6x5x4", dims=(6, 5, 4), K=("lognormal", 1.0, 2.0, 15),
                       bc_type=(1, 1, 1, 1, 1, 1), bc_val=(1e-14, 1e-14, 0.0, 0.0, 0.0, 0.0)))
     cases.append(dict(name="sources_10x8x6", dims=(10, 8, 6), K=("lognormal", 1.5, 2.0, 16),
                       bc_type=(0, 1, 1, 0, 1, 0), bc_val=(1.0, 0.0, 0.0, 0.3, 0.0, 0.7), sources=17,
-                      visc=1e-3))
+                      visc=1e-3, writers=True))
     return cases
 
 
@@ -58,7 +63,7 @@ def make_K(dims, spec):
     raise ValueError(spec)
 
 
-def run_case(c, tmp):
+def run_case(c, tmp, writers=False):
     nx, ny, nz = c["dims"]
     n = nx * ny * nz
     K = make_K(c["dims"], c["K"])
@@ -84,6 +89,18 @@ def run_case(c, tmp):
             for a in srcs:
                 f.write(np.ascontiguousarray(a, "<f8").tobytes())
     pre = os.path.join(tmp, c["name"])
+    if writers:
+        wdir = os.path.join(tmp, c["name"] + "_out")
+        os.makedirs(wdir, exist_ok=True)
+        subprocess.run([REF, inp, pre, "write", wdir], check=True, stdout=subprocess.DEVNULL)
+        rd = lambda f: np.frombuffer(open(os.path.join(wdir, f), "rb").read(), np.uint8)
+        np.savez_compressed(os.path.join(HERE, "writers", c["name"] + ".npz"),
+                            dims=np.array(c["dims"], np.int32), p12=np.fromfile(pre + ".p12", "<f8"),
+                            v=np.fromfile(pre + ".v", "<f8").reshape(n, 7),
+                            pressure_reservoir=rd("pressure_reservoir.dat"), flux_reservoir=rd("flux_reservoir.dat"),
+                            campos=rd("campos.vts"))
+        print(c["name"], "writers")
+        return
     subprocess.run([REF, inp, pre], check=True, stdout=subprocess.DEVNULL)
     meta = json.load(open(pre + ".json"))
     raw = np.fromfile(pre + ".A", dtype="<i4", count=2)
@@ -113,9 +130,12 @@ def run_case(c, tmp):
 def main():
     if not os.path.exists(REF):
         sys.exit("missing %s: run `make -C oracle ref` first" % REF)
+    writers = len(sys.argv) > 1 and sys.argv[1] == "writers"
     with tempfile.TemporaryDirectory() as tmp:
         for c in case_list():
-            run_case(c, tmp)
+            if writers and not c.get("writers"):
+                continue
+            run_case(c, tmp, writers)
 
 
 if __name__ == "__main__":
