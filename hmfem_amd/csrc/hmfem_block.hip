@@ -351,7 +351,7 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
     const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy, ntz = (nz + TZ - 1) / TZ;
     const int ntiles = ntx * nty * ntz;
     const int lb = xcd_block();
-    const int tbeg = lb * tpb, tend = min(ntiles, tbeg + tpb);
+    (void)tpb;   // tiles lb, lb + G, ...: neighbouring tiles are worked on at the same time (L2 reuse of rings)
     const int tx = threadIdx.x % kGsTx, ty = (threadIdx.x / kGsTx) % kGsTy, tz = threadIdx.x / (kGsTx * kGsTy);
     const int kbase = tz * CZ;   // first tile plane of this thread
     double acc[MODE == 2 ? 2 * NR : 1];
@@ -368,7 +368,7 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
         for (int r = 0; r < NR; r++) val[r] = u[s * NR + r] + ocf * uc[I * NR + r];
     };
 
-    for (int t = tbeg; t < tend; t++) {
+    for (int t = lb; t < ntiles; t += gridDim.x) {
         const int ti = t % ntx, tj = (t / ntx) % nty, tk = t / (ntx * nty);
         const int i0 = ti * kGsTx, j0 = tj * kGsTy, k0 = tk * TZ;
         const int i = i0 + tx, j = j0 + ty;

@@ -325,9 +325,13 @@ k_gs_tile(OpArgs<T> o, int sweeps, const FT *__restrict__ f, const T *__restrict
     if (*done) return;
     const int ntiles = gs_tiles(Dims{o.nx, o.ny, o.nz});
     const int lb = xcd_block();
-    const int tbeg = lb * tpb, tend = min(ntiles, tbeg + tpb);
     double acc[2] = {0.0, 0.0};
-    for (int t = tbeg; t < tend; t++)
+    // tiles lb, lb + G, lb + 2G, ...: the blocks work on G consecutive tiles
+    // at a time, so a tile's ring is read while its neighbours' blocks have
+    // the same lines in L2 (contiguous chunks of tpb tiles per block had 1.3x
+    // the algorithmic HBM traffic at 256^3)
+    (void)tpb;
+    for (int t = lb; t < ntiles; t += gridDim.x)
         gs_tile_body<T, FT, MODE, H16>(o, sweeps, f, u, uo, sx, sy, sz, cx, cy, uc, ocf, v, t, acc, h);
     if (MODE == 2) {
         block_sum_n<2, kGsThreads>(acc);
