@@ -1193,6 +1193,9 @@ static void plan_levels(Dims fine, int coarse_max, Level *lev, int &nlev)
         L.d = d;
         nlev++;
         if (d.n() <= coarse_max || nlev == kMaxLevels) break;
+        // 2x2x2 boxes throughout: 4x4x4 boxes below 32^3 (one level fewer)
+        // took 28 CG iterations instead of 16 at 128^3 and 121 instead of 85
+        // on the 256^3 conduit field
         L.fx = d.nx > 1 ? 2 : 1;
         L.fy = d.ny > 1 ? 2 : 1;
         L.fz = d.nz > 1 ? 2 : 1;
