@@ -867,7 +867,7 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
         if (rc) return rc;
     }
     S.D = defl && opt.deflate ? defl : nullptr;
-    if (H.nlev < 3 || H.tail0 <= 0 || H.tail_lds_inv < 0 || H.tail_nops > kTailLdsOps) return HMFEM_ERR_ARG;
+    if (H.nlev < 3 || H.tail0 <= 0 || !tail_in_lds(H)) return HMFEM_ERR_ARG;
     for (int l = 0; l <= H.tail0; l++) {
         const size_t nl = H.lev[l].d.n();
         if (l > 0) S.lv[l].f = ar.take<float>(nl * nr);

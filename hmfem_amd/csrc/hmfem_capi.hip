@@ -78,7 +78,8 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->ocf = 2.5;
     o->coarse_max = kCoarseMax;
     o->check_every = 4;
-    o->tail_max = 512;
+    o->tail_max = 512;   // 4096 (a 16^3 level in the tail, now that it fits in LDS) was slower: one
+                         // workgroup sweeping 4096 cells takes longer than the three multi-block launches
     o->tail_gamma = 2;
     o->use_graph = 1;
     o->tail_lds = 1;
@@ -105,7 +106,7 @@ extern "C" int hmfem_create(hmfem_context **out)
     int dev = 0, lds = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
-        c->lds_limit = (lds < 65536 ? lds : 65536) - 8192;   // room for the kernel's static LDS
+        c->lds_limit = lds - 8192;   // room for the tail kernel's static LDS (gfx950: 160 KB per workgroup)
     *out = c;
     return HMFEM_OK;
 }

@@ -58,11 +58,21 @@ struct Hierarchy {
     int4 *tail_ops = nullptr;
     int tail_nops = 0;
     int tail_lds_bytes = 0;    // dynamic LDS of the tail kernel (0: global mode)
-    int tail_lds_inv = -1;     // LDS offset of the dense inverse (-1: global mode)
+    int tail_lds_inv = -1;     // LDS offset of the dense inverse (-1: global mode; kTailInvGlobal: levels in
+                               // LDS, the inverse read from global memory)
     // fine level stored in FP16 (HMFEM_PREC_FP16): scaled couplings and scale
     __half *f16x = nullptr, *f16y = nullptr, *f16z = nullptr;
     float *f16sq = nullptr;
 };
+
+constexpr int kTailInvGlobal = -2;
+
+// The tail runs as the LDS kernel (its levels fit in LDS and its op list in
+// the kernel's static array).
+inline bool tail_in_lds(const Hierarchy &H)
+{
+    return (H.tail_lds_inv >= 0 || H.tail_lds_inv == kTailInvGlobal) && H.tail_nops <= kTailLdsOps;
+}
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 

@@ -576,7 +576,8 @@ k_tail_lds(const TailLevel<T> *__restrict__ Lg, const int4 *__restrict__ opsg, i
     __shared__ double fs[kCoarseMax];
     if (*done) return;
     T *A = (T *)lds;
-    double *inv = (double *)(lds + inv_off);
+    double *inv_l = inv_off >= 0 ? (double *)(lds + inv_off) : nullptr;
+    const double *inv = inv_l ? inv_l : invg;   // kTailInvGlobal: read from global memory
     const int tid = threadIdx.x;
     if (tid < nlev) {
         const TailLevel<T> &L = Lg[tid];
@@ -585,7 +586,8 @@ k_tail_lds(const TailLevel<T> *__restrict__ Lg, const int4 *__restrict__ opsg, i
         dsc[tid][5] = L.fx; dsc[tid][6] = L.fy; dsc[tid][7] = L.fz;
     }
     for (int q = tid; q < nops; q += kTailThreads) ops[q] = opsg[q];
-    for (int q = tid; q < nc * nc; q += kTailThreads) inv[q] = invg[q];
+    if (inv_l)
+        for (int q = tid; q < nc * nc; q += kTailThreads) inv_l[q] = invg[q];
     __syncthreads();
     for (int l = 0; l < nlev; l++) {
         const TailLevel<T> &L = Lg[l];
@@ -1268,13 +1270,20 @@ static int upload_tail(hipStream_t st, Hierarchy &H, const hmfem_options &opt, s
         t.n = (int)L.d.n();
         t.fx = L.fx; t.fy = L.fy; t.fz = L.fz;
         t.lds_off = (int)off;
-        off += (10 * (size_t)t.n * sizeof(T) + 15) / 16 * 16;
+        // e, wx, wy, wz, di, f, u, t, and r, w (second right-hand side and
+        // saved u of a repeated visit) except on the first tail level, which
+        // the op list visits once
+        const int arrays = l == H.tail0 ? 8 : 10;
+        off += (arrays * (size_t)t.n * sizeof(T) + 15) / 16 * 16;
     }
     size_t inv_off = off;
-    off += (size_t)H.nc * H.nc * sizeof(double);
-    if ((long)off <= lds_limit) {
-        H.tail_lds_bytes = (int)off;
+    const size_t inv_bytes = (size_t)H.nc * H.nc * sizeof(double);
+    if ((long)(off + inv_bytes) <= lds_limit) {
+        H.tail_lds_bytes = (int)(off + inv_bytes);
         H.tail_lds_inv = (int)inv_off;
+    } else if ((long)off <= lds_limit) {   // the levels in LDS, the dense inverse read from global memory
+        H.tail_lds_bytes = (int)off;
+        H.tail_lds_inv = kTailInvGlobal;
     } else {
         H.tail_lds_bytes = 0;
         H.tail_lds_inv = -1;
@@ -1465,7 +1474,7 @@ static T *vcycle_coarse(hipStream_t st, Hierarchy &H, int l, const CycleParams &
                     done);
         return u;
     }
-    if (H.tail0 > 0 && l == H.tail0 && H.tail_lds_inv >= 0 && H.tail_nops <= kTailLdsOps) {
+    if (H.tail0 > 0 && l == H.tail0 && tail_in_lds(H)) {
         PROF_LAUNCH(P, HMFEM_PROF_COARSE, k_tail_lds<T>, dim3(1), dim3(kTailThreads), H.tail_lds_bytes, st,
                     (const TailLevel<T> *)H.tail_levels, H.tail_ops, H.tail_nops, H.inv, H.nc, (T)cp.omega,
                     (T)cp.ocf, H.nlev - H.tail0, H.tail_lds_inv, (const T *)nullptr, (T *)nullptr, 1, done);
@@ -1885,7 +1894,7 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
 int launch_tail_strided(hipStream_t st, const Hierarchy &H, double omega, double ocf, const float *ftop,
                         float *utop, int stride, const int *done)
 {
-    if (H.prec != HMFEM_PREC_FP32 || H.tail0 <= 0 || H.tail_lds_inv < 0 || H.tail_nops > kTailLdsOps)
+    if (H.prec != HMFEM_PREC_FP32 || H.tail0 <= 0 || !tail_in_lds(H))
         return HMFEM_ERR_ARG;
     hipLaunchKernelGGL(k_tail_lds<float>, dim3(1), dim3(kTailThreads), H.tail_lds_bytes, st,
                        (const TailLevel<float> *)H.tail_levels, H.tail_ops, H.tail_nops, H.inv, H.nc, (float)omega,
