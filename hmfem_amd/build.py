@@ -60,6 +60,15 @@ def build_gpu(verbose=False):
     return out
 
 
+def build_driver(verbose=False):
+    """hmfem_run: the C host of the reservoir step (links both libraries)."""
+    out = os.path.join(HERE, "hmfem_run")
+    _run(["gcc", "-O2", "-Wall", "-o", out, os.path.join(HERE, "driver", "hmfem_run.c"),
+          "-I" + os.path.join(ROOT, "include"), "-L" + HERE, "-lhmfem_gpu", "-lhmfem_fields", "-lm",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    return out
+
+
 def build_oracle(verbose=False):
     """The checker (oracle/liboracle.so) and, where the reference tree is
     present, the reference build oracle/_ref/hmfem_ref."""
@@ -72,6 +81,7 @@ def build_oracle(verbose=False):
 def build_all(verbose=False):
     build_fields(verbose)
     build_gpu(verbose)
+    build_driver(verbose)
     build_oracle(verbose)
 
 

@@ -54,17 +54,21 @@ struct hmfem_context {
     }
 };
 
-static hmfem_context *g_default = nullptr;
+// Contexts of the host-buffer drop-ins: one for the 3-D reservoir system and
+// one for 1-D systems (the conduits' compute() calls), so that the 1-D solves
+// between two reservoir passes do not evict the reservoir's cached setup.
+static hmfem_context *g_default = nullptr, *g_default_1d = nullptr;
 
-static hmfem_context *default_ctx()
+static hmfem_context *default_ctx(bool one_d = false)
 {
-    if (!g_default) {
-        if (hmfem_create(&g_default) != HMFEM_OK) {
+    hmfem_context **c = one_d ? &g_default_1d : &g_default;
+    if (!*c) {
+        if (hmfem_create(c) != HMFEM_OK) {
             fprintf(stderr, "hmfem_gpu: cannot create a HIP context (no device?)\n");
             exit(1);
         }
     }
-    return g_default;
+    return *c;
 }
 
 extern "C" void hmfem_default_options(hmfem_options *o)
@@ -126,6 +130,7 @@ extern "C" void hmfem_destroy(hmfem_context *c)
     if (c->defl.h_stage) (void)hipHostFree(c->defl.h_stage);
     (void)hipStreamDestroy(c->st);
     if (c == g_default) g_default = nullptr;
+    if (c == g_default_1d) g_default_1d = nullptr;
     delete c;
 }
 
@@ -592,6 +597,14 @@ extern "C" double hmfem_kernel_bytes_per_cell(int which, int precision)
 
 // ------------------------------------------------------------ host-buffer drop-ins
 
+// HMFEM_REUSE_SETUP=1: the drop-ins reuse the previous solve's hierarchy on
+// the same grid (the fixed-point passes of main.c); default off.
+static int env_reuse()
+{
+    const char *s = getenv("HMFEM_REUSE_SETUP");
+    return s && atoi(s) != 0;
+}
+
 static double env_rtol()
 {
     const char *s = getenv("HMFEM_RTOL");
@@ -688,7 +701,6 @@ static int structured_from_csr(int n, const int *cnt, const int *col, const doub
 
 extern "C" int compute(int *hdr, int *cnt, int *col, double *ele, double *b, double *x)
 {
-    hmfem_context *c = default_ctx();
     int n = hdr[0];
     Dims d;
     std::vector<double> e, wx, wy, wz;
@@ -697,6 +709,7 @@ extern "C" int compute(int *hdr, int *cnt, int *col, double *ele, double *b, dou
                         "general sparse input is not supported by this library\n");
         return HMFEM_ERR_NOT_STRUCTURED;
     }
+    hmfem_context *c = default_ctx(d.ny == 1 && d.nz == 1);
     Stencil<double> A;
     A.e = upload(c, "e", e.data(), n);
     A.wx = upload(c, "wx", wx.data(), n);
@@ -707,6 +720,7 @@ extern "C" int compute(int *hdr, int *cnt, int *col, double *ele, double *b, dou
     hmfem_options opt;
     hmfem_default_options(&opt);
     opt.rtol = env_rtol();
+    opt.reuse_setup = env_reuse();
     hmfem_status st{};
     int rc = solve_on_stencil(c, d, A, db, dx, &opt, &st);
     if (rc) fatal("compute", rc);
@@ -736,6 +750,7 @@ extern "C" void solve_linsys(ELE *perm, IND *indcol, double *b, double *x, Dim N
     hmfem_options opt;
     hmfem_default_options(&opt);
     opt.rtol = env_rtol();
+    opt.reuse_setup = env_reuse();
     hmfem_status st{};
     int rc = hmfem_solve_structured_device(c, N.x, N.y, N.z, dperm, dind, db, dx, &opt, &st);
     if (rc == HMFEM_ERR_NOT_STRUCTURED) {

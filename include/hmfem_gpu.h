@@ -55,7 +55,10 @@ extern FACE boundary;   /* read by fluxos() (flux_reconstruction.c:104-270) */
    perm/indcol as filled by permeabilidade_arestas(), b the RHS, x in/out
    (initial guess in, pressure out). Stops at relative residual 1e-8
    (HMFEM_RTOL overrides). Writes iter_amg. Exits like the reference when the
-   matrix has no entries. */
+   matrix has no entries. With HMFEM_REUSE_SETUP=1 in the environment,
+   solve_linsys() and compute() reuse the previous call's hierarchy on the
+   same grid (hmfem_options.reuse_setup); 1-D systems (the conduits') use a
+   context of their own, so they do not evict the reservoir's. */
 void solve_linsys(ELE *perm, IND *indcol, double *b, double *x, Dim N);
 
 /* compute.cpp:7 compute(); prototype functions.h:53. Count-CSR: hdr[0] = n,
