@@ -48,7 +48,16 @@ void add(std::string &out, const char *fmt, A... a)
 {
     char line[256];
     const int k = snprintf(line, sizeof(line), fmt, a...);
-    out.append(line, k > 0 ? (size_t)k : 0);
+    if (k <= 0) return;
+    if ((size_t)k < sizeof(line)) {
+        out.append(line, (size_t)k);
+        return;
+    }
+    // a longer line (e.g. %.6lf of a huge coordinate): format it again in place
+    const size_t at = out.size();
+    out.resize(at + (size_t)k + 1);
+    snprintf(&out[at], (size_t)k + 1, fmt, a...);
+    out.resize(at + (size_t)k);
 }
 
 // Run body(lo, hi, out) over [0, count) in contiguous slabs on up to 16

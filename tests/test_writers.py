@@ -76,3 +76,16 @@ def test_writers_read_host_dirname(tmp_path):
     text = (out / "pressure_reservoir.dat").read_text()
     assert text.startswith("0.250000 0.250000 0.250000 1.000000000e+00\n")
     assert text.count("\n") == 8 + 4 + 2
+
+
+def test_writers_long_lines(tmp_path):
+    """Coordinates far beyond the usual unit box give lines longer than the
+    formatter's line buffer; they must come out whole, as fprintf prints them."""
+    N = S.make_dim(2, 1, 1)
+    h = (1e300, 1.0, 1.0)
+    S.set_dirname(str(tmp_path))
+    S.save_pressure(np.array([1.0, 2.0]), N, h)
+    text = (tmp_path / "pressure_reservoir.dat").read_text()
+    first = text.splitlines()[0].split()
+    assert len(first) == 4 and float(first[0]) == 0.5e300 and first[3] == "1.000000000e+00"
+    assert first[0] == "%.6f" % 0.5e300
