@@ -76,7 +76,7 @@ __device__ __forceinline__ void gs_tile_body(const OpArgs<T> &o, int sweeps, con
 
     // this thread's coefficients (zero for cells outside the grid)
     C wxp[CZ], wxm[CZ], wyp[CZ], wym[CZ], wzp[CZ], dinv[CZ], ff[CZ], uu[CZ];
-    double fd[MODE == 2 ? CZ : 1];
+    double fd[MODE == 2 ? CZ : 1], vd[MODE == 2 ? CZ : 1];   // MODE 2: f and v for the dot products
 #pragma unroll
     for (int kk = 0; kk < CZ; kk++) {
         const bool act = col && kc + kk < nz;
@@ -99,7 +99,10 @@ __device__ __forceinline__ void gs_tile_body(const OpArgs<T> &o, int sweeps, con
             dinv[kk] = act ? (C)o.di[s] : (C)0;
             ff[kk] = (C)fv;
         }
-        if constexpr (MODE == 2) fd[kk] = fv;
+        if constexpr (MODE == 2) {
+            fd[kk] = fv;
+            vd[kk] = act && v ? v[s] : 0.0;   // loaded now, used after the sweeps
+        }
     }
     C wzlo;
     if constexpr (H16) wzlo = col && kc > 0 && kc < nz ? (C)__half2float(h.wz[s0 - nxy]) : (C)0;
@@ -181,7 +184,7 @@ __device__ __forceinline__ void gs_tile_body(const OpArgs<T> &o, int sweeps, con
             uo[s] = z;
             if constexpr (MODE == 2) {
                 acc[0] += (double)z * fd[kk];
-                if (v) acc[1] += (double)z * v[s];
+                if (v) acc[1] += (double)z * vd[kk];
             }
         }
     }
