@@ -442,7 +442,7 @@ def main():
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get(dom)
 
-    # Plain FP64 hierarchy on the same workload (the north star's comparison).
+    # Plain FP64 hierarchy on the same workload (FP32-AMG against FP64).
     fp64 = None
     if not args.no_fp64 and prec == S.PREC_FP32:
         o64 = S.default_options(rtol=rtol, precision=S.PREC_FP64)

@@ -264,6 +264,43 @@ def test_full_size_c2_properties(smoother, ctx, torch):
     assert abs(outflow - inflow) <= hx * np.sqrt(n) * r * np.linalg.norm(bh) * 1.01
 
 
+def test_full_size_c3_properties(ctx, torch):
+    """256^3 homogeneous K with 20 karst conduits (the conduit benchmark
+    workload) at rtol 1e-8 with the conduit correction: true residual,
+    maximum principle, global mass balance."""
+    from hmfem_amd import fields as F
+    m = 256
+    K = F.conduits(F.homogeneous(m, m, m, 1e-14), m, m, m, 20, 20261202)
+    P = O.Problem((m, m, m), K, (0, 0, 1, 1, 1, 1), (1.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+    _say("C3 256^3")
+    D = Dev(P, torch)
+    n = P.n
+    p = torch.zeros(n, dtype=torch.float64, device="cuda")
+    v = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, v, S.default_options(rtol=1e-8))
+    _say("C3 status", st.as_dict())
+    assert st.status == S.OK and st.deflation_clusters >= 1
+    perm = torch.zeros(n * 7, dtype=torch.float64, device="cuda")
+    indcol = torch.zeros(n * 7, dtype=torch.int32, device="cuda")
+    b = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ctx.assemble(D.g, D.K, D.K, D.K, D.bval, b, perm, indcol)
+    x = p.cpu().numpy()
+    operm = perm.cpu().numpy().view(O.ELE).reshape(n)
+    oind = indcol.cpu().numpy().view(O.IND).reshape(n)
+    del perm, indcol
+    bh = b.cpu().numpy()
+    r = true_residual(operm, oind, bh, x)
+    assert r < 2e-8
+    assert x.min() > -1e-6 and x.max() < 1 + 1e-6
+    vv = v.cpu().numpy().view(S.ELE).reshape(m, m, m)
+    inflow = -vv["l"][:, :, 0].sum()
+    outflow = vv["r"][:, :, -1].sum()
+    res_sum = (O.prod_mat_vec(operm, oind, x) - bh).sum()
+    hx = 1.0 / m
+    assert abs((outflow - inflow) - hx * res_sum) <= 1e-9 * abs(inflow)
+    assert abs(outflow - inflow) <= hx * np.sqrt(n) * r * np.linalg.norm(bh) * 1.01
+
+
 @pytest.mark.parametrize("smoother", [S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS], ids=["jacobi", "blockgs"])
 def test_graph_replay_matches_eager(smoother, ctx, torch):
     """The captured CG iteration replayed from a HIP graph gives the same
@@ -337,6 +374,36 @@ def _expected_clusters(P, ratio):
     sizes = np.bincount(lab.ravel())[1:]
     keep = np.sort(sizes[sizes >= 8])[::-1][:256]
     return int(keep.size), int(keep.sum())
+
+
+def test_full_size_c4_properties(ctx, torch):
+    """The 96-instance benchmark batch (edges 32-96, lognormal and conduit
+    fields) at rtol 1e-8: every instance converged, its true residual against
+    the oracle's own assembly of that instance, and the maximum principle."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    w = bench.WORKLOADS["c4"]
+    inst = bench.c4_instances(w)
+    _say("C4 batch", len(inst))
+    K = torch.from_numpy(np.concatenate([k for _, k, _ in inst])).to("cuda")
+    big = max(max(d[0] * d[1], d[1] * d[2], d[0] * d[2]) for d, _, _ in inst)
+    bval = [torch.full((big,), 1.0, dtype=torch.float64, device="cuda")] + \
+           [torch.zeros(big, dtype=torch.float64, device="cuda") for _ in range(5)]
+    ntot = sum(d[0] * d[1] * d[2] for d, _, _ in inst)
+    p = torch.zeros(ntot, dtype=torch.float64, device="cuda")
+    st, per = ctx.reservoir_batch([d for d, _, _ in inst], K, K, K, bval, p, None, S.default_options(rtol=1e-8))
+    _say("C4 status", st.as_dict())
+    assert st.status == S.OK and np.all(per["iterations"] > 0)
+    xb = p.cpu().numpy()
+    off = 0
+    for d, k, desc in inst:
+        n = d[0] * d[1] * d[2]
+        P = O.Problem(d, k, (0, 0, 1, 1, 1, 1), (1.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+        perm, ind, b = O.assemble(P)
+        x = xb[off:off + n]
+        assert true_residual(perm, ind, b, x) < 2e-8, (d, desc)
+        assert x.min() > -1e-6 and x.max() < 1 + 1e-6, (d, desc)
+        off += n
 
 
 def test_batch_deflation(ctx, torch):
