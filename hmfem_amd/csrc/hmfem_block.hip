@@ -23,6 +23,9 @@
 namespace hmfem {
 
 constexpr int kMaxRhs = 8;
+// (The MODE 2 instance holds 169 VGPRs, one block per CU; two cells per
+// thread with 1024-thread blocks, or a 128-VGPR cap, spill 272-580 bytes per
+// lane.)
 // z extent of a block-smoother tile: 32 x 8 x 8 tiles, two threads per
 // column. At 256^3 with four right-hand sides a CG iteration takes 3.19 ms
 // against 3.05 for 32 x 8 x 4 tiles (one thread per column), 3.82 (32 x 8 x 4,
@@ -334,15 +337,15 @@ constexpr int kBThreads = kGsTx * kGsTy * (kBTz / kBCz);
 // MODE 1: from u + ocf * uc[aggregate] (uo must not alias u); MODE 2: MODE 1
 // plus the CG dot partials <z, f>_r, <z, v>_r of this block's tiles
 // (Pdot[lb][2 NR]).
-template <int NR, typename FT, int MODE>
-__global__ void __launch_bounds__(kBThreads)
+template <int NR, typename FT, int MODE, int CZ = kBCz, int NTHREADS = kGsTx * kGsTy * (kBTz / CZ)>
+__global__ void __launch_bounds__(NTHREADS)
 k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *__restrict__ u,
             float *__restrict__ uo, int sx, int sy, int sz, int cx, int cy, const float *__restrict__ uc, float ocf,
             const double *__restrict__ v, int tpb, double *Pdot, const int *done)
 {
-    constexpr int TZ = kBTz, CZ = kBCz, NTZ = kBTz / kBCz;
+    constexpr int TZ = kBTz, NTZ = kBTz / CZ;
     constexpr int RING = kGsRing;
-    constexpr int RPER = (TZ * RING + kBThreads - 1) / kBThreads;
+    constexpr int RPER = (TZ * RING + NTHREADS - 1) / NTHREADS;
     static_assert(CZ % 2 == 0, "colour pairs");
     if (*done) return;
     __shared__ float sh[TZ][kGsTy + 2][kGsTx + 2][NR];
@@ -398,7 +401,7 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
         if (tz == NTZ - 1) start(i, j, k0 + TZ, zhi);
 #pragma unroll
         for (int m = 0; m < RPER; m++) {
-            const int q = threadIdx.x + m * kBThreads;
+            const int q = threadIdx.x + m * NTHREADS;
             int hx = 0, hy = 0, hz = 0;
             if (q < TZ * RING) gs_ring_cell(q, hx, hy, hz);
             if (q < TZ * RING) start(i0 + hx - 1, j0 + hy - 1, k0 + hz, ring[m]);
@@ -464,7 +467,7 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
         }
     }
     if constexpr (MODE == 2) {
-        block_sum_n<2 * NR, kBThreads>(acc);
+        block_sum_n<2 * NR, NTHREADS>(acc);
         if (threadIdx.x == 0)
             for (int q = 0; q < 2 * NR; q++) Pdot[(long)2 * NR * lb + q] = acc[q];
     }
