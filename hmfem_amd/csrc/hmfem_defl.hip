@@ -41,13 +41,21 @@ k_diag_hist(OpArgs<double> o, unsigned *hist)
     for (int q = threadIdx.x; q < kHistBins; q += kBlock) h[q] = 0;
     __syncthreads();
     const int n = o.nx * o.ny * o.nz;
+    int run_bin = -1;   // runs of equal bins counted in a register, as in k_diag_hist_inst
+    unsigned run = 0;
     for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
         Cell c = cell_at(o, s);
         const double d = diag_at<double, double>(o, c);
         int b = d > 0 ? ilogb(d) + 1100 : 0;
         b = b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
-        atomicAdd(&h[b], 1u);
+        if (b != run_bin) {
+            if (run) atomicAdd(&h[run_bin], run);
+            run_bin = b;
+            run = 0;
+        }
+        run++;
     }
+    if (run) atomicAdd(&h[run_bin], run);
     __syncthreads();
     for (int q = threadIdx.x; q < kHistBins; q += kBlock)
         if (h[q]) atomicAdd(&hist[q], h[q]);
@@ -93,14 +101,25 @@ k_diag_hist_inst(const DefInst *__restrict__ I, const int *__restrict__ hmap, St
     o.e = A.e + in.off; o.wx = A.wx + in.off; o.wy = A.wy + in.off; o.wz = A.wz + in.off; o.di = nullptr;
     o.nx = in.nx; o.ny = in.ny; o.nz = in.nz;
     const int l0 = (blockIdx.x - in.hblk0) * kBlock * kHistCells + threadIdx.x;
+    // runs of equal bins are counted in a register (fields are smooth or
+    // piecewise constant, so most of a thread's cells share a bin): few LDS
+    // atomics on the crowded bins
+    int run_bin = -1;
+    unsigned run = 0;
     for (int t = 0; t < kHistCells; t++) {
         const int l = l0 + t * kBlock;
         if (l >= in.n) break;
         const double d = diag_at<double, double>(o, cell_at(o, l));
         int b = d > 0 ? ilogb(d) + 1100 : 0;
         b = b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
-        atomicAdd(&h[b], 1u);
+        if (b != run_bin) {
+            if (run) atomicAdd(&h[run_bin], run);
+            run_bin = b;
+            run = 0;
+        }
+        run++;
     }
+    if (run) atomicAdd(&h[run_bin], run);
     __syncthreads();
     for (int t = threadIdx.x; t < kHistBins; t += kBlock)
         if (h[t]) atomicAdd(&hist[(long)q * kHistBins + t], h[t]);
@@ -438,8 +457,10 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     HMFEM_CHECK(hipStreamSynchronize(st));
     const int ms = std::min(hl[0], (int)ub);
     if (ms < kMinCluster) return HMFEM_OK;
-    std::vector<int> cell(hl + 1, hl + 1 + ms);   // in index order: a deterministic labelling
-    std::vector<int> cin(ms);                     // instance of each strong cell
+    DeflationHost &W = D.h;
+    auto &cell = W.cell, &cin = W.cin;
+    cell.assign(hl + 1, hl + 1 + ms);   // in index order: a deterministic labelling
+    cin.resize(ms);                     // instance of each strong cell
     for (int a = 0, q = 0; a < ms; a++) {
         while (q + 1 < ninst && cell[a] >= inst[q + 1].off) q++;
         cin[a] = q;
@@ -449,8 +470,10 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     // the +x, +y, +z neighbours by one monotone sweep per direction (cell[]
     // is sorted and a neighbour stays inside the instance), the -x, -y, -z
     // ones by symmetry
-    std::vector<int> nbr(6L * ms, -1);   // [a][-x,+x,-y,+y,-z,+z]
-    std::vector<unsigned char> up(ms);   // bit f: the +f neighbour is inside the instance
+    auto &nbr = W.nbr;
+    auto &up = W.up;
+    nbr.assign(6L * ms, -1);   // [a][-x,+x,-y,+y,-z,+z]
+    up.resize(ms);             // bit f: the +f neighbour is inside the instance
     for (int a = 0; a < ms; a++) {
         const DefInst &in = inst[cin[a]];
         const int l = cell[a] - in.off, i = l % in.nx, jk = l / in.nx, j = jk % in.ny, k = jk / in.ny;
@@ -470,7 +493,8 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
         }
     }
     // union-find over the +x, +y, +z links
-    std::vector<int> par(ms);
+    auto &par = W.par;
+    par.resize(ms);
     std::iota(par.begin(), par.end(), 0);
     for (int a = 0; a < ms; a++)
         for (int f = 1; f < 6; f += 2) {
@@ -479,9 +503,12 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
             int ra = find_root(par, a), rb = find_root(par, b);
             if (ra != rb) par[std::max(ra, rb)] = std::min(ra, rb);
         }
-    std::vector<int> root(ms), size(ms, 0);
+    auto &root = W.root, &size = W.size;
+    root.resize(ms);
+    size.assign(ms, 0);
     for (int a = 0; a < ms; a++) { root[a] = find_root(par, a); size[root[a]]++; }
-    std::vector<int> roots;
+    auto &roots = W.roots;
+    roots.clear();
     for (int a = 0; a < ms; a++)
         if (root[a] == a && size[a] >= kMinCluster) roots.push_back(a);
     std::stable_sort(roots.begin(), roots.end(), [&](int x, int y) { return size[x] > size[y]; });
@@ -489,10 +516,12 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     std::sort(roots.begin(), roots.end());   // cluster order: by smallest member (so by instance)
     const int nc = (int)roots.size();
     if (nc == 0) return HMFEM_OK;
-    std::vector<int> cid(ms, -1);
+    auto &cid = W.cid;
+    cid.assign(ms, -1);
     for (int q = 0; q < nc; q++) cid[roots[q]] = q;
     for (int a = 0; a < ms; a++) cid[a] = cid[root[a]];
-    std::vector<int> ptr(nc + 1, 0), lcell, lcomp, lord, linst;
+    auto &ptr = W.ptr, &lcell = W.lcell, &lcomp = W.lcomp, &lord = W.lord, &linst = W.linst;
+    ptr.assign(nc + 1, 0);
     for (int a = 0; a < ms; a++)
         if (cid[a] >= 0) ptr[cid[a] + 1]++;
     for (int q = 0; q < nc; q++) ptr[q + 1] += ptr[q];
@@ -502,7 +531,8 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     lord.resize(m);
     linst.resize(m);
     {
-        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        auto &fill = W.fill;
+        fill.assign(ptr.begin(), ptr.end() - 1);
         for (int a = 0; a < ms; a++)
             if (cid[a] >= 0) {
                 const int q = fill[cid[a]]++;
@@ -513,18 +543,22 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
             }
     }
     // clusters of each instance: [irange[q], irange[q+1])
-    std::vector<int> irange(ninst + 1, 0);
+    auto &irange = W.irange;
+    irange.assign(ninst + 1, 0);
     for (int q = 0; q < nc; q++) irange[cin[roots[q]] + 1]++;
     for (int q = 0; q < ninst; q++) irange[q + 1] += irange[q];
 
     // pieces of at most kPiece cells per cluster, contiguous in the list
-    std::vector<int> pbh(1, 0), cph(1, 0);
+    auto &pbh = W.pbh, &cph = W.cph;
+    pbh.assign(1, 0);
+    cph.assign(1, 0);
     for (int q = 0; q < nc; q++) {
         for (int b = ptr[q]; b < ptr[q + 1]; b += kPiece) pbh.push_back(std::min(b + kPiece, ptr[q + 1]));
         cph.push_back((int)pbh.size() - 1);
     }
     const int np = (int)pbh.size() - 1;
-    std::vector<int> pc(pbh);
+    auto &pc = W.pc;
+    pc.assign(pbh.begin(), pbh.end());
     pc.insert(pc.end(), cph.begin(), cph.end());
     pc.insert(pc.end(), irange.begin(), irange.end());
     for (int q = 0; q < nc; q++) pc.push_back(cin[roots[q]]);   // instance of each cluster
@@ -545,20 +579,31 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     D.psv = D.psr + per * np;
     D.pext = D.psv + per * np;
     int *dlinst = D.comp + m;
-    HMFEM_CHECK(hipMemcpyAsync(D.cells, lcell.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
-    HMFEM_CHECK(hipMemcpyAsync(D.comp, lcomp.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
-    HMFEM_CHECK(hipMemcpyAsync(dlinst, linst.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
-    HMFEM_CHECK(hipMemcpyAsync(D.pb, pc.data(), sizeof(int) * pc.size(), hipMemcpyHostToDevice, st));
+    // the lists go up through the pinned staging; the coefficient copy below
+    // reuses it only after these copies (same stream)
+    {
+        const size_t lb = sizeof(int) * (3 * (size_t)m + pc.size()), cbytes = sizeof(double) * 7 * (size_t)m;
+        if (stage_reserve(D, std::max(lb, cbytes))) return HMFEM_ERR_NOMEM;   // the strong list is copied out
+        int *hs = (int *)D.h_stage;
+        memcpy(hs, lcell.data(), sizeof(int) * m);
+        memcpy(hs + m, lcomp.data(), sizeof(int) * m);
+        memcpy(hs + 2 * (size_t)m, linst.data(), sizeof(int) * m);
+        memcpy(hs + 3 * (size_t)m, pc.data(), sizeof(int) * pc.size());
+        HMFEM_CHECK(hipMemcpyAsync(D.cells, hs, sizeof(int) * m, hipMemcpyHostToDevice, st));
+        HMFEM_CHECK(hipMemcpyAsync(D.comp, hs + m, sizeof(int) * 2 * (size_t)m, hipMemcpyHostToDevice, st));
+        HMFEM_CHECK(hipMemcpyAsync(D.pb, hs + 3 * (size_t)m, sizeof(int) * pc.size(), hipMemcpyHostToDevice, st));
+    }
     hipLaunchKernelGGL(k_defl_gather, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, dinst,
                        (const int *)dlinst, A, D.cells, m, coef);
-    const double *w = (const double *)D.h_stage;   // 7 m <= 7 ub doubles: reserved above
+    const double *w = (const double *)D.h_stage;   // 7 m doubles: reserved above
     HMFEM_CHECK(hipMemcpyAsync(D.h_stage, coef, sizeof(double) * 7 * m, hipMemcpyDeviceToHost, st));
     HMFEM_CHECK(hipStreamSynchronize(st));
 
     // E = Z^T A Z in a fixed order: E_aa = sum (e + couplings leaving the
     // cluster), E_ab = - couplings between clusters a and b (block diagonal
     // over the instances)
-    std::vector<double> E((size_t)nc * nc, 0.0);
+    auto &E = W.E;
+    E.assign((size_t)nc * nc, 0.0);
     for (int q = 0; q < m; q++) {
         const int a = lcomp[q];
         const double *ws = &w[7L * q];
@@ -577,12 +622,13 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     // inverse via Cholesky E = L L^T (E is symmetric positive definite),
     // block by block: E has no coupling between clusters of different
     // instances
-    std::vector<double> inv((size_t)nc * nc, 0.0);
+    auto &inv = W.inv, &L = W.L, &y = W.y, &xb = W.xb;
+    inv.assign((size_t)nc * nc, 0.0);
     for (int q = 0; q < ninst; q++) {
         const int lo = irange[q], nb = irange[q + 1] - lo;
         if (nb == 0) continue;
         auto Eb = [&](int i, int j) { return E[(size_t)(lo + i) * nc + lo + j]; };
-        std::vector<double> L((size_t)nb * nb);
+        L.resize((size_t)nb * nb);
         for (int i = 0; i < nb; i++)
             for (int j = 0; j < nb; j++) L[(size_t)i * nb + j] = Eb(i, j);
         for (int j = 0; j < nb; j++) {
@@ -597,7 +643,8 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
                 L[(size_t)i * nb + j] = t / djj;
             }
         }
-        std::vector<double> y(nb), xb((size_t)nb * nb);
+        y.resize(nb);
+        xb.resize((size_t)nb * nb);
         for (int col = 0; col < nb; col++) {   // solve L L^T x = e_col
             for (int i = 0; i < nb; i++) {
                 double t = i == col ? 1.0 : 0.0;
@@ -613,7 +660,9 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
         for (int i = 0; i < nb; i++)
             for (int j = 0; j < nb; j++) inv[(size_t)(lo + i) * nc + lo + j] = xb[(size_t)i * nb + j];
     }
-    HMFEM_CHECK(hipMemcpyAsync(D.einv, inv.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
+    if (stage_reserve(D, sizeof(double) * nc * nc)) return HMFEM_ERR_NOMEM;   // the coefficients are used up
+    memcpy(D.h_stage, inv.data(), sizeof(double) * nc * nc);
+    HMFEM_CHECK(hipMemcpyAsync(D.einv, D.h_stage, sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipStreamSynchronize(st));
     D.nc = nc;
     D.m = m;
@@ -708,8 +757,8 @@ int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Sten
     HMFEM_CHECK(hipMemcpyAsync(D.h_stage, hist, sizeof(unsigned) * kHistBins * (size_t)ninst, hipMemcpyDeviceToHost,
                                st));
     HMFEM_CHECK(hipStreamSynchronize(st));
-    std::vector<unsigned> hh((const unsigned *)D.h_stage, (const unsigned *)D.h_stage + kHistBins * (size_t)ninst);
-    return finish_clusters(st, inst, dinst, bmap, G, A, hh.data(), ratio, D, alloc, actx);
+    // finish_clusters reads the histograms before it reuses the staging
+    return finish_clusters(st, inst, dinst, bmap, G, A, (const unsigned *)D.h_stage, ratio, D, alloc, actx);
 }
 
 template <int NR>

@@ -156,6 +156,16 @@ struct CgBuffers {
 };
 // Additive correction on clusters of strongly coupled cells (hmfem_defl.hip).
 // nc == 0: no correction for this operator.
+// Host work arrays of the cluster setup, kept by the context so that repeated
+// setups reuse their memory (fresh vectors of a few MB cost their page faults
+// every time).
+struct DeflationHost {
+    std::vector<int> cell, cin, nbr, par, root, size, roots, cid, ptr, lcell, lcomp, lord, linst, irange, fill,
+        pbh, cph, pc;
+    std::vector<unsigned char> up;
+    std::vector<double> E, inv, L, y, xb;
+};
+
 struct Deflation {
     int nc = 0, m = 0, np = 0;               // clusters, cells listed (grouped by cluster), pieces
     int *cells = nullptr, *comp = nullptr;   // cell index and cluster of each listed cell
@@ -176,6 +186,7 @@ struct Deflation {
     size_t h_stage_bytes = 0;
     bool pending = false;                    // deflation_begin queued a histogram
     unsigned gen = 0;                        // bumped by every setup (single, block or batch)
+    DeflationHost h;
 };
 // deflation_begin queues the diagonal histogram and its copy to the host;
 // deflation_finish (after other work has been queued) decides and builds the
