@@ -1492,12 +1492,16 @@ static T *vcycle_coarse(hipStream_t st, Hierarchy &H, int l, const CycleParams &
     Level &C = H.lev[l + 1];
     if (cp.smoother == HMFEM_SMOOTH_BLOCK_GS) {
         const int GT = grid_for(gs_tiles(L.d), 1);
-        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 0>), dim3(GT), dim3(kGsThreads), 0, st, o, cp.gs, f,
+        // the coarse levels use the fine level's sweep count: 1, 2, 3, 4 sweeps
+        // there gave 22, 18, 16, 16 CG iterations at 128^3 (3.92, 3.30, 3.08,
+        // 3.13 ms) and 103, 83, 85, 88 on the 256^3 conduit field
+        const int cgs = cp.gs;
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 0>), dim3(GT), dim3(kGsThreads), 0, st, o, cgs, f,
                     (const T *)nullptr, u, 0, 0, 0, 0, 0, (const T *)nullptr, (T)0, (const double *)nullptr, 1,
                     (double *)nullptr, done);
         launch_restrict<T, T>(st, P, HMFEM_PROF_COARSE, L, C, f, (const T *)u, done);
         T *uc = vcycle_coarse<T>(st, H, l + 1, cp, P, done);
-        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 1>), dim3(GT), dim3(kGsThreads), 0, st, o, cp.gs, f,
+        PROF_LAUNCH(P, HMFEM_PROF_COARSE, (k_gs_tile<T, T, 1>), dim3(GT), dim3(kGsThreads), 0, st, o, cgs, f,
                     (const T *)u, t, lg2f(L.fx), lg2f(L.fy), lg2f(L.fz), C.d.nx, C.d.ny, (const T *)uc, (T)cp.ocf,
                     (const double *)nullptr, 1, (double *)nullptr, done);
         L.u = t; L.t = u;
