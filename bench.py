@@ -346,6 +346,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--profile-only", action="store_true", help="timed steps only (for rocprofv3 runs)")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--edge", type=int, default=0, metavar="N",
+                    help="smaller cube edge for the c2/c3/c5 workload (profiling runs; conduit counts scale with "
+                         "the cross-section); the JSON names the size")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -375,6 +378,12 @@ def main():
         print(json.dumps({"copy_bytes": nb, "launches": 5}))
         return
     w = WORKLOADS[args.workload]
+    if args.edge and "dims" in w:
+        e = args.edge
+        nx0 = w["dims"][0]
+        w = dict(w, dims=(e, e, e), label="%s [cube edge %d instead of %d]" % (w["label"], e, nx0))
+        if "conduits" in w:
+            w["conduits"] = max(1, round(w["conduits"] * e * e / (nx0 * nx0)))
     if "batch" in w:
         return run_batch(args, w, torch, S)
     if "block" in w:
