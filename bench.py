@@ -121,6 +121,27 @@ def cpu_baseline(K, dims, budget_s, bc_type=(0, 0, 1, 1, 1, 1), bc_val=(1.0, 0.0
             "sample": "one full %dx%dx%d step of the oracle restatement, single thread; %.2f s" % (nx, ny, nz, t)}
 
 
+def device_stream_gbs(torch, stream, mib=1024, reps=10):
+    """Bytes moved per second by STREAM's Add kernel (c = a + b, FP64 arrays
+    of 1 GiB each, so far past the 256 MB last-level cache) on `stream`: the
+    practical HBM rate the kernels are also read against."""
+    n = (mib << 20) // 8
+    with torch.cuda.stream(stream):
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.ones_like(a)
+        c = torch.empty_like(a)
+        torch.add(a, b, out=c)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            torch.add(a, b, out=c)
+        e1.record(stream)
+    e1.synchronize()
+    gbs = 3 * n * 8 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b, c
+    return gbs
+
+
 def run_batch(args, w, torch, S):
     inst = c4_instances(w)
     dims = [d for d, _, _ in inst]
@@ -450,6 +471,7 @@ def main():
     tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get(dom)
+    stream_gbs = device_stream_gbs(torch, solver_stream)
 
     # Plain FP64 hierarchy on the same workload (FP32-AMG against FP64).
     fp64 = None
@@ -576,6 +598,7 @@ def main():
         "rel_residual": st.rel_residual, "div_max": st.div_max,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "stream_add_gbs": stream_gbs, "frac_of_stream_add": achieved / stream_gbs,
                      "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": 1e3 * avg_ms},
         "kernels": kernels,
         "fp64_hierarchy": fp64,

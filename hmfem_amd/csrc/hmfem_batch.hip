@@ -399,11 +399,16 @@ kb_cg_init(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<dou
     if (threadIdx.x == 0) { Pinit[2 * blockIdx.x] = v[0]; Pinit[2 * blockIdx.x + 1] = v[1]; }
 }
 
+// Per instance: s = z + beta s_old, v = A s, <s, v>, and the previous
+// iteration's x += alpha s_old (s_old of the own cell is read here anyway;
+// kb_cg_update leaves it, kb_cg_finish_x applies it if the instance stops
+// at this kernel).
 template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
 kb_cg_spmv(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<double> A, const ZT *__restrict__ z,
-           double *__restrict__ s0, double *__restrict__ s1, double *__restrict__ v, CgScalars *SC,
-           const double *Pinit, const double *Prr, const double *Pdot, double *Ptau, const double *Pext)
+           double *__restrict__ s0, double *__restrict__ s1, double *__restrict__ v, double *__restrict__ x,
+           CgScalars *SC, const double *Pinit, const double *Prr, const double *Pdot, double *Ptau,
+           const double *Pext)
 {
     const int q = cbmap[blockIdx.x];
     CgScalars *sc = SC + q;
@@ -452,7 +457,8 @@ kb_cg_spmv(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<dou
     const ZT *zl = z + I.off;
     const double *sin = ((k & 1) ? s1 : s0) + I.off;
     double *sout = ((k & 1) ? s0 : s1) + I.off;
-    double *vl = v + I.off;
+    double *vl = v + I.off, *xl = x + I.off;
+    const double alpha_prev = k ? sc->alpha[(k - 1) & 1] : 0.0;
     const int lb = blockIdx.x - I.cblk0;
     const int beg = lb * I.chunk, end = min(I.n, beg + I.chunk);
     auto sv = [&](const Cell &t) -> double { return k == 0 ? (double)zl[t.s] : (double)zl[t.s] + beta * sin[t.s]; };
@@ -464,6 +470,7 @@ kb_cg_spmv(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<dou
             double d;
             double av = apply_at<double, double>(o, c, sv, d);
             double ss = sv(c);
+            if (k) xl[s] = xl[s] + alpha_prev * sin[s];
             sout[s] = ss;
             vl[s] = av;
             acc[0] += ss * av;
@@ -473,9 +480,10 @@ kb_cg_spmv(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<dou
     if (threadIdx.x == 0) Ptau[blockIdx.x] = acc[0];
 }
 
+// Per instance: r -= alpha v and the partials of <r, r>; x += alpha s is
+// left to the next kb_cg_spmv (sc->xpend names the pending iteration).
 __global__ void __launch_bounds__(kBlock)
-kb_cg_update(const CInst *__restrict__ CI, const int *__restrict__ cbmap, const double *__restrict__ s0,
-             const double *__restrict__ s1, const double *__restrict__ v, double *__restrict__ x,
+kb_cg_update(const CInst *__restrict__ CI, const int *__restrict__ cbmap, const double *__restrict__ v,
              double *__restrict__ r, CgScalars *SC, const double *Ptau, double *Prr)
 {
     const int q = cbmap[blockIdx.x];
@@ -487,21 +495,24 @@ kb_cg_update(const CInst *__restrict__ CI, const int *__restrict__ cbmap, const 
     sum_partials<1>(Ptau + I.cblk0, I.cG, t);
     const double tau = fabs(t[0]);
     if (tau == 0.0 || !(tau == tau)) {
-        if (blockIdx.x == I.cblk0 && threadIdx.x == 0) { sc->done = 1; sc->status = HMFEM_ERR_BREAKDOWN; }
+        if (blockIdx.x == I.cblk0 && threadIdx.x == 0) {
+            sc->done = 1; sc->status = HMFEM_ERR_BREAKDOWN;
+            sc->xpend = -1;   // the previous update was applied by this iteration's SpMV
+        }
         return;
     }
     const double alpha = sc->sigma[k & 1] / tau;
-    if (blockIdx.x == I.cblk0 && threadIdx.x == 0) { sc->alpha[k & 1] = alpha; sc->tau = tau; sc->kspmv = k + 1; }
-    const double *s = ((k & 1) ? s0 : s1) + I.off;
+    if (blockIdx.x == I.cblk0 && threadIdx.x == 0) {
+        sc->alpha[k & 1] = alpha; sc->tau = tau; sc->kspmv = k + 1; sc->xpend = k;
+    }
     const double *vl = v + I.off;
-    double *xl = x + I.off, *rl = r + I.off;
+    double *rl = r + I.off;
     const int lb = blockIdx.x - I.cblk0;
     const int beg = lb * I.chunk, end = min(I.n, beg + I.chunk);
     double acc[1] = {0.0};
     for (int base = beg; base < end; base += kBlock) {
         int i = base + threadIdx.x;
         if (i < end) {
-            xl[i] += alpha * s[i];
             double ri = rl[i] - alpha * vl[i];
             rl[i] = ri;
             acc[0] += ri * ri;
@@ -509,6 +520,25 @@ kb_cg_update(const CInst *__restrict__ CI, const int *__restrict__ cbmap, const 
     }
     block_sum<1>(acc);
     if (threadIdx.x == 0) Prr[blockIdx.x] = acc[0];
+}
+
+// After the CG loop, per instance: the x += alpha s of iteration sc->xpend if
+// no SpMV applied it (the instance stopped at the SpMV that would have).
+__global__ void __launch_bounds__(kBlock)
+kb_cg_finish_x(const CInst *__restrict__ CI, const int *__restrict__ cbmap, const double *__restrict__ s0,
+               const double *__restrict__ s1, double *__restrict__ x, const CgScalars *SC)
+{
+    const int q = cbmap[blockIdx.x];
+    const CgScalars *sc = SC + q;
+    const int k = sc->xpend;
+    if (k < 0) return;
+    const CInst I = CI[q];
+    const double *s = ((k & 1) ? s0 : s1) + I.off;
+    const double alpha = sc->alpha[k & 1];
+    double *xl = x + I.off;
+    const int lb = blockIdx.x - I.cblk0;
+    const int beg = lb * I.chunk, end = min(I.n, beg + I.chunk);
+    for (int i = beg + threadIdx.x; i < end; i += kBlock) xl[i] = xl[i] + alpha * s[i];
 }
 
 // ---------------------------------------------------------------- host side
@@ -805,6 +835,7 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
             c.rtol2 = opt.rtol * opt.rtol;
             c.maxit = opt.max_iterations;
             c.stop_rule = opt.stop_rule;
+            c.xpend = -1;
         }
         memcpy(p, init.data(), sizeof(CgScalars) * B.ninst);
         HMFEM_CHECK(hipMemcpyAsync(B.sc, p, sizeof(CgScalars) * B.ninst, hipMemcpyHostToDevice, st));
@@ -866,13 +897,13 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     auto iteration = [&]() -> void * {
         if (prec == HMFEM_PREC_FP64) {
             hipLaunchKernelGGL(kb_cg_spmv<double>, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, A64,
-                               (const double *)z, B.s0, B.s1, B.v, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau, Pext);
+                               (const double *)z, B.s0, B.s1, B.v, x, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau, Pext);
         } else {
             hipLaunchKernelGGL(kb_cg_spmv<float>, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, A64,
-                               (const float *)z, B.s0, B.s1, B.v, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau, Pext);
+                               (const float *)z, B.s0, B.s1, B.v, x, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau, Pext);
         }
-        hipLaunchKernelGGL(kb_cg_update, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, B.s0, B.s1, B.v, x, B.r,
-                           B.sc, B.Ptau, B.Prr);
+        hipLaunchKernelGGL(kb_cg_update, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, B.v, B.r, B.sc, B.Ptau,
+                           B.Prr);
         return vcycle(B.v);
     };
     hipGraphExec_t exec = nullptr;
@@ -909,6 +940,11 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
         }
         if (ndone == ninst || k > opt.max_iterations + every) break;
     }
+    // the x updates that no SpMV applied (part of the solve phase)
+    hipLaunchKernelGGL(kb_cg_finish_x, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, B.s0, B.s1, x, B.sc);
+    HMFEM_CHECK(hipGetLastError());
+    HMFEM_CHECK(hipEventRecord(ev[2], st));
+    HMFEM_CHECK(hipEventSynchronize(ev[2]));
     if (exec) (void)hipGraphExecDestroy(exec);
     float t0 = 0.f, t1 = 0.f;
     (void)hipEventElapsedTime(&t0, ev[0], ev[1]);

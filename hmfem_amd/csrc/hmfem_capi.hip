@@ -79,7 +79,7 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->stop_rule = HMFEM_STOP_RESIDUAL;
     o->nu = 3;
     o->omega = 0.95;
-    o->ocf = 2.5;
+    o->ocf = 2.2;
     o->coarse_max = kCoarseMax;
     o->check_every = 4;
     o->tail_max = 512;   // 4096 (a 16^3 level in the tail, now that it fits in LDS) was slower: one
@@ -269,10 +269,10 @@ struct TuneCand {
     int tail_gamma;
 };
 static const TuneCand kTuneCands[] = {
-    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.5, 2}, {HMFEM_SMOOTH_BLOCK_GS, 2, 3, 2.5, 2},
-    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.0, 2}, {HMFEM_SMOOTH_BLOCK_GS, 2, 3, 2.0, 2},
-    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.8, 2}, {HMFEM_SMOOTH_BLOCK_GS, 4, 3, 2.5, 2},
-    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.5, 1}, {HMFEM_SMOOTH_BLOCK_GS, 3, 2, 2.5, 2},
+    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.2, 2}, {HMFEM_SMOOTH_BLOCK_GS, 2, 3, 2.2, 2},
+    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.5, 2}, {HMFEM_SMOOTH_BLOCK_GS, 2, 3, 2.0, 2},
+    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.0, 2}, {HMFEM_SMOOTH_BLOCK_GS, 4, 3, 2.2, 2},
+    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.2, 1}, {HMFEM_SMOOTH_BLOCK_GS, 3, 2, 2.2, 2},
     {HMFEM_SMOOTH_JACOBI, 3, 3, 2.0, 2},   {HMFEM_SMOOTH_JACOBI, 3, 3, 2.5, 2},
 };
 
@@ -588,7 +588,7 @@ extern "C" double hmfem_kernel_bytes_per_cell(int which, int precision)
 {
     double vs = precision == HMFEM_PREC_FP64 ? 8.0 : 4.0;
     switch (which) {
-    case 0: return 32 + 8 + vs + 8 + 8;
+    case 0: return 32 + 8 + vs + 8 + 8 + 16;   // with the x update of the previous iteration
     case 1: return 4 * vs + 8 + 2 * vs;
     case 2: return 4 * vs + 8 + vs;
     default: return 0.0;

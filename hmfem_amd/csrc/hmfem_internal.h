@@ -70,7 +70,10 @@ struct CgScalars {
     double bb, rr, sigma_first, tau, beta;
     double sigma[2], alpha[2];
     double rtol2;
-    int done, iter, maxit, stop_rule, status, kspmv, kupd, pad;
+    int done, iter, maxit, stop_rule, status, kspmv, kupd;
+    // single-grid CG: the iteration whose x += alpha s is still to be applied
+    // (the next SpMV applies it while it reads s; -1: none)
+    int xpend;
 };
 
 // A captured CG iteration, reused while its launch parameters are unchanged.

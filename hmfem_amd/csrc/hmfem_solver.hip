@@ -1006,11 +1006,15 @@ k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, d
 // also read by the neighbouring tiles, from L2), however large a plane is.
 // One partial <s, v> per logical block (Ptau[lb]; blocks past the last tile
 // write 0).
+// It also applies the previous iteration's x += alpha s_old, which
+// k_cg_update leaves to it (s_old is read here anyway): that saves reading s
+// again and a read-modify-write pass over x per iteration. If the solve stops
+// at this kernel, k_cg_finish_x applies the pending update instead.
 template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
 k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
-            double *__restrict__ v, int zlen, int G, int Gdot, CgScalars *sc, const double *Pinit, const double *Prr,
-            const double *Pdot, double *Ptau)
+            double *__restrict__ v, double *__restrict__ x, int zlen, int G, int Gdot, CgScalars *sc,
+            const double *Pinit, const double *Prr, const double *Pdot, double *Ptau)
 {
     if (sc->done) return;
     int k;
@@ -1019,6 +1023,7 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
     if (stop) return;
     const double *__restrict__ sin = (k & 1) ? s1 : s0;
     double *__restrict__ sout = (k & 1) ? s0 : s1;
+    const double alpha_prev = k ? sc->alpha[(k - 1) & 1] : 0.0;
     __shared__ double sh[3][kGsTy + 2][kGsTx + 2];
     const int nx = o.nx, ny = o.ny, nz = o.nz;
     const long nxy = (long)nx * ny;
@@ -1046,15 +1051,16 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
         // the arithmetic: raw z and s_old of plane kk + 2 and the coefficients
         // of plane kk + 1 are in flight while plane kk is computed.
         struct Raw {
-            double zo, so, zr, sr;
+            double zo, so, zr, sr, xo;
         };
         auto load_raw = [&](int kk) -> Raw {
-            Raw r{0.0, 0.0, 0.0, 0.0};
+            Raw r{0.0, 0.0, 0.0, 0.0, 0.0};
             if (kk >= 0 && kk < nz) {
                 if (col) {
                     const long q = i + (long)j * nx + (long)kk * nxy;
                     r.zo = (double)z[q];
                     if (k) r.so = sin[q];
+                    if (k && kk >= kb && kk < ke) r.xo = x[q];
                 }
                 if (ring_in) {
                     const long q = ri + (long)rj * nx + (long)kk * nxy;
@@ -1068,6 +1074,7 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
             const int slot = (kk + 3) % 3;
             sh[slot][ty + 1][tx + 1] = r.zo + beta * r.so;   // beta = 0 and s_old unread at k = 0
             if (has_ring) sh[slot][ry][rx] = r.zr + beta * r.sr;
+            if (col && k && kk >= kb && kk < ke) x[i + (long)j * nx + (long)kk * nxy] = r.xo + alpha_prev * r.so;
         };
         struct Coef {
             double e, xp, xm, yp, ym, zp;
@@ -1119,10 +1126,13 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
     if (threadIdx.x == 0) Ptau[lb] = acc[0];
 }
 
-// Blocks of k_cg_spmv_z (each writes one partial of <s, v>). The kernel
-// holds 78 VGPRs (6 waves per SIMD); measured at 128^3 / 256^3: 1024 blocks
-// 29.6 / 219 us, 512: 36.4 / 321, 1536: 30.5 / 214, 2048: 36.3 / 262 (two
-// rounds of blocks), and capping it at 64 VGPRs spills (43 / 360 us).
+// Blocks of k_cg_spmv_z (each writes one partial of <s, v>). Before it took
+// over the x update the kernel held 78 VGPRs (6 waves per SIMD) and measured,
+// at 128^3 / 256^3: 1024 blocks 29.6 / 219 us, 512: 36.4 / 321, 1536:
+// 30.5 / 214, 2048: 36.3 / 262 (two rounds of blocks); capping it at 64
+// VGPRs spilled (43 / 360 us). With the x update it holds 84 (5 waves):
+// 34.5 / 255 us, against 18.7 -> 11.6 / 158 -> 78 us saved in k_cg_update
+// (loading x at the store instead, 82 VGPRs: 35.2 / 275 us).
 constexpr int kMaxSpmvBlocks = 1024;
 
 // Planes per block of k_cg_spmv_z so that it needs at most maxb blocks (at
@@ -1142,10 +1152,13 @@ inline int spmv_z_blocks(const Dims &d, int zl)
     return grid_for((long)ntxy * ((d.nz + zl - 1) / zl), 1);
 }
 
+// r -= alpha v and the partials of <r, r>; x += alpha s too unless `defer`
+// (then the next k_cg_spmv_z, or k_cg_finish_x, applies it: sc->xpend names
+// the pending iteration).
 __global__ void __launch_bounds__(kBlock)
 k_cg_update(int n, const double *__restrict__ s0, const double *__restrict__ s1, const double *__restrict__ v,
             double *__restrict__ x, double *__restrict__ r, int chunk, int Gtau, CgScalars *sc, const double *Ptau,
-            double *Prr)
+            double *Prr, int defer)
 {
     if (sc->done) return;
     const int k = sc->kupd;
@@ -1154,11 +1167,17 @@ k_cg_update(int n, const double *__restrict__ s0, const double *__restrict__ s1,
     sum_partials<1>(Ptau, Gtau, t);
     const double tau = fabs(t[0]);
     if (tau == 0.0 || !(tau == tau)) {
-        if (xcd_block() == 0 && threadIdx.x == 0) { sc->done = 1; sc->status = HMFEM_ERR_BREAKDOWN; sc->tau = tau; }
+        if (xcd_block() == 0 && threadIdx.x == 0) {
+            sc->done = 1; sc->status = HMFEM_ERR_BREAKDOWN; sc->tau = tau;
+            sc->xpend = -1;   // the previous update was applied by this iteration's SpMV
+        }
         return;
     }
     const double alpha = sc->sigma[k & 1] / tau;
-    if (xcd_block() == 0 && threadIdx.x == 0) { sc->alpha[k & 1] = alpha; sc->tau = tau; sc->kspmv = k + 1; }
+    if (xcd_block() == 0 && threadIdx.x == 0) {
+        sc->alpha[k & 1] = alpha; sc->tau = tau; sc->kspmv = k + 1;
+        if (defer) sc->xpend = k;
+    }
     int lb, beg, end;
     chunk_range(n, chunk, lb, beg, end);
     double acc[1] = {0.0};
@@ -1168,18 +1187,38 @@ k_cg_update(int n, const double *__restrict__ s0, const double *__restrict__ s1,
         for (int q = 0; q < kUnroll; q++) {
             int i = base + q * kBlock + threadIdx.x;
             xs[q] = 0.0; ss[q] = 0.0; rs[q] = 0.0; vs[q] = 0.0;
-            if (i < end) { xs[q] = x[i]; ss[q] = s[i]; rs[q] = r[i]; vs[q] = v[i]; }
+            if (i < end) {
+                rs[q] = r[i]; vs[q] = v[i];
+                if (!defer) { xs[q] = x[i]; ss[q] = s[i]; }
+            }
         }
 #pragma unroll
         for (int q = 0; q < kUnroll; q++) {
             int i = base + q * kBlock + threadIdx.x;
             double ri = rs[q] - alpha * vs[q];
-            if (i < end) { x[i] = xs[q] + alpha * ss[q]; r[i] = ri; }
+            if (i < end) {
+                if (!defer) x[i] = xs[q] + alpha * ss[q];
+                r[i] = ri;
+            }
             acc[0] += ri * ri;
         }
     }
     block_sum<1>(acc);
     if (threadIdx.x == 0) Prr[lb] = acc[0];
+}
+
+// After the CG loop: the x += alpha s of iteration sc->xpend, if no SpMV
+// applied it (the solve stopped at the SpMV that would have).
+__global__ void __launch_bounds__(kBlock)
+k_cg_finish_x(int n, const double *__restrict__ s0, const double *__restrict__ s1, double *__restrict__ x,
+              const CgScalars *sc)
+{
+    const int k = sc->xpend;
+    if (k < 0) return;
+    const double *__restrict__ s = (k & 1) ? s0 : s1;
+    const double alpha = sc->alpha[k & 1];
+    for (long i = (long)xcd_block() * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
+        x[i] = x[i] + alpha * s[i];
 }
 
 // ---------------------------------------------------------------- host side
@@ -1634,6 +1673,7 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     init.rtol2 = opt.rtol * opt.rtol;
     init.maxit = opt.max_iterations;
     init.stop_rule = opt.stop_rule;
+    init.xpend = -1;
     HMFEM_CHECK(hipMemcpyAsync(B.sc, &init, sizeof(CgScalars), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_cg_init, dim3(G), dim3(kBlock), 0, st, o, b, x, B.r, S.ch.chunk, S.Pinit);
     // z = M r: the V-cycle plus, where clusters of strong cells were found, the
@@ -1661,12 +1701,12 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     auto iteration = [&](Prof *PP) -> T * {
         if (zlen > 0)
             PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv_z<T>, dim3(Gz), dim3(kBlock), 0, st, o, (const T *)z, B.s0,
-                        B.s1, B.v, zlen, G, Gdot, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+                        B.s1, B.v, x, zlen, G, Gdot, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
         else
             PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv<T>, dim3(G), dim3(kBlock), 0, st, o, (const T *)z, B.s0,
                         B.s1, B.v, S.ch.chunk, G, Gdot, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
         PROF_LAUNCH(PP, HMFEM_PROF_CG_UPDATE, k_cg_update, dim3(G), dim3(kBlock), 0, st, n, B.s0, B.s1, B.v, x, B.r,
-                    S.ch.chunk, zlen > 0 ? Gz : G, B.sc, S.Ptau, S.Prr);
+                    S.ch.chunk, zlen > 0 ? Gz : G, B.sc, S.Ptau, S.Prr, zlen > 0 ? 1 : 0);
         return precond(B.v, PP);
     };
     hipGraphExec_t exec = nullptr;
@@ -1707,6 +1747,11 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
         HMFEM_CHECK(hipMemcpyAsync(h_sc, B.sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
         HMFEM_CHECK(hipStreamSynchronize(st));
         if (h_sc->done || k > opt.max_iterations + every) break;
+    }
+    if (zlen > 0) {   // the x update that no SpMV applied (the solve stopped at the SpMV that would have)
+        hipLaunchKernelGGL(k_cg_finish_x, dim3(grid_for(std::min<long>(n, 4096L * kBlock), kBlock)), dim3(kBlock), 0,
+                           st, n, B.s0, B.s1, x, B.sc);
+        HMFEM_CHECK(hipGetLastError());
     }
     res.iterations = h_sc->iter;
     res.status = h_sc->done ? h_sc->status : HMFEM_NOT_CONVERGED;
@@ -1758,9 +1803,12 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
             out->precond_ratio = res.sigma_first > 0 ? sqrt(res.sigma / res.sigma_first) : 0.0;
             out->deflation_clusters = 0;
             out->deflation_cells = 0;
-            // per iteration: SpMV (A64 32 B, s in, z in, s out, v out), update
-            // (x, s, r, v in; x, r out), Jacobi (A64, r, v in; z out)
-            out->solve_bytes = (double)d.n() * ((32 + 4 * 8) + 6 * 8 + (32 + 3 * 8)) * (res.iterations + 1);
+            // per iteration: SpMV (A64 32 B, s in, z in, s out, v out; with the
+            // x update also x in and out), update (r, v in; r out; without the
+            // x update also x, s in and x out), Jacobi (A64, r, v in; z out)
+            const double xb = spmv_zlen(d) > 0 ? 16.0 : 0.0;
+            out->solve_bytes =
+                (double)d.n() * ((32 + 4 * 8 + xb) + (48 - 1.5 * xb) + (32 + 3 * 8)) * (res.iterations + 1);
         }
         return HMFEM_OK;
     }
@@ -1840,8 +1888,10 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         double vs = prec == HMFEM_PREC_FP64 ? 8.0 : 4.0;
         double c4 = 4.0 * vs;
         double n1 = (double)H.lev[H.nlev > 1 ? 1 : 0].d.n();
-        sink->bytes[HMFEM_PROF_CG_SPMV] = n0 * (32 + 8 + vs + 8 + 8);        // A64, s_in, z, s_out, v
-        sink->bytes[HMFEM_PROF_CG_UPDATE] = n0 * (6 * 8);                    // x, s, r, v in; x, r out
+        // A64, s_in, z, s_out, v (+ x in and out where the SpMV applies the x update)
+        const double xb = spmv_zlen(d) > 0 ? 16.0 : 0.0;
+        sink->bytes[HMFEM_PROF_CG_SPMV] = n0 * (32 + 8 + vs + 8 + 8 + xb);
+        sink->bytes[HMFEM_PROF_CG_UPDATE] = n0 * (48 - 1.5 * xb);           // r, v in; r out (+ x, s in; x out)
         sink->bytes[HMFEM_PROF_CG_DOTS] = 0.0;                               // fused into the last sweep
         sink->bytes[HMFEM_PROF_SMOOTH0] = n0 * (c4 + 8 + vs);                // A, f, u out
         sink->bytes[HMFEM_PROF_RESTRICT] = n0 * (c4 + 8 + vs) + n1 * vs;     // A, f, u; f_c out
@@ -1889,7 +1939,8 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
             per += nl * (sweeps * (4 * cs + fb + 2 * vs) + (4 * cs + fb + vs));
         }
         double n0 = (double)d.n();
-        per += n0 * (32 + 8 + vs + 8 + 8) + n0 * 48 + n0 * 8;
+        const double xb = spmv_zlen(d) > 0 ? 16.0 : 0.0;   // x update in the SpMV (24 B less in the update)
+        per += n0 * (32 + 8 + vs + 8 + 8 + xb) + n0 * (48 - 1.5 * xb) + n0 * 8;
         out->solve_bytes = per * (res.iterations + 1);
     }
     return HMFEM_OK;

@@ -142,7 +142,7 @@ typedef struct {
     int stop_rule;           /* default HMFEM_STOP_RESIDUAL */
     int nu;                  /* Jacobi sweeps before and after the coarse correction (default 3) */
     double omega;            /* Jacobi damping (default 0.95) */
-    double ocf;              /* coarse-correction scaling (default 2.5) */
+    double ocf;              /* coarse-correction scaling (default 2.2) */
     int coarse_max;          /* largest coarsest level, solved densely (default and maximum 80) */
     int check_every;         /* CG iterations queued between host convergence checks (default 4) */
     int tail_max;            /* levels of at most this many cells run in one single-workgroup
@@ -299,7 +299,9 @@ int hmfem_profile_read(hmfem_context *ctx, double *ms, long *count, double *byte
 void hmfem_profile_reset(hmfem_context *ctx);
 
 /* Algorithmic HBM bytes per cell of the main kernels at a precision, for the
-   roofline record: which = 0 CG SpMV+dot, 1 fused Jacobi sweep, 2 residual+restriction. */
+   roofline record: which = 0 CG SpMV+dot (with the previous iteration's x update, as on
+   grids whose xy plane has at most 1024 tiles of 32x8 cells), 1 fused Jacobi sweep,
+   2 residual+restriction. */
 double hmfem_kernel_bytes_per_cell(int which, int precision);
 
 #ifdef __cplusplus
