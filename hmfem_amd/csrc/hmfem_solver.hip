@@ -1132,7 +1132,9 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
 // 30.5 / 214, 2048: 36.3 / 262 (two rounds of blocks); capping it at 64
 // VGPRs spilled (43 / 360 us). With the x update it holds 84 (5 waves):
 // 34.5 / 255 us, against 18.7 -> 11.6 / 158 -> 78 us saved in k_cg_update
-// (loading x at the store instead, 82 VGPRs: 35.2 / 275 us).
+// (loading x at the store instead, 82 VGPRs: 35.2 / 275 us; beta and
+// alpha moved to scalar registers and z kept in FP32 until the LDS store,
+// 78 VGPRs and 6 waves: 33.9 / 282 us, so more waves are slower here).
 constexpr int kMaxSpmvBlocks = 1024;
 
 // Planes per block of k_cg_spmv_z so that it needs at most maxb blocks (at
