@@ -28,9 +28,12 @@ struct BInst {
     int tile0, ntile;                     // Gauss-Seidel tiles of this instance
 };
 
+constexpr int kBatchZlen = 16;   // planes per block of the batch's z-marching SpMV
+
 struct CInst {                            // fine level, chunked launches (CG, final sweep)
     int nx, ny, nz, n, off, cblk0, cG, chunk;
     int tpb;                              // smoother tiles per chunk block
+    int zblk0, zG, zlen;                  // z-marching SpMV: first block, blocks, planes per block
 };
 
 template <typename T>
@@ -399,18 +402,20 @@ kb_cg_init(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<dou
     if (threadIdx.x == 0) { Pinit[2 * blockIdx.x] = v[0]; Pinit[2 * blockIdx.x + 1] = v[1]; }
 }
 
-// Per instance: s = z + beta s_old, v = A s, <s, v>, and the previous
-// iteration's x += alpha s_old (s_old of the own cell is read here anyway;
-// kb_cg_update leaves it, kb_cg_finish_x applies it if the instance stops
-// at this kernel).
+// Per instance, marching along z (as the single-grid CG SpMV): a block owns
+// a 32 x 8 column tile of its instance and zlen planes, keeps s = z + beta
+// s_old of the tile and its x/y ring in LDS for three planes (loads two
+// planes ahead of the arithmetic), writes s, v = A s and one partial <s, v>,
+// and applies the previous iteration's x += alpha s_old of its cells.
+// zmap[block] names the instance; its blocks are [zblk0, zblk0 + zG).
 template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
-kb_cg_spmv(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<double> A, const ZT *__restrict__ z,
-           double *__restrict__ s0, double *__restrict__ s1, double *__restrict__ v, double *__restrict__ x,
-           CgScalars *SC, const double *Pinit, const double *Prr, const double *Pdot, double *Ptau,
-           const double *Pext)
+kb_cg_spmv_z(const CInst *__restrict__ CI, const int *__restrict__ zmap, LPtr<double> A, const ZT *__restrict__ z,
+             double *__restrict__ s0, double *__restrict__ s1, double *__restrict__ v, double *__restrict__ x,
+             CgScalars *SC, const double *Pinit, const double *Prr, const double *Pdot, double *Ptau,
+             const double *Pext)
 {
-    const int q = cbmap[blockIdx.x];
+    const int q = zmap[blockIdx.x];
     CgScalars *sc = SC + q;
     if (sc->done) return;
     const CInst I = CI[q];
@@ -443,7 +448,7 @@ kb_cg_spmv(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<dou
         if (!(sigma == sigma) || !(beta == beta)) { stop = 1; status = HMFEM_ERR_BREAKDOWN; }
         else if (!stop && k >= sc->maxit) { stop = 1; status = HMFEM_NOT_CONVERGED; }
     }
-    if (blockIdx.x == I.cblk0 && threadIdx.x == 0) {
+    if (blockIdx.x == I.zblk0 && threadIdx.x == 0) {
         if (k == 0) { sc->bb = bb; sc->sigma_first = sigma_first; }
         sc->rr = rr;
         sc->sigma[k & 1] = sigma;
@@ -453,35 +458,111 @@ kb_cg_spmv(const CInst *__restrict__ CI, const int *__restrict__ cbmap, LPtr<dou
         if (stop) { sc->done = 1; sc->status = status; }
     }
     if (stop) return;
-    const OpArgs<double> o = bop(A, I.off, I.nx, I.ny, I.nz);
-    const ZT *zl = z + I.off;
-    const double *sin = ((k & 1) ? s1 : s0) + I.off;
-    double *sout = ((k & 1) ? s0 : s1) + I.off;
-    double *vl = v + I.off, *xl = x + I.off;
     const double alpha_prev = k ? sc->alpha[(k - 1) & 1] : 0.0;
-    const int lb = blockIdx.x - I.cblk0;
-    const int beg = lb * I.chunk, end = min(I.n, beg + I.chunk);
-    auto sv = [&](const Cell &t) -> double { return k == 0 ? (double)zl[t.s] : (double)zl[t.s] + beta * sin[t.s]; };
-    double acc[1] = {0.0};
-    for (int base = beg; base < end; base += kBlock) {
-        int s = base + threadIdx.x;
-        if (s < end) {
-            Cell c = cell_at(o, s);
-            double d;
-            double av = apply_at<double, double>(o, c, sv, d);
-            double ss = sv(c);
-            if (k) xl[s] = xl[s] + alpha_prev * sin[s];
-            sout[s] = ss;
-            vl[s] = av;
-            acc[0] += ss * av;
+    const OpArgs<double> o = bop(A, I.off, I.nx, I.ny, I.nz);
+    const ZT *__restrict__ zl = z + I.off;
+    const double *__restrict__ sin = ((k & 1) ? s1 : s0) + I.off;
+    double *__restrict__ sout = ((k & 1) ? s0 : s1) + I.off;
+    double *__restrict__ vl = v + I.off;
+    double *__restrict__ xl = x + I.off;
+    __shared__ double sh[3][kGsTy + 2][kGsTx + 2];
+    const int nx = I.nx, ny = I.ny, nz = I.nz, zlen = I.zlen;
+    const long nxy = (long)nx * ny;
+    const int ntx = (nx + kGsTx - 1) / kGsTx;
+    const int ntxy = ntx * ((ny + kGsTy - 1) / kGsTy);
+    const int lb = blockIdx.x - I.zblk0;
+    const int tile = lb % ntxy, zc = lb / ntxy;
+    const int i0 = (tile % ntx) * kGsTx, j0 = (tile / ntx) * kGsTy;
+    const int kb = zc * zlen, ke = min(nz, kb + zlen);
+    const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool col = i < nx && j < ny;
+    int rx = 0, ry = 0;
+    const bool has_ring = threadIdx.x < kGsRing;
+    if (has_ring) {
+        int hz;
+        gs_ring_cell(threadIdx.x, rx, ry, hz);
+    }
+    const int ri = i0 + rx - 1, rj = j0 + ry - 1;
+    const bool ring_in = has_ring && ri >= 0 && rj >= 0 && ri < nx && rj < ny;
+    struct Raw {
+        double zo, so, zr, sr, xo;
+    };
+    auto load_raw = [&](int kk) -> Raw {
+        Raw r{0.0, 0.0, 0.0, 0.0, 0.0};
+        if (kk >= 0 && kk < nz) {
+            if (col) {
+                const long c = i + (long)j * nx + (long)kk * nxy;
+                r.zo = (double)zl[c];
+                if (k) r.so = sin[c];
+                if (k && kk >= kb && kk < ke) r.xo = xl[c];
+            }
+            if (ring_in) {
+                const long c = ri + (long)rj * nx + (long)kk * nxy;
+                r.zr = (double)zl[c];
+                if (k) r.sr = sin[c];
+            }
         }
+        return r;
+    };
+    auto store_raw = [&](int kk, const Raw &r) {
+        const int slot = (kk + 3) % 3;
+        sh[slot][ty + 1][tx + 1] = r.zo + beta * r.so;   // beta = 0 and s_old unread at k = 0
+        if (has_ring) sh[slot][ry][rx] = r.zr + beta * r.sr;
+        if (col && k && kk >= kb && kk < ke) xl[i + (long)j * nx + (long)kk * nxy] = r.xo + alpha_prev * r.so;
+    };
+    struct Coef {
+        double e, xp, xm, yp, ym, zp;
+    };
+    auto load_coef = [&](int kk) -> Coef {
+        Coef c{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (col && kk < nz) {
+            const long g = i + (long)j * nx + (long)kk * nxy;
+            c.e = o.e[g];
+            c.xp = i < nx - 1 ? o.wx[g] : 0.0;
+            c.xm = i > 0 ? o.wx[g - 1] : 0.0;
+            c.yp = j < ny - 1 ? o.wy[g] : 0.0;
+            c.ym = j > 0 ? o.wy[g - nx] : 0.0;
+            c.zp = kk < nz - 1 ? o.wz[g] : 0.0;
+        }
+        return c;
+    };
+    double acc[1] = {0.0};
+    store_raw(kb - 1, load_raw(kb - 1));
+    store_raw(kb, load_raw(kb));
+    Raw nxt = load_raw(kb + 1);
+    Coef cf = load_coef(kb);
+    double wz_lo = col && kb > 0 ? o.wz[i + (long)j * nx + (long)(kb - 1) * nxy] : 0.0;
+    for (int kk = kb; kk < ke; kk++) {
+        store_raw(kk + 1, nxt);
+        nxt = load_raw(kk + 2);
+        const Coef cn = load_coef(kk + 1);
+        __syncthreads();
+        const int c = (kk + 3) % 3, lo = (kk + 2) % 3, hi = (kk + 1) % 3;
+        if (col) {
+            const long g = i + (long)j * nx + (long)kk * nxy;
+            const double us = sh[c][ty + 1][tx + 1];
+            double a = cf.e * us;
+            a += cf.xp * (us - sh[c][ty + 1][tx + 2]);
+            a += cf.xm * (us - sh[c][ty + 1][tx]);
+            a += cf.yp * (us - sh[c][ty + 2][tx + 1]);
+            a += cf.ym * (us - sh[c][ty][tx + 1]);
+            a += cf.zp * (us - sh[hi][ty + 1][tx + 1]);
+            a += wz_lo * (us - sh[lo][ty + 1][tx + 1]);
+            wz_lo = cf.zp;
+            sout[g] = us;
+            vl[g] = a;
+            acc[0] += us * a;
+        }
+        cf = cn;
+        __syncthreads();
     }
     block_sum<1>(acc);
     if (threadIdx.x == 0) Ptau[blockIdx.x] = acc[0];
 }
 
 // Per instance: r -= alpha v and the partials of <r, r>; x += alpha s is
-// left to the next kb_cg_spmv (sc->xpend names the pending iteration).
+// left to the next kb_cg_spmv_z (sc->xpend names the pending iteration).
 __global__ void __launch_bounds__(kBlock)
 kb_cg_update(const CInst *__restrict__ CI, const int *__restrict__ cbmap, const double *__restrict__ v,
              double *__restrict__ r, CgScalars *SC, const double *Ptau, double *Prr)
@@ -492,7 +573,7 @@ kb_cg_update(const CInst *__restrict__ CI, const int *__restrict__ cbmap, const 
     const CInst I = CI[q];
     const int k = sc->kupd;
     double t[1];
-    sum_partials<1>(Ptau + I.cblk0, I.cG, t);
+    sum_partials<1>(Ptau + I.zblk0, I.zG, t);   // one partial per block of kb_cg_spmv_z
     const double tau = fabs(t[0]);
     if (tau == 0.0 || !(tau == tau)) {
         if (blockIdx.x == I.cblk0 && threadIdx.x == 0) {
@@ -567,6 +648,8 @@ struct Batch {
     CInst *dch = nullptr;
     int *cbmap = nullptr;
     int ncblk = 0;
+    int *zmap = nullptr;       // kb_cg_spmv_z block -> instance
+    int nzblk = 0;
     double *inv = nullptr;
     int *info = nullptr;
     double *Pinit = nullptr, *Ptau = nullptr, *Prr = nullptr, *Pdot = nullptr;
@@ -654,6 +737,19 @@ void plan_batch(int ninst, const int *dims, int coarse_max, Batch &B)
         cb += G;
     }
     B.ncblk = cb;
+    // z-marching SpMV: 32 x 8 column tiles, kBatchZlen planes per block (at
+    // 96^3, 216 blocks of 16 planes)
+    int zb = 0;
+    for (int q = 0; q < ninst; q++) {
+        CInst &c = B.ch[q];
+        const int ntxy = ((c.nx + kGsTx - 1) / kGsTx) * ((c.ny + kGsTy - 1) / kGsTy);
+        c.zlen = std::min(c.nz, kBatchZlen);   // longer where that keeps the partials at most kMaxReduceBlocks
+        while (c.zlen < c.nz && (long)ntxy * ((c.nz + c.zlen - 1) / c.zlen) > kMaxReduceBlocks) c.zlen++;
+        c.zblk0 = zb;
+        c.zG = ntxy * ((c.nz + c.zlen - 1) / c.zlen);
+        zb += c.zG;
+    }
+    B.nzblk = zb;
 }
 
 size_t batch_bytes(const Batch &B, int prec)
@@ -668,7 +764,7 @@ size_t batch_bytes(const Batch &B, int prec)
     }
     size_t n0 = B.lev[0].ncell;
     b += 4 * al(n0 * 8);                                      // r, s0, s1, v
-    b += al(sizeof(CInst) * B.ninst) + al(sizeof(int) * B.ncblk);
+    b += al(sizeof(CInst) * B.ninst) + al(sizeof(int) * B.ncblk) + al(sizeof(int) * B.nzblk) + al((size_t)B.nzblk * 8);
     b += al((size_t)B.ninst * kCoarseMax * kCoarseMax * 8) + al(sizeof(int) * B.ninst);
     b += 6 * al((size_t)B.ncblk * 8) + al(sizeof(CgScalars) * B.ninst);
     return b + 65536;
@@ -784,22 +880,23 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     B.r = cv.take<double>(n0); B.s0 = cv.take<double>(n0); B.s1 = cv.take<double>(n0); B.v = cv.take<double>(n0);
     B.dch = cv.take<CInst>(B.ninst);
     B.cbmap = cv.take<int>(B.ncblk);
+    B.zmap = cv.take<int>(B.nzblk);
     B.inv = cv.take<double>((size_t)B.ninst * kCoarseMax * kCoarseMax);
     B.info = cv.take<int>(B.ninst);
     B.Pinit = cv.take<double>(2 * (size_t)B.ncblk);
-    B.Ptau = cv.take<double>(B.ncblk);
+    B.Ptau = cv.take<double>(B.nzblk);
     B.Prr = cv.take<double>(B.ncblk);
     B.Pdot = cv.take<double>(2 * (size_t)B.ncblk);
     B.sc = cv.take<CgScalars>(B.ninst);
     B.never = cv.take<int>(1);
-    if (!B.sc || !B.Pdot || !B.inv || !B.never) return HMFEM_ERR_NOMEM;
+    if (!B.sc || !B.Pdot || !B.inv || !B.never || !B.zmap || !B.Ptau) return HMFEM_ERR_NOMEM;
 
     // descriptors
     {
         size_t total = 0;
         for (int l = 0; l < B.nlev; l++)
             total += sizeof(BInst) * B.ninst + sizeof(int) * (B.lev[l].nblk + B.lev[l].ntiles);
-        total += sizeof(CInst) * B.ninst + sizeof(int) * B.ncblk + sizeof(CgScalars) * B.ninst;
+        total += sizeof(CInst) * B.ninst + sizeof(int) * (B.ncblk + B.nzblk) + sizeof(CgScalars) * B.ninst;
         stage.resize(total);
         char *p = stage.data();
         for (int l = 0; l < B.nlev; l++) {
@@ -829,6 +926,12 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
         memcpy(p, cbm.data(), sizeof(int) * B.ncblk);
         HMFEM_CHECK(hipMemcpyAsync(B.cbmap, p, sizeof(int) * B.ncblk, hipMemcpyHostToDevice, st));
         p += sizeof(int) * B.ncblk;
+        std::vector<int> zbm(B.nzblk);
+        for (int q = 0; q < B.ninst; q++)
+            for (int k = 0; k < B.ch[q].zG; k++) zbm[B.ch[q].zblk0 + k] = q;
+        memcpy(p, zbm.data(), sizeof(int) * B.nzblk);
+        HMFEM_CHECK(hipMemcpyAsync(B.zmap, p, sizeof(int) * B.nzblk, hipMemcpyHostToDevice, st));
+        p += sizeof(int) * B.nzblk;
         std::vector<CgScalars> init(B.ninst);
         for (auto &c : init) {
             memset(&c, 0, sizeof(c));
@@ -896,10 +999,10 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     void *z = vcycle(nullptr);
     auto iteration = [&]() -> void * {
         if (prec == HMFEM_PREC_FP64) {
-            hipLaunchKernelGGL(kb_cg_spmv<double>, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, A64,
+            hipLaunchKernelGGL(kb_cg_spmv_z<double>, dim3(B.nzblk), dim3(kBlock), 0, st, B.dch, B.zmap, A64,
                                (const double *)z, B.s0, B.s1, B.v, x, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau, Pext);
         } else {
-            hipLaunchKernelGGL(kb_cg_spmv<float>, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, A64,
+            hipLaunchKernelGGL(kb_cg_spmv_z<float>, dim3(B.nzblk), dim3(kBlock), 0, st, B.dch, B.zmap, A64,
                                (const float *)z, B.s0, B.s1, B.v, x, B.sc, B.Pinit, B.Prr, B.Pdot, B.Ptau, Pext);
         }
         hipLaunchKernelGGL(kb_cg_update, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, B.v, B.r, B.sc, B.Ptau,
