@@ -26,6 +26,7 @@ struct BInst {
     int nx, ny, nz, n, off, blk0, nblk;   // this level
     int fx, fy, fz;                       // aggregation factors to the next level
     int tile0, ntile;                     // Gauss-Seidel tiles of this instance
+    int zblk0, zG, zlen;                  // kb_restrict_z blocks (zlen 0: not 2x2x2, kb_resid_restrict)
 };
 
 constexpr int kBatchZlen = 16;   // planes per block of the batch's z-marching SpMV
@@ -104,7 +105,7 @@ kb_resid_restrict(const BInst *__restrict__ D, const BInst *__restrict__ Dc, con
 {
     using C = T;
     const int q = bmapc[blockIdx.x];
-    if (sc[q].done) return;
+    if (sc[q].done || D[q].zlen > 0) return;   // 2x2x2 instances: kb_restrict_z
     const BInst Ic = Dc[q];
     const int I = (blockIdx.x - Ic.blk0) * kBlock + threadIdx.x;
     if (I >= Ic.n) return;
@@ -136,6 +137,92 @@ kb_resid_restrict(const BInst *__restrict__ D, const BInst *__restrict__ Dc, con
         }
     }
     fc[Ic.off + I] = acc;
+}
+
+// Residual and restriction f_c = P^T (f - A u) of the instances that
+// aggregate 2x2x2 boxes, marching along z as the single-grid kernel: a block
+// owns a 32 x 8 column tile of its instance and zlen (even) planes, keeps its
+// column's u at k-1, k, k+1 in registers and the plane with its x/y ring in a
+// double-buffered LDS plane, sums each box over two planes in a register and
+// over x and y by lane shuffles. zmap[block] names the instance.
+template <typename T, typename FT>
+__global__ void __launch_bounds__(kBlock)
+kb_restrict_z(const BInst *__restrict__ D, const BInst *__restrict__ Dc, const int *__restrict__ zmap, LPtr<T> A,
+              const FT *__restrict__ f, const T *__restrict__ u, T *__restrict__ fc, const CgScalars *sc)
+{
+    const int q = zmap[blockIdx.x];
+    if (sc[q].done) return;
+    const BInst If = D[q], Ic = Dc[q];
+    const OpArgs<T> o = bop(A, If.off, If.nx, If.ny, If.nz);
+    const FT *__restrict__ fl = f + If.off;
+    const T *__restrict__ ul = u + If.off;
+    T *__restrict__ fcl = fc + Ic.off;
+    const int cx = Ic.nx, cy = Ic.ny, cz = Ic.nz, zlen = If.zlen;
+    __shared__ T sh[2][kGsTy + 2][kGsTx + 2];
+    const int nx = o.nx, ny = o.ny, nz = o.nz;
+    const long nxy = (long)nx * ny;
+    const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy;
+    const int ntxy = ntx * nty;
+    const int lb = blockIdx.x - If.zblk0;
+    const int tile = lb % ntxy, zc = lb / ntxy;
+    const int i0 = (tile % ntx) * kGsTx, j0 = (tile / ntx) * kGsTy;
+    const int kb = zc * zlen, ke = min(nz, kb + zlen);
+    const int tx = threadIdx.x % kGsTx, ty = threadIdx.x / kGsTx;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool col = i < nx && j < ny;
+    int rx = 0, ry = 0;
+    const bool has_ring = threadIdx.x < kGsRing;
+    if (has_ring) {
+        int hz;
+        gs_ring_cell(threadIdx.x, rx, ry, hz);
+    }
+    const int ri = i0 + rx - 1, rj = j0 + ry - 1;
+    const bool ring_in = has_ring && ri >= 0 && rj >= 0 && ri < nx && rj < ny;
+    const long cs = i + (long)j * nx, rs = ri + (long)rj * nx;
+    T um = (T)0, uc = (T)0, up = (T)0, wzm = (T)0;
+    if (col) {
+        if (kb > 0) { um = ul[cs + (kb - 1) * nxy]; wzm = o.wz[cs + (kb - 1) * nxy]; }
+        uc = ul[cs + kb * nxy];
+        if (kb + 1 < nz) up = ul[cs + (kb + 1) * nxy];
+    }
+    T acc = (T)0;
+    for (int kk = kb; kk < ke; kk++) {
+        const int bf = kk & 1;
+        sh[bf][ty + 1][tx + 1] = uc;
+        if (has_ring) sh[bf][ry][rx] = ring_in ? ul[rs + kk * nxy] : (T)0;
+        T r = (T)0, e = (T)0, xp = (T)0, xm = (T)0, yp = (T)0, ym = (T)0, zp = (T)0, fq = (T)0, un = (T)0;
+        if (col) {
+            const long g = cs + kk * nxy;
+            e = o.e[g];
+            if (i < nx - 1) xp = o.wx[g];
+            if (i > 0) xm = o.wx[g - 1];
+            if (j < ny - 1) yp = o.wy[g];
+            if (j > 0) ym = o.wy[g - nx];
+            if (kk < nz - 1) zp = o.wz[g];
+            fq = (T)fl[g];
+            if (kk + 2 < nz) un = ul[g + 2 * nxy];
+        }
+        __syncthreads();
+        if (col) {
+            T a = e * uc;
+            a += xp * (uc - sh[bf][ty + 1][tx + 2]);
+            a += xm * (uc - sh[bf][ty + 1][tx]);
+            a += yp * (uc - sh[bf][ty + 2][tx + 1]);
+            a += ym * (uc - sh[bf][ty][tx + 1]);
+            a += zp * (uc - up);
+            a += wzm * (uc - um);
+            r = fq - a;
+            um = uc; uc = up; up = un; wzm = zp;
+        }
+        acc += r;
+        if (((kk - kb) & 1) || kk == ke - 1) {   // block-uniform
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 32);
+            const int ci = i >> 1, cj = j >> 1, ck = kk >> 1;
+            if (!(tx & 1) && !(ty & 1) && ci < cx && cj < cy && ck < cz) fcl[ci + (long)cx * (cj + (long)cy * ck)] = acc;
+            acc = (T)0;
+        }
+    }
 }
 
 template <typename T, typename FT>
@@ -630,6 +717,8 @@ struct BLevel {
     std::vector<BInst> h;
     BInst *d = nullptr;
     int *bmap = nullptr;
+    int *zmap = nullptr;      // kb_restrict_z block -> instance
+    int nzblk = 0, nold = 0;  // its blocks; instances left to kb_resid_restrict
     int nblk = 0;
     long ncell = 0;
     int *tmap = nullptr;      // smoother tile -> instance
@@ -720,6 +809,26 @@ void plan_batch(int ninst, const int *dims, int coarse_max, Batch &B)
         L.ncell = off;
         L.nblk = blk;
     }
+    // residual-restriction of the 2x2x2 instances of each level: z-marching
+    // blocks of 32 x 8 columns and an even number of planes (at most 16)
+    for (int l = 0; l < depth; l++) {
+        BLevel &L = B.lev[l];
+        int zb = 0, nold = 0;
+        for (int q = 0; q < ninst; q++) {
+            BInst &I = L.h[q];
+            I.zblk0 = zb; I.zG = 0; I.zlen = 0;
+            if (l + 1 >= depth) continue;
+            if (I.fx == 2 && I.fy == 2 && I.fz == 2) {
+                I.zlen = std::min(16, (I.nz + 1) & ~1);
+                I.zG = ((I.nx + kGsTx - 1) / kGsTx) * ((I.ny + kGsTy - 1) / kGsTy) * ((I.nz + I.zlen - 1) / I.zlen);
+                zb += I.zG;
+            } else {
+                nold++;
+            }
+        }
+        L.nzblk = zb;
+        L.nold = nold;
+    }
     // chunked layout of the fine level
     B.ch.assign(ninst, CInst{});
     int cb = 0;
@@ -760,7 +869,8 @@ size_t batch_bytes(const Batch &B, int prec)
         if (l > 0) b += 4 * al(n * 8);
         if (vs == 8) b += al(n * 8);
         b += 5 * al(n * 4) + 3 * al(n * vs);
-        b += al(sizeof(BInst) * B.ninst) + al(sizeof(int) * B.lev[l].nblk) + al(sizeof(int) * B.lev[l].ntiles);
+        b += al(sizeof(BInst) * B.ninst) + al(sizeof(int) * B.lev[l].nblk) + al(sizeof(int) * B.lev[l].ntiles) +
+             al(sizeof(int) * B.lev[l].nzblk);
     }
     size_t n0 = B.lev[0].ncell;
     b += 4 * al(n0 * 8);                                      // r, s0, s1, v
@@ -783,6 +893,20 @@ struct BCycle {
     int smoother, gs;
 };
 
+// f_c = P^T (f - A u) from level L to C: the z-marching kernel for the 2x2x2
+// instances, the per-coarse-cell kernel for the others (if any).
+template <typename T, typename FT>
+void launch_restrict(hipStream_t st, const BLevel &L, const BLevel &C, LPtr<T> A, const FT *f, const T *u,
+                     const CgScalars *sc)
+{
+    if (L.nzblk)
+        hipLaunchKernelGGL((kb_restrict_z<T, FT>), dim3(L.nzblk), dim3(kBlock), 0, st, L.d, C.d, L.zmap, A, f, u,
+                           (T *)C.f, sc);
+    if (L.nold)
+        hipLaunchKernelGGL((kb_resid_restrict<T, FT>), dim3(C.nblk), dim3(kBlock), 0, st, L.d, C.d, C.bmap, A, f, u,
+                           (T *)C.f, sc);
+}
+
 template <typename T, typename FT>
 T *vb_level(hipStream_t st, Batch &B, int l, const FT *f, const double *vdot, const BCycle &cp, double *Pdot)
 {
@@ -798,8 +922,7 @@ T *vb_level(hipStream_t st, Batch &B, int l, const FT *f, const double *vdot, co
         BLevel &C = B.lev[l + 1];
         hipLaunchKernelGGL((kb_gs_tile<T, FT, 0>), dim3(L.ntiles), dim3(kGsThreads), 0, st, L.d, C.d, L.tmap, A,
                            cp.gs, f, (const T *)nullptr, u, (const T *)nullptr, (T)0, B.sc);
-        hipLaunchKernelGGL((kb_resid_restrict<T, FT>), dim3(C.nblk), dim3(kBlock), 0, st, L.d, C.d, C.bmap, A, f, u,
-                           (T *)C.f, B.sc);
+        launch_restrict<T, FT>(st, L, C, A, f, u, B.sc);
         T *uc = vb_level<T, T>(st, B, l + 1, (const T *)C.f, nullptr, cp, nullptr);
         if (l == 0) {
             hipLaunchKernelGGL(kb_gs_tile_dots<T>, dim3(B.ncblk), dim3(kGsThreads), 0, st, B.dch, B.cbmap, L.d, C.d,
@@ -820,8 +943,7 @@ T *vb_level(hipStream_t st, Batch &B, int l, const FT *f, const double *vdot, co
         std::swap(u, t);
     }
     BLevel &C = B.lev[l + 1];
-    hipLaunchKernelGGL((kb_resid_restrict<T, FT>), dim3(C.nblk), dim3(kBlock), 0, st, L.d, C.d, C.bmap, A, f, u,
-                       (T *)C.f, B.sc);
+    launch_restrict<T, FT>(st, L, C, A, f, u, B.sc);
     T *uc = vb_level<T, T>(st, B, l + 1, (const T *)C.f, nullptr, cp, nullptr);
     hipLaunchKernelGGL((kb_prolong_jacobi<T, FT>), dim3(L.nblk), dim3(kBlock), 0, st, L.d, C.d, L.bmap, A,
                        (const T *)uc, (T)cp.ocf, f, u, t, om, B.sc);
@@ -874,7 +996,8 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
         L.d = cv.take<BInst>(B.ninst);
         L.bmap = cv.take<int>(L.nblk);
         L.tmap = cv.take<int>(L.ntiles);
-        if (!L.bmap || !L.tmap || !L.t || !L.di32) return HMFEM_ERR_NOMEM;
+        L.zmap = L.nzblk ? cv.take<int>(L.nzblk) : nullptr;
+        if (!L.bmap || !L.tmap || !L.t || !L.di32 || (L.nzblk && !L.zmap)) return HMFEM_ERR_NOMEM;
     }
     size_t n0 = B.lev[0].ncell;
     B.r = cv.take<double>(n0); B.s0 = cv.take<double>(n0); B.s1 = cv.take<double>(n0); B.v = cv.take<double>(n0);
@@ -895,7 +1018,7 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     {
         size_t total = 0;
         for (int l = 0; l < B.nlev; l++)
-            total += sizeof(BInst) * B.ninst + sizeof(int) * (B.lev[l].nblk + B.lev[l].ntiles);
+            total += sizeof(BInst) * B.ninst + sizeof(int) * (B.lev[l].nblk + B.lev[l].ntiles + B.lev[l].nzblk);
         total += sizeof(CInst) * B.ninst + sizeof(int) * (B.ncblk + B.nzblk) + sizeof(CgScalars) * B.ninst;
         stage.resize(total);
         char *p = stage.data();
@@ -916,6 +1039,14 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
             memcpy(p, tm.data(), sizeof(int) * L.ntiles);
             HMFEM_CHECK(hipMemcpyAsync(L.tmap, p, sizeof(int) * L.ntiles, hipMemcpyHostToDevice, st));
             p += sizeof(int) * L.ntiles;
+            if (L.nzblk) {
+                std::vector<int> zm(L.nzblk);
+                for (int q = 0; q < B.ninst; q++)
+                    for (int k = 0; k < L.h[q].zG; k++) zm[L.h[q].zblk0 + k] = q;
+                memcpy(p, zm.data(), sizeof(int) * L.nzblk);
+                HMFEM_CHECK(hipMemcpyAsync(L.zmap, p, sizeof(int) * L.nzblk, hipMemcpyHostToDevice, st));
+                p += sizeof(int) * L.nzblk;
+            }
         }
         std::vector<int> cbm(B.ncblk);
         for (int q = 0; q < B.ninst; q++)
