@@ -650,3 +650,37 @@ def test_batch_stopped_returns_last_iterate(ctx, torch):
             tr = true_residual(operm, oind, ob, xb[off:off + P.n])
             assert abs(tr - rel) <= 1e-6 * rel + 1e-14, (m, q, it, tr, rel)
             off += P.n
+
+
+def test_wide_plane_fallbacks(ctx, torch):
+    """A grid whose xy plane has more than 1024 tiles of 32 x 8 cells (1056 x
+    264: 33 x 33 tiles) takes the cell-per-thread CG SpMV, which leaves the x
+    update in the update kernel; in a batch the same grid gets the z-marching
+    SpMV with more than 1024 partials for the instance. The single solve
+    reaches a true relative residual below 1e-11, a solve cut off at 3
+    iterations returns the iterate whose residual it reports, and the batch
+    reproduces the single solution."""
+    P = make_problem((1056, 264, 3), "lognormal", seed=4, corr=4.0)
+    Q = make_problem((16, 16, 16), "karst", seed=5, conduits=2)
+    _say("wide plane", P.nx, P.ny, P.nz)
+    operm, oind, ob = O.assemble(P)
+    D = Dev(P, torch)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-12))
+    assert st.status == S.OK, st.as_dict()
+    x = p.cpu().numpy()
+    assert true_residual(operm, oind, ob, x) < 1e-11
+    p.zero_()
+    s3 = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-15, max_iterations=3))
+    assert s3.status == S.NOT_CONVERGED and s3.iterations == 3, s3.as_dict()
+    tr = true_residual(operm, oind, ob, p.cpu().numpy())
+    assert abs(tr - s3.rel_residual) <= 1e-6 * s3.rel_residual, (tr, s3.rel_residual)
+    K = torch.from_numpy(np.concatenate([P.Kx, Q.Kx])).to("cuda")
+    big = max(max(R.nx * R.ny, R.ny * R.nz, R.nx * R.nz) for R in (P, Q))
+    bval = [torch.full((big,), 1.0, dtype=torch.float64, device="cuda")] + \
+           [torch.zeros(big, dtype=torch.float64, device="cuda") for _ in range(5)]
+    pb = torch.zeros(P.n + Q.n, dtype=torch.float64, device="cuda")
+    sb, per = ctx.reservoir_batch([(P.nx, P.ny, P.nz), (Q.nx, Q.ny, Q.nz)], K, K, K, bval, pb, None,
+                                  S.default_options(rtol=1e-12))
+    assert sb.status == S.OK, (sb.as_dict(), per)
+    assert rel_l2(pb.cpu().numpy()[:P.n], x) < 1e-9
