@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 namespace hmfem {
@@ -469,44 +470,65 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     // strong neighbours of each strong cell, inside its instance: ordinals of
     // the +x, +y, +z neighbours by one monotone sweep per direction (cell[]
     // is sorted and a neighbour stays inside the instance), the -x, -y, -z
-    // ones by symmetry
+    // ones by symmetry; then a union-find over the +x, +y, +z links. Both stay
+    // inside an instance, so ranges of whole instances (contiguous in the
+    // sorted list) run on host threads of their own. The union keeps the
+    // smaller ordinal as the root, so every root is its component's smallest
+    // member whatever the order of the unions.
     auto &nbr = W.nbr;
     auto &up = W.up;
-    nbr.assign(6L * ms, -1);   // [a][-x,+x,-y,+y,-z,+z]
-    up.resize(ms);             // bit f: the +f neighbour is inside the instance
-    for (int a = 0; a < ms; a++) {
-        const DefInst &in = inst[cin[a]];
-        const int l = cell[a] - in.off, i = l % in.nx, jk = l / in.nx, j = jk % in.ny, k = jk / in.ny;
-        up[a] = (unsigned char)((i + 1 < in.nx) | ((j + 1 < in.ny) << 1) | ((k + 1 < in.nz) << 2));
-    }
-    for (int f = 0; f < 3; f++) {
-        int p = 0;
-        for (int a = 0; a < ms; a++) {
-            if (!((up[a] >> f) & 1)) continue;
+    auto &par = W.par;
+    auto &root = W.root, &size = W.size;
+    nbr.resize(6L * ms);   // [a][-x,+x,-y,+y,-z,+z]
+    up.resize(ms);         // bit f: the +f neighbour is inside the instance
+    par.resize(ms);
+    root.resize(ms);
+    size.resize(ms);
+    auto label = [&](int a0, int a1) {
+        std::fill(nbr.begin() + 6L * a0, nbr.begin() + 6L * a1, -1);
+        for (int a = a0; a < a1; a++) {
             const DefInst &in = inst[cin[a]];
-            const long t = cell[a] + (f == 0 ? 1 : f == 1 ? (long)in.nx : (long)in.nx * in.ny);
-            while (p < ms && cell[p] < t) p++;
-            if (p < ms && cell[p] == t) {
-                nbr[6L * a + 2 * f + 1] = p;
-                nbr[6L * p + 2 * f] = a;
+            const int l = cell[a] - in.off, i = l % in.nx, jk = l / in.nx, j = jk % in.ny, k = jk / in.ny;
+            up[a] = (unsigned char)((i + 1 < in.nx) | ((j + 1 < in.ny) << 1) | ((k + 1 < in.nz) << 2));
+        }
+        for (int f = 0; f < 3; f++) {
+            int p = a0;
+            for (int a = a0; a < a1; a++) {
+                if (!((up[a] >> f) & 1)) continue;
+                const DefInst &in = inst[cin[a]];
+                const long t = cell[a] + (f == 0 ? 1 : f == 1 ? (long)in.nx : (long)in.nx * in.ny);
+                while (p < a1 && cell[p] < t) p++;
+                if (p < a1 && cell[p] == t) {
+                    nbr[6L * a + 2 * f + 1] = p;
+                    nbr[6L * p + 2 * f] = a;
+                }
             }
         }
+        std::iota(par.begin() + a0, par.begin() + a1, a0);
+        for (int a = a0; a < a1; a++)
+            for (int f = 1; f < 6; f += 2) {
+                const int b = nbr[6L * a + f];
+                if (b < 0) continue;
+                int ra = find_root(par, a), rb = find_root(par, b);
+                if (ra != rb) par[std::max(ra, rb)] = std::min(ra, rb);
+            }
+        std::fill(size.begin() + a0, size.begin() + a1, 0);
+        for (int a = a0; a < a1; a++) { root[a] = find_root(par, a); size[root[a]]++; }
+    };
+    {
+        // ranges of whole instances with about ms / nt strong cells each
+        const int nt = ms < 1024 || ninst < 2
+                           ? 1
+                           : (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+        std::vector<int> cut(1, 0);
+        for (int a = 1; a < ms && (int)cut.size() < nt; a++)
+            if (cin[a] != cin[a - 1] && (long)a * nt >= (long)ms * (long)cut.size()) cut.push_back(a);
+        cut.push_back(ms);
+        std::vector<std::thread> th;
+        for (size_t t = 1; t + 1 < cut.size(); t++) th.emplace_back(label, cut[t], cut[t + 1]);
+        label(cut[0], cut[1]);
+        for (auto &x : th) x.join();
     }
-    // union-find over the +x, +y, +z links
-    auto &par = W.par;
-    par.resize(ms);
-    std::iota(par.begin(), par.end(), 0);
-    for (int a = 0; a < ms; a++)
-        for (int f = 1; f < 6; f += 2) {
-            const int b = nbr[6L * a + f];
-            if (b < 0) continue;
-            int ra = find_root(par, a), rb = find_root(par, b);
-            if (ra != rb) par[std::max(ra, rb)] = std::min(ra, rb);
-        }
-    auto &root = W.root, &size = W.size;
-    root.resize(ms);
-    size.assign(ms, 0);
-    for (int a = 0; a < ms; a++) { root[a] = find_root(par, a); size[root[a]]++; }
     auto &roots = W.roots;
     roots.clear();
     for (int a = 0; a < ms; a++)
