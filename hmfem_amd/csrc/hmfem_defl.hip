@@ -525,7 +525,13 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
             if (cin[a] != cin[a - 1] && (long)a * nt >= (long)ms * (long)cut.size()) cut.push_back(a);
         cut.push_back(ms);
         std::vector<std::thread> th;
-        for (size_t t = 1; t + 1 < cut.size(); t++) th.emplace_back(label, cut[t], cut[t + 1]);
+        for (size_t t = 1; t + 1 < cut.size(); t++) {
+            try {
+                th.emplace_back(label, cut[t], cut[t + 1]);
+            } catch (...) {   // no thread to be had: this range on the calling thread
+                label(cut[t], cut[t + 1]);
+            }
+        }
         label(cut[0], cut[1]);
         for (auto &x : th) x.join();
     }

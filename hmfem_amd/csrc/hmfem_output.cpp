@@ -71,7 +71,11 @@ std::vector<std::string> slabs(int count, F body)
     std::vector<std::thread> th;
     for (int t = 0; t < nt; t++) {
         const int lo = (int)((long)count * t / nt), hi = (int)((long)count * (t + 1) / nt);
-        th.emplace_back([&, t, lo, hi] { body(lo, hi, out[t]); });
+        try {
+            th.emplace_back([&, t, lo, hi] { body(lo, hi, out[t]); });
+        } catch (...) {   // no thread to be had: this slab on the calling thread
+            body(lo, hi, out[t]);
+        }
     }
     for (auto &x : th) x.join();
     return out;
