@@ -25,6 +25,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -629,65 +630,95 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
 
     // E = Z^T A Z in a fixed order: E_aa = sum (e + couplings leaving the
     // cluster), E_ab = - couplings between clusters a and b (block diagonal
-    // over the instances)
+    // over the instances); its inverse via Cholesky E = L L^T (E is
+    // symmetric positive definite), block by block. An instance's clusters
+    // and their cells are contiguous in the lists, so ranges of whole
+    // instances run on host threads of their own (each writes only its own
+    // blocks of E and of the inverse).
     auto &E = W.E;
+    auto &inv = W.inv;
     E.assign((size_t)nc * nc, 0.0);
-    for (int q = 0; q < m; q++) {
-        const int a = lcomp[q];
-        const double *ws = &w[7L * q];
-        const int *nb = &nbr[6L * lord[q]];
-        double daa = ws[0];
-        for (int f = 0; f < 6; f++) {
-            const double wf = ws[1 + f];
-            if (wf == 0.0) continue;
-            const int b = nb[f] < 0 ? -1 : cid[nb[f]];
-            if (b == a) continue;
-            daa += wf;
-            if (b >= 0) E[(size_t)a * nc + b] -= wf;
-        }
-        E[(size_t)a * nc + a] += daa;
-    }
-    // inverse via Cholesky E = L L^T (E is symmetric positive definite),
-    // block by block: E has no coupling between clusters of different
-    // instances
-    auto &inv = W.inv, &L = W.L, &y = W.y, &xb = W.xb;
     inv.assign((size_t)nc * nc, 0.0);
-    for (int q = 0; q < ninst; q++) {
-        const int lo = irange[q], nb = irange[q + 1] - lo;
-        if (nb == 0) continue;
-        auto Eb = [&](int i, int j) { return E[(size_t)(lo + i) * nc + lo + j]; };
-        L.resize((size_t)nb * nb);
-        for (int i = 0; i < nb; i++)
-            for (int j = 0; j < nb; j++) L[(size_t)i * nb + j] = Eb(i, j);
-        for (int j = 0; j < nb; j++) {
-            double djj = L[(size_t)j * nb + j];
-            for (int k = 0; k < j; k++) djj -= L[(size_t)j * nb + k] * L[(size_t)j * nb + k];
-            if (!(djj > 0)) return HMFEM_OK;   // not positive definite: leave the correction off
-            djj = sqrt(djj);
-            L[(size_t)j * nb + j] = djj;
-            for (int i = j + 1; i < nb; i++) {
-                double t = L[(size_t)i * nb + j];
-                for (int k = 0; k < j; k++) t -= L[(size_t)i * nb + k] * L[(size_t)j * nb + k];
-                L[(size_t)i * nb + j] = t / djj;
+    std::atomic<int> not_pd{0};
+    auto build_invert = [&](int q0, int q1) {
+        std::vector<double> L, y, xb;
+        for (int q = q0; q < q1; q++) {
+            const int lo = irange[q], nb = irange[q + 1] - lo;
+            if (nb == 0) continue;
+            for (int c = ptr[lo]; c < ptr[lo + nb]; c++) {
+                const int a = lcomp[c];
+                const double *ws = &w[7L * c];
+                const int *nbp = &nbr[6L * lord[c]];
+                double daa = ws[0];
+                for (int f = 0; f < 6; f++) {
+                    const double wf = ws[1 + f];
+                    if (wf == 0.0) continue;
+                    const int b = nbp[f] < 0 ? -1 : cid[nbp[f]];
+                    if (b == a) continue;
+                    daa += wf;
+                    if (b >= 0) E[(size_t)a * nc + b] -= wf;
+                }
+                E[(size_t)a * nc + a] += daa;
+            }
+            auto Eb = [&](int i, int j) { return E[(size_t)(lo + i) * nc + lo + j]; };
+            L.resize((size_t)nb * nb);
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) L[(size_t)i * nb + j] = Eb(i, j);
+            for (int j = 0; j < nb; j++) {
+                double djj = L[(size_t)j * nb + j];
+                for (int k = 0; k < j; k++) djj -= L[(size_t)j * nb + k] * L[(size_t)j * nb + k];
+                if (!(djj > 0)) {   // not positive definite: the correction stays off
+                    not_pd = 1;
+                    return;
+                }
+                djj = sqrt(djj);
+                L[(size_t)j * nb + j] = djj;
+                for (int i = j + 1; i < nb; i++) {
+                    double t = L[(size_t)i * nb + j];
+                    for (int k = 0; k < j; k++) t -= L[(size_t)i * nb + k] * L[(size_t)j * nb + k];
+                    L[(size_t)i * nb + j] = t / djj;
+                }
+            }
+            y.resize(nb);
+            xb.resize((size_t)nb * nb);
+            for (int col = 0; col < nb; col++) {   // solve L L^T x = e_col
+                for (int i = 0; i < nb; i++) {
+                    double t = i == col ? 1.0 : 0.0;
+                    for (int k = 0; k < i; k++) t -= L[(size_t)i * nb + k] * y[k];
+                    y[i] = t / L[(size_t)i * nb + i];
+                }
+                for (int i = nb - 1; i >= 0; i--) {
+                    double t = y[i];
+                    for (int k = i + 1; k < nb; k++) t -= L[(size_t)k * nb + i] * xb[(size_t)k * nb + col];
+                    xb[(size_t)i * nb + col] = t / L[(size_t)i * nb + i];
+                }
+            }
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) inv[(size_t)(lo + i) * nc + lo + j] = xb[(size_t)i * nb + j];
+        }
+    };
+    {
+        // ranges of whole instances with about m / nt listed cells each
+        const int nt = m < 1024 || ninst < 2
+                           ? 1
+                           : (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+        std::vector<int> cut(1, 0);
+        for (int q = 1; q < ninst && (int)cut.size() < nt; q++)
+            if ((long)ptr[irange[q]] * nt >= (long)m * (long)cut.size() && ptr[irange[q]] > ptr[irange[cut.back()]])
+                cut.push_back(q);
+        cut.push_back(ninst);
+        std::vector<std::thread> th;
+        for (size_t t = 1; t + 1 < cut.size(); t++) {
+            try {
+                th.emplace_back(build_invert, cut[t], cut[t + 1]);
+            } catch (...) {   // no thread to be had: this range on the calling thread
+                build_invert(cut[t], cut[t + 1]);
             }
         }
-        y.resize(nb);
-        xb.resize((size_t)nb * nb);
-        for (int col = 0; col < nb; col++) {   // solve L L^T x = e_col
-            for (int i = 0; i < nb; i++) {
-                double t = i == col ? 1.0 : 0.0;
-                for (int k = 0; k < i; k++) t -= L[(size_t)i * nb + k] * y[k];
-                y[i] = t / L[(size_t)i * nb + i];
-            }
-            for (int i = nb - 1; i >= 0; i--) {
-                double t = y[i];
-                for (int k = i + 1; k < nb; k++) t -= L[(size_t)k * nb + i] * xb[(size_t)k * nb + col];
-                xb[(size_t)i * nb + col] = t / L[(size_t)i * nb + i];
-            }
-        }
-        for (int i = 0; i < nb; i++)
-            for (int j = 0; j < nb; j++) inv[(size_t)(lo + i) * nc + lo + j] = xb[(size_t)i * nb + j];
+        build_invert(cut[0], cut[1]);
+        for (auto &x : th) x.join();
     }
+    if (not_pd) return HMFEM_OK;
     if (stage_reserve(D, sizeof(double) * nc * nc)) return HMFEM_ERR_NOMEM;   // the coefficients are used up
     memcpy(D.h_stage, inv.data(), sizeof(double) * nc * nc);
     HMFEM_CHECK(hipMemcpyAsync(D.einv, D.h_stage, sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));

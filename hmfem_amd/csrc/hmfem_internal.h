@@ -166,7 +166,7 @@ struct DeflationHost {
     std::vector<int> cell, cin, nbr, par, root, size, roots, cid, ptr, lcell, lcomp, lord, linst, irange, fill,
         pbh, cph, pc;
     std::vector<unsigned char> up;
-    std::vector<double> E, inv, L, y, xb;
+    std::vector<double> E, inv;
 };
 
 struct Deflation {
