@@ -2,10 +2,13 @@
 
 Workload (one "step" = one reservoir() pass: transmissibility assembly, AMG
 setup, FP64 flexible CG with the FP32 AMG V-cycle to a relative residual of
-1e-8, then flux reconstruction and divergence):
-  128x128x128 cells on the unit cube, lognormal K (ln K ~ N(ln 1e-14, 2^2),
-  squared-exponential correlation of 8 cells, seed 20261202), Dirichlet
-  p=1 / p=0 on the x faces, no-flow elsewhere, lambda = 1.
+1e-8, then flux reconstruction and divergence), by default:
+  256x256x256 cells on the unit cube, K = 1e-14 plus 20 seeded karst
+  conduits (random polylines of 5-15 segments, 1-3 cells thick, K x 10^U(4,6),
+  seed 20261202), Dirichlet p=1 / p=0 on the x faces, no-flow elsewhere,
+  lambda = 1.
+--workload c2 selects the 128^3 lognormal case, c4 the 96-instance batch,
+c5 the 512^3 block of eight right-hand sides, c1 the 16^3 closed-form case.
 Inputs (K and the face values) are resident in HBM before timing starts.
 
 Prints one JSON line (rank 0). Run `python bench.py --help` for options.
@@ -80,23 +83,80 @@ def make_K(w):
     return K
 
 
+def host_cpu():
+    """CPU model and the cores this process may run on."""
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count()
+    return model, ncpu
+
+
+def write_ref_input(path, K, dims, bc_type, bc_val):
+    nx, ny, nz = dims
+    with open(path, "wb") as f:
+        f.write(struct.pack("<3i", nx, ny, nz))
+        f.write(struct.pack("<5d", 1.0, 1.0, 1.0, 1.0, 1e-12))
+        f.write(struct.pack("<6i", *bc_type))
+        f.write(struct.pack("<6d", *bc_val))
+        f.write(struct.pack("<i", 0))
+        kb = np.ascontiguousarray(K, "<f8").tobytes()
+        f.write(kb); f.write(kb); f.write(kb)
+
+
+def cpu_baseline_bounded(K, dims, pcg_iterations, ref_loops, budget_s, bc_type=(0, 0, 1, 1, 1, 1),
+                         bc_val=(1.0, 0.0, 0.0, 0.0, 0.0, 0.0)):
+    """The reference build on the full-size problem, with its PCG cut off
+    after `pcg_iterations` iterations: assembly, count-CSR, AMG setup, the
+    iterations and the fluxes are timed at the real size, and the PCG time is
+    scaled to the `ref_loops` loops (pcg.h's _loop) the reference needs at its
+    own 1e-8 stop on this problem (counted by a full run of the same build)."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "hmfem_ref")
+    model, ncpu = host_cpu()
+    nx, ny, nz = dims
+    with tempfile.TemporaryDirectory() as tmp:
+        inp = os.path.join(tmp, "in.bin")
+        write_ref_input(inp, K, dims, bc_type, bc_val)
+        t0 = time.time()
+        subprocess.run([ref, inp, os.path.join(tmp, "out"), "time", str(pcg_iterations)], check=True,
+                       stdout=subprocess.DEVNULL, timeout=max(120, 8 * budget_s),
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        wall = time.time() - t0
+        meta = json.load(open(os.path.join(tmp, "out.json")))
+    # the PCG ran its initial residual and preconditioner application plus
+    # `pcg_loop - 1` iterations; the full solve runs ref_loops - 1 iterations
+    # plus the same start, so the time scales with the loop count
+    t_pcg = meta["t_pcg"] * ref_loops / meta["pcg_loop"]
+    t = meta["t_assemble"] + meta["t_csr"] + meta["t_setup"] + t_pcg + meta["t_flux"]
+    return {"value": 1.0 / t, "unit": "solves/s", "cores": 1, "kind": "reference",
+            "sample": "the reference build on the full %dx%dx%d problem, single thread: assembly %.2f s, count-CSR "
+                      "%.2f s, AMG setup %.2f s, %d PCG iterations %.2f s, fluxes %.2f s (process wall %.1f s); the "
+                      "PCG time is scaled to the %d loops of the reference's own 1e-8 stop on this problem (its full "
+                      "run in the fixture), %.2f s, for a %.1f s step"
+                      % (nx, ny, nz, meta["t_assemble"], meta["t_csr"], meta["t_setup"], meta["pcg_loop"] - 1,
+                         meta["t_pcg"], meta["t_flux"], wall, ref_loops, t_pcg, t),
+            "cpu": model, "host_cpus": ncpu}
+
+
 def cpu_baseline(K, dims, budget_s, bc_type=(0, 0, 1, 1, 1, 1), bc_val=(1.0, 0.0, 0.0, 0.0, 0.0, 0.0)):
     """Time the reference build (oracle/_ref) on the same problem: one
     reservoir() step single-threaded. Falls back to the oracle restatement."""
     nx, ny, nz = dims
     ref = os.path.join(ROOT, "oracle", "_ref", "hmfem_ref")
     cores = 1
+    model, ncpu = host_cpu()
     if os.path.exists(ref):
         with tempfile.TemporaryDirectory() as tmp:
             inp = os.path.join(tmp, "in.bin")
-            with open(inp, "wb") as f:
-                f.write(struct.pack("<3i", nx, ny, nz))
-                f.write(struct.pack("<5d", 1.0, 1.0, 1.0, 1.0, 1e-12))
-                f.write(struct.pack("<6i", *bc_type))
-                f.write(struct.pack("<6d", *bc_val))
-                f.write(struct.pack("<i", 0))
-                kb = np.ascontiguousarray(K, "<f8").tobytes()
-                f.write(kb); f.write(kb); f.write(kb)
+            write_ref_input(inp, K, dims, bc_type, bc_val)
             t0 = time.time()
             subprocess.run([ref, inp, os.path.join(tmp, "out"), "time"], check=True, stdout=subprocess.DEVNULL,
                            timeout=max(60, 4 * budget_s), env=dict(os.environ, OMP_NUM_THREADS="1"))
@@ -107,7 +167,7 @@ def cpu_baseline(K, dims, budget_s, bc_type=(0, 0, 1, 1, 1, 1), bc_val=(1.0, 0.0
                 "sample": "one full %dx%dx%d step of the reference build (assembly, AMG setup + PCG to its 1e-8 "
                           "stop, fluxes), single thread; %d PCG iterations; %.2f s (process wall %.2f s)"
                           % (nx, ny, nz, meta["iter_amg_1e-8"] - 1, t, wall),
-                "cpu": platform.processor() or platform.machine()}
+                "cpu": model, "host_cpus": ncpu}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as O
     P = O.Problem(dims, K, bc_type, bc_val)
@@ -118,7 +178,27 @@ def cpu_baseline(K, dims, budget_s, bc_type=(0, 0, 1, 1, 1, 1), bc_val=(1.0, 0.0
     O.fluxos(P, x, perm)
     t = time.time() - t0
     return {"value": 1.0 / t, "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": "one full %dx%dx%d step of the oracle restatement, single thread; %.2f s" % (nx, ny, nz, t)}
+            "sample": "one full %dx%dx%d step of the oracle restatement, single thread; %.2f s" % (nx, ny, nz, t),
+            "cpu": model, "host_cpus": ncpu}
+
+
+FIXTURES = {"c2": "c2_128_lognormal", "c3": "c3_256_karst"}
+
+
+def reference_fixture(name, w, K):
+    """The reference's full-size summary for this workload (tests/golden/full),
+    if there is one for exactly this field; with the sampled cell indices."""
+    if name not in FIXTURES:
+        return None
+    path = os.path.join(ROOT, "tests", "golden", "full", FIXTURES[name] + ".npz")
+    if not os.path.exists(path):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from summary_sample import field_digest, sample_index
+    z = np.load(path)
+    if tuple(int(v) for v in z["dims"]) != tuple(w["dims"]) or field_digest(K) != str(z["K_sha256"]):
+        return None
+    return z, sample_index(w["dims"])
 
 
 def device_stream_gbs(torch, stream, mib=1024, reps=10):
@@ -356,9 +436,9 @@ def run_block(args, w, torch, S):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; c3, c4: 3; c5: 1)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 3; c3, c4, c5: 1)")
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; c3: 10; c4: 3; c5: 1)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 3; c3: 2; c4, c5: 1)")
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16"],
                     help="hierarchy precision; fp16 = FP32 hierarchy with the fine level stored in FP16")
     ap.add_argument("--smoother", default=None, choices=["jacobi", "blockgs"],
@@ -373,7 +453,7 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
-    big = {"c3": (3, 1), "c4": (3, 1), "c5": (1, 1)}.get(args.workload, (20, 3))
+    big = {"c3": (10, 2), "c4": (3, 1), "c5": (1, 1)}.get(args.workload, (20, 3))
     if args.steps is None:
         args.steps = big[0]
     if args.warmup is None:
@@ -559,15 +639,41 @@ def main():
                 "rel_residual": s16.rel_residual}
         fp16["speedup_over_fp32"] = fp16["value"] / value
 
+    # accuracy against the reference's own pressure on this problem (its 1e-12
+    # solve, sampled; tests/golden/full), for the timed 1e-8 solve and at 1e-9
+    accuracy = None
+    fx = reference_fixture(args.workload, w, K)
+    if fx is not None:
+        z, idx = fx
+        ref_s = z["p12_s"]
+        rel = lambda x: float(np.linalg.norm(x[idx] - ref_s) / np.linalg.norm(ref_s))
+        step(opt)
+        e8 = rel(p.cpu().numpy())
+        step(S.default_options(rtol=rtol * 0.1, precision=prec, smoother=opt.smoother))
+        e9 = rel(p.cpu().numpy())
+        step(S.default_options(rtol=1e-12, precision=prec, smoother=opt.smoother))
+        e12 = rel(p.cpu().numpy())
+        accuracy = {"rel_l2_vs_reference_1e-12": e8, "rtol": rtol,
+                    "reference_own_1e-8_vs_its_1e-12": float(z["rel_l2_p8_p12"]),
+                    "rel_l2_vs_reference_at_rtol_%g" % (rtol * 0.1): e9,
+                    "rel_l2_vs_reference_at_rtol_1e-12": e12,
+                    "reference_iterations_1e-8": int(z["iter8"]) - 1,
+                    "samples": int(idx.size),
+                    "note": "relative L2 over the fixture's cells (every 8th cell per axis and the three mid-planes) "
+                            "of the GPU pressure against the reference build's pressure at 1e-12"}
+        log("accuracy", accuracy)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
         m = 128
         if n <= m ** 3:
             cpu = cpu_baseline(K, w["dims"], args.cpu_budget)
+        elif fx is not None:
+            cpu = cpu_baseline_bounded(K, w["dims"], 5, int(fx[0]["iter8"]), args.cpu_budget)
         else:
-            # a bounded sample: the 128^3 member of the same family (conduit
-            # count scaled with the cross-section), extrapolated per unknown
+            # a smaller cube of the same family with no reference run at this
+            # size: the 128^3 member, scaled per unknown and labelled so
             ws = dict(w, dims=(m, m, m))
             if "conduits" in ws:
                 ws["conduits"] = max(1, round(w["conduits"] * m * m / (nx * ny)))
@@ -601,6 +707,7 @@ def main():
                      "stream_add_gbs": stream_gbs, "frac_of_stream_add": achieved / stream_gbs,
                      "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": 1e3 * avg_ms},
         "kernels": kernels,
+        "accuracy_vs_reference": accuracy,
         "fp64_hierarchy": fp64,
         "plain_fp64_jacobi_pcg": plain,
         "reuse_setup": reuse,

@@ -527,7 +527,7 @@ kb_cg_spmv_z(const CInst *__restrict__ CI, const int *__restrict__ zmap, LPtr<do
     int stop = 0, status = HMFEM_OK;
     double beta = 0.0, sigma_first = k == 0 ? (sigma == 0.0 ? 1.0 : sigma) : sc->sigma_first;
     if (k == 0) {
-        if (bb == 0.0 || rr <= sc->rtol2 * bb || sigma == 0.0) stop = 1;
+        if (sigma == 0.0 || (sc->stop_rule == HMFEM_STOP_RESIDUAL && (bb == 0.0 || rr <= sc->rtol2 * bb))) stop = 1;
     } else {
         beta = -sc->alpha[(k - 1) & 1] * dots[1] / sc->sigma[(k - 1) & 1];
         if (sc->stop_rule == HMFEM_STOP_RESIDUAL && rr <= sc->rtol2 * bb) stop = 1;

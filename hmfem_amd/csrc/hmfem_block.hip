@@ -149,7 +149,8 @@ __device__ __forceinline__ bool b_iteration_scalars(int G, int Gdot, BlockScalar
         int stop = 0, status = HMFEM_OK;
         const double sf = k == 0 ? (sigma == 0.0 ? 1.0 : sigma) : sc->sigma_first[r];
         if (k == 0) {
-            if (bbs[r] == 0.0 || rrs[r] <= sc->rtol2 * bbs[r] || sigma == 0.0) stop = 1;
+            if (sigma == 0.0 || (sc->stop_rule == HMFEM_STOP_RESIDUAL && (bbs[r] == 0.0 || rrs[r] <= sc->rtol2 * bbs[r])))
+                stop = 1;
         } else {
             beta[r] = -sc->alpha[(k - 1) & 1][r] * dots[NR + r] / sc->sigma[(k - 1) & 1][r];
             if (sc->stop_rule == HMFEM_STOP_RESIDUAL && rrs[r] <= sc->rtol2 * bbs[r]) stop = 1;

@@ -8,10 +8,52 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace hmfem;
+
+// ------------------------------------------------------------ fixed scratch
+//
+// Device memory of hmfem_flux_device_noalloc: the per-block min/max partials
+// of the flux kernel (two doubles per 256-cell block) and the two results,
+// for grids of up to kNoAllocMaxCells cells (8 MiB). One copy per device,
+// held until the process ends. The host-side name `scratch` is not a device
+// address: host code uses only the per-device pointer that
+// hipGetSymbolAddress returns (g_scratch_ptr).
+constexpr long kNoAllocMaxCells = 1L << 27;   // 512^3
+constexpr long kScratchDoubles = 2 * (kNoAllocMaxCells / kBlock) + 32;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices];
+static int g_scratch_rc[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Resolve the scratch array's address on the current device (once per
+// device; called by hmfem_create with the context's device current).
+static double *scratch_for_current_device(int *rc_out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        *rc_out = HMFEM_ERR_HIP;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev] && g_scratch_rc[dev] == 0) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "hmfem_gpu: hipGetSymbolAddress(scratch) on device %d: %s\n", dev, hipGetErrorString(e));
+            g_scratch_rc[dev] = HMFEM_ERR_HIP;
+        } else {
+            g_scratch_ptr[dev] = (double *)ptr;
+        }
+    }
+    *rc_out = g_scratch_ptr[dev] ? HMFEM_OK : g_scratch_rc[dev];
+    return g_scratch_ptr[dev];
+}
 
 extern "C" {
 int iter_amg = 0;
@@ -20,6 +62,8 @@ FACE boundary;
 
 struct hmfem_context {
     hipStream_t st = nullptr;
+    double *scratch = nullptr;   // this device's fixed scratch (hmfem_flux_device_noalloc)
+    int scratch_rc = HMFEM_ERR_HIP;
     hipEvent_t ev[8];
     std::map<std::string, std::pair<void *, size_t>> bufs;
     CgScalars *h_sc = nullptr;
@@ -107,6 +151,7 @@ extern "C" int hmfem_create(hmfem_context **out)
     HMFEM_CHECK(hipHostMalloc((void **)&c->h_sc, sizeof(CgScalars), hipHostMallocDefault));
     HMFEM_CHECK(hipHostMalloc((void **)&c->defl.h_hist, Deflation::kHistBins * sizeof(unsigned), hipHostMallocDefault));
     HMFEM_CHECK(hipEventCreateWithFlags(&c->defl.hist_ready, hipEventDisableTiming));
+    c->scratch = scratch_for_current_device(&c->scratch_rc);
     int dev = 0, lds = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
@@ -252,6 +297,35 @@ extern "C" int hmfem_flux_device(hmfem_context *c, const hmfem_grid *g, const do
         c->get("mm_init", 8);
     }
     int rc = launch_flux(c->st, g, Kx, Ky, Kz, nullptr, bval, p, v, part, cnt, out);
+    if (rc) return rc;
+    double h[2];
+    HMFEM_CHECK(hipMemcpyAsync(h, out, 16, hipMemcpyDeviceToHost, c->st));
+    HMFEM_CHECK(hipStreamSynchronize(c->st));
+    if (div_min) *div_min = h[0];
+    if (div_max) *div_max = h[1];
+    return HMFEM_OK;
+}
+
+extern "C" int hmfem_flux_device_noalloc(hmfem_context *c, const hmfem_grid *g, const double *Kx,
+                                         const double *Ky, const double *Kz, const double *const bval[6],
+                                         const double *p, ELE *v, double *div_min, double *div_max)
+{
+    if (!c || !grid_ok(g) || !Kx || !Ky || !Kz || !bval || !p) return HMFEM_ERR_ARG;
+    const long n = (long)g->nx * g->ny * g->nz;
+    if (n > kNoAllocMaxCells) {
+        fprintf(stderr, "hmfem_gpu: hmfem_flux_device_noalloc: %ld cells, more than its fixed scratch holds (%ld)\n",
+                n, kNoAllocMaxCells);
+        return HMFEM_ERR_ARG;
+    }
+    if (!c->scratch) return c->scratch_rc;
+    // every partial the final kernel reads is written by this call's flux
+    // kernel, so nothing needs clearing; calls that overlap on one device race
+    const long G = grid_for(n, kBlock);
+    double *part = c->scratch;
+    double *out = c->scratch + kScratchDoubles - 32;
+    static_assert(2 * (kNoAllocMaxCells / kBlock) <= kScratchDoubles - 32, "scratch layout");
+    if (2 * G > kScratchDoubles - 32) return HMFEM_ERR_ARG;   // the flux kernel's partials must fit
+    int rc = launch_flux(c->st, g, Kx, Ky, Kz, nullptr, bval, p, v, part, nullptr, out);
     if (rc) return rc;
     double h[2];
     HMFEM_CHECK(hipMemcpyAsync(h, out, 16, hipMemcpyDeviceToHost, c->st));
@@ -605,6 +679,15 @@ static int env_reuse()
     return s && atoi(s) != 0;
 }
 
+// The drop-ins stop as the reference's PCG does (pcg.h:57): <M r, r> at most
+// rtol^2 times its value at the start, so a warm start is tightened further.
+// HMFEM_STOP=residual selects ||b - A x|| <= rtol ||b|| instead.
+static int env_stop_rule()
+{
+    const char *s = getenv("HMFEM_STOP");
+    return s && strcmp(s, "residual") == 0 ? HMFEM_STOP_RESIDUAL : HMFEM_STOP_PRECONDITIONED;
+}
+
 static double env_rtol()
 {
     const char *s = getenv("HMFEM_RTOL");
@@ -705,9 +788,8 @@ extern "C" int compute(int *hdr, int *cnt, int *col, double *ele, double *b, dou
     Dims d;
     std::vector<double> e, wx, wy, wz;
     if (structured_from_csr(n, cnt, col, ele, d, e, wx, wy, wz) != 0) {
-        fprintf(stderr, "hmfem_gpu: compute(): the matrix is not a symmetric 7-point Cartesian stencil; "
-                        "general sparse input is not supported by this library\n");
-        return HMFEM_ERR_NOT_STRUCTURED;
+        fprintf(stderr, "hmfem_gpu: compute(): the matrix is not a symmetric 7-point Cartesian stencil\n");
+        fatal("compute", HMFEM_ERR_NOT_STRUCTURED);
     }
     hmfem_context *c = default_ctx(d.ny == 1 && d.nz == 1);
     Stencil<double> A;
@@ -720,6 +802,7 @@ extern "C" int compute(int *hdr, int *cnt, int *col, double *ele, double *b, dou
     hmfem_options opt;
     hmfem_default_options(&opt);
     opt.rtol = env_rtol();
+    opt.stop_rule = env_stop_rule();
     opt.reuse_setup = env_reuse();
     hmfem_status st{};
     int rc = solve_on_stencil(c, d, A, db, dx, &opt, &st);
@@ -750,6 +833,7 @@ extern "C" void solve_linsys(ELE *perm, IND *indcol, double *b, double *x, Dim N
     hmfem_options opt;
     hmfem_default_options(&opt);
     opt.rtol = env_rtol();
+    opt.stop_rule = env_stop_rule();
     opt.reuse_setup = env_reuse();
     hmfem_status st{};
     int rc = hmfem_solve_structured_device(c, N.x, N.y, N.z, dperm, dind, db, dx, &opt, &st);

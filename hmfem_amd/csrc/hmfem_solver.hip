@@ -937,7 +937,9 @@ __device__ __forceinline__ bool cg_iteration_scalars(int G, int Gdot, CgScalars 
     beta = 0.0;
     double sigma_first = k == 0 ? (sigma == 0.0 ? 1.0 : sigma) : sc->sigma_first;
     if (k == 0) {
-        if (bb == 0.0 || rr <= sc->rtol2 * bb || sigma == 0.0) stop = 1;
+        // the reference stops at once only when <M r0, r0> = 0 (pcg.h:51-57); the residual rule also
+        // when the start already meets it
+        if (sigma == 0.0 || (sc->stop_rule == HMFEM_STOP_RESIDUAL && (bb == 0.0 || rr <= sc->rtol2 * bb))) stop = 1;
     } else {
         beta = -sc->alpha[(k - 1) & 1] * dots[1] / sc->sigma[(k - 1) & 1];
         if (sc->stop_rule == HMFEM_STOP_RESIDUAL && rr <= sc->rtol2 * bb) stop = 1;

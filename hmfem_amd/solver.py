@@ -123,6 +123,8 @@ def lib():
         L.hmfem_reservoir_device.argtypes = [c_vp, P(HmfemGrid)] + [c_vp] * 7 + [c_vp, c_vp, c_vp, P(Options),
                                                                               P(Status)]
         L.hmfem_flux_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(c_dbl), P(c_dbl)]
+        L.hmfem_flux_device_noalloc.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(c_dbl),
+                                                P(c_dbl)]
         L.hmfem_solve_structured_device.argtypes = [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, P(Options),
                                                     P(Status)]
         L.hmfem_profile_enable.argtypes = [c_vp, c_int]
@@ -149,7 +151,7 @@ def lib():
 
 EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", "fluxos", "divvel_pontual",
             "prod_mat_vec", "hmfem_default_options", "hmfem_create", "hmfem_destroy", "hmfem_stream",
-            "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device",
+            "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device", "hmfem_flux_device_noalloc",
             "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
             "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device",
             "hmfem_reservoir_block_device", "hmfem_autotune_device", "save_pressure", "save_flux",
@@ -349,6 +351,17 @@ class Context:
                                      _dptr(p), _dptr(v), ctypes.byref(mn), ctypes.byref(mx))
         if rc != 0:
             raise RuntimeError("hmfem_flux_device failed: %d" % rc)
+        return mn.value, mx.value
+
+    def flux_noalloc(self, g, Kx, Ky, Kz, bval, p, v=None):
+        """hmfem_flux_device_noalloc: the same results, no device allocation."""
+        mn, mx = c_dbl(), c_dbl()
+        _order_after_torch(Kx, Ky, Kz, p, v, *bval)
+        rc = lib().hmfem_flux_device_noalloc(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz),
+                                             self._faces(bval), _dptr(p), _dptr(v), ctypes.byref(mn),
+                                             ctypes.byref(mx))
+        if rc != 0:
+            raise RuntimeError("hmfem_flux_device_noalloc failed: %d" % rc)
         return mn.value, mx.value
 
     def solve_structured(self, nx, ny, nz, perm, indcol, b, x, options=None):

@@ -53,9 +53,12 @@ extern FACE boundary;   /* read by fluxos() (flux_reconstruction.c:104-270) */
 
 /* solve_linearsystem_AMG.c:37 solve_linsys(); prototype functions.h:54.
    perm/indcol as filled by permeabilidade_arestas(), b the RHS, x in/out
-   (initial guess in, pressure out). Stops at relative residual 1e-8
-   (HMFEM_RTOL overrides). Writes iter_amg. Exits like the reference when the
-   matrix has no entries. With HMFEM_REUSE_SETUP=1 in the environment,
+   (initial guess in, pressure out). Stops as the reference's PCG does
+   (pcg.h:57): when <M r, r> has fallen to 1e-16 of its value at the initial
+   guess, so a warm start is tightened further (HMFEM_RTOL replaces 1e-8;
+   HMFEM_STOP=residual stops at ||b - A x|| <= rtol ||b|| instead). Writes
+   iter_amg (iterations + 1, as compute.cpp:46). Exits like the reference when
+   the matrix has no entries. With HMFEM_REUSE_SETUP=1 in the environment,
    solve_linsys() and compute() reuse the previous call's hierarchy on the
    same grid (hmfem_options.reuse_setup); 1-D systems (the conduits') use a
    context of their own, so they do not evict the reservoir's. */
@@ -63,7 +66,10 @@ void solve_linsys(ELE *perm, IND *indcol, double *b, double *x, Dim N);
 
 /* compute.cpp:7 compute(); prototype functions.h:53. Count-CSR: hdr[0] = n,
    hdr[1] = nnz, row i holds the next cnt[i] entries, 0-based columns, FP64
-   values; the matrix must be symmetric. x in/out. Returns 0. */
+   values; the matrix must be symmetric. x in/out. Same stopping rule and
+   iter_amg as solve_linsys(). Returns 0; a matrix the library cannot solve
+   ends the process with a message (the reference's callers ignore the
+   return value, solve_linearsystem_AMG.c:150, :283). */
 int compute(int *hdr, int *cnt, int *col, double *ele, double *b, double *x);
 
 /* permeabilities.c:196 permeabilidade_arestas(); functions.h:31. */
@@ -214,6 +220,18 @@ int hmfem_reservoir_device(hmfem_context *ctx, const hmfem_grid *g, const double
 int hmfem_flux_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
                       const double *Kz, const double *const bval[6], const double *p, ELE *v,
                       double *div_min, double *div_max);
+
+/* No counterpart in the reference: hmfem_flux_device without any device
+   allocation, for callers that cannot allocate (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc). Same arguments and
+   results as hmfem_flux_device; its scratch (the flux kernel's per-block
+   min/max partials) is a fixed 8 MiB device array per GPU, resolved when the
+   context is created, so grids of more than 2^27 cells are rejected with
+   HMFEM_ERR_ARG. Every call on one GPU shares that array: calls that overlap
+   on one GPU (different contexts or streams) race. */
+int hmfem_flux_device_noalloc(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
+                              const double *Kz, const double *const bval[6], const double *p, ELE *v,
+                              double *div_min, double *div_max);
 
 /* Solve on an already assembled structured operator given in AoS form (device
    perm/indcol as from hmfem_assemble_device), b and x device FP64. */

@@ -16,8 +16,15 @@
  *   <prefix>.v      fluxos() on p12, 7 doubles per cell (u,d,l,r,c,t,b)
  *   <prefix>.json   iteration counts, level sizes, timings, divergence
  *
+ *   <prefix>.p8w    a second solve_linsys() started from p8 (warm start, as
+ *                   main.c:460 passes the previous pressure back in)
+ *
  * With a third argument "time" it only times one reservoir() step (assembly,
- * solve_linsys at the reference's 1e-8, fluxos) and writes <prefix>.json.
+ * solve_linsys at the reference's 1e-8, fluxos) and writes <prefix>.json;
+ * "time <k>" stops the PCG after k iterations (a bounded sample of a large
+ * step: the JSON then holds the setup time and the per-iteration time).
+ * With "summary" it writes p8, p8w, p12 and the JSON (no matrix, fluxes or
+ * level sizes): the large cases.
  * With "write <dir>" it also runs the reference's output writers on the tight
  * pressure and its fluxes: save_pressure(), save_flux()
  * (posprocessing_output.c:200-296) and vts_StructuredGrid_file() (vtk.c:8-76)
@@ -178,13 +185,67 @@ int main(int argc, char **argv)
     source_term(&Bb, pm, h, N, boundary, pcv, pwv, KIv, WIv, viscosidade, b);
     double t_assemble = now() - t0;
 
+    const char *mode = argc > 3 ? argv[3] : "";
+    const bool summary = strcmp(mode, "summary") == 0;
+
+    if (strcmp(mode, "time") == 0 && argc > 4) {
+        /* bounded timing sample: the parts of one reservoir() step at this
+           size, with the PCG cut off after k iterations. The count-CSR and
+           the toolbox objects are built as solve_linsys() and compute() build
+           them (solve_linearsystem_AMG.c:57-150, compute.cpp:16-45). */
+        int k = atoi(argv[4]);
+        t0 = now();
+        std::vector<int> cnt(n), col;
+        std::vector<double> ele;
+        col.reserve((size_t)7 * n); ele.reserve((size_t)7 * n);
+        for (int s = 0; s < n; s++) {
+            int c = 0;
+            if (ic[s].d != -1) { col.push_back(ic[s].d); ele.push_back(-pm[s].d); c++; }
+            if (ic[s].l != -1) { col.push_back(ic[s].l); ele.push_back(-pm[s].l); c++; }
+            if (ic[s].c != -1) { col.push_back(ic[s].c); ele.push_back(pm[s].c); c++; }
+            if (ic[s].r != -1) { col.push_back(ic[s].r); ele.push_back(-pm[s].r); c++; }
+            if (ic[s].u != -1) { col.push_back(ic[s].u); ele.push_back(-pm[s].u); c++; }
+            if (ic[s].t != -1) { col.push_back(ic[s].t); ele.push_back(-pm[s].t); c++; }
+            if (ic[s].b != -1) { col.push_back(ic[s].b); ele.push_back(-pm[s].b); c++; }
+            cnt[s] = c;
+        }
+        std::vector<int> nod(n);
+        for (int i = 0; i < n; i++) nod[i] = i;
+        std::vector<double> bv(b, b + n), xv(n, 0.0);
+        double t_csr = now() - t0;
+        t0 = now();
+        communicator<int, double> com(nod);
+        linear_operator<int, double> lin(cnt, col, ele);
+        vector_product<int, double> sca(com);
+        amg_solver<int, double> amg(nod, cnt, col, ele, 16, 1, 0.0, 1.0, true);
+        double t_setup = now() - t0;
+        t0 = now();
+        /* _max = k + 1: the loop runs while ++_loop < _max (pcg.h:57), plus
+           the initial residual and preconditioner application */
+        conjugate_gradient<int, double> cg(lin, amg, sca, 1e-8, k + 1, false);
+        cg(bv, xv);
+        double t_pcg = now() - t0;
+        int done = cg.iterations();
+        for (int i = 0; i < n; i++) p8[i] = xv[i];
+        t0 = now();
+        fluxos(p8, pm, vv, &Bb, h, N);
+        double t_flux = now() - t0;
+        char name[2048];
+        snprintf(name, sizeof(name), "%s.json", argv[2]);
+        FILE *f = fopen(name, "w");
+        fprintf(f, "{\"n\": %d, \"pcg_loop\": %d, \"t_assemble\": %.6f, \"t_csr\": %.6f, \"t_setup\": %.6f, "
+                   "\"t_pcg\": %.6f, \"t_flux\": %.6f}\n", n, done, t_assemble, t_csr, t_setup, t_pcg, t_flux);
+        fclose(f);
+        return 0;
+    }
+
     /* Reference seam at its own stopping rule (compute.cpp:41, 1e-8). */
     t0 = now();
     solve_linsys(pm, ic, b, p8, N);
     double t_solve8 = now() - t0;
     int iter8 = iter_amg;
 
-    if (argc > 3 && strcmp(argv[3], "time") == 0) {
+    if (strcmp(mode, "time") == 0) {
         /* timing mode: one reservoir() step as the reference runs it
            (assembly + solve_linsys at 1e-8) plus fluxos(); no outputs but JSON */
         t0 = now();
@@ -198,6 +259,14 @@ int main(int argc, char **argv)
         fclose(f);
         return 0;
     }
+
+    /* A second pass started from p8, as main.c:460 hands the previous
+       pressure back to solve_linsys(): the stop test is relative to this
+       start's preconditioned residual (pcg.h:42-57). */
+    double *p8w = create_vector(n);
+    for (int i = 0; i < n; i++) p8w[i] = p8[i];
+    solve_linsys(pm, ic, b, p8w, N);
+    int iter8w = iter_amg;
 
     /* Count-CSR exactly as solve_linsys() builds it (solve_linearsystem_AMG.c:57-143). */
     std::vector<int> cnt(n), col;
@@ -236,7 +305,7 @@ int main(int argc, char **argv)
     /* Level sizes: rerun the hierarchy loop of amg.h:1188-1215 with the
        reference's own amg_restriction, recording rows and nonzeros. */
     std::vector<int> lev_rows, lev_nnz;
-    {
+    if (!summary) {
         std::vector<int> anod(nod), acnt(cnt), acol(col);
         std::vector<double> aele(ele);
         amg_restriction<int, double> restr;
@@ -273,14 +342,14 @@ int main(int argc, char **argv)
         if (fabs(dv) < divmin) divmin = fabs(dv);
     }
 
-    if (argc > 4 && strcmp(argv[3], "write") == 0) {
+    if (argc > 4 && strcmp(mode, "write") == 0) {
         dirname = argv[4];
         save_pressure(p12, N, h);
         save_flux(vv, N, h);
         vts_StructuredGrid_file(vv, p12, N, h);
     }
     const char *pre = argv[2];
-    {
+    if (!summary) {
         char name[2048];
         snprintf(name, sizeof(name), "%s.A", pre);
         FILE *f = fopen(name, "wb");
@@ -294,13 +363,15 @@ int main(int argc, char **argv)
     }
     put(pre, ".b", b, sizeof(double) * n);
     put(pre, ".p8", p8, sizeof(double) * n);
+    put(pre, ".p8w", p8w, sizeof(double) * n);
     put(pre, ".p12", p12, sizeof(double) * n);
-    put(pre, ".v", vv, sizeof(ELE) * n);
+    if (!summary) put(pre, ".v", vv, sizeof(ELE) * n);
     {
         char name[2048];
         snprintf(name, sizeof(name), "%s.json", pre);
         FILE *f = fopen(name, "w");
-        fprintf(f, "{\"n\": %d, \"nnz\": %d, \"iter_amg_1e-8\": %d, \"iter_tight\": %d, \"eps_tight\": %.3e,\n", n, nnz, iter8, iter12, eps_tight);
+        fprintf(f, "{\"n\": %d, \"nnz\": %d, \"iter_amg_1e-8\": %d, \"iter_amg_warm\": %d, \"iter_tight\": %d, \"eps_tight\": %.3e,\n",
+                n, nnz, iter8, iter8w, iter12, eps_tight);
         fprintf(f, " \"levels_rows\": [");
         for (size_t i = 0; i < lev_rows.size(); i++) fprintf(f, "%s%d", i ? ", " : "", lev_rows[i]);
         fprintf(f, "], \"levels_nnz\": [");

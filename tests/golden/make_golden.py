@@ -5,12 +5,19 @@ and flux sources compiled in place by ``make -C oracle ref``) on small seeded
 problems and stores inputs and outputs as compressed .npz files. Only this
 script runs the reference build; tests read the .npz files.
 
-Usage:  make -C oracle ref && python tests/golden/make_golden.py [writers]
+Usage:  make -C oracle ref && python tests/golden/make_golden.py [writers | summary [name ...]]
 
 With `writers` it only runs the cases marked `writers`, with the reference's
 output writers (save_pressure, save_flux, vts_StructuredGrid_file) on the
 tight pressure and its fluxes, and stores the three files' bytes (plus the
 p12 and v they were written from) in writers/<name>.npz.
+
+With `summary` it runs the full-size benchmark problems (128^3 lognormal,
+256^3 karst) and stores, in full/<name>.npz, the pressure at the reference's
+own 1e-8 stop (p8), after a warm-started second pass (p8w) and at 1e-12
+(p12) at the cells of tests/summary_sample.py, their full-field norms, sums
+and distances, the iteration counts, the divergence of p12's fluxes, the
+reference's timings in this container, and a digest of the K field.
 """
 import json
 import os
@@ -26,6 +33,9 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from hmfem_amd import fields  # noqa: E402
+
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+from summary_sample import field_digest, sample_index  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "hmfem_ref")
 
@@ -50,6 +60,14 @@ def case_list():
     return cases
 
 
+# full-size benchmark problems: same seeds and fields as bench.py's workloads
+SEED = 20261202
+SUMMARY_CASES = [
+    dict(name="c2_128_lognormal", dims=(128, 128, 128), K=("lognormal", 2.0, 8.0, SEED), **X_DIRICHLET),
+    dict(name="c3_256_karst", dims=(256, 256, 256), K=("karst", 20, SEED), **X_DIRICHLET),
+]
+
+
 def make_K(dims, spec):
     nx, ny, nz = dims
     if spec[0] == "homogeneous":
@@ -61,6 +79,23 @@ def make_K(dims, spec):
         fields.conduits(K, nx, ny, nz, spec[1], spec[2])
         return K
     raise ValueError(spec)
+
+
+def write_input(c, K, srcs, tmp):
+    nx, ny, nz = c["dims"]
+    inp = os.path.join(tmp, c["name"] + ".in")
+    with open(inp, "wb") as f:
+        f.write(struct.pack("<3i", nx, ny, nz))
+        f.write(struct.pack("<5d", 1.0, 1.0, 1.0, c.get("visc", 1.0), 1e-12))
+        f.write(struct.pack("<6i", *c["bc_type"]))
+        f.write(struct.pack("<6d", *c["bc_val"]))
+        f.write(struct.pack("<i", 1 if srcs else 0))
+        for _ in range(3):
+            f.write(np.ascontiguousarray(K, "<f8").tobytes())
+        if srcs:
+            for a in srcs:
+                f.write(np.ascontiguousarray(a, "<f8").tobytes())
+    return inp
 
 
 def run_case(c, tmp, writers=False):
@@ -76,18 +111,7 @@ def run_case(c, tmp, writers=False):
         pc = rng.random(n)
         pw = rng.random(n) * 2.0
         srcs = (KI, WI, pc, pw)
-    inp = os.path.join(tmp, c["name"] + ".in")
-    with open(inp, "wb") as f:
-        f.write(struct.pack("<3i", nx, ny, nz))
-        f.write(struct.pack("<5d", 1.0, 1.0, 1.0, visc, 1e-12))
-        f.write(struct.pack("<6i", *c["bc_type"]))
-        f.write(struct.pack("<6d", *c["bc_val"]))
-        f.write(struct.pack("<i", 1 if srcs else 0))
-        for _ in range(3):
-            f.write(np.ascontiguousarray(K, "<f8").tobytes())
-        if srcs:
-            for a in srcs:
-                f.write(np.ascontiguousarray(a, "<f8").tobytes())
+    inp = write_input(c, K, srcs, tmp)
     pre = os.path.join(tmp, c["name"])
     if writers:
         wdir = os.path.join(tmp, c["name"] + "_out")
@@ -113,8 +137,9 @@ def run_case(c, tmp, writers=False):
         dims=np.array(c["dims"], np.int32), bc_type=np.array(c["bc_type"], np.int32),
         bc_val=np.array(c["bc_val"], np.float64), visc=np.float64(visc), K=K,
         b=np.fromfile(pre + ".b", "<f8"), p8=np.fromfile(pre + ".p8", "<f8"),
-        p12=np.fromfile(pre + ".p12", "<f8"),
+        p12=np.fromfile(pre + ".p12", "<f8"), p8w=np.fromfile(pre + ".p8w", "<f8"),
         iter8=np.int32(meta["iter_amg_1e-8"]), iter12=np.int32(meta["iter_tight"]),
+        iter8w=np.int32(meta["iter_amg_warm"]),
         levels_rows=np.array(meta["levels_rows"], np.int32), levels_nnz=np.array(meta["levels_nnz"], np.int64),
         div_max=np.float64(meta["div_max"]), div_min=np.float64(meta["div_min"]),
     )
@@ -127,9 +152,53 @@ def run_case(c, tmp, writers=False):
     print(c["name"], "n=%d nnz=%d iter8=%d iter12=%d levels=%s" % (n, nnz, out["iter8"], out["iter12"], meta["levels_rows"]))
 
 
+def run_summary(c, tmp):
+    """Full-size case: samples and whole-field figures of p8, p8w and p12."""
+    n = int(np.prod(c["dims"]))
+    K = make_K(c["dims"], c["K"])
+    inp = write_input(c, K, None, tmp)
+    del K
+    pre = os.path.join(tmp, c["name"])
+    subprocess.run([REF, inp, pre, "summary"], check=True, stdout=subprocess.DEVNULL)
+    meta = json.load(open(pre + ".json"))
+    p8 = np.fromfile(pre + ".p8", "<f8")
+    p8w = np.fromfile(pre + ".p8w", "<f8")
+    p12 = np.fromfile(pre + ".p12", "<f8")
+    assert p8.size == n and p12.size == n
+    idx = sample_index(c["dims"])
+    nrm12 = float(np.linalg.norm(p12))
+    out = dict(
+        dims=np.array(c["dims"], np.int32), bc_type=np.array(c["bc_type"], np.int32),
+        bc_val=np.array(c["bc_val"], np.float64), visc=np.float64(c.get("visc", 1.0)),
+        K_spec=np.array(json.dumps(c["K"])), K_sha256=np.array(field_digest(make_K(c["dims"], c["K"]))),
+        sample_idx_count=np.int64(idx.size),
+        p8_s=p8[idx], p12_s=p12[idx],
+        p8_norm2=np.float64(np.linalg.norm(p8)), p8_sum=np.float64(p8.sum()),
+        p12_norm2=np.float64(nrm12), p12_sum=np.float64(p12.sum()),
+        rel_l2_p8_p12=np.float64(np.linalg.norm(p8 - p12) / nrm12),
+        rel_l2_p8w_p12=np.float64(np.linalg.norm(p8w - p12) / nrm12),
+        iter8=np.int32(meta["iter_amg_1e-8"]), iter8w=np.int32(meta["iter_amg_warm"]),
+        iter12=np.int32(meta["iter_tight"]), nnz=np.int64(meta["nnz"]),
+        div_max=np.float64(meta["div_max"]), div_min=np.float64(meta["div_min"]),
+        t_assemble=np.float64(meta["t_assemble"]), t_solve8=np.float64(meta["t_solve_1e-8"]),
+        t_setup=np.float64(meta["t_setup"]), t_pcg_tight=np.float64(meta["t_pcg_tight"]),
+        t_flux=np.float64(meta["t_flux"]),
+    )
+    np.savez_compressed(os.path.join(HERE, "full", c["name"] + ".npz"), **out)
+    print(c["name"], "n=%d iter8=%d iter8w=%d iter12=%d rel_l2(p8,p12)=%.3e t_solve8=%.1f s"
+          % (n, out["iter8"], out["iter8w"], out["iter12"], out["rel_l2_p8_p12"], meta["t_solve_1e-8"]), flush=True)
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("missing %s: run `make -C oracle ref` first" % REF)
+    if len(sys.argv) > 1 and sys.argv[1] == "summary":
+        names = sys.argv[2:]
+        with tempfile.TemporaryDirectory() as tmp:
+            for c in SUMMARY_CASES:
+                if not names or c["name"] in names:
+                    run_summary(c, tmp)
+        return
     writers = len(sys.argv) > 1 and sys.argv[1] == "writers"
     with tempfile.TemporaryDirectory() as tmp:
         for c in case_list():

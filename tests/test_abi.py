@@ -65,3 +65,30 @@ def test_options_defaults():
     assert o.nu >= 1 and 0 < o.omega < 1 and o.coarse_max >= 1
     assert o.smoother in (S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS) and o.gs_sweeps >= 1 and o.autotune == 0
     assert o.deflate == 1 and o.deflate_ratio == 3000.0 and o.precond == S.PRECOND_AMG and o.reuse_setup == 0
+
+
+def test_compute_unsupported_matrix_ends_the_process():
+    """compute() on a matrix it cannot solve (here: not symmetric) ends the
+    process with a message rather than returning with x unchanged: the
+    reference's callers ignore compute()'s return value
+    (solve_linearsystem_AMG.c:150, :283). The check runs before any device
+    work, so this needs no GPU."""
+    import subprocess
+    import sys
+    code = r'''
+import sys
+import numpy as np
+sys.path.insert(0, %r)
+from hmfem_amd import solver as S
+n = 4
+cnt = np.array([2, 3, 3, 2], np.int32)
+col = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3], np.int32)
+ele = np.array([2.0, -1.0, -1.0, 2.0, -1.0, -0.5, 2.0, -1.0, -1.0, 2.0])   # a(2,1) != a(1,2)
+x = np.zeros(n)
+S.compute(np.array([n, col.size], np.int32), cnt, col, ele, np.ones(n), x)
+print("RETURNED", x.tolist())
+''' % ROOT
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "RETURNED" not in r.stdout
+    assert "compute" in r.stderr

@@ -50,6 +50,18 @@ def test_compute_reference_stop_bitexact(fixture):
     assert np.array_equal(x.view(np.uint64), fixture["p8"].view(np.uint64))
 
 
+def test_compute_warm_start_bitexact(fixture):
+    """A second compute() started from the first pass's pressure, as
+    main.c:460 hands p back: the stop test is relative to the warm start's
+    own preconditioned residual (pcg.h:42-57), so the pass keeps iterating."""
+    P = O.problem_from_fixture(fixture)
+    perm, indcol, b = O.assemble(P)
+    cnt, col, ele = O.build_csr(perm, indcol)
+    x, it = O.compute(cnt, col, ele, b, x0=fixture["p8"], eps=1e-8)
+    assert it == int(fixture["iter8w"])
+    assert np.array_equal(x.view(np.uint64), fixture["p8w"].view(np.uint64))
+
+
 def test_tight_solve_levels_and_flux_bitexact(fixture):
     P = O.problem_from_fixture(fixture)
     perm, indcol, b = O.assemble(P)
