@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
 HIP_SOURCES = ["hmfem_assemble.hip", "hmfem_solver.hip", "hmfem_batch.hip", "hmfem_block.hip", "hmfem_defl.hip",
-               "hmfem_capi.hip", "hmfem_output.cpp"]
+               "hmfem_conduit.hip", "hmfem_capi.hip", "hmfem_output.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed FP32 arithmetic instructions: some of their forms have given

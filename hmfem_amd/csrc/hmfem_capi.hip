@@ -7,9 +7,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 using namespace hmfem;
@@ -446,6 +448,214 @@ extern "C" int hmfem_reservoir_device(hmfem_context *c, const hmfem_grid *g, con
         out->t_flux_ms = tf;
     }
     return out->status == HMFEM_OK ? HMFEM_OK : out->status;
+}
+
+// ------------------------------------------------------------ 1-D conduits
+
+// Layout of a conduit set on a grid (checked against the grid and the cases
+// the reference handles) for the conduit kernels: descriptors, the reservoir
+// cell of every conduit cell, the breccias conduit by conduit, and the
+// exchange list (each reservoir cell once, with the conduit cell that the
+// reference's loops write last, reservoir.c:19-45, main.c:404-442).
+struct ConduitLayout {
+    std::vector<ConduitDesc> desc;
+    std::vector<int> res_cell, brec_cell, xres, xcell;
+    std::vector<double> brec_beta, brec_C, kival;
+    int cells = 0, mmax = 0, nbrec = 0;
+};
+
+static int conduit_layout(const hmfem_grid *g, const hmfem_conduit_set *cs, ConduitLayout &L)
+{
+    auto bad = [](int j, const char *why) {
+        fprintf(stderr, "hmfem_gpu: conduit %d: %s\n", j, why);
+        return HMFEM_ERR_ARG;
+    };
+    if (!cs || cs->count < 1 || !cs->cells || !cs->start || !cs->axis || !cs->Kc || !cs->bc_l || !cs->bc_r ||
+        !cs->cont || !cs->ccont || !cs->karst_index || !cs->area || !cs->breccias || cs->max_breccias < 0 ||
+        (cs->max_breccias > 0 && (!cs->breccia_cell || !cs->breccia_beta || !cs->breccia_C)))
+        return HMFEM_ERR_ARG;
+    const int nk = cs->count;
+    const long ext[3] = {g->nx, g->ny, g->nz};
+    const double hax[3] = {g->hx, g->hy, g->hz};
+    L = ConduitLayout();
+    L.desc.resize(nk);
+    std::unordered_map<int, int> last;   // reservoir cell -> conduit cell written last
+    for (int j = 0; j < nk; j++) {
+        const int m = cs->cells[j], ax = cs->axis[j];
+        if (m < 1) return bad(j, "no cells");
+        if (ax < 1 || ax > 3) return bad(j, "axis must be 1 (x), 2 (y) or 3 (z)");
+        long st[3] = {cs->start[3 * j], cs->start[3 * j + 1], cs->start[3 * j + 2]};
+        for (int q = 0; q < 3; q++)
+            if (st[q] < 0 || st[q] >= ext[q]) return bad(j, "start outside the grid");
+        if (st[ax - 1] + m > ext[ax - 1]) return bad(j, "runs past the grid");
+        const int bl = cs->bc_l[j], br = cs->bc_r[j];
+        if ((bl != 0 && bl != 1 && bl != 3) || (br != 0 && br != 1 && br != 3))
+            return bad(j, "end types must be 0 (Dirichlet), 1 (Neumann) or 3 (non-linear Robin)");
+        if (bl == 3 && st[ax - 1] < 1) return bad(j, "a Robin start needs a reservoir cell before it");
+        if (br == 3 && (ax == 3 || st[ax - 1] + m >= ext[ax - 1]))
+            return bad(j, "a Robin far end needs a reservoir cell after it along x or y "
+                          "(the reference has no Robin far end along z)");
+        if (!(cs->Kc[j] > 0) || !(cs->area[j] > 0)) return bad(j, "permeability and area must be positive");
+        const int nb = cs->breccias[j];
+        if (nb < 0 || nb > cs->max_breccias) return bad(j, "breccia count out of range");
+        auto cell_at = [&](long off) {
+            long c[3] = {st[0], st[1], st[2]};
+            c[ax - 1] += off;
+            return c[0] + ext[0] * (c[1] + ext[1] * c[2]);
+        };
+        ConduitDesc &D = L.desc[j];
+        D.off = L.cells;
+        D.m = m;
+        D.axis = ax;
+        D.bc_l = bl;
+        D.bc_r = br;
+        D.nbrec = nb;
+        D.boff = L.nbrec;
+        D.hs = hax[ax - 1];
+        D.Kc = cs->Kc[j];
+        D.KI_c = cs->karst_index[j] / (cs->area[j] * D.hs);   // main.c:410
+        D.cont_l = cs->cont[j];
+        D.cont_r = cs->cont[nk + j];
+        D.ccont_l = cs->ccont[j];
+        D.ccont_r = cs->ccont[nk + j];
+        D.rs_l = bl == 3 ? cell_at(-1) : 0;
+        D.rsa_l = cell_at(0);
+        D.rs_r = br == 3 ? cell_at(m) : 0;
+        D.rsa_r = cell_at(m - 1);
+        for (int i = 0; i < m; i++) {
+            const int r = (int)cell_at(i);
+            L.res_cell.push_back(r);
+            L.kival.push_back(cs->karst_index[j]);
+            last[r] = L.cells + i;
+        }
+        for (int q = 0; q < nb; q++) {
+            const int sb = cs->breccia_cell[q * nk + j];
+            if (sb < 0 || sb >= m - 1) return bad(j, "breccia cell out of range");
+            L.brec_cell.push_back(sb);
+            L.brec_beta.push_back(cs->breccia_beta[q * nk + j]);
+            L.brec_C.push_back(cs->breccia_C[q * nk + j]);
+        }
+        L.cells += m;
+        L.nbrec += nb;
+        L.mmax = std::max(L.mmax, m);
+    }
+    std::vector<std::pair<int, int>> xs(last.begin(), last.end());
+    std::sort(xs.begin(), xs.end());
+    for (auto &pr : xs) {
+        L.xres.push_back(pr.first);
+        L.xcell.push_back(pr.second);
+    }
+    return HMFEM_OK;
+}
+
+template <typename T>
+static T *put_vec(hmfem_context *c, const char *name, const std::vector<T> &h)
+{
+    T *d = (T *)c->get(name, h.size() * sizeof(T) + 16);
+    if (d && !h.empty() && hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->st) != hipSuccess)
+        return nullptr;
+    return d;
+}
+
+extern "C" int hmfem_coupled_device(hmfem_context *c, const hmfem_grid *g, const double *Kx, const double *Ky,
+                                    const double *Kz, const double *const bval[6], const double *WI, const double *pw,
+                                    const hmfem_conduit_set *cs, int max_passes, double tol, double *p, double *pc,
+                                    double *ucl, double *ucr, ELE *v, double *pc_first, double *pass_change,
+                                    const hmfem_options *opt, hmfem_coupled_status *st)
+{
+    if (!c || !grid_ok(g) || !Kx || !Ky || !Kz || !bval || !p || !pc || max_passes < 0) return HMFEM_ERR_ARG;
+    auto t_start = std::chrono::steady_clock::now();
+    ConduitLayout L;
+    int rc = conduit_layout(g, cs, L);
+    if (rc) return rc;
+    const long n = (long)g->nx * g->ny * g->nz;
+    const size_t nc = (size_t)L.cells;
+    ConduitArgs a{};
+    a.desc = put_vec(c, "cd_desc", L.desc);
+    a.res_cell = put_vec(c, "cd_res", L.res_cell);
+    a.brec_cell = put_vec(c, "cd_bcell", L.brec_cell);
+    a.brec_beta = put_vec(c, "cd_bbeta", L.brec_beta);
+    a.brec_C = put_vec(c, "cd_bC", L.brec_C);
+    const int *xres = put_vec(c, "cd_xres", L.xres), *xcell = put_vec(c, "cd_xcell", L.xcell);
+    const double *kival = put_vec(c, "cd_kival", L.kival);
+    a.ucR = (double *)c->get("cd_ucR", L.nbrec * 8 + 16);
+    a.ucL = (double *)c->get("cd_ucL", L.nbrec * 8 + 16);
+    double *pc_old = (double *)c->get("cd_pcold", nc * 8);
+    a.kl = (double *)c->get("cd_kl", nc * 8);
+    a.kr = (double *)c->get("cd_kr", nc * 8);
+    a.kc = (double *)c->get("cd_kc", nc * 8);
+    a.fbuf = (double *)c->get("cd_f", nc * 8);
+    a.cp = (double *)c->get("cd_cp", nc * 8);
+    double *KI = (double *)c->get("cd_KI", n * 8);
+    double *pc_aux = (double *)c->get("cd_pcaux", n * 8);
+    double *p_old = (double *)c->get("cd_pold", n * 8);
+    double *part = (double *)c->get("cd_part", 2 * kRelChangeBlocks * 8);
+    double *dout = (double *)c->get("cd_out", 64);
+    if (!a.desc || !a.res_cell || !a.brec_cell || !a.brec_beta || !a.brec_C || !xres || !xcell || !kival || !a.ucR ||
+        !a.ucL || !pc_old || !a.kl || !a.kr || !a.kc || !a.fbuf || !a.cp || !KI || !pc_aux || !p_old || !part || !dout)
+        return HMFEM_ERR_NOMEM;
+    a.p = p;
+    a.Kx = Kx; a.Ky = Ky; a.Kz = Kz;
+    a.hx = g->hx; a.hy = g->hy; a.hz = g->hz;
+    a.mu = g->viscosidade;
+    a.rho = cs->rho;
+    a.pc_old = pc_old;
+    a.pc = pc;
+    HMFEM_CHECK(hipMemsetAsync(a.ucR, 0, L.nbrec * 8 + 16, c->st));
+    HMFEM_CHECK(hipMemsetAsync(a.ucL, 0, L.nbrec * 8 + 16, c->st));
+    HMFEM_CHECK(hipMemsetAsync(pc_old, 0, nc * 8, c->st));
+    HMFEM_CHECK(hipMemsetAsync(KI, 0, n * 8, c->st));
+    HMFEM_CHECK(hipMemsetAsync(pc_aux, 0, n * 8, c->st));
+
+    hmfem_options def;
+    if (!opt) {   // the reference's stop rule; the operator is fixed after the first pass
+        hmfem_default_options(&def);
+        def.stop_rule = HMFEM_STOP_PRECONDITIONED;
+        def.reuse_setup = 1;
+        opt = &def;
+    }
+    hmfem_coupled_status tmp{};
+    hmfem_coupled_status *out = st ? st : &tmp;
+    *out = hmfem_coupled_status{};
+    bool slow_step = false;
+    auto reservoir = [&](const double *ki, const double *wi, const double *pcx) -> int {
+        hmfem_status rs{};
+        int r = hmfem_reservoir_device(c, g, Kx, Ky, Kz, ki, wi, pcx, pw, bval, p, nullptr, opt, &rs);
+        if (r < 0) return r;
+        if (r == HMFEM_NOT_CONVERGED) slow_step = true;
+        out->iterations += rs.iterations;
+        out->t_reservoir_ms += rs.t_assemble_ms + rs.t_setup_ms + rs.t_solve_ms;
+        return HMFEM_OK;
+    };
+    // main.c:395-396: the first step has no well (WI still zero) and no conduits
+    if ((rc = reservoir(nullptr, nullptr, nullptr))) return rc;
+    HMFEM_CHECK(hipMemcpyAsync(p_old, p, n * 8, hipMemcpyDeviceToDevice, c->st));
+    // main.c:398-446: the karst index on the conduits' cells, then conduit()
+    if ((rc = conduit_scatter(c->st, (int)L.xres.size(), xres, xcell, kival, KI))) return rc;
+    if ((rc = conduit_pass(c->st, a, cs->count, L.mmax))) return rc;
+    if (pc_first) HMFEM_CHECK(hipMemcpyAsync(pc_first, pc, nc * 8, hipMemcpyDeviceToDevice, c->st));
+    double error = 1.0;
+    int passes = 0;
+    while (passes < max_passes && error > tol) {   // main.c:457-514
+        HMFEM_CHECK(hipMemcpyAsync(pc_old, pc, nc * 8, hipMemcpyDeviceToDevice, c->st));
+        if ((rc = conduit_scatter(c->st, (int)L.xres.size(), xres, xcell, pc_old, pc_aux))) return rc;
+        if ((rc = reservoir(KI, WI, pc_aux))) return rc;
+        if ((rc = rel_change(c->st, n, p, p_old, part, dout))) return rc;
+        if ((rc = conduit_pass(c->st, a, cs->count, L.mmax))) return rc;
+        HMFEM_CHECK(hipMemcpyAsync(&error, dout, 8, hipMemcpyDeviceToHost, c->st));
+        HMFEM_CHECK(hipStreamSynchronize(c->st));
+        if (pass_change) pass_change[passes] = error;
+        passes++;
+    }
+    if (ucl && ucr && (rc = conduit_flux(c->st, a, cs->count, ucl, ucr))) return rc;
+    if (v && (rc = hmfem_flux_device(c, g, Kx, Ky, Kz, bval, p, v, nullptr, nullptr))) return rc;
+    HMFEM_CHECK(hipStreamSynchronize(c->st));
+    out->passes = passes;
+    out->change = error;
+    out->status = (error > tol || slow_step) ? HMFEM_NOT_CONVERGED : HMFEM_OK;
+    out->t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    out->t_conduit_ms = out->t_total_ms - out->t_reservoir_ms;
+    return out->status;
 }
 
 extern "C" int hmfem_solve_structured_device(hmfem_context *c, int nx, int ny, int nz, const ELE *perm,

@@ -130,6 +130,35 @@ int launch_divergence(hipStream_t st, const hmfem_grid *h, const ELE *v, double 
                       double *out);
 int launch_prod_mat_vec(hipStream_t st, long n, const ELE *perm, const IND *indcol, const double *x, double *res);
 
+// 1-D karst conduits (hmfem_conduit.hip)
+constexpr int kConduitLdsCells = 1024;   // conduits up to this length are solved in LDS (PCR)
+constexpr int kRelChangeBlocks = 512;    // blocks of the relative-change reduction
+
+struct ConduitDesc {
+    int off, m, axis, bc_l, bc_r, nbrec, boff;   // cells [off, off + m); breccias [boff, boff + nbrec)
+    long rs_l, rsa_l, rs_r, rsa_r;               // Robin ends: reservoir cell outside the end, end cell
+    double hs, Kc, KI_c, cont_l, cont_r, ccont_l, ccont_r;
+};
+
+struct ConduitArgs {
+    const ConduitDesc *desc;
+    const int *res_cell;                         // reservoir cell of each conduit cell
+    const int *brec_cell;                        // per breccia (conduit by conduit): cell index in its conduit
+    const double *brec_beta, *brec_C;
+    double *ucR, *ucL;                           // breccia velocities (carried between passes)
+    const double *p;                             // reservoir pressure (exchange and Robin ends)
+    const double *Kx, *Ky, *Kz;
+    double hx, hy, hz, mu, rho;
+    const double *pc_old;                        // previous pass's conduit pressure
+    double *pc, *kl, *kr, *kc, *fbuf, *cp;       // pressure out, transmissibilities, Thomas scratch
+};
+
+int conduit_pass(hipStream_t st, const ConduitArgs &a, int count, int mmax);
+int conduit_flux(hipStream_t st, const ConduitArgs &a, int count, double *ucl, double *ucr);
+int conduit_scatter(hipStream_t st, int count, const int *res, const int *cell, const double *pc_old,
+                    double *pc_aux);
+int rel_change(hipStream_t st, long n, const double *p, double *p_old, double *partial, double *out);
+
 // per-kernel timing collected by the solver (profile classes below)
 enum {
     HMFEM_PROF_CG_SPMV = 0,

@@ -93,6 +93,56 @@ class BatchStatus(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ConduitSetC(ctypes.Structure):
+    _fields_ = [("count", c_int), ("cells", c_vp), ("start", c_vp), ("axis", c_vp), ("Kc", c_vp), ("bc_l", c_vp),
+                ("bc_r", c_vp), ("cont", c_vp), ("ccont", c_vp), ("karst_index", c_vp), ("area", c_vp),
+                ("breccias", c_vp), ("max_breccias", c_int), ("breccia_cell", c_vp), ("breccia_beta", c_vp),
+                ("breccia_C", c_vp), ("rho", c_dbl)]
+
+
+class CoupledStatus(ctypes.Structure):
+    _fields_ = [("status", c_int), ("passes", c_int), ("change", c_dbl), ("iterations", c_int),
+                ("t_reservoir_ms", c_dbl), ("t_conduit_ms", c_dbl), ("t_total_ms", c_dbl)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ConduitSet:
+    """A conduit set (the reference's data file, main.c:197-300) as host
+    arrays; .struct() is the hmfem_conduit_set the library takes. Keyword
+    arrays: cells, start (3 per conduit), axis, Kc, bc_l, bc_r, cont and ccont
+    ([side * count + j]), karst_index, area, breccias; breccia_cell,
+    breccia_beta, breccia_C as [max_breccias][count]; rho."""
+
+    def __init__(self, cells, start, axis, Kc, bc_l, bc_r, cont, ccont, karst_index, area, breccias,
+                 breccia_cell=None, breccia_beta=None, breccia_C=None, rho=1000.0):
+        i32 = lambda a: np.ascontiguousarray(a, np.int32).reshape(-1)
+        f64 = lambda a: np.ascontiguousarray(a, np.float64).reshape(-1)
+        self.cells, self.start, self.axis = i32(cells), i32(start), i32(axis)
+        self.count = int(self.cells.size)
+        self.Kc, self.bc_l, self.bc_r = f64(Kc), i32(bc_l), i32(bc_r)
+        self.cont, self.ccont = f64(cont), f64(ccont)
+        self.karst_index, self.area, self.breccias = f64(karst_index), f64(area), i32(breccias)
+        if breccia_cell is None or np.asarray(breccia_cell).size == 0:
+            self.max_breccias = 0
+            self.breccia_cell = np.zeros(1, np.int32)
+            self.breccia_beta = np.zeros(1)
+            self.breccia_C = np.zeros(1)
+        else:
+            self.max_breccias = int(np.asarray(breccia_cell).reshape(-1, self.count).shape[0])
+            self.breccia_cell, self.breccia_beta, self.breccia_C = i32(breccia_cell), f64(breccia_beta), f64(breccia_C)
+        self.rho = float(rho)
+        self.total_cells = int(self.cells.sum())
+
+    def struct(self):
+        P = lambda a: a.ctypes.data
+        return ConduitSetC(self.count, P(self.cells), P(self.start), P(self.axis), P(self.Kc), P(self.bc_l),
+                           P(self.bc_r), P(self.cont), P(self.ccont), P(self.karst_index), P(self.area),
+                           P(self.breccias), self.max_breccias, P(self.breccia_cell), P(self.breccia_beta),
+                           P(self.breccia_C), self.rho)
+
+
 _typed = False
 
 
@@ -123,6 +173,9 @@ def lib():
         L.hmfem_reservoir_device.argtypes = [c_vp, P(HmfemGrid)] + [c_vp] * 7 + [c_vp, c_vp, c_vp, P(Options),
                                                                               P(Status)]
         L.hmfem_flux_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(c_dbl), P(c_dbl)]
+        L.hmfem_coupled_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(ConduitSetC),
+                                           c_int, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(Options),
+                                           P(CoupledStatus)]
         L.hmfem_flux_device_noalloc.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(c_dbl),
                                                 P(c_dbl)]
         L.hmfem_solve_structured_device.argtypes = [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, P(Options),
@@ -152,6 +205,7 @@ def lib():
 EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", "fluxos", "divvel_pontual",
             "prod_mat_vec", "hmfem_default_options", "hmfem_create", "hmfem_destroy", "hmfem_stream",
             "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device", "hmfem_flux_device_noalloc",
+            "hmfem_coupled_device",
             "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
             "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device",
             "hmfem_reservoir_block_device", "hmfem_autotune_device", "save_pressure", "save_flux",
@@ -352,6 +406,22 @@ class Context:
         if rc != 0:
             raise RuntimeError("hmfem_flux_device failed: %d" % rc)
         return mn.value, mx.value
+
+    def coupled(self, g, Kx, Ky, Kz, bval, conduits, p, pc, WI=None, pw=None, max_passes=1000, tol=1e-8,
+                ucl=None, ucr=None, v=None, pc_first=None, options=None):
+        """hmfem_coupled_device: main.c's reservoir / conduit fixed-point loop.
+        Returns (status, per-pass relative changes)."""
+        st = CoupledStatus()
+        changes = np.zeros(max(max_passes, 1))
+        cs = conduits.struct()
+        _order_after_torch(Kx, Ky, Kz, p, pc, WI, pw, ucl, ucr, v, pc_first, *bval)
+        rc = lib().hmfem_coupled_device(self.h, ctypes.byref(g), _dptr(Kx), _dptr(Ky), _dptr(Kz), self._faces(bval),
+                                        _dptr(WI), _dptr(pw), ctypes.byref(cs), max_passes, tol, _dptr(p), _dptr(pc),
+                                        _dptr(ucl), _dptr(ucr), _dptr(v), _dptr(pc_first), changes.ctypes.data,
+                                        ctypes.byref(options) if options is not None else None, ctypes.byref(st))
+        if rc < 0:
+            raise RuntimeError("hmfem_coupled_device failed: %d" % rc)
+        return st, changes[:st.passes]
 
     def flux_noalloc(self, g, Kx, Ky, Kz, bval, p, v=None):
         """hmfem_flux_device_noalloc: the same results, no device allocation."""

@@ -302,6 +302,71 @@ int hmfem_reservoir_block_device(hmfem_context *ctx, const hmfem_grid *g, const 
                                  double *const *p, ELE *const *v, const hmfem_options *opt,
                                  hmfem_block_status *st, int *iterations, double *rel_residual, double *div_max);
 
+/* The reference's 1-D karst conduit model (karst.c, main.c:197-300): each
+   conduit is a line of cells along one axis of the reservoir grid, with its
+   own permeability, exchanging fluid with the reservoir cells it runs
+   through (karst index), with collapse breccias (a quadratic head loss
+   between two of its cells) and Dirichlet (0), Neumann (1) or non-linear
+   Robin (3) ends. Host arrays, laid out as the reference's data file
+   (data_<nk>_<type>.dat) lists them: [j] per conduit, [side*count + j] per
+   end (side 0 the start, 1 the far end), [i*count + j] per breccia
+   (i < max_breccias). */
+typedef struct {
+    int count;                   /* number_karst */
+    const int *cells;            /* M[j].s */
+    const int *start;            /* [3*j .. 3*j+2]: M[j].sxi, syi, szi (first cell) */
+    const int *axis;             /* M[j].pos: 1 x, 2 y, 3 z */
+    const double *Kc;            /* conduit permeability */
+    const int *bc_l, *bc_r;      /* BoundKarst->l[j], ->r[j] */
+    const double *cont;          /* cont_karst[side][j]: end value (Robin: R^4) */
+    const double *ccont;         /* C_contorno[side][j] */
+    const double *karst_index;   /* KI */
+    const double *area;          /* Ac */
+    const int *breccias;         /* n_brechas[j] */
+    int max_breccias;
+    const int *breccia_cell;     /* brechas[i][j]: 0 <= cell < M[j].s - 1 */
+    const double *breccia_beta;  /* beta[i][j] = R^4 - 1 (as main.c stores it) */
+    const double *breccia_C;     /* C[i][j] */
+    double rho;                  /* fluid density (main.c:130: 1000) */
+} hmfem_conduit_set;
+
+typedef struct {
+    int status;                  /* HMFEM_OK, HMFEM_NOT_CONVERGED (pass cap reached before tol) or an error */
+    int passes;                  /* fixed-point passes after the first reservoir solve (main.c's iter) */
+    double change;               /* relative change of the reservoir pressure in the last pass */
+    int iterations;              /* CG iterations of all reservoir solves */
+    double t_reservoir_ms;       /* reservoir steps (assembly, setup, solve) */
+    double t_conduit_ms;         /* conduit passes, exchange terms and convergence tests */
+    double t_total_ms;
+} hmfem_coupled_status;
+
+/* The coupled reservoir / conduit model of main.c:395-530 on the device:
+   a reservoir step without well and conduits (main.c:395), then the well
+   (WI, pw: device arrays of n; NULL for none) and the conduits' karst index,
+   a conduit pass, and the fixed-point loop (reservoir step warm-started from
+   the previous pressure with the conduit pressures in its source term, then a
+   conduit pass with the new reservoir pressure) until the reservoir pressure
+   changes by at most tol between passes (main.c:457-514 with tol = 1e-8;
+   the reference's max search over the per-conduit changes keeps the
+   reservoir's) or max_passes passes; then fluxo1d() and fluxos(). p (device,
+   n) is the first step's initial guess (main.c: zero) and the final
+   pressure; pc, ucl, ucr (device, sum of cells: conduit j's cells follow
+   conduit j-1's) receive the conduit pressure and left/right face fluxes;
+   v (device ELE[n] or NULL) the reservoir fluxes; pc_first (device or NULL)
+   the conduit pressure of the pass before the loop; pass_change (host,
+   max_passes, or NULL) each pass's change. The conduit systems are solved
+   exactly (parallel cyclic reduction), where the reference's compute() stops
+   its AMG-PCG at 1e-8. opt applies to every reservoir step (the reference's
+   stop rule, HMFEM_STOP_PRECONDITIONED, keeps the later warm-started passes
+   iterating; reuse_setup = 1 reuses the hierarchy, exact after the first
+   pass since the operator no longer changes); opt NULL: the defaults with
+   those two. */
+int hmfem_coupled_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
+                         const double *Kz, const double *const bval[6], const double *WI, const double *pw,
+                         const hmfem_conduit_set *cs, int max_passes, double tol, double *p, double *pc,
+                         double *ucl, double *ucr, ELE *v, double *pc_first, double *pass_change,
+                         const hmfem_options *opt, hmfem_coupled_status *st);
+
 /* Per-kernel timing: with profiling enabled, every fine-level and CG launch
    of the following calls is bracketed by HIP events on the context's stream
    (capacity: max_launches launches per call). hmfem_profile_read returns, per

@@ -653,3 +653,228 @@ void oracle_divvel_pontual(const oracle_grid *g, const oracle_ele *v, double *di
     *divmin = fabs(minn);
     *divmax = fabs(maxx);
 }
+
+/* ------------------------------------------------------------------ 1-D conduits */
+
+#define KX(cs, i, j) ((i) * (cs)->count + (j))
+
+/* permeabilidade_karst (permeabilities.c:795-867) */
+static void karst_perm(const oracle_conduits *cs, oracle_karst *st, int j, double visc)
+{
+    const int m = cs->cells[j];
+    const double hs = st->hs[j];
+    for (int i = 0; i < m - 1; i++) {
+        double t = hm(st->KS[KX(cs, i, j)], st->KS[KX(cs, i + 1, j)]);
+        st->kr[KX(cs, i, j)] = t / (hs * hs * visc);
+        st->ind_r[i] = i + 1;
+    }
+    for (int i = 1; i < m; i++) {
+        double t = hm(st->KS[KX(cs, i, j)], st->KS[KX(cs, i - 1, j)]);
+        st->kl[KX(cs, i, j)] = t / (hs * hs * visc);
+        st->ind_l[i] = i - 1;
+    }
+    const double S0 = st->KS[KX(cs, 0, j)], Sm = st->KS[KX(cs, m - 1, j)];
+    if (cs->bc_l[j] == 0) { st->kl[KX(cs, 0, j)] = 2 * (S0) / (hs * hs * visc); st->ind_l[0] = -1; }
+    if (cs->bc_l[j] == 1) { st->kl[KX(cs, 0, j)] = 0; st->ind_l[0] = -1; }
+    if (cs->bc_l[j] == 3) { st->kl[KX(cs, 0, j)] = 2 * S0 / (visc * hs * hs); st->ind_l[0] = -1; }
+    if (cs->bc_r[j] == 0) { st->kr[KX(cs, m - 1, j)] = 2 * (Sm) / (hs * hs * visc); st->ind_r[m - 1] = -1; }
+    if (cs->bc_r[j] == 1) { st->kr[KX(cs, m - 1, j)] = 0; st->ind_r[m - 1] = -1; }
+    if (cs->bc_r[j] == 3) { st->kr[KX(cs, m - 1, j)] = 2 * Sm / (visc * hs * hs); st->ind_r[m - 1] = -1; }
+    for (int i = 0; i < m; i++) {
+        st->kc[KX(cs, i, j)] = st->kl[KX(cs, i, j)] + st->kr[KX(cs, i, j)] + st->KI_c[KX(cs, i, j)] / visc;
+        st->ind_c[i] = i;
+    }
+}
+
+/* The reservoir face transmissibility and the K-weighted mean pressure across
+   the face between reservoir cells s (outside the conduit) and sa (its end),
+   as the Robin branches of source_term1d and fluxo1d compute them. */
+static void robin_face(const oracle_grid *g, const double *Kx, const double *Ky, const double *Kz,
+                       const oracle_ele *perm, const double *p_old, int axis, int left, int s, int sa, double *u,
+                       double *lw)
+{
+    const double *K = axis == 1 ? Kx : axis == 2 ? Ky : Kz;
+    double hf = axis == 1 ? g->hx : axis == 2 ? g->hy : g->hz;
+    double T;
+    if (axis == 1) T = left ? perm[s].r : perm[s].l;
+    else if (axis == 2) T = left ? perm[s].u : perm[s].d;
+    else T = left ? perm[s].t : perm[s].b;
+    *u = hf * T * (p_old[s] - p_old[sa]);
+    *lw = (K[s] * p_old[s] + K[sa] * p_old[sa]) / (K[s] + K[sa]);
+}
+
+/* Cell of the reservoir at offset `off` along conduit j's axis from its start. */
+static int conduit_cell(const oracle_conduits *cs, const oracle_grid *g, int j, int off)
+{
+    int i = cs->sx[j], jj = cs->sy[j], k = cs->sz[j];
+    if (cs->axis[j] == 1) i += off;
+    else if (cs->axis[j] == 2) jj += off;
+    else k += off;
+    return PIDX(g, i, jj, k);
+}
+
+/* source_term1d (set_source_term.c:256-414) */
+static void karst_source(const oracle_conduits *cs, oracle_karst *st, int j, const oracle_grid *g, const double *Kx,
+                         const double *Ky, const double *Kz, const oracle_ele *perm, const double *p_old, double visc,
+                         double *f)
+{
+    const int m = cs->cells[j];
+    const double hs = st->hs[j], rho = cs->rho;
+    for (int i = 0; i < m; i++) f[i] = (st->KI_c[KX(cs, i, j)] * st->p_aux[KX(cs, i, j)]) / visc;
+    for (int i = 0; i < cs->nbrec[j]; i++) {
+        const int s = cs->brec[KX(cs, i, j)];
+        const double cR = hm(st->KS[KX(cs, s, j)], st->KS[KX(cs, s + 1, j)]);
+        const double Cb = cs->C[KX(cs, i, j)], beta = cs->beta[KX(cs, i, j)];
+        double R;
+        if (st->ucR_old[KX(cs, i, j)] < 0) R = -Cb * beta;
+        else R = Cb * beta;
+        const double uRo = st->ucR_old[KX(cs, i, j)], uLo = st->ucL_old[KX(cs, i, j)];
+        const double pcs = st->pc_old[KX(cs, s, j)], pcs1 = st->pc_old[KX(cs, s + 1, j)];
+        double uR = (cR / (visc * hs)) * (pcs - pcs1 - rho * uRo * uRo * R);
+        double uL = (cR / (visc * hs)) * (pcs1 - pcs + rho * uLo * uLo * R);
+        st->ucR_old[KX(cs, i, j)] = uR;
+        st->ucL_old[KX(cs, i, j)] = uL;
+        f[s] = f[s] + (cR / (visc * hs)) * (rho * uR * uR * R);
+        f[s + 1] = f[s + 1] - (cR / (visc * hs)) * (rho * uL * uL * R);
+    }
+    /* left end */
+    if (cs->bc_l[j] == 0) f[0] = f[0] + (st->kl[KX(cs, 0, j)]) * cs->cont[KX(cs, 0, j)];
+    if (cs->bc_l[j] == 1) f[0] = f[0] + cs->cont[KX(cs, 0, j)] / (hs);
+    if (cs->bc_l[j] == 3) {
+        int s = conduit_cell(cs, g, j, -1), sa = conduit_cell(cs, g, j, 0);
+        double u, lw, c;
+        robin_face(g, Kx, Ky, Kz, perm, p_old, cs->axis[j], 1, s, sa, &u, &lw);
+        if (u < 0) c = -cs->ccont[KX(cs, 0, j)] * (cs->cont[KX(cs, 0, j)] - 1);
+        else c = cs->ccont[KX(cs, 0, j)] * (cs->cont[KX(cs, 0, j)] - 1);
+        double font = -lw + rho * u * u * c;
+        f[0] = f[0] - 2 * st->KS[KX(cs, 0, j)] / (visc * hs * hs) * font;
+    }
+    /* right end (the reference has no branch for a Robin right end along z;
+       such conduits are rejected by the callers) */
+    if (cs->bc_r[j] == 0) f[m - 1] = f[m - 1] + st->kr[KX(cs, m - 1, j)] * cs->cont[KX(cs, 1, j)];
+    if (cs->bc_r[j] == 1) f[m - 1] = f[m - 1] - cs->cont[KX(cs, 1, j)] / (hs);
+    if (cs->bc_r[j] == 3 && cs->axis[j] != 3) {
+        int s = conduit_cell(cs, g, j, m), sa = conduit_cell(cs, g, j, m - 1);
+        double u, lw, c;
+        robin_face(g, Kx, Ky, Kz, perm, p_old, cs->axis[j], 0, s, sa, &u, &lw);
+        if (u < 0) c = -cs->ccont[KX(cs, 1, j)] * (cs->cont[KX(cs, 1, j)] - 1);
+        else c = cs->ccont[KX(cs, 1, j)] * (cs->cont[KX(cs, 1, j)] - 1);
+        double font = -lw + rho * u * u * c;
+        f[m - 1] = f[m - 1] - font * 2 * st->KS[KX(cs, m - 1, j)] / (visc * hs * hs);
+    }
+}
+
+/* solve_linsys_1d (solve_linearsystem_AMG.c:200-291): rows in the order
+   l, c, r, then compute(). */
+static void karst_solve(const oracle_conduits *cs, const oracle_karst *st, int j, const double *f, double *x,
+                        double eps)
+{
+    const int m = cs->cells[j];
+    int *cnt = (int *)xmalloc(sizeof(int) * (m + 1));
+    int *col = (int *)xmalloc(sizeof(int) * (3 * m + 1));
+    double *ele = (double *)xmalloc(sizeof(double) * (3 * m + 1));
+    int l = 0;
+    for (int s = 0; s < m; s++) {
+        cnt[s] = 0;
+        if (st->ind_l[s] != -1) { col[l] = st->ind_l[s]; ele[l] = -st->kl[KX(cs, s, j)]; l++; cnt[s]++; }
+        if (st->ind_c[s] != -1) { col[l] = st->ind_c[s]; ele[l] = st->kc[KX(cs, s, j)]; l++; cnt[s]++; }
+        if (st->ind_r[s] != -1) { col[l] = st->ind_r[s]; ele[l] = -st->kr[KX(cs, s, j)]; l++; cnt[s]++; }
+    }
+    oracle_compute(m, l, cnt, col, ele, f, x, eps);
+    free(cnt); free(col); free(ele);
+}
+
+/* Thomas algorithm on the same tridiagonal matrix (exact solve; the GPU's
+   batched solve is checked against this). */
+static void karst_solve_direct(const oracle_conduits *cs, const oracle_karst *st, int j, const double *f, double *x)
+{
+    const int m = cs->cells[j];
+    double *cp = (double *)xmalloc(sizeof(double) * m), *dp = (double *)xmalloc(sizeof(double) * m);
+    for (int i = 0; i < m; i++) {
+        double a = i > 0 ? -st->kl[KX(cs, i, j)] : 0.0;
+        double b = st->kc[KX(cs, i, j)];
+        double c = i < m - 1 ? -st->kr[KX(cs, i, j)] : 0.0;
+        double den = i > 0 ? b - a * cp[i - 1] : b;
+        cp[i] = c / den;
+        dp[i] = (i > 0 ? f[i] - a * dp[i - 1] : f[i]) / den;
+    }
+    for (int i = m - 1; i >= 0; i--) x[i] = i < m - 1 ? dp[i] - cp[i] * x[i + 1] : dp[i];
+    free(cp); free(dp);
+}
+
+void oracle_conduit(const oracle_conduits *cs, oracle_karst *st, const oracle_grid *g, const double *Kx,
+                    const double *Ky, const double *Kz, const oracle_ele *perm, const double *p_old, double visc,
+                    double eps1d, int direct)
+{
+    for (int j = 0; j < cs->count; j++) {
+        const int m = cs->cells[j];
+        double *x1 = (double *)calloc(m, sizeof(double)), *f = (double *)calloc(m, sizeof(double));
+        karst_perm(cs, st, j, visc);
+        karst_source(cs, st, j, g, Kx, Ky, Kz, perm, p_old, visc, f);
+        if (direct) karst_solve_direct(cs, st, j, f, x1);
+        else karst_solve(cs, st, j, f, x1, eps1d);
+        for (int i = 0; i < m; i++) st->pc[KX(cs, i, j)] = x1[i];
+        free(f); free(x1);
+    }
+}
+
+void oracle_fluxo1d(const oracle_conduits *cs, const oracle_karst *st, const oracle_grid *g, const double *Kx,
+                    const double *Ky, const double *Kz, const oracle_ele *perm, const double *p_old, double visc,
+                    const double *x, double *ucl, double *ucr)
+{
+    const double rho = cs->rho;
+    for (int j = 0; j < cs->count; j++) {
+        const int m = cs->cells[j];
+        const double hs = st->hs[j];
+        for (int i = 0; i < m - 1; i++)
+            ucr[KX(cs, i, j)] = hs * st->kr[KX(cs, i, j)] * (x[KX(cs, i, j)] - x[KX(cs, i + 1, j)]);
+        for (int i = 1; i < m; i++)
+            ucl[KX(cs, i, j)] = hs * st->kl[KX(cs, i, j)] * (x[KX(cs, i, j)] - x[KX(cs, i - 1, j)]);
+        for (int i = 0; i < cs->nbrec[j]; i++) {
+            const double Cb = cs->C[KX(cs, i, j)], beta = cs->beta[KX(cs, i, j)];
+            double c;
+            if (st->ucR_old[KX(cs, i, j)] < 0) c = -Cb * beta;
+            else c = Cb * beta;
+            const int s = cs->brec[KX(cs, i, j)];
+            const double cR = hm(st->KS[KX(cs, s, j)], st->KS[KX(cs, s + 1, j)]);
+            const double uRo = st->ucR_old[KX(cs, i, j)], uLo = st->ucL_old[KX(cs, i, j)];
+            ucr[KX(cs, s, j)] = (cR / (visc * hs)) * (x[KX(cs, s, j)] - x[KX(cs, s + 1, j)] - 0.5 * rho * uRo * uRo * c);
+            ucl[KX(cs, s + 1, j)] =
+                (cR / (visc * hs)) * (x[KX(cs, s + 1, j)] - x[KX(cs, s, j)] + 0.5 * rho * uLo * uLo * c);
+        }
+        /* left end; a Neumann end reports -cont_karst[1][j], as the reference does */
+        if (cs->bc_l[j] == 0) ucl[KX(cs, 0, j)] = hs * st->kl[KX(cs, 0, j)] * (x[KX(cs, 0, j)] - cs->cont[KX(cs, 0, j)]);
+        if (cs->bc_l[j] == 1) ucl[KX(cs, 0, j)] = -cs->cont[KX(cs, 1, j)];
+        if (cs->bc_l[j] == 3) {
+            int s = conduit_cell(cs, g, j, -1), sa = conduit_cell(cs, g, j, 0);
+            double u, lw, c;
+            robin_face(g, Kx, Ky, Kz, perm, p_old, cs->axis[j], 1, s, sa, &u, &lw);
+            if (u < 0) c = -cs->ccont[KX(cs, 0, j)] * (cs->cont[KX(cs, 0, j)] - 1);
+            else c = cs->ccont[KX(cs, 0, j)] * (cs->cont[KX(cs, 0, j)] - 1);
+            ucl[KX(cs, 0, j)] = (2 * st->KS[KX(cs, 0, j)]) / (visc * hs) * (x[KX(cs, 0, j)] - lw + rho * u * u * c);
+        }
+        /* right end */
+        if (cs->bc_r[j] == 0)
+            ucr[KX(cs, m - 1, j)] = hs * st->kr[KX(cs, m - 1, j)] * (x[KX(cs, m - 1, j)] - cs->cont[KX(cs, 1, j)]);
+        if (cs->bc_r[j] == 1) ucr[KX(cs, m - 1, j)] = cs->cont[KX(cs, 1, j)];
+        if (cs->bc_r[j] == 3) {
+            int s = conduit_cell(cs, g, j, m), sa = conduit_cell(cs, g, j, m - 1);
+            double u, lw, c;
+            robin_face(g, Kx, Ky, Kz, perm, p_old, cs->axis[j], 0, s, sa, &u, &lw);
+            if (u < 0) c = -cs->ccont[KX(cs, 1, j)] * (cs->cont[KX(cs, 1, j)] - 1);
+            else c = cs->ccont[KX(cs, 1, j)] * (cs->cont[KX(cs, 1, j)] - 1);
+            ucr[KX(cs, m - 1, j)] =
+                (2 * st->KS[KX(cs, m - 1, j)]) / (visc * hs) * (x[KX(cs, m - 1, j)] - lw + rho * u * u * c);
+        }
+    }
+}
+
+double oracle_rel_change(int n, const double *y, const double *y_old)
+{
+    double s1 = 0, s2 = 0;
+    for (int i = 0; i < n; i++) {
+        s1 = s1 + (y[i] - y_old[i]) * (y[i] - y_old[i]);
+        s2 = s2 + y[i] * y[i];
+    }
+    return sqrt(s1) / sqrt(s2);
+}

@@ -74,6 +74,60 @@ void oracle_fluxos(const oracle_grid *g, const double *p, const oracle_ele *perm
 /* divvel_pontual (flux_reconstruction.c:281-319) without the print. */
 void oracle_divvel_pontual(const oracle_grid *g, const oracle_ele *v, double *divmin, double *divmax);
 
+/* ---- the 1-D karst conduit model (karst.c, permeabilities.c:795-867,
+   set_source_term.c:256-414, solve_linearsystem_AMG.c:200-291,
+   flux_reconstruction.c:322-496) and its coupling state (main.c:197-514).
+   The reference's matrices X[i][j] (i: cell or breccia, j: conduit) are flat
+   row-major arrays X[i * count + j] with mmax rows (cells) or max_brec rows
+   (breccias). */
+typedef struct {
+    int count;                  /* number_karst */
+    int mmax;                   /* Mmax: cells of the longest conduit */
+    int max_brec;               /* rows of the breccia matrices */
+    const int *cells;           /* M[j].s */
+    const int *sx, *sy, *sz;    /* M[j].sxi, .syi, .szi */
+    const int *axis;            /* M[j].pos: 1 x, 2 y, 3 z */
+    const int *bc_l, *bc_r;     /* BoundKarst->l[j], ->r[j]: 0 Dirichlet, 1 Neumann, 3 Robin */
+    const double *cont;         /* cont_karst[side][j] at [side * count + j] */
+    const double *ccont;        /* C_contorno[side][j] at [side * count + j] */
+    const int *nbrec;           /* n_brechas[j] */
+    const int *brec;            /* brechas[i][j] */
+    const double *beta;         /* beta[i][j] (R^4 - 1, as main.c stores it) */
+    const double *C;            /* C[i][j] */
+    double rho;
+} oracle_conduits;
+
+/* Per-conduit arrays the model keeps (main.c's globals and locals). */
+typedef struct {
+    double *KS;                 /* K->S[i][j] */
+    double *KI_c;               /* KI_c[i][j] */
+    double *hs;                 /* hs[j] */
+    double *p_aux;              /* p_aux[i][j]: reservoir pressure along the conduit */
+    double *pc;                 /* pc[i][j]: conduit pressure (out of conduit()) */
+    double *pc_old;             /* pc_old[i][j]: previous pass (read by source_term1d) */
+    double *ucR_old, *ucL_old;  /* breccia velocities [i][j], updated by source_term1d */
+    double *kl, *kr, *kc;       /* keff->l, r, c [i][j] */
+    int *ind_l, *ind_c, *ind_r; /* indcol1[i].l, .c, .r (mmax entries, shared by the conduits) */
+} oracle_karst;
+
+/* conduit (karst.c:8-42): for each conduit permeabilidade_karst, source_term1d
+   and the 1-D solve; the solve is solve_linsys_1d + compute at `eps1d`
+   (the reference: 1e-8), or with direct = 1 an exact tridiagonal (Thomas)
+   solve of the same matrix. The reservoir's grid, K, perm and p_old feed the
+   non-linear Robin ends. */
+void oracle_conduit(const oracle_conduits *cs, oracle_karst *st, const oracle_grid *g, const double *Kx,
+                    const double *Ky, const double *Kz, const oracle_ele *perm, const double *p_old, double visc,
+                    double eps1d, int direct);
+
+/* erro_relativo (main.c:106-119): sqrt(sum (y - y_old)^2) / sqrt(sum y^2),
+   summed in index order. */
+double oracle_rel_change(int n, const double *y, const double *y_old);
+
+/* fluxo1d (flux_reconstruction.c:322-496): conduit fluxes of x (= pc). */
+void oracle_fluxo1d(const oracle_conduits *cs, const oracle_karst *st, const oracle_grid *g, const double *Kx,
+                    const double *Ky, const double *Kz, const oracle_ele *perm, const double *p_old, double visc,
+                    const double *x, double *ucl, double *ucr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,18 @@
  * step: the JSON then holds the setup time and the per-iteration time).
  * With "summary" it writes p8, p8w, p12 and the JSON (no matrix, fluxes or
  * level sizes): the large cases.
+ * With "karst <data file> <max passes>" it runs the coupled reservoir/conduit
+ * model as main.c:390-530 does: reservoir() without well or conduits, the
+ * conduit set read from the data file (the reference's data_<nk>_<type>.dat
+ * format, parsed as main.c:197-300 parses it), conduit(), then the
+ * fixed-point loop of reservoir() and conduit() until the reservoir pressure
+ * changes by at most 1e-8 between passes (or the pass cap), then fluxo1d()
+ * and fluxos(). The input's WI and pw arrays are the well; its KI and pc are
+ * ignored (the loop sets them). Outputs: <prefix>.karst.p (final reservoir
+ * pressure), .p1 (after the loop's first pass), .pc0 (conduit pressure
+ * before the loop), .pc1 (after the first pass), .pc (final), .ucl/.ucr
+ * (fluxo1d), .ucR_old/.ucL_old (breccia velocities), .v (fluxos), and
+ * <prefix>.karst.json (passes, per-pass change, iteration counts).
  * With "write <dir>" it also runs the reference's output writers on the tight
  * pressure and its fluxes: save_pressure(), save_flux()
  * (posprocessing_output.c:200-296) and vts_StructuredGrid_file() (vtk.c:8-76)
@@ -119,6 +131,190 @@ static void get(FILE *f, void *dst, size_t bytes)
     if (fread(dst, 1, bytes, f) != bytes) { fprintf(stderr, "short input\n"); exit(2); }
 }
 
+static void put_matrix(const char *prefix, const char *ext, double **m, int rows, int cols)
+{
+    std::vector<double> flat((size_t)rows * cols);
+    for (int i = 0; i < rows; i++)
+        for (int j = 0; j < cols; j++) flat[(size_t)i * cols + j] = m[i][j];
+    put(prefix, ext, flat.data(), flat.size() * sizeof(double));
+}
+
+static void fail(const char *what)
+{
+    fprintf(stderr, "karst input: %s\n", what);
+    exit(2);
+}
+
+/* main.c:197-300 and :390-530 for one problem (see the header comment). */
+static int run_karst(const char *datafile, int max_passes, const char *pre, Perm *K, Bound *Bb, const double *WIin,
+                     const double *pwin)
+{
+    const int n = N.t;
+    FILE *arq = fopen(datafile, "r");
+    if (!arq) fail("cannot open the data file");
+    /* the first line holds number_karst values of M[].s: count them */
+    char line[65536];
+    if (!fgets(line, sizeof(line), arq)) fail("empty data file");
+    int nk = 0;
+    for (char *q = strtok(line, " \t\r\n"); q; q = strtok(nullptr, " \t\r\n")) nk++;
+    rewind(arq);
+    number_karst = nk;
+    DimK *Mk = (DimK *)calloc(nk + 1, sizeof(DimK));
+    Kc = create_vector(nk);
+    FACEK BK;
+    BK.l = create_int(nk);
+    BK.r = create_int(nk);
+    cont_karst = create_matrix(2, nk);
+    C_contorno = create_matrix(2, nk);
+    std::vector<double> karst_index(nk), Ac(nk);
+    n_brechas = create_int(nk);
+    int i, j;
+#define RD(fmt, dst) do { if (fscanf(arq, fmt, dst) != 1) fail("short data file"); } while (0)
+    for (i = 0; i < nk; i++) RD("%i", &Mk[i].s);
+    for (i = 0; i < nk; i++) RD("%i", &Mk[i].sxi);
+    for (i = 0; i < nk; i++) RD("%i", &Mk[i].syi);
+    for (i = 0; i < nk; i++) RD("%i", &Mk[i].szi);
+    for (i = 0; i < nk; i++) RD("%i", &Mk[i].pos);
+    for (i = 0; i < nk; i++) RD("%lf", &Kc[i]);
+    for (i = 0; i < nk; i++) RD("%i", &BK.l[i]);
+    for (i = 0; i < nk; i++) RD("%i", &BK.r[i]);
+    for (j = 0; j < 2; j++) for (i = 0; i < nk; i++) RD("%lf", &cont_karst[j][i]);
+    for (j = 0; j < 2; j++) for (i = 0; i < nk; i++) RD("%lf", &C_contorno[j][i]);
+    for (i = 0; i < nk; i++) RD("%lf", &karst_index[i]);
+    for (i = 0; i < nk; i++) RD("%lf", &Ac[i]);
+    for (i = 0; i < nk; i++) RD("%i", &n_brechas[i]);
+    int max_brechas = -1;
+    for (i = 0; i < nk; i++) if (n_brechas[i] >= max_brechas) max_brechas = n_brechas[i];
+    ::beta = create_matrix(max_brechas, nk);
+    for (j = 0; j < max_brechas; j++) for (i = 0; i < nk; i++) { RD("%lf", &::beta[j][i]); ::beta[j][i] = ::beta[j][i] - 1; }
+    C = create_matrix(max_brechas, nk);
+    for (j = 0; j < max_brechas; j++) for (i = 0; i < nk; i++) RD("%lf", &C[j][i]);
+    brechas = create_matrix_int(max_brechas, nk);
+    for (j = 0; j < max_brechas; j++) for (i = 0; i < nk; i++) RD("%d", &brechas[j][i]);
+#undef RD
+    fclose(arq);
+    hs = create_vector(nk);
+    for (i = 0; i < nk; i++) hs[i] = Mk[i].pos == 1 ? h.x : Mk[i].pos == 2 ? h.y : h.z;
+    int mmax = 0;
+    for (i = 0; i < nk; i++) if (Mk[i].s > mmax) mmax = Mk[i].s;
+    Mmax = mmax;
+    pc = create_matrix(Mmax, nk);
+    pc_old = create_matrix(Mmax, nk);
+    double **KI_c = create_matrix(Mmax, nk);
+    double *KIv = create_vector(n);
+    double *WI0 = create_vector(n);
+    double *pw = create_vector(n);
+    for (int s = 0; s < n; s++) pw[s] = pwin[s];
+    K->S = create_matrix(Mmax, nk);
+    for (j = 0; j < nk; j++) for (i = 0; i < Mmax; i++) K->S[i][j] = Kc[j];   /* set_perm_faces */
+    ucL_old = create_matrix(max_brechas, nk);
+    ucR_old = create_matrix(max_brechas, nk);
+    ELE *pm = (ELE *)calloc(n + 1, sizeof(ELE));
+    ELEK keff;
+    keff.l = create_matrix(Mmax, nk); keff.r = create_matrix(Mmax, nk); keff.c = create_matrix(Mmax, nk);
+    ELEK ucf;
+    ucf.l = create_matrix(Mmax, nk); ucf.r = create_matrix(Mmax, nk); ucf.c = create_matrix(Mmax, nk);
+    ELE *vv = (ELE *)calloc(n + 1, sizeof(ELE));
+    IND *ind1 = (IND *)calloc(Mmax + 1, sizeof(IND));
+    IND *ic = (IND *)calloc(n + 1, sizeof(IND));
+    double *p = create_vector(n), *p_old = create_vector(n);
+    std::vector<int> iters;
+
+    /* main.c:395-396: first pass, no well (WI still zero) and no conduits */
+    reservoir(N, K, WI0, KIv, pc_old, ic, Bb, pm, h, boundary, viscosidade, pw, Mk, p);
+    iters.push_back(iter_amg);
+    for (int s = 0; s < n; s++) p_old[s] = p[s];
+    double *WI = create_vector(n);
+    for (int s = 0; s < n; s++) WI[s] = WIin[s];
+    double **p_aux = create_matrix(Mmax, nk);
+    auto cell = [&](int jj, int ii) {
+        int x = Mk[jj].sxi, y = Mk[jj].syi, z = Mk[jj].szi;
+        if (Mk[jj].pos == 1) x += ii; else if (Mk[jj].pos == 2) y += ii; else z += ii;
+        return x + y * N.x + z * N.xy;
+    };
+    for (j = 0; j < nk; j++)
+        for (i = 0; i < Mk[j].s; i++) {
+            int aux = cell(j, i);
+            KIv[aux] = karst_index[j];
+            KI_c[i][j] = karst_index[j] / (Ac[j] * hs[j]);
+            p_aux[i][j] = p_old[aux];
+            K->S[i][j] = Kc[j];
+        }
+    conduit(p_aux, cont_karst, KI_c, viscosidade, Mk, hs, ind1, &BK, &keff, K, pm, p_old, pc);
+    put_matrix(pre, ".karst.pc0", pc, Mmax, nk);
+
+    /* main.c:457-514; the loop's stop test reduces to the reservoir change
+       (error1[number_karst]), the value the reference's max search keeps */
+    double error = 1;
+    int iter = 0;
+    std::vector<double> errs;
+    while (iter < max_passes && error > 1e-8) {
+        for (i = 0; i < Mmax; i++) for (j = 0; j < nk; j++) pc_old[i][j] = pc[i][j];
+        reservoir(N, K, WI, KIv, pc_old, ic, Bb, pm, h, boundary, viscosidade, pw, Mk, p);
+        iters.push_back(iter_amg);
+        double s1 = 0, s2 = 0;
+        for (int s = 0; s < n; s++) { s1 = s1 + (p[s] - p_old[s]) * (p[s] - p_old[s]); s2 = s2 + p[s] * p[s]; }
+        error = sqrt(s1) / sqrt(s2);
+        errs.push_back(error);
+        for (int s = 0; s < n; s++) p_old[s] = p[s];
+        for (j = 0; j < nk; j++) for (i = 0; i < Mk[j].s; i++) p_aux[i][j] = p_old[cell(j, i)];
+        conduit(p_aux, cont_karst, KI_c, viscosidade, Mk, hs, ind1, &BK, &keff, K, pm, p_old, pc);
+        iter++;
+        if (iter == 1) {
+            put(pre, ".karst.p1", p, sizeof(double) * n);
+            put_matrix(pre, ".karst.pc1", pc, Mmax, nk);
+        }
+    }
+    fluxo1d(pc, &keff, &ucf, hs, Mk, cont_karst, &BK, K, pm, p_old);
+    fluxos(p, pm, vv, Bb, h, N);
+    put(pre, ".karst.p", p, sizeof(double) * n);
+    put_matrix(pre, ".karst.pc", pc, Mmax, nk);
+    put_matrix(pre, ".karst.ucl", ucf.l, Mmax, nk);
+    put_matrix(pre, ".karst.ucr", ucf.r, Mmax, nk);
+    put_matrix(pre, ".karst.ucR_old", ucR_old, max_brechas, nk);
+    put_matrix(pre, ".karst.ucL_old", ucL_old, max_brechas, nk);
+    put_matrix(pre, ".karst.kc", keff.c, Mmax, nk);
+    put(pre, ".karst.v", vv, sizeof(ELE) * n);
+    char name[2048];
+    snprintf(name, sizeof(name), "%s.karst.json", pre);
+    FILE *f = fopen(name, "w");
+    fprintf(f, "{\"number_karst\": %d, \"mmax\": %d, \"max_brec\": %d, \"passes\": %d, \"error\": %.17g,\n",
+            nk, Mmax, max_brechas, iter, error);
+    fprintf(f, " \"cells\": [");
+    for (j = 0; j < nk; j++) fprintf(f, "%s%d", j ? ", " : "", Mk[j].s);
+    fprintf(f, "], \"start\": [");
+    for (j = 0; j < nk; j++) fprintf(f, "%s%d, %d, %d", j ? ", " : "", Mk[j].sxi, Mk[j].syi, Mk[j].szi);
+    fprintf(f, "], \"axis\": [");
+    for (j = 0; j < nk; j++) fprintf(f, "%s%d", j ? ", " : "", Mk[j].pos);
+    fprintf(f, "], \"Kc\": [");
+    for (j = 0; j < nk; j++) fprintf(f, "%s%.17g", j ? ", " : "", Kc[j]);
+    fprintf(f, "], \"bc\": [");
+    for (j = 0; j < nk; j++) fprintf(f, "%s%d, %d", j ? ", " : "", BK.l[j], BK.r[j]);
+    fprintf(f, "], \"cont\": [");
+    for (int sd = 0; sd < 2; sd++) for (j = 0; j < nk; j++) fprintf(f, "%s%.17g", sd + j ? ", " : "", cont_karst[sd][j]);
+    fprintf(f, "], \"ccont\": [");
+    for (int sd = 0; sd < 2; sd++) for (j = 0; j < nk; j++) fprintf(f, "%s%.17g", sd + j ? ", " : "", C_contorno[sd][j]);
+    fprintf(f, "], \"karst_index\": [");
+    for (j = 0; j < nk; j++) fprintf(f, "%s%.17g", j ? ", " : "", karst_index[j]);
+    fprintf(f, "], \"area\": [");
+    for (j = 0; j < nk; j++) fprintf(f, "%s%.17g", j ? ", " : "", Ac[j]);
+    fprintf(f, "], \"nbrec\": [");
+    for (j = 0; j < nk; j++) fprintf(f, "%s%d", j ? ", " : "", n_brechas[j]);
+    fprintf(f, "], \"brec\": [");
+    for (i = 0; i < max_brechas; i++) for (j = 0; j < nk; j++) fprintf(f, "%s%d", i + j ? ", " : "", brechas[i][j]);
+    fprintf(f, "], \"beta\": [");
+    for (i = 0; i < max_brechas; i++) for (j = 0; j < nk; j++) fprintf(f, "%s%.17g", i + j ? ", " : "", ::beta[i][j]);
+    fprintf(f, "], \"C\": [");
+    for (i = 0; i < max_brechas; i++) for (j = 0; j < nk; j++) fprintf(f, "%s%.17g", i + j ? ", " : "", C[i][j]);
+    fprintf(f, "],\n \"pass_change\": [");
+    for (size_t q = 0; q < errs.size(); q++) fprintf(f, "%s%.17g", q ? ", " : "", errs[q]);
+    fprintf(f, "], \"iter_amg\": [");
+    for (size_t q = 0; q < iters.size(); q++) fprintf(f, "%s%d", q ? ", " : "", iters[q]);
+    fprintf(f, "]}\n");
+    fclose(f);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 3) { fprintf(stderr, "usage: %s input.bin out_prefix\n", argv[0]); return 2; }
@@ -187,6 +383,12 @@ int main(int argc, char **argv)
 
     const char *mode = argc > 3 ? argv[3] : "";
     const bool summary = strcmp(mode, "summary") == 0;
+    rho = 1000;   /* main.c:130 */
+
+    if (strcmp(mode, "karst") == 0) {
+        if (argc < 6) { fprintf(stderr, "usage: %s input.bin out_prefix karst data_file max_passes\n", argv[0]); return 2; }
+        return run_karst(argv[4], atoi(argv[5]), argv[2], &Kp, &Bb, WIv, pwv);
+    }
 
     if (strcmp(mode, "time") == 0 && argc > 4) {
         /* bounded timing sample: the parts of one reservoir() step at this

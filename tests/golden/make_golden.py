@@ -5,7 +5,7 @@ and flux sources compiled in place by ``make -C oracle ref``) on small seeded
 problems and stores inputs and outputs as compressed .npz files. Only this
 script runs the reference build; tests read the .npz files.
 
-Usage:  make -C oracle ref && python tests/golden/make_golden.py [writers | summary [name ...]]
+Usage:  make -C oracle ref && python tests/golden/make_golden.py [writers | summary [name ...] | karst [name ...]]
 
 With `writers` it only runs the cases marked `writers`, with the reference's
 output writers (save_pressure, save_flux, vts_StructuredGrid_file) on the
@@ -18,6 +18,16 @@ own 1e-8 stop (p8), after a warm-started second pass (p8w) and at 1e-12
 (p12) at the cells of tests/summary_sample.py, their full-field norms, sums
 and distances, the iteration counts, the divergence of p12's fluxes, the
 reference's timings in this container, and a digest of the K field.
+
+With `karst` it runs the coupled reservoir / 1-D conduit model (main.c's
+fixed-point loop over reservoir() and conduit(), then fluxo1d() and
+fluxos()) on the reference's own setup (main.c:127-152, 354-402: K = 1e-13,
+mu = 1e-3, 60 MPa on the four lateral faces, a well column at the grid
+centre) with conduit sets from the reference's data files and one written
+here, and stores karst/<name>.npz: the conduit set, the conduit pressures
+before the loop, after its first pass and at the end, the conduit fluxes and
+breccia velocities, the reservoir pressure at the mid-planes and conduit
+cells with digests of the whole fields, and the per-pass changes and counts.
 """
 import json
 import os
@@ -35,7 +45,7 @@ sys.path.insert(0, ROOT)
 from hmfem_amd import fields  # noqa: E402
 
 sys.path.insert(0, os.path.join(ROOT, "tests"))
-from summary_sample import field_digest, sample_index  # noqa: E402
+from summary_sample import field_digest, karst_sample_index, sample_index  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "hmfem_ref")
 
@@ -152,6 +162,107 @@ def run_case(c, tmp, writers=False):
     print(c["name"], "n=%d nnz=%d iter8=%d iter12=%d levels=%s" % (n, nnz, out["iter8"], out["iter12"], meta["levels_rows"]))
 
 
+# main.c:127-136, 147-152, 316-326, 354-359, 398-402
+KARST_SETUP = dict(K=1.0e-13, visc=1e-3, bc_type=(0, 0, 0, 0, 1, 1), bc_val=(60e6, 60e6, 60e6, 60e6, 0.0, 0.0),
+                   ppoco=10e6, well_index=7.64352339e-13, well_z0=10)
+
+
+def mixed_conduit_file(path):
+    """A conduit set in the reference's data-file format exercising every end
+    type and axis the reference handles: x with Robin ends and breccias, y
+    Dirichlet/Neumann, z with a non-linear Robin start, y with Robin ends,
+    x Neumann/Dirichlet."""
+    cols = [  # s, sxi, syi, szi, pos, Kc, bl, br, cl, cr, Cl, Cr, KI, Ac, nbrec, [R4], [C], [cells]
+        (30, 5, 8, 12, 1, 1e-11, 3, 3, 1.0, 1.0, 1.0, 1.0, 1.0e-12, 3.1415, 2, [1e4, 1e4], [1.0, 1.0], [10, 20]),
+        (25, 24, 3, 20, 2, 5e-12, 0, 1, 55e6, -2e-4, 1.0, 1.0, 5e-13, 2.0, 1, [16.0], [0.5], [12]),
+        (20, 30, 10, 2, 3, 2e-11, 3, 0, 2.0, 58e6, 1e-4, 1.0, 8e-13, 1.5, 0, [], [], []),
+        (15, 10, 5, 5, 2, 1e-11, 3, 3, 1.5, 1.5, 2e-4, 3e-4, 1.0e-12, 3.1415, 1, [4.0], [1.0], [7]),
+        (10, 2, 30, 25, 1, 3e-12, 1, 0, 1e-3, 60e6, 1.0, 1.0, 2e-13, 1.0, 0, [], [], []),
+    ]
+    nk = len(cols)
+    mb = max(c[14] for c in cols)
+    rows = []
+    for q in range(14):
+        rows.append(" ".join(repr(c[q]) if isinstance(c[q], float) else str(c[q]) for c in cols))
+    rows.append(" ".join(str(c[14]) for c in cols))
+    for key in (15, 16, 17):
+        for i in range(mb):
+            rows.append(" ".join(repr(float(c[key][i])) if key != 17 and i < c[14] else
+                                 (str(c[key][i]) if i < c[14] else ("1" if key != 17 else "0")) for c in cols))
+    with open(path, "w") as f:
+        f.write("\n".join(rows) + "\n")
+    return nk
+
+
+KARST_CASES = [
+    # the reference's own run (roda.sh: exe.sh 210 42 20 210 210 100 1 1 2)
+    dict(name="ref_210x42x20_1_2", dims=(210, 42, 20), lengths=(210.0, 210.0, 100.0), data="data_1_2.dat"),
+    # eleven conduits with twenty breccias each
+    dict(name="ref_210x42x20_11_4", dims=(210, 42, 20), lengths=(210.0, 210.0, 100.0), data="data_11_4.dat"),
+    # every end type and axis
+    dict(name="mixed_48x40x30", dims=(48, 40, 30), lengths=(48.0, 100.0, 60.0), data=None),
+]
+
+
+def run_karst(c, tmp, max_passes=1000):
+    nx, ny, nz = c["dims"]
+    n = nx * ny * nz
+    S = KARST_SETUP
+    K = np.full(n, S["K"])
+    WI = np.zeros(n)
+    pw = np.zeros(n)
+    px, py = nx // 2, ny // 2
+    for z in range(S["well_z0"], nz):
+        WI[px + nx * (py + ny * z)] = S["well_index"]
+        pw[px + nx * (py + ny * z)] = S["ppoco"]
+    inp = os.path.join(tmp, c["name"] + ".in")
+    with open(inp, "wb") as f:
+        f.write(struct.pack("<3i", nx, ny, nz))
+        f.write(struct.pack("<5d", *c["lengths"], S["visc"], 1e-12))
+        f.write(struct.pack("<6i", *S["bc_type"]))
+        f.write(struct.pack("<6d", *S["bc_val"]))
+        f.write(struct.pack("<i", 1))
+        for a in (K, K, K, np.zeros(n), WI, np.zeros(n), pw):
+            f.write(np.ascontiguousarray(a, "<f8").tobytes())
+    if c["data"]:
+        data = os.path.join("/root/reference", c["data"])
+    else:
+        data = os.path.join(tmp, c["name"] + ".dat")
+        mixed_conduit_file(data)
+    pre = os.path.join(tmp, c["name"])
+    subprocess.run([REF, inp, pre, "karst", data, str(max_passes)], check=True, stdout=subprocess.DEVNULL)
+    meta = json.load(open(pre + ".karst.json"))
+    nk, mmax, mb = meta["number_karst"], meta["mmax"], meta["max_brec"]
+    rd = lambda ext, rows: np.fromfile(pre + ".karst." + ext, "<f8").reshape(rows, nk)
+    p = np.fromfile(pre + ".karst.p", "<f8")
+    p1 = np.fromfile(pre + ".karst.p1", "<f8") if os.path.exists(pre + ".karst.p1") else np.zeros(n)
+    v = np.fromfile(pre + ".karst.v", "<f8")
+    idx = karst_sample_index(c["dims"], meta["start"], meta["axis"], meta["cells"])
+    out = dict(
+        dims=np.array(c["dims"], np.int32), lengths=np.array(c["lengths"], np.float64),
+        K=np.float64(S["K"]), visc=np.float64(S["visc"]), bc_type=np.array(S["bc_type"], np.int32),
+        bc_val=np.array(S["bc_val"], np.float64), ppoco=np.float64(S["ppoco"]),
+        well_index=np.float64(S["well_index"]), well_z0=np.int32(S["well_z0"]), rho=np.float64(1000.0),
+        cells=np.array(meta["cells"], np.int32), start=np.array(meta["start"], np.int32),
+        axis=np.array(meta["axis"], np.int32), Kc=np.array(meta["Kc"]), bc=np.array(meta["bc"], np.int32),
+        cont=np.array(meta["cont"]), ccont=np.array(meta["ccont"]), karst_index=np.array(meta["karst_index"]),
+        area=np.array(meta["area"]), nbrec=np.array(meta["nbrec"], np.int32),
+        brec=np.array(meta["brec"], np.int32).reshape(max(mb, 0), nk),
+        beta=np.array(meta["beta"]).reshape(max(mb, 0), nk), C=np.array(meta["C"]).reshape(max(mb, 0), nk),
+        pc0=rd("pc0", mmax), pc1=rd("pc1", mmax), pc=rd("pc", mmax), ucl=rd("ucl", mmax), ucr=rd("ucr", mmax),
+        kc=rd("kc", mmax), ucR_old=rd("ucR_old", mb), ucL_old=rd("ucL_old", mb),
+        sample_idx_count=np.int64(idx.size), p_s=p[idx], p1_s=p1[idx],
+        p_sha256=np.array(field_digest(p)), p1_sha256=np.array(field_digest(p1)), v_sha256=np.array(field_digest(v)),
+        p_norm2=np.float64(np.linalg.norm(p)), p_sum=np.float64(p.sum()),
+        passes=np.int32(meta["passes"]), error=np.float64(meta["error"]),
+        pass_change=np.array(meta["pass_change"]), iter_amg=np.array(meta["iter_amg"], np.int32),
+    )
+    os.makedirs(os.path.join(HERE, "karst"), exist_ok=True)
+    np.savez_compressed(os.path.join(HERE, "karst", c["name"] + ".npz"), **out)
+    print(c["name"], "n=%d conduits=%d passes=%d error=%.3e iter_amg=%s" % (n, nk, meta["passes"], meta["error"],
+                                                                            meta["iter_amg"][:6]), flush=True)
+
+
 def run_summary(c, tmp):
     """Full-size case: samples and whole-field figures of p8, p8w and p12."""
     n = int(np.prod(c["dims"]))
@@ -192,6 +303,13 @@ def run_summary(c, tmp):
 def main():
     if not os.path.exists(REF):
         sys.exit("missing %s: run `make -C oracle ref` first" % REF)
+    if len(sys.argv) > 1 and sys.argv[1] == "karst":
+        names = sys.argv[2:]
+        with tempfile.TemporaryDirectory() as tmp:
+            for c in KARST_CASES:
+                if not names or c["name"] in names:
+                    run_karst(c, tmp)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "summary":
         names = sys.argv[2:]
         with tempfile.TemporaryDirectory() as tmp:
