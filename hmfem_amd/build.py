@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
 HIP_SOURCES = ["hmfem_assemble.hip", "hmfem_solver.hip", "hmfem_batch.hip", "hmfem_block.hip", "hmfem_defl.hip",
-               "hmfem_conduit.hip", "hmfem_capi.hip", "hmfem_output.cpp"]
+               "hmfem_conduit.hip", "hmfem_csr.hip", "hmfem_capi.hip", "hmfem_output.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed FP32 arithmetic instructions: some of their forms have given
@@ -63,9 +63,10 @@ def build_gpu(verbose=False):
 def build_driver(verbose=False):
     """hmfem_run: the C host of the reservoir step (links both libraries)."""
     out = os.path.join(HERE, "hmfem_run")
-    _run(["gcc", "-O2", "-Wall", "-o", out, os.path.join(HERE, "driver", "hmfem_run.c"),
-          "-I" + os.path.join(ROOT, "include"), "-L" + HERE, "-lhmfem_gpu", "-lhmfem_fields", "-lm",
-          "-Wl,-rpath,$ORIGIN"], verbose)
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    _run(["gcc", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-o", out, os.path.join(HERE, "driver", "hmfem_run.c"),
+          "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(rocm, "include"), "-L" + HERE, "-lhmfem_gpu",
+          "-lhmfem_fields", "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-lm", "-Wl,-rpath,$ORIGIN"], verbose)
     return out
 
 

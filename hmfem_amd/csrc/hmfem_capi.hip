@@ -658,6 +658,13 @@ extern "C" int hmfem_coupled_device(hmfem_context *c, const hmfem_grid *g, const
     return out->status;
 }
 
+extern "C" int hmfem_hierarchy_level(hmfem_context *c, int level, int *dims, int *fac, double *e, double *wx,
+                                     double *wy, double *wz)
+{
+    if (!c) return HMFEM_ERR_ARG;
+    return hierarchy_level_copy(c->st, c->setup, level, dims, fac, e, wx, wy, wz);
+}
+
 extern "C" int hmfem_solve_structured_device(hmfem_context *c, int nx, int ny, int nz, const ELE *perm,
                                              const IND *indcol, const double *b, double *x,
                                              const hmfem_options *opt, hmfem_status *st)
@@ -992,14 +999,68 @@ static int structured_from_csr(int n, const int *cnt, const int *col, const doub
                                std::vector<double> &e, std::vector<double> &wx, std::vector<double> &wy,
                                std::vector<double> &wz);
 
+// Input check of the general path: counts and columns in range, symmetric
+// values (relative to the rows' diagonals), positive diagonal. 0 if usable;
+// 1 malformed, 2 not symmetric, 3 a diagonal entry not positive.
+static int csr_check(int n, int nnz, const int *cnt, const int *col, const double *ele)
+{
+    if (n < 1 || nnz < 0) return 1;
+    std::vector<long> rp(n + 1, 0);
+    for (int i = 0; i < n; i++) {
+        if (cnt[i] < 0) return 1;
+        rp[i + 1] = rp[i] + cnt[i];
+    }
+    if (rp[n] != nnz) return 1;
+    std::vector<double> dia(n, 0.0);
+    std::unordered_map<long long, double> a;
+    a.reserve((size_t)nnz * 2);
+    for (int i = 0; i < n; i++)
+        for (long q = rp[i]; q < rp[i + 1]; q++) {
+            const int j = col[q];
+            if (j < 0 || j >= n) return 1;
+            if (j == i) dia[i] += ele[q];
+            a[(long long)i * n + j] += ele[q];
+        }
+    for (int i = 0; i < n; i++)
+        if (!(dia[i] > 0)) return 3;
+    for (const auto &kv : a) {
+        const long long i = kv.first / n, j = kv.first % n;
+        if (i == j) continue;
+        auto t = a.find(j * (long long)n + i);
+        const double other = t == a.end() ? 0.0 : t->second;
+        if (fabs(kv.second - other) > 1e-12 * std::max(fabs(dia[i]), fabs(dia[j]))) return 2;
+    }
+    return 0;
+}
+
+static char *ctx_alloc_csr(void *c, size_t bytes, int slot)
+{
+    char name[32];
+    snprintf(name, sizeof(name), "csr_%d", slot);
+    return (char *)((hmfem_context *)c)->get(name, bytes);
+}
+
 extern "C" int compute(int *hdr, int *cnt, int *col, double *ele, double *b, double *x)
 {
     int n = hdr[0];
     Dims d;
     std::vector<double> e, wx, wy, wz;
     if (structured_from_csr(n, cnt, col, ele, d, e, wx, wy, wz) != 0) {
-        fprintf(stderr, "hmfem_gpu: compute(): the matrix is not a symmetric 7-point Cartesian stencil\n");
-        fatal("compute", HMFEM_ERR_NOT_STRUCTURED);
+        // any other sparsity pattern: the general CSR path (aggregation AMG-PCG)
+        int rc = csr_check(n, hdr[1], cnt, col, ele);
+        if (rc) {
+            fprintf(stderr, "hmfem_gpu: compute(): %s\n",
+                    rc == 1 ? "bad count-CSR (counts, sizes or columns out of range)"
+                            : rc == 2 ? "the matrix is not symmetric" : "a diagonal entry is not positive");
+            fatal("compute", HMFEM_ERR_ARG);
+        }
+        hmfem_context *c = default_ctx(true);
+        int its = 0;
+        double rr = 0;
+        rc = csr_solve(c->st, n, cnt, col, ele, b, x, env_rtol(), env_stop_rule(), 10000, ctx_alloc_csr, c, &its, &rr);
+        if (rc < 0) fatal("compute", rc);
+        iter_amg = its + 1;
+        return 0;
     }
     hmfem_context *c = default_ctx(d.ny == 1 && d.nz == 1);
     Stencil<double> A;

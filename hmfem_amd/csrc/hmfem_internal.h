@@ -243,7 +243,17 @@ int deflation_sums_block(hipStream_t st, const Deflation &D, int nr, const doubl
                          double *Pextra, const int *done);
 int deflation_scatter_block(hipStream_t st, const Deflation &D, int nr, float *z, const int *done);
 
+// compute() on a general symmetric count-CSR matrix (hmfem_csr.hip): host
+// arrays in, x (host) in/out; device buffers from alloc(actx, bytes, slot).
+int csr_solve(hipStream_t st, int n, const int *cnt, const int *col, const double *ele, const double *b, double *x,
+              double rtol, int stop_rule, int maxit, char *(*alloc)(void *, size_t, int), void *actx, int *iterations,
+              double *rel_residual);
+
 size_t hierarchy_bytes(Dims fine, const hmfem_options &opt);
+// Level l of the hierarchy a cache holds (sizes, factors, FP64 coefficients to
+// host memory); l < 0: only the number of levels. Returns the level count.
+int hierarchy_level_copy(hipStream_t st, const SetupCache &cache, int l, int *dims, int *fac, double *e, double *wx,
+                         double *wy, double *wz);
 size_t cg_partial_count(long n);
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,

@@ -23,6 +23,7 @@
 
 extern "C" {
 const char *dirname = nullptr;
+int number_karst = 0;
 }
 
 namespace {
@@ -202,4 +203,59 @@ extern "C" int vts_StructuredGrid_file(ELE *v, double *x, Dim N, grid h)
     fwrite(tail.data(), 1, tail.size(), f);
     fclose(f);
     return 0;
+}
+
+namespace {
+
+// One conduit's lines (posprocessing_output.c:453-528, :571-655): the cell
+// centre of step i along the conduit's direction and a value.
+template <typename V>
+void conduit_lines(std::string &o, const DimK &M, grid h, V value)
+{
+    int dx = 0, dy = 0, dz = 0;
+    switch (M.pos) {
+    case 1: dx = 1; break;
+    case 2: dy = 1; break;
+    case 3: dz = 1; break;
+    case 4: dx = 1; dy = 1; break;
+    case 5: dx = 1; dy = -1; break;
+    case 6: dx = -1; dy = 1; break;
+    case 7: dx = -1; dy = -1; break;
+    default: return;   // the reference writes no line for other directions
+    }
+    for (int i = 0; i < M.s; i++)
+        add(o, "%.6lf %.6lf %.6lf %1.9e\n", h.x * ((M.sxi + dx * i) + 0.5), h.y * ((M.syi + dy * i) + 0.5),
+            h.z * ((M.szi + dz * i) + 0.5), value(i));
+}
+
+template <typename V>
+void save_conduits(const char *stem, DimK *M, grid h, V value)
+{
+    for (int j = 0; j < number_karst; j++) {
+        char name[64], path[2048];
+        snprintf(name, sizeof(name), "%s_%i.dat", stem, j);
+        FILE *f = open_out(name, path, sizeof(path));
+        if (!f) {
+            printf("Unable to open file %s. Terminating run... \n", path);
+            exit(0);
+        }
+        std::string o;
+        conduit_lines(o, M[j], h, [&](int i) { return value(i, j); });
+        fwrite(o.data(), 1, o.size(), f);
+        fclose(f);
+    }
+}
+
+} // namespace
+
+extern "C" void save_pressure_1d(double **x, DimK *M, double *hs, grid h)
+{
+    (void)hs;
+    save_conduits("pressure_karst", M, h, [&](int i, int j) { return x[i][j]; });
+}
+
+extern "C" void save_flux_1d(ELEK *uc, DimK *M, double *hs, grid h)
+{
+    (void)hs;
+    save_conduits("flux_karst", M, h, [&](int i, int j) { return 0.5 * (uc->r[i][j] - uc->l[i][j]); });
 }

@@ -1766,6 +1766,26 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     return HMFEM_OK;
 }
 
+int hierarchy_level_copy(hipStream_t st, const SetupCache &cache, int l, int *dims, int *fac, double *e, double *wx,
+                         double *wy, double *wz)
+{
+    if (!cache.valid || cache.hier.size() != sizeof(Hierarchy)) return HMFEM_ERR_ARG;
+    Hierarchy H;
+    memcpy((void *)&H, cache.hier.data(), sizeof(Hierarchy));
+    if (l < 0) return H.nlev;
+    if (l >= H.nlev) return HMFEM_ERR_ARG;
+    const Level &L = H.lev[l];
+    if (dims) { dims[0] = L.d.nx; dims[1] = L.d.ny; dims[2] = L.d.nz; }
+    if (fac) { fac[0] = L.fx; fac[1] = L.fy; fac[2] = L.fz; }
+    const size_t nb = (size_t)L.d.n() * sizeof(double);
+    if (e) HMFEM_CHECK(hipMemcpyAsync(e, L.s64.e, nb, hipMemcpyDeviceToHost, st));
+    if (wx) HMFEM_CHECK(hipMemcpyAsync(wx, L.s64.wx, nb, hipMemcpyDeviceToHost, st));
+    if (wy) HMFEM_CHECK(hipMemcpyAsync(wy, L.s64.wy, nb, hipMemcpyDeviceToHost, st));
+    if (wz) HMFEM_CHECK(hipMemcpyAsync(wz, L.s64.wz, nb, hipMemcpyDeviceToHost, st));
+    HMFEM_CHECK(hipStreamSynchronize(st));
+    return H.nlev;
+}
+
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink, std::vector<char> &stage,

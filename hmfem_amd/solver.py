@@ -176,6 +176,7 @@ def lib():
         L.hmfem_coupled_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(ConduitSetC),
                                            c_int, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(Options),
                                            P(CoupledStatus)]
+        L.hmfem_hierarchy_level.argtypes = [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
         L.hmfem_flux_device_noalloc.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(c_dbl),
                                                 P(c_dbl)]
         L.hmfem_solve_structured_device.argtypes = [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, P(Options),
@@ -196,6 +197,10 @@ def lib():
         L.save_flux.argtypes = [c_vp, Dim, GridH]
         L.save_flux.restype = None
         L.vts_StructuredGrid_file.argtypes = [c_vp, c_vp, Dim, GridH]
+        L.save_pressure_1d.argtypes = [c_vp, c_vp, c_vp, GridH]
+        L.save_pressure_1d.restype = None
+        L.save_flux_1d.argtypes = [c_vp, c_vp, c_vp, GridH]
+        L.save_flux_1d.restype = None
         L.hmfem_set_output_dir.argtypes = [ctypes.c_char_p]
         L.hmfem_set_output_dir.restype = None
         _typed = True
@@ -205,11 +210,12 @@ def lib():
 EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", "fluxos", "divvel_pontual",
             "prod_mat_vec", "hmfem_default_options", "hmfem_create", "hmfem_destroy", "hmfem_stream",
             "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device", "hmfem_flux_device_noalloc",
-            "hmfem_coupled_device",
+            "hmfem_coupled_device", "hmfem_hierarchy_level",
             "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
             "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device",
             "hmfem_reservoir_block_device", "hmfem_autotune_device", "save_pressure", "save_flux",
-            "vts_StructuredGrid_file", "hmfem_set_output_dir", "iter_amg", "boundary", "dirname")
+            "vts_StructuredGrid_file", "hmfem_set_output_dir", "save_pressure_1d", "save_flux_1d", "iter_amg",
+            "boundary", "dirname", "number_karst")
 
 
 def _p(a):
@@ -312,6 +318,47 @@ def save_pressure(x, N, h):
 def save_flux(v, N, h):
     """flux_reservoir.dat (posprocessing_output.c:236); v: ELE array (host)."""
     lib().save_flux(_p(v), N, GridH(*h))
+
+
+class DimK(ctypes.Structure):     # structs.h:107-115
+    _fields_ = [(f, c_int) for f in ("s", "sxi", "syi", "szi", "pos")]
+
+
+class ELEK(ctypes.Structure):     # structs.h:19-25
+    _fields_ = [("l", c_vp), ("r", c_vp), ("c", c_vp)]
+
+
+def _rows(a):
+    """double ** over the rows of a C-contiguous float64 matrix (kept alive by the caller)."""
+    return (c_vp * a.shape[0])(*[a.ctypes.data + i * a.strides[0] for i in range(a.shape[0])])
+
+
+def _conduit_dims(cells, start, axis):
+    M = (DimK * len(cells))()
+    for j, m in enumerate(cells):
+        M[j] = DimK(int(m), int(start[3 * j]), int(start[3 * j + 1]), int(start[3 * j + 2]), int(axis[j]))
+    c_int.in_dll(lib(), "number_karst").value = len(cells)
+    return M
+
+
+def save_pressure_1d(pc, cells, start, axis, h):
+    """pressure_karst_<j>.dat (posprocessing_output.c:437); pc: [cell][conduit]."""
+    x = np.ascontiguousarray(pc, np.float64)
+    rows = _rows(x)
+    M = _conduit_dims(cells, start, axis)
+    hs = np.zeros(len(cells))
+    lib().save_pressure_1d(rows, M, hs.ctypes.data, GridH(*h))
+
+
+def save_flux_1d(ucl, ucr, cells, start, axis, h):
+    """flux_karst_<j>.dat (posprocessing_output.c:536); ucl, ucr: [cell][conduit]."""
+    l = np.ascontiguousarray(ucl, np.float64)
+    r = np.ascontiguousarray(ucr, np.float64)
+    rl, rr = _rows(l), _rows(r)
+    uc = ELEK(ctypes.cast(rl, c_vp), ctypes.cast(rr, c_vp), None)
+    M = _conduit_dims(cells, start, axis)
+    hs = np.zeros(len(cells))
+    lib().save_flux_1d(ctypes.byref(uc), M, hs.ctypes.data, GridH(*h))
 
 
 def vts_StructuredGrid_file(v, x, N, h):
@@ -422,6 +469,25 @@ class Context:
         if rc < 0:
             raise RuntimeError("hmfem_coupled_device failed: %d" % rc)
         return st, changes[:st.passes]
+
+    def hierarchy(self):
+        """Levels of the last single AMG solve: [(dims, factors, e, wx, wy, wz)] (host, FP64)."""
+        nlev = lib().hmfem_hierarchy_level(self.h, -1, None, None, None, None, None, None)
+        if nlev < 0:
+            raise RuntimeError("hmfem_hierarchy_level failed: %d" % nlev)
+        out = []
+        for l in range(nlev):
+            d = np.zeros(3, np.int32)
+            f = np.zeros(3, np.int32)
+            lib().hmfem_hierarchy_level(self.h, l, d.ctypes.data, None, None, None, None, None)
+            n = int(np.prod(d))
+            e, wx, wy, wz = (np.zeros(n) for _ in range(4))
+            rc = lib().hmfem_hierarchy_level(self.h, l, d.ctypes.data, f.ctypes.data, e.ctypes.data, wx.ctypes.data,
+                                             wy.ctypes.data, wz.ctypes.data)
+            if rc < 0:
+                raise RuntimeError("hmfem_hierarchy_level failed: %d" % rc)
+            out.append((tuple(int(v) for v in d), tuple(int(v) for v in f), e, wx, wy, wz))
+        return out
 
     def flux_noalloc(self, g, Kx, Ky, Kz, bval, p, v=None):
         """hmfem_flux_device_noalloc: the same results, no device allocation."""

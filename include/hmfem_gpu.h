@@ -41,6 +41,12 @@ typedef struct Permeab { double *X, *Y, *Z; double **S; int permeability; } Perm
 typedef struct Grid { double x, y, z; } grid;
 /* structs.h:93-104 */
 typedef struct Dimension { int x, y, z, xy, t, px, pz, py; } Dim;
+/* structs.h:19-25: per-conduit matrices [cell][conduit] */
+typedef struct ELEstruct_K { double **l, **r, **c; } ELEK;
+/* structs.h:61-67 */
+typedef struct FacesKarst { int *l, *r; } FACEK;
+/* structs.h:107-115 */
+typedef struct Dimension_karst { int s, sxi, syi, szi, pos; } DimK;
 #endif
 
 /* ---------------------------------------------------------------- layer 1 */
@@ -107,6 +113,17 @@ void save_flux(ELE *v, Dim N, grid h);
    cell centres). Returns -1 if it cannot open the file (the reference
    carries on writing to a null stream). */
 int vts_StructuredGrid_file(ELE *v, double *x, Dim N, grid h);
+
+/* posprocessing_output.c:437 save_pressure_1d(); functions.h:73. Writes
+   pressure_karst_<j>.dat for each of the number_karst conduits (x[i][j]:
+   cell i of conduit j), the cell centres along the conduit's direction
+   M[j].pos (1 x, 2 y, 3 z; 4-7 the diagonals the reference's writer
+   knows). */
+extern int number_karst;    /* variables.h; the host's copy is read by ELF interposition */
+void save_pressure_1d(double **x, DimK *M, double *hs, grid h);
+/* posprocessing_output.c:536 save_flux_1d(); functions.h:72. Writes
+   flux_karst_<j>.dat: cell-centre velocity 0.5 (uc->r - uc->l). */
+void save_flux_1d(ELEK *uc, DimK *M, double *hs, grid h);
 
 /* ---------------------------------------------------------------- layer 2 */
 
@@ -232,6 +249,19 @@ int hmfem_flux_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx,
 int hmfem_flux_device_noalloc(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
                               const double *Kz, const double *const bval[6], const double *p, ELE *v,
                               double *div_min, double *div_max);
+
+/* No counterpart in the reference (its hierarchy is internal to amg_solver,
+   amg.h:1135-1239): the hierarchy of the context's last single AMG solve,
+   for inspection and tests. level < 0: returns the number of levels (the
+   last is solved densely). level >= 0: dims[3] receives its grid, fac[3] the
+   box factors towards the next level, and e, wx, wy, wz (host arrays of
+   nx*ny*nz, each may be NULL) its FP64 flux-form coefficients
+   ((A u)_s = e_s u_s + sum_f w_f (u_s - u_nb), w the coupling to the +x/+y/+z
+   neighbour); returns the number of levels. HMFEM_ERR_ARG if the context
+   holds no hierarchy (none built yet, or a later plain-PCG, block or batch
+   solve replaced it). */
+int hmfem_hierarchy_level(hmfem_context *ctx, int level, int *dims, int *fac, double *e, double *wx, double *wy,
+                          double *wz);
 
 /* Solve on an already assembled structured operator given in AoS form (device
    perm/indcol as from hmfem_assemble_device), b and x device FP64. */

@@ -25,7 +25,7 @@
  * step: the JSON then holds the setup time and the per-iteration time).
  * With "summary" it writes p8, p8w, p12 and the JSON (no matrix, fluxes or
  * level sizes): the large cases.
- * With "karst <data file> <max passes>" it runs the coupled reservoir/conduit
+ * With "karst <data file> <max passes> [<dir>]" it runs the coupled reservoir/conduit
  * model as main.c:390-530 does: reservoir() without well or conduits, the
  * conduit set read from the data file (the reference's data_<nk>_<type>.dat
  * format, parsed as main.c:197-300 parses it), conduit(), then the
@@ -36,7 +36,9 @@
  * pressure), .p1 (after the loop's first pass), .pc0 (conduit pressure
  * before the loop), .pc1 (after the first pass), .pc (final), .ucl/.ucr
  * (fluxo1d), .ucR_old/.ucL_old (breccia velocities), .v (fluxos), and
- * <prefix>.karst.json (passes, per-pass change, iteration counts).
+ * <prefix>.karst.json (passes, per-pass change, iteration counts); with
+ * <dir>, save_pressure_1d() and save_flux_1d() (posprocessing_output.c:
+ * 437-659) write pressure_karst_<j>.dat and flux_karst_<j>.dat there.
  * With "write <dir>" it also runs the reference's output writers on the tight
  * pressure and its fluxes: save_pressure(), save_flux()
  * (posprocessing_output.c:200-296) and vts_StructuredGrid_file() (vtk.c:8-76)
@@ -147,7 +149,7 @@ static void fail(const char *what)
 
 /* main.c:197-300 and :390-530 for one problem (see the header comment). */
 static int run_karst(const char *datafile, int max_passes, const char *pre, Perm *K, Bound *Bb, const double *WIin,
-                     const double *pwin)
+                     const double *pwin, const char *writedir)
 {
     const int n = N.t;
     FILE *arq = fopen(datafile, "r");
@@ -267,6 +269,11 @@ static int run_karst(const char *datafile, int max_passes, const char *pre, Perm
     }
     fluxo1d(pc, &keff, &ucf, hs, Mk, cont_karst, &BK, K, pm, p_old);
     fluxos(p, pm, vv, Bb, h, N);
+    if (writedir) {   /* main.c:526-527 */
+        dirname = writedir;
+        save_pressure_1d(pc, Mk, hs, h);
+        save_flux_1d(&ucf, Mk, hs, h);
+    }
     put(pre, ".karst.p", p, sizeof(double) * n);
     put_matrix(pre, ".karst.pc", pc, Mmax, nk);
     put_matrix(pre, ".karst.ucl", ucf.l, Mmax, nk);
@@ -387,7 +394,7 @@ int main(int argc, char **argv)
 
     if (strcmp(mode, "karst") == 0) {
         if (argc < 6) { fprintf(stderr, "usage: %s input.bin out_prefix karst data_file max_passes\n", argv[0]); return 2; }
-        return run_karst(argv[4], atoi(argv[5]), argv[2], &Kp, &Bb, WIv, pwv);
+        return run_karst(argv[4], atoi(argv[5]), argv[2], &Kp, &Bb, WIv, pwv, argc > 6 ? argv[6] : nullptr);
     }
 
     if (strcmp(mode, "time") == 0 && argc > 4) {

@@ -200,7 +200,7 @@ KARST_CASES = [
     # eleven conduits with twenty breccias each
     dict(name="ref_210x42x20_11_4", dims=(210, 42, 20), lengths=(210.0, 210.0, 100.0), data="data_11_4.dat"),
     # every end type and axis
-    dict(name="mixed_48x40x30", dims=(48, 40, 30), lengths=(48.0, 100.0, 60.0), data=None),
+    dict(name="mixed_48x40x30", dims=(48, 40, 30), lengths=(48.0, 100.0, 60.0), data=None, writers=True),
 ]
 
 
@@ -230,7 +230,9 @@ def run_karst(c, tmp, max_passes=1000):
         data = os.path.join(tmp, c["name"] + ".dat")
         mixed_conduit_file(data)
     pre = os.path.join(tmp, c["name"])
-    subprocess.run([REF, inp, pre, "karst", data, str(max_passes)], check=True, stdout=subprocess.DEVNULL)
+    wdir = os.path.join(tmp, c["name"] + "_out")
+    os.makedirs(wdir, exist_ok=True)
+    subprocess.run([REF, inp, pre, "karst", data, str(max_passes), wdir], check=True, stdout=subprocess.DEVNULL)
     meta = json.load(open(pre + ".karst.json"))
     nk, mmax, mb = meta["number_karst"], meta["mmax"], meta["max_brec"]
     rd = lambda ext, rows: np.fromfile(pre + ".karst." + ext, "<f8").reshape(rows, nk)
@@ -259,6 +261,16 @@ def run_karst(c, tmp, max_passes=1000):
     )
     os.makedirs(os.path.join(HERE, "karst"), exist_ok=True)
     np.savez_compressed(os.path.join(HERE, "karst", c["name"] + ".npz"), **out)
+    if c.get("writers"):
+        # the reference's 1-D writers on the final conduit pressure and fluxes
+        files = {}
+        for j in range(nk):
+            for stem in ("pressure_karst", "flux_karst"):
+                files["%s_%d" % (stem, j)] = np.frombuffer(open(os.path.join(wdir, "%s_%d.dat" % (stem, j)), "rb").read(),
+                                                           np.uint8)
+        np.savez_compressed(os.path.join(HERE, "writers", "conduits_" + c["name"] + ".npz"),
+                            dims=out["dims"], lengths=out["lengths"], cells=out["cells"], start=out["start"],
+                            axis=out["axis"], pc=out["pc"], ucl=out["ucl"], ucr=out["ucr"], **files)
     print(c["name"], "n=%d conduits=%d passes=%d error=%.3e iter_amg=%s" % (n, nk, meta["passes"], meta["error"],
                                                                             meta["iter_amg"][:6]), flush=True)
 

@@ -69,3 +69,64 @@ def test_driver_passes(reuse, tmp_path):
     assert rel_l2(x, xo) < 1e-8
     for f in ("flux_reservoir.dat", "campos.vts"):
         assert (out / f).stat().st_size > 0
+
+
+def write_conduit_file(z, path):
+    """The fixture's conduit set in the reference's data-file format
+    (main.c:197-283 reads it; beta holds R^4 - 1, the file R^4)."""
+    nk = len(z["cells"])
+    st = np.asarray(z["start"]).reshape(nk, 3)
+    bc = np.asarray(z["bc"]).reshape(nk, 2)
+    row = lambda vals: " ".join(repr(float(v)) if isinstance(v, (float, np.floating)) else str(int(v)) for v in vals)
+    lines = [row(z["cells"]), row(st[:, 0]), row(st[:, 1]), row(st[:, 2]), row(z["axis"]),
+             row(np.asarray(z["Kc"], float)), row(bc[:, 0]), row(bc[:, 1])]
+    cont = np.asarray(z["cont"], float).reshape(2, nk)
+    ccont = np.asarray(z["ccont"], float).reshape(2, nk)
+    lines += [row(cont[0]), row(cont[1]), row(ccont[0]), row(ccont[1]), row(np.asarray(z["karst_index"], float)),
+              row(np.asarray(z["area"], float)), row(z["nbrec"])]
+    beta = np.asarray(z["beta"], float).reshape(-1, nk)
+    lines += [row(b + 1.0) for b in beta]
+    lines += [row(c) for c in np.asarray(z["C"], float).reshape(-1, nk)]
+    lines += [row(b) for b in np.asarray(z["brec"]).reshape(-1, nk)]
+    path.write_text("\n".join(lines) + "\n")
+
+
+def read_columns(path, col):
+    return np.array([float(l.split()[col]) for l in open(path) if len(l.split()) > col])
+
+
+@pytest.mark.parametrize("case", ["ref_210x42x20_1_2", "mixed_48x40x30"])
+def test_driver_coupled_conduits(case, tmp_path):
+    """hmfem_run with a conduit data file runs main.c's coupled loop on the
+    device and writes the reference's files; the conduit pressures it prints
+    (pressure_karst_<j>.dat) agree with the reference's coupled run, and the
+    pass count with the reference's."""
+    if not os.path.exists(EXE):
+        pytest.fail("hmfem_run is not built")
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "karst", case + ".npz"))
+    nx, ny, nz = (int(v) for v in z["dims"])
+    data = tmp_path / "conduits.dat"
+    write_conduit_file(z, data)
+    L = [float(v) for v in z["lengths"]]
+    cfg = tmp_path / "cfg.txt"
+    cfg.write_text("grid %d %d %d\nlengths %r %r %r\nviscosity %r\nbc_type %s\nbc_value %s\nfield homogeneous %r\n"
+                   "well_column %d %d %d %r %r\nkarst_data %d %s\n"
+                   % (nx, ny, nz, L[0], L[1], L[2], float(z["visc"]), " ".join(str(int(v)) for v in z["bc_type"]),
+                      " ".join(repr(float(v)) for v in z["bc_val"]), float(z["K"]), nx // 2, ny // 2,
+                      int(z["well_z0"]), float(z["well_index"]), float(z["ppoco"]), len(z["cells"]), data))
+    out = tmp_path / "out"
+    out.mkdir()
+    print("[driver] coupled", case, flush=True)
+    r = subprocess.run([EXE, str(cfg), str(out)], capture_output=True, text=True, timeout=240)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stderr
+    passes = int([l for l in r.stdout.splitlines() if l.startswith("iterations number=")][0].split("=")[1])
+    assert abs(passes - int(z["passes"])) <= max(3, int(z["passes"]) // 10)
+    for j, m in enumerate(z["cells"]):
+        pc = read_columns(out / ("pressure_karst_%d.dat" % j), 3)
+        ref = z["pc"][:m, j]
+        assert pc.size == m
+        assert np.abs(pc - ref).max() <= 1e-7 * np.abs(ref).max(), j
+        assert (out / ("flux_karst_%d.dat" % j)).stat().st_size > 0
+    p = read_pressure(out / "pressure_reservoir.dat", nx * ny * nz)
+    assert abs(np.linalg.norm(p) - float(z["p_norm2"])) <= 1e-7 * float(z["p_norm2"])

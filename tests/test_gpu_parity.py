@@ -250,6 +250,79 @@ def test_warm_start_and_repeat(ctx, torch):
     assert rel_l2(p.cpu().numpy(), p1.cpu().numpy()) < 1e-9
 
 
+def _to_count_csr(A):
+    """scipy CSR -> the reference's count-CSR (hdr, cnt, col, ele)."""
+    A = A.tocsr()
+    A.sort_indices()
+    cnt = np.diff(A.indptr).astype(np.int32)
+    return (np.array([A.shape[0], A.nnz], np.int32), cnt, A.indices.astype(np.int32),
+            A.data.astype(np.float64))
+
+
+def _general_matrices():
+    import scipy.sparse as sp
+    rng = np.random.default_rng(42)
+    out = []
+    # 2-D nine-point operator with anisotropy and a Dirichlet-like diagonal shift
+    nx, ny = 40, 30
+    I = sp.identity
+    T = lambda m, a: sp.diags([-a * np.ones(m - 1), 2 * a * np.ones(m), -a * np.ones(m - 1)], [-1, 0, 1])
+    A = sp.kron(I(ny), T(nx, 1.0)) + sp.kron(T(ny, 0.05), I(nx))
+    A = A + 0.25 * sp.kron(T(ny, 1.0), T(nx, 1.0)) * 0.1 + sp.diags(np.r_[1.0, np.zeros(nx * ny - 2), 1.0])
+    out.append(("ninepoint_40x30", A.tocsr()))
+    # a random symmetric graph Laplacian with a few grounded nodes (unstructured)
+    n = 3000
+    i = rng.integers(0, n, 12000)
+    j = rng.integers(0, n, 12000)
+    w = rng.random(12000) + 0.1
+    keep = i != j
+    W = sp.coo_matrix((w[keep], (i[keep], j[keep])), shape=(n, n))
+    W = (W + W.T).tocsr()
+    L = sp.diags(np.asarray(W.sum(axis=1)).ravel()) - W
+    L = L + sp.diags(np.where(rng.random(n) < 0.01, 1.0, 0.0) + 1e-6)
+    out.append(("graph_3000", L.tocsr()))
+    # the 24^3 lognormal fixture's matrix with its rows and columns permuted
+    z = np.load(os.path.join(GOLD, "lognormal_24.npz"))
+    P = O.problem_from_fixture(z)
+    perm, indcol, b = O.assemble(P)
+    cnt, col, ele = O.build_csr(perm, indcol)
+    rp = np.r_[0, np.cumsum(cnt)]
+    A = sp.csr_matrix((ele, col, rp), shape=(P.n, P.n))
+    q = rng.permutation(P.n)
+    out.append(("lognormal_24_permuted", A[q][:, q].tocsr()))
+    return out
+
+
+@pytest.mark.parametrize("case", [0, 1, 2], ids=["ninepoint_40x30", "graph_3000", "lognormal_24_permuted"])
+def test_compute_general_csr(case, monkeypatch):
+    """compute() on matrices that are not a 7-point Cartesian stencil goes
+    through the general CSR path (aggregation AMG-PCG on the GPU), as the
+    reference's compute() takes any symmetric count-CSR (compute.cpp:7-50):
+    at 1e-12 the solution is a direct solve's; at the reference's own
+    defaults the iteration count stays bounded."""
+    import scipy.sparse.linalg as spla
+    name, A = _general_matrices()[case]
+    n = A.shape[0]
+    rng = np.random.default_rng(case)
+    b = rng.random(n) - 0.3
+    xd = spla.spsolve(A.tocsc(), b)
+    hdr, cnt, col, ele = _to_count_csr(A)
+    _say("general CSR", name, n, A.nnz)
+    monkeypatch.setenv("HMFEM_RTOL", "1e-12")
+    x = np.zeros(n)
+    S.compute(hdr, cnt, col, ele, b.copy(), x)
+    it12 = S.iter_amg()
+    err = rel_l2(x, xd)
+    monkeypatch.delenv("HMFEM_RTOL")
+    x8 = np.zeros(n)
+    S.compute(hdr, cnt, col, ele, b.copy(), x8)
+    it8 = S.iter_amg()
+    _say(name, "iter_amg 1e-12: %d, 1e-8: %d; rel L2 vs direct %.3e (1e-8: %.3e)" % (it12, it8, err, rel_l2(x8, xd)))
+    assert err < 1e-9
+    assert 2 <= it8 <= 200
+    assert rel_l2(x8, xd) < 1e-5
+
+
 @pytest.mark.parametrize("smoother", [S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS], ids=["jacobi", "blockgs"])
 def test_full_size_c2_properties(smoother, ctx, torch):
     """128^3 lognormal sigma=2 (the benchmark workload) at rtol 1e-8:

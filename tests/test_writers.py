@@ -14,7 +14,9 @@ import pytest
 
 from hmfem_amd import solver as S
 
-W = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "writers", "*.npz")))
+ALL = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "writers", "*.npz")))
+W = [p for p in ALL if not os.path.basename(p).startswith("conduits_")]
+WK = [p for p in ALL if os.path.basename(p).startswith("conduits_")]
 
 
 @pytest.mark.parametrize("path", W, ids=[os.path.basename(p)[:-4] for p in W])
@@ -89,3 +91,21 @@ def test_writers_long_lines(tmp_path):
     first = text.splitlines()[0].split()
     assert len(first) == 4 and float(first[0]) == 0.5e300 and first[3] == "1.000000000e+00"
     assert first[0] == "%.6f" % 0.5e300
+
+
+@pytest.mark.parametrize("path", WK, ids=[os.path.basename(p)[:-4] for p in WK])
+def test_conduit_writers_bytes(path, tmp_path):
+    """save_pressure_1d() and save_flux_1d() (posprocessing_output.c:437-659)
+    against the files the reference's own writers wrote for the final
+    conduit pressures and fluxes of a coupled run (every axis)."""
+    z = np.load(path)
+    dims = [int(a) for a in z["dims"]]
+    h = tuple(float(L) / n for L, n in zip(z["lengths"], dims))
+    S.set_dirname(str(tmp_path))
+    S.save_pressure_1d(z["pc"], z["cells"], z["start"], z["axis"], h)
+    S.save_flux_1d(z["ucl"], z["ucr"], z["cells"], z["start"], z["axis"], h)
+    for j in range(len(z["cells"])):
+        for stem in ("pressure_karst", "flux_karst"):
+            got = (tmp_path / ("%s_%d.dat" % (stem, j))).read_bytes()
+            ref = z["%s_%d" % (stem, j)].tobytes()
+            assert got == ref, (stem, j)
