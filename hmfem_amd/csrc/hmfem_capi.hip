@@ -140,6 +140,7 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->deflate_ratio = 3000.0;
     o->precond = HMFEM_PRECOND_AMG;
     o->reuse_setup = 0;
+    o->semicoarsen = 1;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)
@@ -246,7 +247,14 @@ static int solve_on_stencil(hmfem_context *c, Dims d, const Stencil<double> &A, 
     else hmfem_default_options(&opt);
     if (opt.coarse_max > kCoarseMax || opt.coarse_max < 1) opt.coarse_max = kCoarseMax;
     long n = d.n();
-    size_t hb = hierarchy_bytes(d, opt);
+    PlanHint hint;
+    if (opt.semicoarsen && opt.precond == HMFEM_PRECOND_AMG) {
+        double *part = (double *)c->get("axis_partial", 6 * kAxisBlocks * 8);
+        if (!part) return HMFEM_ERR_NOMEM;
+        int rc = axis_strength(c->st, d, A, part, hint);
+        if (rc) return rc;
+    }
+    size_t hb = hierarchy_bytes(d, opt, &hint);
     char *arena = (char *)c->get("arena", hb);
     CgBuffers B;
     B.r = (double *)c->get("r", n * 8);
@@ -266,7 +274,7 @@ static int solve_on_stencil(hmfem_context *c, Dims d, const Stencil<double> &A, 
     hmfem_status tmp{};
     hmfem_status *out = st ? st : &tmp;
     return solve_structured(c->st, d, A, b, x, opt, arena, hb, B, c->h_sc, info, out, c->ev, sink_of(c), c->stage,
-                            &c->graph, c->lds_limit, &c->defl, ctx_alloc_defl, c, &c->setup);
+                            &c->graph, c->lds_limit, &c->defl, ctx_alloc_defl, c, &c->setup, &hint);
 }
 
 // ------------------------------------------------------------ device API
@@ -663,6 +671,12 @@ extern "C" int hmfem_hierarchy_level(hmfem_context *c, int level, int *dims, int
 {
     if (!c) return HMFEM_ERR_ARG;
     return hierarchy_level_copy(c->st, c->setup, level, dims, fac, e, wx, wy, wz);
+}
+
+extern "C" int hmfem_hierarchy_coarse_inverse(hmfem_context *c, double *inv)
+{
+    if (!c) return HMFEM_ERR_ARG;
+    return hierarchy_coarse_inverse(c->st, c->setup, inv);
 }
 
 extern "C" int hmfem_solve_structured_device(hmfem_context *c, int nx, int ny, int nz, const ELE *perm,

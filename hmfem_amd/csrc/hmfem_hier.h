@@ -112,7 +112,8 @@ inline OpArgs<double> opargs<double>(const Level &L)
 
 // Build the hierarchy from a fine FP64 stencil (device), inside `ar`.
 int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const hmfem_options &opt, Arena &ar,
-                    Hierarchy &H, int *d_info, std::vector<char> &stage, long lds_limit);
+                    Hierarchy &H, int *d_info, std::vector<char> &stage, long lds_limit,
+                    const PlanHint *hint = nullptr);
 
 // One tail cycle of an FP32 hierarchy in LDS mode on a right-hand side read
 // from ftop[s * stride] (top tail level), result to utop[s * stride].

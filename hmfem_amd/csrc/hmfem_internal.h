@@ -91,6 +91,17 @@ struct SetupCache {
     unsigned dgen = 0;        // Deflation::gen at the build
 };
 
+// Per-axis coupling strength of a fine operator (geometric means of its
+// nonzero x, y, z couplings) for the coarsening plan: an axis is coarsened
+// only where its coupling is at least kSemiTheta times the strongest axis's
+// (strength-based semi-coarsening, which follows anisotropy from the cell
+// sizes or from K). off: 2x2x2 boxes throughout.
+struct PlanHint {
+    bool on = false;
+    double m[3] = {0.0, 0.0, 0.0};
+};
+constexpr double kSemiTheta = 0.25;
+
 // Grid launch helpers with an XCD-aware block order: logical block b of a
 // launch with G blocks (G a multiple of 8) is placed so that each XCD gets a
 // contiguous range of the domain and keeps its neighbours in its own L2.
@@ -249,18 +260,24 @@ int csr_solve(hipStream_t st, int n, const int *cnt, const int *col, const doubl
               double rtol, int stop_rule, int maxit, char *(*alloc)(void *, size_t, int), void *actx, int *iterations,
               double *rel_residual);
 
-size_t hierarchy_bytes(Dims fine, const hmfem_options &opt);
+size_t hierarchy_bytes(Dims fine, const hmfem_options &opt, const PlanHint *hint = nullptr);
+// The coupling strength of a fine FP64 stencil (device), sampled every 16th
+// cell; partial needs kAxisBlocks * 6 doubles of device memory.
+constexpr int kAxisBlocks = 256;
+int axis_strength(hipStream_t st, Dims d, const Stencil<double> &A, double *partial, PlanHint &hint);
 // Level l of the hierarchy a cache holds (sizes, factors, FP64 coefficients to
 // host memory); l < 0: only the number of levels. Returns the level count.
 int hierarchy_level_copy(hipStream_t st, const SetupCache &cache, int l, int *dims, int *fac, double *e, double *wx,
                          double *wy, double *wz);
+// The dense inverse of that hierarchy's coarsest level (nc x nc, host); returns nc.
+int hierarchy_coarse_inverse(hipStream_t st, const SetupCache &cache, double *inv);
 size_t cg_partial_count(long n);
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink,
                      std::vector<char> &stage, GraphCache *graph, long lds_limit, Deflation *defl = nullptr,
                      char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr,
-                     SetupCache *cache = nullptr);
+                     SetupCache *cache = nullptr, const PlanHint *hint = nullptr);
 
 // batched solve (hmfem_batch.hip) and batch layout / kernels (hmfem_assemble.hip)
 int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A0, const double *b, double *x,

@@ -807,62 +807,120 @@ k_dinv(OpArgs<T> o, T *di)
     di[s] = (T)(1.0 / (double)diag_at<T, double>(o, c));
 }
 
-// Dense inverse of the coarsest operator by Gauss-Jordan elimination in LDS
-// (SPD, no pivoting needed), FP64, one block of 1024 threads.
+// Dense inverse of the coarsest operator (SPD, n <= kCoarseMax) by block
+// Gauss-Jordan elimination in LDS, FP64, one block of 1024 threads (16
+// waves). The matrix is padded with the identity to N = 16 ceil(n / 16);
+// for each 16-wide block column k: the diagonal block is inverted in place
+// (scalar Gauss-Jordan, P = A_kk^-1), then
+//   A_kj <- P A_kj, A_ij <- A_ij - A_ik A_kj, A_ik <- -A_ik P  (i, j != k)
+// as 16x16 tile products on the matrix cores (v_mfma_f64_16x16x4f64, four
+// K-steps of 4 per tile), one tile per wave at a time. At the end A holds A^-1.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+// C + s X Y for 16x16 tiles X at (xr, xc) and Y at (yr, yc) of the N x N
+// matrix M in LDS. Lane l supplies X[l % 16][k], Y[k][l % 16] with
+// k = 4 kk + l / 16, and holds C[4 q + l / 16][l % 16] in register q (the
+// instruction's lane layout, measured on the device).
+__device__ __forceinline__ mfma_d4 tile_mma(const double *M, int N, int xr, int xc, int yr, int yc, mfma_d4 c,
+                                            double s)
+{
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+        const int k = 4 * kk + (l >> 4);
+        const double a = s * M[(xr + (l & 15)) * N + xc + k];
+        const double b = M[(yr + k) * N + yc + (l & 15)];
+        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    return c;
+}
+
+__device__ __forceinline__ mfma_d4 tile_load(const double *M, int N, int r0, int c0)
+{
+    const int l = threadIdx.x & 63;
+    mfma_d4 c;
+#pragma unroll
+    for (int q = 0; q < 4; q++) c[q] = M[(r0 + 4 * q + (l >> 4)) * N + c0 + (l & 15)];
+    return c;
+}
+
+__device__ __forceinline__ void tile_store(double *M, int N, int r0, int c0, mfma_d4 c)
+{
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < 4; q++) M[(r0 + 4 * q + (l >> 4)) * N + c0 + (l & 15)] = c[q];
+}
+
 __global__ void __launch_bounds__(1024)
 k_dense_invert(OpArgs<double> o, double *inv, int *info)
 {
-    extern __shared__ double A[];            // n*n
-    __shared__ double colk[kCoarseMax], rowk[kCoarseMax];
+    extern __shared__ double A[];            // N*N
+    __shared__ double colk[16], rowk[16];
     const int n = o.nx * o.ny * o.nz;
+    const int N = (n + 15) / 16 * 16, T = N / 16;
     const int nx = o.nx, nxy = o.nx * o.ny;
-    for (int q = threadIdx.x; q < n * n; q += blockDim.x) A[q] = 0.0;
+    for (int q = threadIdx.x; q < N * N; q += blockDim.x) A[q] = (q / N == q % N && q / N >= n) ? 1.0 : 0.0;
     __syncthreads();
     for (int s = threadIdx.x; s < n; s += blockDim.x) {
         Cell c = cell_at(o, s);
         double d = diag_at<double, double>(o, c);
-        A[s * n + s] = d;
-        if (c.i < o.nx - 1) A[s * n + s + 1] = -o.wx[s];
-        if (c.i > 0) A[s * n + s - 1] = -o.wx[s - 1];
-        if (c.j < o.ny - 1) A[s * n + s + nx] = -o.wy[s];
-        if (c.j > 0) A[s * n + s - nx] = -o.wy[s - nx];
-        if (c.k < o.nz - 1) A[s * n + s + nxy] = -o.wz[s];
-        if (c.k > 0) A[s * n + s - nxy] = -o.wz[s - nxy];
+        A[s * N + s] = d;
+        if (c.i < o.nx - 1) A[s * N + s + 1] = -o.wx[s];
+        if (c.i > 0) A[s * N + s - 1] = -o.wx[s - 1];
+        if (c.j < o.ny - 1) A[s * N + s + nx] = -o.wy[s];
+        if (c.j > 0) A[s * N + s - nx] = -o.wy[s - nx];
+        if (c.k < o.nz - 1) A[s * N + s + nxy] = -o.wz[s];
+        if (c.k > 0) A[s * N + s - nxy] = -o.wz[s - nxy];
     }
     __syncthreads();
-    // Gauss-Jordan, two barriers per pivot: the pivot column and the scaled
-    // pivot row go to separate arrays first, then every entry is updated from
-    // them. Each thread owns up to kOwn fixed entries (no index division in
-    // the loop).
-    constexpr int kOwn = (kCoarseMax * kCoarseMax + 1023) / 1024;
-    int oi[kOwn], oj[kOwn];
-#pragma unroll
-    for (int m = 0; m < kOwn; m++) {
-        const int q = threadIdx.x + m * 1024;
-        oi[m] = q < n * n ? q / n : -1;
-        oj[m] = q < n * n ? q - oi[m] * n : 0;
-    }
-    for (int k = 0; k < n; k++) {
-        const double piv = A[k * n + k];
-        if (!(piv > 0.0)) { if (threadIdx.x == 0) *info = k + 1; return; }   // uniform: every thread read piv
-        const double rp = 1.0 / piv;
-        if (threadIdx.x < n) {
-            const int t = threadIdx.x;
-            colk[t] = A[t * n + k];
-            rowk[t] = (t == k ? 1.0 : A[k * n + t]) * rp;
+    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    for (int kb = 0; kb < T; kb++) {
+        const int k0 = 16 * kb;
+        // P = A_kk^-1 in place: scalar Gauss-Jordan, one entry per thread of the first 256
+        for (int p = 0; p < 16; p++) {
+            const double piv = A[(k0 + p) * N + k0 + p];
+            if (!(piv > 0.0)) { if (threadIdx.x == 0) *info = k0 + p + 1; return; }   // uniform
+            const double rp = 1.0 / piv;
+            if (threadIdx.x < 16) {
+                const int t = threadIdx.x;
+                colk[t] = A[(k0 + t) * N + k0 + p];
+                rowk[t] = (t == p ? 1.0 : A[(k0 + p) * N + k0 + t]) * rp;
+            }
+            __syncthreads();
+            if (threadIdx.x < 256) {
+                const int i = threadIdx.x >> 4, j = threadIdx.x & 15;
+                double &x = A[(k0 + i) * N + k0 + j];
+                x = i == p ? rowk[j] : ((j == p ? 0.0 : x) - colk[i] * rowk[j]);
+            }
+            __syncthreads();
+        }
+        // row panel: A_kj <- P A_kj
+        for (int jb = wave; jb < T; jb += nwaves) {
+            if (jb == kb) continue;
+            mfma_d4 c = {0.0, 0.0, 0.0, 0.0};
+            c = tile_mma(A, N, k0, k0, k0, 16 * jb, c, 1.0);
+            tile_store(A, N, k0, 16 * jb, c);
         }
         __syncthreads();
-#pragma unroll
-        for (int m = 0; m < kOwn; m++) {
-            const int i = oi[m], j = oj[m];
-            if (i < 0) continue;
-            const int q = i * n + j;
-            if (i == k) A[q] = rowk[j];
-            else A[q] = ((j == k) ? 0.0 : A[q]) - colk[i] * rowk[j];
+        // trailing tiles: A_ij <- A_ij - A_ik A_kj
+        for (int t = wave; t < T * T; t += nwaves) {
+            const int ib = t / T, jb = t - ib * T;
+            if (ib == kb || jb == kb) continue;
+            mfma_d4 c = tile_load(A, N, 16 * ib, 16 * jb);
+            c = tile_mma(A, N, 16 * ib, k0, k0, 16 * jb, c, -1.0);
+            tile_store(A, N, 16 * ib, 16 * jb, c);
+        }
+        __syncthreads();
+        // column panel: A_ik <- -A_ik P
+        for (int ib = wave; ib < T; ib += nwaves) {
+            if (ib == kb) continue;
+            mfma_d4 c = {0.0, 0.0, 0.0, 0.0};
+            c = tile_mma(A, N, 16 * ib, k0, k0, k0, c, -1.0);   // the wave's own tile: read, then written
+            tile_store(A, N, 16 * ib, k0, c);
         }
         __syncthreads();
     }
-    for (int q = threadIdx.x; q < n * n; q += blockDim.x) inv[q] = A[q];
+    for (int q = threadIdx.x; q < n * n; q += blockDim.x) inv[q] = A[(q / n) * N + q % n];
     if (threadIdx.x == 0) *info = 0;
 }
 
@@ -1228,25 +1286,72 @@ k_cg_finish_x(int n, const double *__restrict__ s0, const double *__restrict__ s
 // ---------------------------------------------------------------- host side
 
 
-static void plan_levels(Dims fine, int coarse_max, Level *lev, int &nlev)
+static void plan_levels(Dims fine, int coarse_max, Level *lev, int &nlev, const PlanHint *hint = nullptr)
 {
     nlev = 0;
     Dims d = fine;
+    // coupling strength per axis, carried down the levels: a coarse face
+    // between boxes sums the fine faces it covers, so a coarse x coupling is
+    // about fy * fz fine ones (and so on)
+    double m[3] = {1.0, 1.0, 1.0};
+    const bool semi = hint && hint->on;
+    if (semi)
+        for (int a = 0; a < 3; a++) m[a] = hint->m[a];
     while (true) {
         Level &L = lev[nlev];
         L = Level();
         L.d = d;
         nlev++;
         if (d.n() <= coarse_max || nlev == kMaxLevels) break;
-        // 2x2x2 boxes throughout: 4x4x4 boxes below 32^3 (one level fewer)
-        // took 28 CG iterations instead of 16 at 128^3 and 121 instead of 85
-        // on the 256^3 conduit field
-        L.fx = d.nx > 1 ? 2 : 1;
-        L.fy = d.ny > 1 ? 2 : 1;
-        L.fz = d.nz > 1 ? 2 : 1;
+        // 2x2x2 boxes where the couplings are about isotropic: 4x4x4 boxes
+        // below 32^3 (one level fewer) took 28 CG iterations instead of 16 at
+        // 128^3 and 121 instead of 85 on the 256^3 conduit field
+        const int ext[3] = {d.nx, d.ny, d.nz};
+        double top = 0.0;
+        for (int a = 0; a < 3; a++)
+            if (ext[a] > 1) top = fmax(top, m[a]);
+        int f[3];
+        for (int a = 0; a < 3; a++) f[a] = ext[a] > 1 && (!semi || !(top > 0) || m[a] >= kSemiTheta * top) ? 2 : 1;
+        L.fx = f[0]; L.fy = f[1]; L.fz = f[2];
+        const double m0 = m[0] * f[1] * f[2], m1 = m[1] * f[0] * f[2], m2 = m[2] * f[0] * f[1];
+        m[0] = m0; m[1] = m1; m[2] = m2;
         Dims c{(d.nx + L.fx - 1) / L.fx, (d.ny + L.fy - 1) / L.fy, (d.nz + L.fz - 1) / L.fz};
         d = c;
     }
+}
+
+// Per-block sums of ln w and counts of nonzero w per axis over every 16th
+// cell (6 partials per block).
+__global__ void __launch_bounds__(kBlock) k_axis_logmean(OpArgs<double> o, double *partial)
+{
+    const long n = (long)o.nx * o.ny * o.nz;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (long s = ((long)blockIdx.x * kBlock + threadIdx.x) * 16; s < n; s += (long)gridDim.x * kBlock * 16) {
+        const double w[3] = {o.wx[s], o.wy[s], o.wz[s]};
+        for (int a = 0; a < 3; a++)
+            if (w[a] > 0) { acc[2 * a] += log(w[a]); acc[2 * a + 1] += 1.0; }
+    }
+    block_sum<6>(acc);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; q++) partial[6 * blockIdx.x + q] = acc[q];
+}
+
+int axis_strength(hipStream_t st, Dims d, const Stencil<double> &A, double *partial, PlanHint &hint)
+{
+    OpArgs<double> o;
+    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
+    o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
+    hipLaunchKernelGGL(k_axis_logmean, dim3(kAxisBlocks), dim3(kBlock), 0, st, o, partial);
+    HMFEM_CHECK(hipGetLastError());
+    std::vector<double> h(6 * kAxisBlocks);
+    HMFEM_CHECK(hipMemcpyAsync(h.data(), partial, h.size() * 8, hipMemcpyDeviceToHost, st));
+    HMFEM_CHECK(hipStreamSynchronize(st));
+    double sum[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < kAxisBlocks; b++)
+        for (int q = 0; q < 6; q++) sum[q] += h[6 * b + q];
+    hint.on = true;
+    for (int a = 0; a < 3; a++) hint.m[a] = sum[2 * a + 1] > 0 ? exp(sum[2 * a] / sum[2 * a + 1]) : 0.0;
+    return HMFEM_OK;
 }
 
 static int tail_start(const Level *lev, int nlev, long tail_max)
@@ -1259,11 +1364,11 @@ static int tail_start(const Level *lev, int nlev, long tail_max)
 
 static int clamp_coarse(int c) { return c > kCoarseMax ? kCoarseMax : (c < 1 ? 1 : c); }
 
-size_t hierarchy_bytes(Dims fine, const hmfem_options &opt)
+size_t hierarchy_bytes(Dims fine, const hmfem_options &opt, const PlanHint *hint)
 {
     Level lev[kMaxLevels];
     int nlev;
-    plan_levels(fine, clamp_coarse(opt.coarse_max), lev, nlev);
+    plan_levels(fine, clamp_coarse(opt.coarse_max), lev, nlev, hint);
     size_t b = 0;
     size_t vs = opt.precision == HMFEM_PREC_FP64 ? 8 : 4;
     for (int l = 0; l < nlev; l++) {
@@ -1349,9 +1454,9 @@ static int upload_tail(hipStream_t st, Hierarchy &H, const hmfem_options &opt, s
 
 // Build the hierarchy from a fine FP64 stencil (device), inside `arena`.
 int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const hmfem_options &opt, Arena &ar,
-                    Hierarchy &H, int *d_info, std::vector<char> &stage, long lds_limit)
+                    Hierarchy &H, int *d_info, std::vector<char> &stage, long lds_limit, const PlanHint *hint)
 {
-    plan_levels(fine, clamp_coarse(opt.coarse_max), H.lev, H.nlev);
+    plan_levels(fine, clamp_coarse(opt.coarse_max), H.lev, H.nlev, hint);
     int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
     H.prec = prec;
     H.tail0 = tail_start(H.lev, H.nlev, opt.tail_max);
@@ -1415,7 +1520,8 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const 
     {
         Level &L = H.lev[H.nlev - 1];
         OpArgs<double> o = opargs<double>(L);
-        size_t lds = (size_t)H.nc * H.nc * sizeof(double);
+        const size_t N = (size_t)(H.nc + 15) / 16 * 16;   // padded to whole 16x16 tiles
+        size_t lds = N * N * sizeof(double);
         hipLaunchKernelGGL(k_dense_invert, dim3(1), dim3(1024), lds, st, o, H.inv, d_info);
     }
     HMFEM_CHECK(hipGetLastError());
@@ -1786,11 +1892,23 @@ int hierarchy_level_copy(hipStream_t st, const SetupCache &cache, int l, int *di
     return H.nlev;
 }
 
+int hierarchy_coarse_inverse(hipStream_t st, const SetupCache &cache, double *inv)
+{
+    if (!cache.valid || cache.hier.size() != sizeof(Hierarchy)) return HMFEM_ERR_ARG;
+    Hierarchy H;
+    memcpy((void *)&H, cache.hier.data(), sizeof(Hierarchy));
+    if (inv) {
+        HMFEM_CHECK(hipMemcpyAsync(inv, H.inv, (size_t)H.nc * H.nc * sizeof(double), hipMemcpyDeviceToHost, st));
+        HMFEM_CHECK(hipStreamSynchronize(st));
+    }
+    return H.nc;
+}
+
 int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const double *b, double *x,
                      const hmfem_options &opt, char *arena, size_t arena_bytes, CgBuffers &B, CgScalars *h_sc,
                      int *d_info, hmfem_status *out, hipEvent_t ev[4], ProfileSink *sink, std::vector<char> &stage,
                      GraphCache *graph, long lds_limit, Deflation *defl, char *(*dalloc)(void *, size_t, int),
-                     void *dctx, SetupCache *cache)
+                     void *dctx, SetupCache *cache, const PlanHint *hint)
 {
     Prof prof, *P = nullptr;
     if (sink && sink->ev && sink->cap >= 2) {
@@ -1844,6 +1962,13 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
                             opt.smoother, opt.deflate};
         put(&d, sizeof(d)); put(&arena, sizeof(arena)); put(&arena_bytes, sizeof(arena_bytes));
         put(ints, sizeof(ints)); put(&opt.deflate_ratio, sizeof(double)); put(&lds_limit, sizeof(lds_limit));
+        Level lev[kMaxLevels];   // the coarsening plan
+        int nlev = 0;
+        plan_levels(d, clamp_coarse(opt.coarse_max), lev, nlev, hint);
+        for (int l = 0; l < nlev; l++) {
+            const int f[3] = {lev[l].fx, lev[l].fy, lev[l].fz};
+            put(f, sizeof(f));
+        }
     }
     static_assert(std::is_trivially_copyable<Hierarchy>::value, "the hierarchy is cached as bytes");
     const bool reuse = opt.reuse_setup && cache && cache->valid && cache->key == key &&
@@ -1861,7 +1986,7 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
                 if (rc) return rc;
             }
         }
-        rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit);
+        rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit, hint);
         if (rc) return rc;
         if (defl && defl->pending) {
             rc = deflation_finish(st, d, A, opt.deflate_ratio, *defl, dalloc, dctx);

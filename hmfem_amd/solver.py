@@ -62,7 +62,7 @@ class Options(ctypes.Structure):
                 ("nu", c_int), ("omega", c_dbl), ("ocf", c_dbl), ("coarse_max", c_int), ("check_every", c_int),
                 ("tail_max", c_int), ("tail_gamma", c_int), ("use_graph", c_int), ("tail_lds", c_int),
                 ("smoother", c_int), ("gs_sweeps", c_int), ("autotune", c_int), ("deflate", c_int),
-                ("deflate_ratio", c_dbl), ("precond", c_int), ("reuse_setup", c_int)]
+                ("deflate_ratio", c_dbl), ("precond", c_int), ("reuse_setup", c_int), ("semicoarsen", c_int)]
 
 
 class Status(ctypes.Structure):
@@ -177,6 +177,7 @@ def lib():
                                            c_int, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(Options),
                                            P(CoupledStatus)]
         L.hmfem_hierarchy_level.argtypes = [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
+        L.hmfem_hierarchy_coarse_inverse.argtypes = [c_vp, c_vp]
         L.hmfem_flux_device_noalloc.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, P(c_dbl),
                                                 P(c_dbl)]
         L.hmfem_solve_structured_device.argtypes = [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, P(Options),
@@ -210,7 +211,7 @@ def lib():
 EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", "fluxos", "divvel_pontual",
             "prod_mat_vec", "hmfem_default_options", "hmfem_create", "hmfem_destroy", "hmfem_stream",
             "hmfem_assemble_device", "hmfem_reservoir_device", "hmfem_flux_device", "hmfem_flux_device_noalloc",
-            "hmfem_coupled_device", "hmfem_hierarchy_level",
+            "hmfem_coupled_device", "hmfem_hierarchy_level", "hmfem_hierarchy_coarse_inverse",
             "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
             "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device",
             "hmfem_reservoir_block_device", "hmfem_autotune_device", "save_pressure", "save_flux",
@@ -488,6 +489,15 @@ class Context:
                 raise RuntimeError("hmfem_hierarchy_level failed: %d" % rc)
             out.append((tuple(int(v) for v in d), tuple(int(v) for v in f), e, wx, wy, wz))
         return out
+
+    def coarse_inverse(self):
+        """The dense inverse of the last hierarchy's coarsest level (host, nc x nc)."""
+        nc = lib().hmfem_hierarchy_coarse_inverse(self.h, None)
+        if nc < 0:
+            raise RuntimeError("hmfem_hierarchy_coarse_inverse failed: %d" % nc)
+        inv = np.zeros((nc, nc))
+        lib().hmfem_hierarchy_coarse_inverse(self.h, inv.ctypes.data)
+        return inv
 
     def flux_noalloc(self, g, Kx, Ky, Kz, bval, p, v=None):
         """hmfem_flux_device_noalloc: the same results, no device allocation."""

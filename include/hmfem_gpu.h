@@ -195,6 +195,10 @@ typedef struct {
                                 operator while the CG solves the current one (for repeated solves with
                                 small operator changes, e.g. the fixed-point passes of reservoir());
                                 default 0 */
+    int semicoarsen;         /* 1: coarsen an axis only where its couplings are at least a quarter of the
+                                strongest axis's (geometric means over the fine operator): anisotropic cells
+                                or K get boxes of 2x1x1, 2x2x1, ... until the couplings even out; isotropic
+                                problems keep 2x2x2 boxes (single solves; default 1) */
 } hmfem_options;
 
 typedef struct {
@@ -262,6 +266,11 @@ int hmfem_flux_device_noalloc(hmfem_context *ctx, const hmfem_grid *g, const dou
    solve replaced it). */
 int hmfem_hierarchy_level(hmfem_context *ctx, int level, int *dims, int *fac, double *e, double *wx, double *wy,
                           double *wz);
+/* The dense FP64 inverse of that hierarchy's coarsest level (nc x nc,
+   row-major, host; may be NULL to query nc), as the setup computed it on the
+   matrix cores; returns nc or HMFEM_ERR_ARG. No counterpart in the reference,
+   whose coarsest "solve" is a diagonal scaling (amg.h:1361-1362). */
+int hmfem_hierarchy_coarse_inverse(hmfem_context *ctx, double *inv);
 
 /* Solve on an already assembled structured operator given in AoS form (device
    perm/indcol as from hmfem_assemble_device), b and x device FP64. */

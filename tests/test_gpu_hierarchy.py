@@ -104,6 +104,32 @@ def test_levels_are_galerkin_products(prec, ctx, torch):
         _say("level", l, dc, "max |A_c - P^T A P| / max |A| = %.3e" % err)
         assert err < 1e-14, l
         assert np.allclose(Ac, Ac.T, rtol=0, atol=1e-15 * np.abs(Ac).max())
+    # the coarsest level's dense inverse (block Gauss-Jordan on the matrix cores)
+    inv = ctx.coarse_inverse()
+    Ac = dense(levels[-1][0], *levels[-1][2:])
+    assert inv.shape == Ac.shape
+    ref = np.linalg.inv(Ac)
+    e_inv = np.abs(inv - ref).max() / np.abs(ref).max()
+    e_id = np.abs(inv @ Ac - np.eye(Ac.shape[0])).max()
+    _say("coarsest %s: |inv - numpy inv| / |inv| = %.3e, |inv A - I| = %.3e" % (Ac.shape, e_inv, e_id))
+    assert e_inv < 1e-12 and e_id < 1e-10
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 5), (3, 3, 3), (5, 4, 4), (2, 1, 1)], ids=["80", "27", "80b", "2"])
+def test_coarse_inverse_sizes(dims, ctx, torch):
+    """The dense coarsest inverse on levels of 80 (five tiles of 16), 27 and
+    2 cells (padded to whole tiles): a grid that is its own coarsest level."""
+    P = make_problem(dims, "lognormal", seed=3, sigma=1.5, corr=2.0)
+    D = Dev(P, torch)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-12))
+    assert st.status == S.OK and st.levels == 1
+    lv = ctx.hierarchy()
+    Ac = dense(lv[0][0], *lv[0][2:])
+    inv = ctx.coarse_inverse()
+    e_id = np.abs(inv @ Ac - np.eye(Ac.shape[0])).max()
+    _say("coarsest", dims, "|inv A - I| = %.3e" % e_id)
+    assert e_id < 1e-10
 
 
 # CG iterations at the default options when these ceilings were set, with
