@@ -239,6 +239,9 @@ def run_batch(args, w, torch, S):
     opt = S.default_options(rtol=1e-8, precision=prec, check_every=2)
     if args.smoother:
         opt.smoother = S.SMOOTH_JACOBI if args.smoother == "jacobi" else S.SMOOTH_BLOCK_GS
+    for name, val in (("gs_sweeps", args.gs_sweeps), ("ocf", args.ocf)):
+        if val is not None:
+            setattr(opt, name, val)
 
     def step(o=opt):
         with torch.cuda.stream(solver_stream):
@@ -443,6 +446,8 @@ def main():
                     help="hierarchy precision; fp16 = FP32 hierarchy with the fine level stored in FP16")
     ap.add_argument("--smoother", default=None, choices=["jacobi", "blockgs"],
                     help="smoother of the multi-block levels (default: the library's)")
+    ap.add_argument("--gs-sweeps", type=int, default=None, help="block Gauss-Seidel sweeps (default: the library's)")
+    ap.add_argument("--ocf", type=float, default=None, help="coarse-correction scaling (default: the library's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--profile-only", action="store_true", help="timed steps only (for rocprofv3 runs)")
@@ -507,6 +512,9 @@ def main():
     opt = S.default_options(rtol=rtol, precision=prec)
     if args.smoother:
         opt.smoother = S.SMOOTH_JACOBI if args.smoother == "jacobi" else S.SMOOTH_BLOCK_GS
+    for name, val in (("gs_sweeps", args.gs_sweeps), ("ocf", args.ocf)):
+        if val is not None:
+            setattr(opt, name, val)
 
     solver_stream = torch.cuda.ExternalStream(ctx.stream)
 

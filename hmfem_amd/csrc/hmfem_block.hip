@@ -700,7 +700,7 @@ static int b_vcycle_coarse(hipStream_t st, Hierarchy &H, BlockState &S, int l, c
     }
     Level &C = H.lev[l + 1];
     const int GT = grid_for(b_tiles(L.d), 1);
-    const int sw = opt.gs_sweeps < 1 ? 1 : opt.gs_sweeps;
+    const int sw = opt.gs_sweeps < 1 ? 3 : opt.gs_sweeps;   // 0: three sweeps
     hipLaunchKernelGGL((k_b_gs_tile<NR, float, 0>), dim3(GT), dim3(kBThreads), 0, st, opargs<float>(L), sw,
                        (const float *)X.f, (const float *)nullptr, X.u, 0, 0, 0, 0, 0, (const float *)nullptr, 0.f,
                        (const double *)nullptr, 1, (double *)nullptr, S.done);
@@ -722,7 +722,7 @@ static int b_vcycle_fine(hipStream_t st, Hierarchy &H, BlockState &S, const doub
     Level &L = H.lev[0], &C = H.lev[1];
     BlockLevelVecs &X = S.lv[0];
     const int nt = b_tiles(L.d);
-    const int sw = opt.gs_sweeps < 1 ? 1 : opt.gs_sweeps;
+    const int sw = opt.gs_sweeps < 1 ? 3 : opt.gs_sweeps;   // 0: three sweeps
     const bool defl = S.D && S.D->nc > 0;
     if (defl) {   // per-cluster sums of r and v, the correction and its dot partials (row G of Pdot)
         int rc = deflation_sums_block(st, *S.D, NR, S.R, Vp, S.Pdot + (long)2 * NR * S.ch.G, S.done);

@@ -134,7 +134,7 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->use_graph = 1;
     o->tail_lds = 1;
     o->smoother = HMFEM_SMOOTH_BLOCK_GS;
-    o->gs_sweeps = 3;
+    o->gs_sweeps = 0;   // the path's own: 2 on a single grid, 3 in batched and block solves
     o->autotune = 0;
     o->deflate = 1;
     o->deflate_ratio = 3000.0;
@@ -353,7 +353,7 @@ struct TuneCand {
     int tail_gamma;
 };
 static const TuneCand kTuneCands[] = {
-    {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.2, 2}, {HMFEM_SMOOTH_BLOCK_GS, 2, 3, 2.2, 2},
+    {HMFEM_SMOOTH_BLOCK_GS, 2, 3, 2.2, 2}, {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.2, 2},
     {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.5, 2}, {HMFEM_SMOOTH_BLOCK_GS, 2, 3, 2.0, 2},
     {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.0, 2}, {HMFEM_SMOOTH_BLOCK_GS, 4, 3, 2.2, 2},
     {HMFEM_SMOOTH_BLOCK_GS, 3, 3, 2.2, 1}, {HMFEM_SMOOTH_BLOCK_GS, 3, 2, 2.2, 2},

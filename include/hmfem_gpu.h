@@ -177,7 +177,8 @@ typedef struct {
                                 sweeps) or HMFEM_SMOOTH_BLOCK_GS (gs_sweeps red-black Gauss-Seidel sweeps
                                 inside 32x8x8 tiles, one pass per smoothing step; the default); the
                                 levels of the single-workgroup kernel always use Jacobi */
-    int gs_sweeps;           /* sweeps per smoothing step of HMFEM_SMOOTH_BLOCK_GS (default 3) */
+    int gs_sweeps;           /* sweeps per smoothing step of HMFEM_SMOOTH_BLOCK_GS; default 0: two in single
+                                solves, three in batched and block solves (measured best on each) */
     int autotune;            /* 1: hmfem_reservoir_device first runs hmfem_autotune_device for each new
                                 grid shape / precision / tolerance of the context and then uses the
                                 options it chose (default 0) */

@@ -63,7 +63,7 @@ def test_options_defaults():
     o = S.default_options()
     assert o.rtol == 1e-8 and o.max_iterations == 10000 and o.precision == S.PREC_FP32
     assert o.nu >= 1 and 0 < o.omega < 1 and o.coarse_max >= 1
-    assert o.smoother in (S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS) and o.gs_sweeps >= 1 and o.autotune == 0
+    assert o.smoother in (S.SMOOTH_JACOBI, S.SMOOTH_BLOCK_GS) and o.gs_sweeps >= 0 and o.autotune == 0
     assert o.deflate == 1 and o.deflate_ratio == 3000.0 and o.precond == S.PRECOND_AMG and o.reuse_setup == 0
 
 
