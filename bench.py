@@ -576,12 +576,14 @@ def main():
         fp64 = {"value": k64 / (time.perf_counter() - ta), "iterations": s64.iterations}
         fp64["speedup_fp32_over_fp64"] = value / fp64["value"]
         # accuracy of the FP32-AMG solution against an FP64 solve driven to 1e-12
-        pf = p.clone()
+        # the copies run on the solver's stream: the next step's p.zero_() is queued there too
         step(opt)
-        p32 = p.clone()
+        with torch.cuda.stream(solver_stream):
+            p32 = p.clone()
         step(S.default_options(rtol=1e-12, precision=S.PREC_FP64))
-        pf = p.clone()
-        fp64["rel_l2_fp32amg_vs_fp64_1e-12"] = float(torch.linalg.norm(p32 - pf) / torch.linalg.norm(pf))
+        with torch.cuda.stream(solver_stream):
+            pf = p.clone()
+            fp64["rel_l2_fp32amg_vs_fp64_1e-12"] = float(torch.linalg.norm(p32 - pf) / torch.linalg.norm(pf))
 
     # Plain FP64 PCG (diagonal preconditioner, no hierarchy) on the same workload
     plain = None
