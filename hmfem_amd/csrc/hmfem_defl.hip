@@ -8,26 +8,29 @@
 // correction adds, per cluster c of "strong" cells (diagonal d_s at least
 // `ratio` times the median diagonal, 7-point connected, >= kMinCluster cells),
 // one piecewise-constant basis vector 1_c:
-//     z = V(r) + Z E^-1 Z^T r,   Z = [1_c], E = Z^T A Z  (dense, FP64),
-// which keeps the preconditioner symmetric positive definite. No counterpart
-// in the reference (its AMG has no such stage, amg.h:1135-1239).
+//     z = V(r) + Z E^-1 Z^T r,   Z = [1_c], E = Z^T A Z  (FP64),
+// which keeps the preconditioner symmetric positive definite. The clusters
+// are connected components of the strong cells, so no coupling joins two of
+// them and E is diagonal. No counterpart in the reference (its AMG has no
+// such stage, amg.h:1135-1239).
 //
-// Setup: a histogram of log2(d) gives the median; the flags of strong cells
-// go to the host, which finds the clusters (union-find over the strong cells
-// in index order, deterministic), gathers the coefficients of their cells on
-// the device, builds E in a fixed order and inverts it (Cholesky). Per CG
-// iteration: per-cluster sums of r and v (one block per cluster, fixed order),
-// c = E^-1 (Z^T r) with the dot products c.(Z^T r), c.(Z^T v) written as one
-// more partial of the fine-level dots, and z += Z c after the V-cycle.
+// Setup: a histogram of log2(d) gives the median (on the host, overlapped
+// with the hierarchy's setup); on the device, the strong cells are compacted
+// in index order and labelled by a lock-free union-find (deterministic: each
+// root is its component's smallest member); the host picks the 256 largest
+// components from their (root, size) table and lays out the per-cluster
+// arrays; the device places every cell in its cluster's list in index order
+// and sums E's diagonal in a fixed order. Per CG iteration: per-piece sums of
+// r and v (fixed order), c = E^-1 (Z^T r) with the dot products c.(Z^T r),
+// c.(Z^T v) written as one more partial of the fine-level dots, and
+// z += Z c after the V-cycle.
 #include "hmfem_stencil.h"
 
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
-#include <atomic>
-#include <numeric>
-#include <thread>
+#include <array>
 #include <vector>
 
 namespace hmfem {
@@ -229,31 +232,6 @@ k_strong_emit(const DefInst *__restrict__ I, const int *__restrict__ bmap, const
     if (at < cap) list[at] = in.off + (blockIdx.x - in.blk0) * kBlock + threadIdx.x;
 }
 
-// Coefficients of listed cells (global index, instance cinst[q]): e and the
-// couplings to -x,+x,-y,+y,-z,+z (0 where none).
-__global__ void __launch_bounds__(kBlock)
-k_defl_gather(const DefInst *__restrict__ I, const int *__restrict__ cinst, Stencil<double> A,
-              const int *__restrict__ cells, int m, double *out)
-{
-    const int q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= m) return;
-    const DefInst in = I[cinst ? cinst[q] : 0];
-    const int s = cells[q];
-    OpArgs<double> o;
-    o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
-    o.nx = in.nx; o.ny = in.ny; o.nz = in.nz;
-    const Cell c = cell_at(o, s - in.off);
-    const int nx = in.nx, nxy = in.nx * in.ny;
-    double *w = out + 7L * q;
-    w[0] = o.e[s];
-    w[1] = c.i > 0 ? o.wx[s - 1] : 0.0;
-    w[2] = c.i < in.nx - 1 ? o.wx[s] : 0.0;
-    w[3] = c.j > 0 ? o.wy[s - nx] : 0.0;
-    w[4] = c.j < in.ny - 1 ? o.wy[s] : 0.0;
-    w[5] = c.k > 0 ? o.wz[s - nxy] : 0.0;
-    w[6] = c.k < in.nz - 1 ? o.wz[s] : 0.0;
-}
-
 // The per-iteration kernels take NR interleaved right-hand sides (vectors
 // [cell][NR]; NR = 1 for a single solve); per-cluster values are [cluster][NR].
 
@@ -284,16 +262,18 @@ k_defl_sums(const int *__restrict__ pb, const int *__restrict__ cells, const dou
         for (int j = 0; j < NR; j++) { psr[p * NR + j] = acc[j]; psv[p * NR + j] = acc[NR + j]; }
 }
 
-// Per cluster sr, sv = sums over its pieces (fixed order); c = E^-1 sr; the
-// extra dot partials <Zc, r> = c.sr and <Zc, v> = c.sv (Pextra[0..NR) and
-// Pextra[NR..2NR)). E^-1 is symmetric: thread a reads column a, coalesced.
+// Per cluster sr, sv = sums over its pieces (fixed order); c = E^-1 sr (E is
+// diagonal: einv[a] = 1 / E_aa); the extra dot partials <Zc, r> = c.sr and
+// <Zc, v> = c.sv (Pextra[0..NR) and Pextra[NR..2NR)).
 template <int NR>
 __global__ void __launch_bounds__(kBlock)
 k_defl_apply(int nc, const int *__restrict__ cp, const double *__restrict__ einv, const double *__restrict__ psr,
              const double *__restrict__ psv, double *c, double *Pextra, const int *done)
 {
     if (*done) return;
-    __shared__ double srs[kMaxClusters * NR], svs[kMaxClusters * NR];
+    double acc[2 * NR];
+#pragma unroll
+    for (int j = 0; j < 2 * NR; j++) acc[j] = 0.0;
     for (int a = threadIdx.x; a < nc; a += kBlock) {
         double tr[NR], tv[NR];
 #pragma unroll
@@ -301,27 +281,13 @@ k_defl_apply(int nc, const int *__restrict__ cp, const double *__restrict__ einv
         for (int p = cp[a]; p < cp[a + 1]; p++)
 #pragma unroll
             for (int j = 0; j < NR; j++) { tr[j] += psr[p * NR + j]; tv[j] += psv[p * NR + j]; }
-#pragma unroll
-        for (int j = 0; j < NR; j++) { srs[a * NR + j] = tr[j]; svs[a * NR + j] = tv[j]; }
-    }
-    __syncthreads();
-    double acc[2 * NR];
-#pragma unroll
-    for (int j = 0; j < 2 * NR; j++) acc[j] = 0.0;
-    for (int a = threadIdx.x; a < nc; a += kBlock) {
-        double t[NR];
-#pragma unroll
-        for (int j = 0; j < NR; j++) t[j] = 0.0;
-        for (int b = 0; b < nc; b++) {
-            const double e = einv[(long)b * nc + a];
-#pragma unroll
-            for (int j = 0; j < NR; j++) t[j] += e * srs[b * NR + j];
-        }
+        const double ei = einv[a];
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            c[a * NR + j] = t[j];
-            acc[j] += t[j] * srs[a * NR + j];
-            acc[NR + j] += t[j] * svs[a * NR + j];
+            const double t = ei * tr[j];
+            c[a * NR + j] = t;
+            acc[j] += t * tr[j];
+            acc[NR + j] += t * tv[j];
         }
     }
     block_sum<2 * NR>(acc);
@@ -329,14 +295,14 @@ k_defl_apply(int nc, const int *__restrict__ cp, const double *__restrict__ einv
         for (int j = 0; j < 2 * NR; j++) Pextra[j] = acc[j];
 }
 
-// The batch form of k_defl_apply (one right-hand side per instance): c = E^-1
-// sr over all clusters (E is block diagonal over the instances), and per
-// instance q the extra dot partials over its clusters [irange[q],
-// irange[q+1]): Pext[2q] = c.sr, Pext[2q+1] = c.sv (0 without clusters).
+// The batch form of k_defl_apply (one right-hand side per instance): c =
+// E^-1 sr over all clusters, and per instance q the extra dot partials over
+// its clusters [irange[q], irange[q+1]): Pext[2q] = c.sr, Pext[2q+1] = c.sv
+// (0 without clusters).
 __global__ void __launch_bounds__(kBlock)
 k_defl_apply_inst(int nc, const int *__restrict__ cp, const double *__restrict__ einv,
                   const double *__restrict__ psr, const double *__restrict__ psv, const int *__restrict__ irange,
-                  const int *__restrict__ cinst, int ninst, double *c, double *Pext)
+                  int ninst, double *c, double *Pext)
 {
     __shared__ double srs[kMaxClusters], svs[kMaxClusters], cs[kMaxClusters];
     for (int a = threadIdx.x; a < nc; a += kBlock) {
@@ -344,14 +310,8 @@ k_defl_apply_inst(int nc, const int *__restrict__ cp, const double *__restrict__
         for (int p = cp[a]; p < cp[a + 1]; p++) { tr += psr[p]; tv += psv[p]; }
         srs[a] = tr;
         svs[a] = tv;
-    }
-    __syncthreads();
-    for (int a = threadIdx.x; a < nc; a += kBlock) {   // E^-1 is block diagonal over the instances
-        const int q = cinst[a];
-        double t = 0.0;
-        for (int b = irange[q]; b < irange[q + 1]; b++) t += einv[(long)b * nc + a] * srs[b];
-        cs[a] = t;
-        c[a] = t;
+        cs[a] = einv[a] * tr;
+        c[a] = cs[a];
     }
     __syncthreads();
     for (int q = threadIdx.x; q < ninst; q += kBlock) {
@@ -377,6 +337,264 @@ k_defl_scatter(const int *__restrict__ cells, const int *__restrict__ comp, int 
     for (int j = 0; j < NR; j++) z[s * NR + j] = (T)((double)z[s * NR + j] + c[cl * NR + j]);
 }
 
+// ------------------------------------------------- clusters on the device
+//
+// The strong cells (sorted list, index order) are labelled by connected
+// components on the device: a lock-free union-find in which the larger of
+// two roots is hooked under the smaller by compare-and-swap (as in the
+// ECL-CC method of Jaiganesh and Burtscher, 2018), so every root ends as its
+// component's smallest member whatever the order of the unions: the labels
+// are deterministic. A failed swap means another thread's succeeded, so the
+// retry loop always ends. Parent pointers only ever point to smaller
+// ordinals (no cycles); path halving writes an ancestor, which a concurrent
+// writer may replace by another ancestor.
+
+__device__ __forceinline__ int par_load(const int *p, int a)
+{
+    return __hip_atomic_load(p + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void par_store(int *p, int a, int v)
+{
+    __hip_atomic_store(p + a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ int cc_find(int *par, int a)
+{
+    while (true) {
+        const int p = par_load(par, a);
+        if (p == a) return a;
+        const int g = par_load(par, p);
+        if (g == p) return p;
+        par_store(par, a, g);   // path halving
+        a = g;
+    }
+}
+
+__device__ void cc_unite(int *par, int a, int b)
+{
+    int ra = cc_find(par, a), rb = cc_find(par, b);
+    while (ra != rb) {
+        if (ra > rb) { const int t = ra; ra = rb; rb = t; }
+        int seen = rb;
+        if (__hip_atomic_compare_exchange_strong(par + rb, &seen, ra, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+        ra = cc_find(par, ra);
+        rb = cc_find(par, seen);
+    }
+}
+
+// index of key in the sorted a[lo, hi), or -1
+__device__ __forceinline__ int sorted_find(const int *__restrict__ a, int lo, int hi, int key)
+{
+    int h = hi;
+    while (lo < h) {
+        const int mid = (lo + h) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else h = mid;
+    }
+    return lo < hi && a[lo] == key ? lo : -1;
+}
+
+// Work arrays of the device labelling (ub: bound on the strong cells).
+struct CcWork {
+    int *par, *root, *cin, *size, *cid, *lord, *nbr;   // [ub] each, nbr [6 ub]: -x,+x,-y,+y,-z,+z
+    int *ncand, *cand;                                  // candidates (root, size, instance) x ccap
+    int *ccount;                                        // per chunk of kChunk strong cells, per cluster
+    int *sel;                                           // chosen roots (ascending), then the cluster offsets
+};
+constexpr int kChunk = 1024;
+
+// Per strong cell a < ms: its instance, its six strong neighbours inside the
+// instance (ordinals, -1: none), parent = itself, size 0.
+__global__ void __launch_bounds__(kBlock)
+k_cc_links(const DefInst *__restrict__ I, int ninst, const int *__restrict__ list, const int *__restrict__ pms,
+           int cap, CcWork w)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *w.ncand = 0;
+    const int ms = min(*pms, cap);
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= ms) return;
+    const int s = list[a];
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (I[mid].off <= s) lo = mid;
+        else hi = mid - 1;
+    }
+    const DefInst in = I[lo];
+    const int l = s - in.off, i = l % in.nx, jk = l / in.nx, j = jk % in.ny, k = jk / in.ny;
+    const int sy = in.nx, sz = in.nx * in.ny;
+    // a neighbour at +-step lies within step places of a in the sorted list
+    int *nb = w.nbr + 6L * a;
+    nb[0] = i > 0 ? sorted_find(list, max(0, a - 1), a, s - 1) : -1;
+    nb[1] = i + 1 < in.nx ? sorted_find(list, a + 1, min(ms, a + 2), s + 1) : -1;
+    nb[2] = j > 0 ? sorted_find(list, max(0, a - sy), a, s - sy) : -1;
+    nb[3] = j + 1 < in.ny ? sorted_find(list, a + 1, min(ms, a + 1 + sy), s + sy) : -1;
+    nb[4] = k > 0 ? sorted_find(list, max(0, a - sz), a, s - sz) : -1;
+    nb[5] = k + 1 < in.nz ? sorted_find(list, a + 1, (int)min((long)ms, (long)a + 1 + sz), s + sz) : -1;
+    w.cin[a] = lo;
+    w.par[a] = a;
+    w.size[a] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cc_hook(const int *__restrict__ pms, int cap, CcWork w)
+{
+    const int ms = min(*pms, cap);
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= ms) return;
+    const int *nb = w.nbr + 6L * a;
+    for (int f = 1; f < 6; f += 2)
+        if (nb[f] >= 0) cc_unite(w.par, a, nb[f]);
+}
+
+// roots and component sizes (after every union has finished)
+__global__ void __launch_bounds__(kBlock) k_cc_roots(const int *__restrict__ pms, int cap, CcWork w)
+{
+    const int ms = min(*pms, cap);
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= ms) return;
+    const int r = cc_find(w.par, a);
+    w.root[a] = r;
+    atomicAdd(w.size + r, 1);
+}
+
+// components of at least kMinCluster cells: (root, size, instance), in any
+// order (the host sorts them)
+__global__ void __launch_bounds__(kBlock) k_cc_cands(const int *__restrict__ pms, int cap, int ccap, CcWork w)
+{
+    const int ms = min(*pms, cap);
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= ms || w.root[a] != a || w.size[a] < kMinCluster) return;
+    const int t = atomicAdd(w.ncand, 1);
+    if (t < ccap) {
+        w.cand[3 * t] = a;
+        w.cand[3 * t + 1] = w.size[a];
+        w.cand[3 * t + 2] = w.cin[a];
+    }
+}
+
+// cluster of each strong cell (the rank of its root among the chosen roots, or -1)
+__global__ void __launch_bounds__(kBlock) k_cc_ordinal(const int *__restrict__ pms, int cap, int nc, CcWork w)
+{
+    const int ms = min(*pms, cap);
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= ms) return;
+    w.cid[a] = sorted_find(w.sel, 0, nc, w.root[a]);
+}
+
+// Placing the cells of every cluster, in index order, at their offset in
+// the cluster-grouped list: per chunk of kChunk strong cells the count of
+// each cluster, the counts turned into offsets chunk after chunk, then every
+// cell at its cluster's offset for its chunk plus its rank among the chunk's
+// cells of the same cluster (waves by ballots, then the waves before it).
+__global__ void __launch_bounds__(kChunk) k_place_count(const int *__restrict__ pms, int cap, int nc, CcWork w)
+{
+    __shared__ int cnt[kMaxClusters];
+    for (int c = threadIdx.x; c < nc; c += kChunk) cnt[c] = 0;
+    __syncthreads();
+    const int ms = min(*pms, cap);
+    const int a = blockIdx.x * kChunk + threadIdx.x;
+    const int c = a < ms ? w.cid[a] : -1;
+    if (c >= 0) atomicAdd(cnt + c, 1);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nc; q += kChunk) w.ccount[(long)blockIdx.x * nc + q] = cnt[q];
+}
+
+__global__ void __launch_bounds__(kMaxClusters) k_place_scan(int nc, int nchunks, CcWork w)
+{
+    const int c = threadIdx.x;
+    if (c >= nc) return;
+    int run = w.sel[kMaxClusters + c];   // the cluster's offset in the list
+    for (int b = 0; b < nchunks; b++) {
+        const long q = (long)b * nc + c;
+        const int v = w.ccount[q];
+        w.ccount[q] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(kChunk)
+k_place_emit(const int *__restrict__ list, const int *__restrict__ pms, int cap, int nc, int m, CcWork w, int *cells,
+             int *comp)
+{
+    constexpr int kW = kChunk / 64;
+    __shared__ int wc[kW][kMaxClusters];
+    for (int q = threadIdx.x; q < kW * kMaxClusters; q += kChunk) wc[q / kMaxClusters][q % kMaxClusters] = 0;
+    __syncthreads();
+    const int ms = min(*pms, cap);
+    const int a = blockIdx.x * kChunk + threadIdx.x;
+    const int c = a < ms ? w.cid[a] : -1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long left = __ballot(c >= 0), mine = 0;
+    while (left) {   // one distinct cluster of the wave per round (wave-uniform loop)
+        const int lead = __ffsll((long long)left) - 1;
+        const int lc = __shfl(c, lead);
+        const unsigned long long same = __ballot(c == lc) & left;
+        if (c == lc) mine = same;
+        if (lane == lead) wc[wv][lc] = __popcll(same);
+        left &= ~same;
+    }
+    __syncthreads();
+    if (c < 0) return;
+    int rank = __popcll(mine & ((1ull << lane) - 1ull));
+    for (int q = 0; q < wv; q++) rank += wc[q][c];
+    const int pos = w.ccount[(long)blockIdx.x * nc + c] + rank;
+    if (pos < m) {
+        cells[pos] = list[a];
+        comp[pos] = c;
+        comp[m + pos] = w.cin[a];
+        w.lord[pos] = a;
+    }
+}
+
+// E = Z^T A Z is diagonal: a strong neighbour of a strong cell is in its
+// component, so no coupling joins two clusters. Per listed cell its share of
+// E_aa: e plus the couplings to cells outside its cluster (not strong).
+__global__ void __launch_bounds__(kBlock)
+k_defl_eshare(const DefInst *__restrict__ I, Stencil<double> A, const int *__restrict__ cells,
+              const int *__restrict__ comp, int m, CcWork w, double *share)
+{
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= m) return;
+    const DefInst in = I[comp[m + q]];
+    const int s = cells[q];
+    const int l = s - in.off, i = l % in.nx, jk = l / in.nx, j = jk % in.ny, k = jk / in.ny;
+    const int nx = in.nx, nxy = in.nx * in.ny;
+    const int *nb = w.nbr + 6L * w.lord[q];
+    double t = A.e[s];
+    const double wf[6] = {i > 0 ? A.wx[s - 1] : 0.0, i < in.nx - 1 ? A.wx[s] : 0.0,
+                          j > 0 ? A.wy[s - nx] : 0.0, j < in.ny - 1 ? A.wy[s] : 0.0,
+                          k > 0 ? A.wz[s - nxy] : 0.0, k < in.nz - 1 ? A.wz[s] : 0.0};
+    for (int f = 0; f < 6; f++)
+        if (nb[f] < 0) t += wf[f];
+    share[q] = t;
+}
+
+// per piece: the sum of its cells' shares (fixed order)
+__global__ void __launch_bounds__(kBlock) k_defl_epiece(const int *__restrict__ pb, const double *__restrict__ share,
+                                                          double *pe)
+{
+    const int p = blockIdx.x;
+    double acc[1] = {0.0};
+    for (int q = pb[p] + threadIdx.x; q < pb[p + 1]; q += kBlock) acc[0] += share[q];
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) pe[p] = acc[0];
+}
+
+// E_aa from its pieces, and 1 / E_aa (0 for a cluster with E_aa <= 0,
+// whose correction then stays off)
+__global__ void __launch_bounds__(kBlock)
+k_defl_einv(int nc, const int *__restrict__ cp, const double *__restrict__ pe, double *einv)
+{
+    for (int a = threadIdx.x; a < nc; a += kBlock) {
+        double e = 0.0;
+        for (int p = cp[a]; p < cp[a + 1]; p++) e += pe[p];
+        einv[a] = e > 0.0 ? 1.0 / e : 0.0;
+    }
+}
+
 // ---------------------------------------------------------------- host side
 
 // pinned host staging of at least `bytes` (grown on demand, freed with the context)
@@ -391,16 +609,12 @@ static int stage_reserve(Deflation &D, size_t bytes)
     return 0;
 }
 
-static int find_root(std::vector<int> &p, int a)
-{
-    while (p[a] != a) { p[a] = p[p[a]]; a = p[a]; }
-    return a;
-}
-
 // Setup shared by the single grid and the batch. hist: per instance the
 // host copy of its histogram; inst: the instances (thr filled here); dinst:
-// their device copy (nullptr: upload to slot 0 after the histogram, at
-// hist_bytes); bmap/G: the per-cell kernels' block map and grid.
+// their device copy; bmap/G: the per-cell kernels' block map and grid.
+// Everything per cell runs on the device; the host chooses the clusters
+// from the table of components (root, size, instance) and lays out the
+// per-cluster index arrays: one round trip of a few KB.
 static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *dinst, const int *bmap, int G,
                            const Stencil<double> &A, const unsigned *hist, double ratio, Deflation &D,
                            char *(*alloc)(void *, size_t, int), void *actx)
@@ -432,153 +646,79 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     if (above < kMinCluster) return HMFEM_OK;
     if (ub > (1L << 22)) return HMFEM_OK;   // not conduits: a large part of the domain
 
-    // the strong cells, compacted on the device in index order (at most ub)
-    // slot 1: masks (G * kWavesPerBlock words), block offsets (G + 1), list
-    // (ub; right after the total boff[G], so that one copy fetches both), scan
-    // tile totals (nt)
+    // slot 1: masks (G * kWavesPerBlock words), block offsets (G + 1), the
+    // strong list (ub), scan tile totals (nt), then the labelling's arrays
     const int nt = (G + kScanThreads - 1) / kScanThreads;
     if (nt > kScanThreads) return HMFEM_OK;   // beyond 2^28 cells
+    const int cap = (int)ub, ccap = cap / kMinCluster + 1, nchunks = (cap + kChunk - 1) / kChunk;
     const size_t mb_bytes = sizeof(unsigned long long) * (size_t)G * kWavesPerBlock;
-    char *sbuf = alloc(actx, mb_bytes + sizeof(int) * ((size_t)G + 1 + nt + ub), 1);
+    const size_t ints = (size_t)G + 1 + nt + (size_t)cap * 13 + 1 + 3 * (size_t)ccap +
+                        (size_t)nchunks * kMaxClusters + 2 * (size_t)kMaxClusters + 1;
+    char *sbuf = alloc(actx, mb_bytes + sizeof(int) * ints, 1);
     if (!sbuf) return HMFEM_ERR_NOMEM;
     unsigned long long *mask = (unsigned long long *)sbuf;
     int *boff = (int *)(sbuf + mb_bytes);
     int *slist = boff + G + 1;
-    int *tot = slist + ub;
+    int *tot = slist + cap;
+    CcWork w;
+    w.par = tot + nt;
+    w.root = w.par + cap;
+    w.cin = w.root + cap;
+    w.size = w.cin + cap;
+    w.cid = w.size + cap;
+    w.lord = w.cid + cap;
+    w.nbr = w.lord + cap;
+    w.ncand = w.nbr + 6L * cap;
+    w.cand = w.ncand + 1;
+    w.ccount = w.cand + 3L * ccap;
+    w.sel = w.ccount + (long)nchunks * kMaxClusters;
+    const int *pms = boff + G;   // the number of strong cells (capped at cap by every kernel)
     HMFEM_CHECK(hipMemcpyAsync(dinst, inst.data(), sizeof(DefInst) * ninst, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_strong_mask, dim3(G), dim3(kBlock), 0, st, dinst, bmap, A, mask, boff);
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, st, boff, G, tot);
     hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, st, tot, nt, boff, G);
     hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(kScanThreads), 0, st, boff, G, (const int *)tot);
-    hipLaunchKernelGGL(k_strong_emit, dim3(G), dim3(kBlock), 0, st, dinst, bmap, mask, boff, slist, (int)ub);
-    // the total (boff[G]) and the list in one copy to pinned staging
-    const size_t cb = sizeof(int) * (1 + ub);
-    if (stage_reserve(D, cb + sizeof(double) * 7 * (size_t)ub)) return HMFEM_ERR_NOMEM;
-    int *hl = (int *)D.h_stage;
-    HMFEM_CHECK(hipMemcpyAsync(hl, boff + G, cb, hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_strong_emit, dim3(G), dim3(kBlock), 0, st, dinst, bmap, mask, boff, slist, cap);
+    const dim3 gl((cap + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_cc_links, gl, dim3(kBlock), 0, st, (const DefInst *)dinst, ninst, (const int *)slist, pms,
+                       cap, w);
+    hipLaunchKernelGGL(k_cc_hook, gl, dim3(kBlock), 0, st, pms, cap, w);
+    hipLaunchKernelGGL(k_cc_roots, gl, dim3(kBlock), 0, st, pms, cap, w);
+    hipLaunchKernelGGL(k_cc_cands, gl, dim3(kBlock), 0, st, pms, cap, ccap, w);
+
+    // the components to the host: the count, then the table (a first part in
+    // the same copy; the rest, if any, in a second)
+    const int first = std::min(ccap, 4096);
+    if (stage_reserve(D, sizeof(int) * (1 + 3 * (size_t)ccap))) return HMFEM_ERR_NOMEM;
+    int *hc = (int *)D.h_stage;
+    HMFEM_CHECK(hipMemcpyAsync(hc, w.ncand, sizeof(int) * (1 + 3 * (size_t)first), hipMemcpyDeviceToHost, st));
     HMFEM_CHECK(hipStreamSynchronize(st));
-    const int ms = std::min(hl[0], (int)ub);
-    if (ms < kMinCluster) return HMFEM_OK;
+    const int ncand = std::min(hc[0], ccap);
+    if (ncand > first) {
+        HMFEM_CHECK(hipMemcpyAsync(hc + 1 + 3 * first, w.cand + 3 * first, sizeof(int) * 3 * (size_t)(ncand - first),
+                                   hipMemcpyDeviceToHost, st));
+        HMFEM_CHECK(hipStreamSynchronize(st));
+    }
+    if (ncand == 0) return HMFEM_OK;
+    // the kMaxClusters largest (ties: the smaller root first), then in root
+    // order, which is instance order (the list is sorted by cell index)
     DeflationHost &W = D.h;
-    auto &cell = W.cell, &cin = W.cin;
-    cell.assign(hl + 1, hl + 1 + ms);   // in index order: a deterministic labelling
-    cin.resize(ms);                     // instance of each strong cell
-    for (int a = 0, q = 0; a < ms; a++) {
-        while (q + 1 < ninst && cell[a] >= inst[q + 1].off) q++;
-        cin[a] = q;
-    }
-
-    // strong neighbours of each strong cell, inside its instance: ordinals of
-    // the +x, +y, +z neighbours by one monotone sweep per direction (cell[]
-    // is sorted and a neighbour stays inside the instance), the -x, -y, -z
-    // ones by symmetry; then a union-find over the +x, +y, +z links. Both stay
-    // inside an instance, so ranges of whole instances (contiguous in the
-    // sorted list) run on host threads of their own. The union keeps the
-    // smaller ordinal as the root, so every root is its component's smallest
-    // member whatever the order of the unions.
-    auto &nbr = W.nbr;
-    auto &up = W.up;
-    auto &par = W.par;
-    auto &root = W.root, &size = W.size;
-    nbr.resize(6L * ms);   // [a][-x,+x,-y,+y,-z,+z]
-    up.resize(ms);         // bit f: the +f neighbour is inside the instance
-    par.resize(ms);
-    root.resize(ms);
-    size.resize(ms);
-    auto label = [&](int a0, int a1) {
-        std::fill(nbr.begin() + 6L * a0, nbr.begin() + 6L * a1, -1);
-        for (int a = a0; a < a1; a++) {
-            const DefInst &in = inst[cin[a]];
-            const int l = cell[a] - in.off, i = l % in.nx, jk = l / in.nx, j = jk % in.ny, k = jk / in.ny;
-            up[a] = (unsigned char)((i + 1 < in.nx) | ((j + 1 < in.ny) << 1) | ((k + 1 < in.nz) << 2));
-        }
-        for (int f = 0; f < 3; f++) {
-            int p = a0;
-            for (int a = a0; a < a1; a++) {
-                if (!((up[a] >> f) & 1)) continue;
-                const DefInst &in = inst[cin[a]];
-                const long t = cell[a] + (f == 0 ? 1 : f == 1 ? (long)in.nx : (long)in.nx * in.ny);
-                while (p < a1 && cell[p] < t) p++;
-                if (p < a1 && cell[p] == t) {
-                    nbr[6L * a + 2 * f + 1] = p;
-                    nbr[6L * p + 2 * f] = a;
-                }
-            }
-        }
-        std::iota(par.begin() + a0, par.begin() + a1, a0);
-        for (int a = a0; a < a1; a++)
-            for (int f = 1; f < 6; f += 2) {
-                const int b = nbr[6L * a + f];
-                if (b < 0) continue;
-                int ra = find_root(par, a), rb = find_root(par, b);
-                if (ra != rb) par[std::max(ra, rb)] = std::min(ra, rb);
-            }
-        std::fill(size.begin() + a0, size.begin() + a1, 0);
-        for (int a = a0; a < a1; a++) { root[a] = find_root(par, a); size[root[a]]++; }
-    };
-    {
-        // ranges of whole instances with about ms / nt strong cells each
-        const int nt = ms < 1024 || ninst < 2
-                           ? 1
-                           : (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-        std::vector<int> cut(1, 0);
-        for (int a = 1; a < ms && (int)cut.size() < nt; a++)
-            if (cin[a] != cin[a - 1] && (long)a * nt >= (long)ms * (long)cut.size()) cut.push_back(a);
-        cut.push_back(ms);
-        std::vector<std::thread> th;
-        for (size_t t = 1; t + 1 < cut.size(); t++) {
-            try {
-                th.emplace_back(label, cut[t], cut[t + 1]);
-            } catch (...) {   // no thread to be had: this range on the calling thread
-                label(cut[t], cut[t + 1]);
-            }
-        }
-        label(cut[0], cut[1]);
-        for (auto &x : th) x.join();
-    }
-    auto &roots = W.roots;
-    roots.clear();
-    for (int a = 0; a < ms; a++)
-        if (root[a] == a && size[a] >= kMinCluster) roots.push_back(a);
-    std::stable_sort(roots.begin(), roots.end(), [&](int x, int y) { return size[x] > size[y]; });
-    if ((int)roots.size() > kMaxClusters) roots.resize(kMaxClusters);
-    std::sort(roots.begin(), roots.end());   // cluster order: by smallest member (so by instance)
-    const int nc = (int)roots.size();
-    if (nc == 0) return HMFEM_OK;
-    auto &cid = W.cid;
-    cid.assign(ms, -1);
-    for (int q = 0; q < nc; q++) cid[roots[q]] = q;
-    for (int a = 0; a < ms; a++) cid[a] = cid[root[a]];
-    auto &ptr = W.ptr, &lcell = W.lcell, &lcomp = W.lcomp, &lord = W.lord, &linst = W.linst;
+    auto &cand = W.cand;
+    cand.resize(ncand);
+    for (int q = 0; q < ncand; q++) cand[q] = {hc[1 + 3 * q], hc[2 + 3 * q], hc[3 + 3 * q]};
+    std::sort(cand.begin(), cand.end(), [](const std::array<int, 3> &x, const std::array<int, 3> &y) {
+        return x[1] != y[1] ? x[1] > y[1] : x[0] < y[0];
+    });
+    if ((int)cand.size() > kMaxClusters) cand.resize(kMaxClusters);
+    std::sort(cand.begin(), cand.end(),
+              [](const std::array<int, 3> &x, const std::array<int, 3> &y) { return x[0] < y[0]; });
+    const int nc = (int)cand.size();
+    // cluster offsets in the list, pieces of at most kPiece cells, the
+    // clusters of each instance, the instance of each cluster
+    auto &ptr = W.ptr, &pbh = W.pbh, &cph = W.cph, &irange = W.irange, &pc = W.pc;
     ptr.assign(nc + 1, 0);
-    for (int a = 0; a < ms; a++)
-        if (cid[a] >= 0) ptr[cid[a] + 1]++;
-    for (int q = 0; q < nc; q++) ptr[q + 1] += ptr[q];
+    for (int q = 0; q < nc; q++) ptr[q + 1] = ptr[q] + cand[q][1];
     const int m = ptr[nc];
-    lcell.resize(m);
-    lcomp.resize(m);
-    lord.resize(m);
-    linst.resize(m);
-    {
-        auto &fill = W.fill;
-        fill.assign(ptr.begin(), ptr.end() - 1);
-        for (int a = 0; a < ms; a++)
-            if (cid[a] >= 0) {
-                const int q = fill[cid[a]]++;
-                lcell[q] = cell[a];
-                lcomp[q] = cid[a];
-                lord[q] = a;
-                linst[q] = cin[a];
-            }
-    }
-    // clusters of each instance: [irange[q], irange[q+1])
-    auto &irange = W.irange;
-    irange.assign(ninst + 1, 0);
-    for (int q = 0; q < nc; q++) irange[cin[roots[q]] + 1]++;
-    for (int q = 0; q < ninst; q++) irange[q + 1] += irange[q];
-
-    // pieces of at most kPiece cells per cluster, contiguous in the list
-    auto &pbh = W.pbh, &cph = W.cph;
     pbh.assign(1, 0);
     cph.assign(1, 0);
     for (int q = 0; q < nc; q++) {
@@ -586,142 +726,52 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
         cph.push_back((int)pbh.size() - 1);
     }
     const int np = (int)pbh.size() - 1;
-    auto &pc = W.pc;
+    irange.assign(ninst + 1, 0);
+    for (int q = 0; q < nc; q++) irange[cand[q][2] + 1]++;
+    for (int q = 0; q < ninst; q++) irange[q + 1] += irange[q];
     pc.assign(pbh.begin(), pbh.end());
     pc.insert(pc.end(), cph.begin(), cph.end());
     pc.insert(pc.end(), irange.begin(), irange.end());
-    for (int q = 0; q < nc; q++) pc.push_back(cin[roots[q]]);   // instance of each cluster
+    for (int q = 0; q < nc; q++) pc.push_back(cand[q][2]);
 
-    // device copies, coefficients of the listed cells
+    // device buffers of the correction
     D.cells = (int *)alloc(actx, sizeof(int) * m, 2);
     D.comp = (int *)alloc(actx, sizeof(int) * 2 * (size_t)m, 3);   // cluster, then instance, of each cell
     D.pb = (int *)alloc(actx, sizeof(int) * pc.size(), 4);
-    D.einv = (double *)alloc(actx, sizeof(double) * nc * nc, 5);
+    D.einv = (double *)alloc(actx, sizeof(double) * nc, 5);
     const size_t per = (size_t)Deflation::kMaxRhs;
     D.c = (double *)alloc(actx, sizeof(double) * (per * (kMaxClusters + 2 * (size_t)np) + 2 * (size_t)ninst), 6);
-    double *coef = (double *)alloc(actx, sizeof(double) * 7 * m, 7);
-    if (!D.cells || !D.comp || !D.pb || !D.einv || !D.c || !coef) return HMFEM_ERR_NOMEM;
+    double *share = (double *)alloc(actx, sizeof(double) * ((size_t)m + np), 7);
+    if (!D.cells || !D.comp || !D.pb || !D.einv || !D.c || !share) return HMFEM_ERR_NOMEM;
     D.cp = D.pb + np + 1;
     D.irange = D.cp + nc + 1;
     D.cinst = D.irange + ninst + 1;
     D.psr = D.c + per * kMaxClusters;
     D.psv = D.psr + per * np;
     D.pext = D.psv + per * np;
-    int *dlinst = D.comp + m;
-    // the lists go up through the pinned staging; the coefficient copy below
-    // reuses it only after these copies (same stream)
-    {
-        const size_t lb = sizeof(int) * (3 * (size_t)m + pc.size()), cbytes = sizeof(double) * 7 * (size_t)m;
-        if (stage_reserve(D, std::max(lb, cbytes))) return HMFEM_ERR_NOMEM;   // the strong list is copied out
-        int *hs = (int *)D.h_stage;
-        memcpy(hs, lcell.data(), sizeof(int) * m);
-        memcpy(hs + m, lcomp.data(), sizeof(int) * m);
-        memcpy(hs + 2 * (size_t)m, linst.data(), sizeof(int) * m);
-        memcpy(hs + 3 * (size_t)m, pc.data(), sizeof(int) * pc.size());
-        HMFEM_CHECK(hipMemcpyAsync(D.cells, hs, sizeof(int) * m, hipMemcpyHostToDevice, st));
-        HMFEM_CHECK(hipMemcpyAsync(D.comp, hs + m, sizeof(int) * 2 * (size_t)m, hipMemcpyHostToDevice, st));
-        HMFEM_CHECK(hipMemcpyAsync(D.pb, hs + 3 * (size_t)m, sizeof(int) * pc.size(), hipMemcpyHostToDevice, st));
-    }
-    hipLaunchKernelGGL(k_defl_gather, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, dinst,
-                       (const int *)dlinst, A, D.cells, m, coef);
-    const double *w = (const double *)D.h_stage;   // 7 m doubles: reserved above
-    HMFEM_CHECK(hipMemcpyAsync(D.h_stage, coef, sizeof(double) * 7 * m, hipMemcpyDeviceToHost, st));
-    HMFEM_CHECK(hipStreamSynchronize(st));
-
-    // E = Z^T A Z in a fixed order: E_aa = sum (e + couplings leaving the
-    // cluster), E_ab = - couplings between clusters a and b (block diagonal
-    // over the instances); its inverse via Cholesky E = L L^T (E is
-    // symmetric positive definite), block by block. An instance's clusters
-    // and their cells are contiguous in the lists, so ranges of whole
-    // instances run on host threads of their own (each writes only its own
-    // blocks of E and of the inverse).
-    auto &E = W.E;
-    auto &inv = W.inv;
-    E.assign((size_t)nc * nc, 0.0);
-    inv.assign((size_t)nc * nc, 0.0);
-    std::atomic<int> not_pd{0};
-    auto build_invert = [&](int q0, int q1) {
-        std::vector<double> L, y, xb;
-        for (int q = q0; q < q1; q++) {
-            const int lo = irange[q], nb = irange[q + 1] - lo;
-            if (nb == 0) continue;
-            for (int c = ptr[lo]; c < ptr[lo + nb]; c++) {
-                const int a = lcomp[c];
-                const double *ws = &w[7L * c];
-                const int *nbp = &nbr[6L * lord[c]];
-                double daa = ws[0];
-                for (int f = 0; f < 6; f++) {
-                    const double wf = ws[1 + f];
-                    if (wf == 0.0) continue;
-                    const int b = nbp[f] < 0 ? -1 : cid[nbp[f]];
-                    if (b == a) continue;
-                    daa += wf;
-                    if (b >= 0) E[(size_t)a * nc + b] -= wf;
-                }
-                E[(size_t)a * nc + a] += daa;
-            }
-            auto Eb = [&](int i, int j) { return E[(size_t)(lo + i) * nc + lo + j]; };
-            L.resize((size_t)nb * nb);
-            for (int i = 0; i < nb; i++)
-                for (int j = 0; j < nb; j++) L[(size_t)i * nb + j] = Eb(i, j);
-            for (int j = 0; j < nb; j++) {
-                double djj = L[(size_t)j * nb + j];
-                for (int k = 0; k < j; k++) djj -= L[(size_t)j * nb + k] * L[(size_t)j * nb + k];
-                if (!(djj > 0)) {   // not positive definite: the correction stays off
-                    not_pd = 1;
-                    return;
-                }
-                djj = sqrt(djj);
-                L[(size_t)j * nb + j] = djj;
-                for (int i = j + 1; i < nb; i++) {
-                    double t = L[(size_t)i * nb + j];
-                    for (int k = 0; k < j; k++) t -= L[(size_t)i * nb + k] * L[(size_t)j * nb + k];
-                    L[(size_t)i * nb + j] = t / djj;
-                }
-            }
-            y.resize(nb);
-            xb.resize((size_t)nb * nb);
-            for (int col = 0; col < nb; col++) {   // solve L L^T x = e_col
-                for (int i = 0; i < nb; i++) {
-                    double t = i == col ? 1.0 : 0.0;
-                    for (int k = 0; k < i; k++) t -= L[(size_t)i * nb + k] * y[k];
-                    y[i] = t / L[(size_t)i * nb + i];
-                }
-                for (int i = nb - 1; i >= 0; i--) {
-                    double t = y[i];
-                    for (int k = i + 1; k < nb; k++) t -= L[(size_t)k * nb + i] * xb[(size_t)k * nb + col];
-                    xb[(size_t)i * nb + col] = t / L[(size_t)i * nb + i];
-                }
-            }
-            for (int i = 0; i < nb; i++)
-                for (int j = 0; j < nb; j++) inv[(size_t)(lo + i) * nc + lo + j] = xb[(size_t)i * nb + j];
-        }
-    };
-    {
-        // ranges of whole instances with about m / nt listed cells each
-        const int nt = m < 1024 || ninst < 2
-                           ? 1
-                           : (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-        std::vector<int> cut(1, 0);
-        for (int q = 1; q < ninst && (int)cut.size() < nt; q++)
-            if ((long)ptr[irange[q]] * nt >= (long)m * (long)cut.size() && ptr[irange[q]] > ptr[irange[cut.back()]])
-                cut.push_back(q);
-        cut.push_back(ninst);
-        std::vector<std::thread> th;
-        for (size_t t = 1; t + 1 < cut.size(); t++) {
-            try {
-                th.emplace_back(build_invert, cut[t], cut[t + 1]);
-            } catch (...) {   // no thread to be had: this range on the calling thread
-                build_invert(cut[t], cut[t + 1]);
-            }
-        }
-        build_invert(cut[0], cut[1]);
-        for (auto &x : th) x.join();
-    }
-    if (not_pd) return HMFEM_OK;
-    if (stage_reserve(D, sizeof(double) * nc * nc)) return HMFEM_ERR_NOMEM;   // the coefficients are used up
-    memcpy(D.h_stage, inv.data(), sizeof(double) * nc * nc);
-    HMFEM_CHECK(hipMemcpyAsync(D.einv, D.h_stage, sizeof(double) * nc * nc, hipMemcpyHostToDevice, st));
+    // chosen roots (kMaxClusters slots), cluster offsets, then the piece
+    // arrays, up through the pinned staging (the table above is used up)
+    const size_t sel_ints = 2 * (size_t)kMaxClusters + 1;
+    if (stage_reserve(D, sizeof(int) * (sel_ints + pc.size()))) return HMFEM_ERR_NOMEM;
+    int *hs = (int *)D.h_stage;
+    for (int q = 0; q < kMaxClusters; q++) hs[q] = q < nc ? cand[q][0] : 0;
+    for (int q = 0; q <= kMaxClusters; q++) hs[kMaxClusters + q] = ptr[std::min(q, nc)];
+    memcpy(hs + sel_ints, pc.data(), sizeof(int) * pc.size());
+    HMFEM_CHECK(hipMemcpyAsync(w.sel, hs, sizeof(int) * sel_ints, hipMemcpyHostToDevice, st));
+    HMFEM_CHECK(hipMemcpyAsync(D.pb, hs + sel_ints, sizeof(int) * pc.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_cc_ordinal, gl, dim3(kBlock), 0, st, pms, cap, nc, w);
+    hipLaunchKernelGGL(k_place_count, dim3(nchunks), dim3(kChunk), 0, st, pms, cap, nc, w);
+    hipLaunchKernelGGL(k_place_scan, dim3(1), dim3(kMaxClusters), 0, st, nc, nchunks, w);
+    hipLaunchKernelGGL(k_place_emit, dim3(nchunks), dim3(kChunk), 0, st, (const int *)slist, pms, cap, nc, m, w,
+                       D.cells, D.comp);
+    hipLaunchKernelGGL(k_defl_eshare, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, st, (const DefInst *)dinst,
+                       A, (const int *)D.cells, (const int *)D.comp, m, w, share);
+    hipLaunchKernelGGL(k_defl_epiece, dim3(np), dim3(kBlock), 0, st, (const int *)D.pb, (const double *)share,
+                       share + m);
+    hipLaunchKernelGGL(k_defl_einv, dim3(1), dim3(kBlock), 0, st, nc, (const int *)D.cp, (const double *)(share + m),
+                       D.einv);
+    HMFEM_CHECK(hipGetLastError());
+    // the staging is rewritten by the next setup: these copies must be done
     HMFEM_CHECK(hipStreamSynchronize(st));
     D.nc = nc;
     D.m = m;
@@ -856,7 +906,7 @@ void deflation_sums_batch(hipStream_t st, const Deflation &D, const double *r, c
 {
     hipLaunchKernelGGL(k_defl_sums<1>, dim3(D.np), dim3(kBlock), 0, st, D.pb, D.cells, r, v, D.psr, D.psv, never);
     hipLaunchKernelGGL(k_defl_apply_inst, dim3(1), dim3(kBlock), 0, st, D.nc, D.cp, D.einv, D.psr, D.psv, D.irange,
-                       D.cinst, D.ninst, D.c, D.pext);
+                       D.ninst, D.c, D.pext);
 }
 
 int deflation_sums_block(hipStream_t st, const Deflation &D, int nr, const double *r, const double *v,

@@ -14,6 +14,7 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <array>
 #include <vector>
 
 #include "../../include/hmfem_gpu.h"
@@ -203,10 +204,8 @@ struct CgBuffers {
 // setups reuse their memory (fresh vectors of a few MB cost their page faults
 // every time).
 struct DeflationHost {
-    std::vector<int> cell, cin, nbr, par, root, size, roots, cid, ptr, lcell, lcomp, lord, linst, irange, fill,
-        pbh, cph, pc;
-    std::vector<unsigned char> up;
-    std::vector<double> E, inv;
+    std::vector<std::array<int, 3>> cand;   // components: root, size, instance
+    std::vector<int> ptr, pbh, cph, irange, pc;
 };
 
 struct Deflation {
@@ -214,7 +213,7 @@ struct Deflation {
     int *cells = nullptr, *comp = nullptr;   // cell index and cluster of each listed cell
     int *pb = nullptr;                       // piece p covers listed cells [pb[p], pb[p+1])
     int *cp = nullptr;                       // cluster q owns pieces [cp[q], cp[q+1])
-    double *einv = nullptr;                  // (Z^T A Z)^-1, nc x nc (symmetric)
+    double *einv = nullptr;                  // 1 / E_aa (E = Z^T A Z is diagonal), nc
     double *c = nullptr;                     // per-cluster correction [cluster][rhs]
     double *psr = nullptr, *psv = nullptr;   // per-piece sums of r and v [piece][rhs]
     int ninst = 0;                           // grids of the setup (batch instances; 1 for a single grid)
