@@ -39,6 +39,7 @@ struct BlockScalars {
     double sigma[2][kMaxRhs], alpha[2][kMaxRhs];
     double rtol2;
     int done[kMaxRhs], status[kMaxRhs], iter[kMaxRhs];
+    int k0[kMaxRhs];   // the block iteration at which the lane's right-hand side started
     int all_done, kspmv, kupd, maxit, stop_rule, nr;
 };
 
@@ -115,6 +116,34 @@ k_b_init(OpArgs<double> o, const double *__restrict__ B, const double *__restric
         for (int q = 0; q < 2 * NR; q++) P[(long)2 * NR * lb + q] = acc[q];
 }
 
+// r = b - A x for lane `lane` only (a right-hand side put into a finished
+// lane), its partials (b.b, r.r) into the lane's slots of P[lb][2 NR].
+template <int NR>
+__global__ void __launch_bounds__(kBlock)
+k_b_init_lane(OpArgs<double> o, const double *__restrict__ B, const double *__restrict__ X, double *__restrict__ R,
+              int lane, int chunk, double *P)
+{
+    const int n = o.nx * o.ny * o.nz;
+    int lb, beg, end;
+    chunk_range(n, chunk, lb, beg, end);
+    double acc[2] = {0.0, 0.0};
+    for (int s = beg + threadIdx.x; s < end; s += kBlock) {
+        Nb<double> b = nb_at(o, s);
+        double ax = b.e * X[(long)s * NR + lane];
+        for (int f = 0; f < 6; f++) ax += b.w[f] * (X[(long)s * NR + lane] - X[(long)b.q[f] * NR + lane]);
+        const double bv = B[(long)s * NR + lane];
+        const double rv = bv - ax;
+        R[(long)s * NR + lane] = rv;
+        acc[0] += bv * bv;
+        acc[1] += rv * rv;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        P[(long)2 * NR * lb + lane] = acc[0];
+        P[(long)2 * NR * lb + NR + lane] = acc[1];
+    }
+}
+
 // Start of block CG iteration k: per right-hand side the stopping tests on
 // the partial sums of the previous kernels and beta; live[r] = 0 for a
 // finished one. Returns true once every right-hand side has finished. The dot
@@ -122,21 +151,25 @@ k_b_init(OpArgs<double> o, const double *__restrict__ B, const double *__restric
 template <int NR>
 __device__ __forceinline__ bool b_iteration_scalars(int G, int Gdot, BlockScalars *sc, const double *Pinit,
                                                     const double *Prr, const double *Pdot, int &k, double (&beta)[NR],
-                                                    int (&live)[NR])
+                                                    int (&live)[NR], int (&fresh)[NR])
 {
+    // a lane's own iteration count kr = k - k0: kr = 0 for a right-hand side
+    // started at this iteration (its b.b and r.r are in Pinit)
     k = sc->kspmv;
-    double dots[2 * NR], rrs[NR], bbs[NR];
+    double dots[2 * NR], rrs[NR], bbs[NR], ir[2 * NR], t[NR];
     sum_partials<2 * NR>(Pdot, Gdot, dots);
-    if (k == 0) {
-        double ir[2 * NR];
-        sum_partials<2 * NR>(Pinit, G, ir);
+    int any_fresh = 0;
 #pragma unroll
-        for (int r = 0; r < NR; r++) { bbs[r] = ir[r]; rrs[r] = ir[NR + r]; }
-    } else {
-        double t[NR];
-        sum_partials<NR>(Prr, G, t);
+    for (int r = 0; r < NR; r++) {
+        fresh[r] = !sc->done[r] && k == sc->k0[r];
+        any_fresh |= fresh[r];
+    }
+    if (any_fresh) sum_partials<2 * NR>(Pinit, G, ir);
+    if (k > 0) sum_partials<NR>(Prr, G, t);
 #pragma unroll
-        for (int r = 0; r < NR; r++) { rrs[r] = t[r]; bbs[r] = sc->bb[r]; }
+    for (int r = 0; r < NR; r++) {
+        bbs[r] = fresh[r] ? ir[r] : sc->bb[r];
+        rrs[r] = fresh[r] ? ir[NR + r] : (k > 0 ? t[r] : 0.0);
     }
     int all = 1;
     const bool writer = xcd_block() == 0 && threadIdx.x == 0;
@@ -145,10 +178,11 @@ __device__ __forceinline__ bool b_iteration_scalars(int G, int Gdot, BlockScalar
         beta[r] = 0.0;
         live[r] = 0;
         if (sc->done[r]) continue;
+        const int kr = k - sc->k0[r];
         const double sigma = fabs(dots[r]);
         int stop = 0, status = HMFEM_OK;
-        const double sf = k == 0 ? (sigma == 0.0 ? 1.0 : sigma) : sc->sigma_first[r];
-        if (k == 0) {
+        const double sf = kr == 0 ? (sigma == 0.0 ? 1.0 : sigma) : sc->sigma_first[r];
+        if (kr == 0) {
             if (sigma == 0.0 || (sc->stop_rule == HMFEM_STOP_RESIDUAL && (bbs[r] == 0.0 || rrs[r] <= sc->rtol2 * bbs[r])))
                 stop = 1;
         } else {
@@ -156,14 +190,14 @@ __device__ __forceinline__ bool b_iteration_scalars(int G, int Gdot, BlockScalar
             if (sc->stop_rule == HMFEM_STOP_RESIDUAL && rrs[r] <= sc->rtol2 * bbs[r]) stop = 1;
             if (sc->stop_rule == HMFEM_STOP_PRECONDITIONED && sigma <= sc->rtol2 * sf) stop = 1;
             if (!(sigma == sigma) || !(beta[r] == beta[r])) { stop = 1; status = HMFEM_ERR_BREAKDOWN; }
-            else if (!stop && k >= sc->maxit) { stop = 1; status = HMFEM_NOT_CONVERGED; }
+            else if (!stop && kr >= sc->maxit) { stop = 1; status = HMFEM_NOT_CONVERGED; }
         }
         if (writer) {
-            if (k == 0) { sc->bb[r] = bbs[r]; sc->sigma_first[r] = sf; }
+            if (kr == 0) { sc->bb[r] = bbs[r]; sc->sigma_first[r] = sf; }
             sc->rr[r] = rrs[r];
             sc->sigma[k & 1][r] = sigma;
             sc->beta[r] = beta[r];
-            if (stop) { sc->done[r] = 1; sc->status[r] = status; sc->iter[r] = k; }
+            if (stop) { sc->done[r] = 1; sc->status[r] = status; sc->iter[r] = kr; }
         }
         if (stop) beta[r] = 0.0;
         else { live[r] = 1; all = 0; }
@@ -187,8 +221,8 @@ k_b_spmv_z(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S
     if (sc->all_done) return;
     int k;
     double beta[NR];
-    int live[NR];
-    if (b_iteration_scalars<NR>(G, Gdot, sc, Pinit, Prr, Pdot, k, beta, live)) return;
+    int live[NR], fresh[NR];
+    if (b_iteration_scalars<NR>(G, Gdot, sc, Pinit, Prr, Pdot, k, beta, live, fresh)) return;
     const double *__restrict__ sin = (k & 1) ? S1 : S0;
     double *__restrict__ sout = (k & 1) ? S0 : S1;
     __shared__ double sh[3][kGsTy + 2][kGsTx + 2][NR];
@@ -223,11 +257,11 @@ k_b_spmv_z(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S
                 double a = 0.0, b = 0.0;
                 if (in && col && live[r]) {
                     const long q = (i + (long)j * nx + (long)kk * nxy) * NR + r;
-                    a = (double)Z[q] + (k ? beta[r] * sin[q] : 0.0);
+                    a = (double)Z[q] + (fresh[r] ? 0.0 : beta[r] * sin[q]);
                 }
                 if (in && ring_in && live[r]) {
                     const long q = (ri + (long)rj * nx + (long)kk * nxy) * NR + r;
-                    b = (double)Z[q] + (k ? beta[r] * sin[q] : 0.0);
+                    b = (double)Z[q] + (fresh[r] ? 0.0 : beta[r] * sin[q]);
                 }
                 sh[slot][ty + 1][tx + 1][r] = a;
                 if (has_ring) sh[slot][ry][rx][r] = b;
@@ -290,7 +324,7 @@ k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, 
         if (sc->done[r]) continue;
         const double tau = fabs(t[r]);
         if (tau == 0.0 || !(tau == tau)) {
-            if (writer) { sc->done[r] = 1; sc->status[r] = HMFEM_ERR_BREAKDOWN; sc->iter[r] = k; }
+            if (writer) { sc->done[r] = 1; sc->status[r] = HMFEM_ERR_BREAKDOWN; sc->iter[r] = k - sc->k0[r]; }
             continue;
         }
         alpha[r] = sc->sigma[k & 1][r] / tau;
@@ -329,6 +363,9 @@ k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, 
 // ---------------------------------------------------------------- V-cycle kernels
 
 constexpr int kBCz = 4;                          // cells of a tile column per thread
+// gs_sweeps 0: two sweeps (the 256^3 block workload, eight right-hand sides:
+// 277 block iterations and 9.17 solves/s with two, 308 and 7.91 with three)
+constexpr int kBlockSweeps = 2;
 constexpr int kBThreads = kGsTx * kGsTy * (kBTz / kBCz);
 
 // Block red-black Gauss-Seidel on tiles of kGsTx x kGsTy x kBTz cells for NR
@@ -337,8 +374,11 @@ constexpr int kBThreads = kGsTx * kGsTy * (kBTz / kBCz);
 // neighbour across their boundary is read from LDS. MODE 0: from zero;
 // MODE 1: from u + ocf * uc[aggregate] (uo must not alias u); MODE 2: MODE 1
 // plus the CG dot partials <z, f>_r, <z, v>_r of this block's tiles
-// (Pdot[lb][2 NR]).
-template <int NR, typename FT, int MODE, int CZ = kBCz, int NTHREADS = kGsTx * kGsTy * (kBTz / CZ)>
+// (Pdot[lb][2 NR]). The tile's coefficients are loaded once; the right-hand
+// sides are smoothed NP at a time (four at once held 169 VGPRs in MODE 2:
+// two waves per SIMD).
+template <int NR, typename FT, int MODE, int CZ = kBCz, int NTHREADS = kGsTx * kGsTy * (kBTz / CZ),
+          int NP = NR>
 __global__ void __launch_bounds__(NTHREADS)
 k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *__restrict__ u,
             float *__restrict__ uo, int sx, int sy, int sz, int cx, int cy, const float *__restrict__ uc, float ocf,
@@ -348,8 +388,9 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
     constexpr int RING = kGsRing;
     constexpr int RPER = (TZ * RING + NTHREADS - 1) / NTHREADS;
     static_assert(CZ % 2 == 0, "colour pairs");
+    static_assert(NR % NP == 0, "whole passes");
     if (*done) return;
-    __shared__ float sh[TZ][kGsTy + 2][kGsTx + 2][NR];
+    __shared__ float sh[TZ][kGsTy + 2][kGsTx + 2][NP];
     const int nx = o.nx, ny = o.ny, nz = o.nz;
     const long nxy = (long)nx * ny;
     const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy, ntz = (nz + TZ - 1) / TZ;
@@ -362,14 +403,17 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
     if constexpr (MODE == 2)
         for (int q = 0; q < 2 * NR; q++) acc[q] = 0.0;
 
-    auto start = [&](int ii, int jj, int kk, float (&val)[NR]) {
+    // right-hand sides rb .. rb + NP - 1 of the cell's starting value (32-bit
+    // offsets: the host keeps n * NR below 2^32, so that the loads take one
+    // address register each)
+    auto start = [&](int ii, int jj, int kk, int rb, float (&val)[NP]) {
 #pragma unroll
-        for (int r = 0; r < NR; r++) val[r] = 0.f;
+        for (int r = 0; r < NP; r++) val[r] = 0.f;
         if (MODE == 0 || ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) return;
-        const long s = ii + (long)jj * nx + (long)kk * nxy;
-        const long I = (ii >> sx) + (long)cx * ((jj >> sy) + (long)cy * (kk >> sz));
+        const unsigned s = (unsigned)ii + (unsigned)jj * (unsigned)nx + (unsigned)kk * (unsigned)nxy;
+        const unsigned I = (unsigned)(ii >> sx) + (unsigned)cx * ((unsigned)(jj >> sy) + (unsigned)cy * (unsigned)(kk >> sz));
 #pragma unroll
-        for (int r = 0; r < NR; r++) val[r] = u[s * NR + r] + ocf * uc[I * NR + r];
+        for (int r = 0; r < NP; r++) val[r] = u[s * NR + rb + r] + ocf * uc[I * NR + rb + r];
     };
 
     for (int t = lb; t < ntiles; t += gridDim.x) {
@@ -380,7 +424,7 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
         const int kc = k0 + kbase;   // grid plane of this thread's first cell
         const long s0 = i + (long)j * nx + (long)kc * nxy;
 
-        float wxp[CZ], wxm[CZ], wyp[CZ], wym[CZ], wzp[CZ], dinv[CZ], ff[CZ][NR], uu[CZ][NR];
+        float wxp[CZ], wxm[CZ], wyp[CZ], wym[CZ], wzp[CZ], dinv[CZ];
 #pragma unroll
         for (int kk = 0; kk < CZ; kk++) {
             const bool act = col && kc + kk < nz;
@@ -391,80 +435,94 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
             wym[kk] = act && j > 0 ? o.wy[s - nx] : 0.f;
             wzp[kk] = act ? o.wz[s] : 0.f;
             dinv[kk] = act ? o.di[s] : 0.f;
-#pragma unroll
-            for (int r = 0; r < NR; r++) ff[kk][r] = act ? (float)f[s * NR + r] : 0.f;
         }
         const float wzlo = col && kc > 0 && kc < nz ? o.wz[s0 - nxy] : 0.f;
-        float zlo[NR], zhi[NR], ring[RPER][NR];
-#pragma unroll
-        for (int kk = 0; kk < CZ; kk++) start(i, j, kc + kk, uu[kk]);
-        if (tz == 0) start(i, j, k0 - 1, zlo);
-        if (tz == NTZ - 1) start(i, j, k0 + TZ, zhi);
-#pragma unroll
-        for (int m = 0; m < RPER; m++) {
-            const int q = threadIdx.x + m * NTHREADS;
-            int hx = 0, hy = 0, hz = 0;
-            if (q < TZ * RING) gs_ring_cell(q, hx, hy, hz);
-            if (q < TZ * RING) start(i0 + hx - 1, j0 + hy - 1, k0 + hz, ring[m]);
-#pragma unroll
-            for (int r = 0; r < NR; r++)
-                if (q < TZ * RING) sh[hz][hy][hx][r] = ring[m][r];
-        }
-#pragma unroll
-        for (int kk = 0; kk < CZ; kk++)
-#pragma unroll
-            for (int r = 0; r < NR; r++) sh[kbase + kk][ty + 1][tx + 1][r] = uu[kk][r];
-        __syncthreads();
 
-        for (int sw = 0; sw < sweeps; sw++) {
+#pragma unroll 1
+        for (int rb = 0; rb < NR; rb += NP) {
+            float ff[CZ][NP], uu[CZ][NP];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int c = MODE == 0 ? h : 1 - h;
-                const int sel = (c + i + j + kc) & 1;
+            for (int kk = 0; kk < CZ; kk++) {
+                const bool act = col && kc + kk < nz;
+                const unsigned s = (unsigned)(s0 + kk * nxy) * NR + rb;
 #pragma unroll
-                for (int m = 0; m < CZ / 2; m++) {
-                    const int ka = 2 * m, kb = 2 * m + 1;
-                    const int kk = kbase + ka + sel;   // tile plane of the cell updated
-                    const float cxp = sel ? wxp[kb] : wxp[ka], cxm = sel ? wxm[kb] : wxm[ka];
-                    const float cyp = sel ? wyp[kb] : wyp[ka], cym = sel ? wym[kb] : wym[ka];
-                    const float czp = sel ? wzp[kb] : wzp[ka];
-                    const float czm = sel ? wzp[ka] : (m > 0 ? wzp[ka - 1] : wzlo);
-                    const float di_ = sel ? dinv[kb] : dinv[ka];
-#pragma unroll
-                    for (int r = 0; r < NR; r++) {
-                        float below, above;
-                        if (m > 0) below = uu[ka - 1][r];
-                        else below = tz == 0 ? zlo[r] : sh[kbase - 1][ty + 1][tx + 1][r];
-                        if (kb + 1 < CZ) above = uu[kb + 1][r];
-                        else above = tz == NTZ - 1 ? zhi[r] : sh[kbase + CZ][ty + 1][tx + 1][r];
-                        const float uzm = sel ? uu[ka][r] : below;
-                        const float uzp = sel ? above : uu[kb][r];
-                        const float a = (sel ? ff[kb][r] : ff[ka][r]) + cxp * sh[kk][ty + 1][tx + 2][r] +
-                                        cxm * sh[kk][ty + 1][tx][r] + cyp * sh[kk][ty + 2][tx + 1][r] +
-                                        cym * sh[kk][ty][tx + 1][r] + czp * uzp + czm * uzm;
-                        const float un = a * di_;
-                        uu[ka][r] = sel ? uu[ka][r] : un;
-                        uu[kb][r] = sel ? un : uu[kb][r];
-                        sh[kk][ty + 1][tx + 1][r] = un;
-                    }
-                }
-                __syncthreads();
+                for (int r = 0; r < NP; r++) ff[kk][r] = act ? (float)f[s + r] : 0.f;
             }
-        }
+            float zlo[NP], zhi[NP], ring[RPER][NP];
+#pragma unroll
+            for (int kk = 0; kk < CZ; kk++) start(i, j, kc + kk, rb, uu[kk]);
+            if (tz == 0) start(i, j, k0 - 1, rb, zlo);
+            if (tz == NTZ - 1) start(i, j, k0 + TZ, rb, zhi);
+#pragma unroll
+            for (int m = 0; m < RPER; m++) {
+                const int q = threadIdx.x + m * NTHREADS;
+                int hx = 0, hy = 0, hz = 0;
+                if (q < TZ * RING) gs_ring_cell(q, hx, hy, hz);
+                if (q < TZ * RING) start(i0 + hx - 1, j0 + hy - 1, k0 + hz, rb, ring[m]);
+#pragma unroll
+                for (int r = 0; r < NP; r++)
+                    if (q < TZ * RING) sh[hz][hy][hx][r] = ring[m][r];
+            }
+#pragma unroll
+            for (int kk = 0; kk < CZ; kk++)
+#pragma unroll
+                for (int r = 0; r < NP; r++) sh[kbase + kk][ty + 1][tx + 1][r] = uu[kk][r];
+            __syncthreads();
 
+            for (int sw = 0; sw < sweeps; sw++) {
 #pragma unroll
-        for (int kk = 0; kk < CZ; kk++) {
-            if (col && kc + kk < nz) {
-                const long s = s0 + kk * nxy;
+                for (int h = 0; h < 2; h++) {
+                    const int c = MODE == 0 ? h : 1 - h;
+                    const int sel = (c + i + j + kc) & 1;
 #pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    uo[s * NR + r] = uu[kk][r];
-                    if constexpr (MODE == 2) {
-                        acc[r] += (double)uu[kk][r] * (double)f[s * NR + r];
-                        if (v) acc[NR + r] += (double)uu[kk][r] * v[s * NR + r];
+                    for (int m = 0; m < CZ / 2; m++) {
+                        const int ka = 2 * m, kb = 2 * m + 1;
+                        const int kk = kbase + ka + sel;   // tile plane of the cell updated
+                        const float cxp = sel ? wxp[kb] : wxp[ka], cxm = sel ? wxm[kb] : wxm[ka];
+                        const float cyp = sel ? wyp[kb] : wyp[ka], cym = sel ? wym[kb] : wym[ka];
+                        const float czp = sel ? wzp[kb] : wzp[ka];
+                        const float czm = sel ? wzp[ka] : (m > 0 ? wzp[ka - 1] : wzlo);
+                        const float di_ = sel ? dinv[kb] : dinv[ka];
+#pragma unroll
+                        for (int r = 0; r < NP; r++) {
+                            float below, above;
+                            if (m > 0) below = uu[ka - 1][r];
+                            else below = tz == 0 ? zlo[r] : sh[kbase - 1][ty + 1][tx + 1][r];
+                            if (kb + 1 < CZ) above = uu[kb + 1][r];
+                            else above = tz == NTZ - 1 ? zhi[r] : sh[kbase + CZ][ty + 1][tx + 1][r];
+                            const float uzm = sel ? uu[ka][r] : below;
+                            const float uzp = sel ? above : uu[kb][r];
+                            const float a = (sel ? ff[kb][r] : ff[ka][r]) + cxp * sh[kk][ty + 1][tx + 2][r] +
+                                            cxm * sh[kk][ty + 1][tx][r] + cyp * sh[kk][ty + 2][tx + 1][r] +
+                                            cym * sh[kk][ty][tx + 1][r] + czp * uzp + czm * uzm;
+                            const float un = a * di_;
+                            uu[ka][r] = sel ? uu[ka][r] : un;
+                            uu[kb][r] = sel ? un : uu[kb][r];
+                            sh[kk][ty + 1][tx + 1][r] = un;
+                        }
                     }
+                    __syncthreads();
                 }
             }
+
+#pragma unroll
+            for (int kk = 0; kk < CZ; kk++) {
+                if (col && kc + kk < nz) {
+                    const unsigned s = (unsigned)(s0 + kk * nxy) * NR + rb;
+#pragma unroll
+                    for (int r = 0; r < NP; r++) {
+                        uo[s + r] = uu[kk][r];
+                        if constexpr (MODE == 2) {
+                            // <z, r> from the FP32 copy of r this pass already holds: the
+                            // FP64 r of the tile is gone from L2 by now, and z itself is FP32
+                            acc[rb + r] += (double)uu[kk][r] * (double)ff[kk][r];
+                            if (v) acc[NR + rb + r] += (double)uu[kk][r] * v[s + r];
+                        }
+                    }
+                }
+            }
+            // the last half-sweep's barrier is the last LDS read of this pass:
+            // the next pass (or tile) may fill the LDS at once
         }
     }
     if constexpr (MODE == 2) {
@@ -700,7 +758,7 @@ static int b_vcycle_coarse(hipStream_t st, Hierarchy &H, BlockState &S, int l, c
     }
     Level &C = H.lev[l + 1];
     const int GT = grid_for(b_tiles(L.d), 1);
-    const int sw = opt.gs_sweeps < 1 ? 3 : opt.gs_sweeps;   // 0: three sweeps
+    const int sw = opt.gs_sweeps < 1 ? kBlockSweeps : opt.gs_sweeps;
     hipLaunchKernelGGL((k_b_gs_tile<NR, float, 0>), dim3(GT), dim3(kBThreads), 0, st, opargs<float>(L), sw,
                        (const float *)X.f, (const float *)nullptr, X.u, 0, 0, 0, 0, 0, (const float *)nullptr, 0.f,
                        (const double *)nullptr, 1, (double *)nullptr, S.done);
@@ -722,7 +780,7 @@ static int b_vcycle_fine(hipStream_t st, Hierarchy &H, BlockState &S, const doub
     Level &L = H.lev[0], &C = H.lev[1];
     BlockLevelVecs &X = S.lv[0];
     const int nt = b_tiles(L.d);
-    const int sw = opt.gs_sweeps < 1 ? 3 : opt.gs_sweeps;   // 0: three sweeps
+    const int sw = opt.gs_sweeps < 1 ? kBlockSweeps : opt.gs_sweeps;
     const bool defl = S.D && S.D->nc > 0;
     if (defl) {   // per-cluster sums of r and v, the correction and its dot partials (row G of Pdot)
         int rc = deflation_sums_block(st, *S.D, NR, S.R, Vp, S.Pdot + (long)2 * NR * S.ch.G, S.done);
@@ -777,7 +835,7 @@ size_t block_bytes(Dims fine, int nr, const hmfem_options &opt)
 
 template <int NR>
 static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d, BlockState &S,
-                    const hmfem_options &opt, BlockScalars *h_sc)
+                    const hmfem_options &opt, BlockScalars *h_sc, const double *Bk, BlockFeed *feed, int *kdone)
 {
     const int n = (int)d.n();
     OpArgs<double> o;
@@ -796,6 +854,13 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
     HMFEM_CHECK(hipGetLastError());
     int k = 0;
     const int every = opt.check_every < 1 ? 1 : opt.check_every;
+    // lanes refilled from the feed: a finished lane hands its result over and
+    // takes the next right-hand side (its own k0, its residual from
+    // k_b_init_lane), and the V-cycle is run again for every lane so that the
+    // new one has z and <z, r> (the others get the same values again)
+    bool busy[kMaxRhs] = {};
+    for (int r = 0; r < NR; r++) busy[r] = feed != nullptr;
+    int kcap = opt.max_iterations + every;
     while (true) {
         for (int q = 0; q < every; q++, k++) {
             hipLaunchKernelGGL(k_b_spmv_z<NR>, dim3(Gz), dim3(kBlock), 0, st, o, (const float *)S.Z, S.S0, S.S1,
@@ -808,8 +873,36 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
         HMFEM_CHECK(hipGetLastError());
         HMFEM_CHECK(hipMemcpyAsync(h_sc, S.sc, sizeof(BlockScalars), hipMemcpyDeviceToHost, st));
         HMFEM_CHECK(hipStreamSynchronize(st));
-        if (h_sc->all_done || k > opt.max_iterations + every) break;
+        if (feed) {
+            bool refilled = false;
+            for (int r = 0; r < NR; r++) {
+                if (!busy[r] || !h_sc->done[r]) continue;
+                const double rel = h_sc->bb[r] > 0 ? sqrt(h_sc->rr[r] / h_sc->bb[r]) : 0.0;
+                feed->done(r, h_sc->iter[r], rel, h_sc->status[r]);
+                busy[r] = false;
+                if (!feed->load(r)) continue;
+                hipLaunchKernelGGL(k_b_init_lane<NR>, dim3(G), dim3(kBlock), 0, st, o, Bk, (const double *)S.X, S.R,
+                                   r, S.ch.chunk, S.Pinit);
+                h_sc->done[r] = 0;
+                h_sc->status[r] = HMFEM_OK;
+                h_sc->iter[r] = 0;
+                h_sc->k0[r] = h_sc->kspmv;
+                busy[r] = true;
+                refilled = true;
+            }
+            if (refilled) {
+                h_sc->all_done = 0;
+                kcap = h_sc->kspmv + opt.max_iterations + every;
+                HMFEM_CHECK(hipMemcpyAsync(S.sc, h_sc, sizeof(BlockScalars), hipMemcpyHostToDevice, st));
+                rc = b_vcycle_fine<NR>(st, H, S, S.V, opt);
+                if (rc) return rc;
+                HMFEM_CHECK(hipGetLastError());
+                continue;
+            }
+        }
+        if (h_sc->all_done || k > kcap) break;
     }
+    if (kdone) *kdone = h_sc->kspmv;
     return HMFEM_OK;
 }
 
@@ -818,7 +911,8 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
 int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const double *Bk, double *Xk,
                 const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *actx, long lds_limit,
                 std::vector<char> &stage, int *iterations, double *rel_residual, int *status, double *setup_ms,
-                double *solve_ms, Deflation *defl, bool defl_setup, char *(*dalloc)(void *, size_t, int), void *dctx)
+                double *solve_ms, Deflation *defl, bool defl_setup, char *(*dalloc)(void *, size_t, int), void *dctx,
+                BlockFeed *feed, int *block_iterations)
 {
     if (nr != 2 && nr != 4) return HMFEM_ERR_ARG;
     hmfem_options opt = opt_in;
@@ -894,11 +988,11 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
     switch (nr) {
     case 2:
         hipLaunchKernelGGL(k_b_init<2>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
-        rc = block_cg<2>(st, H, A, d, S, opt, &hs);
+        rc = block_cg<2>(st, H, A, d, S, opt, &hs, Bk, feed, block_iterations);
         break;
     default:
         hipLaunchKernelGGL(k_b_init<4>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
-        rc = block_cg<4>(st, H, A, d, S, opt, &hs);
+        rc = block_cg<4>(st, H, A, d, S, opt, &hs, Bk, feed, block_iterations);
         break;
     }
     if (rc) return rc;

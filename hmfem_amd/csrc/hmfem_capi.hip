@@ -154,6 +154,7 @@ extern "C" int hmfem_create(hmfem_context **out)
     HMFEM_CHECK(hipHostMalloc((void **)&c->h_sc, sizeof(CgScalars), hipHostMallocDefault));
     HMFEM_CHECK(hipHostMalloc((void **)&c->defl.h_hist, Deflation::kHistBins * sizeof(unsigned), hipHostMallocDefault));
     HMFEM_CHECK(hipEventCreateWithFlags(&c->defl.hist_ready, hipEventDisableTiming));
+    HMFEM_CHECK(hipEventCreateWithFlags(&c->defl.stage_done, hipEventDisableTiming));
     c->scratch = scratch_for_current_device(&c->scratch_rc);
     int dev = 0, lds = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -175,6 +176,7 @@ extern "C" void hmfem_destroy(hmfem_context *c)
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->defl.h_hist) (void)hipHostFree(c->defl.h_hist);
     if (c->defl.hist_ready) (void)hipEventDestroy(c->defl.hist_ready);
+    if (c->defl.stage_done) (void)hipEventDestroy(c->defl.stage_done);
     if (c->defl.h_stage) (void)hipHostFree(c->defl.h_stage);
     (void)hipStreamDestroy(c->st);
     if (c == g_default) g_default = nullptr;
@@ -434,6 +436,7 @@ extern "C" int hmfem_reservoir_device(hmfem_context *c, const hmfem_grid *g, con
     A.wz = (double *)c->get("wz", n * 8);
     double *b = (double *)c->get("b", n * 8);
     if (!A.e || !A.wx || !A.wy || !A.wz || !b) return HMFEM_ERR_NOMEM;
+    A.eface = !KI && !WI;   // without well or conduit terms the diagonal excess lives on the faces only
     hmfem_status tmp{};
     hmfem_status *out = st ? st : &tmp;
     HMFEM_CHECK(hipEventRecord(c->ev[4], c->st));
@@ -729,6 +732,7 @@ extern "C" int hmfem_reservoir_block_device(hmfem_context *c, const hmfem_grid *
     const long n = d.n();
     // groups of at most four right-hand sides (two-wide for one or two)
     const int nr = nrhs <= 2 ? 2 : 4;
+    if ((double)n * nr >= 4294967296.0) return HMFEM_ERR_ARG;   // the block smoother's 32-bit offsets
     Stencil<double> A;
     A.e = (double *)c->get("e", n * 8);
     A.wx = (double *)c->get("wx", n * 8);
@@ -744,37 +748,60 @@ extern "C" int hmfem_reservoir_block_device(hmfem_context *c, const hmfem_grid *
     int it[8] = {0}, stat[8] = {0};
     double rel[8] = {0};
     double t_asm = 0, setup_ms = 0, solve_ms = 0;
-    for (int g0 = 0; g0 < nrhs; g0 += nr) {
-        HMFEM_CHECK(hipEventRecord(c->ev[4], c->st));
-        for (int r = 0; r < nr; r++) {
-            const int R = g0 + r;
-            if (R < nrhs) {
-                const bool first = g0 == 0 && r == 0;
-                int rc = launch_assemble(c->st, g, Kx, Ky, Kz, nullptr, nullptr, nullptr, nullptr, bval + 6 * R,
-                                         first ? A.e : nullptr, first ? A.wx : nullptr, first ? A.wy : nullptr,
-                                         first ? A.wz : nullptr, b, nullptr, nullptr);
-                if (rc) return rc;
-                block_put(c->st, n, nr, r, b, q ? q[R] : nullptr, Bk);
-                block_put(c->st, n, nr, r, p[R], nullptr, Xk);
-            } else {
-                block_put(c->st, n, nr, r, nullptr, nullptr, Bk);
-                block_put(c->st, n, nr, r, nullptr, nullptr, Xk);
-            }
-        }
-        HMFEM_CHECK(hipGetLastError());
-        HMFEM_CHECK(hipEventRecord(c->ev[5], c->st));
-        double su = 0, so = 0;
-        int rc = block_solve(c->st, d, A, nr, Bk, Xk, opt, ctx_alloc_block, c, c->lds_limit, c->stage, it + g0,
-                             rel + g0, stat + g0, &su, &so, &c->defl, g0 == 0, ctx_alloc_defl, c);
-        if (rc) return rc;
-        for (int r = 0; r < nr && g0 + r < nrhs; r++) block_get(c->st, n, nr, r, Xk, p[g0 + r]);
+    // the right-hand sides go through nr lanes: the first nr at the start,
+    // each further one into the first lane that finishes (one hierarchy, no
+    // lane waits for a slower one while work is left)
+    int lane_rhs[4] = {-1, -1, -1, -1};
+    int next = 0;
+    int rc_feed = HMFEM_OK;
+    auto load = [&](int lane) -> bool {
+        if (next >= nrhs || rc_feed) return false;
+        const int R = next++;
+        const bool first = R == 0;
+        int rc = launch_assemble(c->st, g, Kx, Ky, Kz, nullptr, nullptr, nullptr, nullptr, bval + 6 * R,
+                                 first ? A.e : nullptr, first ? A.wx : nullptr, first ? A.wy : nullptr,
+                                 first ? A.wz : nullptr, b, nullptr, nullptr);
+        if (rc) { rc_feed = rc; return false; }
+        block_put(c->st, n, nr, lane, b, q ? q[R] : nullptr, Bk);
+        block_put(c->st, n, nr, lane, p[R], nullptr, Xk);
+        lane_rhs[lane] = R;
+        return true;
+    };
+    auto done = [&](int lane, int its, double rr, int status) {
+        const int R = lane_rhs[lane];
+        if (R < 0) return;
+        it[R] = its;
+        rel[R] = rr;
+        stat[R] = status;
+        block_get(c->st, n, nr, lane, Xk, p[R]);
+        lane_rhs[lane] = -1;
+    };
+    HMFEM_CHECK(hipEventRecord(c->ev[4], c->st));
+    for (int r = 0; r < nr; r++) {
+        if (load(r)) continue;
+        if (rc_feed) return rc_feed;
+        block_put(c->st, n, nr, r, nullptr, nullptr, Bk);   // an idle lane: b = 0 stops at once
+        block_put(c->st, n, nr, r, nullptr, nullptr, Xk);
+    }
+    HMFEM_CHECK(hipGetLastError());
+    HMFEM_CHECK(hipEventRecord(c->ev[5], c->st));
+    BlockFeed feed{done, load};
+    int lit[4] = {0}, lstat[4] = {0}, kblock = 0;
+    double lrel[4] = {0};
+    int rc = block_solve(c->st, d, A, nr, Bk, Xk, opt, ctx_alloc_block, c, c->lds_limit, c->stage, lit, lrel, lstat,
+                         &setup_ms, &solve_ms, &c->defl, true, ctx_alloc_defl, c, &feed, &kblock);
+    if (rc) return rc;
+    if (rc_feed) return rc_feed;
+    for (int r = 0; r < nr; r++)   // lanes that had not finished when the solve stopped
+        if (lane_rhs[r] >= 0) done(r, lit[r], lrel[r], lstat[r] == HMFEM_OK ? HMFEM_NOT_CONVERGED : lstat[r]);
+    if (next < nrhs) return HMFEM_NOT_CONVERGED;   // cannot happen: every lane finishes or is capped
+    {
         float ta = 0.f;
         HMFEM_CHECK(hipEventSynchronize(c->ev[5]));
         (void)hipEventElapsedTime(&ta, c->ev[4], c->ev[5]);
-        t_asm += ta;
-        setup_ms += su;
-        solve_ms += so;
+        t_asm = ta;
     }
+    out->block_iterations = kblock;
     HMFEM_CHECK(hipGetLastError());
     out->t_assemble_ms = t_asm;
     out->t_setup_ms = setup_ms;

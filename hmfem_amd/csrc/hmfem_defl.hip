@@ -600,6 +600,10 @@ k_defl_einv(int nc, const int *__restrict__ cp, const double *__restrict__ pe, d
 // pinned host staging of at least `bytes` (grown on demand, freed with the context)
 static int stage_reserve(Deflation &D, size_t bytes)
 {
+    // the host writes the staging (or frees it) only after the copies the
+    // last setup queued from it
+    if (D.stage_busy && D.stage_done && hipEventSynchronize(D.stage_done) != hipSuccess) return 1;
+    D.stage_busy = false;
     if (D.h_stage_bytes >= bytes && D.h_stage) return 0;
     if (D.h_stage) (void)hipHostFree(D.h_stage);
     D.h_stage = nullptr;
@@ -771,8 +775,12 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     hipLaunchKernelGGL(k_defl_einv, dim3(1), dim3(kBlock), 0, st, nc, (const int *)D.cp, (const double *)(share + m),
                        D.einv);
     HMFEM_CHECK(hipGetLastError());
-    // the staging is rewritten by the next setup: these copies must be done
-    HMFEM_CHECK(hipStreamSynchronize(st));
+    if (D.stage_done) {
+        HMFEM_CHECK(hipEventRecord(D.stage_done, st));
+        D.stage_busy = true;
+    } else {
+        HMFEM_CHECK(hipStreamSynchronize(st));
+    }
     D.nc = nc;
     D.m = m;
     D.np = np;

@@ -15,6 +15,7 @@
 #include <stdio.h>
 
 #include <array>
+#include <functional>
 #include <vector>
 
 #include "../../include/hmfem_gpu.h"
@@ -48,6 +49,7 @@ template <typename T>
 struct Stencil {
     T *e = nullptr, *wx = nullptr, *wy = nullptr, *wz = nullptr;
     T *di = nullptr;   // inverse diagonal (smoothers); computed at setup
+    long eface = 0;    // 1: e is zero except on cells of the grid's six faces (no well or conduit terms)
 };
 
 struct Level {
@@ -224,8 +226,10 @@ struct Deflation {
     static constexpr int kMaxRhs = 8;        // interleaved right-hand sides of the per-cluster buffers
     unsigned *h_hist = nullptr;              // pinned host copy of the diagonal histogram (context-owned)
     hipEvent_t hist_ready = nullptr;         // recorded after that copy (context-owned)
-    void *h_stage = nullptr;                 // pinned staging of the cell list and coefficients
+    void *h_stage = nullptr;                 // pinned staging of the component table and cluster arrays
     size_t h_stage_bytes = 0;
+    hipEvent_t stage_done = nullptr;         // recorded after the last copy out of the staging (context-owned)
+    bool stage_busy = false;                 // such a copy may still be queued
     bool pending = false;                    // deflation_begin queued a histogram
     unsigned gen = 0;                        // bumped by every setup (single, block or batch)
     DeflationHost h;
@@ -288,12 +292,22 @@ int batch_grids(hipStream_t st, int ninst, const int *dims, const double *length
                 std::vector<char> &stage, void *d_grids, int *d_bmap, int *nblk_out);
 size_t batch_grid_bytes(int ninst);
 
+// A block solve's lanes (right-hand-side slots) refilled from a queue: when
+// lane r has finished, done(r, iterations, rel_residual, status) takes its
+// result out of X[.][r], then load(r) puts the next right-hand side into
+// B[.][r] and X[.][r] (false: none left, the lane stays idle).
+struct BlockFeed {
+    std::function<void(int, int, double, int)> done;
+    std::function<bool(int)> load;
+};
+
 // block solve of several right-hand sides (hmfem_block.hip): B, X interleaved [n][nr]
 int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const double *Bk, double *Xk,
                 const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *actx, long lds_limit,
                 std::vector<char> &stage, int *iterations, double *rel_residual, int *status, double *setup_ms,
                 double *solve_ms, Deflation *defl = nullptr, bool defl_setup = false,
-                char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr);
+                char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr, BlockFeed *feed = nullptr,
+                int *block_iterations = nullptr);
 size_t block_bytes(Dims fine, int nr, const hmfem_options &opt);
 void block_put(hipStream_t st, long n, int nr, int r, const double *a, const double *add, double *Bk);
 void block_get(hipStream_t st, long n, int nr, int r, const double *Bk, double *a);

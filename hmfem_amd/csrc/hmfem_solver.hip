@@ -138,7 +138,7 @@ k_resid_restrict(OpArgs<T> o, int fx, int fy, int fz, int cx, int cy, int cz, co
 template <typename T, typename FT>
 __global__ void __launch_bounds__(kBlock)
 k_restrict_z(OpArgs<T> o, int zlen, int cx, int cy, int cz, const FT *__restrict__ f, const T *__restrict__ u,
-             T *__restrict__ fc, const int *done)
+             T *__restrict__ fc, const int *done, int eface)
 {
     if (*done) return;
     __shared__ T sh[2][kGsTy + 2][kGsTx + 2];
@@ -163,6 +163,8 @@ k_restrict_z(OpArgs<T> o, int zlen, int cx, int cy, int cz, const FT *__restrict
     const int ri = i0 + rx - 1, rj = j0 + ry - 1;
     const bool ring_in = has_ring && ri >= 0 && rj >= 0 && ri < nx && rj < ny;
     const long cs = i + (long)j * nx, rs = ri + (long)rj * nx;
+    // with eface, e is read only on the grid's faces (zero elsewhere)
+    const bool eall = !eface || i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
     T um = (T)0, uc = (T)0, up = (T)0, wzm = (T)0;
     if (col) {
         if (kb > 0) { um = u[cs + (kb - 1) * nxy]; wzm = o.wz[cs + (kb - 1) * nxy]; }
@@ -177,7 +179,7 @@ k_restrict_z(OpArgs<T> o, int zlen, int cx, int cy, int cz, const FT *__restrict
         T r = (T)0, e = (T)0, xp = (T)0, xm = (T)0, yp = (T)0, ym = (T)0, zp = (T)0, fq = (T)0, un = (T)0;
         if (col) {
             const long q = cs + kk * nxy;
-            e = o.e[q];
+            if (eall || kk == 0 || kk == nz - 1) e = o.e[q];
             if (i < nx - 1) xp = o.wx[q];
             if (i > 0) xm = o.wx[q - 1];
             if (j < ny - 1) yp = o.wy[q];
@@ -1070,11 +1072,15 @@ k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, d
 // k_cg_update leaves to it (s_old is read here anyway): that saves reading s
 // again and a read-modify-write pass over x per iteration. If the solve stops
 // at this kernel, k_cg_finish_x applies the pending update instead.
+#ifndef HMFEM_SPMV_SLOTS
+#define HMFEM_SPMV_SLOTS 4
+#endif
+constexpr int kSpmvSlots = HMFEM_SPMV_SLOTS;
 template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
 k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
             double *__restrict__ v, double *__restrict__ x, int zlen, int G, int Gdot, CgScalars *sc,
-            const double *Pinit, const double *Prr, const double *Pdot, double *Ptau)
+            const double *Pinit, const double *Prr, const double *Pdot, double *Ptau, int eface)
 {
     if (sc->done) return;
     int k;
@@ -1084,7 +1090,7 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
     const double *__restrict__ sin = (k & 1) ? s1 : s0;
     double *__restrict__ sout = (k & 1) ? s0 : s1;
     const double alpha_prev = k ? sc->alpha[(k - 1) & 1] : 0.0;
-    __shared__ double sh[3][kGsTy + 2][kGsTx + 2];
+    __shared__ double sh[kSpmvSlots][kGsTy + 2][kGsTx + 2];
     const int nx = o.nx, ny = o.ny, nz = o.nz;
     const long nxy = (long)nx * ny;
     const int ntx = (nx + kGsTx - 1) / kGsTx, nty = (ny + kGsTy - 1) / kGsTy;
@@ -1131,7 +1137,7 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
             return r;
         };
         auto store_raw = [&](int kk, const Raw &r) {
-            const int slot = (kk + 3) % 3;
+            const int slot = (kk + kSpmvSlots) % kSpmvSlots;
             sh[slot][ty + 1][tx + 1] = r.zo + beta * r.so;   // beta = 0 and s_old unread at k = 0
             if (has_ring) sh[slot][ry][rx] = r.zr + beta * r.sr;
             if (col && k && kk >= kb && kk < ke) x[i + (long)j * nx + (long)kk * nxy] = r.xo + alpha_prev * r.so;
@@ -1139,11 +1145,13 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
         struct Coef {
             double e, xp, xm, yp, ym, zp;
         };
+        // with eface, e is read only on the grid's faces (zero elsewhere)
+        const bool eall = !eface || i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
         auto load_coef = [&](int kk) -> Coef {
             Coef c{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
             if (col && kk < nz) {
                 const long q = i + (long)j * nx + (long)kk * nxy;
-                c.e = o.e[q];
+                if (eall || kk == 0 || kk == nz - 1) c.e = o.e[q];
                 c.xp = i < nx - 1 ? o.wx[q] : 0.0;
                 c.xm = i > 0 ? o.wx[q - 1] : 0.0;
                 c.yp = j < ny - 1 ? o.wy[q] : 0.0;
@@ -1162,7 +1170,8 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
             nxt = load_raw(kk + 2);
             const Coef cn = load_coef(kk + 1);
             __syncthreads();
-            const int c = (kk + 3) % 3, lo = (kk + 2) % 3, hi = (kk + 1) % 3;
+            const int c = (kk + kSpmvSlots) % kSpmvSlots, lo = (kk + kSpmvSlots - 1) % kSpmvSlots,
+                      hi = (kk + 1) % kSpmvSlots;
             if (col) {
                 const long q = i + (long)j * nx + (long)kk * nxy;
                 const double us = sh[c][ty + 1][tx + 1];
@@ -1179,7 +1188,7 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
                 acc[0] += us * a;
             }
             cf = cn;
-            __syncthreads();
+            if (kSpmvSlots == 3) __syncthreads();   // four slots: the next store goes to a plane nobody reads
         }
     }
     block_sum<1>(acc);
@@ -1195,7 +1204,10 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
 // (loading x at the store instead, 82 VGPRs: 35.2 / 275 us; beta and
 // alpha moved to scalar registers and z kept in FP32 until the LDS store,
 // 78 VGPRs and 6 waves: 33.9 / 282 us, so more waves are slower here).
-constexpr int kMaxSpmvBlocks = 1024;
+#ifndef HMFEM_SPMV_BLOCKS
+#define HMFEM_SPMV_BLOCKS 1024
+#endif
+constexpr int kMaxSpmvBlocks = HMFEM_SPMV_BLOCKS;
 
 // Planes per block of k_cg_spmv_z so that it needs at most maxb blocks (at
 // least 4 planes each); 0 if a single plane has more tiles than that.
@@ -1469,6 +1481,8 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const 
             L.s64.e = ar.take<double>(n); L.s64.wx = ar.take<double>(n);
             L.s64.wy = ar.take<double>(n); L.s64.wz = ar.take<double>(n);
         }
+        // a box sum of cells off the faces is off the faces: every level keeps eface
+        L.s64.eface = L.s32.eface = s0.eface;
         if (prec == HMFEM_PREC_FP64) L.s64.di = ar.take<double>(n);
         L.s32.e = ar.take<float>(n); L.s32.wx = ar.take<float>(n);
         L.s32.wy = ar.take<float>(n); L.s32.wz = ar.take<float>(n); L.s32.di = ar.take<float>(n);
@@ -1580,7 +1594,7 @@ static void launch_restrict(hipStream_t st, Prof *P, int cls, const Level &L, co
         const int zl = restrict_zlen(L.d);
         const int nb = ((L.d.nx + kGsTx - 1) / kGsTx) * ((L.d.ny + kGsTy - 1) / kGsTy) * ((L.d.nz + zl - 1) / zl);
         PROF_LAUNCH(P, cls, (k_restrict_z<T, FT>), dim3(grid_for(nb, 1)), dim3(kBlock), 0, st, o, zl, C.d.nx,
-                    C.d.ny, C.d.nz, f, u, (T *)C.f, done);
+                    C.d.ny, C.d.nz, f, u, (T *)C.f, done, (int)L.s64.eface);
     } else {
         PROF_LAUNCH(P, cls, (k_resid_restrict<T, FT, 0>), dim3(grid_for(C.d.n(), kBlock)), dim3(kBlock), 0, st, o,
                     L.fx, L.fy, L.fz, C.d.nx, C.d.ny, C.d.nz, f, u, (T *)C.f, done);
@@ -1813,7 +1827,7 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     auto iteration = [&](Prof *PP) -> T * {
         if (zlen > 0)
             PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv_z<T>, dim3(Gz), dim3(kBlock), 0, st, o, (const T *)z, B.s0,
-                        B.s1, B.v, x, zlen, G, Gdot, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
+                        B.s1, B.v, x, zlen, G, Gdot, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau, (int)A.eface);
         else
             PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv<T>, dim3(G), dim3(kBlock), 0, st, o, (const T *)z, B.s0,
                         B.s1, B.v, S.ch.chunk, G, Gdot, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
@@ -2041,11 +2055,16 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         double n1 = (double)H.lev[H.nlev > 1 ? 1 : 0].d.n();
         // A64, s_in, z, s_out, v (+ x in and out where the SpMV applies the x update)
         const double xb = spmv_zlen(d) > 0 ? 16.0 : 0.0;
-        sink->bytes[HMFEM_PROF_CG_SPMV] = n0 * (32 + 8 + vs + 8 + 8 + xb);
+        // with eface the z-marching SpMV and restriction read e on the face cells only
+        const bool ez = A.eface && xb > 0;
+        const double nin = (double)std::max(d.nx - 2, 0) * std::max(d.ny - 2, 0) * std::max(d.nz - 2, 0);
+        const double e64 = ez ? 8.0 * (n0 - nin) : 8.0 * n0;
+        const double er = A.eface && H.lev[0].fx == 2 && H.lev[0].fy == 2 && H.lev[0].fz == 2 ? vs * (n0 - nin) : vs * n0;
+        sink->bytes[HMFEM_PROF_CG_SPMV] = e64 + n0 * (24 + 8 + vs + 8 + 8 + xb);
         sink->bytes[HMFEM_PROF_CG_UPDATE] = n0 * (48 - 1.5 * xb);           // r, v in; r out (+ x, s in; x out)
         sink->bytes[HMFEM_PROF_CG_DOTS] = 0.0;                               // fused into the last sweep
         sink->bytes[HMFEM_PROF_SMOOTH0] = n0 * (c4 + 8 + vs);                // A, f, u out
-        sink->bytes[HMFEM_PROF_RESTRICT] = n0 * (c4 + 8 + vs) + n1 * vs;     // A, f, u; f_c out
+        sink->bytes[HMFEM_PROF_RESTRICT] = er + n0 * (c4 - vs + 8 + vs) + n1 * vs;   // A, f, u; f_c out
         sink->bytes[HMFEM_PROF_PROLONG] = n0 * (c4 + 8 + 2 * vs) + n1 * vs;  // A, f, u, u_c; u out
         sink->bytes[HMFEM_PROF_JACOBI] = n0 * (c4 + 8 + 2 * vs + 8);         // A, f, u, v; z out (+dots)
         if (opt.smoother == HMFEM_SMOOTH_BLOCK_GS) {

@@ -78,7 +78,8 @@ class Status(ctypes.Structure):
 class BlockStatus(ctypes.Structure):
     _fields_ = [("status", c_int), ("max_iterations", c_int), ("t_assemble_ms", c_dbl), ("t_setup_ms", c_dbl),
                 ("t_solve_ms", c_dbl), ("t_flux_ms", c_dbl), ("footprint_bytes", c_dbl),
-                ("deflation_clusters", c_int), ("deflation_cells", c_int)]
+                ("deflation_clusters", c_int), ("deflation_cells", c_int),
+                ("block_iterations", c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -178,7 +178,7 @@ typedef struct {
                                 inside 32x8x8 tiles, one pass per smoothing step; the default); the
                                 levels of the single-workgroup kernel always use Jacobi */
     int gs_sweeps;           /* sweeps per smoothing step of HMFEM_SMOOTH_BLOCK_GS; default 0: two in single
-                                solves, three in batched and block solves (measured best on each) */
+                                and block solves, three in batched solves (measured best on each) */
     int autotune;            /* 1: hmfem_reservoir_device first runs hmfem_autotune_device for each new
                                 grid shape / precision / tolerance of the context and then uses the
                                 options it chose (default 0) */
@@ -322,10 +322,12 @@ int hmfem_reservoir_batch_device(hmfem_context *ctx, int ninst, const int *dims,
    the initial guess in and the pressure out; v[r] (device ELE[n], or NULL)
    receives the fluxes. Per right-hand side: iterations, rel_residual and
    div_max (host arrays of nrhs, or NULL). Uses the FP32 hierarchy and the
-   block Gauss-Seidel smoother whatever opt says; the right-hand sides are
-   solved in groups of four (two if there are only one or two), the last
-   group padded with zero right-hand sides. The conduit correction
-   (opt.deflate) is built once per call and shared by every group. No counterpart
+   block Gauss-Seidel smoother whatever opt says; the right-hand sides go
+   through four lanes (two if there are only one or two) of one interleaved
+   solve: the first four start together and each further one takes the first
+   lane whose right-hand side has converged, so that no lane idles while
+   work is left (the lane restarts its own CG; the hierarchy and the conduit
+   correction, opt.deflate, are built once per call). No counterpart
    in the reference's API: it is the interleaved multi-RHS solve of
    generic_operator.h:335-391 behind the reservoir() step. */
 typedef struct {
@@ -335,6 +337,7 @@ typedef struct {
     double footprint_bytes;  /* device memory the call used */
     int deflation_clusters;  /* clusters of the conduit correction (0: none) */
     int deflation_cells;
+    int block_iterations;    /* iterations of the interleaved solve (every lane advances once per iteration) */
 } hmfem_block_status;
 
 int hmfem_reservoir_block_device(hmfem_context *ctx, const hmfem_grid *g, const double *Kx, const double *Ky,
