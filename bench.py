@@ -274,6 +274,33 @@ def run_batch(args, w, torch, S, emit=True):
         torch.cuda.synchronize()
         fp64 = {"value": nb * k64 / (time.perf_counter() - ta)}
         fp64["speedup_fp32_over_fp64"] = value / fp64["value"]
+    # per-instance options from the batch autotuner (opt.autotune: tuned on
+    # the first call, which is not timed)
+    tuned = None
+    if getattr(args, "tune", not args.no_fp64) and prec == S.PREC_FP32:
+        ot = S.default_options(rtol=1e-8, check_every=2, autotune=1)
+        ot.smoother = opt.smoother
+        t_tune = time.perf_counter()
+        step(ot)
+        torch.cuda.synchronize()
+        t_tune = time.perf_counter() - t_tune
+        ta = time.perf_counter()
+        kt = max(3, args.steps)
+        for _ in range(kt):
+            stt, pert = step(ot)
+        torch.cuda.synchronize()
+        t_tuned = time.perf_counter() - ta
+        dt = np.sort(pert["done_ms"])
+        gs_i, ocf_i, _ = ctx.autotune_batch(dims, Kd, Kd, Kd, bval, None, S.default_options(rtol=1e-8, check_every=2))
+        pairs = {}
+        for a, b in zip(gs_i.tolist(), np.round(ocf_i, 3).tolist()):
+            pairs["%d sweeps, scaling %g" % (a, b)] = pairs.get("%d sweeps, scaling %g" % (a, b), 0) + 1
+        tuned = {"value": nb * kt / t_tuned, "tuning_s": t_tune,
+                 "per_instance_done_ms": {"p50": float(np.percentile(dt, 50)), "p99": float(np.percentile(dt, 99)),
+                                          "max": float(dt[-1])},
+                 "iterations": {"min": int(pert["iterations"].min()), "max": int(pert["iterations"].max())},
+                 "instances_per_option": pairs}
+        tuned["speedup_over_defaults"] = tuned["value"] / value
     cpu = None
     if not args.no_cpu:
         # the reference build on the batch's first instances, up to the budget
@@ -300,7 +327,7 @@ def run_batch(args, w, torch, S, emit=True):
         "max_rel_residual": float(per["rel_residual"].max()),
         "phase_ms": {"assemble": st.t_assemble_ms, "setup": st.t_setup_ms, "solve": st.t_solve_ms,
                      "flux": st.t_flux_ms},
-        "fp64_hierarchy": fp64, "cpu_baseline": cpu,
+        "fp64_hierarchy": fp64, "autotuned": tuned, "cpu_baseline": cpu,
     }
     if emit:
         print(json.dumps(out))
@@ -312,14 +339,14 @@ def run_batch(args, w, torch, S, emit=True):
 
 def batch_summary(torch, S):
     """The 96-instance batch workload, short (1 warmup + 3 timed batches, no
-    CPU or FP64 legs): solves/s and per-instance completion percentiles, for
-    the main line."""
+    CPU or FP64 legs; the autotuned options' line): solves/s and per-instance
+    completion percentiles, for the main line."""
     a = argparse.Namespace(steps=3, warmup=1, no_fp64=True, no_cpu=True, precision="fp32", smoother=None,
-                           gs_sweeps=None, ocf=None, cpu_budget=0.0)
+                           gs_sweeps=None, ocf=None, cpu_budget=0.0, tune=True)
     r = run_batch(a, WORKLOADS["c4"], torch, S, emit=False)
     return {"workload": r["config"]["workload"], "value": r["value"], "unit": "solves/s",
             "ms_per_batch": r["ms_per_step"], "per_instance_done_ms": r["per_instance_done_ms"],
-            "iterations": r["iterations"], "setup_ms": r["phase_ms"]["setup"]}
+            "iterations": r["iterations"], "setup_ms": r["phase_ms"]["setup"], "autotuned": r["autotuned"]}
 
 
 def block_rhs(nx, ny, nz, nrhs):

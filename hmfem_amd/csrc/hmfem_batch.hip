@@ -27,6 +27,8 @@ struct BInst {
     int fx, fy, fz;                       // aggregation factors to the next level
     int tile0, ntile;                     // Gauss-Seidel tiles of this instance
     int zblk0, zG, zlen;                  // kb_restrict_z blocks (zlen 0: not 2x2x2, kb_resid_restrict)
+    int gs;                               // the instance's own Gauss-Seidel sweeps (0: the batch's)
+    float ocf;                            // the instance's own coarse-correction scaling (0: the batch's)
 };
 
 constexpr int kBatchZlen = 16;   // planes per block of the batch's z-marching SpMV
@@ -238,6 +240,7 @@ kb_prolong_jacobi(const BInst *__restrict__ D, const BInst *__restrict__ Dc, con
     const int s = (blockIdx.x - I.blk0) * kBlock + threadIdx.x;
     if (s >= I.n) return;
     const BInst Ic = Dc[q];
+    if (I.ocf > 0.f) ocf = (T)I.ocf;
     const OpArgs<T> o = bop(A, I.off, I.nx, I.ny, I.nz);
     const FT *fl = f + I.off;
     const T *ul = u + I.off;
@@ -299,6 +302,8 @@ kb_gs_tile(const BInst *__restrict__ D, const BInst *__restrict__ Dc, const int 
     const int q = tmap[blockIdx.x];
     if (sc[q].done) return;
     const BInst I = D[q];
+    if (I.gs > 0) sweeps = I.gs;
+    if (I.ocf > 0.f) ocf = (T)I.ocf;
     const OpArgs<T> o = bop(A, I.off, I.nx, I.ny, I.nz);
     double acc[2] = {0.0, 0.0};
     if (MODE == 0) {
@@ -324,6 +329,8 @@ kb_gs_tile_dots(const CInst *__restrict__ CI, const int *__restrict__ cbmap, con
     if (sc[q].done) return;
     const CInst C = CI[q];
     const BInst I = D[q], Ic = Dc[q];
+    if (I.gs > 0) sweeps = I.gs;
+    if (I.ocf > 0.f) ocf = (T)I.ocf;
     const OpArgs<T> o = bop(A, I.off, I.nx, I.ny, I.nz);
     const int lb = blockIdx.x - C.cblk0;
     const int tbeg = lb * C.tpb, tend = min(I.ntile, tbeg + C.tpb);
@@ -968,7 +975,8 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
                 const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *alloc_ctx, CgScalars *h_sc,
                 GraphCache *graph, std::vector<char> &stage, hipEvent_t ev[4], int *iters_out,
                 double *relres_out, double *done_ms_out, int *status_out, double *setup_ms, double *solve_ms,
-                Deflation *defl, char *(*dalloc)(void *, size_t, int), void *dctx)
+                Deflation *defl, char *(*dalloc)(void *, size_t, int), void *dctx, const int *inst_gs,
+                const double *inst_ocf)
 {
     hmfem_options opt = opt_in;
     if (opt.nu < 2) opt.nu = 2;
@@ -976,6 +984,12 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     const int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
     Batch B;
     plan_batch(ninst, dims, coarse_max, B);
+    // per-instance cycle options (the batch autotuner's), on every level
+    for (int l = 0; l < B.nlev; l++)
+        for (int q = 0; q < ninst; q++) {
+            B.lev[l].h[q].gs = inst_gs && inst_gs[q] > 0 ? inst_gs[q] : 0;
+            B.lev[l].h[q].ocf = inst_ocf && inst_ocf[q] > 0 ? (float)inst_ocf[q] : 0.f;
+        }
     size_t need = batch_bytes(B, prec);
     char *arena = alloc(alloc_ctx, need);
     if (!arena) return HMFEM_ERR_NOMEM;

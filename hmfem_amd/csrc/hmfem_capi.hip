@@ -80,6 +80,11 @@ struct hmfem_context {
     long lds_limit = 65536;    // dynamic LDS a workgroup may use on this device
     // options chosen by the autotuner, per grid shape and precision
     std::map<std::vector<int>, hmfem_options> tuned;
+    struct BatchTuned {
+        std::vector<int> gs;
+        std::vector<double> ocf;
+    };
+    std::map<std::vector<int>, BatchTuned> tuned_batch;   // per-instance cycle options of a batch shape
 
     // device buffer `name` with at least `bytes`, grown on demand
     void *get(const char *name, size_t bytes)
@@ -837,11 +842,11 @@ extern "C" int hmfem_reservoir_block_device(hmfem_context *c, const hmfem_grid *
 
 static char *ctx_alloc(void *c, size_t bytes) { return (char *)((hmfem_context *)c)->get("batch_arena", bytes); }
 
-extern "C" int hmfem_reservoir_batch_device(hmfem_context *c, int ninst, const int *dims, const double *lengths,
-                                            double viscosidade, const int bc_type[6], const double *const bval[6],
-                                            const double *Kx, const double *Ky, const double *Kz, double *p, ELE *v,
-                                            const hmfem_options *opt_in, hmfem_batch_status *st, int *iterations,
-                                            double *rel_residual, double *done_ms, double *div_max)
+static int batch_device(hmfem_context *c, int ninst, const int *dims, const double *lengths, double viscosidade,
+                        const int bc_type[6], const double *const bval[6], const double *Kx, const double *Ky,
+                        const double *Kz, double *p, ELE *v, const hmfem_options *opt_in, hmfem_batch_status *st,
+                        int *iterations, double *rel_residual, double *done_ms, double *div_max, const int *inst_gs,
+                        const double *inst_ocf)
 {
     if (!c || ninst < 1 || !dims || !bc_type || !bval || !Kx || !Ky || !Kz || !p) return HMFEM_ERR_ARG;
     long total = 0;
@@ -877,7 +882,7 @@ extern "C" int hmfem_reservoir_batch_device(hmfem_context *c, int ninst, const i
     std::vector<char> stage2;
     rc = batch_solve(c->st, ninst, dims, A, b, p, opt, ctx_alloc, c, hsc.data(), opt.use_graph ? &c->graph : nullptr,
                      stage2, c->ev, its.data(), rr.data(), dm.data(), sts.data(), &setup, &solve, &c->defl,
-                     ctx_alloc_defl, c);
+                     ctx_alloc_defl, c, inst_gs, inst_ocf);
     if (rc) return rc;
     float ta = 0.f;
     (void)hipEventElapsedTime(&ta, c->ev[4], c->ev[5]);
@@ -914,6 +919,140 @@ extern "C" int hmfem_reservoir_batch_device(hmfem_context *c, int ninst, const i
             for (int q = 0; q < ninst; q++) div_max[q] = h[2 * q + 1];
     }
     return out->status == HMFEM_OK ? HMFEM_OK : out->status;
+}
+
+
+static bool batch_args_ok(int ninst, const int *dims, const int *bc_type, const double *const *bval, const double *Kx,
+                          const double *Ky, const double *Kz, double viscosidade)
+{
+    if (ninst < 1 || !dims || !bc_type || !bval || !Kx || !Ky || !Kz || !(viscosidade > 0)) return false;
+    long total = 0;
+    for (int q = 0; q < ninst; q++) {
+        if (dims[3 * q] < 1 || dims[3 * q + 1] < 1 || dims[3 * q + 2] < 1) return false;
+        total += (long)dims[3 * q] * dims[3 * q + 1] * dims[3 * q + 2];
+    }
+    return total < 2147483647L;
+}
+
+static std::vector<int> batch_tune_key(int ninst, const int *dims, const hmfem_options &o)
+{
+    std::vector<int> k(dims, dims + 3 * (size_t)ninst);
+    k.insert(k.end(), {ninst, o.precision, o.smoother, o.stop_rule, o.deflate,
+                       (int)(-log10(o.rtol > 0 ? o.rtol : 1e-8) * 100)});
+    return k;
+}
+
+// Block Gauss-Seidel sweeps and coarse-correction scaling for every instance
+// of a batch: every candidate pair runs on the whole batch (from the caller's
+// initial guess p, kept), its cost per cell-iteration is its time over the
+// sum of the instances' iterations x cells, each instance takes the pair with
+// the least iterations x cells x that cost, and the per-instance choice is
+// kept only if a run with it beats the best single pair for the whole batch.
+extern "C" int hmfem_autotune_batch_device(hmfem_context *c, int ninst, const int *dims, const double *lengths,
+                                           double viscosidade, const int bc_type[6], const double *const bval[6],
+                                           const double *Kx, const double *Ky, const double *Kz, const double *p,
+                                           const hmfem_options *opt_in, int *gs_out, double *ocf_out,
+                                           double *best_ms)
+{
+    if (!c || !batch_args_ok(ninst, dims, bc_type, bval, Kx, Ky, Kz, viscosidade) || !gs_out || !ocf_out)
+        return HMFEM_ERR_ARG;
+    hmfem_options base;
+    if (opt_in) base = *opt_in;
+    else hmfem_default_options(&base);
+    base.autotune = 0;
+    long total = 0;
+    std::vector<double> ncell(ninst);
+    for (int q = 0; q < ninst; q++) {
+        ncell[q] = (double)dims[3 * q] * dims[3 * q + 1] * dims[3 * q + 2];
+        total += (long)ncell[q];
+    }
+    double *x = (double *)c->get("tune_batch_x", total * 8);
+    if (!x) return HMFEM_ERR_NOMEM;
+    struct Cand { int gs; double ocf; };
+    static const Cand cands[] = {{2, 2.2}, {3, 2.2}, {4, 2.2}, {2, 2.0}, {3, 2.0}, {4, 2.0}, {3, 2.5}, {4, 2.5}};
+    const int nc = (int)(sizeof(cands) / sizeof(cands[0]));
+    std::vector<std::vector<int>> its(nc, std::vector<int>(ninst));
+    std::vector<double> tms(nc, 1e300);
+    std::vector<int> gi(ninst);
+    std::vector<double> oi(ninst);
+    // one timed run (after one that builds the graphs) of a per-instance choice
+    auto run = [&](const int *g, const double *o, int *it_out, double &ms) -> int {
+        ms = 1e300;
+        for (int rep = 0; rep < 2; rep++) {
+            if (p) HMFEM_CHECK(hipMemcpyAsync(x, p, total * 8, hipMemcpyDeviceToDevice, c->st));
+            else HMFEM_CHECK(hipMemsetAsync(x, 0, total * 8, c->st));
+            hmfem_batch_status s{};
+            int rc = batch_device(c, ninst, dims, lengths, viscosidade, bc_type, bval, Kx, Ky, Kz, x, nullptr,
+                                  &base, &s, it_out, nullptr, nullptr, nullptr, g, o);
+            if (rc < 0) return rc;
+            if (s.status != HMFEM_OK) return HMFEM_NOT_CONVERGED;
+            ms = s.t_setup_ms + s.t_solve_ms;
+        }
+        return HMFEM_OK;
+    };
+    int best = -1;
+    for (int k = 0; k < nc; k++) {
+        std::fill(gi.begin(), gi.end(), cands[k].gs);
+        std::fill(oi.begin(), oi.end(), cands[k].ocf);
+        int rc = run(gi.data(), oi.data(), its[k].data(), tms[k]);
+        if (rc < 0) return rc;
+        if (rc == HMFEM_OK && (best < 0 || tms[k] < tms[best])) best = k;
+    }
+    if (best < 0) return HMFEM_NOT_CONVERGED;
+    // cost per cell-iteration of each pair, then each instance's cheapest pair
+    std::vector<double> w(nc, 0.0);
+    for (int k = 0; k < nc; k++) {
+        double work = 0;
+        for (int q = 0; q < ninst; q++) work += its[k][q] * ncell[q];
+        w[k] = tms[k] < 1e300 && work > 0 ? tms[k] / work : 1e300;
+    }
+    for (int q = 0; q < ninst; q++) {
+        int kb = best;
+        for (int k = 0; k < nc; k++)
+            if (w[k] < 1e300 && its[k][q] * w[k] < its[kb][q] * w[kb]) kb = k;
+        gi[q] = cands[kb].gs;
+        oi[q] = cands[kb].ocf;
+    }
+    std::vector<int> itp(ninst);
+    double tp = 1e300;
+    int rc = run(gi.data(), oi.data(), itp.data(), tp);
+    if (rc < 0) return rc;
+    if (!(rc == HMFEM_OK && tp < tms[best])) {
+        std::fill(gi.begin(), gi.end(), cands[best].gs);
+        std::fill(oi.begin(), oi.end(), cands[best].ocf);
+        tp = tms[best];
+    }
+    for (int q = 0; q < ninst; q++) { gs_out[q] = gi[q]; ocf_out[q] = oi[q]; }
+    if (best_ms) *best_ms = tp;
+    return HMFEM_OK;
+}
+
+extern "C" int hmfem_reservoir_batch_device(hmfem_context *c, int ninst, const int *dims, const double *lengths,
+                                            double viscosidade, const int bc_type[6], const double *const bval[6],
+                                            const double *Kx, const double *Ky, const double *Kz, double *p, ELE *v,
+                                            const hmfem_options *opt_in, hmfem_batch_status *st, int *iterations,
+                                            double *rel_residual, double *done_ms, double *div_max)
+{
+    if (!c || !p || !batch_args_ok(ninst, dims, bc_type, bval, Kx, Ky, Kz, viscosidade)) return HMFEM_ERR_ARG;
+    if (!opt_in || !opt_in->autotune)
+        return batch_device(c, ninst, dims, lengths, viscosidade, bc_type, bval, Kx, Ky, Kz, p, v, opt_in, st,
+                            iterations, rel_residual, done_ms, div_max, nullptr, nullptr);
+    // autotune: per-instance options of this batch shape, tuned on its first call
+    const auto key = batch_tune_key(ninst, dims, *opt_in);
+    auto it = c->tuned_batch.find(key);
+    if (it == c->tuned_batch.end()) {
+        hmfem_context::BatchTuned t;
+        t.gs.resize(ninst);
+        t.ocf.resize(ninst);
+        int rc = hmfem_autotune_batch_device(c, ninst, dims, lengths, viscosidade, bc_type, bval, Kx, Ky, Kz, p, opt_in,
+                                             t.gs.data(), t.ocf.data(), nullptr);
+        if (rc) return rc;
+        it = c->tuned_batch.emplace(key, std::move(t)).first;
+    }
+    hmfem_options o = *opt_in;
+    o.autotune = 0;
+    return batch_device(c, ninst, dims, lengths, viscosidade, bc_type, bval, Kx, Ky, Kz, p, v, &o, st, iterations,
+                        rel_residual, done_ms, div_max, it->second.gs.data(), it->second.ocf.data());
 }
 
 extern "C" double hmfem_kernel_bytes_per_cell(int which, int precision)

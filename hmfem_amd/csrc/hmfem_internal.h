@@ -287,7 +287,8 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
                 const hmfem_options &opt_in, char *(*alloc)(void *, size_t), void *alloc_ctx, CgScalars *h_sc,
                 GraphCache *graph, std::vector<char> &stage, hipEvent_t ev[4], int *iters_out,
                 double *relres_out, double *done_ms_out, int *status_out, double *setup_ms, double *solve_ms,
-                Deflation *defl = nullptr, char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr);
+                Deflation *defl = nullptr, char *(*dalloc)(void *, size_t, int) = nullptr, void *dctx = nullptr,
+                const int *inst_gs = nullptr, const double *inst_ocf = nullptr);
 int batch_grids(hipStream_t st, int ninst, const int *dims, const double *lengths, double mu, const int bc[6],
                 std::vector<char> &stage, void *d_grids, int *d_bmap, int *nblk_out);
 size_t batch_grid_bytes(int ninst);

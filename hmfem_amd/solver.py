@@ -192,6 +192,8 @@ def lib():
         L.hmfem_reservoir_block_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
                                                    c_vp, P(Options), P(BlockStatus), c_vp, c_vp, c_vp]
         L.hmfem_autotune_device.argtypes = [c_vp, P(HmfemGrid), c_vp, c_vp, c_vp, c_vp, P(Options), P(c_dbl)]
+        L.hmfem_autotune_batch_device.argtypes = [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                                  c_vp, P(Options), c_vp, c_vp, P(c_dbl)]
         L.hmfem_kernel_bytes_per_cell.argtypes = [c_int, c_int]
         L.hmfem_kernel_bytes_per_cell.restype = c_dbl
         L.save_pressure.argtypes = [c_vp, Dim, GridH]
@@ -215,7 +217,8 @@ EXPORTED = ("solve_linsys", "compute", "permeabilidade_arestas", "source_term", 
             "hmfem_coupled_device", "hmfem_hierarchy_level", "hmfem_hierarchy_coarse_inverse",
             "hmfem_solve_structured_device", "hmfem_profile_enable", "hmfem_profile_read",
             "hmfem_profile_reset", "hmfem_kernel_bytes_per_cell", "hmfem_reservoir_batch_device",
-            "hmfem_reservoir_block_device", "hmfem_autotune_device", "save_pressure", "save_flux",
+            "hmfem_reservoir_block_device", "hmfem_autotune_device", "hmfem_autotune_batch_device", "save_pressure",
+            "save_flux",
             "vts_StructuredGrid_file", "hmfem_set_output_dir", "save_pressure_1d", "save_flux_1d", "iter_amg",
             "boundary", "dirname", "number_karst")
 
@@ -557,6 +560,28 @@ class Context:
         if rc != 0:
             raise RuntimeError("hmfem_autotune_device failed: %d" % rc)
         return o, ms.value
+
+    def autotune_batch(self, dims, Kx, Ky, Kz, bval, p=None, options=None, viscosidade=1.0,
+                       bc_type=(0, 0, 1, 1, 1, 1), lengths=None):
+        """Per-instance block Gauss-Seidel sweeps and coarse-correction scaling
+        for this batch (p: the initial guess, kept; None: zero); returns
+        (sweeps, scaling, best ms) with numpy arrays of one entry per instance."""
+        dims_a = np.ascontiguousarray(np.asarray(dims, np.int32).reshape(-1, 3))
+        ninst = dims_a.shape[0]
+        len_a = None if lengths is None else np.ascontiguousarray(np.asarray(lengths, np.float64).reshape(-1, 3))
+        bct = np.ascontiguousarray(np.asarray(bc_type, np.int32))
+        gs = np.zeros(ninst, np.int32)
+        ocf = np.zeros(ninst)
+        ms = c_dbl()
+        o = options if options is not None else default_options()
+        _order_after_torch(Kx, Ky, Kz, p, *bval)
+        rc = lib().hmfem_autotune_batch_device(self.h, ninst, _p(dims_a), None if len_a is None else _p(len_a),
+                                               viscosidade, _p(bct), self._faces(bval), _dptr(Kx), _dptr(Ky),
+                                               _dptr(Kz), _dptr(p), ctypes.byref(o), _p(gs), _p(ocf),
+                                               ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError("hmfem_autotune_batch_device failed: %d" % rc)
+        return gs, ocf, ms.value
 
     def reservoir_block(self, g, Kx, Ky, Kz, bvals, p, v=None, q=None, options=None):
         """Solve len(p) right-hand sides of one grid together. bvals[r]: six

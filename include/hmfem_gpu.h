@@ -181,7 +181,9 @@ typedef struct {
                                 and block solves, three in batched solves (measured best on each) */
     int autotune;            /* 1: hmfem_reservoir_device first runs hmfem_autotune_device for each new
                                 grid shape / precision / tolerance of the context and then uses the
-                                options it chose (default 0) */
+                                options it chose; hmfem_reservoir_batch_device likewise runs
+                                hmfem_autotune_batch_device for each new batch shape and uses its
+                                per-instance options (default 0) */
     int deflate;             /* 1: add a coarse correction on each 7-point-connected cluster of at least
                                 8 cells whose diagonal exceeds deflate_ratio times the median diagonal
                                 (karst conduits); off when there are none (default 1; single, block and
@@ -311,6 +313,21 @@ int hmfem_reservoir_batch_device(hmfem_context *ctx, int ninst, const int *dims,
                                  const double *Kx, const double *Ky, const double *Kz, double *p, ELE *v,
                                  const hmfem_options *opt, hmfem_batch_status *st, int *iterations,
                                  double *rel_residual, double *done_ms, double *div_max);
+
+/* Batch autotuner: block Gauss-Seidel sweeps and coarse-correction scaling
+   for every instance (host arrays gs_sweeps[ninst], ocf[ninst]). Every
+   candidate pair (sweeps 2-4, scaling 2.0-2.5) solves the whole batch from
+   the initial guess p (device, kept; NULL: zero) twice; each instance takes
+   the pair with the least iterations x cells x cost per cell-iteration of
+   that pair (its batch time over the summed work), and the per-instance
+   choice is kept if a batch run with it is faster than the best single pair,
+   else every instance gets that pair. best_ms (may be NULL): the chosen
+   setup + solve time. The other fields of *opt are kept. No counterpart in
+   the reference (one cycle for every solve, amg.h:1135-1239). */
+int hmfem_autotune_batch_device(hmfem_context *ctx, int ninst, const int *dims, const double *lengths,
+                                double viscosidade, const int bc_type[6], const double *const bval[6],
+                                const double *Kx, const double *Ky, const double *Kz, const double *p,
+                                const hmfem_options *opt, int *gs_sweeps, double *ocf, double *best_ms);
 
 /* Block solve: nrhs (1..8) right-hand sides of one operator, solved
    together (one hierarchy; every kernel reads the coefficients once for all

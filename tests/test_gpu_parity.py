@@ -721,6 +721,40 @@ def test_batch_uneven_matches_single(smoother, ctx, torch):
         off += P.n
 
 
+def test_batch_autotune(ctx, torch):
+    """The batch autotuner gives every instance sweeps and a scaling from its
+    candidate list; with them (through opt.autotune, twice: the second call
+    reuses the tuned options) the batch reproduces each instance's
+    default-option solution."""
+    specs = [((32, 24, 16), "lognormal", 11), ((32, 32, 24), "karst", 12), ((24, 20, 16), "lognormal", 13),
+             ((40, 24, 24), "karst", 14), ((16, 16, 16), "homogeneous", 15), ((36, 20, 28), "karst", 16)]
+    probs = [make_problem(d, k, seed=s, sigma=2.0, corr=4.0, conduits=3) for d, k, s in specs]
+    _say("batch autotune", [P.n for P in probs])
+    K = torch.from_numpy(np.concatenate([P.Kx for P in probs])).to("cuda")
+    big = max(max(P.nx * P.ny, P.ny * P.nz, P.nx * P.nz) for P in probs)
+    bval = [torch.full((big,), 1.0, dtype=torch.float64, device="cuda")] + \
+           [torch.zeros(big, dtype=torch.float64, device="cuda") for _ in range(5)]
+    ntot = sum(P.n for P in probs)
+    dims = [(P.nx, P.ny, P.nz) for P in probs]
+    gs, ocf, ms = ctx.autotune_batch(dims, K, K, K, bval, None, S.default_options(rtol=1e-8))
+    _say("tuned sweeps", gs.tolist(), "scaling", ocf.tolist(), "%.2f ms" % ms)
+    assert gs.shape == (len(probs),) and set(gs.tolist()) <= {2, 3, 4}
+    assert set(np.round(ocf, 6).tolist()) <= {2.0, 2.2, 2.5} and ms > 0
+    p0 = torch.zeros(ntot, dtype=torch.float64, device="cuda")
+    st0, _ = ctx.reservoir_batch(dims, K, K, K, bval, p0, None, S.default_options(rtol=1e-12))
+    assert st0.status == S.OK
+    x0 = p0.cpu().numpy()
+    for rep in range(2):
+        p = torch.zeros(ntot, dtype=torch.float64, device="cuda")
+        st, per = ctx.reservoir_batch(dims, K, K, K, bval, p, None, S.default_options(rtol=1e-12, autotune=1))
+        assert st.status == S.OK, (rep, st.as_dict(), per)
+        x = p.cpu().numpy()
+        off = 0
+        for P, d in zip(probs, dims):
+            assert rel_l2(x[off:off + P.n], x0[off:off + P.n]) < 1e-9, (rep, d)
+            off += P.n
+
+
 def test_batch_stopped_returns_last_iterate(ctx, torch):
     """Batch instances cut off by max_iterations (and one that converges
     before the cap) return the iterates whose residuals they report: the
