@@ -405,6 +405,7 @@ struct CcWork {
     int *sel;                                           // chosen roots (ascending), then the cluster offsets
 };
 constexpr int kChunk = 1024;
+constexpr int kJumpRounds = 8;   // chains of up to 256 links flattened (longer ones are still walked)
 
 // Per strong cell a < ms: its instance, its six strong neighbours inside the
 // instance (ordinals, -1: none), parent = itself, size 0.
@@ -449,15 +450,53 @@ __global__ void __launch_bounds__(kBlock) k_cc_hook(const int *__restrict__ pms,
         if (nb[f] >= 0) cc_unite(w.par, a, nb[f]);
 }
 
-// roots and component sizes (after every union has finished)
-__global__ void __launch_bounds__(kBlock) k_cc_roots(const int *__restrict__ pms, int cap, CcWork w)
+// Pointer jumping after the unions: par[a] = par[par[a]]. Unions of cells
+// along a conduit leave chains (each root hooked under the next smaller
+// one), and walking a chain is one dependent load per link (140 us on the
+// 256^3 conduit field); every round halves the chains. Reads may see another
+// thread's update of this round or not: either is an ancestor, and parents
+// only ever point to smaller ordinals, so the roots stay what they are.
+__global__ void __launch_bounds__(kBlock) k_cc_jump(const int *__restrict__ pms, int cap, CcWork w)
 {
     const int ms = min(*pms, cap);
     const int a = blockIdx.x * kBlock + threadIdx.x;
     if (a >= ms) return;
-    const int r = cc_find(w.par, a);
-    w.root[a] = r;
-    atomicAdd(w.size + r, 1);
+    int *par = w.par;
+    const int p = par[a];
+    const int g = par[p];
+    if (g != p) par[a] = g;
+}
+
+// roots and component sizes (after every union has finished: the parent
+// pointers no longer change, so plain loads through the caches walk them).
+// The cells of one conduit are mostly neighbours in the list, so the lanes
+// of a wave with the same root add their count with one atomic (one atomic
+// per cell on a handful of sizes serialised: 140 us on the 256^3 conduit
+// field).
+__global__ void __launch_bounds__(kBlock) k_cc_roots(const int *__restrict__ pms, int cap, CcWork w)
+{
+    const int ms = min(*pms, cap);
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    int r = -1;
+    if (a < ms) {
+        const int *__restrict__ par = w.par;
+        int p = par[a];
+        r = a;
+        while (p != r) {
+            r = p;
+            p = par[r];
+        }
+        w.root[a] = r;
+    }
+    const int lane = threadIdx.x & 63;
+    unsigned long long left = __ballot(r >= 0);
+    while (left) {   // one distinct root of the wave per round (wave-uniform loop)
+        const int lead = __ffsll((long long)left) - 1;
+        const int lr = __shfl(r, lead);
+        const unsigned long long same = __ballot(r == lr) & left;
+        if (lane == lead) atomicAdd(w.size + lr, __popcll(same));
+        left &= ~same;
+    }
 }
 
 // components of at least kMinCluster cells: (root, size, instance), in any
@@ -687,6 +726,8 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     hipLaunchKernelGGL(k_cc_links, gl, dim3(kBlock), 0, st, (const DefInst *)dinst, ninst, (const int *)slist, pms,
                        cap, w);
     hipLaunchKernelGGL(k_cc_hook, gl, dim3(kBlock), 0, st, pms, cap, w);
+    for (int round = 0; round < kJumpRounds; round++)
+        hipLaunchKernelGGL(k_cc_jump, gl, dim3(kBlock), 0, st, pms, cap, w);
     hipLaunchKernelGGL(k_cc_roots, gl, dim3(kBlock), 0, st, pms, cap, w);
     hipLaunchKernelGGL(k_cc_cands, gl, dim3(kBlock), 0, st, pms, cap, ccap, w);
 

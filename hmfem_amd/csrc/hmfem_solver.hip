@@ -1338,7 +1338,11 @@ __global__ void __launch_bounds__(kBlock) k_axis_logmean(OpArgs<double> o, doubl
 {
     const long n = (long)o.nx * o.ny * o.nz;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (long s = ((long)blockIdx.x * kBlock + threadIdx.x) * 16; s < n; s += (long)gridDim.x * kBlock * 16) {
+    // 64 consecutive cells out of every 1024 (whole cache lines; every 16th
+    // cell alone fetched every line: 103 instead of 8 us at 256^3)
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x;; q += (long)gridDim.x * kBlock) {
+        const long s = (q >> 6) * 1024 + (q & 63);
+        if (s >= n) break;
         const double w[3] = {o.wx[s], o.wy[s], o.wz[s]};
         for (int a = 0; a < 3; a++)
             if (w[a] > 0) { acc[2 * a] += log(w[a]); acc[2 * a + 1] += 1.0; }
