@@ -634,6 +634,179 @@ k_defl_einv(int nc, const int *__restrict__ cp, const double *__restrict__ pe, d
     }
 }
 
+// ------------------------------------------------- the balancing form
+//
+// z = Q r + (I - Q A) V (I - A Q) r with Q = Z E^-1 Z^T: the V-cycle sees the
+// residual with the clusters' constant modes taken out, and its output is
+// projected so that those modes are exactly Q's. With c1 = E^-1 Z^T r:
+//   r' = r - A Z c1,  y = V(r'),  c2 = E^-1 Z^T A y,  z = y + Z (c1 - c2).
+// A Z c1 is nonzero on the cluster cells (alpha_s c1_a, alpha_s = (A 1_a)_s)
+// and on the cells just outside them (-sum of couplings x c1): the records,
+// built once per setup. Z^T A y sums, per cluster, alpha_s y_s minus the
+// couplings to outside cells times their y (fixed order, per piece).
+
+__device__ __forceinline__ int face_step(int f, int nx, int nxy)
+{
+    const int m = f >> 1, sgn = (f & 1) ? 1 : -1;
+    return sgn * (m == 0 ? 1 : m == 1 ? nx : nxy);
+}
+
+__device__ __forceinline__ bool face_in(int f, int i, int j, int k, const DefInst &in)
+{
+    switch (f) {
+    case 0: return i > 0;
+    case 1: return i + 1 < in.nx;
+    case 2: return j > 0;
+    case 3: return j + 1 < in.ny;
+    case 4: return k > 0;
+    default: return k + 1 < in.nz;
+    }
+}
+
+// coupling of global cell s to its neighbour across face f (the face exists)
+__device__ __forceinline__ double face_w(const Stencil<double> &A, long s, int f, int nx, int nxy)
+{
+    switch (f) {
+    case 0: return A.wx[s - 1];
+    case 1: return A.wx[s];
+    case 2: return A.wy[s - nx];
+    case 3: return A.wy[s];
+    case 4: return A.wz[s - nxy];
+    default: return A.wz[s];
+    }
+}
+
+// One thread per (listed cell, face) whose neighbour t is not strong: t is
+// recorded by the first of its own faces (in face order) that leads to a
+// cluster cell, so every outside cell gets exactly one record (the records'
+// order does not matter: each is used on its own).
+__global__ void __launch_bounds__(kBlock)
+k_bnn_records(const DefInst *__restrict__ I, Stencil<double> A, const int *__restrict__ cells,
+              const int *__restrict__ comp, int m, const int *__restrict__ list, const int *__restrict__ pms, int cap,
+              CcWork w, int *rec_cell, int *rec_cl, double *rec_w, int *rec_count, int rec_max)
+{
+    const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= 6L * m) return;
+    const int q = (int)(idx / 6), f = (int)(idx % 6);
+    const int ms = min(*pms, cap);
+    const DefInst in = I[comp[m + q]];
+    const int s = cells[q];
+    const int nx = in.nx, nxy = in.nx * in.ny;
+    int l = s - in.off, i = l % nx, jk = l / nx, j = jk % in.ny, k = jk / in.ny;
+    if (!face_in(f, i, j, k, in) || w.nbr[6L * w.lord[q] + f] >= 0) return;
+    const int t = s + face_step(f, nx, nxy);
+    l = t - in.off; i = l % nx; jk = l / nx; j = jk % in.ny; k = jk / in.ny;
+    auto cluster_of = [&](int u) {
+        const int b = sorted_find(list, 0, ms, u);
+        return b >= 0 ? w.cid[b] : -1;
+    };
+    const int own = f ^ 1;   // the face of t that leads back to s
+    for (int g = 0; g < own; g++)
+        if (face_in(g, i, j, k, in) && cluster_of(t + face_step(g, nx, nxy)) >= 0) return;
+    const int rk = atomicAdd(rec_count, 1);
+    if (rk >= rec_max) return;
+    rec_cell[rk] = t;
+    for (int g = 0; g < 6; g++) {
+        int cl = -1;
+        double wg = 0.0;
+        if (face_in(g, i, j, k, in)) {
+            cl = cluster_of(t + face_step(g, nx, nxy));
+            if (cl >= 0) wg = face_w(A, t, g, nx, nxy);
+        }
+        rec_cl[6L * rk + g] = cl;
+        rec_w[6L * rk + g] = wg;
+    }
+}
+
+// r' = r - A Z c1 in place (the overwritten values saved): listed cells
+// first, then the records.
+__global__ void __launch_bounds__(kBlock)
+k_bnn_pre(int m, const int *__restrict__ cells, const int *__restrict__ comp, const double *__restrict__ alpha,
+          const double *__restrict__ c, const int *__restrict__ rec_cell, const int *__restrict__ rec_cl,
+          const double *__restrict__ rec_w, const int *__restrict__ rec_count, int rec_max, double *r,
+          double *rsave, const int *done)
+{
+    if (*done) return;
+    const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (idx < m) {
+        const int q = (int)idx;
+        const long s = cells[q];
+        const double rv = r[s];
+        rsave[q] = rv;
+        r[s] = rv - alpha[q] * c[comp[q]];
+        return;
+    }
+    const long k = idx - m;
+    if (k >= min(*rec_count, rec_max)) return;
+    const long t = rec_cell[k];
+    const double rv = r[t];
+    rsave[m + k] = rv;
+    double a = rv;
+    for (int g = 0; g < 6; g++) {
+        const int cl = rec_cl[6 * k + g];
+        if (cl >= 0) a += rec_w[6 * k + g] * c[cl];
+    }
+    r[t] = a;
+}
+
+// r restored; blocks [0, np): per piece the sum of alpha_s y_s minus the
+// couplings of its cells to outside cells times their y (Z^T A y, fixed
+// order); the blocks after them restore the records.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_bnn_post(const DefInst *__restrict__ I, Stencil<double> A, int np, const int *__restrict__ pb,
+           const int *__restrict__ cells, const int *__restrict__ comp, int m, const double *__restrict__ alpha,
+           const int *__restrict__ lord, const int *__restrict__ nbr, const T *__restrict__ y,
+           const int *__restrict__ rec_cell, const int *__restrict__ rec_count, int rec_max,
+           const double *__restrict__ rsave, double *r, double *pe2, const int *done)
+{
+    if (*done) return;
+    if ((int)blockIdx.x >= np) {
+        const long k = (long)(blockIdx.x - np) * kBlock + threadIdx.x;
+        if (k < min(*rec_count, rec_max)) r[rec_cell[k]] = rsave[m + k];
+        return;
+    }
+    const int p = blockIdx.x;
+    double acc[1] = {0.0};
+    for (int q = pb[p] + threadIdx.x; q < pb[p + 1]; q += kBlock) {
+        const long s = cells[q];
+        r[s] = rsave[q];
+        const DefInst in = I[comp[m + q]];
+        const int nx = in.nx, nxy = in.nx * in.ny;
+        const int l = (int)s - in.off, i = l % nx, jk = l / nx, j = jk % in.ny, k = jk / in.ny;
+        double h = alpha[q] * (double)y[s];
+        const int *nb = nbr + 6L * lord[q];
+        for (int f = 0; f < 6; f++)
+            if (nb[f] < 0 && face_in(f, i, j, k, in))
+                h -= face_w(A, s, f, nx, nxy) * (double)y[s + face_step(f, nx, nxy)];
+        acc[0] += h;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) pe2[p] = acc[0];
+}
+
+// per cluster: g = Z^T A y, c2 = E^-1 g, c = c1 - c2 (for the scatter); the
+// extra dot partials of z = y + Z c: <z, r> - <y, r'> = c1.g + c.(Z^T r) and
+// <z, v> - <y, v> = c.(Z^T v)
+__global__ void __launch_bounds__(kBlock)
+k_bnn_apply(int nc, const int *__restrict__ cp, const double *__restrict__ einv, const double *__restrict__ psr,
+            const double *__restrict__ psv, const double *__restrict__ pe2, double *c, double *Pextra, const int *done)
+{
+    if (*done) return;
+    double acc[2] = {0.0, 0.0};
+    for (int a = threadIdx.x; a < nc; a += kBlock) {
+        double g = 0.0, sr = 0.0, sv = 0.0;
+        for (int p = cp[a]; p < cp[a + 1]; p++) { g += pe2[p]; sr += psr[p]; sv += psv[p]; }
+        const double c1 = c[a];
+        const double d = c1 - einv[a] * g;
+        c[a] = d;
+        acc[0] += c1 * g + d * sr;
+        acc[1] += d * sv;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) { Pextra[0] = acc[0]; Pextra[1] = acc[1]; }
+}
+
 // ---------------------------------------------------------------- host side
 
 // pinned host staging of at least `bytes` (grown on demand, freed with the context)
@@ -782,18 +955,29 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     // device buffers of the correction
     D.cells = (int *)alloc(actx, sizeof(int) * m, 2);
     D.comp = (int *)alloc(actx, sizeof(int) * 2 * (size_t)m, 3);   // cluster, then instance, of each cell
-    D.pb = (int *)alloc(actx, sizeof(int) * pc.size(), 4);
+    // the piece arrays, then the balancing form's records (cell, clusters of six faces, count)
+    const int rec_max = 6 * m;
+    D.pb = (int *)alloc(actx, sizeof(int) * (pc.size() + 1 + 7 * (size_t)rec_max), 4);
     D.einv = (double *)alloc(actx, sizeof(double) * nc, 5);
     const size_t per = (size_t)Deflation::kMaxRhs;
     D.c = (double *)alloc(actx, sizeof(double) * (per * (kMaxClusters + 2 * (size_t)np) + 2 * (size_t)ninst), 6);
-    double *share = (double *)alloc(actx, sizeof(double) * ((size_t)m + np), 7);
+    // alpha (m), per-piece E (np), per-piece Z^T A y (np), saved r (m + rec_max), record couplings (6 rec_max)
+    double *share = (double *)alloc(actx, sizeof(double) * ((size_t)m + 2 * (size_t)np + m + 7 * (size_t)rec_max), 7);
     if (!D.cells || !D.comp || !D.pb || !D.einv || !D.c || !share) return HMFEM_ERR_NOMEM;
     D.cp = D.pb + np + 1;
     D.irange = D.cp + nc + 1;
     D.cinst = D.irange + ninst + 1;
+    D.rec_count = D.pb + pc.size();
+    D.rec_cell = D.rec_count + 1;
+    D.rec_cl = D.rec_cell + rec_max;
     D.psr = D.c + per * kMaxClusters;
     D.psv = D.psr + per * np;
     D.pext = D.psv + per * np;
+    D.alpha = share;
+    D.pe2 = share + m + np;
+    D.rsave = D.pe2 + np;
+    D.rec_w = D.rsave + m + rec_max;
+    D.rec_max = rec_max;
     // chosen roots (kMaxClusters slots), cluster offsets, then the piece
     // arrays, up through the pinned staging (the table above is used up)
     const size_t sel_ints = 2 * (size_t)kMaxClusters + 1;
@@ -815,6 +999,14 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
                        share + m);
     hipLaunchKernelGGL(k_defl_einv, dim3(1), dim3(kBlock), 0, st, nc, (const int *)D.cp, (const double *)(share + m),
                        D.einv);
+    // the balancing form's records: the cells just outside the clusters
+    HMFEM_CHECK(hipMemsetAsync(D.rec_count, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_bnn_records, dim3((unsigned)((6L * m + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                       (const DefInst *)dinst, A, (const int *)D.cells, (const int *)D.comp, m, (const int *)slist,
+                       pms, cap, w, D.rec_cell, D.rec_cl, D.rec_w, D.rec_count, rec_max);
+    D.lord = w.lord;
+    D.nbr = w.nbr;
+    D.inst = dinst;
     HMFEM_CHECK(hipGetLastError());
     if (D.stage_done) {
         HMFEM_CHECK(hipEventRecord(D.stage_done, st));
@@ -949,6 +1141,40 @@ void deflation_scatter(hipStream_t st, const Deflation &D, float *z, const int *
 void deflation_scatter(hipStream_t st, const Deflation &D, double *z, const int *done)
 {
     defl_scatter<1, double>(st, D, z, done);
+}
+
+void deflation_pre(hipStream_t st, const Deflation &D, double *r, const int *done)
+{
+    const long n = (long)D.m + D.rec_max;
+    hipLaunchKernelGGL(k_bnn_pre, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, D.m,
+                       (const int *)D.cells, (const int *)D.comp, (const double *)D.alpha, (const double *)D.c,
+                       (const int *)D.rec_cell, (const int *)D.rec_cl, (const double *)D.rec_w,
+                       (const int *)D.rec_count, D.rec_max, r, D.rsave, done);
+}
+
+template <typename T>
+static void bnn_post(hipStream_t st, const Deflation &D, const Stencil<double> &A, double *r, const T *y,
+                     double *Pextra, const int *done)
+{
+    const unsigned G = (unsigned)(D.np + (D.rec_max + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((k_bnn_post<T>), dim3(G), dim3(kBlock), 0, st, (const DefInst *)D.inst, A, D.np,
+                       (const int *)D.pb, (const int *)D.cells, (const int *)D.comp, D.m, (const double *)D.alpha,
+                       D.lord, D.nbr, y, (const int *)D.rec_cell, (const int *)D.rec_count, D.rec_max,
+                       (const double *)D.rsave, r, D.pe2, done);
+    hipLaunchKernelGGL(k_bnn_apply, dim3(1), dim3(kBlock), 0, st, D.nc, (const int *)D.cp, (const double *)D.einv,
+                       (const double *)D.psr, (const double *)D.psv, (const double *)D.pe2, D.c, Pextra, done);
+}
+
+void deflation_post(hipStream_t st, const Deflation &D, const Stencil<double> &A, double *r, const float *y,
+                    double *Pextra, const int *done)
+{
+    bnn_post<float>(st, D, A, r, y, Pextra, done);
+}
+
+void deflation_post(hipStream_t st, const Deflation &D, const Stencil<double> &A, double *r, const double *y,
+                    double *Pextra, const int *done)
+{
+    bnn_post<double>(st, D, A, r, y, Pextra, done);
 }
 
 void deflation_sums_batch(hipStream_t st, const Deflation &D, const double *r, const double *v, const int *never)

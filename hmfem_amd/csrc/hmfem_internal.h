@@ -222,6 +222,16 @@ struct Deflation {
     int *irange = nullptr;                   // instance q owns clusters [irange[q], irange[q+1])
     int *cinst = nullptr;                    // instance of each cluster
     double *pext = nullptr;                  // batch: per-instance extra dot partials [instance][2]
+    // balancing form (single grid): r' = r - A Z c1 before the V-cycle, its
+    // output y projected after it (z = y + Z (c1 - E^-1 Z^T A y))
+    double *alpha = nullptr;                 // (A 1_a)_s of each listed cell s (e + couplings leaving a)
+    const int *lord = nullptr, *nbr = nullptr;   // strong ordinal of each listed cell; six strong neighbours
+    const void *inst = nullptr;              // the instance descriptors (coordinates of listed cells)
+    int *rec_cell = nullptr, *rec_cl = nullptr, *rec_count = nullptr;   // cells just outside the clusters
+    double *rec_w = nullptr;                 // their couplings to cluster cells [record][face] (cluster rec_cl)
+    double *rsave = nullptr;                 // r of the listed cells, then of the records, while r' is in r
+    double *pe2 = nullptr;                   // per-piece sums of (A Z)^T y
+    int rec_max = 0;                         // records room (6 m)
     static constexpr int kHistBins = 2048;   // bins of floor(log2 diagonal) + 1100
     static constexpr int kMaxRhs = 8;        // interleaved right-hand sides of the per-cluster buffers
     unsigned *h_hist = nullptr;              // pinned host copy of the diagonal histogram (context-owned)
@@ -251,6 +261,16 @@ int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Sten
 void deflation_sums_batch(hipStream_t st, const Deflation &D, const double *r, const double *v, const int *never);
 void deflation_scatter(hipStream_t st, const Deflation &D, float *z, const int *done);
 void deflation_scatter(hipStream_t st, const Deflation &D, double *z, const int *done);
+// The balancing form around the V-cycle of a single grid (after
+// deflation_sums): deflation_pre puts r' = r - A Z c1 into r (saving what it
+// overwrites), deflation_post restores r, turns c into c1 - E^-1 Z^T A y for
+// the V-cycle's output y and rewrites the extra dot partials for
+// z = y + Z c; deflation_scatter then adds Z c.
+void deflation_pre(hipStream_t st, const Deflation &D, double *r, const int *done);
+void deflation_post(hipStream_t st, const Deflation &D, const Stencil<double> &A, double *r, const float *y,
+                    double *Pextra, const int *done);
+void deflation_post(hipStream_t st, const Deflation &D, const Stencil<double> &A, double *r, const double *y,
+                    double *Pextra, const int *done);
 // The same for nr (2 or 4) interleaved right-hand sides ([cell][nr] vectors;
 // Pextra gets c.sr for each, then c.sv for each).
 int deflation_sums_block(hipStream_t st, const Deflation &D, int nr, const double *r, const double *v,

@@ -187,7 +187,11 @@ typedef struct {
     int deflate;             /* 1: add a coarse correction on each 7-point-connected cluster of at least
                                 8 cells whose diagonal exceeds deflate_ratio times the median diagonal
                                 (karst conduits); off when there are none (default 1; single, block and
-                                batched solves; in a batch each instance has its own threshold) */
+                                batched solves; in a batch each instance has its own threshold).
+                                2: single solves use the balancing form instead (the V-cycle on
+                                r - A Z c, its output projected onto the complement of the clusters'
+                                modes: fewer iterations, a larger error at a given residual); block
+                                and batched solves treat 2 as 1 */
     double deflate_ratio;    /* default 3000 */
     int precond;             /* HMFEM_PRECOND_AMG (default), or HMFEM_PRECOND_JACOBI: z = r / diag(A) in
                                 FP64 with no hierarchy, the plain FP64 PCG that the AMG is measured

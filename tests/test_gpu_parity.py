@@ -421,27 +421,30 @@ def test_graph_replay_matches_eager(smoother, ctx, torch):
         assert np.array_equal(p.cpu().numpy().view(np.uint64), ref.view(np.uint64)), rep
 
 
-def test_deflation_karst(ctx, torch):
-    """The cluster correction on a conduit field: same solution as without it
-    (and as the oracle) at 1e-12, fewer iterations at 1e-8, and a captured-graph
-    replay bit-identical to eager launches."""
+@pytest.mark.parametrize("mode", [1, 2], ids=["additive", "balancing"])
+def test_deflation_karst(mode, ctx, torch):
+    """The cluster correction on a conduit field (additive, the default, and
+    the balancing form): same solution as without it (and as the oracle) at
+    1e-12, fewer iterations at 1e-8, and a captured-graph replay
+    bit-identical to eager launches."""
     P = make_problem((64, 64, 48), "karst", seed=21, conduits=8)
-    _say("deflation karst")
+    _say("deflation karst, mode", mode)
     D = Dev(P, torch)
     p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
     its = {}
-    for dfl in (0, 1):
+    for dfl in (0, mode):
         p.zero_()
         st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None, S.default_options(rtol=1e-8, deflate=dfl))
         assert st.status == S.OK
         its[dfl] = st.iterations
-    assert its[1] < its[0], its
+    _say("iterations", its)
+    assert its[mode] < its[0], its
     xo, _, _, _ = oracle_solution(P, eps=1e-12)
     ref = None
     for use_graph in (0, 1, 1):
         p.zero_()
         st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None,
-                           S.default_options(rtol=1e-12, deflate=1, use_graph=use_graph))
+                           S.default_options(rtol=1e-12, deflate=mode, use_graph=use_graph))
         assert st.status == S.OK
         x = p.cpu().numpy()
         assert rel_l2(x, xo) < PRESSURE_TOL
