@@ -151,25 +151,39 @@ k_b_init_lane(OpArgs<double> o, const double *__restrict__ B, const double *__re
 template <int NR>
 __device__ __forceinline__ bool b_iteration_scalars(int G, int Gdot, BlockScalars *sc, const double *Pinit,
                                                     const double *Prr, const double *Pdot, int &k, double (&beta)[NR],
-                                                    int (&live)[NR], int (&fresh)[NR])
+                                                    int (&live)[NR])
 {
     // a lane's own iteration count kr = k - k0: kr = 0 for a right-hand side
     // started at this iteration (its b.b and r.r are in Pinit)
     k = sc->kspmv;
-    double dots[2 * NR], rrs[NR], bbs[NR], ir[2 * NR], t[NR];
+    double dots[2 * NR], rrs[NR], bbs[NR];
     sum_partials<2 * NR>(Pdot, Gdot, dots);
-    int any_fresh = 0;
+    int fresh[NR];
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-        fresh[r] = !sc->done[r] && k == sc->k0[r];
-        any_fresh |= fresh[r];
-    }
-    if (any_fresh) sum_partials<2 * NR>(Pinit, G, ir);
-    if (k > 0) sum_partials<NR>(Prr, G, t);
+    for (int r = 0; r < NR; r++) fresh[r] = !sc->done[r] && k == sc->k0[r];
+    {
+        // per lane r.r (and b.b of a fresh lane) from its own partials: Pinit
+        // for a fresh lane, Prr otherwise, in one pass (few registers live)
+        double acc[2 * NR];
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-        bbs[r] = fresh[r] ? ir[r] : sc->bb[r];
-        rrs[r] = fresh[r] ? ir[NR + r] : (k > 0 ? t[r] : 0.0);
+        for (int q = 0; q < 2 * NR; q++) acc[q] = 0.0;
+        for (int b = threadIdx.x; b < G; b += kBlock) {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                if (fresh[r]) {
+                    acc[r] += Pinit[(long)2 * NR * b + NR + r];
+                    acc[NR + r] += Pinit[(long)2 * NR * b + r];
+                } else if (k > 0) {
+                    acc[r] += Prr[(long)NR * b + r];
+                }
+            }
+        }
+        block_sum<2 * NR>(acc);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            rrs[r] = acc[r];
+            bbs[r] = fresh[r] ? acc[NR + r] : sc->bb[r];
+        }
     }
     int all = 1;
     const bool writer = xcd_block() == 0 && threadIdx.x == 0;
@@ -209,6 +223,18 @@ __device__ __forceinline__ bool b_iteration_scalars(int G, int Gdot, BlockScalar
     return all != 0;
 }
 
+// The stopping tests and beta of block CG iteration sc->kspmv (one block).
+template <int NR>
+__global__ void __launch_bounds__(kBlock)
+k_b_scalars(int G, int Gdot, BlockScalars *sc, const double *Pinit, const double *Prr, const double *Pdot)
+{
+    if (sc->all_done) return;
+    int k;
+    double beta[NR];
+    int live[NR];
+    (void)b_iteration_scalars<NR>(G, Gdot, sc, Pinit, Prr, Pdot, k, beta, live);
+}
+
 // Block CG SpMV marching along z (as the single-RHS kernel): s = z + beta s_old
 // for every live right-hand side (0 for finished ones), v = A s, partials
 // <s, v>_r per logical block (Ptau[lb][NR]; blocks past the last tile write 0).
@@ -218,11 +244,19 @@ k_b_spmv_z(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S
            double *__restrict__ V, int zlen, int G, int Gdot, BlockScalars *sc, const double *Pinit,
            const double *Prr, const double *Pdot, double *Ptau)
 {
+    // the iteration's scalars come from k_b_scalars (one block, just before):
+    // uniform loads, so beta and the live flags sit in scalar registers (the
+    // stopping tests in every block held 135 VGPRs: three waves per SIMD)
     if (sc->all_done) return;
-    int k;
+    (void)G; (void)Gdot; (void)Pinit; (void)Prr; (void)Pdot;
+    const int k = sc->kspmv;
     double beta[NR];
-    int live[NR], fresh[NR];
-    if (b_iteration_scalars<NR>(G, Gdot, sc, Pinit, Prr, Pdot, k, beta, live, fresh)) return;
+    int live[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        live[r] = !sc->done[r];
+        beta[r] = live[r] ? sc->beta[r] : 0.0;
+    }
     const double *__restrict__ sin = (k & 1) ? S1 : S0;
     double *__restrict__ sout = (k & 1) ? S0 : S1;
     __shared__ double sh[3][kGsTy + 2][kGsTx + 2][NR];
@@ -255,13 +289,15 @@ k_b_spmv_z(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S
 #pragma unroll
             for (int r = 0; r < NR; r++) {
                 double a = 0.0, b = 0.0;
+                // a lane that restarts has beta 0, and its s_old is 0: it
+                // was idle (s = 0) on the iteration before its restart
                 if (in && col && live[r]) {
                     const long q = (i + (long)j * nx + (long)kk * nxy) * NR + r;
-                    a = (double)Z[q] + (fresh[r] ? 0.0 : beta[r] * sin[q]);
+                    a = (double)Z[q] + (k ? beta[r] * sin[q] : 0.0);
                 }
                 if (in && ring_in && live[r]) {
                     const long q = (ri + (long)rj * nx + (long)kk * nxy) * NR + r;
-                    b = (double)Z[q] + (fresh[r] ? 0.0 : beta[r] * sin[q]);
+                    b = (double)Z[q] + (k ? beta[r] * sin[q] : 0.0);
                 }
                 sh[slot][ty + 1][tx + 1][r] = a;
                 if (has_ring) sh[slot][ry][rx][r] = b;
@@ -374,9 +410,11 @@ constexpr int kBThreads = kGsTx * kGsTy * (kBTz / kBCz);
 // neighbour across their boundary is read from LDS. MODE 0: from zero;
 // MODE 1: from u + ocf * uc[aggregate] (uo must not alias u); MODE 2: MODE 1
 // plus the CG dot partials <z, f>_r, <z, v>_r of this block's tiles
-// (Pdot[lb][2 NR]). The tile's coefficients are loaded once; the right-hand
-// sides are smoothed NP at a time (four at once held 169 VGPRs in MODE 2:
-// two waves per SIMD).
+// (Pdot[lb][2 NR]). The tile's coefficients are loaded once and the
+// right-hand sides smoothed NP at a time: two at a time gives four waves per
+// SIMD (128 VGPRs with a small spill) but fetches the interleaved lines of
+// f, u and v once per pass; all four at once (two waves) was faster on the
+// 256^3 block workload (9.79 against 9.57 right-hand sides/s).
 template <int NR, typename FT, int MODE, int CZ = kBCz, int NTHREADS = kGsTx * kGsTy * (kBTz / CZ),
           int NP = NR>
 __global__ void __launch_bounds__(NTHREADS)
@@ -863,6 +901,8 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
     int kcap = opt.max_iterations + every;
     while (true) {
         for (int q = 0; q < every; q++, k++) {
+            hipLaunchKernelGGL(k_b_scalars<NR>, dim3(1), dim3(kBlock), 0, st, G, Gdot, S.sc,
+                               (const double *)S.Pinit, (const double *)S.Prr, (const double *)S.Pdot);
             hipLaunchKernelGGL(k_b_spmv_z<NR>, dim3(Gz), dim3(kBlock), 0, st, o, (const float *)S.Z, S.S0, S.S1,
                                S.V, zlen, G, Gdot, S.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
             hipLaunchKernelGGL(k_b_update<NR>, dim3(G), dim3(kBlock), 0, st, n, S.S0, S.S1, S.V, S.X, S.R,
