@@ -974,21 +974,22 @@ k_cg_init(OpArgs<double> o, const double *__restrict__ b, const double *__restri
 // Start of a CG iteration k (read from the device): the stopping tests on
 // the partial sums of the previous kernels, beta (Polak-Ribiere form), and the
 // scalars written by one thread. Returns true if the solve stops here.
+template <int NT = kBlock>
 __device__ __forceinline__ bool cg_iteration_scalars(int G, int Gdot, CgScalars *sc, const double *Pinit,
                                                      const double *Prr, const double *Pdot, int &k, double &beta)
 {
     k = sc->kspmv;
     double dots[2];
-    sum_partials<2>(Pdot, Gdot, dots);
+    sum_partials_n<2, NT>(Pdot, Gdot, dots);
     double bb, rr;
     if (k == 0) {
         double ir[2];
-        sum_partials<2>(Pinit, G, ir);
+        sum_partials_n<2, NT>(Pinit, G, ir);
         bb = ir[0];
         rr = ir[1];
     } else {
         double t[1];
-        sum_partials<1>(Prr, G, t);
+        sum_partials_n<1, NT>(Prr, G, t);
         rr = t[0];
         bb = sc->bb;
     }
@@ -1019,20 +1020,33 @@ __device__ __forceinline__ bool cg_iteration_scalars(int G, int Gdot, CgScalars 
     return stop != 0;
 }
 
+// The stopping tests and beta of CG iteration sc->kspmv (one block; the
+// partials summed in the order every SpMV block used before, so the iterates
+// are the same: with 1024 threads the 256^3 conduit case took 71 iterations
+// instead of 69, rounding alone).
+constexpr int kScalarThreads = kBlock;
+__global__ void __launch_bounds__(kScalarThreads)
+k_cg_scalars(int G, int Gdot, CgScalars *sc, const double *Pinit, const double *Prr, const double *Pdot)
+{
+    if (sc->done) return;
+    int k;
+    double beta;
+    (void)cg_iteration_scalars<kScalarThreads>(G, Gdot, sc, Pinit, Prr, Pdot, k, beta);
+}
+
 template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
 k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
           double *__restrict__ v, int chunk, int G, int Gdot, CgScalars *sc, const double *Pinit, const double *Prr,
           const double *Pdot, double *Ptau)
 {
-    if (sc->done) return;
+    if (sc->done) return;   // the iteration's scalars: k_cg_scalars, just before
+    (void)Gdot; (void)Pinit; (void)Prr; (void)Pdot;
     const int n = o.nx * o.ny * o.nz;
-    int k;
-    double beta;
-    const bool stop = cg_iteration_scalars(G, Gdot, sc, Pinit, Prr, Pdot, k, beta);
+    const int k = sc->kspmv;
+    const double beta = sc->beta;
     const double *__restrict__ sin = (k & 1) ? s1 : s0;
     double *__restrict__ sout = (k & 1) ? s0 : s1;
-    if (stop) return;
     int lb, beg, end;
     chunk_range(n, chunk, lb, beg, end);
     double acc[1] = {0.0};
@@ -1082,11 +1096,12 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
             double *__restrict__ v, double *__restrict__ x, int zlen, int G, int Gdot, CgScalars *sc,
             const double *Pinit, const double *Prr, const double *Pdot, double *Ptau, int eface)
 {
+    // the iteration's scalars come from k_cg_scalars (one block, just before):
+    // uniform loads instead of every block summing the partials again
     if (sc->done) return;
-    int k;
-    double beta;
-    const bool stop = cg_iteration_scalars(G, Gdot, sc, Pinit, Prr, Pdot, k, beta);
-    if (stop) return;
+    (void)G; (void)Gdot; (void)Pinit; (void)Prr; (void)Pdot;
+    const int k = sc->kspmv;
+    const double beta = sc->beta;
     const double *__restrict__ sin = (k & 1) ? s1 : s0;
     double *__restrict__ sout = (k & 1) ? s0 : s1;
     const double alpha_prev = k ? sc->alpha[(k - 1) & 1] : 0.0;
@@ -1837,6 +1852,8 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     // One CG iteration; identical launches every iteration (the index and the
     // buffer parity are read from the device).
     auto iteration = [&](Prof *PP) -> T * {
+        PROF_LAUNCH(PP, HMFEM_PROF_CG_DOTS, k_cg_scalars, dim3(1), dim3(kScalarThreads), 0, st, G, Gdot, B.sc,
+                    (const double *)S.Pinit, (const double *)S.Prr, (const double *)S.Pdot);
         if (zlen > 0)
             PROF_LAUNCH(PP, HMFEM_PROF_CG_SPMV, k_cg_spmv_z<T>, dim3(Gz), dim3(kBlock), 0, st, o, (const T *)z, B.s0,
                         B.s1, B.v, x, zlen, G, Gdot, B.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau, (int)A.eface);

@@ -130,6 +130,16 @@ __device__ __forceinline__ void sum_partials(const double *P, int G, double (&ou
     block_sum<NV>(out);
 }
 
+// sum_partials for a block of NT threads
+template <int NV, int NT>
+__device__ __forceinline__ void sum_partials_n(const double *P, int G, double (&out)[NV])
+{
+    for (int q = 0; q < NV; q++) out[q] = 0.0;
+    for (int b = threadIdx.x; b < G; b += NT)
+        for (int q = 0; q < NV; q++) out[q] += P[(long)NV * b + q];
+    block_sum_n<NV, NT>(out);
+}
+
 // Contiguous range of cells of this (logical) block in a chunked launch.
 __device__ __forceinline__ void chunk_range(int n, int chunk, int &lb, int &beg, int &end)
 {
