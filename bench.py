@@ -601,6 +601,7 @@ def main():
     avg_ms = d["ms"] / d["count"]
     achieved = d["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
     kernels = {k: {"avg_us": 1e3 * x["ms"] / x["count"], "launches_per_step": x["count"] / prof_steps,
+                   "bytes_per_launch": x["bytes_per_launch"] or None,
                    "gbps": x["bytes_per_launch"] / (x["ms"] / x["count"] * 1e-3) / 1e9 if x["bytes_per_launch"] else None}
                for k, x in prof.items() if x["count"] > 0}
     traffic = None

@@ -23,9 +23,11 @@
 namespace hmfem {
 
 constexpr int kMaxRhs = 8;
-// (The MODE 2 instance holds 169 VGPRs, one block per CU; two cells per
-// thread with 1024-thread blocks, or a 128-VGPR cap, spill 272-580 bytes per
-// lane.)
+// (The MODE 2 instance holds 205 VGPRs, one 512-thread block per CU. Two cells
+// per thread with 1024-thread blocks, or a 128-VGPR cap, spill 272-580 bytes
+// per lane; fewer right-hand sides per pass (two: 137 VGPRs, one: 101) with
+// the dot partials summed per wave in LDS ran the 256^3 block solve in 854
+// and 921 ms against 794.)
 // z extent of a block-smoother tile: 32 x 8 x 8 tiles, two threads per
 // column. At 256^3 with four right-hand sides a CG iteration takes 3.19 ms
 // against 3.05 for 32 x 8 x 4 tiles (one thread per column), 3.82 (32 x 8 x 4,
@@ -486,6 +488,18 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
 #pragma unroll
                 for (int r = 0; r < NP; r++) ff[kk][r] = act ? (float)f[s + r] : 0.f;
             }
+            // MODE 2: v for <z, v>, loaded with the pass's other values and held
+            // through the sweeps (205 VGPRs, still two waves per SIMD; loaded
+            // after the sweeps its latency showed: 808 -> 794 ms per 256^3 block solve)
+            double vv[MODE == 2 ? CZ : 1][NP];
+            if constexpr (MODE == 2)
+#pragma unroll
+                for (int kk = 0; kk < CZ; kk++) {
+                    const bool act = v && col && kc + kk < nz;
+                    const unsigned s = (unsigned)(s0 + kk * nxy) * NR + rb;
+#pragma unroll
+                    for (int r = 0; r < NP; r++) vv[kk][r] = act ? v[s + r] : 0.0;
+                }
             float zlo[NP], zhi[NP], ring[RPER][NP];
 #pragma unroll
             for (int kk = 0; kk < CZ; kk++) start(i, j, kc + kk, rb, uu[kk]);
@@ -554,7 +568,7 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
                             // <z, r> from the FP32 copy of r this pass already holds: the
                             // FP64 r of the tile is gone from L2 by now, and z itself is FP32
                             acc[rb + r] += (double)uu[kk][r] * (double)ff[kk][r];
-                            if (v) acc[NR + rb + r] += (double)uu[kk][r] * v[s + r];
+                            acc[NR + rb + r] += (double)uu[kk][r] * vv[kk][r];
                         }
                     }
                 }

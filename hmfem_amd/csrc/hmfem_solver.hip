@@ -2092,7 +2092,7 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         const double er = A.eface && H.lev[0].fx == 2 && H.lev[0].fy == 2 && H.lev[0].fz == 2 ? vs * (n0 - nin) : vs * n0;
         sink->bytes[HMFEM_PROF_CG_SPMV] = e64 + n0 * (24 + 8 + vs + 8 + 8 + xb);
         sink->bytes[HMFEM_PROF_CG_UPDATE] = n0 * (48 - 1.5 * xb);           // r, v in; r out (+ x, s in; x out)
-        sink->bytes[HMFEM_PROF_CG_DOTS] = 0.0;                               // fused into the last sweep
+        sink->bytes[HMFEM_PROF_CG_DOTS] = 0.0;                               // one workgroup over the partial sums
         sink->bytes[HMFEM_PROF_SMOOTH0] = n0 * (c4 + 8 + vs);                // A, f, u out
         sink->bytes[HMFEM_PROF_RESTRICT] = er + n0 * (c4 - vs + 8 + vs) + n1 * vs;   // A, f, u; f_c out
         sink->bytes[HMFEM_PROF_PROLONG] = n0 * (c4 + 8 + 2 * vs) + n1 * vs;  // A, f, u, u_c; u out
