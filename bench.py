@@ -397,6 +397,8 @@ def run_block(args, w, torch, S):
     for name, val in (("gs_sweeps", args.gs_sweeps), ("ocf", args.ocf)):
         if val is not None:
             setattr(opt, name, val)
+    if args.no_compact:
+        opt.block_compact = 0
 
     def step():
         with torch.cuda.stream(solver_stream):
@@ -469,7 +471,8 @@ def run_block(args, w, torch, S):
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
         "config": {"workload": w["label"], "cells": n, "right_hand_sides": nr, "rtol": 1e-8,
-                   "precision": "FP32 AMG in FP64 flexible CG, %d right-hand sides through 4 interleaved lanes" % nr},
+                   "precision": "FP32 AMG in FP64 flexible CG, %d right-hand sides through 4 interleaved lanes%s"
+                   % (nr, "" if args.no_compact else ", 2 once no right-hand side is left to start")},
         "iterations": its.tolist(), "block_iterations": st.block_iterations,
         "max_rel_residual": float(per["rel_residual"].max()),
         "div_max": per["div_max"].tolist(),
@@ -494,6 +497,8 @@ def main():
                     help="smoother of the multi-block levels (default: the library's)")
     ap.add_argument("--gs-sweeps", type=int, default=None, help="block Gauss-Seidel sweeps (default: the library's)")
     ap.add_argument("--ocf", type=float, default=None, help="coarse-correction scaling (default: the library's)")
+    ap.add_argument("--no-compact", action="store_true",
+                    help="c5: keep four lanes to the end (no switch to two once the feed is empty)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--profile-only", action="store_true", help="timed steps only (for rocprofv3 runs)")

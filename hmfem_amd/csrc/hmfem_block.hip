@@ -752,6 +752,41 @@ k_b_get(long n, int nr, int r, const double *__restrict__ Bk, double *__restrict
     a[s] = Bk[s * nr + r];
 }
 
+// dst[s][0..m) <- src[s][lanes[0..m)] for m (1 or 2) lanes of a four-lane
+// block (the compaction to two lanes; a lane of dst without a source is 0).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_b_pack2(long n, const T *__restrict__ src, int la, int lb, T *__restrict__ dst)
+{
+    long s = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    dst[2 * s] = la >= 0 ? src[4 * s + la] : (T)0;
+    dst[2 * s + 1] = lb >= 0 ? src[4 * s + lb] : (T)0;
+}
+
+// The dot partials of a four-lane block for two of its lanes: rows of w
+// values per lane group ([row][g][4] -> [row][g][2], g < groups).
+__global__ void __launch_bounds__(kBlock)
+k_b_pack2_partials(int rows, int groups, const double *__restrict__ src, int la, int lb, double *__restrict__ dst)
+{
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= rows * groups) return;
+    const int row = q / groups, g = q % groups;
+    const double *a = src + (long)row * groups * 4 + g * 4;
+    double *b = dst + (long)row * groups * 2 + g * 2;
+    b[0] = la >= 0 ? a[la] : 0.0;
+    b[1] = lb >= 0 ? a[lb] : 0.0;
+}
+
+// Lane r of a two-lane block -> lane q of a four-lane block.
+__global__ void __launch_bounds__(kBlock)
+k_b_unpack2(long n, const double *__restrict__ src, int r, double *__restrict__ dst, int q)
+{
+    long s = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    dst[4 * s + q] = src[2 * s + r];
+}
+
 // ---------------------------------------------------------------- host side
 
 struct BlockLevelVecs {
@@ -885,9 +920,20 @@ size_t block_bytes(Dims fine, int nr, const hmfem_options &opt)
     return b;
 }
 
+// block_cg<4> returns this (an internal code) when the feed is empty and at
+// most two lanes are still busy: the solve goes on with two lanes, whose
+// kernels move half the bytes per iteration (a waiting lane of the four
+// still moves its interleaved values).
+constexpr int kBlockCompact = 1 << 20;
+
+// Resume (resume = true): the state is that of a poll (S.Z and the dot
+// partials current, h_sc the host copy of the scalars); lane r of this block
+// is lane lane_map[r] of the caller's four-lane Xk4, into which a finished
+// lane's x is copied before feed->done(lane_map[r], ...).
 template <int NR>
 static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d, BlockState &S,
-                    const hmfem_options &opt, BlockScalars *h_sc, const double *Bk, BlockFeed *feed, int *kdone)
+                    const hmfem_options &opt, BlockScalars *h_sc, const double *Bk, BlockFeed *feed, int *kdone,
+                    bool resume = false, const int *lane_map = nullptr, double *Xk4 = nullptr, int *busy_out = nullptr)
 {
     const int n = (int)d.n();
     OpArgs<double> o;
@@ -901,18 +947,22 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
     int zlen = 4;
     while ((long)ntxy * ((d.nz + zlen - 1) / zlen) > kMaxReduceBlocks) zlen++;
     const int Gz = grid_for((long)ntxy * ((d.nz + zlen - 1) / zlen), 1);
-    int rc = b_vcycle_fine<NR>(st, H, S, nullptr, opt);
-    if (rc) return rc;
-    HMFEM_CHECK(hipGetLastError());
-    int k = 0;
+    int rc = HMFEM_OK;
+    if (!resume) {
+        rc = b_vcycle_fine<NR>(st, H, S, nullptr, opt);
+        if (rc) return rc;
+        HMFEM_CHECK(hipGetLastError());
+    }
     const int every = opt.check_every < 1 ? 1 : opt.check_every;
+    int k = resume ? h_sc->kspmv : 0;
     // lanes refilled from the feed: a finished lane hands its result over and
     // takes the next right-hand side (its own k0, its residual from
     // k_b_init_lane), and the V-cycle is run again for every lane so that the
     // new one has z and <z, r> (the others get the same values again)
     bool busy[kMaxRhs] = {};
-    for (int r = 0; r < NR; r++) busy[r] = feed != nullptr;
-    int kcap = opt.max_iterations + every;
+    for (int r = 0; r < NR; r++) busy[r] = feed != nullptr && (!resume || !h_sc->done[r]);
+    bool feed_empty = resume;
+    int kcap = k + opt.max_iterations + every;
     while (true) {
         for (int q = 0; q < every; q++, k++) {
             hipLaunchKernelGGL(k_b_scalars<NR>, dim3(1), dim3(kBlock), 0, st, G, Gdot, S.sc,
@@ -932,9 +982,18 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
             for (int r = 0; r < NR; r++) {
                 if (!busy[r] || !h_sc->done[r]) continue;
                 const double rel = h_sc->bb[r] > 0 ? sqrt(h_sc->rr[r] / h_sc->bb[r]) : 0.0;
-                feed->done(r, h_sc->iter[r], rel, h_sc->status[r]);
+                if (lane_map) {
+                    hipLaunchKernelGGL(k_b_unpack2, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                                       (long)n, (const double *)S.X, r, Xk4, lane_map[r]);
+                    feed->done(lane_map[r], h_sc->iter[r], rel, h_sc->status[r]);
+                } else {
+                    feed->done(r, h_sc->iter[r], rel, h_sc->status[r]);
+                }
                 busy[r] = false;
-                if (!feed->load(r)) continue;
+                if (feed_empty || !feed->load(r)) {
+                    feed_empty = true;
+                    continue;
+                }
                 hipLaunchKernelGGL(k_b_init_lane<NR>, dim3(G), dim3(kBlock), 0, st, o, Bk, (const double *)S.X, S.R,
                                    r, S.ch.chunk, S.Pinit);
                 h_sc->done[r] = 0;
@@ -953,10 +1012,89 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
                 HMFEM_CHECK(hipGetLastError());
                 continue;
             }
+            int nbusy = 0;
+            for (int r = 0; r < NR; r++) nbusy += busy[r];
+            if (NR == 4 && busy_out && feed_empty && nbusy > 0 && nbusy <= 2 && !h_sc->all_done) {
+                int m = 0;
+                busy_out[0] = busy_out[1] = -1;
+                for (int r = 0; r < NR; r++)
+                    if (busy[r]) busy_out[m++] = r;
+                if (kdone) *kdone = h_sc->kspmv;
+                return kBlockCompact;
+            }
         }
         if (h_sc->all_done || k > kcap) break;
     }
     if (kdone) *kdone = h_sc->kspmv;
+    return HMFEM_OK;
+}
+
+// Continue a four-lane solve stopped by kBlockCompact with its busy lanes
+// lanes[0..1] (-1: none) as a two-lane solve. The two-lane vectors take the
+// space of four-lane ones whose values are not needed at a poll: R and the
+// search direction into V and the spare S buffer, X and Z into R. On return
+// X (Xk, four lanes) holds both lanes' results and hs the four-lane scalars.
+static int block_compact(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d, BlockState &S,
+                         const hmfem_options &opt, BlockScalars &hs, BlockFeed *feed, int *kdone, const int *lanes)
+{
+    const long n = d.n();
+    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+    const int la = lanes[0], lb = lanes[1];
+    const int k = hs.kspmv;                                   // the next iteration
+    double *sin4 = (k & 1) ? S.S1 : S.S0, *spare = (k & 1) ? S.S0 : S.S1;
+    double *R2 = S.V, *V2 = S.V + 2 * n;
+    double *S0_2 = spare, *S1_2 = spare + 2 * n;
+    double *X2 = S.R;
+    float *Z2 = (float *)(S.R + 2 * n);
+    hipLaunchKernelGGL(k_b_pack2<double>, dim3(nb), dim3(kBlock), 0, st, n, (const double *)S.R, la, lb, R2);
+    hipLaunchKernelGGL(k_b_pack2<double>, dim3(nb), dim3(kBlock), 0, st, n, (const double *)sin4, la, lb,
+                       (k & 1) ? S1_2 : S0_2);
+    hipLaunchKernelGGL(k_b_pack2<double>, dim3(nb), dim3(kBlock), 0, st, n, (const double *)S.X, la, lb, X2);
+    hipLaunchKernelGGL(k_b_pack2<float>, dim3(nb), dim3(kBlock), 0, st, n, (const float *)S.Z, la, lb, Z2);
+    const int G = S.ch.G;
+    const int Gdot = S.D && S.D->nc > 0 ? G + 1 : G;
+    // partials of the next iteration's scalars: <z, r>, <z, v> rows (Pdot) and r.r rows (Prr)
+    hipLaunchKernelGGL(k_b_pack2_partials, dim3((2 * Gdot + kBlock - 1) / kBlock), dim3(kBlock), 0, st, Gdot, 2,
+                       (const double *)S.Pdot, la, lb, S.Ptau);
+    hipLaunchKernelGGL(k_b_pack2_partials, dim3((G + kBlock - 1) / kBlock), dim3(kBlock), 0, st, G, 1,
+                       (const double *)S.Prr, la, lb, S.Pinit);
+    HMFEM_CHECK(hipGetLastError());
+    BlockState S2 = S;
+    S2.nr = 2;
+    S2.R = R2; S2.V = V2; S2.S0 = S0_2; S2.S1 = S1_2; S2.X = X2; S2.Z = Z2;
+    S2.Pdot = S.Ptau; S2.Ptau = S.Pdot; S2.Prr = S.Pinit; S2.Pinit = S.Prr;
+    BlockScalars h2 = hs;
+    const int map[2] = {la, lb};
+    for (int r = 0; r < kMaxRhs; r++) {
+        const int q = r < 2 ? map[r] : -1;
+        if (q < 0) {
+            h2.done[r] = 1; h2.status[r] = HMFEM_OK; h2.iter[r] = 0; h2.k0[r] = 0;
+            h2.bb[r] = h2.rr[r] = h2.sigma_first[r] = h2.beta[r] = h2.tau[r] = 0.0;
+            h2.sigma[0][r] = h2.sigma[1][r] = h2.alpha[0][r] = h2.alpha[1][r] = 0.0;
+            continue;
+        }
+        h2.done[r] = hs.done[q]; h2.status[r] = hs.status[q]; h2.iter[r] = hs.iter[q]; h2.k0[r] = hs.k0[q];
+        h2.bb[r] = hs.bb[q]; h2.rr[r] = hs.rr[q]; h2.sigma_first[r] = hs.sigma_first[q];
+        h2.beta[r] = hs.beta[q]; h2.tau[r] = hs.tau[q];
+        for (int p = 0; p < 2; p++) { h2.sigma[p][r] = hs.sigma[p][q]; h2.alpha[p][r] = hs.alpha[p][q]; }
+    }
+    h2.nr = 2;
+    h2.all_done = 0;
+    HMFEM_CHECK(hipMemcpyAsync(S.sc, &h2, sizeof(BlockScalars), hipMemcpyHostToDevice, st));
+    int rc = block_cg<2>(st, H, A, d, S2, opt, &h2, nullptr, feed, kdone, true, map, S.X, nullptr);
+    if (rc) return rc;
+    // lanes still busy at the end (capped): their x back into the four-lane block
+    for (int r = 0; r < 2; r++) {
+        const int q = map[r];
+        if (q < 0) continue;
+        if (!h2.done[r])
+            hipLaunchKernelGGL(k_b_unpack2, dim3(nb), dim3(kBlock), 0, st, n, (const double *)X2, r, S.X, q);
+        hs.done[q] = h2.done[r]; hs.status[q] = h2.status[r]; hs.iter[q] = h2.iter[r];
+        hs.bb[q] = h2.bb[r]; hs.rr[q] = h2.rr[r];
+    }
+    hs.kspmv = h2.kspmv;
+    hs.all_done = h2.all_done;
+    HMFEM_CHECK(hipGetLastError());
     return HMFEM_OK;
 }
 
@@ -1044,10 +1182,14 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
         hipLaunchKernelGGL(k_b_init<2>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
         rc = block_cg<2>(st, H, A, d, S, opt, &hs, Bk, feed, block_iterations);
         break;
-    default:
+    default: {
         hipLaunchKernelGGL(k_b_init<4>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
-        rc = block_cg<4>(st, H, A, d, S, opt, &hs, Bk, feed, block_iterations);
+        int lanes[2] = {-1, -1};
+        rc = block_cg<4>(st, H, A, d, S, opt, &hs, Bk, feed, block_iterations, false, nullptr, nullptr,
+                         opt.block_compact ? lanes : nullptr);
+        if (rc == kBlockCompact) rc = block_compact(st, H, A, d, S, opt, hs, feed, block_iterations, lanes);
         break;
+    }
     }
     if (rc) return rc;
     HMFEM_CHECK(hipEventRecord(e2, st));

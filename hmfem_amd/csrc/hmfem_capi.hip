@@ -139,13 +139,14 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->use_graph = 1;
     o->tail_lds = 1;
     o->smoother = HMFEM_SMOOTH_BLOCK_GS;
-    o->gs_sweeps = 0;   // the path's own: 2 on a single grid, 3 in batched and block solves
+    o->gs_sweeps = 0;   // the path's own: 2 in single and block solves, 3 in batched solves
     o->autotune = 0;
     o->deflate = 1;
     o->deflate_ratio = 3000.0;
     o->precond = HMFEM_PRECOND_AMG;
     o->reuse_setup = 0;
     o->semicoarsen = 1;
+    o->block_compact = 1;
 }
 
 extern "C" int hmfem_create(hmfem_context **out)

@@ -62,7 +62,8 @@ class Options(ctypes.Structure):
                 ("nu", c_int), ("omega", c_dbl), ("ocf", c_dbl), ("coarse_max", c_int), ("check_every", c_int),
                 ("tail_max", c_int), ("tail_gamma", c_int), ("use_graph", c_int), ("tail_lds", c_int),
                 ("smoother", c_int), ("gs_sweeps", c_int), ("autotune", c_int), ("deflate", c_int),
-                ("deflate_ratio", c_dbl), ("precond", c_int), ("reuse_setup", c_int), ("semicoarsen", c_int)]
+                ("deflate_ratio", c_dbl), ("precond", c_int), ("reuse_setup", c_int), ("semicoarsen", c_int),
+                ("block_compact", c_int)]
 
 
 class Status(ctypes.Structure):

@@ -206,6 +206,9 @@ typedef struct {
                                 strongest axis's (geometric means over the fine operator): anisotropic cells
                                 or K get boxes of 2x1x1, 2x2x1, ... until the couplings even out; isotropic
                                 problems keep 2x2x2 boxes (single solves; default 1) */
+    int block_compact;       /* 1: a block solve through four lanes goes on with two once no right-hand
+                                side is left to start and at most two lanes are busy (their vectors are
+                                repacked; the kernels then move half the bytes per iteration); default 1 */
 } hmfem_options;
 
 typedef struct {
