@@ -1086,10 +1086,7 @@ k_cg_spmv(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, d
 // k_cg_update leaves to it (s_old is read here anyway): that saves reading s
 // again and a read-modify-write pass over x per iteration. If the solve stops
 // at this kernel, k_cg_finish_x applies the pending update instead.
-#ifndef HMFEM_SPMV_SLOTS
-#define HMFEM_SPMV_SLOTS 4
-#endif
-constexpr int kSpmvSlots = HMFEM_SPMV_SLOTS;
+constexpr int kSpmvSlots = 4;   // LDS planes of s (three need a second barrier per plane)
 template <typename ZT>
 __global__ void __launch_bounds__(kBlock)
 k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0, double *__restrict__ s1,
@@ -1219,10 +1216,7 @@ k_cg_spmv_z(OpArgs<double> o, const ZT *__restrict__ z, double *__restrict__ s0,
 // (loading x at the store instead, 82 VGPRs: 35.2 / 275 us; beta and
 // alpha moved to scalar registers and z kept in FP32 until the LDS store,
 // 78 VGPRs and 6 waves: 33.9 / 282 us, so more waves are slower here).
-#ifndef HMFEM_SPMV_BLOCKS
-#define HMFEM_SPMV_BLOCKS 1024
-#endif
-constexpr int kMaxSpmvBlocks = HMFEM_SPMV_BLOCKS;
+constexpr int kMaxSpmvBlocks = 1024;
 
 // Planes per block of k_cg_spmv_z so that it needs at most maxb blocks (at
 // least 4 planes each); 0 if a single plane has more tiles than that.
