@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <string>
 #include <thread>
 #include <vector>
@@ -70,15 +71,29 @@ std::vector<std::string> slabs(int count, F body)
     nt = std::max(1, std::min(nt, count));
     std::vector<std::string> out(nt);
     std::vector<std::thread> th;
+    std::atomic<bool> failed{false};
+    // an exception (out of memory growing a slab's text) must not leave a
+    // thread or cross the C interface: it is noted and reported after the join
+    auto run = [&](int lo, int hi, std::string &o) {
+        try {
+            body(lo, hi, o);
+        } catch (...) {
+            failed = true;
+        }
+    };
     for (int t = 0; t < nt; t++) {
         const int lo = (int)((long)count * t / nt), hi = (int)((long)count * (t + 1) / nt);
         try {
-            th.emplace_back([&, t, lo, hi] { body(lo, hi, out[t]); });
+            th.emplace_back([&, t, lo, hi] { run(lo, hi, out[t]); });
         } catch (...) {   // no thread to be had: this slab on the calling thread
-            body(lo, hi, out[t]);
+            run(lo, hi, out[t]);
         }
     }
     for (auto &x : th) x.join();
+    if (failed) {
+        fprintf(stderr, "hmfem: out of memory formatting an output file\n");
+        exit(1);
+    }
     return out;
 }
 
