@@ -92,7 +92,7 @@ __device__ __forceinline__ void apply_block(const Nb<T> &b, V val, C (&out)[NR])
 template <int NR>
 __global__ void __launch_bounds__(kBlock)
 k_b_init(OpArgs<double> o, const double *__restrict__ B, const double *__restrict__ X, double *__restrict__ R,
-         int chunk, double *P)
+         float *__restrict__ R32, int chunk, double *P)
 {
     const int n = o.nx * o.ny * o.nz;
     int lb, beg, end;
@@ -109,6 +109,7 @@ k_b_init(OpArgs<double> o, const double *__restrict__ B, const double *__restric
             const double bv = B[(long)s * NR + r];
             const double rv = bv - ax[r];
             R[(long)s * NR + r] = rv;
+            R32[(long)s * NR + r] = (float)rv;
             acc[r] += bv * bv;
             acc[NR + r] += rv * rv;
         }
@@ -123,7 +124,7 @@ k_b_init(OpArgs<double> o, const double *__restrict__ B, const double *__restric
 template <int NR>
 __global__ void __launch_bounds__(kBlock)
 k_b_init_lane(OpArgs<double> o, const double *__restrict__ B, const double *__restrict__ X, double *__restrict__ R,
-              int lane, int chunk, double *P)
+              float *__restrict__ R32, int lane, int chunk, double *P)
 {
     const int n = o.nx * o.ny * o.nz;
     int lb, beg, end;
@@ -136,6 +137,7 @@ k_b_init_lane(OpArgs<double> o, const double *__restrict__ B, const double *__re
         const double bv = B[(long)s * NR + lane];
         const double rv = bv - ax;
         R[(long)s * NR + lane] = rv;
+        R32[(long)s * NR + lane] = (float)rv;
         acc[0] += bv * bv;
         acc[1] += rv * rv;
     }
@@ -346,8 +348,8 @@ k_b_spmv_z(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S
 template <int NR>
 __global__ void __launch_bounds__(kBlock)
 k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, const double *__restrict__ V,
-           double *__restrict__ X, double *__restrict__ R, int chunk, int Gtau, BlockScalars *sc, const double *Ptau,
-           double *Prr)
+           double *__restrict__ X, double *__restrict__ R, float *__restrict__ R32, int chunk, int Gtau,
+           BlockScalars *sc, const double *Ptau, double *Prr)
 {
     if (sc->all_done) return;
     const int k = sc->kupd;
@@ -388,6 +390,7 @@ k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, 
         const double ri = R[q] - am * V[q];
         X[q] += am * S[q];
         R[q] = ri;
+        R32[q] = (float)ri;
         my += ri * ri;
     }
     double acc[NR];
@@ -797,6 +800,7 @@ struct BlockState {
     int nr;
     Chunking ch;
     double *R, *S0, *S1, *V, *X;
+    float *R32;                       // FP32 copy of R, the fine-level V-cycle's input
     float *Z;                         // fine-level V-cycle output (interleaved)
     BlockScalars *sc;
     double *Pinit, *Ptau, *Prr, *Pdot;
@@ -873,15 +877,15 @@ static int b_vcycle_fine(hipStream_t st, Hierarchy &H, BlockState &S, const doub
         int rc = deflation_sums_block(st, *S.D, NR, S.R, Vp, S.Pdot + (long)2 * NR * S.ch.G, S.done);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL((k_b_gs_tile<NR, double, 0>), dim3(grid_for(nt, 1)), dim3(kBThreads), 0, st, opargs<float>(L),
-                       sw, (const double *)S.R, (const float *)nullptr, X.u, 0, 0, 0, 0, 0, (const float *)nullptr,
+    hipLaunchKernelGGL((k_b_gs_tile<NR, float, 0>), dim3(grid_for(nt, 1)), dim3(kBThreads), 0, st, opargs<float>(L),
+                       sw, (const float *)S.R32, (const float *)nullptr, X.u, 0, 0, 0, 0, 0, (const float *)nullptr,
                        0.f, (const double *)nullptr, 1, (double *)nullptr, S.done);
-    b_restrict<NR, double>(st, L, C, S.R, X.u, S.lv[1].f, S.done);
+    b_restrict<NR, float>(st, L, C, S.R32, X.u, S.lv[1].f, S.done);
     int rc = b_vcycle_coarse<NR>(st, H, S, 1, opt);
     if (rc) return rc;
     const int tpb = (nt + S.ch.G - 1) / S.ch.G;
-    hipLaunchKernelGGL((k_b_gs_tile<NR, double, 2>), dim3(S.ch.G), dim3(kBThreads), 0, st, opargs<float>(L), sw,
-                       (const double *)S.R, (const float *)X.u, S.Z, lg2(L.fx), lg2(L.fy), lg2(L.fz), C.d.nx,
+    hipLaunchKernelGGL((k_b_gs_tile<NR, float, 2>), dim3(S.ch.G), dim3(kBThreads), 0, st, opargs<float>(L), sw,
+                       (const float *)S.R32, (const float *)X.u, S.Z, lg2(L.fx), lg2(L.fy), lg2(L.fz), C.d.nx,
                        C.d.ny, (const float *)S.lv[1].u, (float)opt.ocf, Vp, tpb, S.Pdot, S.done);
     if (defl) return deflation_scatter_block(st, *S.D, NR, S.Z, S.done);
     return HMFEM_OK;
@@ -908,7 +912,7 @@ size_t block_bytes(Dims fine, int nr, const hmfem_options &opt)
     const size_t n = fine.n();
     size_t b = hierarchy_bytes(fine, o);
     b += 4 * align256(n * nr * 8);                                   // R, S0, S1, V
-    b += align256(n * nr * 4);                                       // Z
+    b += 2 * align256(n * nr * 4);                                   // Z, R32
     Dims d = fine;                                                   // f, u, t per level
     while (true) {
         b += 3 * align256((size_t)d.n() * nr * 4);
@@ -969,7 +973,7 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
                                (const double *)S.Pinit, (const double *)S.Prr, (const double *)S.Pdot);
             hipLaunchKernelGGL(k_b_spmv_z<NR>, dim3(Gz), dim3(kBlock), 0, st, o, (const float *)S.Z, S.S0, S.S1,
                                S.V, zlen, G, Gdot, S.sc, S.Pinit, S.Prr, S.Pdot, S.Ptau);
-            hipLaunchKernelGGL(k_b_update<NR>, dim3(G), dim3(kBlock), 0, st, n, S.S0, S.S1, S.V, S.X, S.R,
+            hipLaunchKernelGGL(k_b_update<NR>, dim3(G), dim3(kBlock), 0, st, n, S.S0, S.S1, S.V, S.X, S.R, S.R32,
                                S.ch.chunk, Gz, S.sc, S.Ptau, S.Prr);
             rc = b_vcycle_fine<NR>(st, H, S, S.V, opt);
             if (rc) return rc;
@@ -995,7 +999,7 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
                     continue;
                 }
                 hipLaunchKernelGGL(k_b_init_lane<NR>, dim3(G), dim3(kBlock), 0, st, o, Bk, (const double *)S.X, S.R,
-                                   r, S.ch.chunk, S.Pinit);
+                                   S.R32, r, S.ch.chunk, S.Pinit);
                 h_sc->done[r] = 0;
                 h_sc->status[r] = HMFEM_OK;
                 h_sc->iter[r] = 0;
@@ -1032,7 +1036,7 @@ static int block_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims
 // Continue a four-lane solve stopped by kBlockCompact with its busy lanes
 // lanes[0..1] (-1: none) as a two-lane solve. The two-lane vectors take the
 // space of four-lane ones whose values are not needed at a poll: R and the
-// search direction into V and the spare S buffer, X and Z into R. On return
+// search direction into V and the spare S buffer, X, Z and R32 into R. On return
 // X (Xk, four lanes) holds both lanes' results and hs the four-lane scalars.
 static int block_compact(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d, BlockState &S,
                          const hmfem_options &opt, BlockScalars &hs, BlockFeed *feed, int *kdone, const int *lanes)
@@ -1045,12 +1049,13 @@ static int block_compact(hipStream_t st, Hierarchy &H, const Stencil<double> &A,
     double *R2 = S.V, *V2 = S.V + 2 * n;
     double *S0_2 = spare, *S1_2 = spare + 2 * n;
     double *X2 = S.R;
-    float *Z2 = (float *)(S.R + 2 * n);
+    float *Z2 = (float *)(S.R + 2 * n), *R32_2 = (float *)(S.R + 3 * n);
     hipLaunchKernelGGL(k_b_pack2<double>, dim3(nb), dim3(kBlock), 0, st, n, (const double *)S.R, la, lb, R2);
     hipLaunchKernelGGL(k_b_pack2<double>, dim3(nb), dim3(kBlock), 0, st, n, (const double *)sin4, la, lb,
                        (k & 1) ? S1_2 : S0_2);
     hipLaunchKernelGGL(k_b_pack2<double>, dim3(nb), dim3(kBlock), 0, st, n, (const double *)S.X, la, lb, X2);
     hipLaunchKernelGGL(k_b_pack2<float>, dim3(nb), dim3(kBlock), 0, st, n, (const float *)S.Z, la, lb, Z2);
+    hipLaunchKernelGGL(k_b_pack2<float>, dim3(nb), dim3(kBlock), 0, st, n, (const float *)S.R32, la, lb, R32_2);
     const int G = S.ch.G;
     const int Gdot = S.D && S.D->nc > 0 ? G + 1 : G;
     // partials of the next iteration's scalars: <z, r>, <z, v> rows (Pdot) and r.r rows (Prr)
@@ -1061,7 +1066,7 @@ static int block_compact(hipStream_t st, Hierarchy &H, const Stencil<double> &A,
     HMFEM_CHECK(hipGetLastError());
     BlockState S2 = S;
     S2.nr = 2;
-    S2.R = R2; S2.V = V2; S2.S0 = S0_2; S2.S1 = S1_2; S2.X = X2; S2.Z = Z2;
+    S2.R = R2; S2.V = V2; S2.S0 = S0_2; S2.S1 = S1_2; S2.X = X2; S2.Z = Z2; S2.R32 = R32_2;
     S2.Pdot = S.Ptau; S2.Ptau = S.Pdot; S2.Prr = S.Pinit; S2.Pinit = S.Prr;
     BlockScalars h2 = hs;
     const int map[2] = {la, lb};
@@ -1124,13 +1129,14 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
     S.R = ar.take<double>(n * nr); S.S0 = ar.take<double>(n * nr); S.S1 = ar.take<double>(n * nr);
     S.V = ar.take<double>(n * nr); S.X = Xk;
     S.Z = ar.take<float>(n * nr);
+    S.R32 = ar.take<float>(n * nr);
     S.sc = ar.take<BlockScalars>(1);
     S.Pinit = ar.take<double>((size_t)(kMaxReduceBlocks + 1) * 2 * kMaxRhs);
     S.Ptau = ar.take<double>((size_t)(kMaxReduceBlocks + 1) * 2 * kMaxRhs);
     S.Prr = ar.take<double>((size_t)(kMaxReduceBlocks + 1) * 2 * kMaxRhs);
     S.Pdot = ar.take<double>((size_t)(kMaxReduceBlocks + 1) * 2 * kMaxRhs);
     int *d_info = ar.take<int>(64);
-    if (!S.R || !S.S0 || !S.S1 || !S.V || !S.Z || !S.sc || !S.Pdot || !d_info) return HMFEM_ERR_NOMEM;
+    if (!S.R || !S.S0 || !S.S1 || !S.V || !S.Z || !S.R32 || !S.sc || !S.Pdot || !d_info) return HMFEM_ERR_NOMEM;
     S.done = &S.sc->all_done;
 
     hipEvent_t e0, e1, e2;
@@ -1179,11 +1185,13 @@ int block_solve(hipStream_t st, Dims d, const Stencil<double> &A, int nr, const 
     BlockScalars hs{};
     switch (nr) {
     case 2:
-        hipLaunchKernelGGL(k_b_init<2>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
+        hipLaunchKernelGGL(k_b_init<2>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.R32, S.ch.chunk,
+                           S.Pinit);
         rc = block_cg<2>(st, H, A, d, S, opt, &hs, Bk, feed, block_iterations);
         break;
     default: {
-        hipLaunchKernelGGL(k_b_init<4>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.ch.chunk, S.Pinit);
+        hipLaunchKernelGGL(k_b_init<4>, dim3(S.ch.G), dim3(kBlock), 0, st, o, Bk, Xk, S.R, S.R32, S.ch.chunk,
+                           S.Pinit);
         int lanes[2] = {-1, -1};
         rc = block_cg<4>(st, H, A, d, S, opt, &hs, Bk, feed, block_iterations, false, nullptr, nullptr,
                          opt.block_compact ? lanes : nullptr);
