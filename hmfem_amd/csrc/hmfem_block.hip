@@ -86,6 +86,33 @@ __device__ __forceinline__ void apply_block(const Nb<T> &b, V val, C (&out)[NR])
     }
 }
 
+// The NR values of one cell of an interleaved block vector ([cell][NR],
+// 16-byte aligned for NR = 2 and 4) in vector loads.
+template <int NR>
+__device__ __forceinline__ void ld_lanes(const double *p, double (&v)[NR])
+{
+    static_assert(NR % 2 == 0, "pairs");
+#pragma unroll
+    for (int h = 0; h < NR / 2; h++) {
+        const double2 t = reinterpret_cast<const double2 *>(p)[h];
+        v[2 * h] = t.x;
+        v[2 * h + 1] = t.y;
+    }
+}
+
+template <int NR>
+__device__ __forceinline__ void ld_lanes(const float *p, float (&v)[NR])
+{
+    static_assert(NR == 2 || NR == 4, "two or four lanes");
+    if constexpr (NR == 4) {
+        const float4 t = *reinterpret_cast<const float4 *>(p);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+        const float2 t = *reinterpret_cast<const float2 *>(p);
+        v[0] = t.x; v[1] = t.y;
+    }
+}
+
 // ---------------------------------------------------------------- CG kernels
 
 // r = b - A x, partials (b.b, r.r) per right-hand side: P[lb][2 NR].
@@ -287,24 +314,34 @@ k_b_spmv_z(OpArgs<double> o, const float *__restrict__ Z, double *__restrict__ S
         }
         const int ri = i0 + rx - 1, rj = j0 + ry - 1;
         const bool ring_in = has_ring && ri >= 0 && rj >= 0 && ri < nx && rj < ny;
+        // a cell's NR values of z and s_old in vector loads (all lanes: a
+        // finished lane's z and s are finite, s = 0 since it finished; its
+        // value is then dropped), s = z + beta s_old for the live ones
+        auto cell_s = [&](long c, double (&a)[NR]) {
+            float zv[NR];
+            double sv[NR];
+            ld_lanes<NR>(Z + c * NR, zv);
+            if (k) ld_lanes<NR>(sin + c * NR, sv);
+            else
+#pragma unroll
+                for (int r = 0; r < NR; r++) sv[r] = 0.0;
+#pragma unroll
+            for (int r = 0; r < NR; r++) a[r] = live[r] ? (double)zv[r] + (k ? beta[r] * sv[r] : 0.0) : 0.0;
+        };
         auto fill = [&](int kk) {
             const int slot = (kk + 3) % 3;
             const bool in = kk >= 0 && kk < nz;
+            double a[NR], b[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) { a[r] = 0.0; b[r] = 0.0; }
+            // a lane that restarts has beta 0 (and its s_old is 0: it was
+            // idle, s = 0, on the iteration before its restart)
+            if (in && col) cell_s(i + (long)j * nx + (long)kk * nxy, a);
+            if (in && ring_in) cell_s(ri + (long)rj * nx + (long)kk * nxy, b);
 #pragma unroll
             for (int r = 0; r < NR; r++) {
-                double a = 0.0, b = 0.0;
-                // a lane that restarts has beta 0, and its s_old is 0: it
-                // was idle (s = 0) on the iteration before its restart
-                if (in && col && live[r]) {
-                    const long q = (i + (long)j * nx + (long)kk * nxy) * NR + r;
-                    a = (double)Z[q] + (k ? beta[r] * sin[q] : 0.0);
-                }
-                if (in && ring_in && live[r]) {
-                    const long q = (ri + (long)rj * nx + (long)kk * nxy) * NR + r;
-                    b = (double)Z[q] + (k ? beta[r] * sin[q] : 0.0);
-                }
-                sh[slot][ty + 1][tx + 1][r] = a;
-                if (has_ring) sh[slot][ry][rx][r] = b;
+                sh[slot][ty + 1][tx + 1][r] = a[r];
+                if (has_ring) sh[slot][ry][rx][r] = b[r];
             }
         };
         fill(kb - 1);
@@ -488,8 +525,15 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
             for (int kk = 0; kk < CZ; kk++) {
                 const bool act = col && kc + kk < nz;
                 const unsigned s = (unsigned)(s0 + kk * nxy) * NR + rb;
+                if constexpr (MODE == 0) {   // (the vector load takes it past 128 VGPRs: one workgroup per CU)
 #pragma unroll
-                for (int r = 0; r < NP; r++) ff[kk][r] = act ? (float)f[s + r] : 0.f;
+                    for (int r = 0; r < NP; r++) ff[kk][r] = act ? (float)f[s + r] : 0.f;
+                } else {
+                    FT fv[NP];
+                    if (act) ld_lanes<NP>(f + s, fv);
+#pragma unroll
+                    for (int r = 0; r < NP; r++) ff[kk][r] = act ? (float)fv[r] : 0.f;
+                }
             }
             // MODE 2: v for <z, v>, loaded with the pass's other values and held
             // through the sweeps (205 VGPRs, still two waves per SIMD; loaded
@@ -500,8 +544,10 @@ k_b_gs_tile(OpArgs<float> o, int sweeps, const FT *__restrict__ f, const float *
                 for (int kk = 0; kk < CZ; kk++) {
                     const bool act = v && col && kc + kk < nz;
                     const unsigned s = (unsigned)(s0 + kk * nxy) * NR + rb;
+                    if (act) ld_lanes<NP>(v + s, vv[kk]);
+                    else
 #pragma unroll
-                    for (int r = 0; r < NP; r++) vv[kk][r] = act ? v[s + r] : 0.0;
+                        for (int r = 0; r < NP; r++) vv[kk][r] = 0.0;
                 }
             float zlo[NP], zhi[NP], ring[RPER][NP];
 #pragma unroll
