@@ -365,12 +365,13 @@ def block_rhs(nx, ny, nz, nrhs):
 def block_bytes_per_iteration(dims, nr):
     """Algorithmic HBM bytes of one block CG iteration on the fine level
     (coefficients once, nr interleaved right-hand sides): SpMV (A64, s, z in;
-    s, v out), update (x, s, r, v in; x, r out), pre-smoothing (wx, wy, wz, di,
-    r in; u out), residual-restriction (A32, r, u in), post-smoothing (wx, wy,
-    wz, di, r, u, v in; z out). Coarse levels add about 1/7 and are left out."""
+    s, v out), update (x, s, r, v in; x, r and its FP32 copy out),
+    pre-smoothing (wx, wy, wz, di, FP32 r in; u out), residual-restriction
+    (A32, FP32 r, u in), post-smoothing (wx, wy, wz, di, FP32 r, u, v in; z
+    out). Coarse levels add about 1/7 and are left out."""
     n = dims[0] * dims[1] * dims[2]
-    per = (32 + nr * (8 + 4 + 8 + 8)) + nr * 48 + (16 + nr * (8 + 4)) + (16 + nr * (8 + 4)) + \
-        (16 + nr * (8 + 4 + 8 + 4))
+    per = (32 + nr * (8 + 4 + 8 + 8)) + nr * 52 + (16 + nr * (4 + 4)) + (16 + nr * (4 + 4)) + \
+        (16 + nr * (4 + 4 + 8 + 4))
     return n * per
 
 
