@@ -235,8 +235,8 @@ k_perm_to_stencil(int nx, int ny, int nz, const ELE *perm, const IND *indcol, do
 
 // Flux of one cell (flux_reconstruction.c:30-273) and its divergence
 // (flux_reconstruction.c:299).
-__device__ __forceinline__ double flux_cell(const GridArgs &g, const Faces6 &f, const FaceVals &B, const double *p,
-                                            long s, int i, int j, int k, ELE *v)
+__device__ __forceinline__ double flux_cell_to(const GridArgs &g, const Faces6 &f, const FaceVals &B,
+                                               const double *p, long s, int i, int j, int k, double *o)
 {
 #pragma clang fp contract(off)
     const long nx = g.nx, nxy = (long)g.nx * g.ny;
@@ -254,11 +254,27 @@ __device__ __forceinline__ double flux_cell(const GridArgs &g, const Faces6 &f, 
     else { long l = (long)j * g.nx + i; vt = g.bc[4] == 0 ? g.hz * f.t * (ps - B.t[l]) : B.t[l]; }
     if (k > 0) vb = g.hz * f.b * (ps - p[s - nxy]);
     else { long l = (long)j * g.nx + i; vb = g.bc[5] == 0 ? g.hz * f.b * (ps - B.b[l]) : -B.b[l]; }
-    if (v) {
-        double *o = (double *)(v + s);   // u,d,l,r,c,t,b; c is left untouched
+    if (o) {   // u,d,l,r,c,t,b; c is left untouched
         o[0] = vu; o[1] = vd; o[2] = vl; o[3] = vr; o[5] = vt; o[6] = vb;
     }
     return vr / g.hx + vl / g.hx + vd / g.hy + vu / g.hy + vb / g.hz + vt / g.hz;
+}
+
+__device__ __forceinline__ double flux_cell(const GridArgs &g, const Faces6 &f, const FaceVals &B, const double *p,
+                                            long s, int i, int j, int k, ELE *v)
+{
+    return flux_cell_to(g, f, B, p, s, i, j, k, v ? (double *)(v + s) : nullptr);
+}
+
+// The block's cells' ELE records, staged in LDS by flux_cell_to, written out
+// as consecutive 8-byte values (every lane of a wave on one contiguous run;
+// each thread's own record would be six stores 56 bytes apart), c skipped.
+__device__ __forceinline__ void flux_store_block(const double *stage, long s0, long n, ELE *v)
+{
+    const long cnt = min((long)kBlock, n - s0);
+    double *out = (double *)(v + s0);
+    for (int q = threadIdx.x; q < cnt * 7; q += kBlock)
+        if (q % 7 != 4) out[q] = stage[q];
 }
 
 // Max/min |div| of a block, written as one partial pair per block; the
@@ -308,17 +324,20 @@ __global__ void __launch_bounds__(kBlock)
 k_flux_from_K(GridArgs g, const double *Kx, const double *Ky, const double *Kz, FaceVals B, const double *p,
               ELE *v, double *partial)
 {
+    __shared__ double stage[kBlock * 7];
     long n = (long)g.nx * g.ny * g.nz;
-    long s = (long)xcd_block() * kBlock + threadIdx.x;
+    const long s0 = (long)xcd_block() * kBlock;
+    long s = s0 + threadIdx.x;
     double mx = 0.0, mn = 100.0;
     if (s < n) {
         int i, j, k;
         cell_coords(g, s, i, j, k);
         Faces6 f = faces_from_K(g, Kx, Ky, Kz, s, i, j, k);
-        double dv = fabs(flux_cell(g, f, B, p, s, i, j, k, v));
+        double dv = fabs(flux_cell_to(g, f, B, p, s, i, j, k, v ? stage + 7 * threadIdx.x : nullptr));
         mx = dv; mn = dv;
     }
-    minmax_block(mx, mn, partial);
+    minmax_block(mx, mn, partial);   // (its barrier also orders the LDS writes before the reads)
+    if (v && s0 < n) flux_store_block(stage, s0, n, v);
 }
 
 __global__ void __launch_bounds__(kBlock)
