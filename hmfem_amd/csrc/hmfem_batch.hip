@@ -408,6 +408,19 @@ kb_to_f32(long n, const double *e, const double *wx, const double *wy, const dou
     e32[s] = (float)e[s]; wx32[s] = (float)wx[s]; wy32[s] = (float)wy[s]; wz32[s] = (float)wz[s];
 }
 
+// The fine level's FP32 copy and its inverse diagonal in one pass, per instance.
+__global__ void __launch_bounds__(kBlock)
+kb_to_f32_dinv(const BInst *__restrict__ D, const int *__restrict__ bmap, LPtr<double> A, float *e32, float *wx32,
+               float *wy32, float *wz32, float *di32)
+{
+    const int q = bmap[blockIdx.x];
+    const BInst I = D[q];
+    const int s = (blockIdx.x - I.blk0) * kBlock + threadIdx.x;
+    if (s >= I.n) return;
+    const OpArgs<double> o = bop(A, I.off, I.nx, I.ny, I.nz);
+    to_f32_dinv_cell(o, cell_at(o, s), e32 + I.off, wx32 + I.off, wy32 + I.off, wz32 + I.off, di32 + I.off);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_dinv(const BInst *__restrict__ D, const int *__restrict__ bmap, LPtr<T> A, T *di)
@@ -1091,9 +1104,14 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
 
     HMFEM_CHECK(hipEventRecord(ev[0], st));
     // hierarchy
-    hipLaunchKernelGGL(kb_to_f32, dim3((unsigned)((n0 + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, (long)n0,
-                       B.lev[0].e64, B.lev[0].wx64, B.lev[0].wy64, B.lev[0].wz64, B.lev[0].e32, B.lev[0].wx32,
-                       B.lev[0].wy32, B.lev[0].wz32);
+    if (prec == HMFEM_PREC_FP64)
+        hipLaunchKernelGGL(kb_to_f32, dim3((unsigned)((n0 + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, (long)n0,
+                           B.lev[0].e64, B.lev[0].wx64, B.lev[0].wy64, B.lev[0].wz64, B.lev[0].e32, B.lev[0].wx32,
+                           B.lev[0].wy32, B.lev[0].wz32);
+    else   // with the fine level's inverse diagonal
+        hipLaunchKernelGGL(kb_to_f32_dinv, dim3(B.lev[0].nblk), dim3(kBlock), 0, st, B.lev[0].d, B.lev[0].bmap,
+                           lp<double>(B.lev[0]), B.lev[0].e32, B.lev[0].wx32, B.lev[0].wy32, B.lev[0].wz32,
+                           B.lev[0].di32);
     for (int l = 0; l + 1 < B.nlev; l++) {
         BLevel &L = B.lev[l], &C = B.lev[l + 1];
         hipLaunchKernelGGL(kb_rap, dim3(C.nblk), dim3(kBlock), 0, st, L.d, C.d, C.bmap, lp<double>(L), C.e64, C.wx64,
@@ -1104,7 +1122,7 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
         if (prec == HMFEM_PREC_FP64)
             hipLaunchKernelGGL(kb_dinv<double>, dim3(L.nblk), dim3(kBlock), 0, st, L.d, L.bmap, lp<double>(L),
                                L.di64);
-        else
+        else if (l > 0)
             hipLaunchKernelGGL(kb_dinv<float>, dim3(L.nblk), dim3(kBlock), 0, st, L.d, L.bmap, lp<float>(L), L.di32);
     }
     hipLaunchKernelGGL(kb_dense_invert, dim3(B.ninst), dim3(1024), 0, st, B.lev[B.nlev - 1].d,

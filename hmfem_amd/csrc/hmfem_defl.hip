@@ -833,7 +833,7 @@ static int stage_reserve(Deflation &D, size_t bytes)
 // per-cluster index arrays: one round trip of a few KB.
 static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *dinst, const int *bmap, int G,
                            const Stencil<double> &A, const unsigned *hist, double ratio, Deflation &D,
-                           char *(*alloc)(void *, size_t, int), void *actx)
+                           char *(*alloc)(void *, size_t, int), void *actx, bool records)
 {
     const int ninst = (int)inst.size();
     // per instance: median bin, threshold ~ ratio x median, count certainly
@@ -1000,10 +1000,12 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     hipLaunchKernelGGL(k_defl_einv, dim3(1), dim3(kBlock), 0, st, nc, (const int *)D.cp, (const double *)(share + m),
                        D.einv);
     // the balancing form's records: the cells just outside the clusters
+    // (none unless asked for: the additive form does not read them)
     HMFEM_CHECK(hipMemsetAsync(D.rec_count, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_bnn_records, dim3((unsigned)((6L * m + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
-                       (const DefInst *)dinst, A, (const int *)D.cells, (const int *)D.comp, m, (const int *)slist,
-                       pms, cap, w, D.rec_cell, D.rec_cl, D.rec_w, D.rec_count, rec_max);
+    if (records)
+        hipLaunchKernelGGL(k_bnn_records, dim3((unsigned)((6L * m + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                           (const DefInst *)dinst, A, (const int *)D.cells, (const int *)D.comp, m,
+                           (const int *)slist, pms, cap, w, D.rec_cell, D.rec_cl, D.rec_w, D.rec_count, rec_max);
     D.lord = w.lord;
     D.nbr = w.nbr;
     D.inst = dinst;
@@ -1049,7 +1051,7 @@ int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation 
 }
 
 int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ratio, Deflation &D,
-                     char *(*alloc)(void *, size_t, int), void *actx)
+                     char *(*alloc)(void *, size_t, int), void *actx, bool records)
 {
     if (!D.pending) return HMFEM_OK;
     D.pending = false;
@@ -1060,7 +1062,7 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     std::vector<DefInst> inst(1, DefInst{d.nx, d.ny, d.nz, (int)n, 0, 0, INFINITY, 0});
     HMFEM_CHECK(hipEventSynchronize(D.hist_ready));
     return finish_clusters(st, inst, dinst, nullptr, (int)((n + kBlock - 1) / kBlock), A, D.h_hist, ratio, D,
-                           alloc, actx);
+                           alloc, actx, records);
 }
 
 int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
@@ -1108,7 +1110,7 @@ int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Sten
                                st));
     HMFEM_CHECK(hipStreamSynchronize(st));
     // finish_clusters reads the histograms before it reuses the staging
-    return finish_clusters(st, inst, dinst, bmap, G, A, (const unsigned *)D.h_stage, ratio, D, alloc, actx);
+    return finish_clusters(st, inst, dinst, bmap, G, A, (const unsigned *)D.h_stage, ratio, D, alloc, actx, false);
 }
 
 template <int NR>

@@ -246,11 +246,13 @@ struct Deflation {
 };
 // deflation_begin queues the diagonal histogram and its copy to the host;
 // deflation_finish (after other work has been queued) decides and builds the
-// correction. D.nc == 0 afterwards if there is none.
+// correction. D.nc == 0 afterwards if there is none. records: also find the
+// cells just outside the clusters that the balancing form (opt.deflate 2)
+// needs.
 int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation &D,
                     char *(*alloc)(void *, size_t, int), void *actx);
 int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ratio, Deflation &D,
-                     char *(*alloc)(void *, size_t, int), void *actx);
+                     char *(*alloc)(void *, size_t, int), void *actx, bool records = false);
 void deflation_sums(hipStream_t st, const Deflation &D, const double *r, const double *v, double *Pextra,
                     const int *done);
 // Batch: the instances (dims[3q..3q+2], concatenated in one index space) get

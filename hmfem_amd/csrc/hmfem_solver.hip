@@ -771,6 +771,16 @@ k_to_f32(long n, const double *e, const double *wx, const double *wy, const doub
     e32[s] = (float)e[s]; wx32[s] = (float)wx[s]; wy32[s] = (float)wy[s]; wz32[s] = (float)wz[s];
 }
 
+// The fine level's FP32 copy and its inverse diagonal in one pass.
+__global__ void __launch_bounds__(kBlock)
+k_to_f32_dinv(OpArgs<double> o, float *e32, float *wx32, float *wy32, float *wz32, float *di32)
+{
+    const int n = o.nx * o.ny * o.nz;
+    const int s = xcd_block() * kBlock + threadIdx.x;
+    if (s >= n) return;
+    to_f32_dinv_cell(o, cell_at(o, s), e32, wx32, wy32, wz32, di32);
+}
+
 // sq = sqrt(d) of the fine level from the FP64 stencil.
 __global__ void __launch_bounds__(kBlock)
 k_fine16_scale(OpArgs<double> o, float *sq)
@@ -1519,8 +1529,12 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const 
     {
         Level &L = H.lev[0];
         long n = L.d.n();
-        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, n, L.s64.e, L.s64.wx, L.s64.wy,
-                           L.s64.wz, L.s32.e, L.s32.wx, L.s32.wy, L.s32.wz);
+        if (prec == HMFEM_PREC_FP64)
+            hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, n, L.s64.e, L.s64.wx,
+                               L.s64.wy, L.s64.wz, L.s32.e, L.s32.wx, L.s32.wy, L.s32.wz);
+        else   // with the fine level's inverse diagonal
+            hipLaunchKernelGGL(k_to_f32_dinv, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, opargs<double>(L),
+                               L.s32.e, L.s32.wx, L.s32.wy, L.s32.wz, L.s32.di);
         if (H.f16sq) {
             hipLaunchKernelGGL(k_fine16_scale, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, opargs<double>(L),
                                H.f16sq);
@@ -1541,7 +1555,7 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const 
         int G = grid_for(L.d.n(), kBlock);
         if (prec == HMFEM_PREC_FP64)
             hipLaunchKernelGGL(k_dinv<double>, dim3(G), dim3(kBlock), 0, st, opargs<double>(L), L.s64.di);
-        else
+        else if (l > 0)
             hipLaunchKernelGGL(k_dinv<float>, dim3(G), dim3(kBlock), 0, st, opargs<float>(L), L.s32.di);
     }
     {
@@ -2029,7 +2043,7 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
         rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit, hint);
         if (rc) return rc;
         if (defl && defl->pending) {
-            rc = deflation_finish(st, d, A, opt.deflate_ratio, *defl, dalloc, dctx);
+            rc = deflation_finish(st, d, A, opt.deflate_ratio, *defl, dalloc, dctx, opt.deflate == 2);
             if (rc) return rc;
         }
         if (cache) {

@@ -55,6 +55,23 @@ __device__ __forceinline__ C diag_at(const OpArgs<T> &o, const Cell &c)
     return d;
 }
 
+// The fine level's FP32 copy (e, wx, wy, wz) from the FP64 stencil and its
+// inverse diagonal in one pass: the diagonal is summed from the FP32-rounded
+// couplings in diag_at's order, so di is what diag_at<float, double> on the
+// copy gives.
+__device__ __forceinline__ void to_f32_dinv_cell(const OpArgs<double> &o, const Cell &c, float *e32, float *wx32,
+                                                 float *wy32, float *wz32, float *di32)
+{
+    const int nx = o.nx, nxy = o.nx * o.ny, s = c.s;
+    const float e = (float)o.e[s], x = (float)o.wx[s], y = (float)o.wy[s], z = (float)o.wz[s];
+    e32[s] = e; wx32[s] = x; wy32[s] = y; wz32[s] = z;
+    double d = (double)e + (double)x + (double)y + (double)z;
+    if (c.i > 0) d += (double)(float)o.wx[s - 1];
+    if (c.j > 0) d += (double)(float)o.wy[s - nx];
+    if (c.k > 0) d += (double)(float)o.wz[s - nxy];
+    di32[s] = (float)(1.0 / d);
+}
+
 // (A u)_s in flux form and the diagonal, for a functor u(Cell) giving the
 // value of u at a cell (so callers can build u on the fly).
 template <typename T, typename C, typename U>
