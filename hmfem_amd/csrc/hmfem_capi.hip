@@ -8,9 +8,12 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -62,8 +65,199 @@ int iter_amg = 0;
 FACE boundary;
 }
 
+// Copies between pageable host memory (the reference's malloc'ed arrays) and
+// the device through a ring of pinned chunks, for the host-buffer drop-ins:
+// the host side of a chunk (a memcpy split over worker threads) overlaps the
+// DMA of the chunk before it. On the MI355X box a pageable hipMemcpy moves
+// 27 GB/s to the device and 17 GB/s back, a pinned one 57 GB/s, and eight
+// threads copy pageable to pinned memory at 73 GB/s. Any failure (pinned
+// memory, threads) falls back to plain hipMemcpyAsync.
+struct HostCopy {
+    static constexpr size_t kChunk = size_t(32) << 20;   // a multiple of 56 B is taken for ELE records
+    static constexpr int kSlots = 3;
+    static constexpr size_t kMin = size_t(8) << 20;      // smaller copies go straight to hipMemcpyAsync
+    char *pin[kSlots] = {};
+    hipEvent_t ev[kSlots] = {};
+    bool inflight[kSlots] = {};
+    int state = 0;   // 0: not set up, 1: ready, -1: unavailable
+    std::vector<std::thread> th;
+    std::mutex m;
+    std::condition_variable cv, cv_done;
+    std::function<void(int, int)> job;
+    int gen = 0, left = 0, nparts = 1;
+    bool quit = false;
+
+    bool setup()
+    {
+        if (state) return state > 0;
+        state = -1;
+        for (int q = 0; q < kSlots; q++) {
+            if (hipHostMalloc((void **)&pin[q], kChunk, hipHostMallocDefault) != hipSuccess) { pin[q] = nullptr; return false; }
+            if (hipEventCreateWithFlags(&ev[q], hipEventDisableTiming) != hipSuccess) { ev[q] = nullptr; return false; }
+        }
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int want = (int)std::min(8u, std::max(1u, hw));
+        for (int t = 1; t < want; t++) {
+            try {
+                th.emplace_back([this, t] { worker(t); });
+            } catch (...) {
+                break;
+            }
+        }
+        nparts = 1 + (int)th.size();
+        state = 1;
+        return true;
+    }
+
+    void worker(int part)
+    {
+        int seen = 0;
+        while (true) {
+            std::function<void(int, int)> f;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return quit || gen != seen; });
+                if (quit) return;
+                seen = gen;
+                f = job;
+            }
+            f(part, nparts);
+            std::lock_guard<std::mutex> lk(m);
+            if (--left == 0) cv_done.notify_one();
+        }
+    }
+
+    // f(part, nparts) on every thread of the pool, the caller's included
+    void par(const std::function<void(int, int)> &f)
+    {
+        if (nparts == 1) { f(0, 1); return; }
+        {
+            std::lock_guard<std::mutex> lk(m);
+            job = f;
+            left = nparts - 1;
+            gen++;
+        }
+        cv.notify_all();
+        f(0, nparts);
+        std::unique_lock<std::mutex> lk(m);
+        cv_done.wait(lk, [&] { return left == 0; });
+    }
+
+    void wait_slot(int q)
+    {
+        if (inflight[q]) (void)hipEventSynchronize(ev[q]);
+        inflight[q] = false;
+    }
+
+    static void split(size_t bytes, int part, int nparts, size_t &lo, size_t &hi)
+    {
+        const size_t per = (bytes / nparts + 63) & ~size_t(63);
+        lo = std::min(bytes, per * part);
+        hi = std::min(bytes, lo + per);
+    }
+
+    // host -> device; the host buffer may be reused once this returns
+    hipError_t h2d(hipStream_t st, void *dst, const void *src, size_t bytes)
+    {
+        if (bytes < kMin || !setup()) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+        size_t off = 0;
+        for (int i = 0; off < bytes; i++) {
+            const int q = i % kSlots;
+            const size_t len = std::min(kChunk, bytes - off);
+            wait_slot(q);
+            const char *s = (const char *)src + off;
+            char *p = pin[q];
+            par([&](int part, int np) {
+                size_t lo, hi;
+                split(len, part, np, lo, hi);
+                if (hi > lo) memcpy(p + lo, s + lo, hi - lo);
+            });
+            hipError_t e = hipMemcpyAsync((char *)dst + off, p, len, hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) return e;
+            if ((e = hipEventRecord(ev[q], st)) != hipSuccess) return e;
+            inflight[q] = true;
+            off += len;
+        }
+        return hipSuccess;
+    }
+
+    // device -> host, done when this returns. rec > 0: the buffer is an array
+    // of rec-byte records of which only the first `keep` bytes and the
+    // `tail` bytes after a skipped field of `skip` bytes are written (the ELE
+    // records of fluxos(), whose c the reference does not write); rec = 0:
+    // every byte.
+    hipError_t d2h(hipStream_t st, void *dst, const void *src, size_t bytes, size_t rec = 0, size_t keep = 0,
+                   size_t skip = 0)
+    {
+        if (!rec && (bytes < kMin || !setup())) {
+            hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
+            return e != hipSuccess ? e : hipStreamSynchronize(st);
+        }
+        if (!setup()) return hipErrorOutOfMemory;
+        const size_t chunk = rec ? kChunk / rec * rec : kChunk;
+        const size_t nch = (bytes + chunk - 1) / chunk;
+        auto issue = [&](size_t i) -> hipError_t {
+            const int q = (int)(i % kSlots);
+            const size_t off = i * chunk, len = std::min(chunk, bytes - off);
+            hipError_t e = hipMemcpyAsync(pin[q], (const char *)src + off, len, hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) return e;
+            if ((e = hipEventRecord(ev[q], st)) != hipSuccess) return e;
+            inflight[q] = true;
+            return hipSuccess;
+        };
+        for (int q = 0; q < kSlots; q++) wait_slot(q);
+        for (size_t i = 0; i < nch && i < (size_t)kSlots; i++) {
+            hipError_t e = issue(i);
+            if (e != hipSuccess) return e;
+        }
+        for (size_t i = 0; i < nch; i++) {
+            const int q = (int)(i % kSlots);
+            const size_t off = i * chunk, len = std::min(chunk, bytes - off);
+            wait_slot(q);
+            const char *p = pin[q];
+            char *d = (char *)dst + off;
+            par([&](int part, int np) {
+                if (!rec) {
+                    size_t lo, hi;
+                    split(len, part, np, lo, hi);
+                    if (hi > lo) memcpy(d + lo, p + lo, hi - lo);
+                    return;
+                }
+                const size_t nrec = len / rec, per = (nrec + np - 1) / np;
+                const size_t r0 = std::min(nrec, per * part), r1 = std::min(nrec, r0 + per);
+                const size_t tail = rec - keep - skip;
+                for (size_t r = r0; r < r1; r++) {
+                    memcpy(d + r * rec, p + r * rec, keep);
+                    memcpy(d + r * rec + keep + skip, p + r * rec + keep + skip, tail);
+                }
+            });
+            if (i + kSlots < nch) {
+                hipError_t e = issue(i + kSlots);
+                if (e != hipSuccess) return e;
+            }
+        }
+        return hipSuccess;
+    }
+
+    ~HostCopy()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            quit = true;
+        }
+        cv.notify_all();
+        for (auto &t : th) t.join();
+        for (int q = 0; q < kSlots; q++) {
+            if (inflight[q]) (void)hipEventSynchronize(ev[q]);
+            if (ev[q]) (void)hipEventDestroy(ev[q]);
+            if (pin[q]) (void)hipHostFree(pin[q]);
+        }
+    }
+};
+
 struct hmfem_context {
     hipStream_t st = nullptr;
+    HostCopy hc;               // pinned staging of the host-buffer drop-ins' large copies
     double *scratch = nullptr;   // this device's fixed scratch (hmfem_flux_device_noalloc)
     int scratch_rc = HMFEM_ERR_HIP;
     hipEvent_t ev[8];
@@ -1098,19 +1292,26 @@ static void face_types(FACE f, int bc[6])
     bc[0] = f.l; bc[1] = f.r; bc[2] = f.u; bc[3] = f.d; bc[4] = f.t; bc[5] = f.b;
 }
 
-template <typename T>
-static T *upload(hmfem_context *c, const char *name, const T *h, size_t count)
-{
-    T *d = (T *)c->get(name, count * sizeof(T) + 16);
-    if (d && h) (void)hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, c->st);
-    return d;
-}
-
 static void fatal(const char *what, int rc)
 {
     fprintf(stderr, "hmfem_gpu: %s failed (code %d)\n", what, rc);
     exit(1);
 }
+
+template <typename T>
+static T *upload(hmfem_context *c, const char *name, const T *h, size_t count)
+{
+    T *d = (T *)c->get(name, count * sizeof(T) + 16);
+    if (d && h) (void)c->hc.h2d(c->st, d, h, count * sizeof(T));
+    return d;
+}
+
+// device -> host of a drop-in's output (done on return)
+static void download(hmfem_context *c, void *h, const void *d, size_t bytes, const char *what)
+{
+    if (c->hc.d2h(c->st, h, d, bytes) != hipSuccess) fatal(what, HMFEM_ERR_HIP);
+}
+
 
 // Device copies of the six Bound arrays, sized by face.
 static void upload_bound(hmfem_context *c, const Bound *B, Dim N, const double *dv[6])
@@ -1152,9 +1353,8 @@ extern "C" int permeabilidade_arestas(Perm *K, ELE *perm, IND *indcol, grid h, D
     int rc = launch_assemble(c->st, &g, dKx, dKy, dKz, dKI, dWI, nullptr, nullptr, zero_faces, nullptr, nullptr,
                              nullptr, nullptr, db, dperm, dind);
     if (rc) fatal("permeabilidade_arestas", rc);
-    (void)hipMemcpyAsync(perm, dperm, n * sizeof(ELE), hipMemcpyDeviceToHost, c->st);
-    (void)hipMemcpyAsync(indcol, dind, n * sizeof(IND), hipMemcpyDeviceToHost, c->st);
-    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("permeabilidade_arestas", HMFEM_ERR_HIP);
+    download(c, perm, dperm, n * sizeof(ELE), "permeabilidade_arestas");
+    download(c, indcol, dind, n * sizeof(IND), "permeabilidade_arestas");
     return 0;
 }
 
@@ -1172,8 +1372,7 @@ extern "C" void source_term(Bound *B, ELE *perm, grid h, Dim N, FACE bt, double 
     double *db = upload(c, "b_tmp", b, n);
     int rc = launch_source_from_perm(c->st, &g, dperm, nullptr, dKI, dWI, dpc, dpw, dv, db);
     if (rc) fatal("source_term", rc);
-    (void)hipMemcpyAsync(b, db, n * 8, hipMemcpyDeviceToHost, c->st);
-    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("source_term", HMFEM_ERR_HIP);
+    download(c, b, db, n * 8, "source_term");
 }
 
 static int structured_from_csr(int n, const int *cnt, const int *col, const double *ele, Dims &d,
@@ -1259,8 +1458,7 @@ extern "C" int compute(int *hdr, int *cnt, int *col, double *ele, double *b, dou
     hmfem_status st{};
     int rc = solve_on_stencil(c, d, A, db, dx, &opt, &st);
     if (rc) fatal("compute", rc);
-    (void)hipMemcpyAsync(x, dx, (size_t)n * 8, hipMemcpyDeviceToHost, c->st);
-    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("compute", HMFEM_ERR_HIP);
+    download(c, x, dx, (size_t)n * 8, "compute");
     iter_amg = st.iterations + 1;
     return 0;
 }
@@ -1311,8 +1509,7 @@ extern "C" void solve_linsys(ELE *perm, IND *indcol, double *b, double *x, Dim N
         return;
     }
     if (rc < 0) fatal("solve_linsys", rc);
-    (void)hipMemcpyAsync(x, dx, n * 8, hipMemcpyDeviceToHost, c->st);
-    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("solve_linsys", HMFEM_ERR_HIP);
+    download(c, x, dx, n * 8, "solve_linsys");
     iter_amg = st.iterations + 1;
 }
 
@@ -1327,7 +1524,10 @@ extern "C" int fluxos(double *p, ELE *perm, ELE *v, Bound *B, grid h, Dim N)
     upload_bound(c, B, N, dv);
     ELE *dperm = upload(c, "perm", perm, n);
     double *dp = upload(c, "x", p, n);
-    ELE *dvout = upload(c, "vout", v, n);   // keeps v[s].c, which fluxos() does not write
+    // v[s].c is not written (as by the reference): the copy back skips it, or,
+    // without the pinned staging, v goes up first so that the copy keeps it
+    const bool skip_c = c->hc.setup();
+    ELE *dvout = skip_c ? (ELE *)c->get("vout", n * sizeof(ELE) + 16) : upload(c, "vout", v, n);
     double *part = (double *)c->get("mm_partial", (size_t)2 * grid_for(n, kBlock) * 8 + 64);
     unsigned *cnt = (unsigned *)c->get("mm_counter", 256);
     double *out = (double *)c->get("mm_out", 64);
@@ -1338,7 +1538,9 @@ extern "C" int fluxos(double *p, ELE *perm, ELE *v, Bound *B, grid h, Dim N)
     int rc = launch_flux(c->st, &g, nullptr, nullptr, nullptr, dperm, dv, dp, dvout, part, cnt, out);
     if (rc) fatal("fluxos", rc);
     double hmm[2];
-    (void)hipMemcpyAsync(v, dvout, n * sizeof(ELE), hipMemcpyDeviceToHost, c->st);
+    hipError_t e = skip_c ? c->hc.d2h(c->st, v, dvout, n * sizeof(ELE), sizeof(ELE), offsetof(ELE, c), sizeof(double))
+                          : c->hc.d2h(c->st, v, dvout, n * sizeof(ELE));
+    if (e != hipSuccess) fatal("fluxos", HMFEM_ERR_HIP);
     (void)hipMemcpyAsync(hmm, out, 16, hipMemcpyDeviceToHost, c->st);
     if (hipStreamSynchronize(c->st) != hipSuccess) fatal("fluxos", HMFEM_ERR_HIP);
     divergence_print(hmm[0], hmm[1]);
@@ -1376,8 +1578,7 @@ extern "C" void prod_mat_vec(ELE *perm, IND *indcol, double *x, double *res, Dim
     double *dres = upload(c, "res", res, n);
     int rc = launch_prod_mat_vec(c->st, (long)n, dperm, dind, dx, dres);
     if (rc) fatal("prod_mat_vec", rc);
-    (void)hipMemcpyAsync(res, dres, n * 8, hipMemcpyDeviceToHost, c->st);
-    if (hipStreamSynchronize(c->st) != hipSuccess) fatal("prod_mat_vec", HMFEM_ERR_HIP);
+    download(c, res, dres, n * 8, "prod_mat_vec");
 }
 
 // Recognise a count-CSR matrix that is a symmetric 7-point operator on an

@@ -87,10 +87,12 @@ def test_dropin_assembly_source_flux_bitexact(path):
     # fluxes of the reference's own tight pressure
     S.set_boundary(P.bc_type)
     v = np.zeros(n, S.ELE)
+    v["c"] = np.arange(n) + 0.5   # fluxos() leaves c as it was, as the reference's does
     S.fluxos(z["p12"], perm, v, faces, P.h, N)
     ov = O.fluxos(P, z["p12"], operm)
     for f in ("u", "d", "l", "r", "t", "b"):
         assert np.array_equal(v[f].view(np.uint64), ov[f].view(np.uint64)), f
+    assert np.array_equal(v["c"], np.arange(n) + 0.5)
     if "v" in z.files:
         ref = z["v"]
         for k, f in enumerate(("u", "d", "l", "r", "c", "t", "b")):
