@@ -211,12 +211,13 @@ k_restrict_z(OpArgs<T> o, int zlen, int cx, int cy, int cz, const FT *__restrict
     }
 }
 
-// Planes per block of k_restrict_z: even, at most 16, and enough blocks to
-// fill the device (2048 when the level allows it).
+// Planes per block of k_restrict_z: even, at most 12, and enough blocks to
+// fill the device (2048 when the level allows it). At most 12: 102 us per
+// 256^3 launch against 106 at 16 and 8, 107 at 32 and 109 at 24.
 inline int restrict_zlen(const Dims &d)
 {
     const long ntxy = (long)((d.nx + kGsTx - 1) / kGsTx) * ((d.ny + kGsTy - 1) / kGsTy);
-    long zl = 16;
+    long zl = 12;
     while (zl > 2 && ntxy * ((d.nz + zl - 1) / zl) < 2048) zl -= 2;
     return (int)zl;
 }
