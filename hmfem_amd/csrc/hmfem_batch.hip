@@ -1103,6 +1103,18 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     }
 
     HMFEM_CHECK(hipEventRecord(ev[0], st));
+    // conduit correction, first step: the per-instance histograms and their
+    // copy to the host, which overlaps the hierarchy's setup queued next
+    const Deflation *D = nullptr;
+    if (defl) {
+        defl->nc = 0;
+        defl->pending = false;
+        defl->gen++;
+        if (opt.deflate && dalloc) {
+            int rc = deflation_begin_batch(st, ninst, dims, A0, opt.deflate_ratio, *defl, dalloc, dctx);
+            if (rc) return rc;
+        }
+    }
     // hierarchy
     if (prec == HMFEM_PREC_FP64)
         hipLaunchKernelGGL(kb_to_f32, dim3((unsigned)((n0 + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, (long)n0,
@@ -1128,18 +1140,12 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     hipLaunchKernelGGL(kb_dense_invert, dim3(B.ninst), dim3(1024), 0, st, B.lev[B.nlev - 1].d,
                        lp<double>(B.lev[B.nlev - 1]), B.inv, B.info);
     HMFEM_CHECK(hipGetLastError());
-    // conduit correction: per-instance thresholds and clusters
-    const Deflation *D = nullptr;
-    if (defl) {
-        defl->nc = 0;
-        defl->pending = false;
-        defl->gen++;
-        if (opt.deflate && dalloc) {
-            int rc = deflation_setup_batch(st, ninst, dims, A0, opt.deflate_ratio, *defl, dalloc, dctx);
-            if (rc) return rc;
-            if (defl->nc > 0) D = defl;
-        }
+    // conduit correction, second step: per-instance thresholds and clusters
+    if (defl && defl->pending) {
+        int rc = deflation_finish_batch(st, A0, *defl, dalloc, dctx);
+        if (rc) return rc;
     }
+    if (defl && defl->nc > 0) D = defl;
     HMFEM_CHECK(hipMemsetAsync(B.never, 0, sizeof(int), st));
     HMFEM_CHECK(hipEventRecord(ev[1], st));
 

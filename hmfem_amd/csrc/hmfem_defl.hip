@@ -1068,6 +1068,14 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
 int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
                           Deflation &D, char *(*alloc)(void *, size_t, int), void *actx)
 {
+    int rc = deflation_begin_batch(st, ninst, dims, A, ratio, D, alloc, actx);
+    if (rc) return rc;
+    return deflation_finish_batch(st, A, D, alloc, actx);
+}
+
+int deflation_begin_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
+                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx)
+{
     D.nc = 0;
     D.m = 0;
     D.ninst = 0;
@@ -1108,9 +1116,27 @@ int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Sten
     if (stage_reserve(D, sizeof(unsigned) * kHistBins * (size_t)ninst)) return HMFEM_ERR_NOMEM;
     HMFEM_CHECK(hipMemcpyAsync(D.h_stage, hist, sizeof(unsigned) * kHistBins * (size_t)ninst, hipMemcpyDeviceToHost,
                                st));
-    HMFEM_CHECK(hipStreamSynchronize(st));
+    HMFEM_CHECK(hipEventRecord(D.hist_ready, st));
+    D.h.binst.assign((const char *)inst.data(), (const char *)inst.data() + sizeof(DefInst) * inst.size());
+    D.h.bdinst = dinst;
+    D.h.bbmap = bmap;
+    D.h.bG = G;
+    D.h.bratio = ratio;
+    D.pending = true;
+    return HMFEM_OK;
+}
+
+int deflation_finish_batch(hipStream_t st, const Stencil<double> &A, Deflation &D,
+                           char *(*alloc)(void *, size_t, int), void *actx)
+{
+    if (!D.pending) return HMFEM_OK;
+    D.pending = false;
+    std::vector<DefInst> inst(D.h.binst.size() / sizeof(DefInst));
+    memcpy((void *)inst.data(), D.h.binst.data(), D.h.binst.size());
+    HMFEM_CHECK(hipEventSynchronize(D.hist_ready));
     // finish_clusters reads the histograms before it reuses the staging
-    return finish_clusters(st, inst, dinst, bmap, G, A, (const unsigned *)D.h_stage, ratio, D, alloc, actx, false);
+    return finish_clusters(st, inst, (DefInst *)D.h.bdinst, D.h.bbmap, D.h.bG, A, (const unsigned *)D.h_stage,
+                           D.h.bratio, D, alloc, actx, false);
 }
 
 template <int NR>

@@ -208,6 +208,13 @@ struct CgBuffers {
 struct DeflationHost {
     std::vector<std::array<int, 3>> cand;   // components: root, size, instance
     std::vector<int> ptr, pbh, cph, irange, pc;
+    // a batch setup between deflation_begin_batch and deflation_finish_batch:
+    // the instance descriptors (bytes), their device copy, the block map
+    std::vector<char> binst;
+    void *bdinst = nullptr;
+    const int *bbmap = nullptr;
+    int bG = 0;
+    double bratio = 0.0;
 };
 
 struct Deflation {
@@ -260,6 +267,13 @@ void deflation_sums(hipStream_t st, const Deflation &D, const double *r, const d
 // dot partials go to D.pext. `never` points at a device int that stays 0.
 int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
                           Deflation &D, char *(*alloc)(void *, size_t, int), void *actx);
+// The same in two steps, so that the histograms' round trip to the host
+// overlaps other queued work (the batch's hierarchy setup): begin queues the
+// per-instance histograms and their copy, finish decides and builds.
+int deflation_begin_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
+                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx);
+int deflation_finish_batch(hipStream_t st, const Stencil<double> &A, Deflation &D,
+                           char *(*alloc)(void *, size_t, int), void *actx);
 void deflation_sums_batch(hipStream_t st, const Deflation &D, const double *r, const double *v, const int *never);
 void deflation_scatter(hipStream_t st, const Deflation &D, float *z, const int *done);
 void deflation_scatter(hipStream_t st, const Deflation &D, double *z, const int *done);
