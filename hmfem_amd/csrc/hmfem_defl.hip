@@ -39,8 +39,19 @@ constexpr int kHistBins = Deflation::kHistBins;
 constexpr int kMinCluster = 8;
 constexpr int kMaxClusters = 256;
 
+// Bin of a diagonal value: floor(log2 d) + 1100, clamped to the table (d <= 0
+// or NaN: bin 0).
+__device__ __forceinline__ int diag_bin(double d)
+{
+    int b = d > 0 ? ilogb(d) + 1100 : 0;
+    return b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
+}
+
+// The histograms also keep each cell's bin (2 bytes a cell), from which the
+// strong-cell mask decides every cell outside the threshold's own bin without
+// reading the stencil again.
 __global__ void __launch_bounds__(kBlock)
-k_diag_hist(OpArgs<double> o, unsigned *hist)
+k_diag_hist(OpArgs<double> o, unsigned *hist, unsigned short *bins)
 {
     __shared__ unsigned h[kHistBins];
     for (int q = threadIdx.x; q < kHistBins; q += kBlock) h[q] = 0;
@@ -50,9 +61,8 @@ k_diag_hist(OpArgs<double> o, unsigned *hist)
     unsigned run = 0;
     for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
         Cell c = cell_at(o, s);
-        const double d = diag_at<double, double>(o, c);
-        int b = d > 0 ? ilogb(d) + 1100 : 0;
-        b = b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
+        const int b = diag_bin(diag_at<double, double>(o, c));
+        bins[s] = (unsigned short)b;
         if (b != run_bin) {
             if (run) atomicAdd(&h[run_bin], run);
             run_bin = b;
@@ -75,6 +85,7 @@ struct DefInst {
     int off, blk0;
     double thr;   // strong-cell threshold of the instance (inf: none)
     int hblk0;    // first block of the instance in the histogram launch
+    int tb;       // diagonal bin of thr (kHistBins: none)
 };
 
 constexpr int kHistCells = 16;   // cells per thread of the batch histogram
@@ -95,7 +106,8 @@ __device__ __forceinline__ bool inst_cell(const DefInst *__restrict__ I, const i
 // Per-instance histogram of floor(log2 d); block b covers kBlock * kHistCells
 // cells of instance hmap[b].
 __global__ void __launch_bounds__(kBlock)
-k_diag_hist_inst(const DefInst *__restrict__ I, const int *__restrict__ hmap, Stencil<double> A, unsigned *hist)
+k_diag_hist_inst(const DefInst *__restrict__ I, const int *__restrict__ hmap, Stencil<double> A, unsigned *hist,
+                 unsigned short *bins)
 {
     __shared__ unsigned h[kHistBins];
     for (int q = threadIdx.x; q < kHistBins; q += kBlock) h[q] = 0;
@@ -114,9 +126,8 @@ k_diag_hist_inst(const DefInst *__restrict__ I, const int *__restrict__ hmap, St
     for (int t = 0; t < kHistCells; t++) {
         const int l = l0 + t * kBlock;
         if (l >= in.n) break;
-        const double d = diag_at<double, double>(o, cell_at(o, l));
-        int b = d > 0 ? ilogb(d) + 1100 : 0;
-        b = b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
+        const int b = diag_bin(diag_at<double, double>(o, cell_at(o, l)));
+        bins[in.off + l] = (unsigned short)b;
         if (b != run_bin) {
             if (run) atomicAdd(&h[run_bin], run);
             run_bin = b;
@@ -132,19 +143,26 @@ k_diag_hist_inst(const DefInst *__restrict__ I, const int *__restrict__ hmap, St
 
 // Strong cells (diagonal above the instance's threshold) in index order, in
 // three steps: a bit mask per wave and a count per block, an exclusive scan
-// of the counts, and the ordered list of global indices.
+// of the counts, and the ordered list of global indices. A cell whose bin
+// lies above the threshold's bin tb is strong (d >= 2^(tb + 1 - 1100) >
+// thr), one below it is not (d < 2^(tb - 1100) <= thr); only cells in bin tb
+// (which also holds the clamped ends of the table when tb is an end) compare
+// their diagonal with thr.
 constexpr int kWavesPerBlock = kBlock / 64;
 
 __global__ void __launch_bounds__(kBlock)
 k_strong_mask(const DefInst *__restrict__ I, const int *__restrict__ bmap, Stencil<double> A,
-              unsigned long long *mask, int *bcnt)
+              const unsigned short *__restrict__ bins, unsigned long long *mask, int *bcnt)
 {
     __shared__ int wc[kWavesPerBlock];
     DefInst in;
     OpArgs<double> o;
     Cell c;
     bool f = false;
-    if (inst_cell(I, bmap, A, in, o, c)) f = diag_at<double, double>(o, c) > in.thr;
+    if (inst_cell(I, bmap, A, in, o, c)) {
+        const int b = bins[in.off + (blockIdx.x - in.blk0) * kBlock + threadIdx.x];
+        f = b > in.tb || (b == in.tb && diag_at<double, double>(o, c) > in.thr);
+    }
     const unsigned long long m = __ballot(f);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
@@ -832,7 +850,8 @@ static int stage_reserve(Deflation &D, size_t bytes)
 // from the table of components (root, size, instance) and lays out the
 // per-cluster index arrays: one round trip of a few KB.
 static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *dinst, const int *bmap, int G,
-                           const Stencil<double> &A, const unsigned *hist, double ratio, Deflation &D,
+                           const Stencil<double> &A, const unsigned *hist, const unsigned short *bins, double ratio,
+                           Deflation &D,
                            char *(*alloc)(void *, size_t, int), void *actx, bool records)
 {
     const int ninst = (int)inst.size();
@@ -843,6 +862,7 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
         const unsigned *h = hist + (long)q * kHistBins;
         DefInst &in = inst[q];
         in.thr = INFINITY;
+        in.tb = kHistBins;
         long acc = 0;
         int mb = 0;
         for (; mb < kHistBins; mb++) {
@@ -856,6 +876,7 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
         for (int b = tb; b < kHistBins; b++) uq += h[b];
         if (aq < kMinCluster) continue;
         in.thr = thr;
+        in.tb = std::min(std::max(tb, 0), kHistBins - 1);
         above += aq;
         ub += uq;
     }
@@ -890,7 +911,7 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     w.sel = w.ccount + (long)nchunks * kMaxClusters;
     const int *pms = boff + G;   // the number of strong cells (capped at cap by every kernel)
     HMFEM_CHECK(hipMemcpyAsync(dinst, inst.data(), sizeof(DefInst) * ninst, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_strong_mask, dim3(G), dim3(kBlock), 0, st, dinst, bmap, A, mask, boff);
+    hipLaunchKernelGGL(k_strong_mask, dim3(G), dim3(kBlock), 0, st, dinst, bmap, A, bins, mask, boff);
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, st, boff, G, tot);
     hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, st, tot, nt, boff, G);
     hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(kScanThreads), 0, st, boff, G, (const int *)tot);
@@ -1023,6 +1044,10 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     return HMFEM_OK;
 }
 
+// Slot 0 of a single-grid setup: the histogram, the instance descriptor, then
+// the cells' bins.
+static size_t slot0_bins_offset() { return (kHistBins * sizeof(unsigned) + 256 + sizeof(DefInst) + 255) & ~(size_t)255; }
+
 int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation &D,
                     char *(*alloc)(void *, size_t, int), void *actx)
 {
@@ -1035,15 +1060,15 @@ int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation 
     OpArgs<double> o;
     o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
     o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
-    // slot 0: the histogram, then the instance descriptor
-    char *h0 = alloc(actx, kHistBins * sizeof(unsigned) + 256 + sizeof(DefInst), 0);
+    char *h0 = alloc(actx, slot0_bins_offset() + sizeof(unsigned short) * (size_t)n, 0);
     if (!h0) return HMFEM_ERR_NOMEM;
     unsigned *hist = (unsigned *)h0;
+    unsigned short *bins = (unsigned short *)(h0 + slot0_bins_offset());
     HMFEM_CHECK(hipMemsetAsync(hist, 0, kHistBins * sizeof(unsigned), st));
     // a grid-stride loop over at most 512 blocks keeps the global bin updates
     // few (2048 blocks: 35 us instead of 25 at 128^3)
     const int G = std::min(grid_for(n, kBlock), 512);
-    hipLaunchKernelGGL(k_diag_hist, dim3(G), dim3(kBlock), 0, st, o, hist);
+    hipLaunchKernelGGL(k_diag_hist, dim3(G), dim3(kBlock), 0, st, o, hist, bins);
     HMFEM_CHECK(hipMemcpyAsync(D.h_hist, hist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HMFEM_CHECK(hipEventRecord(D.hist_ready, st));
     D.pending = true;
@@ -1057,12 +1082,12 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     D.pending = false;
     if (!(ratio > 1.0)) return HMFEM_OK;
     const long n = d.n();
-    DefInst *dinst = (DefInst *)(alloc(actx, kHistBins * sizeof(unsigned) + 256 + sizeof(DefInst), 0) +
-                                 kHistBins * sizeof(unsigned));
-    std::vector<DefInst> inst(1, DefInst{d.nx, d.ny, d.nz, (int)n, 0, 0, INFINITY, 0});
+    char *h0 = alloc(actx, slot0_bins_offset() + sizeof(unsigned short) * (size_t)n, 0);
+    DefInst *dinst = (DefInst *)(h0 + kHistBins * sizeof(unsigned));
+    std::vector<DefInst> inst(1, DefInst{d.nx, d.ny, d.nz, (int)n, 0, 0, INFINITY, 0, kHistBins});
     HMFEM_CHECK(hipEventSynchronize(D.hist_ready));
-    return finish_clusters(st, inst, dinst, nullptr, (int)((n + kBlock - 1) / kBlock), A, D.h_hist, ratio, D,
-                           alloc, actx, records);
+    return finish_clusters(st, inst, dinst, nullptr, (int)((n + kBlock - 1) / kBlock), A, D.h_hist,
+                           (const unsigned short *)(h0 + slot0_bins_offset()), ratio, D, alloc, actx, records);
 }
 
 int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
@@ -1092,25 +1117,29 @@ int deflation_begin_batch(hipStream_t st, int ninst, const int *dims, const Sten
         in.blk0 = (int)bm.size();
         in.hblk0 = (int)hm.size();
         in.thr = INFINITY;
+        in.tb = kHistBins;
         bm.insert(bm.end(), (in.n + kBlock - 1) / kBlock, q);
         hm.insert(hm.end(), (in.n + kBlock * kHistCells - 1) / (kBlock * kHistCells), q);
         off += in.n;
     }
     const int G = (int)bm.size(), GH = (int)hm.size();
-    // slot 0: per-instance histograms, the block maps, the instance descriptors
+    // slot 0: per-instance histograms, the block maps, the instance
+    // descriptors, the cells' bins
     auto up256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t hb = up256(sizeof(unsigned) * kHistBins * (size_t)ninst), bb = up256(sizeof(int) * (size_t)(G + GH));
-    char *h0 = alloc(actx, hb + bb + sizeof(DefInst) * ninst, 0);
+    const size_t ib = up256(sizeof(DefInst) * ninst);
+    char *h0 = alloc(actx, hb + bb + ib + sizeof(unsigned short) * (size_t)off, 0);
     if (!h0) return HMFEM_ERR_NOMEM;
     unsigned *hist = (unsigned *)h0;
     int *bmap = (int *)(h0 + hb), *hmap = bmap + G;
     DefInst *dinst = (DefInst *)(h0 + hb + bb);
+    unsigned short *bins = (unsigned short *)(h0 + hb + bb + ib);
     bm.insert(bm.end(), hm.begin(), hm.end());
     HMFEM_CHECK(hipMemsetAsync(hist, 0, sizeof(unsigned) * kHistBins * (size_t)ninst, st));
     HMFEM_CHECK(hipMemcpyAsync(bmap, bm.data(), sizeof(int) * (G + GH), hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipMemcpyAsync(dinst, inst.data(), sizeof(DefInst) * ninst, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_diag_hist_inst, dim3(GH), dim3(kBlock), 0, st, (const DefInst *)dinst, (const int *)hmap, A,
-                       hist);
+                       hist, bins);
     // the histograms go through the pinned staging, which finish_clusters
     // reads before it reuses (and may regrow) it
     if (stage_reserve(D, sizeof(unsigned) * kHistBins * (size_t)ninst)) return HMFEM_ERR_NOMEM;
@@ -1120,6 +1149,7 @@ int deflation_begin_batch(hipStream_t st, int ninst, const int *dims, const Sten
     D.h.binst.assign((const char *)inst.data(), (const char *)inst.data() + sizeof(DefInst) * inst.size());
     D.h.bdinst = dinst;
     D.h.bbmap = bmap;
+    D.h.bbins = bins;
     D.h.bG = G;
     D.h.bratio = ratio;
     D.pending = true;
@@ -1136,7 +1166,7 @@ int deflation_finish_batch(hipStream_t st, const Stencil<double> &A, Deflation &
     HMFEM_CHECK(hipEventSynchronize(D.hist_ready));
     // finish_clusters reads the histograms before it reuses the staging
     return finish_clusters(st, inst, (DefInst *)D.h.bdinst, D.h.bbmap, D.h.bG, A, (const unsigned *)D.h_stage,
-                           D.h.bratio, D, alloc, actx, false);
+                           D.h.bbins, D.h.bratio, D, alloc, actx, false);
 }
 
 template <int NR>

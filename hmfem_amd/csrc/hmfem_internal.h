@@ -209,10 +209,12 @@ struct DeflationHost {
     std::vector<std::array<int, 3>> cand;   // components: root, size, instance
     std::vector<int> ptr, pbh, cph, irange, pc;
     // a batch setup between deflation_begin_batch and deflation_finish_batch:
-    // the instance descriptors (bytes), their device copy, the block map
+    // the instance descriptors (bytes), their device copy, the block map, the
+    // cells' diagonal bins
     std::vector<char> binst;
     void *bdinst = nullptr;
     const int *bbmap = nullptr;
+    const unsigned short *bbins = nullptr;
     int bG = 0;
     double bratio = 0.0;
 };
