@@ -421,6 +421,60 @@ kb_to_f32_dinv(const BInst *__restrict__ D, const int *__restrict__ bmap, LPtr<d
     to_f32_dinv_cell(o, cell_at(o, s), e32 + I.off, wx32 + I.off, wy32 + I.off, wz32 + I.off, di32 + I.off);
 }
 
+// The same pass, which also takes the conduit correction's per-instance
+// histograms of the FP64 diagonal's bins (kHistBins per instance) and keeps
+// each cell's bin. Block b covers the fine level's blocks [b kConvSub, (b + 1)
+// kConvSub): the bins of the instance of its first block are summed in LDS
+// (runs of equal bins counted in a register first), those of a later
+// instance (a block that crosses an instance's end) go to the global bins
+// directly. Per-launch atomics on the crowded bins stay few: one flush per
+// 4096 cells instead of one per wave.
+constexpr int kConvSub = 16;
+
+__global__ void __launch_bounds__(kBlock)
+kb_to_f32_dinv_hist(const BInst *__restrict__ D, const int *__restrict__ bmap, int nblk, LPtr<double> A,
+                    float *e32, float *wx32, float *wy32, float *wz32, float *di32, unsigned *hist,
+                    unsigned short *bins)
+{
+    constexpr int NB = Deflation::kHistBins;
+    __shared__ unsigned h[NB];
+    for (int t = threadIdx.x; t < NB; t += kBlock) h[t] = 0;
+    __syncthreads();
+    const int sb0 = blockIdx.x * kConvSub;
+    const int q0 = bmap[sb0];
+    int rq = q0, rb = 0;
+    unsigned run = 0;
+    auto flush = [&]() {
+        if (!run) return;
+        if (rq == q0) atomicAdd(&h[rb], run);
+        else atomicAdd(&hist[(long)rq * NB + rb], run);
+    };
+    for (int t = 0; t < kConvSub; t++) {
+        const int sb = sb0 + t;
+        if (sb >= nblk) break;
+        const int q = bmap[sb];
+        const BInst I = D[q];
+        const int s = (sb - I.blk0) * kBlock + threadIdx.x;
+        if (s >= I.n) continue;
+        const OpArgs<double> o = bop(A, I.off, I.nx, I.ny, I.nz);
+        const Cell c = cell_at(o, s);
+        const int b = diag_bin(diag_at<double, double>(o, c), NB);
+        bins[I.off + s] = (unsigned short)b;
+        to_f32_dinv_cell(o, c, e32 + I.off, wx32 + I.off, wy32 + I.off, wz32 + I.off, di32 + I.off);
+        if (q != rq || b != rb) {
+            flush();
+            rq = q;
+            rb = b;
+            run = 0;
+        }
+        run++;
+    }
+    flush();
+    __syncthreads();
+    for (int t = threadIdx.x; t < NB; t += kBlock)
+        if (h[t]) atomicAdd(&hist[(long)q0 * NB + t], h[t]);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_dinv(const BInst *__restrict__ D, const int *__restrict__ bmap, LPtr<T> A, T *di)
@@ -1103,27 +1157,39 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     }
 
     HMFEM_CHECK(hipEventRecord(ev[0], st));
-    // conduit correction, first step: the per-instance histograms and their
-    // copy to the host, which overlaps the hierarchy's setup queued next
+    // conduit correction, first step: the per-instance histograms (in the
+    // FP32 hierarchy, taken by the fine level's conversion pass) and their
+    // copy to the host, which overlaps the rest of the hierarchy's setup
     const Deflation *D = nullptr;
+    const bool fuse_hist = prec != HMFEM_PREC_FP64;
     if (defl) {
         defl->nc = 0;
         defl->pending = false;
         defl->gen++;
         if (opt.deflate && dalloc) {
-            int rc = deflation_begin_batch(st, ninst, dims, A0, opt.deflate_ratio, *defl, dalloc, dctx);
+            int rc = deflation_begin_batch(st, ninst, dims, A0, opt.deflate_ratio, *defl, dalloc, dctx, fuse_hist);
             if (rc) return rc;
         }
     }
+    const bool hist_here = fuse_hist && defl && defl->pending;
     // hierarchy
     if (prec == HMFEM_PREC_FP64)
         hipLaunchKernelGGL(kb_to_f32, dim3((unsigned)((n0 + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, (long)n0,
                            B.lev[0].e64, B.lev[0].wx64, B.lev[0].wy64, B.lev[0].wz64, B.lev[0].e32, B.lev[0].wx32,
                            B.lev[0].wy32, B.lev[0].wz32);
+    else if (hist_here)   // with the fine level's inverse diagonal and the conduit correction's histograms
+        hipLaunchKernelGGL(kb_to_f32_dinv_hist, dim3((B.lev[0].nblk + kConvSub - 1) / kConvSub), dim3(kBlock), 0, st,
+                           B.lev[0].d, B.lev[0].bmap, B.lev[0].nblk, lp<double>(B.lev[0]), B.lev[0].e32,
+                           B.lev[0].wx32, B.lev[0].wy32, B.lev[0].wz32, B.lev[0].di32, defl->h.bhist,
+                           defl->h.bbins);
     else   // with the fine level's inverse diagonal
         hipLaunchKernelGGL(kb_to_f32_dinv, dim3(B.lev[0].nblk), dim3(kBlock), 0, st, B.lev[0].d, B.lev[0].bmap,
                            lp<double>(B.lev[0]), B.lev[0].e32, B.lev[0].wx32, B.lev[0].wy32, B.lev[0].wz32,
                            B.lev[0].di32);
+    if (hist_here) {
+        int rc = deflation_hist_queued_batch(st, *defl);
+        if (rc) return rc;
+    }
     for (int l = 0; l + 1 < B.nlev; l++) {
         BLevel &L = B.lev[l], &C = B.lev[l + 1];
         hipLaunchKernelGGL(kb_rap, dim3(C.nblk), dim3(kBlock), 0, st, L.d, C.d, C.bmap, lp<double>(L), C.e64, C.wx64,

@@ -39,14 +39,6 @@ constexpr int kHistBins = Deflation::kHistBins;
 constexpr int kMinCluster = 8;
 constexpr int kMaxClusters = 256;
 
-// Bin of a diagonal value: floor(log2 d) + 1100, clamped to the table (d <= 0
-// or NaN: bin 0).
-__device__ __forceinline__ int diag_bin(double d)
-{
-    int b = d > 0 ? ilogb(d) + 1100 : 0;
-    return b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
-}
-
 // The histograms also keep each cell's bin (2 bytes a cell), from which the
 // strong-cell mask decides every cell outside the threshold's own bin without
 // reading the stencil again.
@@ -61,7 +53,7 @@ k_diag_hist(OpArgs<double> o, unsigned *hist, unsigned short *bins)
     unsigned run = 0;
     for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
         Cell c = cell_at(o, s);
-        const int b = diag_bin(diag_at<double, double>(o, c));
+        const int b = diag_bin(diag_at<double, double>(o, c), kHistBins);
         bins[s] = (unsigned short)b;
         if (b != run_bin) {
             if (run) atomicAdd(&h[run_bin], run);
@@ -126,7 +118,7 @@ k_diag_hist_inst(const DefInst *__restrict__ I, const int *__restrict__ hmap, St
     for (int t = 0; t < kHistCells; t++) {
         const int l = l0 + t * kBlock;
         if (l >= in.n) break;
-        const int b = diag_bin(diag_at<double, double>(o, cell_at(o, l)));
+        const int b = diag_bin(diag_at<double, double>(o, cell_at(o, l)), kHistBins);
         bins[in.off + l] = (unsigned short)b;
         if (b != run_bin) {
             if (run) atomicAdd(&h[run_bin], run);
@@ -1090,16 +1082,8 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
                            (const unsigned short *)(h0 + slot0_bins_offset()), ratio, D, alloc, actx, records);
 }
 
-int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
-                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx)
-{
-    int rc = deflation_begin_batch(st, ninst, dims, A, ratio, D, alloc, actx);
-    if (rc) return rc;
-    return deflation_finish_batch(st, A, D, alloc, actx);
-}
-
 int deflation_begin_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
-                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx)
+                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx, bool external_hist)
 {
     D.nc = 0;
     D.m = 0;
@@ -1138,21 +1122,29 @@ int deflation_begin_batch(hipStream_t st, int ninst, const int *dims, const Sten
     HMFEM_CHECK(hipMemsetAsync(hist, 0, sizeof(unsigned) * kHistBins * (size_t)ninst, st));
     HMFEM_CHECK(hipMemcpyAsync(bmap, bm.data(), sizeof(int) * (G + GH), hipMemcpyHostToDevice, st));
     HMFEM_CHECK(hipMemcpyAsync(dinst, inst.data(), sizeof(DefInst) * ninst, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_diag_hist_inst, dim3(GH), dim3(kBlock), 0, st, (const DefInst *)dinst, (const int *)hmap, A,
-                       hist, bins);
-    // the histograms go through the pinned staging, which finish_clusters
-    // reads before it reuses (and may regrow) it
-    if (stage_reserve(D, sizeof(unsigned) * kHistBins * (size_t)ninst)) return HMFEM_ERR_NOMEM;
-    HMFEM_CHECK(hipMemcpyAsync(D.h_stage, hist, sizeof(unsigned) * kHistBins * (size_t)ninst, hipMemcpyDeviceToHost,
-                               st));
-    HMFEM_CHECK(hipEventRecord(D.hist_ready, st));
     D.h.binst.assign((const char *)inst.data(), (const char *)inst.data() + sizeof(DefInst) * inst.size());
     D.h.bdinst = dinst;
     D.h.bbmap = bmap;
+    D.h.bhist = hist;
     D.h.bbins = bins;
     D.h.bG = G;
     D.h.bratio = ratio;
     D.pending = true;
+    if (external_hist) return HMFEM_OK;
+    hipLaunchKernelGGL(k_diag_hist_inst, dim3(GH), dim3(kBlock), 0, st, (const DefInst *)dinst, (const int *)hmap, A,
+                       hist, bins);
+    return deflation_hist_queued_batch(st, D);
+}
+
+int deflation_hist_queued_batch(hipStream_t st, Deflation &D)
+{
+    if (!D.pending) return HMFEM_OK;
+    // the histograms go through the pinned staging, which finish_clusters
+    // reads before it reuses (and may regrow) it
+    const size_t bytes = sizeof(unsigned) * kHistBins * (D.h.binst.size() / sizeof(DefInst));
+    if (stage_reserve(D, bytes)) return HMFEM_ERR_NOMEM;
+    HMFEM_CHECK(hipMemcpyAsync(D.h_stage, D.h.bhist, bytes, hipMemcpyDeviceToHost, st));
+    HMFEM_CHECK(hipEventRecord(D.hist_ready, st));
     return HMFEM_OK;
 }
 

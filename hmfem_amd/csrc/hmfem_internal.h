@@ -214,7 +214,8 @@ struct DeflationHost {
     std::vector<char> binst;
     void *bdinst = nullptr;
     const int *bbmap = nullptr;
-    const unsigned short *bbins = nullptr;
+    unsigned *bhist = nullptr;
+    unsigned short *bbins = nullptr;
     int bG = 0;
     double bratio = 0.0;
 };
@@ -267,13 +268,16 @@ void deflation_sums(hipStream_t st, const Deflation &D, const double *r, const d
 // Batch: the instances (dims[3q..3q+2], concatenated in one index space) get
 // their own thresholds and clusters; per CG iteration the per-instance extra
 // dot partials go to D.pext. `never` points at a device int that stays 0.
-int deflation_setup_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
-                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx);
-// The same in two steps, so that the histograms' round trip to the host
-// overlaps other queued work (the batch's hierarchy setup): begin queues the
-// per-instance histograms and their copy, finish decides and builds.
+// It runs in steps, so that the histograms' round trip to the host overlaps
+// other queued work (the batch's hierarchy setup): begin queues the
+// per-instance histograms and their copy, finish decides and builds. With
+// external_hist, begin only clears the histograms (D.h.bhist, kHistBins per
+// instance) and leaves them and the cells' bins (D.h.bbins, by batch index)
+// to a pass of the caller's (kb_to_f32_dinv), after which
+// deflation_hist_queued_batch queues the copy.
 int deflation_begin_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,
-                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx);
+                          Deflation &D, char *(*alloc)(void *, size_t, int), void *actx, bool external_hist = false);
+int deflation_hist_queued_batch(hipStream_t st, Deflation &D);
 int deflation_finish_batch(hipStream_t st, const Stencil<double> &A, Deflation &D,
                            char *(*alloc)(void *, size_t, int), void *actx);
 void deflation_sums_batch(hipStream_t st, const Deflation &D, const double *r, const double *v, const int *never);

@@ -72,6 +72,14 @@ __device__ __forceinline__ void to_f32_dinv_cell(const OpArgs<double> &o, const 
     di32[s] = (float)(1.0 / d);
 }
 
+// Bin of a diagonal value for the conduit correction's histograms:
+// floor(log2 d) + 1100, clamped to [0, nbins) (d <= 0 or NaN: bin 0).
+__device__ __forceinline__ int diag_bin(double d, int nbins)
+{
+    int b = d > 0 ? ilogb(d) + 1100 : 0;
+    return b < 0 ? 0 : (b >= nbins ? nbins - 1 : b);
+}
+
 // (A u)_s in flux form and the diagonal, for a functor u(Cell) giving the
 // value of u at a cell (so callers can build u on the fly).
 template <typename T, typename C, typename U>
