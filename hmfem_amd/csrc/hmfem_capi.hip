@@ -355,6 +355,8 @@ extern "C" int hmfem_create(hmfem_context **out)
     HMFEM_CHECK(hipHostMalloc((void **)&c->defl.h_hist, Deflation::kHistBins * sizeof(unsigned), hipHostMallocDefault));
     HMFEM_CHECK(hipEventCreateWithFlags(&c->defl.hist_ready, hipEventDisableTiming));
     HMFEM_CHECK(hipEventCreateWithFlags(&c->defl.stage_done, hipEventDisableTiming));
+    HMFEM_CHECK(hipStreamCreateWithFlags(&c->defl.side, hipStreamNonBlocking));
+    HMFEM_CHECK(hipEventCreateWithFlags(&c->defl.side_done, hipEventDisableTiming));
     c->scratch = scratch_for_current_device(&c->scratch_rc);
     int dev = 0, lds = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -368,6 +370,7 @@ extern "C" void hmfem_destroy(hmfem_context *c)
 {
     if (!c) return;
     (void)hipStreamSynchronize(c->st);
+    if (c->defl.side) (void)hipStreamSynchronize(c->defl.side);
     for (auto &kv : c->bufs)
         if (kv.second.first) (void)hipFree(kv.second.first);
     for (auto e : c->prof_ev) (void)hipEventDestroy(e);
@@ -378,6 +381,8 @@ extern "C" void hmfem_destroy(hmfem_context *c)
     if (c->defl.hist_ready) (void)hipEventDestroy(c->defl.hist_ready);
     if (c->defl.stage_done) (void)hipEventDestroy(c->defl.stage_done);
     if (c->defl.h_stage) (void)hipHostFree(c->defl.h_stage);
+    if (c->defl.side_done) (void)hipEventDestroy(c->defl.side_done);
+    if (c->defl.side) (void)hipStreamDestroy(c->defl.side);
     (void)hipStreamDestroy(c->st);
     if (c == g_default) g_default = nullptr;
     if (c == g_default_1d) g_default_1d = nullptr;

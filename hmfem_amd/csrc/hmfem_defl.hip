@@ -82,17 +82,12 @@ struct DefInst {
 
 constexpr int kHistCells = 16;   // cells per thread of the batch histogram
 
-__device__ __forceinline__ bool inst_cell(const DefInst *__restrict__ I, const int *__restrict__ bmap,
-                                          const Stencil<double> &A, DefInst &in, OpArgs<double> &o, Cell &c)
+__device__ __forceinline__ OpArgs<double> inst_op(const DefInst &in, const Stencil<double> &A)
 {
-    const int q = bmap ? bmap[blockIdx.x] : 0;
-    in = I[q];
-    const int l = (blockIdx.x - in.blk0) * kBlock + threadIdx.x;
+    OpArgs<double> o;
     o.e = A.e + in.off; o.wx = A.wx + in.off; o.wy = A.wy + in.off; o.wz = A.wz + in.off; o.di = nullptr;
     o.nx = in.nx; o.ny = in.ny; o.nz = in.nz;
-    if (l >= in.n) return false;
-    c = cell_at(o, l);
-    return true;
+    return o;
 }
 
 // Per-instance histogram of floor(log2 d); block b covers kBlock * kHistCells
@@ -139,33 +134,45 @@ k_diag_hist_inst(const DefInst *__restrict__ I, const int *__restrict__ hmap, St
 // lies above the threshold's bin tb is strong (d >= 2^(tb + 1 - 1100) >
 // thr), one below it is not (d < 2^(tb - 1100) <= thr); only cells in bin tb
 // (which also holds the clamped ends of the table when tb is an end) compare
-// their diagonal with thr.
+// their diagonal with thr. The mask and the list are kept per group of
+// kBlock cells (group g: cells of the per-cell block map's block g); a
+// workgroup of these two kernels takes kMaskSub groups, as one group is too
+// little work for a workgroup's launch.
 constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kMaskSub = 8;
 
 __global__ void __launch_bounds__(kBlock)
-k_strong_mask(const DefInst *__restrict__ I, const int *__restrict__ bmap, Stencil<double> A,
+k_strong_mask(const DefInst *__restrict__ I, const int *__restrict__ bmap, int G, Stencil<double> A,
               const unsigned short *__restrict__ bins, unsigned long long *mask, int *bcnt)
 {
-    __shared__ int wc[kWavesPerBlock];
-    DefInst in;
-    OpArgs<double> o;
-    Cell c;
-    bool f = false;
-    if (inst_cell(I, bmap, A, in, o, c)) {
-        const int b = bins[in.off + (blockIdx.x - in.blk0) * kBlock + threadIdx.x];
-        f = b > in.tb || (b == in.tb && diag_at<double, double>(o, c) > in.thr);
-    }
-    const unsigned long long m = __ballot(f);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        mask[(long)blockIdx.x * kWavesPerBlock + w] = m;
-        wc[w] = __popcll(m);
+    __shared__ int wc[kMaskSub][kWavesPerBlock];
+    const int w = threadIdx.x >> 6, g0 = blockIdx.x * kMaskSub;
+    for (int t = 0; t < kMaskSub; t++) {
+        const int g = g0 + t;
+        if (g >= G) break;
+        const DefInst in = I[bmap ? bmap[g] : 0];
+        const int l = (g - in.blk0) * kBlock + threadIdx.x;
+        bool f = false;
+        if (l < in.n) {
+            const int b = bins[in.off + l];
+            if (b == in.tb) {
+                const OpArgs<double> o = inst_op(in, A);
+                f = diag_at<double, double>(o, cell_at(o, l)) > in.thr;
+            } else {
+                f = b > in.tb;
+            }
+        }
+        const unsigned long long m = __ballot(f);
+        if ((threadIdx.x & 63) == 0) {
+            mask[(long)g * kWavesPerBlock + w] = m;
+            wc[t][w] = __popcll(m);
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kMaskSub && g0 + (int)threadIdx.x < G) {
         int t = 0;
-        for (int q = 0; q < kWavesPerBlock; q++) t += wc[q];
-        bcnt[blockIdx.x] = t;
+        for (int q = 0; q < kWavesPerBlock; q++) t += wc[threadIdx.x][q];
+        bcnt[g0 + threadIdx.x] = t;
     }
 }
 
@@ -229,17 +236,21 @@ k_scan_add(int *cnt, int G, const int *tot)
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_strong_emit(const DefInst *__restrict__ I, const int *__restrict__ bmap, const unsigned long long *__restrict__ mask,
-              const int *__restrict__ boff, int *list, int cap)
+k_strong_emit(const DefInst *__restrict__ I, const int *__restrict__ bmap, int G,
+              const unsigned long long *__restrict__ mask, const int *__restrict__ boff, int *list, int cap)
 {
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const unsigned long long m = mask[(long)blockIdx.x * kWavesPerBlock + w];
-    if (!((m >> lane) & 1ull)) return;
-    int at = boff[blockIdx.x];
-    for (int q = 0; q < w; q++) at += __popcll(mask[(long)blockIdx.x * kWavesPerBlock + q]);
-    at += __popcll(m & ((1ull << lane) - 1ull));
-    const DefInst in = I[bmap ? bmap[blockIdx.x] : 0];
-    if (at < cap) list[at] = in.off + (blockIdx.x - in.blk0) * kBlock + threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g0 = blockIdx.x * kMaskSub;
+    for (int t = 0; t < kMaskSub; t++) {
+        const int g = g0 + t;
+        if (g >= G) break;
+        const unsigned long long m = mask[(long)g * kWavesPerBlock + w];
+        if (!((m >> lane) & 1ull)) continue;
+        int at = boff[g];
+        for (int q = 0; q < w; q++) at += __popcll(mask[(long)g * kWavesPerBlock + q]);
+        at += __popcll(m & ((1ull << lane) - 1ull));
+        const DefInst in = I[bmap ? bmap[g] : 0];
+        if (at < cap) list[at] = in.off + (g - in.blk0) * kBlock + threadIdx.x;
+    }
 }
 
 // The per-iteration kernels take NR interleaved right-hand sides (vectors
@@ -903,11 +914,12 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
     w.sel = w.ccount + (long)nchunks * kMaxClusters;
     const int *pms = boff + G;   // the number of strong cells (capped at cap by every kernel)
     HMFEM_CHECK(hipMemcpyAsync(dinst, inst.data(), sizeof(DefInst) * ninst, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_strong_mask, dim3(G), dim3(kBlock), 0, st, dinst, bmap, A, bins, mask, boff);
+    const dim3 gm((G + kMaskSub - 1) / kMaskSub);
+    hipLaunchKernelGGL(k_strong_mask, gm, dim3(kBlock), 0, st, dinst, bmap, G, A, bins, mask, boff);
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, st, boff, G, tot);
     hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, st, tot, nt, boff, G);
     hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(kScanThreads), 0, st, boff, G, (const int *)tot);
-    hipLaunchKernelGGL(k_strong_emit, dim3(G), dim3(kBlock), 0, st, dinst, bmap, mask, boff, slist, cap);
+    hipLaunchKernelGGL(k_strong_emit, gm, dim3(kBlock), 0, st, dinst, bmap, G, mask, boff, slist, cap);
     const dim3 gl((cap + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_cc_links, gl, dim3(kBlock), 0, st, (const DefInst *)dinst, ninst, (const int *)slist, pms,
                        cap, w);
@@ -1156,9 +1168,20 @@ int deflation_finish_batch(hipStream_t st, const Stencil<double> &A, Deflation &
     std::vector<DefInst> inst(D.h.binst.size() / sizeof(DefInst));
     memcpy((void *)inst.data(), D.h.binst.data(), D.h.binst.size());
     HMFEM_CHECK(hipEventSynchronize(D.hist_ready));
+    // on the side stream, if there is one, so that the cluster setup and its
+    // round trip to the host overlap the hierarchy's last kernels (the dense
+    // inverses use one workgroup per instance); it reads only the fine level
+    // and the bins, which precede hist_ready on st, and st waits for its end
+    hipStream_t ds = D.side && D.side_done ? D.side : st;
+    if (ds != st) HMFEM_CHECK(hipStreamWaitEvent(ds, D.hist_ready, 0));
     // finish_clusters reads the histograms before it reuses the staging
-    return finish_clusters(st, inst, (DefInst *)D.h.bdinst, D.h.bbmap, D.h.bG, A, (const unsigned *)D.h_stage,
-                           D.h.bbins, D.h.bratio, D, alloc, actx, false);
+    int rc = finish_clusters(ds, inst, (DefInst *)D.h.bdinst, D.h.bbmap, D.h.bG, A, (const unsigned *)D.h_stage,
+                             D.h.bbins, D.h.bratio, D, alloc, actx, false);
+    if (ds != st) {
+        HMFEM_CHECK(hipEventRecord(D.side_done, ds));
+        HMFEM_CHECK(hipStreamWaitEvent(st, D.side_done, 0));
+    }
+    return rc;
 }
 
 template <int NR>
