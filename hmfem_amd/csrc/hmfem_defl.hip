@@ -1090,8 +1090,15 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     DefInst *dinst = (DefInst *)(h0 + kHistBins * sizeof(unsigned));
     std::vector<DefInst> inst(1, DefInst{d.nx, d.ny, d.nz, (int)n, 0, 0, INFINITY, 0, kHistBins});
     HMFEM_CHECK(hipEventSynchronize(D.hist_ready));
-    return finish_clusters(st, inst, dinst, nullptr, (int)((n + kBlock - 1) / kBlock), A, D.h_hist,
-                           (const unsigned short *)(h0 + slot0_bins_offset()), ratio, D, alloc, actx, records);
+    hipStream_t ds = D.side && D.side_done ? D.side : st;   // as in deflation_finish_batch
+    if (ds != st) HMFEM_CHECK(hipStreamWaitEvent(ds, D.hist_ready, 0));
+    int rc = finish_clusters(ds, inst, dinst, nullptr, (int)((n + kBlock - 1) / kBlock), A, D.h_hist,
+                             (const unsigned short *)(h0 + slot0_bins_offset()), ratio, D, alloc, actx, records);
+    if (ds != st) {
+        HMFEM_CHECK(hipEventRecord(D.side_done, ds));
+        HMFEM_CHECK(hipStreamWaitEvent(st, D.side_done, 0));
+    }
+    return rc;
 }
 
 int deflation_begin_batch(hipStream_t st, int ninst, const int *dims, const Stencil<double> &A, double ratio,

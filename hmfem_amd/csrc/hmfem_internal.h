@@ -251,7 +251,7 @@ struct Deflation {
     hipEvent_t stage_done = nullptr;         // recorded after the last copy out of the staging (context-owned)
     bool stage_busy = false;                 // such a copy may still be queued
     bool pending = false;                    // deflation_begin queued a histogram
-    hipStream_t side = nullptr;              // a batch's cluster setup runs here, beside its hierarchy (context-owned)
+    hipStream_t side = nullptr;              // the cluster setup runs here, beside the hierarchy's (context-owned)
     hipEvent_t side_done = nullptr;          // recorded on side at the end of that setup (context-owned)
     unsigned gen = 0;                        // bumped by every setup (single, block or batch)
     DeflationHost h;
