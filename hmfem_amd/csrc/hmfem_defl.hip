@@ -1053,7 +1053,7 @@ static int finish_clusters(hipStream_t st, std::vector<DefInst> &inst, DefInst *
 static size_t slot0_bins_offset() { return (kHistBins * sizeof(unsigned) + 256 + sizeof(DefInst) + 255) & ~(size_t)255; }
 
 int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation &D,
-                    char *(*alloc)(void *, size_t, int), void *actx)
+                    char *(*alloc)(void *, size_t, int), void *actx, bool external_hist)
 {
     D.nc = 0;
     D.m = 0;
@@ -1069,13 +1069,23 @@ int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation 
     unsigned *hist = (unsigned *)h0;
     unsigned short *bins = (unsigned short *)(h0 + slot0_bins_offset());
     HMFEM_CHECK(hipMemsetAsync(hist, 0, kHistBins * sizeof(unsigned), st));
+    D.pending = true;
+    D.h.bhist = hist;
+    D.h.bbins = bins;
+    if (external_hist) return HMFEM_OK;
     // a grid-stride loop over at most 512 blocks keeps the global bin updates
     // few (2048 blocks: 35 us instead of 25 at 128^3)
     const int G = std::min(grid_for(n, kBlock), 512);
     hipLaunchKernelGGL(k_diag_hist, dim3(G), dim3(kBlock), 0, st, o, hist, bins);
-    HMFEM_CHECK(hipMemcpyAsync(D.h_hist, hist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    return deflation_hist_queued(st, D);
+}
+
+int deflation_hist_queued(hipStream_t st, Deflation &D)
+{
+    if (!D.pending || !D.h.bhist) return HMFEM_OK;
+    HMFEM_CHECK(hipMemcpyAsync(D.h_hist, D.h.bhist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HMFEM_CHECK(hipEventRecord(D.hist_ready, st));
-    D.pending = true;
+    D.h.bhist = nullptr;   // queued
     return HMFEM_OK;
 }
 
@@ -1087,6 +1097,15 @@ int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ra
     if (!(ratio > 1.0)) return HMFEM_OK;
     const long n = d.n();
     char *h0 = alloc(actx, slot0_bins_offset() + sizeof(unsigned short) * (size_t)n, 0);
+    if (D.h.bhist) {   // begun with external_hist, but no pass took the histogram
+        OpArgs<double> o;
+        o.e = A.e; o.wx = A.wx; o.wy = A.wy; o.wz = A.wz; o.di = nullptr;
+        o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
+        hipLaunchKernelGGL(k_diag_hist, dim3(std::min(grid_for(n, kBlock), 512)), dim3(kBlock), 0, st, o, D.h.bhist,
+                           D.h.bbins);
+        int rc = deflation_hist_queued(st, D);
+        if (rc) return rc;
+    }
     DefInst *dinst = (DefInst *)(h0 + kHistBins * sizeof(unsigned));
     std::vector<DefInst> inst(1, DefInst{d.nx, d.ny, d.nz, (int)n, 0, 0, INFINITY, 0, kHistBins});
     HMFEM_CHECK(hipEventSynchronize(D.hist_ready));

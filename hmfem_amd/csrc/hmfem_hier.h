@@ -111,9 +111,12 @@ inline OpArgs<double> opargs<double>(const Level &L)
 }
 
 // Build the hierarchy from a fine FP64 stencil (device), inside `ar`.
+// dh: a conduit-correction setup begun with external_hist, whose histogram
+// the fine level's FP32 conversion pass then takes (and queues its copy).
+struct Deflation;
 int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const hmfem_options &opt, Arena &ar,
                     Hierarchy &H, int *d_info, std::vector<char> &stage, long lds_limit,
-                    const PlanHint *hint = nullptr);
+                    const PlanHint *hint = nullptr, Deflation *dh = nullptr);
 
 // One tail cycle of an FP32 hierarchy in LDS mode on a right-hand side read
 // from ftop[s * stride] (top tail level), result to utop[s * stride].

@@ -261,8 +261,13 @@ struct Deflation {
 // correction. D.nc == 0 afterwards if there is none. records: also find the
 // cells just outside the clusters that the balancing form (opt.deflate 2)
 // needs.
+// With external_hist, deflation_begin only clears the histogram (D.h.bhist)
+// and leaves it and the cells' bins (D.h.bbins) to the caller's pass over the
+// fine level (k_to_f32_dinv_hist), after which deflation_hist_queued queues
+// the copy; deflation_finish takes the histogram itself if that never came.
 int deflation_begin(hipStream_t st, Dims d, const Stencil<double> &A, Deflation &D,
-                    char *(*alloc)(void *, size_t, int), void *actx);
+                    char *(*alloc)(void *, size_t, int), void *actx, bool external_hist = false);
+int deflation_hist_queued(hipStream_t st, Deflation &D);
 int deflation_finish(hipStream_t st, Dims d, const Stencil<double> &A, double ratio, Deflation &D,
                      char *(*alloc)(void *, size_t, int), void *actx, bool records = false);
 void deflation_sums(hipStream_t st, const Deflation &D, const double *r, const double *v, double *Pextra,

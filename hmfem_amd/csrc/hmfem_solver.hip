@@ -783,6 +783,46 @@ k_to_f32_dinv(OpArgs<double> o, float *e32, float *wx32, float *wy32, float *wz3
     to_f32_dinv_cell(o, cell_at(o, s), e32, wx32, wy32, wz32, di32);
 }
 
+// The same pass, which also takes the conduit correction's histogram of the
+// FP64 diagonal's bins (Deflation::kHistBins) and keeps each cell's bin.
+// Block b covers cells [b kBlock kConvCells, (b + 1) kBlock kConvCells);
+// runs of equal bins are counted in a register, then summed in LDS, and the
+// block adds its non-empty bins to the global ones (few global atomics on
+// the crowded bins).
+constexpr int kConvCells = 16;
+
+__global__ void __launch_bounds__(kBlock)
+k_to_f32_dinv_hist(OpArgs<double> o, float *e32, float *wx32, float *wy32, float *wz32, float *di32,
+                   unsigned *hist, unsigned short *bins)
+{
+    constexpr int NB = Deflation::kHistBins;
+    __shared__ unsigned h[NB];
+    for (int t = threadIdx.x; t < NB; t += kBlock) h[t] = 0;
+    __syncthreads();
+    const int n = o.nx * o.ny * o.nz;
+    const int s0 = blockIdx.x * kBlock * kConvCells + threadIdx.x;
+    int rb = 0;
+    unsigned run = 0;
+    for (int t = 0; t < kConvCells; t++) {
+        const int s = s0 + t * kBlock;
+        if (s >= n) break;
+        const Cell c = cell_at(o, s);
+        const int b = diag_bin(diag_at<double, double>(o, c), NB);
+        bins[s] = (unsigned short)b;
+        to_f32_dinv_cell(o, c, e32, wx32, wy32, wz32, di32);
+        if (b != rb) {
+            if (run) atomicAdd(&h[rb], run);
+            rb = b;
+            run = 0;
+        }
+        run++;
+    }
+    if (run) atomicAdd(&h[rb], run);
+    __syncthreads();
+    for (int t = threadIdx.x; t < NB; t += kBlock)
+        if (h[t]) atomicAdd(&hist[t], h[t]);
+}
+
 // sq = sqrt(d) of the fine level from the FP64 stencil.
 __global__ void __launch_bounds__(kBlock)
 k_fine16_scale(OpArgs<double> o, float *sq)
@@ -1491,7 +1531,8 @@ static int upload_tail(hipStream_t st, Hierarchy &H, const hmfem_options &opt, s
 
 // Build the hierarchy from a fine FP64 stencil (device), inside `arena`.
 int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const hmfem_options &opt, Arena &ar,
-                    Hierarchy &H, int *d_info, std::vector<char> &stage, long lds_limit, const PlanHint *hint)
+                    Hierarchy &H, int *d_info, std::vector<char> &stage, long lds_limit, const PlanHint *hint,
+                    Deflation *dh)
 {
     plan_levels(fine, clamp_coarse(opt.coarse_max), H.lev, H.nlev, hint);
     int prec = opt.precision == HMFEM_PREC_FP64 ? HMFEM_PREC_FP64 : HMFEM_PREC_FP32;
@@ -1534,9 +1575,16 @@ int build_hierarchy(hipStream_t st, Dims fine, const Stencil<double> &s0, const 
         if (prec == HMFEM_PREC_FP64)
             hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, n, L.s64.e, L.s64.wx,
                                L.s64.wy, L.s64.wz, L.s32.e, L.s32.wx, L.s32.wy, L.s32.wz);
-        else   // with the fine level's inverse diagonal
+        else if (dh && dh->pending && dh->h.bhist) {   // with the inverse diagonal and the histogram
+            hipLaunchKernelGGL(k_to_f32_dinv_hist, dim3((unsigned)((n + kBlock * kConvCells - 1) / (kBlock * kConvCells))),
+                               dim3(kBlock), 0, st, opargs<double>(L), L.s32.e, L.s32.wx, L.s32.wy, L.s32.wz,
+                               L.s32.di, dh->h.bhist, dh->h.bbins);
+            int rc = deflation_hist_queued(st, *dh);
+            if (rc) return rc;
+        } else {   // with the fine level's inverse diagonal
             hipLaunchKernelGGL(k_to_f32_dinv, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, opargs<double>(L),
                                L.s32.e, L.s32.wx, L.s32.wy, L.s32.wz, L.s32.di);
+        }
         if (H.f16sq) {
             hipLaunchKernelGGL(k_fine16_scale, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, opargs<double>(L),
                                H.f16sq);
@@ -2038,11 +2086,12 @@ int solve_structured(hipStream_t st, Dims d, const Stencil<double> &A, const dou
             defl->pending = false;
             defl->gen++;
             if (opt.deflate && dalloc) {
-                rc = deflation_begin(st, d, A, *defl, dalloc, dctx);
+                // in the FP32 hierarchy the conversion pass takes the histogram
+                rc = deflation_begin(st, d, A, *defl, dalloc, dctx, opt.precision != HMFEM_PREC_FP64);
                 if (rc) return rc;
             }
         }
-        rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit, hint);
+        rc = build_hierarchy(st, d, A, opt, ar, H, d_info, stage, lds_limit, hint, defl);
         if (rc) return rc;
         if (defl && defl->pending) {
             rc = deflation_finish(st, d, A, opt.deflate_ratio, *defl, dalloc, dctx, opt.deflate == 2);
