@@ -430,6 +430,7 @@ kb_to_f32_dinv(const BInst *__restrict__ D, const int *__restrict__ bmap, LPtr<d
 // directly. Per-launch atomics on the crowded bins stay few: one flush per
 // 4096 cells instead of one per wave.
 constexpr int kConvSub = 16;
+constexpr int kConvBatch = 4;
 
 __global__ void __launch_bounds__(kBlock)
 kb_to_f32_dinv_hist(const BInst *__restrict__ D, const int *__restrict__ bmap, int nblk, LPtr<double> A,
@@ -449,25 +450,39 @@ kb_to_f32_dinv_hist(const BInst *__restrict__ D, const int *__restrict__ bmap, i
         if (rq == q0) atomicAdd(&h[rb], run);
         else atomicAdd(&hist[(long)rq * NB + rb], run);
     };
-    for (int t = 0; t < kConvSub; t++) {
-        const int sb = sb0 + t;
-        if (sb >= nblk) break;
-        const int q = bmap[sb];
-        const BInst I = D[q];
-        const int s = (sb - I.blk0) * kBlock + threadIdx.x;
-        if (s >= I.n) continue;
-        const OpArgs<double> o = bop(A, I.off, I.nx, I.ny, I.nz);
-        const Cell c = cell_at(o, s);
-        const int b = diag_bin(diag_at<double, double>(o, c), NB);
-        bins[I.off + s] = (unsigned short)b;
-        to_f32_dinv_cell(o, c, e32 + I.off, wx32 + I.off, wy32 + I.off, wz32 + I.off, di32 + I.off);
-        if (q != rq || b != rb) {
-            flush();
-            rq = q;
-            rb = b;
-            run = 0;
+    // kConvBatch blocks' cells at a time, all their loads first (a lane past
+    // its instance's end, or a block past the last, loads a cell of its own
+    // and stores nothing)
+    for (int t0 = 0; t0 < kConvSub; t0 += kConvBatch) {
+        Cell c[kConvBatch];
+        ConvLoads L[kConvBatch];
+        int qq[kConvBatch];
+        bool live[kConvBatch];
+#pragma unroll
+        for (int u = 0; u < kConvBatch; u++) {
+            const int sb = min(sb0 + t0 + u, nblk - 1);
+            const int q = bmap[sb];
+            const BInst I = D[q];
+            const int s = (sb - I.blk0) * kBlock + threadIdx.x;
+            live[u] = sb0 + t0 + u < nblk && s < I.n;
+            qq[u] = q;
+            const OpArgs<double> o = bop(A, I.off, I.nx, I.ny, I.nz);
+            c[u] = cell_at(o, live[u] ? s : I.n - 1);
+            c[u].s += I.off;   // batch index: the loads and stores below are on the whole batch's arrays
+            L[u] = conv_load(bop(A, 0, I.nx, I.ny, I.nz), c[u]);
         }
-        run++;
+#pragma unroll
+        for (int u = 0; u < kConvBatch; u++) {
+            if (!live[u]) continue;
+            const int b = conv_store(L[u], c[u], e32, wx32, wy32, wz32, di32, bins, NB);
+            if (qq[u] != rq || b != rb) {
+                flush();
+                rq = qq[u];
+                rb = b;
+                run = 0;
+            }
+            run++;
+        }
     }
     flush();
     __syncthreads();

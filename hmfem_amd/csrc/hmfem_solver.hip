@@ -788,8 +788,10 @@ k_to_f32_dinv(OpArgs<double> o, float *e32, float *wx32, float *wy32, float *wz3
 // Block b covers cells [b kBlock kConvCells, (b + 1) kBlock kConvCells);
 // runs of equal bins are counted in a register, then summed in LDS, and the
 // block adds its non-empty bins to the global ones (few global atomics on
-// the crowded bins).
+// the crowded bins). The cells are taken kConvBatch at a time, all their
+// loads first.
 constexpr int kConvCells = 16;
+constexpr int kConvBatch = 4;
 
 __global__ void __launch_bounds__(kBlock)
 k_to_f32_dinv_hist(OpArgs<double> o, float *e32, float *wx32, float *wy32, float *wz32, float *di32,
@@ -803,19 +805,26 @@ k_to_f32_dinv_hist(OpArgs<double> o, float *e32, float *wx32, float *wy32, float
     const int s0 = blockIdx.x * kBlock * kConvCells + threadIdx.x;
     int rb = 0;
     unsigned run = 0;
-    for (int t = 0; t < kConvCells; t++) {
-        const int s = s0 + t * kBlock;
-        if (s >= n) break;
-        const Cell c = cell_at(o, s);
-        const int b = diag_bin(diag_at<double, double>(o, c), NB);
-        bins[s] = (unsigned short)b;
-        to_f32_dinv_cell(o, c, e32, wx32, wy32, wz32, di32);
-        if (b != rb) {
-            if (run) atomicAdd(&h[rb], run);
-            rb = b;
-            run = 0;
+    for (int t0 = 0; t0 < kConvCells; t0 += kConvBatch) {
+        Cell c[kConvBatch];
+        ConvLoads L[kConvBatch];
+#pragma unroll
+        for (int u = 0; u < kConvBatch; u++) {
+            const int s = s0 + (t0 + u) * kBlock;
+            c[u] = cell_at(o, s < n ? s : n - 1);
+            L[u] = conv_load(o, c[u]);
         }
-        run++;
+#pragma unroll
+        for (int u = 0; u < kConvBatch; u++) {
+            if (s0 + (t0 + u) * kBlock >= n) break;
+            const int b = conv_store(L[u], c[u], e32, wx32, wy32, wz32, di32, bins, NB);
+            if (b != rb) {
+                if (run) atomicAdd(&h[rb], run);
+                rb = b;
+                run = 0;
+            }
+            run++;
+        }
     }
     if (run) atomicAdd(&h[rb], run);
     __syncthreads();

@@ -80,6 +80,48 @@ __device__ __forceinline__ int diag_bin(double d, int nbins)
     return b < 0 ? 0 : (b >= nbins ? nbins - 1 : b);
 }
 
+// The conversion pass with the conduit correction's histogram, split into its
+// loads and the rest so that a thread can have several cells' loads in
+// flight: every load is unconditional (a neighbour off the grid reads the
+// cell itself and is not added), and conv_store adds in diag_at's order, so
+// the FP64 diagonal's bin is that of diag_at<double, double> and di32 that of
+// to_f32_dinv_cell.
+struct ConvLoads {
+    double e, x, y, z, xm, ym, zm;
+};
+
+__device__ __forceinline__ ConvLoads conv_load(const OpArgs<double> &o, const Cell &c)
+{
+    const int nx = o.nx, nxy = o.nx * o.ny, s = c.s;
+    ConvLoads L;
+    L.e = o.e[s]; L.x = o.wx[s]; L.y = o.wy[s]; L.z = o.wz[s];
+    L.xm = o.wx[c.i > 0 ? s - 1 : s];
+    L.ym = o.wy[c.j > 0 ? s - nx : s];
+    L.zm = o.wz[c.k > 0 ? s - nxy : s];
+    return L;
+}
+
+// Stores cell c's FP32 copy, inverse diagonal and bin; returns the bin.
+__device__ __forceinline__ int conv_store(const ConvLoads &L, const Cell &c, float *e32, float *wx32, float *wy32,
+                                          float *wz32, float *di32, unsigned short *bins, int nbins)
+{
+    const int s = c.s;
+    double d = L.e + L.x + L.y + L.z;
+    d = c.i > 0 ? d + L.xm : d;
+    d = c.j > 0 ? d + L.ym : d;
+    d = c.k > 0 ? d + L.zm : d;
+    const int b = diag_bin(d, nbins);
+    bins[s] = (unsigned short)b;
+    const float e = (float)L.e, x = (float)L.x, y = (float)L.y, z = (float)L.z;
+    e32[s] = e; wx32[s] = x; wy32[s] = y; wz32[s] = z;
+    double d32 = (double)e + (double)x + (double)y + (double)z;
+    d32 = c.i > 0 ? d32 + (double)(float)L.xm : d32;
+    d32 = c.j > 0 ? d32 + (double)(float)L.ym : d32;
+    d32 = c.k > 0 ? d32 + (double)(float)L.zm : d32;
+    di32[s] = (float)(1.0 / d32);
+    return b;
+}
+
 // (A u)_s in flux form and the diagonal, for a functor u(Cell) giving the
 // value of u at a cell (so callers can build u on the fly).
 template <typename T, typename C, typename U>
