@@ -577,6 +577,63 @@ def test_deflation_clusters(kind, dims, seed, ctx, torch):
     assert st.status == S.OK and st.deflation_clusters == 0 and st.deflation_cells == 0
 
 
+def _threshold_bin_problem():
+    """A homogeneous 40x36x28 grid with 25 x-lines of raised permeability:
+    20 spread over three decades and five whose diagonals lie just below and
+    above the conduit correction's threshold, inside its power-of-two bin."""
+    P = make_problem((40, 36, 28), "homogeneous", seed=5)
+    K = P.Kx.reshape(P.nz, P.ny, P.nx)
+    ms = list(np.geomspace(300.0, 300000.0, 20)) + [7500.0, 7600.0, 7700.0, 7800.0, 7900.0]
+    q = 0
+    for k in range(3, P.nz - 2, 4):
+        for j in range(3, P.ny - 2, 4):
+            if q < len(ms):
+                K[k, j, 4:P.nx - 4] *= ms[q]
+                q += 1
+    P.Ky[:] = P.Kx
+    P.Kz[:] = P.Kx
+    return P
+
+
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_FP64], ids=["fp32", "fp64"])
+def test_deflation_threshold_bin(prec, ctx, torch):
+    """Strong cells in the threshold's own power-of-two bin are decided by
+    their diagonal, those in other bins by the bin alone: on a field with
+    cells of that bin on both sides of the threshold (checked here on the
+    oracle's diagonal) the clusters match the independent labelling, with the
+    histogram taken in the FP32 conversion pass (fp32) or by its own pass
+    (fp64), on a single grid and in a batch."""
+    P = _threshold_bin_problem()
+    ratio = S.default_options().deflate_ratio
+    perm, _, _ = O.assemble(P)
+    d = np.abs(perm["c"])
+    b = np.clip(np.frexp(d)[1] - 1 + 1100, 0, 2047)
+    hist = np.bincount(b, minlength=2048)
+    mb = int(np.argmax(2 * np.cumsum(hist) >= P.n))
+    thr = ratio * 2.0 ** (mb - 1100) * np.sqrt(2.0)
+    tb = int(np.frexp(thr)[1] - 1 + 1100)
+    assert ((b == tb) & (d > thr)).sum() > 0 and ((b == tb) & (d <= thr)).sum() > 0
+    nc, m = _expected_clusters(P, ratio)
+    _say("threshold bin", prec, nc, m)
+    D = Dev(P, torch)
+    p = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    st = ctx.reservoir(D.g, D.K, D.K, D.K, D.bval, p, None,
+                       S.default_options(rtol=1e-8, deflate=1, precision=prec))
+    assert st.status == S.OK
+    assert (st.deflation_clusters, st.deflation_cells) == (nc, m)
+    Q = make_problem((24, 20, 16), "karst", seed=22, conduits=4)
+    e2 = _expected_clusters(Q, ratio)
+    K = torch.from_numpy(np.concatenate([P.Kx, Q.Kx])).to("cuda")
+    big = max(max(X.nx * X.ny, X.ny * X.nz, X.nx * X.nz) for X in (P, Q))
+    bval = [torch.full((big,), 1.0, dtype=torch.float64, device="cuda")] + \
+           [torch.zeros(big, dtype=torch.float64, device="cuda") for _ in range(5)]
+    pb = torch.zeros(P.n + Q.n, dtype=torch.float64, device="cuda")
+    st, per = ctx.reservoir_batch([(P.nx, P.ny, P.nz), (Q.nx, Q.ny, Q.nz)], K, K, K, bval, pb, None,
+                                  S.default_options(rtol=1e-8, deflate=1, precision=prec))
+    assert st.status == S.OK, (st.as_dict(), per)
+    assert (st.deflation_clusters, st.deflation_cells) == (nc + e2[0], m + e2[1])
+
+
 def test_reuse_setup(ctx, torch):
     """opt.reuse_setup: a second solve of a changed operator on the same grid
     reuses the first hierarchy and conduit correction and still reaches the
