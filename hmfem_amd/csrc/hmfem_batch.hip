@@ -1231,9 +1231,11 @@ int batch_solve(hipStream_t st, int ninst, const int *dims, const Stencil<double
     HMFEM_CHECK(hipEventRecord(ev[1], st));
 
     // CG
-    // gs_sweeps 0: three sweeps for a batch of small grids (96 uneven instances: 1592 solves/s with three,
-    // 1401 with two)
-    BCycle cp{opt.nu, opt.omega, opt.ocf, opt.smoother, opt.gs_sweeps < 1 ? 3 : opt.gs_sweeps};
+    // gs_sweeps 0 / ocf 0: four sweeps at a scaling of 2.0 for a batch of small grids (96 uneven
+    // instances, 32^3 to 96^3: 1803-1811 solves/s; three at 2.2 1655, three at 2.0 1733, four at 1.9
+    // 1793, four at 2.1 1730, five at 2.0 1739)
+    BCycle cp{opt.nu, opt.omega, ocf_of(opt, kOcfBatch), opt.smoother,
+              opt.gs_sweeps < 1 ? kSweepsBatch : opt.gs_sweeps};
     LPtr<double> A64 = lp<double>(B.lev[0]);
     hipLaunchKernelGGL(kb_cg_init, dim3(B.ncblk), dim3(kBlock), 0, st, B.dch, B.cbmap, A64, b, x, B.r, B.Pinit);
     auto vcycle = [&](const double *vd) -> void * {

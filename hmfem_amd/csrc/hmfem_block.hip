@@ -443,7 +443,7 @@ k_b_update(int n, const double *__restrict__ S0, const double *__restrict__ S1, 
 constexpr int kBCz = 4;                          // cells of a tile column per thread
 // gs_sweeps 0: two sweeps (the 256^3 block workload, eight right-hand sides:
 // 277 block iterations and 9.17 solves/s with two, 308 and 7.91 with three)
-constexpr int kBlockSweeps = 2;
+constexpr int kBlockSweeps = kSweepsSingle;
 constexpr int kBThreads = kGsTx * kGsTy * (kBTz / kBCz);
 
 // Block red-black Gauss-Seidel on tiles of kGsTx x kGsTy x kBTz cells for NR
@@ -888,7 +888,7 @@ static int b_vcycle_coarse(hipStream_t st, Hierarchy &H, BlockState &S, int l, c
     BlockLevelVecs &X = S.lv[l];
     if (l == H.tail0) {
         for (int r = 0; r < NR; r++) {
-            int rc = launch_tail_strided(st, H, opt.omega, opt.ocf, X.f + r, X.u + r, NR, S.done);
+            int rc = launch_tail_strided(st, H, opt.omega, ocf_of(opt, kOcfSingle), X.f + r, X.u + r, NR, S.done);
             if (rc) return rc;
         }
         return HMFEM_OK;
@@ -904,7 +904,7 @@ static int b_vcycle_coarse(hipStream_t st, Hierarchy &H, BlockState &S, int l, c
     if (rc) return rc;
     hipLaunchKernelGGL((k_b_gs_tile<NR, float, 1>), dim3(GT), dim3(kBThreads), 0, st, opargs<float>(L), sw,
                        (const float *)X.f, (const float *)X.u, X.t, lg2(L.fx), lg2(L.fy), lg2(L.fz), C.d.nx, C.d.ny,
-                       (const float *)S.lv[l + 1].u, (float)opt.ocf, (const double *)nullptr, 1, (double *)nullptr,
+                       (const float *)S.lv[l + 1].u, (float)ocf_of(opt, kOcfSingle), (const double *)nullptr, 1, (double *)nullptr,
                        S.done);
     float *tmp = X.u; X.u = X.t; X.t = tmp;
     return HMFEM_OK;
@@ -932,7 +932,7 @@ static int b_vcycle_fine(hipStream_t st, Hierarchy &H, BlockState &S, const doub
     const int tpb = (nt + S.ch.G - 1) / S.ch.G;
     hipLaunchKernelGGL((k_b_gs_tile<NR, float, 2>), dim3(S.ch.G), dim3(kBThreads), 0, st, opargs<float>(L), sw,
                        (const float *)S.R32, (const float *)X.u, S.Z, lg2(L.fx), lg2(L.fy), lg2(L.fz), C.d.nx,
-                       C.d.ny, (const float *)S.lv[1].u, (float)opt.ocf, Vp, tpb, S.Pdot, S.done);
+                       C.d.ny, (const float *)S.lv[1].u, (float)ocf_of(opt, kOcfSingle), Vp, tpb, S.Pdot, S.done);
     if (defl) return deflation_scatter_block(st, *S.D, NR, S.Z, S.done);
     return HMFEM_OK;
 }

@@ -324,7 +324,7 @@ extern "C" void hmfem_default_options(hmfem_options *o)
     o->stop_rule = HMFEM_STOP_RESIDUAL;
     o->nu = 3;
     o->omega = 0.95;
-    o->ocf = 2.2;
+    o->ocf = 0.0;   // the path's own
     o->coarse_max = kCoarseMax;
     o->check_every = 4;
     o->tail_max = 512;   // 4096 (a 16^3 level in the tail, now that it fits in LDS) was slower: one

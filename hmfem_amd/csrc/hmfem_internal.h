@@ -195,6 +195,13 @@ struct ProfileSink {
 };
 
 // defined in hmfem_solver.hip
+// The paths' own cycle settings where the options leave them at 0 (measured
+// best on each: single-grid and block solves two sweeps at 2.2, batches four
+// at 2.0).
+constexpr double kOcfSingle = 2.2, kOcfBatch = 2.0;
+constexpr int kSweepsSingle = 2, kSweepsBatch = 4;
+inline double ocf_of(const hmfem_options &o, double own) { return o.ocf > 0 ? o.ocf : own; }
+
 struct CgBuffers {
     double *r, *s0, *s1, *v;
     CgScalars *sc;

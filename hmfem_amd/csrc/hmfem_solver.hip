@@ -1870,7 +1870,8 @@ static int run_cg(hipStream_t st, Hierarchy &H, const Stencil<double> &A, Dims d
     o.nx = d.nx; o.ny = d.ny; o.nz = d.nz;
     // gs_sweeps 0: two sweeps on a single grid (256^3 conduits: 69 CG iterations and 16.3 solves/s with two,
     // 80 and 14.1 with three; 128^3 lognormal: 17 and 287 against 16 and 290)
-    CycleParams cp{opt.nu < 1 ? 1 : opt.nu, opt.omega, opt.ocf, opt.smoother, opt.gs_sweeps < 1 ? 2 : opt.gs_sweeps};
+    CycleParams cp{opt.nu < 1 ? 1 : opt.nu, opt.omega, ocf_of(opt, kOcfSingle), opt.smoother,
+                   opt.gs_sweeps < 1 ? kSweepsSingle : opt.gs_sweeps};
     CgState S;
     S.B = B;
     S.ch = chunking(n);

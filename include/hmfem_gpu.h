@@ -165,7 +165,8 @@ typedef struct {
     int stop_rule;           /* default HMFEM_STOP_RESIDUAL */
     int nu;                  /* Jacobi sweeps before and after the coarse correction (default 3) */
     double omega;            /* Jacobi damping (default 0.95) */
-    double ocf;              /* coarse-correction scaling (default 2.2) */
+    double ocf;              /* coarse-correction scaling; default 0: 2.2 in single and block solves,
+                                2.0 in batched solves (measured best on each) */
     int coarse_max;          /* largest coarsest level, solved densely (default and maximum 80) */
     int check_every;         /* CG iterations queued between host convergence checks (default 4) */
     int tail_max;            /* levels of at most this many cells run in one single-workgroup
@@ -178,7 +179,7 @@ typedef struct {
                                 inside 32x8x8 tiles, one pass per smoothing step; the default); the
                                 levels of the single-workgroup kernel always use Jacobi */
     int gs_sweeps;           /* sweeps per smoothing step of HMFEM_SMOOTH_BLOCK_GS; default 0: two in single
-                                and block solves, three in batched solves (measured best on each) */
+                                and block solves, four in batched solves (measured best on each) */
     int autotune;            /* 1: hmfem_reservoir_device first runs hmfem_autotune_device for each new
                                 grid shape / precision / tolerance of the context and then uses the
                                 options it chose; hmfem_reservoir_batch_device likewise runs
