@@ -239,7 +239,7 @@ def run_batch(args, w, torch, S, emit=True):
     opt = S.default_options(rtol=1e-8, precision=prec, check_every=2)
     if args.smoother:
         opt.smoother = S.SMOOTH_JACOBI if args.smoother == "jacobi" else S.SMOOTH_BLOCK_GS
-    for name, val in (("gs_sweeps", args.gs_sweeps), ("ocf", args.ocf)):
+    for name, val in (("gs_sweeps", args.gs_sweeps), ("ocf", args.ocf), ("deflate_ratio", args.deflate_ratio)):
         if val is not None:
             setattr(opt, name, val)
 
@@ -395,7 +395,7 @@ def run_block(args, w, torch, S):
     ctx = S.Context()
     solver_stream = torch.cuda.ExternalStream(ctx.stream)
     opt = S.default_options(rtol=1e-8)
-    for name, val in (("gs_sweeps", args.gs_sweeps), ("ocf", args.ocf)):
+    for name, val in (("gs_sweeps", args.gs_sweeps), ("ocf", args.ocf), ("deflate_ratio", args.deflate_ratio)):
         if val is not None:
             setattr(opt, name, val)
     if args.no_compact:
@@ -498,6 +498,8 @@ def main():
                     help="smoother of the multi-block levels (default: the library's)")
     ap.add_argument("--gs-sweeps", type=int, default=None, help="block Gauss-Seidel sweeps (default: the library's)")
     ap.add_argument("--ocf", type=float, default=None, help="coarse-correction scaling (default: the library's)")
+    ap.add_argument("--deflate-ratio", type=float, default=None,
+                    help="conduit correction: strong-cell threshold over the median diagonal (default: the library's)")
     ap.add_argument("--no-compact", action="store_true",
                     help="c5: keep four lanes to the end (no switch to two once the feed is empty)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -566,7 +568,7 @@ def main():
     opt = S.default_options(rtol=rtol, precision=prec)
     if args.smoother:
         opt.smoother = S.SMOOTH_JACOBI if args.smoother == "jacobi" else S.SMOOTH_BLOCK_GS
-    for name, val in (("gs_sweeps", args.gs_sweeps), ("ocf", args.ocf)):
+    for name, val in (("gs_sweeps", args.gs_sweeps), ("ocf", args.ocf), ("deflate_ratio", args.deflate_ratio)):
         if val is not None:
             setattr(opt, name, val)
 
