@@ -480,7 +480,9 @@ def _expected_clusters(P, ratio):
 def test_full_size_c4_properties(ctx, torch):
     """The 96-instance benchmark batch (edges 32-96, lognormal and conduit
     fields) at rtol 1e-8: every instance converged, its true residual against
-    the oracle's own assembly of that instance, and the maximum principle."""
+    the oracle's own assembly of that instance, and the maximum principle;
+    the batch's default cycle (four sweeps at a scaling of 2.0) keeps its
+    slowest instance at 75 iterations (94 at three sweeps and 2.2)."""
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     import bench
     w = bench.WORKLOADS["c4"]
@@ -495,6 +497,7 @@ def test_full_size_c4_properties(ctx, torch):
     st, per = ctx.reservoir_batch([d for d, _, _ in inst], K, K, K, bval, p, None, S.default_options(rtol=1e-8))
     _say("C4 status", st.as_dict())
     assert st.status == S.OK and np.all(per["iterations"] > 0)
+    assert per["iterations"].max() <= 80, per["iterations"].max()
     xb = p.cpu().numpy()
     off = 0
     for d, k, desc in inst:
