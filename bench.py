@@ -342,7 +342,7 @@ def batch_summary(torch, S):
     CPU or FP64 legs; the autotuned options' line): solves/s and per-instance
     completion percentiles, for the main line."""
     a = argparse.Namespace(steps=3, warmup=1, no_fp64=True, no_cpu=True, precision="fp32", smoother=None,
-                           gs_sweeps=None, ocf=None, cpu_budget=0.0, tune=True)
+                           gs_sweeps=None, ocf=None, deflate_ratio=None, cpu_budget=0.0, tune=True)
     r = run_batch(a, WORKLOADS["c4"], torch, S, emit=False)
     return {"workload": r["config"]["workload"], "value": r["value"], "unit": "solves/s",
             "ms_per_batch": r["ms_per_step"], "per_instance_done_ms": r["per_instance_done_ms"],
