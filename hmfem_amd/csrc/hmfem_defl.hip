@@ -147,20 +147,28 @@ k_strong_mask(const DefInst *__restrict__ I, const int *__restrict__ bmap, int G
 {
     __shared__ int wc[kMaskSub][kWavesPerBlock];
     const int w = threadIdx.x >> 6, g0 = blockIdx.x * kMaskSub;
+    // the groups' loads first (instance, then bins), so that a thread has
+    // kMaskSub of each in flight instead of a chain of three per group; a
+    // group past the last (g >= G) reads group G - 1's and stores nothing
+    int q[kMaskSub], l[kMaskSub], b[kMaskSub];
+#pragma unroll
+    for (int t = 0; t < kMaskSub; t++) q[t] = bmap ? bmap[min(g0 + t, G - 1)] : 0;
+#pragma unroll
+    for (int t = 0; t < kMaskSub; t++) {
+        const int g = min(g0 + t, G - 1);
+        l[t] = (g - I[q[t]].blk0) * kBlock + threadIdx.x;
+        b[t] = l[t] < I[q[t]].n ? bins[I[q[t]].off + l[t]] : -1;
+    }
+#pragma unroll
     for (int t = 0; t < kMaskSub; t++) {
         const int g = g0 + t;
         if (g >= G) break;
-        const DefInst in = I[bmap ? bmap[g] : 0];
-        const int l = (g - in.blk0) * kBlock + threadIdx.x;
-        bool f = false;
-        if (l < in.n) {
-            const int b = bins[in.off + l];
-            if (b == in.tb) {
-                const OpArgs<double> o = inst_op(in, A);
-                f = diag_at<double, double>(o, cell_at(o, l)) > in.thr;
-            } else {
-                f = b > in.tb;
-            }
+        const int tb = I[q[t]].tb;
+        bool f = b[t] > tb;
+        if (b[t] == tb) {
+            const DefInst in = I[q[t]];
+            const OpArgs<double> o = inst_op(in, A);
+            f = diag_at<double, double>(o, cell_at(o, l[t])) > in.thr;
         }
         const unsigned long long m = __ballot(f);
         if ((threadIdx.x & 63) == 0) {
@@ -240,10 +248,14 @@ k_strong_emit(const DefInst *__restrict__ I, const int *__restrict__ bmap, int G
               const unsigned long long *__restrict__ mask, const int *__restrict__ boff, int *list, int cap)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g0 = blockIdx.x * kMaskSub;
+    unsigned long long mm[kMaskSub];   // the groups' masks first (most groups have no strong cell)
+#pragma unroll
+    for (int t = 0; t < kMaskSub; t++) mm[t] = mask[(long)min(g0 + t, G - 1) * kWavesPerBlock + w];
+#pragma unroll
     for (int t = 0; t < kMaskSub; t++) {
         const int g = g0 + t;
         if (g >= G) break;
-        const unsigned long long m = mask[(long)g * kWavesPerBlock + w];
+        const unsigned long long m = mm[t];
         if (!((m >> lane) & 1ull)) continue;
         int at = boff[g];
         for (int q = 0; q < w; q++) at += __popcll(mask[(long)g * kWavesPerBlock + q]);
