@@ -579,7 +579,20 @@ __global__ void __launch_bounds__(kMaxClusters) k_place_scan(int nc, int nchunks
     const int c = threadIdx.x;
     if (c >= nc) return;
     int run = w.sel[kMaxClusters + c];   // the cluster's offset in the list
-    for (int b = 0; b < nchunks; b++) {
+    // eight chunks' counts loaded before their offsets are stored (one load
+    // latency per eight chunks instead of per chunk)
+    int b = 0;
+    for (; b + 8 <= nchunks; b += 8) {
+        int v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = w.ccount[(long)(b + u) * nc + c];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            w.ccount[(long)(b + u) * nc + c] = run;
+            run += v[u];
+        }
+    }
+    for (; b < nchunks; b++) {
         const long q = (long)b * nc + c;
         const int v = w.ccount[q];
         w.ccount[q] = run;
